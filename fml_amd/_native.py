@@ -1,0 +1,125 @@
+"""ctypes binding of libfmlhip.so (the C ABI declared in include/fmlh.h).
+
+There is no CPU fallback: importing works without a GPU (so the package can
+be inspected and the symbol table checked), but every compute call needs the
+built library and a visible HIP device, and raises otherwise.
+
+Device memory and streams come from PyTorch (ROCm build): buffers are torch
+tensors on ``cuda`` devices, passed as raw pointers, and each call is
+enqueued on ``torch.cuda.current_stream()``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfmlhip.so")
+
+c_void_p = ctypes.c_void_p
+c_int = ctypes.c_int
+c_int64 = ctypes.c_int64
+c_double = ctypes.c_double
+c_size_t = ctypes.c_size_t
+c_dbl_p = ctypes.POINTER(ctypes.c_double)
+
+#: name -> (restype, argtypes); mirrors include/fmlh.h
+SIGNATURES = {
+    "fmlh_abi_version": (c_int, []),
+    "fmlh_last_error": (ctypes.c_char_p, []),
+    "fmlh_device_cu_count": (c_int, []),
+    "fmlh_coulomb_sorted": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p,
+                                    c_void_p, c_void_p, c_void_p]),
+    "fmlh_gaussian_kernel": (c_int, [c_void_p, c_int64, c_int, c_void_p, c_int64, c_int, c_int, c_double,
+                                     c_void_p, c_int64, c_void_p]),
+    "fmlh_gaussian_kernel_f32": (c_int, [c_void_p, c_int64, c_int, c_void_p, c_int64, c_int, c_int,
+                                         c_double, c_int, c_void_p, c_void_p, c_int64, c_void_p]),
+    "fmlh_laplacian_kernel": (c_int, [c_void_p, c_int64, c_int, c_void_p, c_int64, c_int, c_int, c_double,
+                                      c_void_p, c_int64, c_void_p]),
+    "fmlh_laplacian_kernel_f32": (c_int, [c_void_p, c_int64, c_int, c_void_p, c_int64, c_int, c_int,
+                                          c_double, c_void_p, c_void_p, c_int64, c_void_p]),
+    "fmlh_arad_ws_bytes": (c_size_t, [c_int, c_int, c_int, c_int]),
+    "fmlh_arad_kernels": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
+                                  c_int, c_int, c_dbl_p, c_int, c_double, c_double, c_double, c_double,
+                                  c_int, c_void_p, c_void_p, c_size_t, c_void_p]),
+    "fmlh_arad_atomic_ws_bytes": (c_size_t, [c_int, c_int, c_int]),
+    "fmlh_arad_atomic": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
+                                 c_int, c_int, c_dbl_p, c_int, c_double, c_double, c_double, c_double,
+                                 c_void_p, c_void_p, c_size_t, c_void_p]),
+    "fmlh_potrf_ws_bytes": (c_size_t, [c_int]),
+    "fmlh_potrf": (c_int, [c_void_p, c_int64, c_int, c_void_p, c_void_p, c_size_t, c_void_p]),
+    "fmlh_potrs_ws_bytes": (c_size_t, [c_int, c_int]),
+    "fmlh_potrs": (c_int, [c_void_p, c_int64, c_int, c_void_p, c_int64, c_int, c_void_p, c_size_t,
+                           c_void_p]),
+    "fmlh_spotrf_ws_bytes": (c_size_t, [c_int]),
+    "fmlh_spotrf": (c_int, [c_void_p, c_int64, c_int, c_void_p, c_void_p, c_size_t, c_void_p]),
+    "fmlh_spotrs_ws_bytes": (c_size_t, [c_int, c_int]),
+    "fmlh_spotrs": (c_int, [c_void_p, c_int64, c_int, c_void_p, c_int64, c_int, c_void_p, c_size_t,
+                            c_void_p]),
+    "fmlh_add_diagonal": (c_int, [c_void_p, c_int64, c_int, c_double, c_void_p]),
+    "fmlh_sym_residual": (c_int, [c_void_p, c_int64, c_int, c_double, c_void_p, c_void_p, c_void_p,
+                                  c_void_p]),
+    "fmlh_gemv_t": (c_int, [c_void_p, c_int64, c_int, c_int, c_void_p, c_void_p, c_void_p]),
+}
+
+ARAD_GLOBAL, ARAD_SYMMETRIC, ARAD_KERNEL, ARAD_DISTANCE = 0, 1, 2, 3
+ARAD_ATOMIC_KERNELS, ARAD_ATOMIC_DISTANCE = 4, 5
+FMLH_EHIP = -1000
+
+
+class NativeError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load(path: str = LIB_PATH):
+    """Load libfmlhip.so and bind every symbol of the C ABI."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise NativeError(
+            f"{path} is missing: build it with `python -m fml_amd.build` "
+            "(there is no CPU fallback)")
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc: int, fn: str):
+    if rc == 0:
+        return
+    if rc == FMLH_EHIP:
+        raise NativeError(f"{fn}: HIP error: {load().fmlh_last_error().decode()}")
+    raise ValueError(f"{fn}: invalid argument #{-rc}")
+
+
+def call(fn: str, *args):
+    rc = getattr(load(), fn)(*args)
+    check(rc, fn)
+
+
+def torch_mod():
+    import torch
+    if not torch.cuda.is_available():
+        raise NativeError("no HIP device is visible; fml_amd runs on MI355X (gfx950) only")
+    load()
+    return torch
+
+
+def stream_ptr():
+    import torch
+    return c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def ptr(t):
+    """Device pointer of a tensor (None -> NULL)."""
+    if t is None:
+        return c_void_p(0)
+    return c_void_p(t.data_ptr())

@@ -1,0 +1,374 @@
+// (e) Blocked Cholesky factorisation and solve of a symmetric positive
+// definite matrix, LAPACK dpotrf('U') / dpotrs('U') semantics
+// (fml/math/fcho_solve.f90:35,42).
+//
+// Storage: the reference's Fortran-order (n, n) array whose UPPER triangle
+// holds A.  Read as row-major with the same leading dimension, that upper
+// triangle is the LOWER triangle of M (M[r][c] = a[r*lda + c], r >= c), and
+// A = U^T U with U upper becomes M = L L^T with L = U^T lower.  Everything
+// below works on that row-major lower view, so the strictly upper part of M
+// (the strictly lower part of the caller's array) is never touched.
+//
+// Factorisation (right-looking, two block levels):
+//   for each 256-column outer panel:
+//     for each 64-column inner block inside it:
+//       k_potf2   diagonal 64x64 block in LDS (one workgroup)
+//       k_trsm    rows below: M[r][k] <- M[r][k] L_kk^{-T} (one thread/row)
+//       k_update  panel columns right of the block: C -= P Q^T on MFMA
+//     k_update    trailing matrix right of the outer panel, K = 256 on MFMA
+// A non-positive (or NaN) pivot sets *info to its 1-based index, as LAPACK;
+// every later kernel sees *info != 0 and returns, so the factor is left
+// undefined but no kernel waits on anything.
+//
+// Solve: blocked forward (L w = b) and backward (L^T x = w) substitution
+// over 64-row blocks: a multi-workgroup partial product of the already
+// solved part followed by a one-workgroup triangular solve of the block.
+#include <math.h>
+
+#include "common.h"
+#include "gemm_nt.h"
+
+namespace {
+
+using namespace fmlh;
+
+constexpr int kNB = 64;     // inner block
+constexpr int kNBO = 256;   // outer panel
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_potf2(T *__restrict__ a, int64_t lda, int k0, int b,
+                                               int32_t *__restrict__ info) {
+  __shared__ T S[kNB][kNB + 1];
+  __shared__ int fail;
+  if (*info != 0) return;
+  const int tid = threadIdx.x;
+  for (int idx = tid; idx < b * b; idx += 256) {
+    const int r = idx / b, c = idx % b;
+    if (c <= r) S[r][c] = a[(int64_t)(k0 + r) * lda + k0 + c];
+  }
+  if (tid == 0) fail = 0;
+  __syncthreads();
+  for (int j = 0; j < b; ++j) {
+    if (tid == 0) {
+      const T d = S[j][j];
+      if (!(d > T(0)))
+        fail = j + 1;
+      else
+        S[j][j] = sqrt(d);
+    }
+    __syncthreads();
+    if (fail) {
+      if (tid == 0) *info = k0 + fail;
+      return;
+    }
+    const T piv = S[j][j];
+    for (int r = j + 1 + tid; r < b; r += 256) S[r][j] = S[r][j] / piv;
+    __syncthreads();
+    const int w = b - j - 1;
+    for (int idx = tid; idx < w * w; idx += 256) {
+      const int r = j + 1 + idx / w, c = j + 1 + idx % w;
+      if (c <= r) S[r][c] -= S[r][j] * S[c][j];
+    }
+    __syncthreads();
+  }
+  for (int idx = tid; idx < b * b; idx += 256) {
+    const int r = idx / b, c = idx % b;
+    if (c <= r) a[(int64_t)(k0 + r) * lda + k0 + c] = S[r][c];
+  }
+}
+
+// Rows r in [r0, n): x = M[r][k0:k0+b] solves x L^T = m_r (forward
+// substitution against the factored diagonal block, held in LDS).
+template <typename T>
+__global__ __launch_bounds__(256) void k_trsm(T *__restrict__ a, int64_t lda, int n, int k0, int b,
+                                              int r0, const int32_t *__restrict__ info) {
+  __shared__ T L[kNB][kNB + 1];
+  if (*info != 0) return;
+  for (int idx = threadIdx.x; idx < b * b; idx += 256) {
+    const int r = idx / b, c = idx % b;
+    L[r][c] = (c <= r) ? a[(int64_t)(k0 + r) * lda + k0 + c] : T(0);
+  }
+  __syncthreads();
+  const int r = r0 + blockIdx.x * 256 + threadIdx.x;
+  if (r >= n) return;
+  T *row = a + (int64_t)r * lda + k0;
+  T x[kNB];
+#pragma unroll
+  for (int c = 0; c < kNB; ++c) {
+    if (c < b) {
+      T v = row[c];
+#pragma unroll
+      for (int t = 0; t < c; ++t) v -= x[t] * L[c][t];
+      x[c] = v / L[c][c];
+    } else {
+      x[c] = T(0);
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < kNB; ++c)
+    if (c < b) row[c] = x[c];
+}
+
+// C[r][c] -= sum_{t in [kb, ke)} M[r][t] M[c][t] for r in [r0, r1),
+// c in [c0, c1), c <= r.  Grid x over column tiles, y over row tiles.
+template <typename T>
+__global__ __launch_bounds__(256) void k_update(T *__restrict__ a, int64_t lda, int r0, int r1,
+                                                int c0, int c1, int kb, int ke,
+                                                const int32_t *__restrict__ info) {
+  __shared__ TileSmem<T> s;
+  if (*info != 0) return;
+  const int tr = r0 + blockIdx.y * kTile;
+  const int tc = c0 + blockIdx.x * kTile;
+  if (tc > tr + kTile - 1) return;  // tile entirely above the diagonal
+  TileAcc<T> acc;
+  tile_nt<T, false>(a + (int64_t)tr * lda + kb, lda, r1 - tr, a + (int64_t)tc * lda + kb, lda,
+                    c1 - tc, ke - kb, s, acc);
+#pragma unroll
+  for (int im = 0; im < 2; ++im)
+#pragma unroll
+    for (int in = 0; in < 2; ++in)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gr = tr + tile_row(im, r);
+        const int gc = tc + tile_col(in);
+        if (gr < r1 && gc < c1 && gc <= gr) a[(int64_t)gr * lda + gc] -= acc.v[im][in][r];
+      }
+}
+
+__global__ void k_set_int_impl(int32_t *p, int v) { *p = v; }
+
+template <typename T>
+int potrf_impl(T *a, int64_t lda, int n, int32_t *info, hipStream_t st) {
+  hipLaunchKernelGGL(k_set_int_impl, dim3(1), dim3(1), 0, st, info, 0);
+  FMLH_LAUNCH_CHECK("k_set_int");
+  for (int K0 = 0; K0 < n; K0 += kNBO) {
+    const int kend = min(K0 + kNBO, n);
+    for (int k0 = K0; k0 < kend; k0 += kNB) {
+      const int b = min(kNB, kend - k0);
+      hipLaunchKernelGGL(k_potf2<T>, dim3(1), dim3(256), 0, st, a, lda, k0, b, info);
+      FMLH_LAUNCH_CHECK("k_potf2");
+      const int r0 = k0 + b;
+      if (r0 < n) {
+        hipLaunchKernelGGL(k_trsm<T>, dim3(ceil_div(n - r0, 256)), dim3(256), 0, st, a, lda, n, k0,
+                           b, r0, info);
+        FMLH_LAUNCH_CHECK("k_trsm");
+        if (r0 < kend) {  // rest of the outer panel
+          dim3 g(ceil_div(kend - r0, kTile), ceil_div(n - r0, kTile));
+          hipLaunchKernelGGL(k_update<T>, g, dim3(256), 0, st, a, lda, r0, n, r0, kend, k0,
+                             k0 + b, info);
+          FMLH_LAUNCH_CHECK("k_update(panel)");
+        }
+      }
+    }
+    if (kend < n) {
+      dim3 g(ceil_div(n - kend, kTile), ceil_div(n - kend, kTile));
+      hipLaunchKernelGGL(k_update<T>, g, dim3(256), 0, st, a, lda, kend, n, kend, n, K0, kend,
+                         info);
+      FMLH_LAUNCH_CHECK("k_update(trailing)");
+    }
+  }
+  return 0;
+}
+
+// ---- triangular solves -------------------------------------------------
+
+constexpr int kChunk = 256;
+
+// Forward partials: part[ch][r*nrhs + j] = sum_{c in chunk ch, c < k0}
+// M[k0+r][c] x[c + j*ldx].  Each wave owns 16 rows, lanes span columns.
+template <typename T>
+__global__ __launch_bounds__(256) void k_fwd_partial(const T *__restrict__ a, int64_t lda, int k0,
+                                                     int b, const T *__restrict__ x, int64_t ldx,
+                                                     int nrhs, T *__restrict__ part) {
+  const int ch = blockIdx.x;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int cbeg = ch * kChunk;
+  const int cend = min(cbeg + kChunk, k0);
+  for (int j = 0; j < nrhs; ++j) {
+    for (int rr = 0; rr < 16; ++rr) {
+      const int r = wave * 16 + rr;
+      T acc = T(0);
+      if (r < b) {
+        const T *row = a + (int64_t)(k0 + r) * lda;
+        for (int c = cbeg + lane; c < cend; c += 64) acc += row[c] * x[c + (int64_t)j * ldx];
+      }
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+      if (lane == 0 && r < b) part[((int64_t)ch * kNB + r) * nrhs + j] = acc;
+    }
+  }
+}
+
+// Backward partials: part[ch][r*nrhs + j] = sum_{t in chunk ch, t >= k0+b}
+// M[t][k0+r] x[t + j*ldx].  Lanes span r; waves split t, combined in order.
+template <typename T>
+__global__ __launch_bounds__(256) void k_bwd_partial(const T *__restrict__ a, int64_t lda, int n,
+                                                     int k0, int b, const T *__restrict__ x,
+                                                     int64_t ldx, int nrhs, T *__restrict__ part) {
+  __shared__ T red[4][kNB];
+  const int ch = blockIdx.x;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int tbeg = k0 + b + ch * kChunk;
+  const int tend = min(tbeg + kChunk, n);
+  for (int j = 0; j < nrhs; ++j) {
+    T acc = T(0);
+    if (lane < b)
+      for (int t = tbeg + wave; t < tend; t += 4)
+        acc += a[(int64_t)t * lda + k0 + lane] * x[t + (int64_t)j * ldx];
+    red[wave][lane] = acc;
+    __syncthreads();
+    if (threadIdx.x < 64 && lane < b) {
+      const T v = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
+      part[((int64_t)ch * kNB + lane) * nrhs + j] = v;
+    }
+    __syncthreads();
+  }
+}
+
+// Triangular solve of one diagonal block; forward (lower) or backward
+// (transposed).  rhs = x_block - sum_ch part[ch] (ascending ch).
+template <typename T, bool kForward>
+__global__ __launch_bounds__(256) void k_block_solve(const T *__restrict__ a, int64_t lda, int k0,
+                                                     int b, T *__restrict__ x, int64_t ldx,
+                                                     int nrhs, const T *__restrict__ part,
+                                                     int nch) {
+  __shared__ T L[kNB][kNB + 1];
+  __shared__ T rhs[kNB];
+  for (int idx = threadIdx.x; idx < b * b; idx += 256) {
+    const int r = idx / b, c = idx % b;
+    L[r][c] = (c <= r) ? a[(int64_t)(k0 + r) * lda + k0 + c] : T(0);
+  }
+  for (int j = 0; j < nrhs; ++j) {
+    __syncthreads();
+    if ((int)threadIdx.x < b) {
+      const int r = threadIdx.x;
+      T v = x[k0 + r + (int64_t)j * ldx];
+      for (int ch = 0; ch < nch; ++ch) v -= part[((int64_t)ch * kNB + r) * nrhs + j];
+      rhs[r] = v;
+    }
+    __syncthreads();
+    if (kForward) {
+      for (int c = 0; c < b; ++c) {
+        if (threadIdx.x == 0) rhs[c] = rhs[c] / L[c][c];
+        __syncthreads();
+        for (int r = c + 1 + threadIdx.x; r < b; r += 256) rhs[r] -= L[r][c] * rhs[c];
+        __syncthreads();
+      }
+    } else {
+      for (int c = b - 1; c >= 0; --c) {
+        if (threadIdx.x == 0) rhs[c] = rhs[c] / L[c][c];
+        __syncthreads();
+        for (int r = threadIdx.x; r < c; r += 256) rhs[r] -= L[c][r] * rhs[c];
+        __syncthreads();
+      }
+    }
+    if ((int)threadIdx.x < b) x[k0 + threadIdx.x + (int64_t)j * ldx] = rhs[threadIdx.x];
+  }
+}
+
+template <typename T>
+size_t potrs_ws(int n, int nrhs) {
+  return WsCarver::need(sizeof(T) * (size_t)ceil_div(n > 0 ? n : 1, kChunk) * kNB *
+                        (nrhs > 0 ? nrhs : 1));
+}
+
+template <typename T>
+int potrs_impl(const T *a, int64_t lda, int n, T *x, int64_t ldx, int nrhs, void *ws,
+               hipStream_t st) {
+  T *part = static_cast<T *>(ws);
+  const int nblk = ceil_div(n, kNB);
+  for (int kb = 0; kb < nblk; ++kb) {
+    const int k0 = kb * kNB, b = min(kNB, n - k0);
+    const int nch = ceil_div(k0, kChunk);
+    if (nch > 0) {
+      hipLaunchKernelGGL(k_fwd_partial<T>, dim3(nch), dim3(256), 0, st, a, lda, k0, b, x, ldx,
+                         nrhs, part);
+      FMLH_LAUNCH_CHECK("k_fwd_partial");
+    }
+    hipLaunchKernelGGL((k_block_solve<T, true>), dim3(1), dim3(256), 0, st, a, lda, k0, b, x, ldx,
+                       nrhs, part, nch);
+    FMLH_LAUNCH_CHECK("k_block_solve<fwd>");
+  }
+  for (int kb = nblk - 1; kb >= 0; --kb) {
+    const int k0 = kb * kNB, b = min(kNB, n - k0);
+    const int nch = ceil_div(n - (k0 + b), kChunk);
+    if (nch > 0) {
+      hipLaunchKernelGGL(k_bwd_partial<T>, dim3(nch), dim3(256), 0, st, a, lda, n, k0, b, x, ldx,
+                         nrhs, part);
+      FMLH_LAUNCH_CHECK("k_bwd_partial");
+    }
+    hipLaunchKernelGGL((k_block_solve<T, false>), dim3(1), dim3(256), 0, st, a, lda, k0, b, x,
+                       ldx, nrhs, part, nch);
+    FMLH_LAUNCH_CHECK("k_block_solve<bwd>");
+  }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t fmlh_potrf_ws_bytes(int n) { return n >= 0 ? 0 : 0; }
+size_t fmlh_spotrf_ws_bytes(int n) { return n >= 0 ? 0 : 0; }
+size_t fmlh_potrs_ws_bytes(int n, int nrhs) { return potrs_ws<double>(n, nrhs); }
+size_t fmlh_spotrs_ws_bytes(int n, int nrhs) { return potrs_ws<float>(n, nrhs); }
+
+int fmlh_potrf(double *a, int64_t lda, int n, int32_t *info, void *ws, size_t ws_bytes,
+               void *stream) {
+  (void)ws;
+  (void)ws_bytes;
+  FMLH_ARG_CHECK(n >= 0, 3);
+  FMLH_ARG_CHECK(lda >= (n > 1 ? n : 1), 2);
+  FMLH_ARG_CHECK(info != nullptr, 4);
+  if (n > 0) FMLH_ARG_CHECK(a != nullptr, 1);
+  if (n == 0) {
+    hipLaunchKernelGGL(k_set_int_impl, dim3(1), dim3(1), 0, as_stream(stream), info, 0);
+    FMLH_LAUNCH_CHECK("k_set_int");
+    return 0;
+  }
+  return potrf_impl<double>(a, lda, n, info, as_stream(stream));
+}
+
+int fmlh_spotrf(float *a, int64_t lda, int n, int32_t *info, void *ws, size_t ws_bytes,
+                void *stream) {
+  (void)ws;
+  (void)ws_bytes;
+  FMLH_ARG_CHECK(n >= 0, 3);
+  FMLH_ARG_CHECK(lda >= (n > 1 ? n : 1), 2);
+  FMLH_ARG_CHECK(info != nullptr, 4);
+  if (n > 0) FMLH_ARG_CHECK(a != nullptr, 1);
+  if (n == 0) {
+    hipLaunchKernelGGL(k_set_int_impl, dim3(1), dim3(1), 0, as_stream(stream), info, 0);
+    FMLH_LAUNCH_CHECK("k_set_int");
+    return 0;
+  }
+  return potrf_impl<float>(a, lda, n, info, as_stream(stream));
+}
+
+int fmlh_potrs(const double *a, int64_t lda, int n, double *x, int64_t ldx, int nrhs, void *ws,
+               size_t ws_bytes, void *stream) {
+  FMLH_ARG_CHECK(n >= 0, 3);
+  FMLH_ARG_CHECK(lda >= (n > 1 ? n : 1), 2);
+  FMLH_ARG_CHECK(nrhs >= 0, 6);
+  FMLH_ARG_CHECK(ldx >= (n > 1 ? n : 1), 5);
+  if (n == 0 || nrhs == 0) return 0;
+  FMLH_ARG_CHECK(a != nullptr, 1);
+  FMLH_ARG_CHECK(x != nullptr, 4);
+  FMLH_ARG_CHECK(ws != nullptr && ws_bytes >= potrs_ws<double>(n, nrhs), 7);
+  return potrs_impl<double>(a, lda, n, x, ldx, nrhs, ws, as_stream(stream));
+}
+
+int fmlh_spotrs(const float *a, int64_t lda, int n, float *x, int64_t ldx, int nrhs, void *ws,
+                size_t ws_bytes, void *stream) {
+  FMLH_ARG_CHECK(n >= 0, 3);
+  FMLH_ARG_CHECK(lda >= (n > 1 ? n : 1), 2);
+  FMLH_ARG_CHECK(nrhs >= 0, 6);
+  FMLH_ARG_CHECK(ldx >= (n > 1 ? n : 1), 5);
+  if (n == 0 || nrhs == 0) return 0;
+  FMLH_ARG_CHECK(a != nullptr, 1);
+  FMLH_ARG_CHECK(x != nullptr, 4);
+  FMLH_ARG_CHECK(ws != nullptr && ws_bytes >= potrs_ws<float>(n, nrhs), 7);
+  return potrs_impl<float>(a, lda, n, x, ldx, nrhs, ws, as_stream(stream));
+}
+
+}  // extern "C"

@@ -1,0 +1,54 @@
+// Shared helpers for libfmlhip (gfx950 only).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fmlh.h"
+
+namespace fmlh {
+
+// Records a HIP error for fmlh_last_error() and returns FMLH_EHIP.
+int record_error(hipError_t e, const char *where);
+
+// Checks the launch that was just queued.
+inline int check_launch(const char *where) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return record_error(e, where);
+  return 0;
+}
+
+inline int ceil_div(int a, int b) { return (a + b - 1) / b; }
+inline int64_t ceil_div64(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+// 256-byte aligned carve-out from a caller workspace.
+struct WsCarver {
+  char *base;
+  size_t size;
+  size_t used = 0;
+  WsCarver(void *p, size_t n) : base(static_cast<char *>(p)), size(n) {}
+  template <typename T>
+  T *take(size_t count) {
+    size_t off = (used + 255) & ~size_t(255);
+    size_t bytes = count * sizeof(T);
+    if (off + bytes > size || base == nullptr) return nullptr;
+    used = off + bytes;
+    return reinterpret_cast<T *>(base + off);
+  }
+  static size_t need(size_t bytes) { return (bytes + 255) & ~size_t(255); }
+};
+
+}  // namespace fmlh
+
+#define FMLH_ARG_CHECK(cond, idx) \
+  do {                            \
+    if (!(cond)) return -(idx);   \
+  } while (0)
+
+#define FMLH_LAUNCH_CHECK(where)                  \
+  do {                                            \
+    int rc__ = ::fmlh::check_launch(where);       \
+    if (rc__) return rc__;                        \
+  } while (0)

@@ -1,0 +1,351 @@
+// (b) Gaussian and (c) Laplacian kernels, plus the small FP64 helpers the
+// KRR composition uses (diagonal shift, symmetric residual, K^T alpha).
+//
+// Gaussian  K(j,i) = exp(-0.5/sigma^2 * ||a_j - b_i||^2)
+//   fml/kernels/fkernels.f90:317-349.  ||a||^2 + ||b||^2 - 2 a.b with the dot
+//   products on MFMA (64x64 tiles, see gemm_nt.h) and exp fused in the
+//   epilogue; an entry whose GEMM-form distance is within the cancellation
+//   band (d2 <= tau*(||a||^2+||b||^2)) is recomputed by direct differences
+//   in the reference's order (sum of squares ascending over t).
+// Laplacian K(j,i) = exp(-1/sigma * sum_t |a_jt - b_it|)
+//   fml/kernels/fkernels.f90:351-377.  64x64 output tile per 256 threads,
+//   4x4 entries per thread, operands staged k-major in LDS; every entry's
+//   sum runs over t ascending, so the sums equal the reference's sequential
+//   sum() bit for bit.
+#include <math.h>
+
+#include "common.h"
+#include "gemm_nt.h"
+
+namespace {
+
+using namespace fmlh;
+
+template <typename T>
+__device__ __forceinline__ T direct_sqdist(const T *__restrict__ a, const T *__restrict__ b, int d) {
+#pragma clang fp contract(off)
+  T s = T(0);
+  for (int t = 0; t < d; ++t) {
+    T diff = a[t] - b[t];
+    s = s + diff * diff;
+  }
+  return s;
+}
+
+__device__ __forceinline__ double kexp(double x) { return exp(x); }
+__device__ __forceinline__ float kexp(float x) { return expf(x); }
+
+template <typename O>
+__device__ __forceinline__ void kstore(O *p, double v) {
+  __builtin_nontemporal_store((O)v, p);
+}
+
+// Gaussian on MFMA.  Grid: x over A (na) tiles, y over B (nb) tiles.
+// OutT is float or double; the exponent is formed in double for double
+// output.
+template <typename T, typename OutT>
+__global__ __launch_bounds__(256) void k_gaussian(const T *__restrict__ A, int64_t lda, int na,
+                                                  const T *__restrict__ B, int64_t ldb, int nb,
+                                                  int d, double inv, OutT *__restrict__ K,
+                                                  int64_t ldk, T tau) {
+  __shared__ TileSmem<T> s;
+  const int n0 = blockIdx.x * kTile;
+  const int m0 = blockIdx.y * kTile;
+  TileAcc<T> acc;
+  tile_nt<T, true>(B + (int64_t)m0 * ldb, ldb, nb - m0, A + (int64_t)n0 * lda, lda, na - n0, d, s,
+                   acc);
+#pragma unroll
+  for (int im = 0; im < 2; ++im)
+#pragma unroll
+    for (int in = 0; in < 2; ++in)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = tile_row(im, r);
+        const int n = tile_col(in);
+        const int gm = m0 + m, gn = n0 + n;
+        if (gm < nb && gn < na) {
+          const T pn = s.pn[m], qn = s.qn[n];
+          T d2 = pn + qn - T(2) * acc.v[im][in][r];
+          if (d2 <= tau * (pn + qn))
+            d2 = direct_sqdist(A + (int64_t)gn * lda, B + (int64_t)gm * ldb, d);
+          if (d2 < T(0)) d2 = T(0);
+          double v;
+          if (sizeof(OutT) == 8)
+            v = exp(inv * (double)d2);
+          else
+            v = (double)kexp((T)((T)inv * d2));
+          kstore(K + (int64_t)gn + (int64_t)gm * ldk, v);
+        }
+      }
+}
+
+// Laplacian on VALU.  Thread (tx, ty) owns A samples n = tx*4+j and B
+// samples m = ty*4+i of the 64x64 tile.
+constexpr int kLapBK = 16;
+constexpr int kLapStride = 64 + 2;
+
+template <typename T, typename OutT>
+__global__ __launch_bounds__(256) void k_laplacian(const T *__restrict__ A, int64_t lda, int na,
+                                                   const T *__restrict__ B, int64_t ldb, int nb,
+                                                   int d, double inv, OutT *__restrict__ K,
+                                                   int64_t ldk) {
+#pragma clang fp contract(off)
+  __shared__ T As[2][kLapBK * kLapStride];
+  __shared__ T Bs[2][kLapBK * kLapStride];
+  const int tid = threadIdx.x;
+  const int tx = tid & 15, ty = tid >> 4;
+  const int n0 = blockIdx.x * 64, m0 = blockIdx.y * 64;
+  const int lrow = tid >> 2, lk = (tid & 3) * 4;
+  const bool arow_ok = n0 + lrow < na, brow_ok = m0 + lrow < nb;
+  const T *arow = A + (int64_t)(n0 + lrow) * lda;
+  const T *brow = B + (int64_t)(m0 + lrow) * ldb;
+  T ra[4], rb[4];
+  T acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = T(0);
+
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int k = k0 + lk + c;
+      ra[c] = (arow_ok && k < d) ? arow[k] : T(0);
+      rb[c] = (brow_ok && k < d) ? brow[k] : T(0);
+    }
+  };
+  auto lstore = [&](int buf) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      As[buf][(lk + c) * kLapStride + lrow] = ra[c];
+      Bs[buf][(lk + c) * kLapStride + lrow] = rb[c];
+    }
+  };
+  const int nk = (d + kLapBK - 1) / kLapBK;
+  gload(0);
+  lstore(0);
+  __syncthreads();
+  for (int kt = 0; kt < nk; ++kt) {
+    const int buf = kt & 1;
+    if (kt + 1 < nk) gload((kt + 1) * kLapBK);
+    const int kmax = min(kLapBK, d - kt * kLapBK);
+    for (int k = 0; k < kmax; ++k) {
+      T a[4], b[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) a[j] = As[buf][k * kLapStride + tx * 4 + j];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) b[i] = Bs[buf][k * kLapStride + ty * 4 + i];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = acc[i][j] + fabs(a[j] - b[i]);
+    }
+    if (kt + 1 < nk) lstore(buf ^ 1);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int gm = m0 + ty * 4 + i;
+    if (gm >= nb) continue;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int gn = n0 + tx * 4 + j;
+      if (gn >= na) continue;
+      double v;
+      if (sizeof(OutT) == 8)
+        v = exp(inv * (double)acc[i][j]);
+      else
+        v = (double)kexp((T)((T)inv * acc[i][j]));
+      kstore(K + (int64_t)gn + (int64_t)gm * ldk, v);
+    }
+  }
+}
+
+__global__ void k_add_diagonal(double *a, int64_t lda, int n, double lambda) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[(int64_t)i + (int64_t)i * lda] += lambda;
+}
+
+// Upper-triangle symmetric product, part 1: t_i = sum_{j>i} A(i,j) x_j.
+// Thread i walks j; consecutive threads read consecutive addresses.
+__global__ __launch_bounds__(256) void k_symv_rows(const double *__restrict__ a, int64_t lda, int n,
+                                                   const double *__restrict__ x,
+                                                   double *__restrict__ t) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = 0.0;
+  for (int j = i + 1; j < n; ++j) s += a[(int64_t)i + (int64_t)j * lda] * x[j];
+  t[i] = s;
+}
+
+// Part 2 and the residual: one workgroup per column j,
+// r_j = y_j - (sum_{i<=j} A(i,j) x_i + t_j + lambda x_j).
+__global__ __launch_bounds__(256) void k_symv_cols_residual(
+    const double *__restrict__ a, int64_t lda, int n, double lambda,
+    const double *__restrict__ x, const double *__restrict__ y, const double *t, double *r) {
+  __shared__ double part[256];
+  const int j = blockIdx.x;
+  const double *col = a + (int64_t)j * lda;
+  double s = 0.0;
+  for (int i = threadIdx.x; i <= j; i += 256) s += col[i] * x[i];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) r[j] = y[j] - (part[0] + t[j] + lambda * x[j]);
+}
+
+// y_i = sum_j K(j,i) alpha_j for K (na, nb) Fortran order; one workgroup
+// per output, fixed reduction order.
+__global__ __launch_bounds__(256) void k_gemv_t(const double *__restrict__ k, int64_t ldk, int na,
+                                                const double *__restrict__ alpha,
+                                                double *__restrict__ y) {
+  __shared__ double part[256];
+  const int i = blockIdx.x;
+  const double *col = k + (int64_t)i * ldk;
+  double s = 0.0;
+  for (int j = threadIdx.x; j < na; j += 256) s += col[j] * alpha[j];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) y[i] = part[0];
+}
+
+template <typename T>
+int check_dense_args(const T *a, int64_t lda, int na, const T *b, int64_t ldb, int nb, int d,
+                     double sigma, const void *k, int64_t ldk) {
+  FMLH_ARG_CHECK(na >= 0, 3);
+  FMLH_ARG_CHECK(nb >= 0, 6);
+  FMLH_ARG_CHECK(d >= 0, 7);
+  FMLH_ARG_CHECK(lda >= d, 2);
+  FMLH_ARG_CHECK(ldb >= d, 5);
+  FMLH_ARG_CHECK(sigma != 0.0, 8);
+  FMLH_ARG_CHECK(ldk >= na, 10);
+  if (na > 0 && nb > 0) {
+    FMLH_ARG_CHECK(a != nullptr || d == 0, 1);
+    FMLH_ARG_CHECK(b != nullptr || d == 0, 4);
+    FMLH_ARG_CHECK(k != nullptr, 9);
+  }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fmlh_gaussian_kernel(const double *a, int64_t lda, int na, const double *b, int64_t ldb,
+                         int nb, int d, double sigma, double *k, int64_t ldk, void *stream) {
+  int rc = check_dense_args(a, lda, na, b, ldb, nb, d, sigma, k, ldk);
+  if (rc) return rc;
+  if (na == 0 || nb == 0) return 0;
+  const double inv = -0.5 / (sigma * sigma);
+  dim3 grid(ceil_div(na, kTile), ceil_div(nb, kTile));
+  hipLaunchKernelGGL((k_gaussian<double, double>), grid, dim3(256), 0, as_stream(stream), a, lda,
+                     na, b, ldb, nb, d, inv, k, ldk, 1e-3);
+  FMLH_LAUNCH_CHECK("k_gaussian<f64>");
+  return 0;
+}
+
+int fmlh_gaussian_kernel_bf16x3(const float *a, int64_t lda, int na, const float *b, int64_t ldb,
+                                int nb, int d, double sigma, float *k32, double *k64, int64_t ldk,
+                                void *stream);
+
+int fmlh_gaussian_kernel_f32(const float *a, int64_t lda, int na, const float *b, int64_t ldb,
+                             int nb, int d, double sigma, int mode, float *k32, double *k64,
+                             int64_t ldk, void *stream) {
+  int rc = check_dense_args(a, lda, na, b, ldb, nb, d, sigma, k32 ? (void *)k32 : (void *)k64, ldk);
+  if (rc) return rc;
+  FMLH_ARG_CHECK(mode == 0 || mode == 1, 9);
+  FMLH_ARG_CHECK((k32 == nullptr) != (k64 == nullptr), 10);
+  if (na == 0 || nb == 0) return 0;
+  if (mode == 1) return fmlh_gaussian_kernel_bf16x3(a, lda, na, b, ldb, nb, d, sigma, k32, k64, ldk, stream);
+  const double inv = -0.5 / (sigma * sigma);
+  dim3 grid(ceil_div(na, kTile), ceil_div(nb, kTile));
+  if (k32)
+    hipLaunchKernelGGL((k_gaussian<float, float>), grid, dim3(256), 0, as_stream(stream), a, lda,
+                       na, b, ldb, nb, d, inv, k32, ldk, 1e-2f);
+  else
+    hipLaunchKernelGGL((k_gaussian<float, double>), grid, dim3(256), 0, as_stream(stream), a, lda,
+                       na, b, ldb, nb, d, inv, k64, ldk, 1e-2f);
+  FMLH_LAUNCH_CHECK("k_gaussian<f32>");
+  return 0;
+}
+
+int fmlh_laplacian_kernel(const double *a, int64_t lda, int na, const double *b, int64_t ldb,
+                          int nb, int d, double sigma, double *k, int64_t ldk, void *stream) {
+  int rc = check_dense_args(a, lda, na, b, ldb, nb, d, sigma, k, ldk);
+  if (rc) return rc;
+  if (na == 0 || nb == 0) return 0;
+  const double inv = -1.0 / sigma;
+  dim3 grid(ceil_div(na, 64), ceil_div(nb, 64));
+  hipLaunchKernelGGL((k_laplacian<double, double>), grid, dim3(256), 0, as_stream(stream), a, lda,
+                     na, b, ldb, nb, d, inv, k, ldk);
+  FMLH_LAUNCH_CHECK("k_laplacian<f64>");
+  return 0;
+}
+
+int fmlh_laplacian_kernel_f32(const float *a, int64_t lda, int na, const float *b, int64_t ldb,
+                              int nb, int d, double sigma, float *k32, double *k64, int64_t ldk,
+                              void *stream) {
+  int rc = check_dense_args(a, lda, na, b, ldb, nb, d, sigma, k32 ? (void *)k32 : (void *)k64, ldk);
+  if (rc) return rc;
+  FMLH_ARG_CHECK((k32 == nullptr) != (k64 == nullptr), 9);
+  if (na == 0 || nb == 0) return 0;
+  const double inv = -1.0 / sigma;
+  dim3 grid(ceil_div(na, 64), ceil_div(nb, 64));
+  if (k32)
+    hipLaunchKernelGGL((k_laplacian<float, float>), grid, dim3(256), 0, as_stream(stream), a, lda,
+                       na, b, ldb, nb, d, inv, k32, ldk);
+  else
+    hipLaunchKernelGGL((k_laplacian<float, double>), grid, dim3(256), 0, as_stream(stream), a, lda,
+                       na, b, ldb, nb, d, inv, k64, ldk);
+  FMLH_LAUNCH_CHECK("k_laplacian<f32>");
+  return 0;
+}
+
+int fmlh_add_diagonal(double *a, int64_t lda, int n, double lambda, void *stream) {
+  FMLH_ARG_CHECK(n >= 0, 3);
+  FMLH_ARG_CHECK(lda >= n, 2);
+  if (n == 0) return 0;
+  FMLH_ARG_CHECK(a != nullptr, 1);
+  hipLaunchKernelGGL(k_add_diagonal, dim3(ceil_div(n, 256)), dim3(256), 0, as_stream(stream), a,
+                     lda, n, lambda);
+  FMLH_LAUNCH_CHECK("k_add_diagonal");
+  return 0;
+}
+
+int fmlh_sym_residual(const double *a, int64_t lda, int n, double lambda, const double *x,
+                      const double *y, double *r, void *stream) {
+  FMLH_ARG_CHECK(n >= 0, 3);
+  FMLH_ARG_CHECK(lda >= n, 2);
+  if (n == 0) return 0;
+  FMLH_ARG_CHECK(a && x && y && r, 1);
+  // r doubles as the scratch vector t: each column's workgroup reads t_j
+  // and then writes r_j = ... for the same j only.
+  hipLaunchKernelGGL(k_symv_rows, dim3(ceil_div(n, 256)), dim3(256), 0, as_stream(stream), a, lda,
+                     n, x, r);
+  FMLH_LAUNCH_CHECK("k_symv_rows");
+  hipLaunchKernelGGL(k_symv_cols_residual, dim3(n), dim3(256), 0, as_stream(stream), a, lda, n,
+                     lambda, x, y, r, r);
+  FMLH_LAUNCH_CHECK("k_symv_cols_residual");
+  return 0;
+}
+
+int fmlh_gemv_t(const double *k, int64_t ldk, int na, int nb, const double *alpha, double *y,
+                void *stream) {
+  FMLH_ARG_CHECK(na >= 0, 3);
+  FMLH_ARG_CHECK(nb >= 0, 4);
+  FMLH_ARG_CHECK(ldk >= na, 2);
+  if (nb == 0) return 0;
+  FMLH_ARG_CHECK(k && alpha && y, 1);
+  hipLaunchKernelGGL(k_gemv_t, dim3(nb), dim3(256), 0, as_stream(stream), k, ldk, na, alpha, y);
+  FMLH_LAUNCH_CHECK("k_gemv_t");
+  return 0;
+}
+
+}  // extern "C"

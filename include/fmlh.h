@@ -1,0 +1,202 @@
+/*
+ * fmlh.h -- C ABI of libfmlhip.so, the MI355X (gfx950) implementation of the
+ * fml kernel-ridge-regression hot path.
+ *
+ * Conventions
+ *   - Every pointer argument is DEVICE memory unless named *_host.
+ *   - `stream` is a hipStream_t passed as void* (NULL = the null stream).
+ *     Every call is asynchronous on that stream unless documented otherwise.
+ *   - Return value: 0 on success; -k when argument k (1-based) is invalid;
+ *     FMLH_EHIP when a HIP runtime call failed (see fmlh_last_error()).
+ *   - Matrices are described by (pointer, leading dimension in elements).
+ *     "Sample-major" descriptor sets hold one sample per row of `ld`
+ *     elements, which is the memory layout of the reference's Fortran
+ *     (d, n) column-per-sample arrays.
+ *   - Dense kernel outputs are Fortran ordered like the reference's:
+ *     K(j, i) for sample j of A and sample i of B lives at k[j + i*ldk].
+ *   - No entry point allocates device memory.  Scratch space is passed in by
+ *     the caller (`ws`, `ws_bytes`) and sized by the matching *_ws_bytes().
+ */
+#ifndef FMLH_H
+#define FMLH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FMLH_EHIP (-1000)
+
+/* ABI version (bumped on any signature change). */
+int fmlh_abi_version(void);
+/* Text of the last HIP error seen by this thread ("" if none). */
+const char *fmlh_last_error(void);
+/* Number of compute units of the current device (0 on error). */
+int fmlh_device_cu_count(void);
+
+/* ------------------------------------------------------------------------
+ * (a) Sorted Coulomb matrices, one workgroup per molecule.
+ * Replaces fgenerate_coulomb_matrix
+ *   (fml/representations/frepresentations.f90:23-110; Python entry
+ *    Molecule.generate_coulomb_matrix, fml/fml.py:64-66).
+ * z:       concatenated nuclear charges (total atoms) f64
+ * r:       concatenated coordinates (total atoms, 3) f64 row-major
+ * offsets: nmol+1 int32 prefix offsets into z / r
+ * cm:      (nmol, nmax*(nmax+1)/2) f64 row-major; each row is the packed
+ *          row-major lower triangle of the row-norm-sorted matrix
+ * cm32:    optional (may be NULL) float copy of cm, same layout
+ * perm:    optional (may be NULL) (nmol, nmax) int32: sorted position ->
+ *          original atom index (0-based), -1 padded
+ * status:  optional (may be NULL) (nmol) int32: 0 ok, 1 natoms > nmax
+ *          (that molecule's row is left zero; the reference writes out of
+ *          bounds there, frepresentations.f90:96-103)
+ * ---------------------------------------------------------------------- */
+int fmlh_coulomb_sorted(const double *z, const double *r, const int32_t *offsets,
+                        int nmol, int nmax, double *cm, float *cm32,
+                        int32_t *perm, int32_t *status, void *stream);
+
+/* ------------------------------------------------------------------------
+ * (b) Gaussian kernel K(j,i) = exp(-0.5/sigma^2 * ||a_j - b_i||^2)
+ * Replaces fgaussian_kernel (fml/kernels/fkernels.f90:317-349; Python entry
+ * gaussian_kernel, fml/kernels/kernels.py:167-196).
+ * FP64: row norms + FP64 MFMA GEMM + fused exp epilogue; pairs whose
+ * GEMM-form distance is within the cancellation band are recomputed by
+ * direct differences.
+ * ---------------------------------------------------------------------- */
+int fmlh_gaussian_kernel(const double *a, int64_t lda, int na,
+                         const double *b, int64_t ldb, int nb, int d,
+                         double sigma, double *k, int64_t ldk, void *stream);
+
+/* Precision forms on float descriptors.  mode: 0 = FP32 MFMA products with
+ * FP32 accumulation; 1 = split-BF16x3 MFMA (hi*hi + hi*lo + lo*hi) with FP32
+ * accumulation.  Both recompute near pairs by direct FP32 differences.
+ * Output is float (k32) or double (k64); exactly one must be non-NULL. */
+int fmlh_gaussian_kernel_f32(const float *a, int64_t lda, int na,
+                             const float *b, int64_t ldb, int nb, int d,
+                             double sigma, int mode, float *k32, double *k64,
+                             int64_t ldk, void *stream);
+
+/* ------------------------------------------------------------------------
+ * (c) Laplacian kernel K(j,i) = exp(-1/sigma * sum_t |a_jt - b_it|)
+ * Replaces flaplacian_kernel (fml/kernels/fkernels.f90:351-377; Python entry
+ * laplacian_kernel, fml/kernels/kernels.py:136-164).  LDS-tiled VALU L1
+ * distance; each entry's sum runs over t in ascending order exactly as the
+ * reference's sum() does.
+ * ---------------------------------------------------------------------- */
+int fmlh_laplacian_kernel(const double *a, int64_t lda, int na,
+                          const double *b, int64_t ldb, int nb, int d,
+                          double sigma, double *k, int64_t ldk, void *stream);
+int fmlh_laplacian_kernel_f32(const float *a, int64_t lda, int na,
+                              const float *b, int64_t ldb, int nb, int d,
+                              double sigma, float *k32, double *k64,
+                              int64_t ldk, void *stream);
+
+/* ------------------------------------------------------------------------
+ * (d) ARAD kernels.
+ * Molecular descriptor sets: q (nm, amax, 5, amax) f64 C order, i.e. atom
+ * (i, a) owns a contiguous (5, amax) channel-major block; zt (nm, amax, 2)
+ * f64 PTP (row, col) types; n (nm) int32 atom counts.
+ * mode selects the reference routine reproduced:
+ *   FMLH_ARAD_GLOBAL     fget_kernels_arad           farad_kernels.f90:439-616
+ *                        out (nsig, nm1, nm2) Fortran order
+ *   FMLH_ARAD_SYMMETRIC  fget_symmetric_kernels_arad farad_kernels.f90:619-755
+ *                        (set 2 ignored) out (nsig, nm1, nm1) Fortran order
+ *   FMLH_ARAD_KERNEL     fget_kernel_arad            farad_kernels.f90:95-227
+ *                        (set 2 ignored; one sigma; width .2, cut 6, rw 1,
+ *                        cw .5 forced) out (nm1, nm1) Fortran order
+ *   FMLH_ARAD_DISTANCE   atomic_arad_l2_distance_all farad.f90:452-600
+ *                        out (nm1, nm2, amax, amax) Fortran order
+ *                        (sigmas unused)
+ * Python entries: get_atomic_kernels_arad / get_atomic_symmetric_kernels_arad
+ * (fml/kernels/kernels.py:229-349), get_l2_distance_arad
+ * (fml/math/distance.py:120-189).
+ * ---------------------------------------------------------------------- */
+#define FMLH_ARAD_GLOBAL 0
+#define FMLH_ARAD_SYMMETRIC 1
+#define FMLH_ARAD_KERNEL 2
+#define FMLH_ARAD_DISTANCE 3
+
+/* precision: 0 = FP64 throughout (the plain form); 1 = FP32 inner overlap
+ * (exp, products and accumulation in float), FP64 self terms, distance
+ * combination and kernel sums. */
+size_t fmlh_arad_ws_bytes(int nm1, int nm2, int amax, int mode);
+int fmlh_arad_kernels(int mode, const double *q1, const double *zt1,
+                      const int32_t *n1, int nm1, const double *q2,
+                      const double *zt2, const int32_t *n2, int nm2, int amax,
+                      const double *sigmas_host, int nsig, double width,
+                      double cut, double r_width, double c_width,
+                      int precision, double *out, void *ws, size_t ws_bytes,
+                      void *stream);
+
+/* Atomic (per-atom) forms.  q (na, 5, amax), zt (na, 2), n (na) int32 =
+ * atom count of the molecule each atom belongs to.
+ *   FMLH_ARAD_ATOMIC_KERNELS  fget_atomic_kernels_arad
+ *                             farad_kernels.f90:757-899
+ *                             out (nsig, na1, na2) Fortran order
+ *   FMLH_ARAD_ATOMIC_DISTANCE fget_atomic_distance_arad
+ *                             farad_kernels.f90:902-1030, written as the
+ *                             well-defined (na1, na2) Fortran-order D(i,j)
+ *                             (the reference stores kernel(j,i) into an
+ *                             (na1,na2) array, farad_kernels.f90:932,1019) */
+#define FMLH_ARAD_ATOMIC_KERNELS 4
+#define FMLH_ARAD_ATOMIC_DISTANCE 5
+size_t fmlh_arad_atomic_ws_bytes(int na1, int na2, int amax);
+int fmlh_arad_atomic(int mode, const double *q1, const double *zt1,
+                     const int32_t *n1, int na1, const double *q2,
+                     const double *zt2, const int32_t *n2, int na2, int amax,
+                     const double *sigmas_host, int nsig, double width,
+                     double cut, double r_width, double c_width, double *out,
+                     void *ws, size_t ws_bytes, void *stream);
+
+/* ------------------------------------------------------------------------
+ * (e) Cholesky factorisation and solve of a symmetric positive definite
+ * matrix held in Fortran order, reading and writing only its UPPER
+ * triangle: A = U^T U, as LAPACK dpotrf('U') / dpotrs('U').
+ * Replaces fcho_solve (fml/math/fcho_solve.f90:23-47 -> dpotrf line 35,
+ * dpotrs line 42; Python entry cho_solve, fml/math/cho_solve.py:33-58).
+ * a: (n, n) Fortran order, lda >= n; overwritten by U in its upper triangle
+ * (the strictly lower part is not touched).
+ * info (device int32): 0, or k > 0 when the leading minor of order k is not
+ * positive definite (as LAPACK).  The factor is undefined after a failure.
+ * ---------------------------------------------------------------------- */
+size_t fmlh_potrf_ws_bytes(int n);
+int fmlh_potrf(double *a, int64_t lda, int n, int32_t *info, void *ws,
+               size_t ws_bytes, void *stream);
+/* Solve U^T U x = b in place for nrhs right-hand sides, x (n, nrhs) Fortran
+ * order with leading dimension ldx. */
+size_t fmlh_potrs_ws_bytes(int n, int nrhs);
+int fmlh_potrs(const double *a, int64_t lda, int n, double *x, int64_t ldx,
+               int nrhs, void *ws, size_t ws_bytes, void *stream);
+
+/* Single-precision factor for the mixed-precision form (same layout and
+ * semantics on float data). */
+size_t fmlh_spotrf_ws_bytes(int n);
+int fmlh_spotrf(float *a, int64_t lda, int n, int32_t *info, void *ws,
+                size_t ws_bytes, void *stream);
+size_t fmlh_spotrs_ws_bytes(int n, int nrhs);
+int fmlh_spotrs(const float *a, int64_t lda, int n, float *x, int64_t ldx,
+                int nrhs, void *ws, size_t ws_bytes, void *stream);
+
+/* ------------------------------------------------------------------------
+ * Helpers for the KRR composition (no single reference counterpart; the
+ * README's user pattern, README.md:33-69).
+ * ---------------------------------------------------------------------- */
+/* a(i,i) += lambda for i < n (Fortran order, lda). */
+int fmlh_add_diagonal(double *a, int64_t lda, int n, double lambda,
+                      void *stream);
+/* r = y - (A + lambda I) x using only A's upper triangle (symmetric), all
+ * in FP64.  A is the UNfactored matrix (n, n) Fortran order. */
+int fmlh_sym_residual(const double *a, int64_t lda, int n, double lambda,
+                      const double *x, const double *y, double *r,
+                      void *stream);
+/* y(i) = sum_j K(j, i) * alpha(j) for K (na, nb) Fortran order: the
+ * prediction K_test^T alpha. */
+int fmlh_gemv_t(const double *k, int64_t ldk, int na, int nb,
+                const double *alpha, double *y, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FMLH_H */

@@ -1,0 +1,371 @@
+"""Test oracle for the fml KRR hot path -- test infrastructure only.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` may import this package, and only as the checker (or the timed
+CPU baseline); the product package ``fml_amd`` never does.
+
+Three independent sources:
+
+* ``liboracle_fml.so`` -- C restatement of the reference Fortran loops
+  (oracle/fml_oracle.c), IEEE order, no FP contraction.
+* ``_ref/libfml_ref.so`` -- the reference's own Fortran compiled unchanged by
+  oracle/Makefile (present only where the reference tree was available at
+  build time; it travels with the tree as a built artefact).
+* pure numpy / Python restatements in :mod:`oracle.pyref` (ARAD descriptor,
+  LAPACK ``dpotrf``/``dpotrs`` through scipy for ``cho_solve``).
+
+Layout conventions follow the reference Python API: descriptor matrices for
+the dense kernels are ``(d, n)`` with one sample per column, kernels come back
+Fortran ordered ``(na, nb)``; ARAD descriptors are ``(nm, amax, 5, amax)``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_c_int_p = ctypes.POINTER(ctypes.c_int)
+_c_dbl_p = ctypes.POINTER(ctypes.c_double)
+
+
+def _dp(a):
+    return a.ctypes.data_as(_c_dbl_p)
+
+
+def _ip(a):
+    return a.ctypes.data_as(_c_int_p)
+
+
+def build():
+    """Compile liboracle_fml.so (and the reference build when its tree exists)."""
+    import subprocess
+    target = "ref" if os.path.isdir("/root/reference/fml") else "all"
+    subprocess.run(["make", "-s", "-C", _HERE, target], check=True)
+
+
+def _load(path):
+    lib = ctypes.CDLL(path)
+    return lib
+
+
+_lib = None
+_ref = {}
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        p = os.path.join(_HERE, "liboracle_fml.so")
+        if not os.path.exists(p):
+            build()
+        _lib = _load(p)
+        D, I = ctypes.c_double, ctypes.c_int
+        _lib.oracle_coulomb_matrix.argtypes = [_c_dbl_p, _c_dbl_p, I, I, _c_dbl_p, _c_int_p]
+        _lib.oracle_coulomb_matrices.argtypes = [_c_dbl_p, _c_dbl_p, _c_int_p, I, I, _c_dbl_p, _c_int_p]
+        for f in ("oracle_gaussian_kernel", "oracle_laplacian_kernel"):
+            getattr(_lib, f).argtypes = [_c_dbl_p, I, _c_dbl_p, I, I, D, _c_dbl_p]
+            getattr(_lib, f).restype = None
+        _lib.oracle_kernels_arad.argtypes = [_c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_int_p, _c_int_p,
+                                             _c_dbl_p, I, I, I, I, D, D, D, D, _c_dbl_p]
+        _lib.oracle_symmetric_kernels_arad.argtypes = [_c_dbl_p, _c_dbl_p, _c_int_p, _c_dbl_p, I, I, I,
+                                                       D, D, D, D, _c_dbl_p]
+        _lib.oracle_kernel_arad.argtypes = [_c_dbl_p, _c_dbl_p, _c_int_p, D, I, I, _c_dbl_p]
+        _lib.oracle_atomic_kernels_arad.argtypes = [_c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_int_p,
+                                                    _c_int_p, _c_dbl_p, I, I, I, I, D, D, D, D, _c_dbl_p]
+        _lib.oracle_atomic_distance_arad.argtypes = [_c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_int_p,
+                                                     _c_int_p, I, I, I, D, D, D, D, _c_dbl_p]
+        _lib.oracle_arad_l2_distance_all.argtypes = [_c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_int_p,
+                                                     _c_int_p, I, I, I, D, D, D, D, _c_dbl_p]
+        _lib.oracle_cho_solve.argtypes = [_c_dbl_p, I, _c_dbl_p, _c_dbl_p]
+        for f in ("oracle_kernels_arad", "oracle_symmetric_kernels_arad", "oracle_kernel_arad",
+                  "oracle_atomic_kernels_arad", "oracle_atomic_distance_arad",
+                  "oracle_arad_l2_distance_all"):
+            getattr(_lib, f).restype = None
+    return _lib
+
+
+def ref_available(variant: str = "libfml_ref") -> bool:
+    return os.path.exists(os.path.join(_HERE, "_ref", variant + ".so"))
+
+
+def ref(variant: str = "libfml_ref"):
+    """ctypes handle on the flang build of the reference Fortran."""
+    if variant not in _ref:
+        p = os.path.join(_HERE, "_ref", variant + ".so")
+        if not os.path.exists(p):
+            raise FileNotFoundError(p + " (build with: make -C oracle ref)")
+        r = _load(p)
+        D, I = ctypes.c_double, ctypes.c_int
+        r.ref_coulomb_matrix.argtypes = [_c_dbl_p, _c_dbl_p, I, I, _c_dbl_p]
+        for f in ("ref_gaussian_kernel", "ref_laplacian_kernel"):
+            getattr(r, f).argtypes = [_c_dbl_p, I, _c_dbl_p, I, I, _c_dbl_p, D]
+        r.ref_kernels_arad.argtypes = [_c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_int_p, _c_int_p,
+                                       _c_dbl_p, I, I, I, I, D, D, D, D, _c_dbl_p]
+        r.ref_symmetric_kernels_arad.argtypes = [_c_dbl_p, _c_dbl_p, _c_int_p, _c_dbl_p, I, I, I,
+                                                 D, D, D, D, _c_dbl_p]
+        r.ref_kernel_arad.argtypes = [_c_dbl_p, _c_dbl_p, _c_int_p, D, I, I, _c_dbl_p]
+        r.ref_atomic_kernels_arad.argtypes = [_c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_int_p, _c_int_p,
+                                              _c_dbl_p, I, I, I, I, D, D, D, D, _c_dbl_p]
+        r.ref_atomic_distance_arad.argtypes = [_c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_int_p, _c_int_p,
+                                               I, I, I, D, D, D, D, _c_dbl_p]
+        r.ref_arad_l2_distance_all.argtypes = [_c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_int_p, _c_int_p,
+                                               I, I, I, D, D, D, D, _c_dbl_p]
+        r.ref_cho_solve.argtypes = [_c_dbl_p, I, _c_dbl_p, _c_dbl_p]
+        _ref[variant] = r
+    return _ref[variant]
+
+
+# ---------------------------------------------------------------------------
+# Coulomb matrix
+# ---------------------------------------------------------------------------
+
+def coulomb_matrix(Z, R, nmax):
+    """Sorted Coulomb matrix of one molecule -> (cm[nmax(nmax+1)/2], perm[n])."""
+    Z = np.ascontiguousarray(Z, dtype=np.float64)
+    R = np.ascontiguousarray(R, dtype=np.float64).reshape(-1, 3)
+    n = Z.shape[0]
+    cm = np.zeros(nmax * (nmax + 1) // 2)
+    perm = np.zeros(max(n, 1), dtype=np.int32)
+    rc = lib().oracle_coulomb_matrix(_dp(Z), _dp(R), n, nmax, _dp(cm), _ip(perm))
+    if rc != 0:
+        raise ValueError("natoms > nmax")
+    return cm, perm[:n]
+
+
+def coulomb_matrices(Zcat, Rcat, offsets, nmax):
+    Zcat = np.ascontiguousarray(Zcat, dtype=np.float64)
+    Rcat = np.ascontiguousarray(Rcat, dtype=np.float64).reshape(-1, 3)
+    offsets = np.ascontiguousarray(offsets, dtype=np.int32)
+    nmol = offsets.shape[0] - 1
+    cm = np.zeros((nmol, nmax * (nmax + 1) // 2))
+    perm = np.zeros((nmol, nmax), dtype=np.int32)
+    rc = lib().oracle_coulomb_matrices(_dp(Zcat), _dp(Rcat), _ip(offsets), nmol, nmax, _dp(cm), _ip(perm))
+    if rc != 0:
+        raise ValueError("natoms > nmax")
+    return cm, perm
+
+
+def ref_coulomb_matrix(Z, R, nmax, variant="libfml_ref"):
+    Z = np.ascontiguousarray(Z, dtype=np.float64)
+    R = np.asfortranarray(np.asarray(R, dtype=np.float64).reshape(-1, 3))
+    cm = np.zeros(nmax * (nmax + 1) // 2)
+    ref(variant).ref_coulomb_matrix(_dp(Z), _dp(R), Z.shape[0], nmax, _dp(cm))
+    return cm
+
+
+# ---------------------------------------------------------------------------
+# Dense kernels; A, B are (d, n) with one sample per column
+# ---------------------------------------------------------------------------
+
+def _kernel(fn, A, B, sigma):
+    At = np.ascontiguousarray(np.asarray(A, dtype=np.float64).T)
+    Bt = np.ascontiguousarray(np.asarray(B, dtype=np.float64).T)
+    na, d = At.shape
+    nb = Bt.shape[0]
+    K = np.zeros((na, nb), order="F")
+    fn(_dp(At), na, _dp(Bt), nb, d, float(sigma), K.ctypes.data_as(_c_dbl_p))
+    return K
+
+
+def gaussian_kernel(A, B, sigma):
+    return _kernel(lib().oracle_gaussian_kernel, A, B, sigma)
+
+
+def laplacian_kernel(A, B, sigma):
+    return _kernel(lib().oracle_laplacian_kernel, A, B, sigma)
+
+
+def _ref_kernel(fn, A, B, sigma):
+    Af = np.asfortranarray(np.asarray(A, dtype=np.float64))
+    Bf = np.asfortranarray(np.asarray(B, dtype=np.float64))
+    d, na = Af.shape
+    nb = Bf.shape[1]
+    K = np.zeros((na, nb), order="F")
+    fn(_dp(Af), na, _dp(Bf), nb, d, K.ctypes.data_as(_c_dbl_p), float(sigma))
+    return K
+
+
+def ref_gaussian_kernel(A, B, sigma, variant="libfml_ref"):
+    return _ref_kernel(ref(variant).ref_gaussian_kernel, A, B, sigma)
+
+
+def ref_laplacian_kernel(A, B, sigma, variant="libfml_ref"):
+    return _ref_kernel(ref(variant).ref_laplacian_kernel, A, B, sigma)
+
+
+# ---------------------------------------------------------------------------
+# ARAD kernels.  q: (nm, amax, 5, amax); za: (nm, amax, 2) PTP (row, col);
+# n: int32 atom counts.
+# ---------------------------------------------------------------------------
+
+def _c(a, dt=np.float64):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+def kernels_arad(q1, q2, z1, z2, n1, n2, sigmas, width=0.2, cut=6.0, rw=1.0, cw=0.5):
+    q1, q2, z1, z2 = _c(q1), _c(q2), _c(z1), _c(z2)
+    n1, n2, sig = _c(n1, np.int32), _c(n2, np.int32), _c(sigmas)
+    nm1, nm2, amax, ns = q1.shape[0], q2.shape[0], q1.shape[1], sig.shape[0]
+    out = np.zeros((ns, nm1, nm2), order="F")
+    lib().oracle_kernels_arad(_dp(q1), _dp(q2), _dp(z1), _dp(z2), _ip(n1), _ip(n2), _dp(sig),
+                              nm1, nm2, ns, amax, width, cut, rw, cw, out.ctypes.data_as(_c_dbl_p))
+    return out
+
+
+def symmetric_kernels_arad(q1, z1, n1, sigmas, width=0.2, cut=6.0, rw=1.0, cw=0.5):
+    q1, z1, n1, sig = _c(q1), _c(z1), _c(n1, np.int32), _c(sigmas)
+    nm1, amax, ns = q1.shape[0], q1.shape[1], sig.shape[0]
+    out = np.zeros((ns, nm1, nm1), order="F")
+    lib().oracle_symmetric_kernels_arad(_dp(q1), _dp(z1), _ip(n1), _dp(sig), nm1, ns, amax,
+                                        width, cut, rw, cw, out.ctypes.data_as(_c_dbl_p))
+    return out
+
+
+def kernel_arad(q, z, n, sigma):
+    q, z, n = _c(q), _c(z), _c(n, np.int32)
+    nm, amax = q.shape[0], q.shape[1]
+    out = np.zeros((nm, nm), order="F")
+    lib().oracle_kernel_arad(_dp(q), _dp(z), _ip(n), float(sigma), nm, amax, out.ctypes.data_as(_c_dbl_p))
+    return out
+
+
+def atomic_kernels_arad(q1, q2, z1, z2, n1, n2, sigmas, width=0.2, cut=6.0, rw=1.0, cw=0.5):
+    """q: (na, 5, amax) per-atom descriptors, z: (na, 2), n: (na,) int32."""
+    q1, q2, z1, z2 = _c(q1), _c(q2), _c(z1), _c(z2)
+    n1, n2, sig = _c(n1, np.int32), _c(n2, np.int32), _c(sigmas)
+    na1, na2, amax, ns = q1.shape[0], q2.shape[0], q1.shape[2], sig.shape[0]
+    out = np.zeros((ns, na1, na2), order="F")
+    lib().oracle_atomic_kernels_arad(_dp(q1), _dp(q2), _dp(z1), _dp(z2), _ip(n1), _ip(n2), _dp(sig),
+                                     na1, na2, ns, amax, width, cut, rw, cw,
+                                     out.ctypes.data_as(_c_dbl_p))
+    return out
+
+
+def atomic_distance_arad(q1, q2, z1, z2, n1, n2, width=0.2, cut=6.0, rw=1.0, cw=0.5):
+    """Well-defined (na1, na2) D(i, j) (see oracle_atomic_distance_arad)."""
+    q1, q2, z1, z2 = _c(q1), _c(q2), _c(z1), _c(z2)
+    n1, n2 = _c(n1, np.int32), _c(n2, np.int32)
+    na1, na2, amax = q1.shape[0], q2.shape[0], q1.shape[2]
+    out = np.zeros((na1, na2), order="F")
+    lib().oracle_atomic_distance_arad(_dp(q1), _dp(q2), _dp(z1), _dp(z2), _ip(n1), _ip(n2),
+                                      na1, na2, amax, width, cut, rw, cw, out.ctypes.data_as(_c_dbl_p))
+    return out
+
+
+def arad_l2_distance_all(q1, q2, z1, z2, n1, n2, width=0.2, cut=6.0, rw=1.0, cw=0.5):
+    q1, q2, z1, z2 = _c(q1), _c(q2), _c(z1), _c(z2)
+    n1, n2 = _c(n1, np.int32), _c(n2, np.int32)
+    nm1, nm2, amax = q1.shape[0], q2.shape[0], q1.shape[1]
+    out = np.zeros((nm1, nm2, amax, amax), order="F")
+    lib().oracle_arad_l2_distance_all(_dp(q1), _dp(q2), _dp(z1), _dp(z2), _ip(n1), _ip(n2),
+                                      nm1, nm2, amax, width, cut, rw, cw, out.ctypes.data_as(_c_dbl_p))
+    return out
+
+
+def _f(a, dt=np.float64):
+    return np.asfortranarray(a, dtype=dt)
+
+
+def ref_kernels_arad(q1, q2, z1, z2, n1, n2, sigmas, width=0.2, cut=6.0, rw=1.0, cw=0.5,
+                     variant="libfml_ref"):
+    q1, q2, z1, z2 = _f(q1), _f(q2), _f(z1), _f(z2)
+    n1, n2, sig = _c(n1, np.int32), _c(n2, np.int32), _c(sigmas)
+    nm1, nm2, amax, ns = q1.shape[0], q2.shape[0], q1.shape[1], sig.shape[0]
+    out = np.zeros((ns, nm1, nm2), order="F")
+    ref(variant).ref_kernels_arad(_dp(q1), _dp(q2), _dp(z1), _dp(z2), _ip(n1), _ip(n2), _dp(sig),
+                                  nm1, nm2, ns, amax, width, cut, rw, cw, out.ctypes.data_as(_c_dbl_p))
+    return out
+
+
+def ref_symmetric_kernels_arad(q1, z1, n1, sigmas, width=0.2, cut=6.0, rw=1.0, cw=0.5,
+                               variant="libfml_ref"):
+    q1, z1, n1, sig = _f(q1), _f(z1), _c(n1, np.int32), _c(sigmas)
+    nm1, amax, ns = q1.shape[0], q1.shape[1], sig.shape[0]
+    out = np.zeros((ns, nm1, nm1), order="F")
+    ref(variant).ref_symmetric_kernels_arad(_dp(q1), _dp(z1), _ip(n1), _dp(sig), nm1, ns, amax,
+                                            width, cut, rw, cw, out.ctypes.data_as(_c_dbl_p))
+    return out
+
+
+def ref_kernel_arad(q, z, n, sigma, variant="libfml_ref"):
+    q, z, n = _f(q), _f(z), _c(n, np.int32)
+    nm, amax = q.shape[0], q.shape[1]
+    out = np.zeros((nm, nm), order="F")
+    ref(variant).ref_kernel_arad(_dp(q), _dp(z), _ip(n), float(sigma), nm, amax, out.ctypes.data_as(_c_dbl_p))
+    return out
+
+
+def ref_atomic_kernels_arad(q1, q2, z1, z2, n1, n2, sigmas, width=0.2, cut=6.0, rw=1.0, cw=0.5,
+                            variant="libfml_ref"):
+    q1, q2, z1, z2 = _f(q1), _f(q2), _f(z1), _f(z2)
+    n1, n2, sig = _c(n1, np.int32), _c(n2, np.int32), _c(sigmas)
+    na1, na2, amax, ns = q1.shape[0], q2.shape[0], q1.shape[2], sig.shape[0]
+    out = np.zeros((ns, na1, na2), order="F")
+    ref(variant).ref_atomic_kernels_arad(_dp(q1), _dp(q2), _dp(z1), _dp(z2), _ip(n1), _ip(n2), _dp(sig),
+                                         na1, na2, ns, amax, width, cut, rw, cw,
+                                         out.ctypes.data_as(_c_dbl_p))
+    return out
+
+
+def ref_atomic_distance_arad(q1, q2, z1, z2, n1, n2, width=0.2, cut=6.0, rw=1.0, cw=0.5,
+                             variant="libfml_ref"):
+    """Raw reference output: a (na1, na2) Fortran array holding D(i,j) at
+    kernel(j,i) -- only meaningful for na1 == na2 (farad_kernels.f90:932,1019)."""
+    q1, q2, z1, z2 = _f(q1), _f(q2), _f(z1), _f(z2)
+    n1, n2 = _c(n1, np.int32), _c(n2, np.int32)
+    na1, na2, amax = q1.shape[0], q2.shape[0], q1.shape[2]
+    if na1 != na2:
+        raise ValueError("the reference routine writes out of bounds unless na1 == na2")
+    out = np.zeros((na1, na2), order="F")
+    ref(variant).ref_atomic_distance_arad(_dp(q1), _dp(q2), _dp(z1), _dp(z2), _ip(n1), _ip(n2),
+                                          na1, na2, amax, width, cut, rw, cw, out.ctypes.data_as(_c_dbl_p))
+    return out
+
+
+def ref_arad_l2_distance_all(q1, q2, z1, z2, n1, n2, width=0.2, cut=6.0, rw=1.0, cw=0.5,
+                             variant="libfml_ref"):
+    q1, q2, z1, z2 = _f(q1), _f(q2), _f(z1), _f(z2)
+    n1, n2 = _c(n1, np.int32), _c(n2, np.int32)
+    nm1, nm2, amax = q1.shape[0], q2.shape[0], q1.shape[1]
+    out = np.zeros((nm1, nm2, amax, amax), order="F")
+    ref(variant).ref_arad_l2_distance_all(_dp(q1), _dp(q2), _dp(z1), _dp(z2), _ip(n1), _ip(n2),
+                                          nm1, nm2, amax, width, cut, rw, cw, out.ctypes.data_as(_c_dbl_p))
+    return out
+
+
+def ref_cho_solve(A, y, variant="libfml_ref"):
+    """fcho_solve on a private Fortran copy of A (the reference overwrites it)."""
+    Af = np.array(A, dtype=np.float64, order="F", copy=True)
+    y = _c(y)
+    x = np.zeros_like(y)
+    ref(variant).ref_cho_solve(Af.ctypes.data_as(_c_dbl_p), Af.shape[0], _dp(y), _dp(x))
+    return x
+
+
+# ---------------------------------------------------------------------------
+# cho_solve: LAPACK contract through scipy (fml/math/fcho_solve.f90:23-47)
+# ---------------------------------------------------------------------------
+
+def cho_solve(A, y):
+    """dpotrf('U') + dpotrs('U') on the upper triangle of A, as fcho_solve.
+    Returns (x, info)."""
+    from scipy.linalg import lapack
+    c, info = lapack.dpotrf(np.asarray(A, dtype=np.float64), lower=0, clean=0)
+    if info != 0:
+        return np.full(A.shape[0], np.nan), info
+    x, info2 = lapack.dpotrs(c, np.asarray(y, dtype=np.float64), lower=0)
+    return x, info2
+
+
+def cho_solve_c(A, y):
+    """Unblocked C Cholesky (upper) solve; for tiny closed-form cross-checks."""
+    Af = np.asfortranarray(A, dtype=np.float64)
+    y = _c(y)
+    x = np.zeros_like(y)
+    info = lib().oracle_cho_solve(Af.ctypes.data_as(_c_dbl_p), Af.shape[0], _dp(y), _dp(x))
+    return x, info
+
+
+from .pyref import (arad_describe, arad_distl2_py, ptp_types, NUCLEAR_CHARGE_ORACLE)  # noqa: E402,F401

@@ -1,0 +1,609 @@
+/*
+ * fml_oracle.c -- CPU restatement of the fml KRR hot path, used ONLY as a
+ * test oracle and as the "port" CPU baseline.  Nothing in the product
+ * (fml_amd/) links or calls this file; tests, the smoke check and the CPU
+ * baseline leg of bench.py are its only users.
+ *
+ * Every routine follows the loop nest and the expression order of the
+ * reference Fortran it cites, evaluated in IEEE double precision with FP
+ * contraction disabled (built with -ffp-contract=off), so that a sum is the
+ * sequential left-to-right sum the Fortran source writes.
+ *
+ * Layout conventions (all row-major C arrays unless stated):
+ *   coordinates        R[natoms][3]
+ *   descriptor sets    A[n][d]   (one sample per row; this is the memory
+ *                      layout of the reference's Fortran (d, n) arrays)
+ *   dense kernels      K stored Fortran order: K(j,i) at K[j + i*na]
+ *   ARAD descriptors   q[nm][amax][5][amax]  (numpy C order of the
+ *                      reference's (nm, amax, 5, amax) arrays)
+ *   ARAD types         z[nm][amax][2]
+ *   ARAD atomic descr. q[na][5][amax], z[na][2]
+ *
+ * Parity status of each routine is stated where it is defined.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define EXPORT __attribute__((visibility("default")))
+
+/* ------------------------------------------------------------------ */
+/* Sorted Coulomb matrix                                               */
+/* ------------------------------------------------------------------ */
+
+/*
+ * fml/representations/frepresentations.f90:23-110 (fgenerate_coulomb_matrix)
+ *
+ * diag: 0.5*Z**2.4 (line 66); pair: Zi*Zj/sqrt(sum((Rj-Ri)**2)) (75-76).
+ * Row norms follow the single-thread order of the two OpenMP REDUCTION
+ * loops (64, 72): the first loop leaves r_k = diag_k, the second
+ * accumulates a private sum starting at zero over partners in index order
+ * (i<k as "j" first, then i=k over j>k) and adds it to r_k at the end.
+ * Selection sort (87-91): minloc picks the FIRST minimum and places it last,
+ * so equal norms are ordered with the higher original index first.
+ * Packed output (94-103): cm(m(m-1)/2 + n) = M(sorted(m), sorted(n)), n<=m,
+ * zero padded to nmax(nmax+1)/2.
+ *
+ * Returns 0, or -1 when natoms > nmax (the reference writes out of bounds
+ * there, frepresentations.f90:96-103) or natoms < 0.
+ * perm (optional, may be NULL) receives sorted_atoms - 1.
+ * Parity: pinned by the flang build of the reference (oracle/_ref), by hand
+ * computed molecules and by invariants (tests/test_oracle.py).
+ */
+EXPORT int oracle_coulomb_matrix(const double *z, const double *r, int natoms,
+                                 int nmax, double *cm, int *perm) {
+  if (natoms < 0 || natoms > nmax) return -1;
+  const int n = natoms;
+  const int packed = nmax * (nmax + 1) / 2;
+  double *pdm = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n * n : 1));
+  double *norms = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+  double *priv = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+  int *sorted = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+
+  for (int i = 0; i < n; ++i) {
+    double p = 0.5 * pow(z[i], 2.4);
+    norms[i] = 0.0 + p;
+    pdm[i * n + i] = p;
+  }
+  for (int i = 0; i < n; ++i) priv[i] = 0.0;
+  for (int i = 0; i < n; ++i) {
+    for (int j = i + 1; j < n; ++j) {
+      double dx = r[j * 3 + 0] - r[i * 3 + 0];
+      double dy = r[j * 3 + 1] - r[i * 3 + 1];
+      double dz = r[j * 3 + 2] - r[i * 3 + 2];
+      double s = dx * dx;
+      s = s + dy * dy;
+      s = s + dz * dz;
+      double p = z[i] * z[j] / sqrt(s);
+      priv[j] = priv[j] + p;
+      priv[i] = priv[i] + p;
+      pdm[i * n + j] = p;
+      pdm[j * n + i] = p;
+    }
+  }
+  for (int i = 0; i < n; ++i) norms[i] = norms[i] + priv[i];
+
+  for (int i = 0; i < n; ++i) {
+    int jmin = 0;
+    for (int j = 1; j < n; ++j)
+      if (norms[j] < norms[jmin]) jmin = j;
+    sorted[n - 1 - i] = jmin;
+    norms[jmin] = 1.7976931348623157e308; /* huge(1d0), line 59 */
+  }
+
+  for (int k = 0; k < packed; ++k) cm[k] = 0.0;
+  for (int m = 0; m < n; ++m) {
+    int i = sorted[m];
+    int idx = m * (m + 1) / 2;
+    for (int q = 0; q <= m; ++q) cm[idx + q] = pdm[i * n + sorted[q]];
+  }
+  if (perm)
+    for (int m = 0; m < n; ++m) perm[m] = sorted[m];
+  free(pdm);
+  free(norms);
+  free(priv);
+  free(sorted);
+  return 0;
+}
+
+/* Batched form over concatenated molecules: offsets has nmol+1 entries. */
+EXPORT int oracle_coulomb_matrices(const double *z, const double *r,
+                                   const int *offsets, int nmol, int nmax,
+                                   double *cm, int *perm) {
+  const int packed = nmax * (nmax + 1) / 2;
+  int rc = 0;
+#pragma omp parallel for schedule(dynamic, 16) reduction(| : rc)
+  for (int m = 0; m < nmol; ++m) {
+    int a0 = offsets[m];
+    int na = offsets[m + 1] - a0;
+    rc |= oracle_coulomb_matrix(z + a0, r + 3 * (size_t)a0, na, nmax,
+                                cm + (size_t)m * packed,
+                                perm ? perm + (size_t)m * nmax : NULL);
+  }
+  return rc ? -1 : 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Gaussian and Laplacian kernels                                      */
+/* ------------------------------------------------------------------ */
+
+/*
+ * fml/kernels/fkernels.f90:317-349 (fgaussian_kernel)
+ * inv = -0.5/(sigma*sigma) (334); K(j,i) = exp(inv * sum((a_j-b_i)**2))
+ * (341-342), the sum sequential over d.  A: na x d rows, B: nb x d rows,
+ * K Fortran order (na, nb).
+ * Parity: closed form, brute force numpy, flang build (oracle/_ref).
+ */
+EXPORT void oracle_gaussian_kernel(const double *a, int na, const double *b,
+                                   int nb, int d, double sigma, double *k) {
+  const double inv = -0.5 / (sigma * sigma);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < nb; ++i) {
+    const double *bi = b + (size_t)i * d;
+    for (int j = 0; j < na; ++j) {
+      const double *aj = a + (size_t)j * d;
+      double s = 0.0;
+      for (int t = 0; t < d; ++t) {
+        double diff = aj[t] - bi[t];
+        s = s + diff * diff;
+      }
+      k[(size_t)i * na + j] = exp(inv * s);
+    }
+  }
+}
+
+/*
+ * fml/kernels/fkernels.f90:351-377 (flaplacian_kernel)
+ * inv = -1/sigma (367); K(j,i) = exp(inv * sum(abs(a_j-b_i))) (372).
+ */
+EXPORT void oracle_laplacian_kernel(const double *a, int na, const double *b,
+                                    int nb, int d, double sigma, double *k) {
+  const double inv = -1.0 / sigma;
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < nb; ++i) {
+    const double *bi = b + (size_t)i * d;
+    for (int j = 0; j < na; ++j) {
+      const double *aj = a + (size_t)j * d;
+      double s = 0.0;
+      for (int t = 0; t < d; ++t) s = s + fabs(aj[t] - bi[t]);
+      k[(size_t)i * na + j] = exp(inv * s);
+    }
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* ARAD                                                                */
+/* ------------------------------------------------------------------ */
+
+typedef struct {
+  double width, cut, rw, cw;
+} arad_par;
+
+/*
+ * fml/kernels/farad_kernels.f90:29-90 (atomic_distl2); identical copy at
+ * fml/math/farad.f90:62-123.
+ * x1, x2 point at one atom's (5, amax) block stored channel-major:
+ * channel c of neighbour m at x[c*amax + m].  s1, s2 are the precomputed
+ * 1-sin(d*inv_cut) vectors of length >= n.
+ * Expression order: d=(exp(r2*inv_w)*s1)*s2 (78); d = d*(((rw2/(rw2+dr^2))
+ * *cw2)/(cw2+dc^2)) (80-81); acc = acc + d*((1+x4*x4')+x5*x5') (83).
+ */
+static double arad_distl2(const double *x1, const double *x2, int amax, int n1,
+                          int n2, const double *s1, const double *s2,
+                          const arad_par *p) {
+  const double inv_w = -1.0 / (4.0 * p->width * p->width);
+  const double maxg = (8.0 * p->width) * (8.0 * p->width);
+  const double rw2 = p->rw * p->rw;
+  const double cw2 = p->cw * p->cw;
+  double acc = 0.0;
+  for (int m1 = 0; m1 < n1; ++m1) {
+    if (x1[m1] > p->cut) break;
+    for (int m2 = 0; m2 < n2; ++m2) {
+      if (x2[m2] > p->cut) break;
+      double dr = x2[m2] - x1[m1];
+      double r2 = dr * dr;
+      if (r2 < maxg) {
+        double d = exp(r2 * inv_w) * s1[m1] * s2[m2];
+        double e1 = x1[amax + m1] - x2[amax + m2];
+        double e2 = x1[2 * amax + m1] - x2[2 * amax + m2];
+        d = d * (rw2 / (rw2 + e1 * e1) * cw2 / (cw2 + e2 * e2));
+        double ang = 1.0 + x1[3 * amax + m1] * x2[3 * amax + m2];
+        ang = ang + x1[4 * amax + m1] * x2[4 * amax + m2];
+        acc = acc + d * ang;
+      }
+    }
+  }
+  return acc;
+}
+
+/* (rw2/(rw2+dz1^2) * cw2/(cw2+dz2^2)) as written at farad_kernels.f90:590-592 */
+static inline double arad_stoch(double za0, double za1, double zb0, double zb1,
+                                const arad_par *p) {
+  const double rw2 = p->rw * p->rw;
+  const double cw2 = p->cw * p->cw;
+  double e1 = za0 - zb0;
+  double e2 = za1 - zb1;
+  return rw2 / (rw2 + e1 * e1) * cw2 / (cw2 + e2 * e2);
+}
+
+/* single-precision eps literals of the reference, widened to double */
+static const double EPS_5E12 = (double)5.0e-12f;  /* farad_kernels.f90:505 */
+static const double EPS_1E11 = (double)1.0e-11f;  /* farad_kernels.f90:148 */
+
+/* sin terms for a (nm, amax, 5, amax) descriptor set.
+ * guard=1: only where d < cut, else 0 (farad_kernels.f90:521-545);
+ * guard=0: unconditional (farad_kernels.f90:689-698, 163-171). */
+static double *arad_sin_mol(const double *q, const int *n, int nm, int amax,
+                            double cut, int guard) {
+  const double pi = 4.0 * atan(1.0);
+  const double inv_cut = pi / (2.0 * cut);
+  double *s = (double *)calloc((size_t)nm * amax * amax, sizeof(double));
+  for (int i = 0; i < nm; ++i)
+    for (int a = 0; a < n[i]; ++a)
+      for (int m = 0; m < n[i]; ++m) {
+        double d = q[(((size_t)i * amax + a) * 5 + 0) * amax + m];
+        if (!guard || d < cut)
+          s[((size_t)i * amax + a) * amax + m] = 1.0 - sin(d * inv_cut);
+      }
+  return s;
+}
+
+static double *arad_self_mol(const double *q, const int *n, int nm, int amax,
+                             const double *s, const arad_par *p) {
+  double *self = (double *)calloc((size_t)nm * amax, sizeof(double));
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int i = 0; i < nm; ++i)
+    for (int a = 0; a < n[i]; ++a) {
+      const double *x = q + ((size_t)i * amax + a) * 5 * amax;
+      const double *sa = s + ((size_t)i * amax + a) * amax;
+      self[(size_t)i * amax + a] = arad_distl2(x, x, amax, n[i], n[i], sa, sa, p);
+    }
+  return self;
+}
+
+/* D = self1+self2 - 2*S*T, clamped by eps (farad_kernels.f90:590-594) */
+static inline double arad_pair_distance(double self1, double self2, double S,
+                                        double T, double eps) {
+  double l2 = self1 + self2 - 2.0 * S * T;
+  if (fabs(l2) < eps) l2 = 0.0;
+  return l2;
+}
+
+/*
+ * fml/kernels/farad_kernels.f90:439-616 (fget_kernels_arad)
+ * Output kernels(nsig, nm1, nm2) in Fortran order:
+ *   K(k,i,j) at out[k + nsig*(i + nm1*j)].
+ * The per-sigma sum runs over atomic_distance(:ni,:nj) in column-major
+ * order (i1 fastest), exp(D*inv_sigma2(k)) (602-604).
+ */
+EXPORT void oracle_kernels_arad(const double *q1, const double *q2,
+                                const double *z1, const double *z2,
+                                const int *n1, const int *n2,
+                                const double *sigmas, int nm1, int nm2,
+                                int nsig, int amax, double width, double cut,
+                                double rw, double cw, double *out) {
+  arad_par p = {width, cut, rw, cw};
+  double *inv_s2 = (double *)malloc(sizeof(double) * (size_t)nsig);
+  for (int k = 0; k < nsig; ++k) inv_s2[k] = -1.0 / (sigmas[k] * sigmas[k]);
+  double *s1 = arad_sin_mol(q1, n1, nm1, amax, cut, 1);
+  double *s2 = arad_sin_mol(q2, n2, nm2, amax, cut, 1);
+  double *self1 = arad_self_mol(q1, n1, nm1, amax, s1, &p);
+  double *self2 = arad_self_mol(q2, n2, nm2, amax, s2, &p);
+#pragma omp parallel
+  {
+    double *ad = (double *)malloc(sizeof(double) * (size_t)amax * amax);
+#pragma omp for schedule(dynamic, 1)
+    for (int j = 0; j < nm2; ++j) {
+      for (int i = 0; i < nm1; ++i) {
+        int ni = n1[i], nj = n2[j];
+        for (int a = 0; a < ni; ++a)
+          for (int b = 0; b < nj; ++b) {
+            const double *x1 = q1 + ((size_t)i * amax + a) * 5 * amax;
+            const double *x2 = q2 + ((size_t)j * amax + b) * 5 * amax;
+            double S = arad_distl2(x1, x2, amax, ni, nj,
+                                   s1 + ((size_t)i * amax + a) * amax,
+                                   s2 + ((size_t)j * amax + b) * amax, &p);
+            const double *za = z1 + ((size_t)i * amax + a) * 2;
+            const double *zb = z2 + ((size_t)j * amax + b) * 2;
+            double T = arad_stoch(za[0], za[1], zb[0], zb[1], &p);
+            ad[a + (size_t)amax * b] =
+                arad_pair_distance(self1[(size_t)i * amax + a],
+                                   self2[(size_t)j * amax + b], S, T, EPS_5E12);
+          }
+        for (int k = 0; k < nsig; ++k) {
+          double acc = 0.0;
+          for (int b = 0; b < nj; ++b)
+            for (int a = 0; a < ni; ++a)
+              acc = acc + exp(ad[a + (size_t)amax * b] * inv_s2[k]);
+          out[k + (size_t)nsig * (i + (size_t)nm1 * j)] = acc;
+        }
+      }
+    }
+    free(ad);
+  }
+  free(inv_s2);
+  free(s1);
+  free(s2);
+  free(self1);
+  free(self2);
+}
+
+/*
+ * fml/kernels/farad_kernels.f90:619-755 (fget_symmetric_kernels_arad)
+ * As fget_kernels_arad for X vs X, sin terms without the <cut guard
+ * (689-698), i<=j computed with set-1 as i, then mirrored (743-744).
+ */
+EXPORT void oracle_symmetric_kernels_arad(const double *q1, const double *z1,
+                                          const int *n1, const double *sigmas,
+                                          int nm1, int nsig, int amax,
+                                          double width, double cut, double rw,
+                                          double cw, double *out) {
+  arad_par p = {width, cut, rw, cw};
+  double *inv_s2 = (double *)malloc(sizeof(double) * (size_t)nsig);
+  for (int k = 0; k < nsig; ++k) inv_s2[k] = -1.0 / (sigmas[k] * sigmas[k]);
+  double *s1 = arad_sin_mol(q1, n1, nm1, amax, cut, 0);
+  double *self1 = arad_self_mol(q1, n1, nm1, amax, s1, &p);
+#pragma omp parallel
+  {
+    double *ad = (double *)malloc(sizeof(double) * (size_t)amax * amax);
+#pragma omp for schedule(dynamic, 1)
+    for (int j = 0; j < nm1; ++j) {
+      for (int i = 0; i <= j; ++i) {
+        int ni = n1[i], nj = n1[j];
+        for (int a = 0; a < ni; ++a)
+          for (int b = 0; b < nj; ++b) {
+            const double *x1 = q1 + ((size_t)i * amax + a) * 5 * amax;
+            const double *x2 = q1 + ((size_t)j * amax + b) * 5 * amax;
+            double S = arad_distl2(x1, x2, amax, ni, nj,
+                                   s1 + ((size_t)i * amax + a) * amax,
+                                   s1 + ((size_t)j * amax + b) * amax, &p);
+            const double *za = z1 + ((size_t)i * amax + a) * 2;
+            const double *zb = z1 + ((size_t)j * amax + b) * 2;
+            double T = arad_stoch(za[0], za[1], zb[0], zb[1], &p);
+            ad[a + (size_t)amax * b] =
+                arad_pair_distance(self1[(size_t)i * amax + a],
+                                   self1[(size_t)j * amax + b], S, T, EPS_5E12);
+          }
+        for (int k = 0; k < nsig; ++k) {
+          double acc = 0.0;
+          for (int b = 0; b < nj; ++b)
+            for (int a = 0; a < ni; ++a)
+              acc = acc + exp(ad[a + (size_t)amax * b] * inv_s2[k]);
+          out[k + (size_t)nsig * (i + (size_t)nm1 * j)] = acc;
+          out[k + (size_t)nsig * (j + (size_t)nm1 * i)] = acc;
+        }
+      }
+    }
+    free(ad);
+  }
+  free(inv_s2);
+  free(s1);
+  free(self1);
+}
+
+/*
+ * fml/kernels/farad_kernels.f90:95-227 (fget_kernel_arad)
+ * One sigma; width .2, cut 6.0, rw 1.0, cw .5 hard-coded (101-104);
+ * eps = real(1.0e-11) (148); no <cut guard on sin (163-171);
+ * K(i,j) = K(j,i) = sum of the full atomic_kernel work array (216-217),
+ * whose untouched entries are zero, so the sum equals the sum over
+ * (:ni,:nj) in column-major order.  Output (nm, nm) Fortran order.
+ */
+EXPORT void oracle_kernel_arad(const double *q, const double *z, const int *n,
+                               double sigma, int nm, int amax, double *out) {
+  arad_par p = {0.2, 6.0, 1.0, 0.5};
+  const double inv_s2 = -1.0 / (sigma * sigma);
+  double *s1 = arad_sin_mol(q, n, nm, amax, p.cut, 0);
+  double *self1 = arad_self_mol(q, n, nm, amax, s1, &p);
+#pragma omp parallel
+  {
+    double *ad = (double *)malloc(sizeof(double) * (size_t)amax * amax);
+#pragma omp for schedule(dynamic, 1)
+    for (int j = 0; j < nm; ++j) {
+      for (int i = 0; i <= j; ++i) {
+        int ni = n[i], nj = n[j];
+        for (int a = 0; a < ni; ++a)
+          for (int b = 0; b < nj; ++b) {
+            const double *x1 = q + ((size_t)i * amax + a) * 5 * amax;
+            const double *x2 = q + ((size_t)j * amax + b) * 5 * amax;
+            double S = arad_distl2(x1, x2, amax, ni, nj,
+                                   s1 + ((size_t)i * amax + a) * amax,
+                                   s1 + ((size_t)j * amax + b) * amax, &p);
+            const double *za = z + ((size_t)i * amax + a) * 2;
+            const double *zb = z + ((size_t)j * amax + b) * 2;
+            double T = arad_stoch(za[0], za[1], zb[0], zb[1], &p);
+            double l2 = arad_pair_distance(self1[(size_t)i * amax + a],
+                                           self1[(size_t)j * amax + b], S, T,
+                                           EPS_1E11);
+            ad[a + (size_t)amax * b] = exp(inv_s2 * l2);
+          }
+        double acc = 0.0;
+        for (int b = 0; b < nj; ++b)
+          for (int a = 0; a < ni; ++a) acc = acc + ad[a + (size_t)amax * b];
+        out[i + (size_t)nm * j] = acc;
+        out[j + (size_t)nm * i] = acc;
+      }
+    }
+    free(ad);
+  }
+  free(s1);
+  free(self1);
+}
+
+/* sin terms for per-atom descriptors q[na][5][amax] (no guard;
+ * farad_kernels.f90:836-852) */
+static double *arad_sin_atom(const double *q, const int *n, int na, int amax,
+                             double cut) {
+  const double pi = 4.0 * atan(1.0);
+  const double inv_cut = pi / (2.0 * cut);
+  double *s = (double *)calloc((size_t)na * amax, sizeof(double));
+  for (int i = 0; i < na; ++i)
+    for (int m = 0; m < n[i]; ++m)
+      s[(size_t)i * amax + m] = 1.0 - sin(q[(size_t)i * 5 * amax + m] * inv_cut);
+  return s;
+}
+
+/* shared body of fget_atomic_kernels_arad / fget_atomic_distance_arad:
+ * D(i,j) for atom i of set 1 and atom j of set 2 (farad_kernels.f90:873-889,
+ * 1006-1019). */
+static void arad_atomic_distances(const double *q1, const double *q2,
+                                  const double *z1, const double *z2,
+                                  const int *n1, const int *n2, int na1,
+                                  int na2, int amax, const arad_par *p,
+                                  double *dmat /* D(i,j) at i + na1*j */) {
+  double *s1 = arad_sin_atom(q1, n1, na1, amax, p->cut);
+  double *s2 = arad_sin_atom(q2, n2, na2, amax, p->cut);
+  double *self1 = (double *)malloc(sizeof(double) * (size_t)(na1 > 0 ? na1 : 1));
+  double *self2 = (double *)malloc(sizeof(double) * (size_t)(na2 > 0 ? na2 : 1));
+  for (int i = 0; i < na1; ++i) {
+    const double *x = q1 + (size_t)i * 5 * amax;
+    self1[i] = arad_distl2(x, x, amax, n1[i], n1[i], s1 + (size_t)i * amax,
+                           s1 + (size_t)i * amax, p);
+  }
+  for (int i = 0; i < na2; ++i) {
+    const double *x = q2 + (size_t)i * 5 * amax;
+    self2[i] = arad_distl2(x, x, amax, n2[i], n2[i], s2 + (size_t)i * amax,
+                           s2 + (size_t)i * amax, p);
+  }
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int j = 0; j < na2; ++j)
+    for (int i = 0; i < na1; ++i) {
+      double S = arad_distl2(q1 + (size_t)i * 5 * amax, q2 + (size_t)j * 5 * amax,
+                             amax, n1[i], n2[j], s1 + (size_t)i * amax,
+                             s2 + (size_t)j * amax, p);
+      double T = arad_stoch(z1[2 * (size_t)i], z1[2 * (size_t)i + 1],
+                            z2[2 * (size_t)j], z2[2 * (size_t)j + 1], p);
+      dmat[i + (size_t)na1 * j] =
+          arad_pair_distance(self1[i], self2[j], S, T, EPS_5E12);
+    }
+  free(s1);
+  free(s2);
+  free(self1);
+  free(self2);
+}
+
+/*
+ * fml/kernels/farad_kernels.f90:757-899 (fget_atomic_kernels_arad)
+ * kernels(k,i,j) = exp(D(i,j)*inv_sigma2(k)) (887), Fortran order
+ * (nsig, na1, na2).
+ */
+EXPORT void oracle_atomic_kernels_arad(const double *q1, const double *q2,
+                                       const double *z1, const double *z2,
+                                       const int *n1, const int *n2,
+                                       const double *sigmas, int na1, int na2,
+                                       int nsig, int amax, double width,
+                                       double cut, double rw, double cw,
+                                       double *out) {
+  arad_par p = {width, cut, rw, cw};
+  double *dm = (double *)malloc(sizeof(double) * (size_t)na1 * na2 + 8);
+  arad_atomic_distances(q1, q2, z1, z2, n1, n2, na1, na2, amax, &p, dm);
+  for (int k = 0; k < nsig; ++k) {
+    double inv = -1.0 / (sigmas[k] * sigmas[k]);
+    for (size_t t = 0; t < (size_t)na1 * na2; ++t)
+      out[k + (size_t)nsig * t] = exp(dm[t] * inv);
+  }
+  free(dm);
+}
+
+/*
+ * fml/kernels/farad_kernels.f90:902-1030 (fget_atomic_distance_arad)
+ * The reference stores D(i,j) as kernel(j,i) into a (na1, na2) array (932,
+ * 1019): correct only for na1 == na2.  This restatement writes the
+ * well-defined (na1, na2) Fortran-order matrix D(i,j) at out[i + na1*j];
+ * the transposed reference layout equals it for na1 == na2 after a
+ * transpose.
+ */
+EXPORT void oracle_atomic_distance_arad(const double *q1, const double *q2,
+                                        const double *z1, const double *z2,
+                                        const int *n1, const int *n2, int na1,
+                                        int na2, int amax, double width,
+                                        double cut, double rw, double cw,
+                                        double *out) {
+  arad_par p = {width, cut, rw, cw};
+  arad_atomic_distances(q1, q2, z1, z2, n1, n2, na1, na2, amax, &p, out);
+}
+
+/*
+ * fml/math/farad.f90:452-600 (atomic_arad_l2_distance_all)
+ * dmatrix(i,j,i1,j1) Fortran order (nm1, nm2, amax, amax), zero where
+ * i1 >= n1(i) or j1 >= n2(j) (565); sin terms without the guard (520-540).
+ */
+EXPORT void oracle_arad_l2_distance_all(const double *q1, const double *q2,
+                                        const double *z1, const double *z2,
+                                        const int *n1, const int *n2, int nm1,
+                                        int nm2, int amax, double width,
+                                        double cut, double rw, double cw,
+                                        double *out) {
+  arad_par p = {width, cut, rw, cw};
+  double *s1 = arad_sin_mol(q1, n1, nm1, amax, cut, 0);
+  double *s2 = arad_sin_mol(q2, n2, nm2, amax, cut, 0);
+  double *self1 = arad_self_mol(q1, n1, nm1, amax, s1, &p);
+  double *self2 = arad_self_mol(q2, n2, nm2, amax, s2, &p);
+  const size_t total = (size_t)nm1 * nm2 * amax * amax;
+  for (size_t t = 0; t < total; ++t) out[t] = 0.0;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int j = 0; j < nm2; ++j)
+    for (int i = 0; i < nm1; ++i) {
+      int ni = n1[i], nj = n2[j];
+      for (int a = 0; a < ni; ++a)
+        for (int b = 0; b < nj; ++b) {
+          const double *x1 = q1 + ((size_t)i * amax + a) * 5 * amax;
+          const double *x2 = q2 + ((size_t)j * amax + b) * 5 * amax;
+          double S = arad_distl2(x1, x2, amax, ni, nj,
+                                 s1 + ((size_t)i * amax + a) * amax,
+                                 s2 + ((size_t)j * amax + b) * amax, &p);
+          const double *za = z1 + ((size_t)i * amax + a) * 2;
+          const double *zb = z2 + ((size_t)j * amax + b) * 2;
+          double T = arad_stoch(za[0], za[1], zb[0], zb[1], &p);
+          out[i + (size_t)nm1 * (j + (size_t)nm2 * (a + (size_t)amax * b))] =
+              arad_pair_distance(self1[(size_t)i * amax + a],
+                                 self2[(size_t)j * amax + b], S, T, EPS_5E12);
+        }
+    }
+  free(s1);
+  free(s2);
+  free(self1);
+  free(self2);
+}
+
+/* ------------------------------------------------------------------ */
+/* Unblocked Cholesky (upper), used only to cross-check LAPACK on tiny  */
+/* closed-form cases.  fml/math/fcho_solve.f90:23-47 calls dpotrf('U')  */
+/* + dpotrs('U'); A is Fortran order and only its upper triangle is     */
+/* read.  Returns the LAPACK-style info (0 ok, k>0 leading minor k).    */
+/* ------------------------------------------------------------------ */
+EXPORT int oracle_cho_solve(const double *a_in, int n, const double *y,
+                            double *x) {
+  double *u = (double *)malloc(sizeof(double) * (size_t)n * n + 8);
+  memcpy(u, a_in, sizeof(double) * (size_t)n * n);
+  /* U(i,j) at u[i + n*j], i <= j */
+  for (int j = 0; j < n; ++j) {
+    double s = u[j + (size_t)n * j];
+    for (int k = 0; k < j; ++k) s -= u[k + (size_t)n * j] * u[k + (size_t)n * j];
+    if (!(s > 0.0)) {
+      free(u);
+      return j + 1;
+    }
+    double ujj = sqrt(s);
+    u[j + (size_t)n * j] = ujj;
+    for (int i = j + 1; i < n; ++i) {
+      double t = u[j + (size_t)n * i];
+      for (int k = 0; k < j; ++k) t -= u[k + (size_t)n * j] * u[k + (size_t)n * i];
+      u[j + (size_t)n * i] = t / ujj;
+    }
+  }
+  for (int i = 0; i < n; ++i) x[i] = y[i];
+  for (int i = 0; i < n; ++i) { /* U^T w = y */
+    double t = x[i];
+    for (int k = 0; k < i; ++k) t -= u[k + (size_t)n * i] * x[k];
+    x[i] = t / u[i + (size_t)n * i];
+  }
+  for (int i = n - 1; i >= 0; --i) { /* U x = w */
+    double t = x[i];
+    for (int k = i + 1; k < n; ++k) t -= u[i + (size_t)n * k] * x[k];
+    x[i] = t / u[i + (size_t)n * i];
+  }
+  free(u);
+  return 0;
+}
