@@ -1,0 +1,220 @@
+#!/usr/bin/env python3
+"""Benchmark of the fml KRR hot path on one MI355X.
+
+Default workload (the global ARAD kernel case): 500 synthetic molecules from
+seed 0 (5-23 atoms, H/C/N/O/S), ARAD descriptors (500, 23, 5, 23) FP64,
+width 0.2, cut 5.0, r_width 1.0, c_width 0.5, the full 500 x 500 molecular
+kernel for sigma in {0.1, 1, 10} in one launch.  One step = one call of the
+GPU entry point (descriptor preparation, self terms, molecule-pair kernel)
+on inputs already resident in HBM.  Metric: kernel entries per second
+(3 sigmas x 500 x 500 entries per step).
+
+Printed: one JSON line with the metric, the roofline of the dominant kernel
+(k_arad_pairs, timed with HIP events on its stream) and a CPU baseline (the
+reference Fortran built by oracle/Makefile, on a bounded row block, all
+host threads).
+
+    python bench.py [--steps K] [--warmup W] [--workload arad|gaussian|laplacian|cholesky]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# MI355X published peaks
+PEAK_HBM_GBS = 8000.0
+PEAK_FP64_TFLOPS = 78.6            # vector and matrix, FMA = 2 flop
+PEAK_FP32_TFLOPS = 157.3
+PEAK_FP64_VALU_TOPS = 39.3         # FP64 VALU lane-instructions/s (256 CU x 64 lanes x 2.4 GHz)
+
+# FP64 VALU instructions the gfx950 code of k_arad_pairs<double> executes per
+# inner (m1, m2) evaluation in its band walk (exp included), counted in the
+# kernel's own ISA ("ARAD roofline").
+ARAD_OPS_PER_EVAL = 33
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="arad", choices=["arad", "gaussian", "laplacian", "cholesky"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-rows", type=int, default=100, help="row block of the CPU baseline sample")
+    return ap.parse_args()
+
+
+def host_threads():
+    v = os.environ.get("OMP_NUM_THREADS")
+    try:
+        return int(v) if v else (os.cpu_count() or 1)
+    except ValueError:
+        return os.cpu_count() or 1
+
+
+def arad_inner_evaluations(X, n, cut, width):
+    """Inner (m1, m2) evaluations of the X-vs-X global kernel: every ordered
+    pair of neighbour slots (u from an atom of set 1, v from an atom of set 2)
+    with (v-u)^2 < (8 width)^2, u, v <= cut.  Counted on the sorted multiset
+    of all neighbour distances (band edges to within one rounding)."""
+    ds = []
+    for i in range(X.shape[0]):
+        for a in range(n[i]):
+            d = X[i, a, 0, :n[i]]
+            c = np.argmax(d > cut) if np.any(d > cut) else n[i]
+            ds.append(d[:c])
+    u = np.sort(np.concatenate(ds))
+    w = 8.0 * width
+    lo = np.searchsorted(u, u - w, side="right")
+    hi = np.searchsorted(u, u + w, side="left")
+    return int(np.sum(hi - lo))
+
+
+def arad_workload(args):
+    import torch
+    import fml_amd._native as nat
+    from fml_amd import synth
+    from fml_amd.arad import ARAD
+    from fml_amd.device import empty, to_device, workspace
+    from fml_amd.kernels.kernels import arad_types
+
+    nm, amax = 500, 23
+    sig = [0.1, 1.0, 10.0]
+    width, cut, rw, cw = 0.2, 5.0, 1.0, 0.5
+    ms = synth.generate(nm, seed=0, nmin=5, nmax=23)
+    ar = ARAD(maxMolSize=amax, maxAts=amax, cut=5.0)
+    X = np.array([ar.describe(ms.molecule(m)[1], ms.molecule(m)[0]) for m in range(nm)])
+    Z = [ms.molecule(m)[0] for m in range(nm)]
+    z, n = arad_types(Z, amax)
+    evals = arad_inner_evaluations(X, n, cut, width)
+
+    lib = nat.load()
+    dq, dz, dn = to_device(X), to_device(z), to_device(n, np.int32)
+    out = empty((nm, nm, len(sig)), np.float64)
+    wsb = lib.fmlh_arad_ws_bytes(nm, nm, amax, nat.ARAD_GLOBAL)
+    ws = workspace(wsb)
+    sarr = (ctypes.c_double * len(sig))(*sig)
+
+    def step():
+        nat.call("fmlh_arad_kernels", nat.ARAD_GLOBAL, nat.ptr(dq), nat.ptr(dz), nat.ptr(dn), nm,
+                 nat.ptr(dq), nat.ptr(dz), nat.ptr(dn), nm, amax, sarr, len(sig), width, cut, rw, cw,
+                 0, nat.ptr(out), nat.ptr(ws), wsb, nat.stream_ptr())
+
+    units_per_step = len(sig) * nm * nm
+    info = {
+        "metric": "kernel entries/s",
+        "unit": "entries/s",
+        "dtype": "f64",
+        "config": {"workload": "global ARAD kernel, 500x500 molecules (5-23 atoms, seed 0), "
+                               "descriptors (500,23,5,23), sigmas {0.1,1,10}, width 0.2, cut 5.0, "
+                               "r_width 1.0, c_width 0.5", "molecules": nm, "amax": amax,
+                   "sigmas": sig, "inner_evaluations_per_step": evals},
+        "kernel": "k_arad_pairs",
+    }
+
+    def roofline(t_main_s):
+        ops = evals * ARAD_OPS_PER_EVAL
+        ach = ops / t_main_s / 1e12
+        return {"bound": "valu", "achieved": round(ach, 3), "peak": PEAK_FP64_VALU_TOPS,
+                "unit": "T FP64-VALU-ops/s", "frac": round(ach / PEAK_FP64_VALU_TOPS, 4),
+                "traffic": None,
+                "inner_evaluations_per_s": round(evals / t_main_s, 1),
+                "ops_per_evaluation": ARAD_OPS_PER_EVAL}
+
+    def cpu_baseline(rows):
+        import oracle
+        kind = "reference" if oracle.ref_available() else "port"
+        fn = oracle.ref_kernels_arad if kind == "reference" else oracle.kernels_arad
+        t0 = time.perf_counter()
+        kc = fn(X[:rows], X, z[:rows], z, n[:rows], n, sig, width, cut, rw, cw)
+        dt = time.perf_counter() - t0
+        g = torch.stack([out[:, :rows, k] for k in range(len(sig))]).cpu().numpy()  # (s, nm, rows)
+        g = np.transpose(g, (0, 2, 1))
+        m = kc > 1e-300
+        err = float(np.max(np.abs(g[m] - kc[m]) / kc[m]))
+        return ({"value": round(len(sig) * rows * nm / dt, 1), "unit": "entries/s",
+                 "cores": host_threads(), "kind": kind,
+                 "sample": f"rows 0..{rows - 1} x all {nm} molecules ({len(sig) * rows * nm} entries), "
+                           f"{dt:.2f} s"}, err)
+
+    return info, step, units_per_step, roofline, cpu_baseline
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); every GPU box here has "
+              "one GPU and multi-GPU sharding is a design note, not a measured path.", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import fml_amd._native as nat
+    torch.cuda.set_device(0)
+    lib = nat.load()
+    lib.fmlh_timing_enable.argtypes = [ctypes.c_int]
+    lib.fmlh_timing_enable.restype = None
+    lib.fmlh_timing_collect.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]
+
+    info, step, units, roofline, cpu_baseline = {"arad": arad_workload}[args.workload](args)
+    print(f"[bench] workload ready: {info['config']['workload']}", file=sys.stderr, flush=True)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    lib.fmlh_timing_collect(None, None)
+    lib.fmlh_timing_enable(1)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e0.record()
+    for _ in range(args.steps):
+        step()
+    e1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    lib.fmlh_timing_enable(0)
+    ms_total = e0.elapsed_time(e1)
+    kms = ctypes.c_double(0.0)
+    kcnt = ctypes.c_int(0)
+    lib.fmlh_timing_collect(ctypes.byref(kms), ctypes.byref(kcnt))
+    t_step = ms_total / 1e3 / args.steps
+    t_kernel = kms.value / 1e3 / max(kcnt.value, 1)
+    print(f"[bench] {args.steps} steps: {ms_total:.3f} ms (wall {wall * 1e3:.1f} ms), "
+          f"dominant kernel avg {t_kernel * 1e3:.3f} ms over {kcnt.value}", file=sys.stderr, flush=True)
+
+    res = {
+        "metric": info["metric"],
+        "value": round(units / t_step, 1),
+        "unit": info["unit"],
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t_step * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": info["dtype"],
+        "data": "synthetic",
+        "config": info["config"],
+        "roofline": roofline(t_kernel),
+        "dominant_kernel_ms": round(t_kernel * 1e3, 4),
+    }
+    if not args.no_cpu:
+        print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
+        cb, err = cpu_baseline(args.cpu_rows)
+        res["cpu_baseline"] = cb
+        res["parity_vs_cpu_baseline_max_rel"] = err
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

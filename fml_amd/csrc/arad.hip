@@ -722,6 +722,7 @@ int fmlh_arad_kernels(int mode, const double *q1, const double *zt1, const int32
   if (lds > kLdsLimit) return -10;
   const int64_t cap = (int64_t)device_cus() * 16;
   const int grid = (int)(pa.npairs < cap ? pa.npairs : cap);
+  timing_begin(st);
   if (precision) {
     if (stage)
       hipLaunchKernelGGL((k_arad_pairs<float, true>), dim3(grid), dim3(kPairThreads), lds, st, s1,
@@ -737,6 +738,7 @@ int fmlh_arad_kernels(int mode, const double *q1, const double *zt1, const int32
       hipLaunchKernelGGL((k_arad_pairs<double, false>), dim3(grid), dim3(kPairThreads), lds, st,
                          s1, s2, pa, p, flag);
   }
+  timing_end(st);
   FMLH_LAUNCH_CHECK("k_arad_pairs");
   return 0;
 }
@@ -783,8 +785,10 @@ int fmlh_arad_atomic(int mode, const double *q1, const double *zt1, const int32_
   pa.n2 = n2;
   pa.out = out;
   dim3 grid(ceil_div(na1, 16), ceil_div(na2, 16));
+  timing_begin(st);
   hipLaunchKernelGGL(k_arad_atomic<double>, grid, dim3(256), 0, st, s1, s2, na1, na2, amax, mode,
                      pa, p, flag);
+  timing_end(st);
   FMLH_LAUNCH_CHECK("k_arad_atomic");
   return 0;
 }

@@ -326,7 +326,10 @@ int fmlh_potrf(double *a, int64_t lda, int n, int32_t *info, void *ws, size_t ws
     FMLH_LAUNCH_CHECK("k_set_int");
     return 0;
   }
-  return potrf_impl<double>(a, lda, n, info, as_stream(stream));
+  timing_begin(as_stream(stream));
+  int rc = potrf_impl<double>(a, lda, n, info, as_stream(stream));
+  timing_end(as_stream(stream));
+  return rc;
 }
 
 int fmlh_spotrf(float *a, int64_t lda, int n, int32_t *info, void *ws, size_t ws_bytes,
@@ -342,7 +345,10 @@ int fmlh_spotrf(float *a, int64_t lda, int n, int32_t *info, void *ws, size_t ws
     FMLH_LAUNCH_CHECK("k_set_int");
     return 0;
   }
-  return potrf_impl<float>(a, lda, n, info, as_stream(stream));
+  timing_begin(as_stream(stream));
+  int rc = potrf_impl<float>(a, lda, n, info, as_stream(stream));
+  timing_end(as_stream(stream));
+  return rc;
 }
 
 int fmlh_potrs(const double *a, int64_t lda, int n, double *x, int64_t ldx, int nrhs, void *ws,

@@ -11,6 +11,11 @@ namespace fmlh {
 // Records a HIP error for fmlh_last_error() and returns FMLH_EHIP.
 int record_error(hipError_t e, const char *where);
 
+// Brackets an entry point's dominant kernel with timing events (no-op
+// unless fmlh_timing_enable(1) was called).
+void timing_begin(hipStream_t st);
+void timing_end(hipStream_t st);
+
 // Checks the launch that was just queued.
 inline int check_launch(const char *where) {
   hipError_t e = hipGetLastError();

@@ -245,8 +245,10 @@ int fmlh_gaussian_kernel(const double *a, int64_t lda, int na, const double *b, 
   if (na == 0 || nb == 0) return 0;
   const double inv = -0.5 / (sigma * sigma);
   dim3 grid(ceil_div(na, kTile), ceil_div(nb, kTile));
+  timing_begin(as_stream(stream));
   hipLaunchKernelGGL((k_gaussian<double, double>), grid, dim3(256), 0, as_stream(stream), a, lda,
                      na, b, ldb, nb, d, inv, k, ldk, 1e-3);
+  timing_end(as_stream(stream));
   FMLH_LAUNCH_CHECK("k_gaussian<f64>");
   return 0;
 }
@@ -266,12 +268,14 @@ int fmlh_gaussian_kernel_f32(const float *a, int64_t lda, int na, const float *b
   if (mode == 1) return fmlh_gaussian_kernel_bf16x3(a, lda, na, b, ldb, nb, d, sigma, k32, k64, ldk, stream);
   const double inv = -0.5 / (sigma * sigma);
   dim3 grid(ceil_div(na, kTile), ceil_div(nb, kTile));
+  timing_begin(as_stream(stream));
   if (k32)
     hipLaunchKernelGGL((k_gaussian<float, float>), grid, dim3(256), 0, as_stream(stream), a, lda,
                        na, b, ldb, nb, d, inv, k32, ldk, 1e-2f);
   else
     hipLaunchKernelGGL((k_gaussian<float, double>), grid, dim3(256), 0, as_stream(stream), a, lda,
                        na, b, ldb, nb, d, inv, k64, ldk, 1e-2f);
+  timing_end(as_stream(stream));
   FMLH_LAUNCH_CHECK("k_gaussian<f32>");
   return 0;
 }
@@ -283,8 +287,10 @@ int fmlh_laplacian_kernel(const double *a, int64_t lda, int na, const double *b,
   if (na == 0 || nb == 0) return 0;
   const double inv = -1.0 / sigma;
   dim3 grid(ceil_div(na, 64), ceil_div(nb, 64));
+  timing_begin(as_stream(stream));
   hipLaunchKernelGGL((k_laplacian<double, double>), grid, dim3(256), 0, as_stream(stream), a, lda,
                      na, b, ldb, nb, d, inv, k, ldk);
+  timing_end(as_stream(stream));
   FMLH_LAUNCH_CHECK("k_laplacian<f64>");
   return 0;
 }
@@ -298,12 +304,14 @@ int fmlh_laplacian_kernel_f32(const float *a, int64_t lda, int na, const float *
   if (na == 0 || nb == 0) return 0;
   const double inv = -1.0 / sigma;
   dim3 grid(ceil_div(na, 64), ceil_div(nb, 64));
+  timing_begin(as_stream(stream));
   if (k32)
     hipLaunchKernelGGL((k_laplacian<float, float>), grid, dim3(256), 0, as_stream(stream), a, lda,
                        na, b, ldb, nb, d, inv, k32, ldk);
   else
     hipLaunchKernelGGL((k_laplacian<float, double>), grid, dim3(256), 0, as_stream(stream), a, lda,
                        na, b, ldb, nb, d, inv, k64, ldk);
+  timing_end(as_stream(stream));
   FMLH_LAUNCH_CHECK("k_laplacian<f32>");
   return 0;
 }

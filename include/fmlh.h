@@ -36,6 +36,14 @@ const char *fmlh_last_error(void);
 /* Number of compute units of the current device (0 on error). */
 int fmlh_device_cu_count(void);
 
+/* Instrumentation: while enabled, every entry point records a pair of HIP
+ * events on its stream around its dominant kernel (the ARAD pair kernel,
+ * the Gaussian / Laplacian tile kernel, the whole factorisation for
+ * potrf).  fmlh_timing_collect waits for the recorded events and returns
+ * the summed milliseconds and the number of pairs since the last collect. */
+void fmlh_timing_enable(int on);
+int fmlh_timing_collect(double *total_ms, int *count);
+
 /* ------------------------------------------------------------------------
  * (a) Sorted Coulomb matrices, one workgroup per molecule.
  * Replaces fgenerate_coulomb_matrix

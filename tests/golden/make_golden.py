@@ -1,0 +1,82 @@
+"""Writes tests/golden/*.npz: inputs and the outputs of the REFERENCE's own
+Fortran (compiled unchanged by oracle/Makefile into oracle/_ref/libfml_ref.so,
+IEEE order) for the hot-path routines.  Run in a tree where that library is
+built (make -C oracle ref):
+
+    python tests/golden/make_golden.py
+
+Inputs: the reference's test molecules tests/xyz/01..10.xyz (copied to
+tests/golden/xyz) and small seeded synthetic sets.  Outputs go next to this
+script; they are data (vectors), not code.
+"""
+import glob
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+import oracle  # noqa: E402
+from fml_amd import synth  # noqa: E402
+from fml_amd.arad import ARAD  # noqa: E402
+from fml_amd.fml import Molecule  # noqa: E402
+from fml_amd.kernels.kernels import arad_types  # noqa: E402
+
+
+def xyz_molecules():
+    mols = []
+    for f in sorted(glob.glob(os.path.join(HERE, "xyz", "*.xyz"))):
+        m = Molecule()
+        m.read_xyz(f)
+        mols.append(m)
+    return mols
+
+
+def main():
+    assert oracle.ref_available(), "build oracle/_ref first (make -C oracle ref)"
+    mols = xyz_molecules()
+    # --- Coulomb matrices: xyz molecules (nmax 23) + 40 synthetic (nmax 29)
+    ms = synth.generate(40, seed=7, nmin=3, nmax=29)
+    cm_xyz = np.array([oracle.ref_coulomb_matrix(m.nuclear_charges, m.coordinates, 23) for m in mols])
+    cm_syn = np.array([oracle.ref_coulomb_matrix(*ms.molecule(i), 29) for i in range(ms.nmol)])
+    np.savez_compressed(os.path.join(HERE, "coulomb.npz"), cm_xyz=cm_xyz, syn_charges=ms.charges,
+                        syn_coords=ms.coords, syn_offsets=ms.offsets, cm_syn=cm_syn)
+    # --- dense kernels on the synthetic CMs, (d, n) column-per-sample
+    A, B = cm_syn[:25].T, cm_syn[25:].T
+    np.savez_compressed(os.path.join(HERE, "dense.npz"), A=A, B=B,
+                        gauss_700=oracle.ref_gaussian_kernel(A, B, 700.0),
+                        gauss_50=oracle.ref_gaussian_kernel(A, B, 50.0),
+                        lap_1e4=oracle.ref_laplacian_kernel(A, B, 1e4),
+                        lap_300=oracle.ref_laplacian_kernel(A, B, 300.0))
+    # --- ARAD on the xyz molecules (size 30, as tests/test_arad.py:332)
+    ar = ARAD(maxMolSize=30, maxAts=30)
+    X = np.array([ar.describe(m.coordinates, m.nuclear_charges) for m in mols])
+    Z = [np.array(m.nuclear_charges) for m in mols]
+    z, n = arad_types(Z, 30)
+    sig = [0.1, 1.0, 10.0, 100.0]
+    qa = np.concatenate([X[m, :n[m]] for m in range(5)])
+    za = np.concatenate([z[m, :n[m]] for m in range(5)])
+    na = np.concatenate([[n[m]] * n[m] for m in range(5)]).astype(np.int32)
+    np.savez_compressed(
+        os.path.join(HERE, "arad.npz"), X=X, z=z, n=n, sigmas=np.array(sig),
+        kernels_default=oracle.ref_kernels_arad(X[:5], X[5:], z[:5], z[5:], n[:5], n[5:], sig),
+        kernels_cut5=oracle.ref_kernels_arad(X[:5], X[5:], z[:5], z[5:], n[:5], n[5:], sig, 0.2, 5.0),
+        symmetric=oracle.ref_symmetric_kernels_arad(X, z, n, sig),
+        kernel_s1=oracle.ref_kernel_arad(X, z, n, 1.0),
+        distance_all=oracle.ref_arad_l2_distance_all(X[:5], X[5:], z[:5], z[5:], n[:5], n[5:]),
+        qa=qa, za=za, na=na,
+        atomic_kernels=oracle.ref_atomic_kernels_arad(qa, qa, za, za, na, na, sig),
+        atomic_distance_ref_layout=oracle.ref_atomic_distance_arad(qa, qa, za, za, na, na))
+    # --- cho_solve (fcho_solve with LAPACK) on a small SPD Gaussian system
+    Kc = oracle.ref_gaussian_kernel(cm_syn.T, cm_syn.T, 700.0)
+    Kc[np.diag_indices_from(Kc)] += 1e-6
+    y = ms.energies.copy()
+    np.savez_compressed(os.path.join(HERE, "cho_solve.npz"), A=Kc, y=y, x=oracle.ref_cho_solve(Kc, y))
+    print("golden fixtures written")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,209 @@
+"""GPU parity of every hot-path routine against the oracle (the C
+restatement, itself bitwise equal to the flang build of the reference
+Fortran -- see tests/test_oracle.py).  Sizes are chosen so the oracle
+finishes in seconds.
+
+Tolerances (stated per check):
+  * Coulomb matrices: bit-exact values and permutations.
+  * Laplacian: the L1 sums are bit-identical, entries differ only by the
+    device exp's rounding: 4e-16 relative.
+  * Gaussian FP64: MFMA dot products + norm form vs sequential sums:
+    1e-12 relative.
+  * ARAD: exp/sin rounding of the device math library feeds
+    D = self1 + self2 - 2 S T, whose absolute error is amplified by
+    1/sigma^2 in the kernel; 1e-12 relative for sigma >= 1 and for the
+    distance forms (absolute, scaled by the self terms), 1e-10 for
+    sigma = 0.1 where the reference's own FMA-contracted build already
+    differs from its IEEE build by 7e-13.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from fml_amd import synth
+from fml_amd.arad import ARAD
+from fml_amd.kernels.kernels import arad_types
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b, floor=1e-300):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    m = np.abs(b) > floor
+    assert np.array_equal(m, np.abs(a) > floor) or np.max(np.abs(a[~m])) < 1e-290
+    return float(np.max(np.abs(a[m] - b[m]) / np.abs(b[m]))) if m.any() else 0.0
+
+
+@pytest.fixture(scope="module")
+def mols():
+    return synth.generate(160, seed=3, nmin=3, nmax=29)
+
+
+@pytest.fixture(scope="module")
+def arad_set():
+    ms = synth.generate(36, seed=5, nmin=3, nmax=23)
+    ar = ARAD(maxMolSize=23, maxAts=23)
+    X = np.array([ar.describe(ms.molecule(m)[1], ms.molecule(m)[0]) for m in range(ms.nmol)])
+    Z = [ms.molecule(m)[0] for m in range(ms.nmol)]
+    z, n = arad_types(Z, 23)
+    return X, Z, z, n
+
+
+def test_coulomb_bitwise(gpu, mols):
+    from fml_amd.representations import generate_coulomb_matrices
+    cm, perm = generate_coulomb_matrices(mols.charges, mols.coords, mols.offsets, nmax=29,
+                                         return_perm=True)
+    ocm, operm = oracle.coulomb_matrices(mols.charges, mols.coords, mols.offsets, 29)
+    assert np.array_equal(cm, ocm)
+    for m in range(mols.nmol):
+        n = mols.natoms[m]
+        assert np.array_equal(perm[m, :n], operm[m, :n])
+        assert np.all(perm[m, n:] == -1)
+
+
+def test_coulomb_ties_and_padding(gpu):
+    """Symmetric molecules (exact row-norm ties) and nmax padding."""
+    from fml_amd.representations import fgenerate_coulomb_matrix
+    # square of 4 H atoms + a centred C: all H norms tie exactly
+    z = np.array([1.0, 1.0, 6.0, 1.0, 1.0])
+    r = np.array([[1, 0, 0], [-1, 0, 0], [0, 0, 0], [0, 1, 0], [0, -1, 0]], dtype=float)
+    for nmax in (5, 7, 12):
+        a = fgenerate_coulomb_matrix(z, r, 5, nmax)
+        b, _ = oracle.coulomb_matrix(z, r, nmax)
+        assert np.array_equal(a, b)
+        assert np.all(a[15:] == 0.0)
+
+
+def test_coulomb_rejects_oversize(gpu):
+    from fml_amd.representations import generate_coulomb_matrices
+    z = np.ones(6)
+    r = np.arange(18, dtype=float).reshape(6, 3)
+    with pytest.raises(ValueError):
+        generate_coulomb_matrices(z, r, np.array([0, 6], dtype=np.int32), nmax=5)
+
+
+def _cm(mols, nmax=29):
+    return oracle.coulomb_matrices(mols.charges, mols.coords, mols.offsets, nmax)[0]
+
+
+def test_gaussian_fp64(gpu, mols):
+    from fml_amd.kernels import gaussian_kernel
+    X = _cm(mols)
+    A, B = X[:100].T, X[100:].T
+    for sigma in (1000.0, 50.0):
+        k = gaussian_kernel(A, B, sigma)
+        ko = oracle.gaussian_kernel(A, B, sigma)
+        assert k.shape == (100, 60) and k.flags.f_contiguous
+        assert _rel(k, ko) < 1e-12
+    # symmetric case: exact ones on the diagonal (near pairs recomputed)
+    kk = gaussian_kernel(A, A, 700.0)
+    assert np.all(np.diag(kk) == 1.0)
+    assert _rel(kk, oracle.gaussian_kernel(A, A, 700.0)) < 1e-12
+
+
+def test_gaussian_ragged_shapes(gpu):
+    from fml_amd.kernels import gaussian_kernel, laplacian_kernel
+    rng = np.random.default_rng(0)
+    for (d, na, nb) in ((1, 1, 1), (5, 3, 70), (17, 65, 129), (33, 64, 64)):
+        A = rng.normal(size=(d, na))
+        B = rng.normal(size=(d, nb))
+        assert _rel(gaussian_kernel(A, B, 2.0), oracle.gaussian_kernel(A, B, 2.0)) < 1e-12
+        assert _rel(laplacian_kernel(A, B, 2.0), oracle.laplacian_kernel(A, B, 2.0)) < 4e-16
+
+
+def test_laplacian_fp64(gpu, mols):
+    from fml_amd.kernels import laplacian_kernel
+    X = _cm(mols)
+    A, B = X[:100].T, X[100:].T
+    k = laplacian_kernel(A, B, 10000.0)
+    assert _rel(k, oracle.laplacian_kernel(A, B, 10000.0)) < 4e-16
+
+
+def test_arad_global(gpu, arad_set):
+    from fml_amd.kernels import get_atomic_kernels_arad
+    X, Z, z, n = arad_set
+    sig = [0.1, 1.0, 10.0]
+    k = get_atomic_kernels_arad(X[:20], X[20:], Z[:20], Z[20:], sig, cut_distance=5.0)
+    ko = oracle.kernels_arad(X[:20], X[20:], z[:20], z[20:], n[:20], n[20:], sig, 0.2, 5.0, 1.0, 0.5)
+    assert k.shape == (3, 20, 16)
+    assert _rel(k[0], ko[0]) < 1e-10
+    assert _rel(k[1], ko[1]) < 1e-12
+    assert _rel(k[2], ko[2]) < 1e-12
+
+
+def test_arad_symmetric_and_kernel(gpu, arad_set):
+    from fml_amd.kernels import get_atomic_symmetric_kernels_arad, fget_kernel_arad
+    X, Z, z, n = arad_set
+    sig = [1.0, 4.0]
+    k = get_atomic_symmetric_kernels_arad(X, Z, sig)
+    ko = oracle.symmetric_kernels_arad(X, z, n, sig, 0.2, 6.0, 1.0, 0.5)
+    assert _rel(k, ko) < 1e-12
+    assert np.array_equal(k, np.transpose(k, (0, 2, 1)))
+    k1 = fget_kernel_arad(X, z, n, 2.0)
+    assert _rel(k1, oracle.kernel_arad(X, z, n, 2.0)) < 1e-12
+
+
+def test_arad_distance_forms(gpu, arad_set):
+    from fml_amd.math import get_l2_distance_arad
+    from fml_amd.kernels import fget_atomic_kernels_arad, fget_atomic_distance_arad
+    X, Z, z, n = arad_set
+    d = get_l2_distance_arad(X[:7], X[7:12], Z[:7], Z[7:12])
+    do = oracle.arad_l2_distance_all(X[:7], X[7:12], z[:7], z[7:12], n[:7], n[7:12])
+    assert d.shape == do.shape
+    scale = np.max(np.abs(do))
+    assert np.max(np.abs(d - do)) <= 1e-13 * scale
+    # padding is exactly zero
+    assert np.all(d[np.abs(do) == 0] == 0)
+    # per-atom forms
+    qa = np.concatenate([X[m, :n[m]] for m in range(5)])
+    za = np.concatenate([z[m, :n[m]] for m in range(5)])
+    na = np.concatenate([[n[m]] * n[m] for m in range(5)]).astype(np.int32)
+    qb, zb, nb = qa[3:40], za[3:40], na[3:40]
+    ka = fget_atomic_kernels_arad(qa, qb, za, zb, na, nb, [1.0, 4.0])
+    assert _rel(ka, oracle.atomic_kernels_arad(qa, qb, za, zb, na, nb, [1.0, 4.0])) < 1e-12
+    da = fget_atomic_distance_arad(qa, qb, za, zb, na, nb)
+    dao = oracle.atomic_distance_arad(qa, qb, za, zb, na, nb)
+    assert np.max(np.abs(da - dao)) <= 1e-13 * np.max(np.abs(dao))
+    # self distances vanish exactly
+    dd = fget_atomic_distance_arad(qa, qa, za, za, na, na)
+    assert np.all(np.diag(dd) == 0.0)
+
+
+def test_cho_solve(gpu, mols):
+    from fml_amd.math import cho_solve
+    X = _cm(mols)[:150]
+    K = oracle.gaussian_kernel(X.T, X.T, 700.0)
+    K[np.diag_indices_from(K)] += 1e-4
+    y = mols.energies[:150]
+    Kc = K.copy()
+    x = cho_solve(K, y)
+    assert np.array_equal(K, Kc)  # caller's matrix untouched
+    xo, info = oracle.cho_solve(K, y)
+    assert info == 0
+    # backward error of a stable Cholesky solve
+    r = K @ x - y
+    be = np.linalg.norm(r) / (np.linalg.norm(K, 2) * np.linalg.norm(x))
+    assert be < 1e-14
+    cond = np.linalg.cond(K)
+    assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 50 * cond * 2.2e-16
+
+
+def test_cho_solve_sizes_and_failure(gpu):
+    from fml_amd.math import cho_solve
+    rng = np.random.default_rng(1)
+    for n in (1, 2, 63, 64, 65, 200, 257, 300):
+        # integer, well conditioned: strictly lower entries in {-1,0,1},
+        # diagonal n so every product below is exact
+        L = np.tril(rng.integers(-1, 2, size=(n, n)), -1).astype(float) + n * np.eye(n)
+        A = L @ L.T
+        xt = rng.integers(-3, 4, size=n).astype(float)
+        y = A @ xt
+        x = cho_solve(A, y)
+        assert np.max(np.abs(x - xt)) < 1e-8 * max(1.0, np.max(np.abs(xt))) * n
+    A = np.eye(5)
+    A[3, 3] = -1.0
+    x = cho_solve(A, np.ones(5))
+    assert np.all(np.isnan(x))
+    with pytest.raises(ValueError):
+        cho_solve(np.ones((3, 4)), np.ones(3))

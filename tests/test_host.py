@@ -1,0 +1,68 @@
+"""Host-side logic that needs no GPU: XYZ reading, tables, the synthetic
+generator, argument checks of the drop-in API."""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+from fml_amd import synth
+from fml_amd.fml import Molecule
+from fml_amd.kernels.kernels import arad_types
+from fml_amd.ptp import PTP
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def test_read_xyz():
+    m = Molecule()
+    m.read_xyz(os.path.join(GOLD, "xyz", "04.xyz"))
+    assert m.natoms == 4
+    assert m.atomtypes == ["C", "C", "H", "H"]
+    assert m.nuclear_charges == [6, 6, 1, 1]
+    assert m.coordinates.shape == (4, 3)
+    assert m.coordinates[0, 0] == 0.899231
+    for f in glob.glob(os.path.join(GOLD, "xyz", "*.xyz")):
+        m = Molecule()
+        m.read_xyz(f)
+        assert m.natoms == len(m.nuclear_charges) == m.coordinates.shape[0]
+
+
+def test_arad_types():
+    z, n = arad_types([[1, 6], [8, 7, 16]], 4)
+    assert list(n) == [2, 3]
+    assert z.shape == (2, 4, 2)
+    assert list(z[1, 2]) == PTP[16] and list(z[0, 3]) == [0, 0]
+
+
+def test_synth_generator():
+    ms = synth.generate(50, seed=0, nmin=5, nmax=23)
+    assert ms.nmol == 50 and ms.offsets[-1] == ms.charges.shape[0]
+    assert ms.natoms.min() >= 5 and ms.natoms.max() <= 23
+    assert set(np.unique(ms.charges)) <= {1, 6, 7, 8, 16}
+    for i in range(ms.nmol):
+        z, r = ms.molecule(i)
+        edge = 1.2 * len(z) ** (1 / 3)
+        assert r.min() >= 0 and r.max() <= edge
+        d = np.sqrt(((r[:, None] - r[None]) ** 2).sum(-1)) + np.eye(len(z)) * 10
+        assert d.min() >= 0.9
+    # deterministic
+    ms2 = synth.generate(50, seed=0, nmin=5, nmax=23)
+    assert np.array_equal(ms.coords, ms2.coords) and np.array_equal(ms.energies, ms2.energies)
+    g = synth.generate(300, seed=1, nmin=3, nmax=29, dist="geometric")
+    assert 10 < g.natoms.mean() < 18
+
+
+def test_cho_solve_shape_errors():
+    from fml_amd.math import cho_solve
+    with pytest.raises(ValueError):
+        cho_solve(np.ones((3, 4)), np.ones(3))
+    with pytest.raises(ValueError):
+        cho_solve(np.eye(3), np.ones(4))
+
+
+def test_arad_shape_asserts():
+    from fml_amd.kernels import get_atomic_kernels_arad
+    X = np.zeros((2, 5, 5, 4))
+    with pytest.raises(AssertionError):
+        get_atomic_kernels_arad(X, X, [[1], [1]], [[1], [1]], [1.0])
