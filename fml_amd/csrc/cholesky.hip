@@ -129,7 +129,7 @@ __global__ __launch_bounds__(256) void k_update(T *__restrict__ a, int64_t lda, 
     for (int in = 0; in < 2; ++in)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int gr = tr + tile_row(im, r);
+        const int gr = tr + tile_row<T>(im, r);
         const int gc = tc + tile_col(in);
         if (gr < r1 && gc < c1 && gc <= gr) a[(int64_t)gr * lda + gc] -= acc.v[im][in][r];
       }

@@ -60,7 +60,7 @@ __global__ __launch_bounds__(256) void k_gaussian(const T *__restrict__ A, int64
     for (int in = 0; in < 2; ++in)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int m = tile_row(im, r);
+        const int m = tile_row<T>(im, r);
         const int n = tile_col(in);
         const int gm = m0 + m, gn = n0 + n;
         if (gm < nb && gn < na) {

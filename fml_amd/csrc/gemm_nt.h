@@ -7,10 +7,12 @@
 // two LDS buffers (one barrier per K step).  LDS rows are padded to 17
 // elements so the 16 rows a 16-lane group reads fall in distinct banks.
 //
-// MFMA 16x16x4 lane layout (CDNA3/CDNA4 ISA, single-block form):
+// MFMA 16x16x4 lane layout on gfx950 (single-block form; checked on the
+// device with an in-bounds probe):
 //   A operand  lane l -> A[l%16][l/16]          (16 x 4)
 //   B operand  lane l -> B[l/16][l%16]          (4 x 16)
-//   D          lane l, reg r -> D[4*(l/16)+r][l%16]
+//   D (f32)    lane l, reg r -> D[4*(l/16) + r][l%16]
+//   D (f64)    lane l, reg r -> D[l/16 + 4*r][l%16]
 #pragma once
 
 #include "common.h"
@@ -54,8 +56,8 @@ struct TileSmem {
   T qn[kTile];  // optional squared row norms of the Q tile
 };
 
-// Result of one tile: acc[im][in][r] is C at
-//   m = wm*32 + im*16 + 4*(lane/16) + r,  n = wn*32 + in*16 + lane%16.
+// Result of one tile: acc[im][in][r] is C at m = tile_row<T>(im, r),
+// n = tile_col(in).
 template <typename T>
 struct TileAcc {
   typename MfmaOp<T>::acc_t v[2][2];
@@ -64,8 +66,10 @@ struct TileAcc {
 __device__ __forceinline__ int tile_lane() { return threadIdx.x & 63; }
 __device__ __forceinline__ int tile_wm() { return (threadIdx.x >> 6) >> 1; }
 __device__ __forceinline__ int tile_wn() { return (threadIdx.x >> 6) & 1; }
+template <typename T>
 __device__ __forceinline__ int tile_row(int im, int r) {
-  return tile_wm() * 32 + im * 16 + 4 * (tile_lane() >> 4) + r;
+  const int q = tile_lane() >> 4;
+  return tile_wm() * 32 + im * 16 + (sizeof(T) == 8 ? q + 4 * r : 4 * q + r);
 }
 __device__ __forceinline__ int tile_col(int in) {
   return tile_wn() * 32 + in * 16 + (tile_lane() & 15);

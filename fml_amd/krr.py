@@ -1,0 +1,194 @@
+"""Kernel ridge regression composed from the hot-path pieces, device resident.
+
+The reference has no KRR driver; users chain the routines (README.md:33-69):
+representations -> kernel -> K + lambda I -> cho_solve -> K_test^T alpha.
+This module is that chain kept on the GPU end to end:
+
+* plain form (``precision="fp64"``): FP64 Gaussian/Laplacian kernel, FP64
+  blocked Cholesky (MFMA trailing updates), FP64 triangular solves;
+* mixed form (``precision="mixed"``): the FP64 kernel matrix is kept for the
+  residual, an FP32 copy (optionally shifted until it factors) is factored
+  on FP32 MFMA, and the FP64 system is solved by conjugate gradients
+  preconditioned with that FP32 factor, every residual and update in FP64
+  (iterative refinement is the special case of one-step convergence).
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _native
+from .device import empty, to_device, torch
+from .kernels.kernels import gaussian_kernel_rows, laplacian_kernel_rows
+from .math.cho_solve import potrf_inplace, potrs_inplace
+
+
+def kernel_matrix(X1, X2, sigma, kernel="gaussian", precision="fp64", out=None, out_dtype=None):
+    """Device tensor (n2, n1) whose [i, j] is K(x1_j, x2_i) (the memory of
+    the reference's Fortran (n1, n2) kernel)."""
+    if kernel == "gaussian":
+        return gaussian_kernel_rows(X1, X2, sigma, precision=precision, out=out, out_dtype=out_dtype)
+    if kernel == "laplacian":
+        return laplacian_kernel_rows(X1, X2, sigma, out=out, out_dtype=out_dtype)
+    raise ValueError(kernel)
+
+
+def add_diagonal(K, lam):
+    n = int(K.shape[0])
+    _native.call("fmlh_add_diagonal", _native.ptr(K), K.stride(0), n, float(lam), _native.stream_ptr())
+
+
+def sym_residual(K, lam, x, y, r):
+    """r = y - (K + lam I) x from the upper triangle of the (Fortran) K."""
+    n = int(K.shape[0])
+    _native.call("fmlh_sym_residual", _native.ptr(K), K.stride(0), n, float(lam), _native.ptr(x),
+                 _native.ptr(y), _native.ptr(r), _native.stream_ptr())
+    return r
+
+
+def predict_from_kernel(Kt, alpha, out=None):
+    """y_i = sum_j K(j, i) alpha_j for the device tensor Kt (nt, n)."""
+    nt, n = int(Kt.shape[0]), int(Kt.shape[1])
+    y = out if out is not None else empty((nt,), np.float64)
+    _native.call("fmlh_gemv_t", _native.ptr(Kt), Kt.stride(0), n, nt, _native.ptr(alpha), _native.ptr(y),
+                 _native.stream_ptr())
+    return y
+
+
+@dataclass
+class FitReport:
+    precision: str
+    n: int
+    seconds: float = 0.0
+    kernel_s: float = 0.0
+    factor_s: float = 0.0
+    solve_s: float = 0.0
+    iterations: int = 0
+    shift: float = 0.0
+    backward_error: float = float("nan")
+    history: list = field(default_factory=list)
+
+
+class KRR:
+    """Device-resident KRR model.  X rows are samples (n, d)."""
+
+    def __init__(self, sigma, lam, kernel="gaussian", precision="fp64", tol=1e-14, max_iter=200):
+        self.sigma = float(sigma)
+        self.lam = float(lam)
+        self.kernel = kernel
+        self.precision = precision
+        self.tol = tol
+        self.max_iter = max_iter
+        self.alpha = None
+        self.X = None
+        self.report = None
+
+    # -- fit ---------------------------------------------------------------
+    def fit(self, X, y, keep_kernel=False):
+        t = torch()
+        self.X = to_device(X, np.float64)
+        yd = to_device(y, np.float64)
+        n = int(self.X.shape[0])
+        rep = FitReport(self.precision, n)
+        t.cuda.synchronize()
+        t0 = time.perf_counter()
+        K = kernel_matrix(self.X, self.X, self.sigma, self.kernel)
+        t.cuda.synchronize()
+        rep.kernel_s = time.perf_counter() - t0
+        if self.precision == "fp64":
+            t1 = time.perf_counter()
+            add_diagonal(K, self.lam)
+            info = potrf_inplace(K)
+            t.cuda.synchronize()
+            rep.factor_s = time.perf_counter() - t1
+            if info != 0:
+                raise np.linalg.LinAlgError(f"K + lambda I is not positive definite (leading minor {info})")
+            t2 = time.perf_counter()
+            alpha = yd.clone()
+            potrs_inplace(K, alpha)
+            t.cuda.synchronize()
+            rep.solve_s = time.perf_counter() - t2
+            self.factor = K if keep_kernel else None
+        elif self.precision == "mixed":
+            alpha = self._fit_mixed(K, yd, rep)
+        else:
+            raise ValueError(self.precision)
+        rep.seconds = time.perf_counter() - t0
+        self.alpha = alpha
+        self.report = rep
+        return self
+
+    def _fit_mixed(self, K, yd, rep):
+        t = torch()
+        n = int(K.shape[0])
+        t1 = time.perf_counter()
+        K32 = K.to(t.float32)
+        shift = 0.0
+        for attempt in range(8):
+            F = K32.clone() if attempt else K32
+            add32 = self.lam + shift
+            idx = t.arange(n, device="cuda")
+            F[idx, idx] += add32
+            info = potrf_inplace(F)
+            if info == 0:
+                break
+            shift = max(shift * 10.0, 1e-5 * float(K32.diagonal().abs().max().item()))
+        else:
+            raise np.linalg.LinAlgError("FP32 factor failed even with a shift")
+        del K32
+        t.cuda.synchronize()
+        rep.factor_s = time.perf_counter() - t1
+        rep.shift = shift
+        t2 = time.perf_counter()
+
+        def precond(r):
+            z = r.to(t.float32)
+            potrs_inplace(F, z)
+            return z.to(t.float64)
+
+        # preconditioned CG on (K + lam I) x = y, FP64 residuals
+        x = precond(yd)
+        r = empty((n,), np.float64)
+        sym_residual(K, self.lam, x, yd, r)
+        ynorm = float(t.linalg.norm(yd).item())
+        z = precond(r)
+        p = z.clone()
+        rz = float(t.dot(r, z).item())
+        Ap = empty((n,), np.float64)
+        zero = t.zeros(n, dtype=t.float64, device="cuda")
+        it = 0
+        for it in range(1, self.max_iter + 1):
+            # Ap = (K + lam I) p  ==  -(0 - (K + lam I) p)
+            sym_residual(K, self.lam, p, zero, Ap)
+            Ap.neg_()
+            pAp = float(t.dot(p, Ap).item())
+            if pAp <= 0:
+                break
+            a = rz / pAp
+            x.add_(p, alpha=a)
+            r.add_(Ap, alpha=-a)
+            rn = float(t.linalg.norm(r).item())
+            rep.history.append(rn / ynorm)
+            if rn <= self.tol * ynorm:
+                break
+            if it % 10 == 0:  # re-anchor the recurrence on the true FP64 residual
+                sym_residual(K, self.lam, x, yd, r)
+            z = precond(r)
+            rz_new = float(t.dot(r, z).item())
+            p.mul_(rz_new / rz).add_(z)
+            rz = rz_new
+        rep.iterations = it
+        t.cuda.synchronize()
+        rep.solve_s = time.perf_counter() - t2
+        sym_residual(K, self.lam, x, yd, r)
+        knorm = float(t.linalg.matrix_norm(K, ord="fro").item()) if n <= 20000 else float("nan")
+        rep.backward_error = float(t.linalg.norm(r).item()) / (knorm * float(t.linalg.norm(x).item()))
+        return x
+
+    # -- predict -----------------------------------------------------------
+    def predict(self, Xt, precision="fp64"):
+        Xd = to_device(Xt, np.float64)
+        Kt = kernel_matrix(self.X, Xd, self.sigma, self.kernel)
+        return predict_from_kernel(Kt, self.alpha)
