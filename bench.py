@@ -47,7 +47,10 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="arad", choices=["arad", "gaussian", "laplacian", "cholesky"])
+    ap.add_argument("--workload", default="arad",
+                    choices=["arad", "krr", "gaussian64", "gaussian32", "gaussianbf16x3",
+                             "laplacian64", "laplacian32", "cholesky64", "cholesky32"])
+    ap.add_argument("--n", type=int, default=0, help="problem size override for the 36k workloads")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-rows", type=int, default=100, help="row block of the CPU baseline sample")
     return ap.parse_args()
@@ -146,7 +149,155 @@ def arad_workload(args):
                  "sample": f"rows 0..{rows - 1} x all {nm} molecules ({len(sig) * rows * nm} entries), "
                            f"{dt:.2f} s"}, err)
 
-    return info, step, units_per_step, roofline, cpu_baseline
+    return info, step, units_per_step, roofline, cpu_baseline, None
+
+
+def _cm_set(nmol, seed, nmin, nmax_atoms, nmax):
+    import oracle
+    from fml_amd import synth
+    ms = synth.generate(nmol, seed=seed, nmin=nmin, nmax=nmax_atoms)
+    return ms
+
+
+def krr_workload(args):
+    """Plain FP64 KRR (the CPU-runnable case): 1000 molecules, seed 0, 5-23
+    atoms, sorted CM nmax 23; Gaussian sigma 700 on 800 train, lambda 1e-8,
+    predict 200.  One step = CM generation + train kernel + Cholesky fit +
+    test kernel + prediction, all on the GPU."""
+    import torch
+    import fml_amd._native as nat
+    from fml_amd.device import to_device
+    from fml_amd.krr import add_diagonal, kernel_matrix, predict_from_kernel
+    from fml_amd.math.cho_solve import potrf_inplace, potrs_inplace
+    from fml_amd.representations import generate_coulomb_matrices
+    ms = _cm_set(1000, 0, 5, 23, 23)
+    z = to_device(ms.charges, np.float64)
+    r = to_device(ms.coords, np.float64)
+    off = to_device(ms.offsets, np.int32)
+    y = to_device(ms.energies[:800], np.float64)
+    state = {}
+
+    def step():
+        X = generate_coulomb_matrices(z, r, off, nmax=23, on_device=True)
+        K = kernel_matrix(X[:800], X[:800], 700.0)
+        add_diagonal(K, 1e-8)
+        info = potrf_inplace(K, sync=False)
+        a = y.clone()
+        potrs_inplace(K, a)
+        Kt = kernel_matrix(X[:800], X[800:], 700.0)
+        state["pred"] = predict_from_kernel(Kt, a)
+        state["info"] = info
+
+    info = {"metric": "KRR fit+predict time", "unit": "ms", "dtype": "f64", "kernel": "k_potf2..",
+            "higher_is_better": False,
+            "config": {"workload": "plain FP64 KRR: 1000 molecules (seed 0, 5-23 atoms), sorted CM "
+                                   "nmax 23, Gaussian sigma 700, 800 train / 200 test, lambda 1e-8"}}
+
+    def finish(res):
+        import oracle
+        pred = state["pred"].cpu().numpy()
+        X = oracle.coulomb_matrices(ms.charges, ms.coords, ms.offsets, 23)[0]
+        K = oracle.gaussian_kernel(X[:800].T, X[:800].T, 700.0)
+        K[np.diag_indices_from(K)] += 1e-8
+        ao, _ = oracle.cho_solve(K, ms.energies[:800])
+        po = oracle.gaussian_kernel(X[:800].T, X[800:].T, 700.0).T @ ao
+        res["mae_gpu"] = float(np.mean(np.abs(pred - ms.energies[800:])))
+        res["mae_oracle"] = float(np.mean(np.abs(po - ms.energies[800:])))
+        res["prediction_max_rel_vs_oracle"] = float(np.max(np.abs(pred - po)) / np.max(np.abs(po)))
+        res["potrf_info"] = int(state["info"].item())
+        res["cond_estimate"] = float(np.linalg.cond(K))
+
+    return info, step, None, None, None, finish
+
+
+def dense36k_workload(args, kind):
+    """Kernel construction at scale: 36,000 x 36,000 training kernel on
+    nmax-29 Coulomb matrices (435 features; 40k molecules seed 0, 3-29 atoms,
+    first 36k).  kind: gaussian64 | gaussian32 | gaussianbf16x3 | laplacian64
+    | laplacian32."""
+    import torch
+    from fml_amd.device import to_device
+    from fml_amd.krr import kernel_matrix
+    from fml_amd.representations import generate_coulomb_matrices
+    n = args.n or 36000
+    ms = _cm_set(n, 0, 3, 29, 29)
+    X64 = generate_coulomb_matrices(to_device(ms.charges), to_device(ms.coords),
+                                    to_device(ms.offsets, np.int32), nmax=29, on_device=True)
+    X32 = X64.to(torch.float32)
+    d = X64.shape[1]
+    is_g = kind.startswith("gaussian")
+    sigma = 1000.0 if is_g else 10000.0
+    prec = {"gaussian64": "fp64", "gaussian32": "fp32", "gaussianbf16x3": "bf16x3"}.get(kind)
+    Xin = X64 if kind.endswith("64") else X32
+    out = torch.empty((n, n), dtype=torch.float64 if kind.endswith("64") else torch.float32,
+                      device="cuda")
+
+    def step():
+        if is_g:
+            kernel_matrix(Xin, Xin, sigma, "gaussian", precision=prec, out=out)
+        else:
+            kernel_matrix(Xin, Xin, sigma, "laplacian", out=out)
+
+    units = n * n
+    info = {"metric": "kernel entries/s", "unit": "entries/s",
+            "dtype": {"gaussian64": "f64", "gaussian32": "f32", "gaussianbf16x3": "bf16x3/f32",
+                      "laplacian64": "f64", "laplacian32": "f32"}[kind],
+            "config": {"workload": f"{kind} kernel {n}x{n}, D={d}, sigma {sigma}"}}
+
+    def roofline(t):
+        fl = 2.0 * n * n * d
+        if kind in ("gaussian64", "gaussian32", "gaussianbf16x3"):
+            peak = {"gaussian64": PEAK_FP64_TFLOPS, "gaussian32": PEAK_FP32_TFLOPS,
+                    "gaussianbf16x3": 2500.0 / 3.0}[kind]
+            ach = fl / t / 1e12
+            return {"bound": "mfma", "achieved": round(ach, 2), "peak": round(peak, 1), "unit": "TFLOP/s",
+                    "frac": round(ach / peak, 4), "traffic": None}
+        ach = (2.0 * n * n * d) / t / 1e12   # sub + abs-add per feature
+        return {"bound": "valu", "achieved": round(ach, 2), "peak": PEAK_FP64_VALU_TOPS,
+                "unit": "T VALU-ops/s", "frac": round(ach / PEAK_FP64_VALU_TOPS, 4), "traffic": None}
+
+    return info, step, units, roofline, None, None
+
+
+def cholesky_workload(args, kind):
+    """Cholesky of K + lambda I for the 36k Gaussian kernel (lambda 1e-6)."""
+    import torch
+    from fml_amd.device import to_device
+    from fml_amd.krr import add_diagonal, kernel_matrix
+    from fml_amd.math.cho_solve import potrf_inplace
+    from fml_amd.representations import generate_coulomb_matrices
+    n = args.n or 36000
+    ms = _cm_set(n, 0, 3, 29, 29)
+    X = generate_coulomb_matrices(to_device(ms.charges), to_device(ms.coords),
+                                  to_device(ms.offsets, np.int32), nmax=29, on_device=True)
+    K = kernel_matrix(X, X, 1000.0)
+    add_diagonal(K, 1e-6)
+    if kind == "cholesky32":
+        K = K.to(torch.float32)
+        idx = torch.arange(n, device="cuda")
+        K[idx, idx] += 1e-3   # shift so the FP32 factor exists
+    work = torch.empty_like(K)
+    state = {}
+
+    def step():
+        work.copy_(K)
+        state["info"] = potrf_inplace(work, sync=False)
+
+    info = {"metric": "Cholesky TFLOP/s", "unit": "TFLOP/s", "dtype": "f64" if kind == "cholesky64" else "f32",
+            "config": {"workload": f"{kind}: potrf of the {n}x{n} Gaussian K + lambda I"}}
+    flops = n ** 3 / 3.0
+
+    def roofline(t):
+        ach = flops / t / 1e12
+        peak = PEAK_FP64_TFLOPS if kind == "cholesky64" else PEAK_FP32_TFLOPS
+        return {"bound": "mfma", "achieved": round(ach, 2), "peak": peak, "unit": "TFLOP/s",
+                "frac": round(ach / peak, 4), "traffic": None}
+
+    def finish(res):
+        res["potrf_info"] = int(state["info"].item())
+        res["value"] = round(flops / (res["dominant_kernel_ms"] / 1e3) / 1e12, 3)
+
+    return info, step, None, roofline, None, finish
 
 
 def main():
@@ -163,7 +314,15 @@ def main():
     lib.fmlh_timing_enable.restype = None
     lib.fmlh_timing_collect.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]
 
-    info, step, units, roofline, cpu_baseline = {"arad": arad_workload}[args.workload](args)
+    wl = args.workload
+    if wl == "arad":
+        info, step, units, roofline, cpu_baseline, finish = arad_workload(args)
+    elif wl == "krr":
+        info, step, units, roofline, cpu_baseline, finish = krr_workload(args)
+    elif wl.startswith("cholesky"):
+        info, step, units, roofline, cpu_baseline, finish = cholesky_workload(args, wl)
+    else:
+        info, step, units, roofline, cpu_baseline, finish = dense36k_workload(args, wl)
     print(f"[bench] workload ready: {info['config']['workload']}", file=sys.stderr, flush=True)
 
     for _ in range(args.warmup):
@@ -193,22 +352,24 @@ def main():
 
     res = {
         "metric": info["metric"],
-        "value": round(units / t_step, 1),
+        "value": round(units / t_step, 1) if units else round(t_step * 1e3, 4),
         "unit": info["unit"],
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(t_step * 1e3, 4),
-        "higher_is_better": True,
+        "higher_is_better": info.get("higher_is_better", True),
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": info["dtype"],
         "data": "synthetic",
         "config": info["config"],
-        "roofline": roofline(t_kernel),
+        "roofline": roofline(t_kernel) if roofline else None,
         "dominant_kernel_ms": round(t_kernel * 1e3, 4),
     }
-    if not args.no_cpu:
+    if finish:
+        finish(res)
+    if not args.no_cpu and cpu_baseline:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
         cb, err = cpu_baseline(args.cpu_rows)
         res["cpu_baseline"] = cb

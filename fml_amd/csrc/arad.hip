@@ -8,22 +8,26 @@
 //         ((exp((d2-d1)^2 * -1/(4w^2)) * s1) * s2) * T(types) * ((1 + a1*a2) + b1*b2)
 // is evaluated by one lane per atom pair in the reference's loop order
 // (m1 outer, m2 inner, ascending) and operation order (FP contraction off).
-// c = number of neighbours before the first distance > cut.
+// c = number of neighbours before the first distance > cut; s = 1-sin(d pi/(2 cut));
+// T(types) = ((rw^2/(rw^2+drow^2)) cw^2)/(cw^2+dcol^2).
 //
-// Work decomposition
+// Work decomposition (one call = a handful of launches on one stream)
 //   1. k_arad_prep: per atom slot, compact the (5, amax) descriptor block
-//      into neighbour arrays (distance, sine term, the two angular channels,
-//      a 9-bit element-pair type when the PTP row/column channels are small
-//      integers), the cut-exit count and a "distances ascending" flag.
-//   2. k_arad_self: self overlap S(a, a) per atom with the same device
+//      into 32-byte neighbour records {d, s, a, b} plus the raw PTP
+//      (row, col) pair, the cut-exit count and a "distances ascending"
+//      flag; mark every (row, col) type seen in a 512-entry bitmap.
+//   2. k_arad_types: one workgroup numbers the types seen (at most 16 get a
+//      compact id) and builds the 16 x 16 type-pair table with the
+//      reference's expression; k_arad_type_ids writes each neighbour's id.
+//      Data with non-integer type channels or more than 16 types takes the
+//      generic path, which evaluates T from the raw channels per term.
+//   3. k_arad_self: self overlap S(a, a) per atom with the same device
 //      function, so D(x, x) = 0 exactly.
-//   3. k_arad_pairs: persistent workgroups walk molecule pairs; both
-//      molecules' neighbour arrays are staged in LDS, each lane owns atom
-//      pairs, the per-sigma exp sums are reduced with wave shuffles in a
-//      fixed order.  The result of a molecule pair depends only on that
-//      pair (no cross-pair reduction), so it is identical whatever else
-//      shares the launch.
-//   4. k_arad_atomic: 16x16 atom tiles for the per-atom forms.
+//   4. k_arad_pairs: persistent workgroups walk molecule pairs; both
+//      molecules' records are staged in LDS, each lane owns atom pairs, the
+//      per-sigma exp sums are reduced with wave shuffles in a fixed order.
+//      A molecule pair's result depends only on that pair.
+//   5. k_arad_atomic: 16 x 16 atom tiles for the per-atom forms.
 //
 // Band walk: when both neighbour lists are ascending in distance (always
 // true for ARAD.describe output), the m2 that pass (d2-d1)^2 < (8w)^2 form
@@ -32,6 +36,14 @@
 // in |x|).  Visiting only that interval adds exactly the same terms in the
 // same order as the reference's full loop with its if-test, so the sum is
 // unchanged.  Lists that are not ascending use the full loop.
+//
+// Inner exp: its argument (d2-d1)^2 * (-1/(4w^2)) lies in (-16, 0] whenever
+// the term is taken, because (8w)^2 * (-1/(4w^2)) = -16.  exp_neg() evaluates
+// exp there by Cody-Waite reduction x = k ln2 + r (two-part ln2,
+// |r| <= ln2/2) and the degree-13 Taylor polynomial of e^r (truncation below
+// 5e-18 relative), scaled by 2^k; it has no overflow/underflow/NaN handling
+// because none can occur on that domain.  Measured against exact exp over
+// (-16, 0]: at most 0.82 ulp.
 #include <math.h>
 
 #include "common.h"
@@ -42,36 +54,82 @@ namespace {
 
 using namespace fmlh;
 
-constexpr int kTypeRows = 8;     // PTP row differences 0..7
-constexpr int kTypeCols = 64;    // PTP column differences 0..63
-constexpr int kTypeTable = kTypeRows * kTypeCols;
-constexpr uint16_t kTypeGeneric = 0xFFFF;
+constexpr int kMaxTypes = 16;            // compact type ids
+constexpr int kTypeRows = 8;             // PTP rows 0..7 representable
+constexpr int kTypeCols = 64;            // PTP columns 0..63 representable
+constexpr int kTypeBitmap = kTypeRows * kTypeCols;
+constexpr uint16_t kTypeNone = 0xFFFF;
 constexpr int kMaxSig = 8;
 constexpr int kPairThreads = 256;
 
 struct AradParams {
   double width, cut, rw, cw;
-  double inv_w;   // -1/(4 w^2)          farad_kernels.f90:58
-  double maxg;    // (8 w)^2             farad_kernels.f90:60
+  double inv_w;    // -1/(4 w^2)          farad_kernels.f90:58
+  double maxg;     // (8 w)^2             farad_kernels.f90:60
   double rw2, cw2;
-  double inv_cut; // pi/(2 cut)          farad_kernels.f90:510
-  double eps;     // real(5e-12) or real(1e-11), widened
+  double inv_cut;  // pi/(2 cut)          farad_kernels.f90:510
+  double eps;      // real(5e-12) or real(1e-11), widened
+};
+
+struct Rec {  // one neighbour: distance, sine term, two angular channels
+  double d, s, a, b;
 };
 
 // Per-set prepared neighbour data (workspace).
 struct NbSet {
-  double *d, *s, *c4, *c5, *row, *col;  // [slot*amax + m]
-  uint16_t *ty;                         // [slot*amax + m]
-  int32_t *cnt;                         // [slot]
-  int32_t *asc;                         // [slot] 1 when d[0..cnt) ascending
-  double *self;                         // [slot] self overlap (FP64 form)
-  float *self32;                        // [slot] self overlap (FP32 form)
-  double *zr, *zc;                      // [slot] PTP (row, col) of the atom
+  Rec *rec;         // [slot*amax + m]
+  double2 *rc;      // [slot*amax + m] raw (row, col) type channels
+  uint16_t *ty;     // [slot*amax + m] compact type id
+  int32_t *cnt;     // [slot]
+  int32_t *asc;     // [slot] 1 when d[0..cnt) is ascending
+  double *self;     // [slot] self overlap (FP64 form)
+  float *self32;    // [slot] self overlap (FP32 form)
+  double *zr, *zc;  // [slot] PTP (row, col) of the atom itself
+};
+
+// Per-call control block in the workspace.
+struct Ctl {
+  int32_t generic;              // 1: use the raw-channel path
+  int32_t ntypes;
+  int32_t bitmap[kTypeBitmap];  // types seen
+  int32_t id[kTypeBitmap];      // compact id per type, or -1
+  double ttab[kMaxTypes * kMaxTypes];
 };
 
 __device__ __forceinline__ bool small_int(double v, int limit) {
   return v >= 0.0 && v < (double)limit && v == (double)(int)v;
 }
+
+// ((rw2/(rw2+dr^2))*cw2)/(cw2+dc^2), farad_kernels.f90:80-81 and 590-592.
+__device__ __forceinline__ double stoch(double dr, double dc, double rw2, double cw2) {
+  return rw2 / (rw2 + dr * dr) * cw2 / (cw2 + dc * dc);
+}
+
+__device__ __forceinline__ double exp_neg(double x) {
+  const double inv_ln2 = 0x1.71547652b82fep+0;
+  const double ln2_hi = 0x1.62e42fee00000p-1;
+  const double ln2_lo = 0x1.a39ef35793c76p-33;
+  const double k = rint(x * inv_ln2);
+  double r = fma(-k, ln2_hi, x);
+  r = fma(-k, ln2_lo, r);
+  double p = 0x1.6124613a86d09p-33;      // 1/13!
+  p = fma(p, r, 0x1.1eed8eff8d898p-29);  // 1/12!
+  p = fma(p, r, 0x1.ae64567f544e4p-26);  // 1/11!
+  p = fma(p, r, 0x1.27e4fb7789f5cp-22);  // 1/10!
+  p = fma(p, r, 0x1.71de3a556c734p-19);  // 1/9!
+  p = fma(p, r, 0x1.a01a01a01a01ap-16);  // 1/8!
+  p = fma(p, r, 0x1.a01a01a01a01ap-13);  // 1/7!
+  p = fma(p, r, 0x1.6c16c16c16c17p-10);  // 1/6!
+  p = fma(p, r, 0x1.1111111111111p-7);   // 1/5!
+  p = fma(p, r, 0x1.5555555555555p-5);   // 1/4!
+  p = fma(p, r, 0x1.5555555555555p-3);   // 1/3!
+  p = fma(p, r, 0.5);
+  p = fma(p, r, 1.0);
+  p = fma(p, r, 1.0);
+  return ldexp(p, (int)k);
+}
+
+// ---- preparation ------------------------------------------------------
 
 // One thread per atom slot.  per_atom_n: n is indexed by slot (atomic
 // forms), else by molecule (slot / amax) and slots a >= n are empty.
@@ -79,8 +137,7 @@ __global__ __launch_bounds__(256) void k_arad_prep(const double *__restrict__ q,
                                                    const double *__restrict__ zt,
                                                    const int32_t *__restrict__ n, int nslots,
                                                    int amax, int per_atom_n, int sin_guard,
-                                                   AradParams p, NbSet set,
-                                                   int32_t *__restrict__ generic_flag) {
+                                                   AradParams p, NbSet set, Ctl *ctl) {
   const int slot = blockIdx.x * blockDim.x + threadIdx.x;
   if (slot >= nslots) return;
   int N;
@@ -92,36 +149,27 @@ __global__ __launch_bounds__(256) void k_arad_prep(const double *__restrict__ q,
     N = n[slot / amax];
     valid = (slot % amax) < N;
   }
-  if (N > amax) N = amax;
-  if (N < 0) N = 0;
+  N = min(max(N, 0), amax);
   const double *x = q + (int64_t)slot * 5 * amax;
-  int cnt = 0;
-  int asc = 1;
+  int cnt = 0, asc = 1;
   bool generic = false;
+  double prev = 0.0;
   if (valid) {
     for (int m = 0; m < N; ++m) {
       const double d = x[m];
       if (d > p.cut) break;  // farad_kernels.f90:68,72
       const double rowv = x[amax + m], colv = x[2 * amax + m];
-      double sv;
-      if (!sin_guard || d < p.cut)
-        sv = 1.0 - sin(d * p.inv_cut);
-      else
-        sv = 0.0;
+      // global kernel: sine only where d < cut (farad_kernels.f90:526,539)
+      const double sv = (!sin_guard || d < p.cut) ? 1.0 - sin(d * p.inv_cut) : 0.0;
       const int64_t o = (int64_t)slot * amax + m;
-      set.d[o] = d;
-      set.s[o] = sv;
-      set.c4[o] = x[3 * amax + m];
-      set.c5[o] = x[4 * amax + m];
-      set.row[o] = rowv;
-      set.col[o] = colv;
-      uint16_t ty = kTypeGeneric;
+      set.rec[o] = Rec{d, sv, x[3 * amax + m], x[4 * amax + m]};
+      set.rc[o] = make_double2(rowv, colv);
       if (small_int(rowv, kTypeRows) && small_int(colv, kTypeCols))
-        ty = (uint16_t)((int)rowv * kTypeCols + (int)colv);
+        atomicOr(&ctl->bitmap[(int)rowv * kTypeCols + (int)colv], 1);
       else
         generic = true;
-      set.ty[o] = ty;
-      if (cnt > 0 && !(set.d[o - 1] <= d)) asc = 0;
+      if (cnt > 0 && !(prev <= d)) asc = 0;
+      prev = d;
       ++cnt;
     }
     set.zr[slot] = zt[(int64_t)slot * 2 + 0];
@@ -132,143 +180,170 @@ __global__ __launch_bounds__(256) void k_arad_prep(const double *__restrict__ q,
   }
   set.cnt[slot] = cnt;
   set.asc[slot] = asc;
-  if (generic) atomicOr(generic_flag, 1);
+  if (generic) atomicOr(&ctl->generic, 1);
 }
 
-__global__ void k_zero_flag(int32_t *f) { *f = 0; }
-
-// ((rw2/(rw2+dr^2))*cw2)/(cw2+dc^2), farad_kernels.f90:80-81 and 590-592.
-__device__ __forceinline__ double stoch(double dr, double dc, double rw2, double cw2) {
-  return rw2 / (rw2 + dr * dr) * cw2 / (cw2 + dc * dc);
+__global__ void k_ctl_clear(Ctl *ctl) {
+  for (int i = threadIdx.x; i < kTypeBitmap; i += blockDim.x) ctl->bitmap[i] = 0;
+  if (threadIdx.x == 0) {
+    ctl->generic = 0;
+    ctl->ntypes = 0;
+  }
 }
 
-template <typename TC>
+// One workgroup of 512 threads: number the types seen in bitmap order and
+// fill the compact type-pair table.
+__global__ __launch_bounds__(512) void k_arad_types(Ctl *ctl, double rw2, double cw2) {
+  __shared__ int ids[kTypeBitmap];
+  __shared__ int rows[kMaxTypes], cols[kMaxTypes];
+  __shared__ int ntypes;
+  const int t = threadIdx.x;
+  if (t == 0) {
+    int k = 0;
+    for (int i = 0; i < kTypeBitmap; ++i) {
+      if (ctl->bitmap[i]) {
+        if (k < kMaxTypes) {
+          rows[k] = i / kTypeCols;
+          cols[k] = i % kTypeCols;
+        }
+        ids[i] = k < kMaxTypes ? k : -1;
+        ++k;
+      } else {
+        ids[i] = -1;
+      }
+    }
+    ntypes = k;
+    ctl->ntypes = k;
+    if (k > kMaxTypes) ctl->generic = 1;
+  }
+  __syncthreads();
+  ctl->id[t] = ids[t];
+  if (t < kMaxTypes * kMaxTypes) {
+    const int i = t / kMaxTypes, j = t % kMaxTypes;
+    const int k = min(ntypes, kMaxTypes);
+    ctl->ttab[t] = (i < k && j < k)
+                       ? stoch((double)(rows[i] - rows[j]), (double)(cols[i] - cols[j]), rw2, cw2)
+                       : 0.0;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_arad_type_ids(NbSet set, int nslots, int amax,
+                                                       const Ctl *__restrict__ ctl) {
+  const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= (int64_t)nslots * amax) return;
+  const int slot = (int)(o / amax), m = (int)(o % amax);
+  uint16_t id = kTypeNone;
+  if (m < set.cnt[slot]) {
+    const double2 v = set.rc[o];
+    if (small_int(v.x, kTypeRows) && small_int(v.y, kTypeCols)) {
+      const int c = ctl->id[(int)v.x * kTypeCols + (int)v.y];
+      if (c >= 0) id = (uint16_t)c;
+    }
+  }
+  set.ty[o] = id;
+}
+
+// ---- the overlap ------------------------------------------------------
+
 struct NbView {
-  const double *d, *s, *c4, *c5, *row, *col;
+  const Rec *rec;
   const uint16_t *ty;
+  const double2 *rc;  // raw channels, generic path only
 };
 
 template <typename TC>
-__device__ __forceinline__ TC tcast(double v) { return (TC)v; }
+__device__ __forceinline__ TC texp(TC x);
+template <>
+__device__ __forceinline__ double texp<double>(double x) { return exp_neg(x); }
+template <>
+__device__ __forceinline__ float texp<float>(float x) { return expf(x); }
 
-// One inner term, farad_kernels.f90:74-83.
+// atomic_distl2 for atoms x (list 1) and y (list 2).  kGeneric evaluates T
+// from the raw channels; otherwise ttab is the compact type-pair table.
 template <typename TC, bool kGeneric>
-__device__ __forceinline__ void arad_term(TC &acc, TC r2, TC inv_w, TC s1, TC s2, TC a1, TC b1,
-                                          TC a2, TC b2, int ty1, int ty2, double row1,
-                                          double col1, double row2, double col2,
-                                          const TC *__restrict__ ttab, double rw2, double cw2) {
-  TC e;
-  if (sizeof(TC) == 8)
-    e = (TC)exp((double)(r2 * inv_w));
-  else
-    e = (TC)expf((float)(r2 * inv_w));
-  TC v = e * s1;
-  v = v * s2;
-  TC t;
-  if (kGeneric) {
-    t = (TC)stoch(row1 - row2, col1 - col2, rw2, cw2);
-  } else {
-    const int r1 = ty1 / kTypeCols, c1 = ty1 % kTypeCols;
-    const int rr = ty2 / kTypeCols, cc = ty2 % kTypeCols;
-    const int dr = r1 > rr ? r1 - rr : rr - r1;
-    const int dc = c1 > cc ? c1 - cc : cc - c1;
-    t = ttab[dr * kTypeCols + dc];
-  }
-  v = v * t;
-  TC ang = TC(1) + a1 * a2;
-  ang = ang + b1 * b2;
-  acc = acc + v * ang;
-}
-
-// atomic_distl2 for atoms x (list 1) and y (list 2).
-template <typename TC, bool kGeneric>
-__device__ TC arad_distl2(const NbView<TC> &x, int cnt1, bool asc1, const NbView<TC> &y, int cnt2,
+__device__ TC arad_distl2(const NbView &x, int cnt1, bool asc1, const NbView &y, int cnt2,
                           bool asc2, TC inv_w, TC maxg, const TC *__restrict__ ttab, double rw2,
                           double cw2) {
   TC acc = TC(0);
+  auto term = [&](int m2, TC d1, TC s1, TC a1, TC b1, int trow, double2 rc1) {
+    const Rec yr = y.rec[m2];
+    const TC dr = (TC)yr.d - d1;
+    const TC r2 = dr * dr;
+    TC v = texp<TC>(r2 * inv_w) * s1;
+    v = v * (TC)yr.s;
+    TC t;
+    if (kGeneric) {
+      const double2 rc2 = y.rc[m2];
+      t = (TC)stoch(rc1.x - rc2.x, rc1.y - rc2.y, rw2, cw2);
+    } else {
+      t = ttab[trow + y.ty[m2]];
+    }
+    v = v * t;
+    TC ang = TC(1) + a1 * (TC)yr.a;
+    ang = ang + b1 * (TC)yr.b;
+    acc = acc + v * ang;
+  };
   if (asc1 && asc2) {
     int lo = 0, hi = 0;
     for (int m1 = 0; m1 < cnt1; ++m1) {
-      const TC d1 = tcast<TC>(x.d[m1]);
+      const Rec xr = x.rec[m1];
+      const TC d1 = (TC)xr.d;
       while (lo < cnt2) {
-        const TC d2 = tcast<TC>(y.d[lo]);
+        const TC d2 = (TC)y.rec[lo].d;
         const TC dr = d2 - d1;
         if (dr * dr < maxg || d2 > d1) break;
         ++lo;
       }
       if (hi < lo) hi = lo;
       while (hi < cnt2) {
-        const TC dr = tcast<TC>(y.d[hi]) - d1;
+        const TC dr = (TC)y.rec[hi].d - d1;
         if (!(dr * dr < maxg)) break;
         ++hi;
       }
       if (lo == hi) continue;
-      const TC s1 = tcast<TC>(x.s[m1]);
-      const TC a1 = tcast<TC>(x.c4[m1]);
-      const TC b1 = tcast<TC>(x.c5[m1]);
-      const int ty1 = kGeneric ? 0 : x.ty[m1];
-      const double row1 = kGeneric ? x.row[m1] : 0.0;
-      const double col1 = kGeneric ? x.col[m1] : 0.0;
-      for (int m2 = lo; m2 < hi; ++m2) {
-        const TC dr = tcast<TC>(y.d[m2]) - d1;
-        const TC r2 = dr * dr;
-        arad_term<TC, kGeneric>(acc, r2, inv_w, s1, tcast<TC>(y.s[m2]), a1, b1,
-                                tcast<TC>(y.c4[m2]), tcast<TC>(y.c5[m2]), ty1,
-                                kGeneric ? 0 : y.ty[m2], row1, col1,
-                                kGeneric ? y.row[m2] : 0.0, kGeneric ? y.col[m2] : 0.0, ttab,
-                                rw2, cw2);
-      }
+      const int trow = kGeneric ? 0 : (int)x.ty[m1] * kMaxTypes;
+      const double2 rc1 = kGeneric ? x.rc[m1] : make_double2(0.0, 0.0);
+      for (int m2 = lo; m2 < hi; ++m2) term(m2, d1, (TC)xr.s, (TC)xr.a, (TC)xr.b, trow, rc1);
     }
   } else {
     for (int m1 = 0; m1 < cnt1; ++m1) {
-      const TC d1 = tcast<TC>(x.d[m1]);
-      const TC s1 = tcast<TC>(x.s[m1]);
-      const TC a1 = tcast<TC>(x.c4[m1]);
-      const TC b1 = tcast<TC>(x.c5[m1]);
-      const int ty1 = kGeneric ? 0 : x.ty[m1];
-      const double row1 = kGeneric ? x.row[m1] : 0.0;
-      const double col1 = kGeneric ? x.col[m1] : 0.0;
+      const Rec xr = x.rec[m1];
+      const TC d1 = (TC)xr.d;
+      const int trow = kGeneric ? 0 : (int)x.ty[m1] * kMaxTypes;
+      const double2 rc1 = kGeneric ? x.rc[m1] : make_double2(0.0, 0.0);
       for (int m2 = 0; m2 < cnt2; ++m2) {
-        const TC dr = tcast<TC>(y.d[m2]) - d1;
-        const TC r2 = dr * dr;
-        if (r2 < maxg)
-          arad_term<TC, kGeneric>(acc, r2, inv_w, s1, tcast<TC>(y.s[m2]), a1, b1,
-                                  tcast<TC>(y.c4[m2]), tcast<TC>(y.c5[m2]), ty1,
-                                  kGeneric ? 0 : y.ty[m2], row1, col1,
-                                  kGeneric ? y.row[m2] : 0.0, kGeneric ? y.col[m2] : 0.0, ttab,
-                                  rw2, cw2);
+        const TC dr = (TC)y.rec[m2].d - d1;
+        if (dr * dr < maxg) term(m2, d1, (TC)xr.s, (TC)xr.a, (TC)xr.b, trow, rc1);
       }
     }
   }
   return acc;
 }
 
-template <typename TC>
-__device__ __forceinline__ NbView<TC> global_view(const NbSet &s, int64_t slot, int amax) {
+__device__ __forceinline__ NbView global_view(const NbSet &s, int64_t slot, int amax) {
   const int64_t o = slot * amax;
-  return NbView<TC>{s.d + o, s.s + o, s.c4 + o, s.c5 + o, s.row + o, s.col + o, s.ty + o};
+  return NbView{s.rec + o, s.ty + o, s.rc + o};
 }
 
 template <typename TC>
-__device__ void fill_ttab(TC *ttab, double rw2, double cw2) {
-  for (int t = threadIdx.x; t < kTypeTable; t += blockDim.x)
-    ttab[t] = (TC)stoch((double)(t / kTypeCols), (double)(t % kTypeCols), rw2, cw2);
+__device__ void load_ttab(TC *ttab, const Ctl *ctl) {
+  for (int t = threadIdx.x; t < kMaxTypes * kMaxTypes; t += blockDim.x) ttab[t] = (TC)ctl->ttab[t];
 }
 
 // Self overlaps, one thread per slot (global-memory operands).
 template <typename TC>
 __global__ __launch_bounds__(256) void k_arad_self(NbSet set, int nslots, int amax, AradParams p,
-                                                   const int32_t *__restrict__ generic_flag) {
-  __shared__ TC ttab[kTypeTable];
-  fill_ttab(ttab, p.rw2, p.cw2);
+                                                   const Ctl *__restrict__ ctl) {
+  __shared__ TC ttab[kMaxTypes * kMaxTypes];
+  load_ttab(ttab, ctl);
   __syncthreads();
   const int slot = blockIdx.x * blockDim.x + threadIdx.x;
   if (slot >= nslots) return;
   const int c = set.cnt[slot];
   const bool asc = set.asc[slot] != 0;
-  NbView<TC> v = global_view<TC>(set, slot, amax);
+  NbView v = global_view(set, slot, amax);
   TC r;
-  if (*generic_flag)
+  if (ctl->generic)
     r = arad_distl2<TC, true>(v, c, asc, v, c, asc, (TC)p.inv_w, (TC)p.maxg, ttab, p.rw2, p.cw2);
   else
     r = arad_distl2<TC, false>(v, c, asc, v, c, asc, (TC)p.inv_w, (TC)p.maxg, ttab, p.rw2, p.cw2);
@@ -278,38 +353,52 @@ __global__ __launch_bounds__(256) void k_arad_self(NbSet set, int nslots, int am
     set.self32[slot] = (float)r;
 }
 
-// Shared-memory staging of one molecule's neighbour arrays.
+// ---- molecule-pair kernel --------------------------------------------
+
+// LDS image of one molecule.
 struct MolStage {
-  double *d, *s, *c4, *c5;
+  Rec *rec;
   uint16_t *ty;
   int *cnt, *asc;
   double *self, *zr, *zc;
 };
 
-__device__ __forceinline__ MolStage carve_stage(char *&p, int amax) {
+__device__ __forceinline__ MolStage carve_stage(char *&p, int amax, bool records) {
   MolStage m;
   const int nn = amax * amax;
-  m.d = (double *)p; p += sizeof(double) * nn;
-  m.s = (double *)p; p += sizeof(double) * nn;
-  m.c4 = (double *)p; p += sizeof(double) * nn;
-  m.c5 = (double *)p; p += sizeof(double) * nn;
-  m.self = (double *)p; p += sizeof(double) * amax;
-  m.zr = (double *)p; p += sizeof(double) * amax;
-  m.zc = (double *)p; p += sizeof(double) * amax;
-  m.ty = (uint16_t *)p; p += sizeof(uint16_t) * ((nn + 3) & ~3);
-  m.cnt = (int *)p; p += sizeof(int) * amax;
-  m.asc = (int *)p; p += sizeof(int) * amax;
+  m.rec = nullptr;
+  m.ty = nullptr;
+  if (records) {
+    m.rec = (Rec *)p;
+    p += sizeof(Rec) * nn;
+  }
+  m.self = (double *)p;
+  p += sizeof(double) * amax;
+  m.zr = (double *)p;
+  p += sizeof(double) * amax;
+  m.zc = (double *)p;
+  p += sizeof(double) * amax;
+  m.cnt = (int *)p;
+  p += sizeof(int) * amax;
+  m.asc = (int *)p;
+  p += sizeof(int) * amax;
+  if (records) {
+    m.ty = (uint16_t *)p;
+    p += sizeof(uint16_t) * ((nn + 7) & ~7);
+  }
   return m;
 }
 
-size_t stage_bytes(int amax) {
+size_t stage_bytes(int amax, bool records) {
   const size_t nn = (size_t)amax * amax;
-  return sizeof(double) * (4 * nn + 3 * amax) + sizeof(uint16_t) * ((nn + 3) & ~size_t(3)) +
-         sizeof(int) * 2 * amax;
+  size_t b = (sizeof(double) * 3 + sizeof(int) * 2) * (size_t)amax;
+  if (records) b += sizeof(Rec) * nn + sizeof(uint16_t) * ((nn + 7) & ~size_t(7));
+  return b;
 }
 
 template <bool kF32Self>
-__device__ void stage_molecule(MolStage &m, const NbSet &set, int mol, int natoms, int amax) {
+__device__ void stage_molecule(MolStage &m, const NbSet &set, int mol, int natoms, int amax,
+                               bool records) {
   const int64_t slot0 = (int64_t)mol * amax;
   for (int a = threadIdx.x; a < natoms; a += blockDim.x) {
     m.cnt[a] = set.cnt[slot0 + a];
@@ -318,13 +407,11 @@ __device__ void stage_molecule(MolStage &m, const NbSet &set, int mol, int natom
     m.zr[a] = set.zr[slot0 + a];
     m.zc[a] = set.zc[slot0 + a];
   }
+  if (!records) return;
   const int total = natoms * amax;
   for (int t = threadIdx.x; t < total; t += blockDim.x) {
     const int64_t o = slot0 * amax + t;
-    m.d[t] = set.d[o];
-    m.s[t] = set.s[o];
-    m.c4[t] = set.c4[o];
-    m.c5[t] = set.c5[o];
+    m.rec[t] = set.rec[o];
     m.ty[t] = set.ty[o];
   }
 }
@@ -334,7 +421,6 @@ __device__ __forceinline__ double wave_sum(double v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   return v;
 }
-
 
 struct PairArgs {
   int mode;  // FMLH_ARAD_*
@@ -352,40 +438,23 @@ __device__ __forceinline__ int64_t dist_index(int i, int j, int a, int b, int nm
   return (int64_t)i + (int64_t)nm1 * ((int64_t)j + (int64_t)nm2 * ((int64_t)a + (int64_t)amax * b));
 }
 
-// Molecule-pair kernel.  kStage: stage both molecules' neighbour arrays in
-// LDS (amax <= 29); otherwise operands are read from the prepared global
-// arrays and only per-atom scalars are staged.
+// kStage: both molecules' records live in LDS (amax <= 29); otherwise the
+// records are read from the prepared global arrays.
 template <typename TC, bool kStage>
 __global__ __launch_bounds__(kPairThreads) void k_arad_pairs(NbSet set1, NbSet set2, PairArgs pa,
                                                               AradParams p,
-                                                              const int32_t *__restrict__ flag) {
+                                                              const Ctl *__restrict__ ctl) {
   extern __shared__ __attribute__((aligned(16))) char smem[];
   char *ptr = smem;
   TC *ttab = (TC *)ptr;
-  ptr += sizeof(double) * kTypeTable;
+  ptr += sizeof(double) * kMaxTypes * kMaxTypes;
   double *red = (double *)ptr;
   ptr += sizeof(double) * kMaxSig * (kPairThreads / 64);
-  MolStage mi, mj;
-  if (kStage) {
-    mi = carve_stage(ptr, pa.amax);
-    mj = carve_stage(ptr, pa.amax);
-  } else {
-    // scalars only
-    const int amax = pa.amax;
-    mi.self = (double *)ptr; ptr += sizeof(double) * amax;
-    mi.zr = (double *)ptr; ptr += sizeof(double) * amax;
-    mi.zc = (double *)ptr; ptr += sizeof(double) * amax;
-    mi.cnt = (int *)ptr; ptr += sizeof(int) * amax;
-    mi.asc = (int *)ptr; ptr += sizeof(int) * amax;
-    mj.self = (double *)ptr; ptr += sizeof(double) * amax;
-    mj.zr = (double *)ptr; ptr += sizeof(double) * amax;
-    mj.zc = (double *)ptr; ptr += sizeof(double) * amax;
-    mj.cnt = (int *)ptr; ptr += sizeof(int) * amax;
-    mj.asc = (int *)ptr; ptr += sizeof(int) * amax;
-  }
-  fill_ttab(ttab, p.rw2, p.cw2);
+  MolStage mi = carve_stage(ptr, pa.amax, kStage);
+  MolStage mj = carve_stage(ptr, pa.amax, kStage);
+  load_ttab(ttab, ctl);
 
-  const bool generic = *flag != 0;
+  const bool generic = ctl->generic != 0;
   const int amax = pa.amax;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const TC inv_w = (TC)p.inv_w, maxg = (TC)p.maxg;
@@ -408,33 +477,11 @@ __global__ __launch_bounds__(kPairThreads) void k_arad_pairs(NbSet set1, NbSet s
     const int nj = min(max(pa.n2[j], 0), amax);
     __syncthreads();  // the previous pair's readers are done with the stages
     if (i != cur_i) {
-      if (kStage) {
-        stage_molecule<sizeof(TC) == 4>(mi, set1, i, ni, amax);
-      } else {
-        const int64_t s0 = (int64_t)i * amax;
-        for (int a = threadIdx.x; a < ni; a += blockDim.x) {
-          mi.cnt[a] = set1.cnt[s0 + a];
-          mi.asc[a] = set1.asc[s0 + a];
-          mi.self[a] = sizeof(TC) == 4 ? (double)set1.self32[s0 + a] : set1.self[s0 + a];
-          mi.zr[a] = set1.zr[s0 + a];
-          mi.zc[a] = set1.zc[s0 + a];
-        }
-      }
+      stage_molecule<sizeof(TC) == 4>(mi, set1, i, ni, amax, kStage);
       cur_i = i;
     }
     if (j != cur_j) {
-      if (kStage) {
-        stage_molecule<sizeof(TC) == 4>(mj, set2, j, nj, amax);
-      } else {
-        const int64_t s0 = (int64_t)j * amax;
-        for (int b = threadIdx.x; b < nj; b += blockDim.x) {
-          mj.cnt[b] = set2.cnt[s0 + b];
-          mj.asc[b] = set2.asc[s0 + b];
-          mj.self[b] = sizeof(TC) == 4 ? (double)set2.self32[s0 + b] : set2.self[s0 + b];
-          mj.zr[b] = set2.zr[s0 + b];
-          mj.zc[b] = set2.zc[s0 + b];
-        }
-      }
+      stage_molecule<sizeof(TC) == 4>(mj, set2, j, nj, amax, kStage);
       cur_j = j;
     }
     __syncthreads();
@@ -452,18 +499,14 @@ __global__ __launch_bounds__(kPairThreads) void k_arad_pairs(NbSet set1, NbSet s
         pa.out[dist_index(i, j, a, b, pa.nm1, pa.nm2, amax)] = 0.0;
         continue;
       }
-      NbView<TC> x, y;
       const int64_t gx = ((int64_t)i * amax + a) * amax, gy = ((int64_t)j * amax + b) * amax;
+      NbView x, y;
       if (kStage) {
-        x = NbView<TC>{mi.d + a * amax, mi.s + a * amax, mi.c4 + a * amax, mi.c5 + a * amax,
-                       set1.row + gx, set1.col + gx, mi.ty + a * amax};
-        y = NbView<TC>{mj.d + b * amax, mj.s + b * amax, mj.c4 + b * amax, mj.c5 + b * amax,
-                       set2.row + gy, set2.col + gy, mj.ty + b * amax};
+        x = NbView{mi.rec + a * amax, mi.ty + a * amax, set1.rc + gx};
+        y = NbView{mj.rec + b * amax, mj.ty + b * amax, set2.rc + gy};
       } else {
-        x = NbView<TC>{set1.d + gx, set1.s + gx, set1.c4 + gx, set1.c5 + gx, set1.row + gx,
-                       set1.col + gx, set1.ty + gx};
-        y = NbView<TC>{set2.d + gy, set2.s + gy, set2.c4 + gy, set2.c5 + gy, set2.row + gy,
-                       set2.col + gy, set2.ty + gy};
+        x = NbView{set1.rec + gx, set1.ty + gx, set1.rc + gx};
+        y = NbView{set2.rec + gy, set2.ty + gy, set2.rc + gy};
       }
       const TC S = generic ? arad_distl2<TC, true>(x, mi.cnt[a], mi.asc[a] != 0, y, mj.cnt[b],
                                                    mj.asc[b] != 0, inv_w, maxg, ttab, p.rw2, p.cw2)
@@ -514,16 +557,16 @@ __global__ __launch_bounds__(kPairThreads) void k_arad_pairs(NbSet set1, NbSet s
 template <typename TC>
 __global__ __launch_bounds__(256) void k_arad_atomic(NbSet set1, NbSet set2, int na1, int na2,
                                                      int amax, int mode, PairArgs pa, AradParams p,
-                                                     const int32_t *__restrict__ flag) {
-  __shared__ TC ttab[kTypeTable];
-  fill_ttab(ttab, p.rw2, p.cw2);
+                                                     const Ctl *__restrict__ ctl) {
+  __shared__ TC ttab[kMaxTypes * kMaxTypes];
+  load_ttab(ttab, ctl);
   __syncthreads();
-  const bool generic = *flag != 0;
+  const bool generic = ctl->generic != 0;
   const int i = blockIdx.x * 16 + (threadIdx.x & 15);
   const int j = blockIdx.y * 16 + (threadIdx.x >> 4);
   if (i >= na1 || j >= na2) return;
-  NbView<TC> x = global_view<TC>(set1, i, amax);
-  NbView<TC> y = global_view<TC>(set2, j, amax);
+  NbView x = global_view(set1, i, amax);
+  NbView y = global_view(set2, j, amax);
   const TC inv_w = (TC)p.inv_w, maxg = (TC)p.maxg;
   const TC S = generic ? arad_distl2<TC, true>(x, set1.cnt[i], set1.asc[i] != 0, y, set2.cnt[j],
                                                set2.asc[j] != 0, inv_w, maxg, ttab, p.rw2, p.cw2)
@@ -549,7 +592,8 @@ __global__ __launch_bounds__(256) void k_arad_atomic(NbSet set1, NbSet set2, int
 size_t set_bytes(int64_t nslots, int amax) {
   const size_t nn = (size_t)nslots * amax;
   size_t b = 0;
-  b += WsCarver::need(sizeof(double) * nn) * 6;
+  b += WsCarver::need(sizeof(Rec) * nn);
+  b += WsCarver::need(sizeof(double2) * nn);
   b += WsCarver::need(sizeof(uint16_t) * nn);
   b += WsCarver::need(sizeof(int32_t) * nslots) * 2;
   b += WsCarver::need(sizeof(double) * nslots) * 3;
@@ -559,12 +603,8 @@ size_t set_bytes(int64_t nslots, int amax) {
 
 bool carve_set(WsCarver &w, int64_t nslots, int amax, NbSet &s) {
   const size_t nn = (size_t)nslots * amax;
-  s.d = w.take<double>(nn);
-  s.s = w.take<double>(nn);
-  s.c4 = w.take<double>(nn);
-  s.c5 = w.take<double>(nn);
-  s.row = w.take<double>(nn);
-  s.col = w.take<double>(nn);
+  s.rec = w.take<Rec>(nn);
+  s.rc = w.take<double2>(nn);
   s.ty = w.take<uint16_t>(nn);
   s.cnt = w.take<int32_t>(nslots);
   s.asc = w.take<int32_t>(nslots);
@@ -572,8 +612,7 @@ bool carve_set(WsCarver &w, int64_t nslots, int amax, NbSet &s) {
   s.zr = w.take<double>(nslots);
   s.zc = w.take<double>(nslots);
   s.self32 = w.take<float>(nslots);
-  return s.d && s.s && s.c4 && s.c5 && s.row && s.col && s.ty && s.cnt && s.asc && s.self &&
-         s.zr && s.zc && s.self32;
+  return s.rec && s.rc && s.ty && s.cnt && s.asc && s.self && s.zr && s.zc && s.self32;
 }
 
 AradParams make_params(double width, double cut, double rw, double cw, bool eps_1e11) {
@@ -582,12 +621,12 @@ AradParams make_params(double width, double cut, double rw, double cw, bool eps_
   p.cut = cut;
   p.rw = rw;
   p.cw = cw;
-  p.inv_w = -1.0 / (4.0 * width * width);   // farad_kernels.f90:58
-  p.maxg = (8.0 * width) * (8.0 * width);   // :60
+  p.inv_w = -1.0 / (4.0 * width * width);  // farad_kernels.f90:58
+  p.maxg = (8.0 * width) * (8.0 * width);  // :60
   p.rw2 = rw * rw;
   p.cw2 = cw * cw;
   const double pi = 4.0 * atan(1.0);
-  p.inv_cut = pi / (2.0 * cut);             // :510
+  p.inv_cut = pi / (2.0 * cut);  // :510
   p.eps = eps_1e11 ? (double)1.0e-11f : (double)5.0e-12f;  // :148, :505
   return p;
 }
@@ -599,49 +638,54 @@ int device_cus() {
 }
 
 size_t pair_lds_bytes(int amax, bool stage) {
-  size_t b = sizeof(double) * (kTypeTable + kMaxSig * (kPairThreads / 64));
-  if (stage)
-    b += 2 * stage_bytes(amax);
-  else
-    b += 2 * (sizeof(double) * 3 + sizeof(int) * 2) * (size_t)amax;
-  return b;
+  return sizeof(double) * (kMaxTypes * kMaxTypes + kMaxSig * (kPairThreads / 64)) +
+         2 * stage_bytes(amax, stage);
 }
 
 constexpr size_t kLdsLimit = 65536;
 
-// Shared preparation for both entry points.
+// Preparation shared by both entry points: control block, neighbour
+// records, type ids, self overlaps.
 int prepare_sets(const double *q1, const double *zt1, const int32_t *n1, int64_t ns1,
                  const double *q2, const double *zt2, const int32_t *n2, int64_t ns2, int amax,
                  bool one_set, int per_atom_n, int guard, int precision, const AradParams &p,
-                 void *ws, size_t ws_bytes, NbSet &s1, NbSet &s2, int32_t *&flag,
-                 hipStream_t st) {
+                 void *ws, size_t ws_bytes, NbSet &s1, NbSet &s2, Ctl *&ctl, hipStream_t st) {
   WsCarver w(ws, ws_bytes);
-  flag = w.take<int32_t>(64);
-  if (!flag || !carve_set(w, ns1, amax, s1)) return -1;
+  ctl = w.take<Ctl>(1);
+  if (!ctl || !carve_set(w, ns1, amax, s1)) return -1;
   if (!one_set) {
     if (!carve_set(w, ns2, amax, s2)) return -1;
   } else {
     s2 = s1;
   }
-  hipLaunchKernelGGL(k_zero_flag, dim3(1), dim3(1), 0, st, flag);
-  FMLH_LAUNCH_CHECK("k_zero_flag");
+  hipLaunchKernelGGL(k_ctl_clear, dim3(1), dim3(256), 0, st, ctl);
+  FMLH_LAUNCH_CHECK("k_ctl_clear");
   hipLaunchKernelGGL(k_arad_prep, dim3((unsigned)ceil_div64(ns1, 256)), dim3(256), 0, st, q1, zt1,
-                     n1, (int)ns1, amax, per_atom_n, guard, p, s1, flag);
+                     n1, (int)ns1, amax, per_atom_n, guard, p, s1, ctl);
   FMLH_LAUNCH_CHECK("k_arad_prep");
   if (!one_set) {
     hipLaunchKernelGGL(k_arad_prep, dim3((unsigned)ceil_div64(ns2, 256)), dim3(256), 0, st, q2,
-                       zt2, n2, (int)ns2, amax, per_atom_n, guard, p, s2, flag);
+                       zt2, n2, (int)ns2, amax, per_atom_n, guard, p, s2, ctl);
     FMLH_LAUNCH_CHECK("k_arad_prep");
+  }
+  hipLaunchKernelGGL(k_arad_types, dim3(1), dim3(kTypeBitmap), 0, st, ctl, p.rw2, p.cw2);
+  FMLH_LAUNCH_CHECK("k_arad_types");
+  for (int k = 0; k < (one_set ? 1 : 2); ++k) {
+    NbSet &s = k == 0 ? s1 : s2;
+    const int64_t ns = k == 0 ? ns1 : ns2;
+    hipLaunchKernelGGL(k_arad_type_ids, dim3((unsigned)ceil_div64(ns * amax, 256)), dim3(256), 0,
+                       st, s, (int)ns, amax, ctl);
+    FMLH_LAUNCH_CHECK("k_arad_type_ids");
   }
   for (int k = 0; k < (one_set ? 1 : 2); ++k) {
     NbSet &s = k == 0 ? s1 : s2;
     const int64_t ns = k == 0 ? ns1 : ns2;
     if (precision)
       hipLaunchKernelGGL(k_arad_self<float>, dim3((unsigned)ceil_div64(ns, 256)), dim3(256), 0, st,
-                         s, (int)ns, amax, p, flag);
+                         s, (int)ns, amax, p, ctl);
     else
       hipLaunchKernelGGL(k_arad_self<double>, dim3((unsigned)ceil_div64(ns, 256)), dim3(256), 0,
-                         st, s, (int)ns, amax, p, flag);
+                         st, s, (int)ns, amax, p, ctl);
     FMLH_LAUNCH_CHECK("k_arad_self");
   }
   return 0;
@@ -654,14 +698,14 @@ extern "C" {
 size_t fmlh_arad_ws_bytes(int nm1, int nm2, int amax, int mode) {
   if (nm1 < 0 || nm2 < 0 || amax < 1) return 0;
   const bool one_set = (mode == FMLH_ARAD_SYMMETRIC || mode == FMLH_ARAD_KERNEL);
-  size_t b = WsCarver::need(sizeof(int32_t) * 64) + set_bytes((int64_t)nm1 * amax, amax);
+  size_t b = WsCarver::need(sizeof(Ctl)) + set_bytes((int64_t)nm1 * amax, amax);
   if (!one_set) b += set_bytes((int64_t)nm2 * amax, amax);
   return b;
 }
 
 size_t fmlh_arad_atomic_ws_bytes(int na1, int na2, int amax) {
   if (na1 < 0 || na2 < 0 || amax < 1) return 0;
-  return WsCarver::need(sizeof(int32_t) * 64) + set_bytes(na1, amax) + set_bytes(na2, amax);
+  return WsCarver::need(sizeof(Ctl)) + set_bytes(na1, amax) + set_bytes(na2, amax);
 }
 
 int fmlh_arad_kernels(int mode, const double *q1, const double *zt1, const int32_t *n1, int nm1,
@@ -696,11 +740,11 @@ int fmlh_arad_kernels(int mode, const double *q1, const double *zt1, const int32
                                                   : make_params(width, cut, r_width, c_width, false);
   hipStream_t st = as_stream(stream);
   NbSet s1, s2;
-  int32_t *flag = nullptr;
+  Ctl *ctl = nullptr;
   const int guard = (mode == FMLH_ARAD_GLOBAL) ? 1 : 0;  // farad_kernels.f90:526,539
   int rc = prepare_sets(q1, zt1, n1, (int64_t)nm1 * amax, q2, zt2, n2, (int64_t)nm2 * amax, amax,
-                        one_set, 0, guard, precision, p, ws, ws_bytes, s1, s2, flag, st);
-  if (rc) return rc < 0 && rc > -100 ? -19 : rc;
+                        one_set, 0, guard, precision, p, ws, ws_bytes, s1, s2, ctl, st);
+  if (rc) return rc == -1 ? -19 : rc;
 
   PairArgs pa;
   pa.mode = mode;
@@ -726,17 +770,17 @@ int fmlh_arad_kernels(int mode, const double *q1, const double *zt1, const int32
   if (precision) {
     if (stage)
       hipLaunchKernelGGL((k_arad_pairs<float, true>), dim3(grid), dim3(kPairThreads), lds, st, s1,
-                         s2, pa, p, flag);
+                         s2, pa, p, ctl);
     else
       hipLaunchKernelGGL((k_arad_pairs<float, false>), dim3(grid), dim3(kPairThreads), lds, st, s1,
-                         s2, pa, p, flag);
+                         s2, pa, p, ctl);
   } else {
     if (stage)
       hipLaunchKernelGGL((k_arad_pairs<double, true>), dim3(grid), dim3(kPairThreads), lds, st, s1,
-                         s2, pa, p, flag);
+                         s2, pa, p, ctl);
     else
       hipLaunchKernelGGL((k_arad_pairs<double, false>), dim3(grid), dim3(kPairThreads), lds, st,
-                         s1, s2, pa, p, flag);
+                         s1, s2, pa, p, ctl);
   }
   timing_end(st);
   FMLH_LAUNCH_CHECK("k_arad_pairs");
@@ -767,10 +811,10 @@ int fmlh_arad_atomic(int mode, const double *q1, const double *zt1, const int32_
   const AradParams p = make_params(width, cut, r_width, c_width, false);
   hipStream_t st = as_stream(stream);
   NbSet s1, s2;
-  int32_t *flag = nullptr;
+  Ctl *ctl = nullptr;
   int rc = prepare_sets(q1, zt1, n1, na1, q2, zt2, n2, na2, amax, false, 1, 0, 0, p, ws, ws_bytes,
-                        s1, s2, flag, st);
-  if (rc) return rc < 0 && rc > -100 ? -18 : rc;
+                        s1, s2, ctl, st);
+  if (rc) return rc == -1 ? -18 : rc;
   PairArgs pa;
   pa.mode = mode;
   pa.nm1 = na1;
@@ -787,7 +831,7 @@ int fmlh_arad_atomic(int mode, const double *q1, const double *zt1, const int32_
   dim3 grid(ceil_div(na1, 16), ceil_div(na2, 16));
   timing_begin(st);
   hipLaunchKernelGGL(k_arad_atomic<double>, grid, dim3(256), 0, st, s1, s2, na1, na2, amax, mode,
-                     pa, p, flag);
+                     pa, p, ctl);
   timing_end(st);
   FMLH_LAUNCH_CHECK("k_arad_atomic");
   return 0;

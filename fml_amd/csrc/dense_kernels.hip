@@ -5,8 +5,8 @@
 //   fml/kernels/fkernels.f90:317-349.  ||a||^2 + ||b||^2 - 2 a.b with the dot
 //   products on MFMA (64x64 tiles, see gemm_nt.h) and exp fused in the
 //   epilogue; an entry whose GEMM-form distance is within the cancellation
-//   band (d2 <= tau*(||a||^2+||b||^2)) is recomputed by direct differences
-//   in the reference's order (sum of squares ascending over t).
+//   band (d2 <= tau*(||a||^2+||b||^2)) is recomputed from direct differences
+//   by its wavefront (near_pairs.h).
 // Laplacian K(j,i) = exp(-1/sigma * sum_t |a_jt - b_it|)
 //   fml/kernels/fkernels.f90:351-377.  64x64 output tile per 256 threads,
 //   4x4 entries per thread, operands staged k-major in LDS; every entry's
@@ -16,21 +16,13 @@
 
 #include "common.h"
 #include "gemm_nt.h"
+#include "near_pairs.h"
 
 namespace {
 
 using namespace fmlh;
 
-template <typename T>
-__device__ __forceinline__ T direct_sqdist(const T *__restrict__ a, const T *__restrict__ b, int d) {
-#pragma clang fp contract(off)
-  T s = T(0);
-  for (int t = 0; t < d; ++t) {
-    T diff = a[t] - b[t];
-    s = s + diff * diff;
-  }
-  return s;
-}
+
 
 __device__ __forceinline__ double kexp(double x) { return exp(x); }
 __device__ __forceinline__ float kexp(float x) { return expf(x); }
@@ -63,11 +55,12 @@ __global__ __launch_bounds__(256) void k_gaussian(const T *__restrict__ A, int64
         const int m = tile_row<T>(im, r);
         const int n = tile_col(in);
         const int gm = m0 + m, gn = n0 + n;
-        if (gm < nb && gn < na) {
-          const T pn = s.pn[m], qn = s.qn[n];
-          T d2 = pn + qn - T(2) * acc.v[im][in][r];
-          if (d2 <= tau * (pn + qn))
-            d2 = direct_sqdist(A + (int64_t)gn * lda, B + (int64_t)gm * ldb, d);
+        const bool valid = gm < nb && gn < na;
+        const T pn = s.pn[m], qn = s.qn[n];
+        T d2 = pn + qn - T(2) * acc.v[im][in][r];
+        const bool near = valid && d2 <= tau * (pn + qn);
+        d2 = recompute_near(near, d2, A, lda, gn, B, ldb, gm, d);
+        if (valid) {
           if (d2 < T(0)) d2 = T(0);
           double v;
           if (sizeof(OutT) == 8)

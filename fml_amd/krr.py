@@ -126,15 +126,15 @@ class KRR:
         t1 = time.perf_counter()
         K32 = K.to(t.float32)
         shift = 0.0
-        for attempt in range(8):
-            F = K32.clone() if attempt else K32
-            add32 = self.lam + shift
-            idx = t.arange(n, device="cuda")
-            F[idx, idx] += add32
+        idx = t.arange(n, device="cuda")
+        dmax = float(K32.diagonal().abs().max().item())
+        for attempt in range(10):
+            F = K32.clone()  # potrf overwrites its argument
+            F[idx, idx] += self.lam + shift
             info = potrf_inplace(F)
             if info == 0:
                 break
-            shift = max(shift * 10.0, 1e-5 * float(K32.diagonal().abs().max().item()))
+            shift = max(shift * 10.0, 1e-6 * dmax)
         else:
             raise np.linalg.LinAlgError("FP32 factor failed even with a shift")
         del K32

@@ -207,3 +207,67 @@ def test_cho_solve_sizes_and_failure(gpu):
     assert np.all(np.isnan(x))
     with pytest.raises(ValueError):
         cho_solve(np.ones((3, 4)), np.ones(3))
+
+
+# ------------------------------------------------------------ precision forms
+
+def test_gaussian_precision_forms(gpu, mols):
+    """FP32-MFMA and split-BF16x3 forms on float descriptors.  Stated error
+    bounds against the FP64 oracle (relative, entries > 1e-300):
+    FP32 5e-5, BF16x3 5e-4 at sigma = 1000 on nmax-29 Coulomb matrices."""
+    import torch
+    from fml_amd.kernels import gaussian_kernel_rows
+    X = _cm(mols)
+    X32 = torch.from_numpy(X.astype(np.float32)).cuda()
+    ko = oracle.gaussian_kernel(X[:100].T, X[100:].T, 1000.0)
+    for prec, tol in (("fp32", 5e-5), ("bf16x3", 5e-4)):
+        for od in (np.float32, np.float64):
+            k = gaussian_kernel_rows(X32[:100], X32[100:], 1000.0, precision=prec, out_dtype=od)
+            kh = k.cpu().numpy().T
+            assert _rel(kh, ko) < tol, (prec, od)
+        kk = gaussian_kernel_rows(X32, X32, 1000.0, precision=prec).cpu().numpy()
+        assert np.all(np.diag(kk) == 1.0)
+
+
+def test_laplacian_fp32(gpu, mols):
+    import torch
+    from fml_amd.kernels import laplacian_kernel_rows
+    X = _cm(mols)
+    X32 = torch.from_numpy(X.astype(np.float32)).cuda()
+    ko = oracle.laplacian_kernel(X[:100].T, X[100:].T, 1e4)
+    k = laplacian_kernel_rows(X32[:100], X32[100:], 1e4).cpu().numpy().T
+    assert _rel(k, ko) < 1e-5
+
+
+def test_arad_fp32_variant(gpu, arad_set):
+    from fml_amd.kernels import get_atomic_kernels_arad
+    X, Z, z, n = arad_set
+    k = get_atomic_kernels_arad(X[:20], X[20:], Z[:20], Z[20:], [1.0, 4.0], precision=1)
+    ko = oracle.kernels_arad(X[:20], X[20:], z[:20], z[20:], n[:20], n[20:], [1.0, 4.0], 0.2, 6.0)
+    assert _rel(k[1], ko[1]) < 1e-4
+    assert _rel(k[0], ko[0]) < 1e-3
+
+
+def test_krr_fp64_and_mixed(gpu):
+    """KRR end to end on the device: the plain FP64 fit reproduces the
+    LAPACK solution's predictions; the mixed form (FP32 factor as CG
+    preconditioner, FP64 residuals) reaches a small FP64 backward error."""
+    from fml_amd.krr import KRR
+    ms = synth.generate(400, seed=0, nmin=5, nmax=23)
+    X = oracle.coulomb_matrices(ms.charges, ms.coords, ms.offsets, 23)[0]
+    y = ms.energies
+    Xtr, Xte, ytr = X[:320], X[320:], y[:320]
+    K = oracle.gaussian_kernel(Xtr.T, Xtr.T, 700.0)
+    K[np.diag_indices_from(K)] += 1e-8
+    ao, info = oracle.cho_solve(K, ytr)
+    po = oracle.gaussian_kernel(Xtr.T, Xte.T, 700.0).T @ ao
+    m = KRR(700.0, 1e-8).fit(Xtr, ytr)
+    p = m.predict(Xte).cpu().numpy()
+    scale = np.max(np.abs(po))
+    assert np.max(np.abs(p - po)) < 1e-6 * scale
+    mae_o = np.mean(np.abs(po - y[320:]))
+    assert abs(np.mean(np.abs(p - y[320:])) - mae_o) <= 1e-3 * mae_o
+    mm = KRR(700.0, 1e-8, precision="mixed", tol=1e-12, max_iter=300).fit(Xtr, ytr)
+    pm = mm.predict(Xte).cpu().numpy()
+    assert mm.report.backward_error < 1e-12
+    assert abs(np.mean(np.abs(pm - y[320:])) - mae_o) <= 1e-2 * mae_o
