@@ -164,10 +164,14 @@ __global__ __launch_bounds__(256) void k_arad_prep(const double *__restrict__ q,
       const int64_t o = (int64_t)slot * amax + m;
       set.rec[o] = Rec{d, sv, x[3 * amax + m], x[4 * amax + m]};
       set.rc[o] = make_double2(rowv, colv);
-      if (small_int(rowv, kTypeRows) && small_int(colv, kTypeCols))
-        atomicOr(&ctl->bitmap[(int)rowv * kTypeCols + (int)colv], 1);
-      else
+      if (small_int(rowv, kTypeRows) && small_int(colv, kTypeCols)) {
+        // a handful of types cover every neighbour: set each bit once
+        // instead of serialising thousands of atomics on the same words
+        int32_t *bit = &ctl->bitmap[(int)rowv * kTypeCols + (int)colv];
+        if (__atomic_load_n(bit, __ATOMIC_RELAXED) == 0) atomicOr(bit, 1);
+      } else {
         generic = true;
+      }
       if (cnt > 0 && !(prev <= d)) asc = 0;
       prev = d;
       ++cnt;
@@ -265,7 +269,8 @@ __device__ TC arad_distl2(const NbView &x, int cnt1, bool asc1, const NbView &y,
                           bool asc2, TC inv_w, TC maxg, const TC *__restrict__ ttab, double rw2,
                           double cw2) {
   TC acc = TC(0);
-  auto term = [&](int m2, TC d1, TC s1, TC a1, TC b1, int trow, double2 rc1) {
+  // d * ((1 + a1 a2) + b1 b2), the quantity farad_kernels.f90:83 adds
+  auto term_value = [&](int m2, TC d1, TC s1, TC a1, TC b1, int trow, double2 rc1) -> TC {
     const Rec yr = y.rec[m2];
     const TC dr = (TC)yr.d - d1;
     const TC r2 = dr * dr;
@@ -281,7 +286,10 @@ __device__ TC arad_distl2(const NbView &x, int cnt1, bool asc1, const NbView &y,
     v = v * t;
     TC ang = TC(1) + a1 * (TC)yr.a;
     ang = ang + b1 * (TC)yr.b;
-    acc = acc + v * ang;
+    return v * ang;
+  };
+  auto term = [&](int m2, TC d1, TC s1, TC a1, TC b1, int trow, double2 rc1) {
+    acc = acc + term_value(m2, d1, s1, a1, b1, trow, rc1);
   };
   if (asc1 && asc2) {
     int lo = 0, hi = 0;
@@ -303,7 +311,16 @@ __device__ TC arad_distl2(const NbView &x, int cnt1, bool asc1, const NbView &y,
       if (lo == hi) continue;
       const int trow = kGeneric ? 0 : (int)x.ty[m1] * kMaxTypes;
       const double2 rc1 = kGeneric ? x.rc[m1] : make_double2(0.0, 0.0);
-      for (int m2 = lo; m2 < hi; ++m2) term(m2, d1, (TC)xr.s, (TC)xr.a, (TC)xr.b, trow, rc1);
+      // two independent terms per trip for instruction-level parallelism,
+      // added to acc one after the other in m2 order
+      int m2 = lo;
+      for (; m2 + 1 < hi; m2 += 2) {
+        const TC v0 = term_value(m2, d1, (TC)xr.s, (TC)xr.a, (TC)xr.b, trow, rc1);
+        const TC v1 = term_value(m2 + 1, d1, (TC)xr.s, (TC)xr.a, (TC)xr.b, trow, rc1);
+        acc = acc + v0;
+        acc = acc + v1;
+      }
+      if (m2 < hi) term(m2, d1, (TC)xr.s, (TC)xr.a, (TC)xr.b, trow, rc1);
     }
   } else {
     for (int m1 = 0; m1 < cnt1; ++m1) {
