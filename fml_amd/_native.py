@@ -83,6 +83,12 @@ def load(path: str = LIB_PATH):
         raise NativeError(
             f"{path} is missing: build it with `python -m fml_amd.build` "
             "(there is no CPU fallback)")
+    # PyTorch ships its own HIP runtime under the same soname; load it first
+    # so this library binds to the runtime that owns the device memory.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(path)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)
