@@ -49,7 +49,9 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="arad",
                     choices=["arad", "krr", "gaussian64", "gaussian32", "gaussianbf16x3",
-                             "laplacian64", "laplacian32", "cholesky64", "cholesky32"])
+                             "laplacian64", "laplacian32", "cholesky64", "cholesky32",
+                             "krr36k_fp64", "krr36k_mixed", "serving"])
+    ap.add_argument("--duration", type=float, default=5.0, help="serving run length in seconds")
     ap.add_argument("--n", type=int, default=0, help="problem size override for the 36k workloads")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-rows", type=int, default=100, help="row block of the CPU baseline sample")
@@ -106,11 +108,12 @@ def arad_workload(args):
     wsb = lib.fmlh_arad_ws_bytes(nm, nm, amax, nat.ARAD_GLOBAL)
     ws = workspace(wsb)
     sarr = (ctypes.c_double * len(sig))(*sig)
+    nh, nhp = nat.host_i32(n)
 
     def step():
         nat.call("fmlh_arad_kernels", nat.ARAD_GLOBAL, nat.ptr(dq), nat.ptr(dz), nat.ptr(dn), nm,
                  nat.ptr(dq), nat.ptr(dz), nat.ptr(dn), nm, amax, sarr, len(sig), width, cut, rw, cw,
-                 0, nat.ptr(out), nat.ptr(ws), wsb, nat.stream_ptr())
+                 0, nat.ptr(out), nat.ptr(ws), wsb, nhp, nhp, nat.stream_ptr())
 
     units_per_step = len(sig) * nm * nm
     info = {
@@ -121,7 +124,7 @@ def arad_workload(args):
                                "descriptors (500,23,5,23), sigmas {0.1,1,10}, width 0.2, cut 5.0, "
                                "r_width 1.0, c_width 0.5", "molecules": nm, "amax": amax,
                    "sigmas": sig, "inner_evaluations_per_step": evals},
-        "kernel": "k_arad_pairs",
+        "kernel": "k_arad_tiles",
     }
 
     def roofline(t_main_s):
@@ -153,7 +156,6 @@ def arad_workload(args):
 
 
 def _cm_set(nmol, seed, nmin, nmax_atoms, nmax):
-    import oracle
     from fml_amd import synth
     ms = synth.generate(nmol, seed=seed, nmin=nmin, nmax=nmax_atoms)
     return ms
@@ -300,8 +302,130 @@ def cholesky_workload(args, kind):
     return info, step, None, roofline, None, finish
 
 
+def _train_36k(n_total, n_train, precision="fp64", sigma=1000.0, lam=1e-6):
+    """Coulomb features (nmax 29, 40k molecules seed 0 3-29 atoms) and a fitted model."""
+    import torch
+    from fml_amd.device import to_device
+    from fml_amd.krr import KRR
+    from fml_amd.representations import generate_coulomb_matrices
+    ms = _cm_set(n_total, 0, 3, 29, 29)
+    X = generate_coulomb_matrices(to_device(ms.charges), to_device(ms.coords),
+                                  to_device(ms.offsets, np.int32), nmax=29, on_device=True)
+    y = ms.energies
+    m = KRR(sigma, lam, precision=precision).fit(X[:n_train], y[:n_train])
+    torch.cuda.synchronize()
+    return ms, X, y, m
+
+
+def krr36k_workload(args, precision):
+    """Precision form at scale: fit a 36,000-molecule Gaussian KRR (sigma
+    1000, lambda 1e-6) and predict 4,000 held-out molecules, FP64 plain form
+    vs the mixed form (FP32 factor preconditioning FP64-residual CG)."""
+    import torch
+    from fml_amd.krr import KRR
+    n = args.n or 40000
+    ntr = int(n * 0.9)
+    ms = _cm_set(n, 0, 3, 29, 29)
+    from fml_amd.device import to_device
+    from fml_amd.representations import generate_coulomb_matrices
+    X = generate_coulomb_matrices(to_device(ms.charges), to_device(ms.coords),
+                                  to_device(ms.offsets, np.int32), nmax=29, on_device=True)
+    y = ms.energies
+    state = {}
+
+    def step():
+        m = KRR(1000.0, 1e-6, precision=precision, tol=1e-13, max_iter=400).fit(X[:ntr], y[:ntr])
+        state["m"] = m
+        state["pred"] = m.predict(X[ntr:])
+
+    info = {"metric": "KRR fit+predict time", "unit": "ms", "dtype": "f64" if precision == "fp64" else "f32+f64",
+            "higher_is_better": False,
+            "config": {"workload": f"Gaussian KRR fit {ntr} + predict {n - ntr} molecules (nmax 29, seed 0), "
+                                   f"sigma 1000, lambda 1e-6, precision {precision}"}}
+
+    def finish(res):
+        m = state["m"]
+        pred = state["pred"].cpu().numpy()
+        res["mae"] = float(np.mean(np.abs(pred - y[ntr:])))
+        rep = m.report
+        res["fit_breakdown_s"] = {"kernel": round(rep.kernel_s, 4), "factor": round(rep.factor_s, 4),
+                                  "solve": round(rep.solve_s, 4), "total": round(rep.seconds, 4)}
+        res["iterations"] = rep.iterations
+        res["shift"] = rep.shift
+        res["alpha"] = m.alpha.cpu().numpy()[:8].tolist()
+        np.save(f"gpurun_out/alpha_{precision}.npy", m.alpha.cpu().numpy())
+        np.save(f"gpurun_out/pred_{precision}.npy", pred)
+
+    return info, step, None, None, None, finish
+
+
+def serving_workload(args):
+    """Serving loop: 36,000-molecule Gaussian KRR model resident in HBM;
+    Poisson predict arrivals at 2,000 req/s, each 1-256 molecules
+    (log-uniform) of 3-29 atoms; fit requests of 1,000-16,000 molecules
+    interleaved, admitted by free HBM.  Reports p50/p99 latency."""
+    import threading
+    import torch
+    from fml_amd import synth
+    from fml_amd.serving import KRRServer, latency_percentiles
+    ntr = args.n or 36000
+    ms, X, y, m = _train_36k(ntr, ntr)
+    pool = synth.generate(4000, seed=2, nmin=3, nmax=29)
+    pool_mols = [pool.molecule(i) for i in range(pool.nmol)]
+    rng = np.random.default_rng(2)
+    duration = args.duration
+    rate = 2000.0
+    srv = KRRServer(X[:ntr], m.alpha, 1000.0, nmax=29, max_batch_molecules=16384).start()
+    # warm-up
+    srv.submit([pool_mols[0][0]], [pool_mols[0][1]]).result(timeout=120)
+    futs = []
+    t_start = time.perf_counter()
+    t_next = t_start
+    fit_times = [t_start + duration * f for f in (0.25, 0.6)]
+    fit_futs = []
+    nmol_total = 0
+    while time.perf_counter() - t_start < duration:
+        t_next += rng.exponential(1.0 / rate)
+        now = time.perf_counter()
+        if t_next > now:
+            time.sleep(t_next - now)
+        k = int(np.floor(np.exp(rng.uniform(0, np.log(257)))))
+        k = min(max(k, 1), 256)
+        idx = rng.integers(0, len(pool_mols), k)
+        futs.append(srv.submit([pool_mols[i][0] for i in idx], [pool_mols[i][1] for i in idx]))
+        nmol_total += k
+        if fit_times and time.perf_counter() >= fit_times[0]:
+            fit_times.pop(0)
+            nf = int(rng.integers(1000, 16001))
+            fit_futs.append((nf, srv.submit_fit(X[:nf].cpu().numpy().astype(np.float32), y[:nf], 1000.0, 1e-6)))
+    res = [f.result(timeout=300) for f in futs]
+    t_end = time.perf_counter()
+    fits = [(nf, f.result(timeout=600)) for nf, f in fit_futs]
+    srv.stop()
+    lat = latency_percentiles([r[1] for r in res])
+    wall = t_end - t_start
+    return {"metric": "serving p50/p99 predict latency", "value": round(lat["p99_ms"], 3), "unit": "ms (p99)",
+            "p50_ms": round(lat["p50_ms"], 3), "p99_ms": round(lat["p99_ms"], 3), "mean_ms": round(lat["mean_ms"], 3),
+            "requests": len(res), "molecules": nmol_total, "wall_s": round(wall, 3),
+            "throughput_req_s": round(len(res) / wall, 1), "throughput_mol_s": round(nmol_total / wall, 1),
+            "batches": len(srv.batches), "mean_batch_requests": round(float(np.mean([b[0] for b in srv.batches])), 2),
+            "fits": [{"n": nf, "seconds": round(s, 3)} for nf, s in srv.fit_log],
+            "higher_is_better": False, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": f"serving: {ntr}-molecule Gaussian KRR model resident, Poisson 2000 req/s, "
+                                   "1-256 molecules per request, fit requests 1k-16k admitted by free HBM",
+                       "duration_s": duration}}
+
+
 def main():
     args = parse()
+    if args.workload == "serving":
+        import torch
+        torch.cuda.set_device(0)
+        out = serving_workload(args)
+        out.update({"n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": out["p50_ms"], "scaling": "weak",
+                    "vs_baseline": None, "roofline": None})
+        print(json.dumps(out), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); every GPU box here has "
               "one GPU and multi-GPU sharding is a design note, not a measured path.", file=sys.stderr)
@@ -319,6 +443,8 @@ def main():
         info, step, units, roofline, cpu_baseline, finish = arad_workload(args)
     elif wl == "krr":
         info, step, units, roofline, cpu_baseline, finish = krr_workload(args)
+    elif wl.startswith("krr36k"):
+        info, step, units, roofline, cpu_baseline, finish = krr36k_workload(args, wl.split("_")[1])
     elif wl.startswith("cholesky"):
         info, step, units, roofline, cpu_baseline, finish = cholesky_workload(args, wl)
     else:

@@ -41,7 +41,7 @@ SIGNATURES = {
     "fmlh_arad_ws_bytes": (c_size_t, [c_int, c_int, c_int, c_int]),
     "fmlh_arad_kernels": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
                                   c_int, c_int, c_dbl_p, c_int, c_double, c_double, c_double, c_double,
-                                  c_int, c_void_p, c_void_p, c_size_t, c_void_p]),
+                                  c_int, c_void_p, c_void_p, c_size_t, c_void_p, c_void_p, c_void_p]),
     "fmlh_arad_atomic_ws_bytes": (c_size_t, [c_int, c_int, c_int]),
     "fmlh_arad_atomic": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
                                  c_int, c_int, c_dbl_p, c_int, c_double, c_double, c_double, c_double,
@@ -129,3 +129,13 @@ def ptr(t):
     if t is None:
         return c_void_p(0)
     return c_void_p(t.data_ptr())
+
+
+def host_i32(a):
+    """(array, pointer) of a contiguous host int32 copy of a; (None, NULL) for
+    device tensors or None."""
+    if a is None or hasattr(a, "is_cuda"):
+        return None, c_void_p(0)
+    import numpy as np
+    h = np.ascontiguousarray(np.asarray(a), dtype=np.int32)
+    return h, c_void_p(h.ctypes.data)

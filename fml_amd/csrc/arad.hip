@@ -45,6 +45,11 @@
 // because none can occur on that domain.  Measured against exact exp over
 // (-16, 0]: at most 0.82 ulp.
 #include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <mutex>
+#include <vector>
 
 #include "common.h"
 
@@ -569,6 +574,178 @@ __global__ __launch_bounds__(kPairThreads) void k_arad_pairs(NbSet set1, NbSet s
   }
 }
 
+// ---- size-sorted molecule-pair tiles ---------------------------------
+//
+// A tile is one set-1 molecule i and J set-2 molecules that all have the
+// same atom count s (a run of the size class's member list, ascending
+// index), J chosen on the host so J * n_i * s atom pairs fill the workgroup.
+// Workgroups take tiles from an atomic counter until the list is exhausted
+// (every workgroup reaches that exit).  Both sides are staged in LDS with
+// stride = atom count; a pair's per-sigma sum is reduced over its own lanes
+// in a fixed order, so it depends only on that pair.
+
+struct Tile {
+  int32_t i;    // set-1 molecule
+  int32_t off;  // first member in the class list
+  int32_t cnt;  // J
+  int32_t s;    // atom count of the J set-2 molecules
+};
+
+struct TileArgs {
+  int mode;
+  int nm1, amax, nsig, triangular;
+  int ntiles;
+  int stage_i_recs;  // LDS capacity (records) of the molecule-i stage
+  int stage_j_recs;  // LDS capacity (records) of the J-molecule stage
+  int stage_j_atoms;
+  double inv_s2[kMaxSig];
+  const int32_t *n1;
+  const Tile *tiles;
+  const int32_t *members;
+  int32_t *counter;
+  double *out;
+};
+
+template <bool kF32Self>
+__device__ void stage_compact(Rec *rec, uint16_t *ty, int *cnt, int *asc, double *self,
+                              double *zr, double *zc, const NbSet &set, int mol, int natoms,
+                              int amax, int atom0) {
+  const int64_t slot0 = (int64_t)mol * amax;
+  for (int a = threadIdx.x; a < natoms; a += blockDim.x) {
+    const int64_t sl = slot0 + a;
+    cnt[atom0 + a] = set.cnt[sl];
+    asc[atom0 + a] = set.asc[sl];
+    self[atom0 + a] = kF32Self ? (double)set.self32[sl] : set.self[sl];
+    zr[atom0 + a] = set.zr[sl];
+    zc[atom0 + a] = set.zc[sl];
+  }
+  const int total = natoms * natoms;
+  for (int t = threadIdx.x; t < total; t += blockDim.x) {
+    const int a = t / natoms, m = t % natoms;
+    const int64_t o = (slot0 + a) * amax + m;
+    rec[(atom0 + a) * natoms + m] = set.rec[o];
+    ty[(atom0 + a) * natoms + m] = set.ty[o];
+  }
+}
+
+template <typename TC>
+__global__ __launch_bounds__(kPairThreads) void k_arad_tiles(NbSet set1, NbSet set2, TileArgs ta,
+                                                              AradParams p,
+                                                              const Ctl *__restrict__ ctl) {
+  extern __shared__ __attribute__((aligned(16))) char smem[];
+  char *ptr = smem;
+  TC *ttab = (TC *)ptr;
+  ptr += sizeof(double) * kMaxTypes * kMaxTypes;
+  int *tile_slot = (int *)ptr;
+  ptr += 16;
+  // molecule-i stage
+  Rec *reci = (Rec *)ptr;
+  ptr += sizeof(Rec) * ta.stage_i_recs;
+  double *selfi = (double *)ptr;
+  ptr += sizeof(double) * ta.amax;
+  double *zri = (double *)ptr;
+  ptr += sizeof(double) * ta.amax;
+  double *zci = (double *)ptr;
+  ptr += sizeof(double) * ta.amax;
+  int *cnti = (int *)ptr;
+  ptr += sizeof(int) * ta.amax;
+  int *asci = (int *)ptr;
+  ptr += sizeof(int) * ta.amax;
+  uint16_t *tyi = (uint16_t *)ptr;
+  ptr += sizeof(uint16_t) * ((ta.stage_i_recs + 7) & ~7);
+  // J-molecule stage (reused as the reduction buffer after compute)
+  char *jbase = ptr;
+  Rec *recj = (Rec *)ptr;
+  ptr += sizeof(Rec) * ta.stage_j_recs;
+  double *selfj = (double *)ptr;
+  ptr += sizeof(double) * ta.stage_j_atoms;
+  double *zrj = (double *)ptr;
+  ptr += sizeof(double) * ta.stage_j_atoms;
+  double *zcj = (double *)ptr;
+  ptr += sizeof(double) * ta.stage_j_atoms;
+  int *cntj = (int *)ptr;
+  ptr += sizeof(int) * ta.stage_j_atoms;
+  int *ascj = (int *)ptr;
+  ptr += sizeof(int) * ta.stage_j_atoms;
+  uint16_t *tyj = (uint16_t *)ptr;
+  double *red = (double *)jbase;
+  load_ttab(ttab, ctl);
+
+  const bool generic = ctl->generic != 0;
+  const int amax = ta.amax;
+  const TC inv_w = (TC)p.inv_w, maxg = (TC)p.maxg;
+  const int nsig = (ta.mode == FMLH_ARAD_KERNEL) ? 1 : ta.nsig;
+  int cur_i = -1;
+  for (;;) {
+    __syncthreads();  // previous tile's readers are done with stages/red
+    if (threadIdx.x == 0) tile_slot[0] = atomicAdd(ta.counter, 1);
+    __syncthreads();
+    const int tix = tile_slot[0];
+    if (tix >= ta.ntiles) break;
+    const Tile tl = ta.tiles[tix];
+    const int i = tl.i, J = tl.cnt, s = tl.s;
+    const int ni = min(max(ta.n1[i], 0), amax);
+    if (i != cur_i) {
+      stage_compact<sizeof(TC) == 4>(reci, tyi, cnti, asci, selfi, zri, zci, set1, i, ni, amax, 0);
+      cur_i = i;
+    }
+    for (int jj = 0; jj < J; ++jj)
+      stage_compact<sizeof(TC) == 4>(recj, tyj, cntj, ascj, selfj, zrj, zcj, set2,
+                                     ta.members[tl.off + jj], s, amax, jj * s);
+    __syncthreads();
+
+    double part[kMaxSig];
+#pragma unroll
+    for (int k = 0; k < kMaxSig; ++k) part[k] = 0.0;
+    const int P = ni * s;
+    const int total = J * P;
+    for (int t = threadIdx.x; t < total; t += kPairThreads) {
+      const int jj = t / P, rem = t - jj * P;
+      const int a = rem % ni, b = rem / ni;
+      const int bj = jj * s + b;  // staged atom index of b
+      const int j = ta.members[tl.off + jj];
+      NbView x{reci + a * ni, tyi + a * ni, set1.rc + ((int64_t)i * amax + a) * amax};
+      NbView y{recj + bj * s, tyj + bj * s, set2.rc + ((int64_t)j * amax + b) * amax};
+      const TC S = generic ? arad_distl2<TC, true>(x, cnti[a], asci[a] != 0, y, cntj[bj],
+                                                   ascj[bj] != 0, inv_w, maxg, ttab, p.rw2, p.cw2)
+                           : arad_distl2<TC, false>(x, cnti[a], asci[a] != 0, y, cntj[bj],
+                                                    ascj[bj] != 0, inv_w, maxg, ttab, p.rw2, p.cw2);
+      const double T = stoch(zri[a] - zrj[bj], zci[a] - zcj[bj], p.rw2, p.cw2);
+      double D = selfi[a] + selfj[bj] - 2.0 * (double)S * T;  // farad_kernels.f90:590-592
+      if (fabs(D) < p.eps) D = 0.0;                            // :594
+      if (ta.mode == FMLH_ARAD_KERNEL) {
+        part[0] += exp(ta.inv_s2[0] * D);  // farad_kernels.f90:211
+      } else {
+#pragma unroll
+        for (int k = 0; k < kMaxSig; ++k)
+          if (k < nsig) part[k] += exp(D * ta.inv_s2[k]);  // farad_kernels.f90:603
+      }
+    }
+    __syncthreads();  // the J stage becomes the reduction buffer
+#pragma unroll
+    for (int k = 0; k < kMaxSig; ++k)
+      if (k < nsig) red[k * kPairThreads + threadIdx.x] = part[k];
+    __syncthreads();
+    if ((int)threadIdx.x < J * nsig) {
+      const int jj = threadIdx.x / nsig, k = threadIdx.x % nsig;
+      const int l0 = (J == 1) ? 0 : jj * P;
+      const int l1 = (J == 1) ? min(P, kPairThreads) : (jj + 1) * P;
+      double v = 0.0;
+      for (int l = l0; l < l1; ++l) v += red[k * kPairThreads + l];
+      const int j = ta.members[tl.off + jj];
+      if (ta.mode == FMLH_ARAD_KERNEL) {
+        ta.out[(int64_t)i + (int64_t)ta.nm1 * j] = v;
+        ta.out[(int64_t)j + (int64_t)ta.nm1 * i] = v;
+      } else {
+        ta.out[k + (int64_t)ta.nsig * ((int64_t)i + (int64_t)ta.nm1 * j)] = v;
+        if (ta.triangular) ta.out[k + (int64_t)ta.nsig * ((int64_t)j + (int64_t)ta.nm1 * i)] = v;
+      }
+    }
+  }
+}
+
+__global__ void k_zero_counter(int32_t *c) { *c = 0; }
+
 // Per-atom forms (farad_kernels.f90:757-1030): one thread per atom pair on
 // a 16 x 16 tile; operands come from the prepared global arrays.
 template <typename TC>
@@ -661,6 +838,120 @@ size_t pair_lds_bytes(int amax, bool stage) {
 
 constexpr size_t kLdsLimit = 65536;
 
+// ---- tile planning (host) --------------------------------------------
+
+struct TilePlan {
+  std::vector<Tile> tiles;
+  std::vector<int32_t> members;  // set-2 indices grouped by atom count
+  int max_ni = 0, max_j_recs = 0, max_j_atoms = 0;
+};
+
+// lanes: atom pairs one tile may hold.  j_recs_cap: LDS record capacity of
+// the J stage.  Tiles are emitted per molecule i, size classes largest
+// first, each class member run chunked into groups of J.
+void plan_tiles(const int32_t *n1h, int nm1, const int32_t *n2h, int nm2, int amax, bool tri,
+                int nsig, int lanes, int j_recs_cap, TilePlan &P) {
+  std::vector<int> cls_off(amax + 2, 0);
+  for (int j = 0; j < nm2; ++j) cls_off[std::min(std::max(n2h[j], 0), amax) + 1]++;
+  for (int s = 0; s <= amax; ++s) cls_off[s + 1] += cls_off[s];
+  P.members.assign(nm2, 0);
+  {
+    std::vector<int> fill(cls_off.begin(), cls_off.end() - 1);
+    for (int j = 0; j < nm2; ++j) P.members[fill[std::min(std::max(n2h[j], 0), amax)]++] = j;
+  }
+  P.tiles.clear();
+  const int jmax_red = std::max(1, kPairThreads / std::max(nsig, 1));
+  for (int i = 0; i < nm1; ++i) {
+    const int ni = std::min(std::max(n1h[i], 0), amax);
+    P.max_ni = std::max(P.max_ni, ni);
+    for (int s = amax; s >= 0; --s) {
+      int lo = cls_off[s];
+      const int hi = cls_off[s + 1];
+      if (tri) lo = (int)(std::lower_bound(P.members.begin() + lo, P.members.begin() + hi, i) -
+                          P.members.begin());
+      if (lo >= hi) continue;
+      const int pairs = ni * s;
+      int J = pairs > 0 ? std::max(1, lanes / pairs) : jmax_red;
+      if (s > 0) J = std::min(J, std::max(1, j_recs_cap / (s * s)));
+      J = std::min(J, jmax_red);
+      for (int o = lo; o < hi; o += J) {
+        const int c = std::min(J, hi - o);
+        P.tiles.push_back(Tile{i, o, c, s});
+        if (pairs > 0) {
+          P.max_j_recs = std::max(P.max_j_recs, c * s * s);
+          P.max_j_atoms = std::max(P.max_j_atoms, c * s);
+        }
+      }
+    }
+  }
+}
+
+struct TileLds {
+  int i_recs, j_recs, j_atoms;
+  size_t bytes;
+};
+
+TileLds tile_lds(int amax, int max_ni, int j_recs, int j_atoms, int nsig) {
+  TileLds L;
+  L.i_recs = std::max(max_ni * max_ni, 1);
+  L.j_atoms = std::max(j_atoms, 1);
+  // the J stage doubles as the reduction buffer (nsig x 256 doubles)
+  const size_t red = sizeof(double) * kPairThreads * std::max(nsig, 1);
+  size_t jb = sizeof(Rec) * (size_t)std::max(j_recs, 1) + 32 * (size_t)L.j_atoms +
+              sizeof(uint16_t) * (((size_t)std::max(j_recs, 1) + 7) & ~size_t(7));
+  int jr = std::max(j_recs, 1);
+  while (jb < red) {  // grow the record area until the stage covers the buffer
+    jr += 64;
+    jb = sizeof(Rec) * (size_t)jr + 32 * (size_t)L.j_atoms + sizeof(uint16_t) * (((size_t)jr + 7) & ~size_t(7));
+  }
+  L.j_recs = jr;
+  L.bytes = sizeof(double) * kMaxTypes * kMaxTypes + 16 + sizeof(Rec) * (size_t)L.i_recs +
+            32 * (size_t)amax + sizeof(uint16_t) * (((size_t)L.i_recs + 7) & ~size_t(7)) + jb;
+  return L;
+}
+
+// Pinned staging of the plan for the asynchronous upload; one upload is in
+// flight at a time (the next call waits on the event of the previous one).
+std::mutex g_plan_mu;
+void *g_plan_host = nullptr;
+size_t g_plan_cap = 0;
+hipEvent_t g_plan_done = nullptr;
+
+int upload_plan(const TilePlan &P, Tile *dtiles, int32_t *dmembers, hipStream_t st) {
+  std::lock_guard<std::mutex> lk(g_plan_mu);
+  const size_t tb = sizeof(Tile) * P.tiles.size(), mb = sizeof(int32_t) * P.members.size();
+  if (g_plan_done) {
+    hipError_t e = hipEventSynchronize(g_plan_done);
+    if (e != hipSuccess) return record_error(e, "hipEventSynchronize(plan)");
+  } else {
+    hipError_t e = hipEventCreateWithFlags(&g_plan_done, hipEventDisableTiming);
+    if (e != hipSuccess) return record_error(e, "hipEventCreate(plan)");
+  }
+  if (tb + mb > g_plan_cap) {
+    if (g_plan_host) hipHostFree(g_plan_host);
+    g_plan_cap = (tb + mb) * 2 + 4096;
+    hipError_t e = hipHostMalloc(&g_plan_host, g_plan_cap, hipHostMallocDefault);
+    if (e != hipSuccess) {
+      g_plan_host = nullptr;
+      g_plan_cap = 0;
+      return record_error(e, "hipHostMalloc(plan)");
+    }
+  }
+  char *h = static_cast<char *>(g_plan_host);
+  if (tb) memcpy(h, P.tiles.data(), tb);
+  if (mb) memcpy(h + tb, P.members.data(), mb);
+  hipError_t e = hipSuccess;
+  if (tb) e = hipMemcpyAsync(dtiles, h, tb, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess && mb) e = hipMemcpyAsync(dmembers, h + tb, mb, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipEventRecord(g_plan_done, st);
+  if (e != hipSuccess) return record_error(e, "hipMemcpyAsync(plan)");
+  return 0;
+}
+
+int64_t max_tiles(int nm1, int nm2, bool tri) {
+  return tri ? (int64_t)nm1 * (nm1 + 1) / 2 + nm1 : (int64_t)nm1 * nm2 + nm1;
+}
+
 // Preparation shared by both entry points: control block, neighbour
 // records, type ids, self overlaps.
 int prepare_sets(const double *q1, const double *zt1, const int32_t *n1, int64_t ns1,
@@ -717,6 +1008,9 @@ size_t fmlh_arad_ws_bytes(int nm1, int nm2, int amax, int mode) {
   const bool one_set = (mode == FMLH_ARAD_SYMMETRIC || mode == FMLH_ARAD_KERNEL);
   size_t b = WsCarver::need(sizeof(Ctl)) + set_bytes((int64_t)nm1 * amax, amax);
   if (!one_set) b += set_bytes((int64_t)nm2 * amax, amax);
+  if (mode != FMLH_ARAD_DISTANCE)  // tile plan: tiles, class member list, counter
+    b += WsCarver::need(sizeof(Tile) * (size_t)max_tiles(nm1, one_set ? nm1 : nm2, one_set)) +
+         WsCarver::need(sizeof(int32_t) * (size_t)(one_set ? nm1 : nm2)) + WsCarver::need(64);
   return b;
 }
 
@@ -729,7 +1023,8 @@ int fmlh_arad_kernels(int mode, const double *q1, const double *zt1, const int32
                       const double *q2, const double *zt2, const int32_t *n2, int nm2, int amax,
                       const double *sigmas_host, int nsig, double width, double cut,
                       double r_width, double c_width, int precision, double *out, void *ws,
-                      size_t ws_bytes, void *stream) {
+                      size_t ws_bytes, const int32_t *n1_host, const int32_t *n2_host,
+                      void *stream) {
   FMLH_ARG_CHECK(mode == FMLH_ARAD_GLOBAL || mode == FMLH_ARAD_SYMMETRIC ||
                      mode == FMLH_ARAD_KERNEL || mode == FMLH_ARAD_DISTANCE,
                  1);
@@ -778,10 +1073,76 @@ int fmlh_arad_kernels(int mode, const double *q1, const double *zt1, const int32
   pa.n2 = one_set ? n1 : n2;
   pa.out = out;
 
+  const int64_t cap = (int64_t)device_cus() * 16;
+
+  // Molecular kernel modes: size-sorted tiles when the molecules fit the LDS
+  // stages (every amax up to 29 does).
+  if (mode != FMLH_ARAD_DISTANCE && amax <= 29) {
+    std::vector<int32_t> h1, h2;
+    const int32_t *n1h = n1_host, *n2h = one_set ? n1_host : n2_host;
+    if (!n1h || !n2h) {  // fetch the atom counts (synchronises the stream)
+      h1.resize(nm1);
+      hipError_t e = hipMemcpyAsync(h1.data(), n1, sizeof(int32_t) * nm1, hipMemcpyDeviceToHost, st);
+      if (e == hipSuccess && !one_set) {
+        h2.resize(nm2);
+        e = hipMemcpyAsync(h2.data(), n2, sizeof(int32_t) * nm2, hipMemcpyDeviceToHost, st);
+      }
+      if (e == hipSuccess) e = hipStreamSynchronize(st);
+      if (e != hipSuccess) return record_error(e, "atom counts to host");
+      n1h = h1.data();
+      n2h = one_set ? h1.data() : h2.data();
+    }
+    const int nsig_eff = (mode == FMLH_ARAD_KERNEL) ? 1 : nsig;
+    TilePlan plan;
+    plan_tiles(n1h, nm1, n2h, nm2, amax, one_set, nsig_eff, kPairThreads, amax * amax, plan);
+    TileLds L = tile_lds(amax, plan.max_ni, plan.max_j_recs, plan.max_j_atoms, nsig_eff);
+    if (L.bytes <= kLdsLimit) {
+      WsCarver w(ws, ws_bytes);
+      w.take<Ctl>(1);
+      NbSet d1, d2;
+      carve_set(w, (int64_t)nm1 * amax, amax, d1);
+      if (!one_set) carve_set(w, (int64_t)nm2 * amax, amax, d2);
+      Tile *dt = w.take<Tile>((size_t)max_tiles(nm1, one_set ? nm1 : nm2, one_set));
+      int32_t *dm = w.take<int32_t>((size_t)(one_set ? nm1 : nm2));
+      int32_t *counter = w.take<int32_t>(16);
+      if (!dt || !dm || !counter) return -19;
+      rc = upload_plan(plan, dt, dm, st);
+      if (rc) return rc;
+      hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, st, counter);
+      FMLH_LAUNCH_CHECK("k_zero_counter");
+      TileArgs ta;
+      ta.mode = mode;
+      ta.nm1 = nm1;
+      ta.amax = amax;
+      ta.nsig = pa.nsig;
+      ta.triangular = pa.triangular;
+      ta.ntiles = (int)plan.tiles.size();
+      ta.stage_i_recs = L.i_recs;
+      ta.stage_j_recs = L.j_recs;
+      ta.stage_j_atoms = L.j_atoms;
+      for (int k = 0; k < kMaxSig; ++k) ta.inv_s2[k] = pa.inv_s2[k];
+      ta.n1 = n1;
+      ta.tiles = dt;
+      ta.members = dm;
+      ta.counter = counter;
+      ta.out = out;
+      const int grid = (int)std::min<int64_t>(std::max<int64_t>(ta.ntiles, 1), cap);
+      timing_begin(st);
+      if (precision)
+        hipLaunchKernelGGL(k_arad_tiles<float>, dim3(grid), dim3(kPairThreads), L.bytes, st, s1, s2,
+                           ta, p, ctl);
+      else
+        hipLaunchKernelGGL(k_arad_tiles<double>, dim3(grid), dim3(kPairThreads), L.bytes, st, s1,
+                           s2, ta, p, ctl);
+      timing_end(st);
+      FMLH_LAUNCH_CHECK("k_arad_tiles");
+      return 0;
+    }
+  }
+
   const bool stage = pair_lds_bytes(amax, true) <= kLdsLimit;
   const size_t lds = pair_lds_bytes(amax, stage);
   if (lds > kLdsLimit) return -10;
-  const int64_t cap = (int64_t)device_cus() * 16;
   const int grid = (int)(pa.npairs < cap ? pa.npairs : cap);
   timing_begin(st);
   if (precision) {

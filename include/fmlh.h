@@ -128,7 +128,10 @@ int fmlh_laplacian_kernel_f32(const float *a, int64_t lda, int na,
 
 /* precision: 0 = FP64 throughout (the plain form); 1 = FP32 inner overlap
  * (exp, products and accumulation in float), FP64 self terms, distance
- * combination and kernel sums. */
+ * combination and kernel sums.
+ * n1_host / n2_host: optional host copies of n1 / n2.  The kernel modes plan
+ * size-sorted molecule-pair tiles from the atom counts on the host; without
+ * the host copies the call fetches them, which synchronises the stream. */
 size_t fmlh_arad_ws_bytes(int nm1, int nm2, int amax, int mode);
 int fmlh_arad_kernels(int mode, const double *q1, const double *zt1,
                       const int32_t *n1, int nm1, const double *q2,
@@ -136,6 +139,7 @@ int fmlh_arad_kernels(int mode, const double *q1, const double *zt1,
                       const double *sigmas_host, int nsig, double width,
                       double cut, double r_width, double c_width,
                       int precision, double *out, void *ws, size_t ws_bytes,
+                      const int32_t *n1_host, const int32_t *n2_host,
                       void *stream);
 
 /* Atomic (per-atom) forms.  q (na, 5, amax), zt (na, 2), n (na) int32 =

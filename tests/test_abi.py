@@ -48,7 +48,7 @@ def test_version_and_errors_without_gpu():
     assert lib.fmlh_gaussian_kernel(None, 0, 5, None, 0, 5, 3, 1.0, None, 5, None) == -2
     assert lib.fmlh_coulomb_sorted(None, None, None, 1, 0, None, None, None, None, None) == -5
     assert lib.fmlh_arad_kernels(9, None, None, None, 1, None, None, None, 1, 5, None, 1, 0.2, 5.0,
-                                 1.0, 0.5, 0, None, None, 0, None) == -1
+                                 1.0, 0.5, 0, None, None, 0, None, None, None) == -1
     # empty problems are no-ops
     assert lib.fmlh_gaussian_kernel(None, 3, 0, None, 3, 0, 3, 1.0, None, 1, None) == 0
     assert lib.fmlh_arad_ws_bytes(10, 10, 23, 0) > 0
