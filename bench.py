@@ -36,9 +36,8 @@ PEAK_FP64_TFLOPS = 78.6            # vector and matrix, FMA = 2 flop
 PEAK_FP32_TFLOPS = 157.3
 PEAK_FP64_VALU_TOPS = 39.3         # FP64 VALU lane-instructions/s (256 CU x 64 lanes x 2.4 GHz)
 
-# FP64 VALU instructions the gfx950 code of k_arad_pairs<double> executes per
-# inner (m1, m2) evaluation in its band walk (exp included), counted in the
-# kernel's own ISA ("ARAD roofline").
+# Algorithmic FP64 operations per inner (m1, m2) evaluation of the overlap
+# term: 13 arithmetic operations plus the exponential counted as 20.
 ARAD_OPS_PER_EVAL = 33
 
 
@@ -50,7 +49,7 @@ def parse():
     ap.add_argument("--workload", default="arad",
                     choices=["arad", "krr", "gaussian64", "gaussian32", "gaussianbf16x3",
                              "laplacian64", "laplacian32", "cholesky64", "cholesky32",
-                             "krr36k_fp64", "krr36k_mixed", "serving"])
+                             "krr36k_fp64", "krr36k_mixed", "serving", "arad4096", "arad4096_fp32"])
     ap.add_argument("--duration", type=float, default=5.0, help="serving run length in seconds")
     ap.add_argument("--n", type=int, default=0, help="problem size override for the 36k workloads")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -84,7 +83,28 @@ def arad_inner_evaluations(X, n, cut, width):
     return int(np.sum(hi - lo))
 
 
-def arad_workload(args):
+def arad_inner_evaluations_xy(X1, n1, X2, n2, cut, width):
+    """As arad_inner_evaluations for two different descriptor sets."""
+    def dists(X, n):
+        ds = []
+        for i in range(X.shape[0]):
+            for a in range(n[i]):
+                d = X[i, a, 0, :n[i]]
+                c = np.argmax(d > cut) if np.any(d > cut) else n[i]
+                ds.append(d[:c])
+        return np.sort(np.concatenate(ds))
+    u, v = dists(X1, n1), dists(X2, n2)
+    w = 8.0 * width
+    lo = np.searchsorted(v, u - w, side="right")
+    hi = np.searchsorted(v, u + w, side="left")
+    return int(np.sum(hi - lo))
+
+
+def arad_workload(args, big=False, precision=0):
+    """big=False: the 500 x 500 global kernel (sigmas 0.1, 1, 10, amax 23).
+    big=True: 4096 x 4096 molecules with 3-29 atoms from a truncated geometric
+    distribution (mean ~14; set 1 seed 0, set 2 seed 1), amax 29, sigmas
+    {1, 4}; precision 1 = FP32 inner overlap."""
     import torch
     import fml_amd._native as nat
     from fml_amd import synth
@@ -92,37 +112,49 @@ def arad_workload(args):
     from fml_amd.device import empty, to_device, workspace
     from fml_amd.kernels.kernels import arad_types
 
-    nm, amax = 500, 23
-    sig = [0.1, 1.0, 10.0]
     width, cut, rw, cw = 0.2, 5.0, 1.0, 0.5
-    ms = synth.generate(nm, seed=0, nmin=5, nmax=23)
+    if big:
+        nm, amax, sig = args.n or 4096, 29, [1.0, 4.0]
+        ms1 = synth.generate(nm, seed=0, nmin=3, nmax=29, dist="geometric")
+        ms2 = synth.generate(nm, seed=1, nmin=3, nmax=29, dist="geometric")
+    else:
+        nm, amax, sig = 500, 23, [0.1, 1.0, 10.0]
+        ms1 = ms2 = synth.generate(nm, seed=0, nmin=5, nmax=23)
     ar = ARAD(maxMolSize=amax, maxAts=amax, cut=5.0)
-    X = np.array([ar.describe(ms.molecule(m)[1], ms.molecule(m)[0]) for m in range(nm)])
-    Z = [ms.molecule(m)[0] for m in range(nm)]
-    z, n = arad_types(Z, amax)
-    evals = arad_inner_evaluations(X, n, cut, width)
+    X1 = np.array([ar.describe(ms1.molecule(m)[1], ms1.molecule(m)[0]) for m in range(nm)])
+    X2 = X1 if ms2 is ms1 else np.array([ar.describe(ms2.molecule(m)[1], ms2.molecule(m)[0])
+                                         for m in range(nm)])
+    z1, n1 = arad_types([ms1.molecule(m)[0] for m in range(nm)], amax)
+    z2, n2 = (z1, n1) if ms2 is ms1 else arad_types([ms2.molecule(m)[0] for m in range(nm)], amax)
+    evals = arad_inner_evaluations_xy(X1, n1, X2, n2, cut, width)
 
     lib = nat.load()
-    dq, dz, dn = to_device(X), to_device(z), to_device(n, np.int32)
+    dq1, dz1, dn1 = to_device(X1), to_device(z1), to_device(n1, np.int32)
+    dq2, dz2, dn2 = to_device(X2), to_device(z2), to_device(n2, np.int32)
     out = empty((nm, nm, len(sig)), np.float64)
     wsb = lib.fmlh_arad_ws_bytes(nm, nm, amax, nat.ARAD_GLOBAL)
     ws = workspace(wsb)
     sarr = (ctypes.c_double * len(sig))(*sig)
-    nh, nhp = nat.host_i32(n)
+    h1, hp1 = nat.host_i32(n1)
+    h2, hp2 = nat.host_i32(n2)
 
     def step():
-        nat.call("fmlh_arad_kernels", nat.ARAD_GLOBAL, nat.ptr(dq), nat.ptr(dz), nat.ptr(dn), nm,
-                 nat.ptr(dq), nat.ptr(dz), nat.ptr(dn), nm, amax, sarr, len(sig), width, cut, rw, cw,
-                 0, nat.ptr(out), nat.ptr(ws), wsb, nhp, nhp, nat.stream_ptr())
+        nat.call("fmlh_arad_kernels", nat.ARAD_GLOBAL, nat.ptr(dq1), nat.ptr(dz1), nat.ptr(dn1), nm,
+                 nat.ptr(dq2), nat.ptr(dz2), nat.ptr(dn2), nm, amax, sarr, len(sig), width, cut, rw, cw,
+                 precision, nat.ptr(out), nat.ptr(ws), wsb, hp1, hp2, nat.stream_ptr())
 
     units_per_step = len(sig) * nm * nm
+    if big:
+        desc = (f"global ARAD kernel, {nm}x{nm} molecules (3-29 atoms, truncated geometric, seeds 0/1), "
+                f"descriptors ({nm},29,5,29), sigmas {{1,4}}, {'FP32' if precision else 'FP64'} overlap")
+    else:
+        desc = ("global ARAD kernel, 500x500 molecules (5-23 atoms, seed 0), descriptors (500,23,5,23), "
+                "sigmas {0.1,1,10}, width 0.2, cut 5.0, r_width 1.0, c_width 0.5")
     info = {
         "metric": "kernel entries/s",
         "unit": "entries/s",
-        "dtype": "f64",
-        "config": {"workload": "global ARAD kernel, 500x500 molecules (5-23 atoms, seed 0), "
-                               "descriptors (500,23,5,23), sigmas {0.1,1,10}, width 0.2, cut 5.0, "
-                               "r_width 1.0, c_width 0.5", "molecules": nm, "amax": amax,
+        "dtype": "f32" if precision else "f64",
+        "config": {"workload": desc, "molecules": nm, "amax": amax,
                    "sigmas": sig, "inner_evaluations_per_step": evals},
         "kernel": "k_arad_tiles",
     }
@@ -141,7 +173,7 @@ def arad_workload(args):
         kind = "reference" if oracle.ref_available() else "port"
         fn = oracle.ref_kernels_arad if kind == "reference" else oracle.kernels_arad
         t0 = time.perf_counter()
-        kc = fn(X[:rows], X, z[:rows], z, n[:rows], n, sig, width, cut, rw, cw)
+        kc = fn(X1[:rows], X2, z1[:rows], z2, n1[:rows], n2, sig, width, cut, rw, cw)
         dt = time.perf_counter() - t0
         g = torch.stack([out[:, :rows, k] for k in range(len(sig))]).cpu().numpy()  # (s, nm, rows)
         g = np.transpose(g, (0, 2, 1))
@@ -441,6 +473,9 @@ def main():
     wl = args.workload
     if wl == "arad":
         info, step, units, roofline, cpu_baseline, finish = arad_workload(args)
+    elif wl.startswith("arad4096"):
+        info, step, units, roofline, cpu_baseline, finish = arad_workload(
+            args, big=True, precision=1 if wl.endswith("fp32") else 0)
     elif wl == "krr":
         info, step, units, roofline, cpu_baseline, finish = krr_workload(args)
     elif wl.startswith("krr36k"):

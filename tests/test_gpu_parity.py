@@ -132,6 +132,27 @@ def test_arad_global(gpu, arad_set):
     assert _rel(k[2], ko[2]) < 1e-12
 
 
+def test_arad_tiles_uneven_sizes(gpu):
+    """Size-sorted tiles: molecules of 1-29 atoms (truncated geometric), amax
+    29, a lone 1-atom molecule, general, symmetric and single-sigma forms."""
+    from fml_amd.kernels import (get_atomic_kernels_arad, get_atomic_symmetric_kernels_arad,
+                                 fget_kernel_arad)
+    ms = synth.generate(44, seed=21, nmin=1, nmax=29, dist="geometric")
+    ar = ARAD(maxMolSize=29, maxAts=29)
+    X = np.array([ar.describe(ms.molecule(m)[1], ms.molecule(m)[0]) for m in range(ms.nmol)])
+    Z = [ms.molecule(m)[0] for m in range(ms.nmol)]
+    z, n = arad_types(Z, 29)
+    assert n.min() <= 2 and n.max() >= 20
+    sig = [1.0, 4.0]
+    k = get_atomic_kernels_arad(X[:30], X[30:], Z[:30], Z[30:], sig, cut_distance=5.0)
+    ko = oracle.kernels_arad(X[:30], X[30:], z[:30], z[30:], n[:30], n[30:], sig, 0.2, 5.0)
+    assert _rel(k, ko) < 1e-12
+    ks = get_atomic_symmetric_kernels_arad(X, Z, sig)
+    assert _rel(ks, oracle.symmetric_kernels_arad(X, z, n, sig)) < 1e-12
+    k1 = fget_kernel_arad(X, z, n, 1.5)
+    assert _rel(k1, oracle.kernel_arad(X, z, n, 1.5)) < 1e-12
+
+
 def test_arad_symmetric_and_kernel(gpu, arad_set):
     from fml_amd.kernels import get_atomic_symmetric_kernels_arad, fget_kernel_arad
     X, Z, z, n = arad_set
