@@ -49,7 +49,7 @@ def parse():
     ap.add_argument("--workload", default="arad",
                     choices=["arad", "krr", "gaussian64", "gaussian32", "gaussianbf16x3",
                              "laplacian64", "laplacian32", "cholesky64", "cholesky32",
-                             "krr36k_fp64", "krr36k_mixed", "serving", "arad4096", "arad4096_fp32"])
+                             "krr36k_fp64", "krr36k_mixed", "serving", "arad4096", "arad4096_fp32", "copy"])
     ap.add_argument("--duration", type=float, default=5.0, help="serving run length in seconds")
     ap.add_argument("--n", type=int, default=0, help="problem size override for the 36k workloads")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -65,26 +65,12 @@ def host_threads():
         return os.cpu_count() or 1
 
 
-def arad_inner_evaluations(X, n, cut, width):
-    """Inner (m1, m2) evaluations of the X-vs-X global kernel: every ordered
-    pair of neighbour slots (u from an atom of set 1, v from an atom of set 2)
-    with (v-u)^2 < (8 width)^2, u, v <= cut.  Counted on the sorted multiset
-    of all neighbour distances (band edges to within one rounding)."""
-    ds = []
-    for i in range(X.shape[0]):
-        for a in range(n[i]):
-            d = X[i, a, 0, :n[i]]
-            c = np.argmax(d > cut) if np.any(d > cut) else n[i]
-            ds.append(d[:c])
-    u = np.sort(np.concatenate(ds))
-    w = 8.0 * width
-    lo = np.searchsorted(u, u - w, side="right")
-    hi = np.searchsorted(u, u + w, side="left")
-    return int(np.sum(hi - lo))
-
-
-def arad_inner_evaluations_xy(X1, n1, X2, n2, cut, width):
-    """As arad_inner_evaluations for two different descriptor sets."""
+def arad_inner_evaluations(X1, n1, X2, n2, cut, width):
+    """Inner (m1, m2) evaluations of the global kernel of set 1 against set 2:
+    every ordered pair of neighbour slots (u from an atom of set 1, v from an
+    atom of set 2) with (v-u)^2 < (8 width)^2, u, v <= cut.  Counted on the
+    sorted multisets of all neighbour distances (band edges to within one
+    rounding)."""
     def dists(X, n):
         ds = []
         for i in range(X.shape[0]):
@@ -93,7 +79,8 @@ def arad_inner_evaluations_xy(X1, n1, X2, n2, cut, width):
                 c = np.argmax(d > cut) if np.any(d > cut) else n[i]
                 ds.append(d[:c])
         return np.sort(np.concatenate(ds))
-    u, v = dists(X1, n1), dists(X2, n2)
+    u = dists(X1, n1)
+    v = u if X2 is X1 else dists(X2, n2)
     w = 8.0 * width
     lo = np.searchsorted(v, u - w, side="right")
     hi = np.searchsorted(v, u + w, side="left")
@@ -126,7 +113,7 @@ def arad_workload(args, big=False, precision=0):
                                          for m in range(nm)])
     z1, n1 = arad_types([ms1.molecule(m)[0] for m in range(nm)], amax)
     z2, n2 = (z1, n1) if ms2 is ms1 else arad_types([ms2.molecule(m)[0] for m in range(nm)], amax)
-    evals = arad_inner_evaluations_xy(X1, n1, X2, n2, cut, width)
+    evals = arad_inner_evaluations(X1, n1, X2, n2, cut, width)
 
     lib = nat.load()
     dq1, dz1, dn1 = to_device(X1), to_device(z1), to_device(n1, np.int32)
@@ -291,6 +278,29 @@ def dense36k_workload(args, kind):
                 "unit": "T VALU-ops/s", "frac": round(ach / PEAK_FP64_VALU_TOPS, 4), "traffic": None}
 
     return info, step, units, roofline, None, None
+
+
+def copy_workload(args):
+    """Plain device-to-device copy of 512 MiB (torch's elementwise copy
+    kernel): the known-size calibration for the HBM FETCH_SIZE / WRITE_SIZE
+    counters."""
+    import torch
+    nbytes = 512 << 20
+    src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+
+    def step():
+        dst.copy_(src)
+
+    info = {"metric": "copy bandwidth", "unit": "GB/s", "dtype": "f64", "kernel": "copy",
+            "config": {"workload": "device copy, 512 MiB read + 512 MiB written"}}
+
+    def roofline(t):
+        ach = 2 * nbytes / t / 1e9
+        return {"bound": "hbm", "achieved": round(ach, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": 2 * nbytes}
+
+    return info, step, 2 * nbytes / 1e9, roofline, None, None
 
 
 def cholesky_workload(args, kind):
@@ -482,6 +492,8 @@ def main():
         info, step, units, roofline, cpu_baseline, finish = krr36k_workload(args, wl.split("_")[1])
     elif wl.startswith("cholesky"):
         info, step, units, roofline, cpu_baseline, finish = cholesky_workload(args, wl)
+    elif wl == "copy":
+        info, step, units, roofline, cpu_baseline, finish = copy_workload(args)
     else:
         info, step, units, roofline, cpu_baseline, finish = dense36k_workload(args, wl)
     print(f"[bench] workload ready: {info['config']['workload']}", file=sys.stderr, flush=True)
@@ -507,7 +519,7 @@ def main():
     kcnt = ctypes.c_int(0)
     lib.fmlh_timing_collect(ctypes.byref(kms), ctypes.byref(kcnt))
     t_step = ms_total / 1e3 / args.steps
-    t_kernel = kms.value / 1e3 / max(kcnt.value, 1)
+    t_kernel = kms.value / 1e3 / kcnt.value if kcnt.value else t_step
     print(f"[bench] {args.steps} steps: {ms_total:.3f} ms (wall {wall * 1e3:.1f} ms), "
           f"dominant kernel avg {t_kernel * 1e3:.3f} ms over {kcnt.value}", file=sys.stderr, flush=True)
 

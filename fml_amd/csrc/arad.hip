@@ -275,6 +275,25 @@ __device__ TC arad_distl2(const NbView &x, int cnt1, bool asc1, const NbView &y,
                           double cw2) {
   TC acc = TC(0);
   // d * ((1 + a1 a2) + b1 b2), the quantity farad_kernels.f90:83 adds
+  // value of the term for record yr (index m2) of the second list
+  auto rec_value = [&](const Rec &yr, uint16_t ty2, int m2, TC d1, TC s1, TC a1, TC b1, int trow,
+                       double2 rc1) -> TC {
+    const TC dr = (TC)yr.d - d1;
+    const TC r2 = dr * dr;
+    TC v = texp<TC>(r2 * inv_w) * s1;
+    v = v * (TC)yr.s;
+    TC t;
+    if (kGeneric) {
+      const double2 rc2 = y.rc[m2];
+      t = (TC)stoch(rc1.x - rc2.x, rc1.y - rc2.y, rw2, cw2);
+    } else {
+      t = ttab[trow + ty2];
+    }
+    v = v * t;
+    TC ang = TC(1) + a1 * (TC)yr.a;
+    ang = ang + b1 * (TC)yr.b;
+    return v * ang;
+  };
   auto term_value = [&](int m2, TC d1, TC s1, TC a1, TC b1, int trow, double2 rc1) -> TC {
     const Rec yr = y.rec[m2];
     const TC dr = (TC)yr.d - d1;
@@ -317,15 +336,28 @@ __device__ TC arad_distl2(const NbView &x, int cnt1, bool asc1, const NbView &y,
       const int trow = kGeneric ? 0 : (int)x.ty[m1] * kMaxTypes;
       const double2 rc1 = kGeneric ? x.rc[m1] : make_double2(0.0, 0.0);
       // two independent terms per trip for instruction-level parallelism,
-      // added to acc one after the other in m2 order
+      // added to acc one after the other in m2 order; the next trip's two
+      // records are read before this trip's arithmetic (indices clamped to
+      // the list, the clamped reads are not used)
+      const TC s1 = (TC)xr.s, a1 = (TC)xr.a, b1 = (TC)xr.b;
+      const int last = cnt2 - 1;
+      Rec r0 = y.rec[lo], r1 = y.rec[min(lo + 1, last)];
+      uint16_t t0 = y.ty[lo], t1 = y.ty[min(lo + 1, last)];
       int m2 = lo;
       for (; m2 + 1 < hi; m2 += 2) {
-        const TC v0 = term_value(m2, d1, (TC)xr.s, (TC)xr.a, (TC)xr.b, trow, rc1);
-        const TC v1 = term_value(m2 + 1, d1, (TC)xr.s, (TC)xr.a, (TC)xr.b, trow, rc1);
+        const int p0 = min(m2 + 2, last), p1 = min(m2 + 3, last);
+        const Rec n0 = y.rec[p0], n1 = y.rec[p1];
+        const uint16_t u0 = y.ty[p0], u1 = y.ty[p1];
+        const TC v0 = rec_value(r0, t0, m2, d1, s1, a1, b1, trow, rc1);
+        const TC v1 = rec_value(r1, t1, m2 + 1, d1, s1, a1, b1, trow, rc1);
         acc = acc + v0;
         acc = acc + v1;
+        r0 = n0;
+        r1 = n1;
+        t0 = u0;
+        t1 = u1;
       }
-      if (m2 < hi) term(m2, d1, (TC)xr.s, (TC)xr.a, (TC)xr.b, trow, rc1);
+      if (m2 < hi) acc = acc + rec_value(r0, t0, m2, d1, s1, a1, b1, trow, rc1);
     }
   } else {
     for (int m1 = 0; m1 < cnt1; ++m1) {
@@ -628,10 +660,13 @@ __device__ void stage_compact(Rec *rec, uint16_t *ty, int *cnt, int *asc, double
   }
 }
 
-template <typename TC>
-__global__ __launch_bounds__(kPairThreads) void k_arad_tiles(NbSet set1, NbSet set2, TileArgs ta,
+// Instantiated per type path; the host launches both and the one that does
+// not match the set's path (decided on the device) exits at once.
+template <typename TC, bool kGeneric>
+__global__ __launch_bounds__(kPairThreads, 4) void k_arad_tiles(NbSet set1, NbSet set2, TileArgs ta,
                                                               AradParams p,
                                                               const Ctl *__restrict__ ctl) {
+  if ((ctl->generic != 0) != kGeneric) return;
   extern __shared__ __attribute__((aligned(16))) char smem[];
   char *ptr = smem;
   TC *ttab = (TC *)ptr;
@@ -671,7 +706,6 @@ __global__ __launch_bounds__(kPairThreads) void k_arad_tiles(NbSet set1, NbSet s
   double *red = (double *)jbase;
   load_ttab(ttab, ctl);
 
-  const bool generic = ctl->generic != 0;
   const int amax = ta.amax;
   const TC inv_w = (TC)p.inv_w, maxg = (TC)p.maxg;
   const int nsig = (ta.mode == FMLH_ARAD_KERNEL) ? 1 : ta.nsig;
@@ -706,10 +740,8 @@ __global__ __launch_bounds__(kPairThreads) void k_arad_tiles(NbSet set1, NbSet s
       const int j = ta.members[tl.off + jj];
       NbView x{reci + a * ni, tyi + a * ni, set1.rc + ((int64_t)i * amax + a) * amax};
       NbView y{recj + bj * s, tyj + bj * s, set2.rc + ((int64_t)j * amax + b) * amax};
-      const TC S = generic ? arad_distl2<TC, true>(x, cnti[a], asci[a] != 0, y, cntj[bj],
-                                                   ascj[bj] != 0, inv_w, maxg, ttab, p.rw2, p.cw2)
-                           : arad_distl2<TC, false>(x, cnti[a], asci[a] != 0, y, cntj[bj],
-                                                    ascj[bj] != 0, inv_w, maxg, ttab, p.rw2, p.cw2);
+      const TC S = arad_distl2<TC, kGeneric>(x, cnti[a], asci[a] != 0, y, cntj[bj], ascj[bj] != 0,
+                                             inv_w, maxg, ttab, p.rw2, p.cw2);
       const double T = stoch(zri[a] - zrj[bj], zci[a] - zcj[bj], p.rw2, p.cw2);
       double D = selfi[a] + selfj[bj] - 2.0 * (double)S * T;  // farad_kernels.f90:590-592
       if (fabs(D) < p.eps) D = 0.0;                            // :594
@@ -1128,12 +1160,17 @@ int fmlh_arad_kernels(int mode, const double *q1, const double *zt1, const int32
       ta.out = out;
       const int grid = (int)std::min<int64_t>(std::max<int64_t>(ta.ntiles, 1), cap);
       timing_begin(st);
-      if (precision)
-        hipLaunchKernelGGL(k_arad_tiles<float>, dim3(grid), dim3(kPairThreads), L.bytes, st, s1, s2,
-                           ta, p, ctl);
-      else
-        hipLaunchKernelGGL(k_arad_tiles<double>, dim3(grid), dim3(kPairThreads), L.bytes, st, s1,
-                           s2, ta, p, ctl);
+      if (precision) {
+        hipLaunchKernelGGL((k_arad_tiles<float, false>), dim3(grid), dim3(kPairThreads), L.bytes, st,
+                           s1, s2, ta, p, ctl);
+        hipLaunchKernelGGL((k_arad_tiles<float, true>), dim3(grid), dim3(kPairThreads), L.bytes, st,
+                           s1, s2, ta, p, ctl);
+      } else {
+        hipLaunchKernelGGL((k_arad_tiles<double, false>), dim3(grid), dim3(kPairThreads), L.bytes, st,
+                           s1, s2, ta, p, ctl);
+        hipLaunchKernelGGL((k_arad_tiles<double, true>), dim3(grid), dim3(kPairThreads), L.bytes, st,
+                           s1, s2, ta, p, ctl);
+      }
       timing_end(st);
       FMLH_LAUNCH_CHECK("k_arad_tiles");
       return 0;

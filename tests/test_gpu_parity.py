@@ -153,6 +153,26 @@ def test_arad_tiles_uneven_sizes(gpu):
     assert _rel(k1, oracle.kernel_arad(X, z, n, 1.5)) < 1e-12
 
 
+def test_arad_generic_type_path(gpu):
+    """More than 16 distinct element types among the neighbours: the kernels
+    take the raw-channel path (T evaluated per term, not from the type
+    table)."""
+    from fml_amd.kernels import get_atomic_kernels_arad, get_atomic_symmetric_kernels_arad
+    ms = synth.generate(24, seed=8, nmin=3, nmax=23)
+    rng = np.random.default_rng(8)
+    Z = [rng.integers(1, 41, size=len(ms.molecule(m)[0])) for m in range(ms.nmol)]
+    assert len(np.unique(np.concatenate(Z))) > 16
+    ar = ARAD(maxMolSize=23, maxAts=23)
+    X = np.array([ar.describe(ms.molecule(m)[1], Z[m]) for m in range(ms.nmol)])
+    z, n = arad_types(Z, 23)
+    sig = [1.0, 4.0]
+    k = get_atomic_kernels_arad(X[:14], X[14:], Z[:14], Z[14:], sig, cut_distance=5.0)
+    ko = oracle.kernels_arad(X[:14], X[14:], z[:14], z[14:], n[:14], n[14:], sig, 0.2, 5.0)
+    assert _rel(k, ko) < 1e-12
+    ks = get_atomic_symmetric_kernels_arad(X, Z, sig)
+    assert _rel(ks, oracle.symmetric_kernels_arad(X, z, n, sig)) < 1e-12
+
+
 def test_arad_symmetric_and_kernel(gpu, arad_set):
     from fml_amd.kernels import get_atomic_symmetric_kernels_arad, fget_kernel_arad
     X, Z, z, n = arad_set
