@@ -66,6 +66,7 @@ constexpr int kTypeBitmap = kTypeRows * kTypeCols;
 constexpr uint16_t kTypeNone = 0xFFFF;
 constexpr int kMaxSig = 8;
 constexpr int kPairThreads = 256;
+constexpr int kTileRounds = 4;  // atom-pair rounds per thread in a tile
 
 struct AradParams {
   double width, cut, rw, cw;
@@ -317,8 +318,13 @@ __device__ TC arad_distl2(const NbView &x, int cnt1, bool asc1, const NbView &y,
   };
   if (asc1 && asc2) {
     int lo = 0, hi = 0;
+    // the next m1's record and type are read one m1 ahead (index clamped)
+    Rec xr = x.rec[0];
+    uint16_t xt = x.ty[0];
     for (int m1 = 0; m1 < cnt1; ++m1) {
-      const Rec xr = x.rec[m1];
+      const int mn = min(m1 + 1, cnt1 - 1);
+      const Rec xnext = x.rec[mn];
+      const uint16_t xtnext = x.ty[mn];
       const TC d1 = (TC)xr.d;
       while (lo < cnt2) {
         const TC d2 = (TC)y.rec[lo].d;
@@ -332,8 +338,8 @@ __device__ TC arad_distl2(const NbView &x, int cnt1, bool asc1, const NbView &y,
         if (!(dr * dr < maxg)) break;
         ++hi;
       }
-      if (lo == hi) continue;
-      const int trow = kGeneric ? 0 : (int)x.ty[m1] * kMaxTypes;
+      if (lo < hi) {
+      const int trow = kGeneric ? 0 : (int)xt * kMaxTypes;
       const double2 rc1 = kGeneric ? x.rc[m1] : make_double2(0.0, 0.0);
       // two independent terms per trip for instruction-level parallelism,
       // added to acc one after the other in m2 order; the next trip's two
@@ -358,6 +364,9 @@ __device__ TC arad_distl2(const NbView &x, int cnt1, bool asc1, const NbView &y,
         t1 = u1;
       }
       if (m2 < hi) acc = acc + rec_value(r0, t0, m2, d1, s1, a1, b1, trow, rc1);
+      }
+      xr = xnext;
+      xt = xtnext;
     }
   } else {
     for (int m1 = 0; m1 < cnt1; ++m1) {
@@ -627,7 +636,6 @@ struct TileArgs {
   int mode;
   int nm1, amax, nsig, triangular;
   int ntiles;
-  int stage_i_recs;  // LDS capacity (records) of the molecule-i stage
   int stage_j_recs;  // LDS capacity (records) of the J-molecule stage
   int stage_j_atoms;
   double inv_s2[kMaxSig];
@@ -638,7 +646,7 @@ struct TileArgs {
   double *out;
 };
 
-template <bool kF32Self>
+template <bool kF32Self, bool kRecs>
 __device__ void stage_compact(Rec *rec, uint16_t *ty, int *cnt, int *asc, double *self,
                               double *zr, double *zc, const NbSet &set, int mol, int natoms,
                               int amax, int atom0) {
@@ -651,6 +659,7 @@ __device__ void stage_compact(Rec *rec, uint16_t *ty, int *cnt, int *asc, double
     zr[atom0 + a] = set.zr[sl];
     zc[atom0 + a] = set.zc[sl];
   }
+  if (!kRecs) return;
   const int total = natoms * natoms;
   for (int t = threadIdx.x; t < total; t += blockDim.x) {
     const int a = t / natoms, m = t % natoms;
@@ -673,9 +682,8 @@ __global__ __launch_bounds__(kPairThreads, 4) void k_arad_tiles(NbSet set1, NbSe
   ptr += sizeof(double) * kMaxTypes * kMaxTypes;
   int *tile_slot = (int *)ptr;
   ptr += 16;
-  // molecule-i stage
-  Rec *reci = (Rec *)ptr;
-  ptr += sizeof(Rec) * ta.stage_i_recs;
+  // molecule-i stage: per-atom values only; its neighbour records are read
+  // from the prepared global arrays (one record per m1, read ahead)
   double *selfi = (double *)ptr;
   ptr += sizeof(double) * ta.amax;
   double *zri = (double *)ptr;
@@ -686,8 +694,6 @@ __global__ __launch_bounds__(kPairThreads, 4) void k_arad_tiles(NbSet set1, NbSe
   ptr += sizeof(int) * ta.amax;
   int *asci = (int *)ptr;
   ptr += sizeof(int) * ta.amax;
-  uint16_t *tyi = (uint16_t *)ptr;
-  ptr += sizeof(uint16_t) * ((ta.stage_i_recs + 7) & ~7);
   // J-molecule stage (reused as the reduction buffer after compute)
   char *jbase = ptr;
   Rec *recj = (Rec *)ptr;
@@ -720,43 +726,52 @@ __global__ __launch_bounds__(kPairThreads, 4) void k_arad_tiles(NbSet set1, NbSe
     const int i = tl.i, J = tl.cnt, s = tl.s;
     const int ni = min(max(ta.n1[i], 0), amax);
     if (i != cur_i) {
-      stage_compact<sizeof(TC) == 4>(reci, tyi, cnti, asci, selfi, zri, zci, set1, i, ni, amax, 0);
+      stage_compact<sizeof(TC) == 4, false>(nullptr, nullptr, cnti, asci, selfi, zri, zci, set1, i,
+                                            ni, amax, 0);
       cur_i = i;
     }
     for (int jj = 0; jj < J; ++jj)
-      stage_compact<sizeof(TC) == 4>(recj, tyj, cntj, ascj, selfj, zrj, zcj, set2,
-                                     ta.members[tl.off + jj], s, amax, jj * s);
+      stage_compact<sizeof(TC) == 4, true>(recj, tyj, cntj, ascj, selfj, zrj, zcj, set2,
+                                           ta.members[tl.off + jj], s, amax, jj * s);
     __syncthreads();
 
-    double part[kMaxSig];
-#pragma unroll
-    for (int k = 0; k < kMaxSig; ++k) part[k] = 0.0;
+    // a thread's atom pairs (rounds of kPairThreads; at most kTileRounds,
+    // the host uses tiles only where amax^2 fits) keep their D values in
+    // registers until the per-sigma exponentials below
+    double D0 = 0.0, D1 = 0.0, D2 = 0.0, D3 = 0.0;
     const int P = ni * s;
     const int total = J * P;
-    for (int t = threadIdx.x; t < total; t += kPairThreads) {
+    int nr = 0;
+    for (int t = threadIdx.x; t < total; t += kPairThreads, ++nr) {
       const int jj = t / P, rem = t - jj * P;
       const int a = rem % ni, b = rem / ni;
       const int bj = jj * s + b;  // staged atom index of b
       const int j = ta.members[tl.off + jj];
-      NbView x{reci + a * ni, tyi + a * ni, set1.rc + ((int64_t)i * amax + a) * amax};
+      const int64_t xs = ((int64_t)i * amax + a) * amax;
+      NbView x{set1.rec + xs, set1.ty + xs, set1.rc + xs};
       NbView y{recj + bj * s, tyj + bj * s, set2.rc + ((int64_t)j * amax + b) * amax};
       const TC S = arad_distl2<TC, kGeneric>(x, cnti[a], asci[a] != 0, y, cntj[bj], ascj[bj] != 0,
                                              inv_w, maxg, ttab, p.rw2, p.cw2);
       const double T = stoch(zri[a] - zrj[bj], zci[a] - zcj[bj], p.rw2, p.cw2);
       double D = selfi[a] + selfj[bj] - 2.0 * (double)S * T;  // farad_kernels.f90:590-592
       if (fabs(D) < p.eps) D = 0.0;                            // :594
-      if (ta.mode == FMLH_ARAD_KERNEL) {
-        part[0] += exp(ta.inv_s2[0] * D);  // farad_kernels.f90:211
-      } else {
-#pragma unroll
-        for (int k = 0; k < kMaxSig; ++k)
-          if (k < nsig) part[k] += exp(D * ta.inv_s2[k]);  // farad_kernels.f90:603
-      }
+      D0 = nr == 0 ? D : D0;
+      D1 = nr == 1 ? D : D1;
+      D2 = nr == 2 ? D : D2;
+      D3 = nr == 3 ? D : D3;
     }
     __syncthreads();  // the J stage becomes the reduction buffer
-#pragma unroll
-    for (int k = 0; k < kMaxSig; ++k)
-      if (k < nsig) red[k * kPairThreads + threadIdx.x] = part[k];
+    // per sigma: the sum over this thread's pairs in round order
+    // (farad_kernels.f90:603; :211 for the single-sigma form)
+    for (int k = 0; k < nsig; ++k) {
+      const double is2 = ta.inv_s2[k];
+      double v = 0.0;
+      if (nr > 0) v += exp(D0 * is2);
+      if (nr > 1) v += exp(D1 * is2);
+      if (nr > 2) v += exp(D2 * is2);
+      if (nr > 3) v += exp(D3 * is2);
+      red[k * kPairThreads + threadIdx.x] = v;
+    }
     __syncthreads();
     if ((int)threadIdx.x < J * nsig) {
       const int jj = threadIdx.x / nsig, k = threadIdx.x % nsig;
@@ -919,13 +934,12 @@ void plan_tiles(const int32_t *n1h, int nm1, const int32_t *n2h, int nm2, int am
 }
 
 struct TileLds {
-  int i_recs, j_recs, j_atoms;
+  int j_recs, j_atoms;
   size_t bytes;
 };
 
 TileLds tile_lds(int amax, int max_ni, int j_recs, int j_atoms, int nsig) {
   TileLds L;
-  L.i_recs = std::max(max_ni * max_ni, 1);
   L.j_atoms = std::max(j_atoms, 1);
   // the J stage doubles as the reduction buffer (nsig x 256 doubles)
   const size_t red = sizeof(double) * kPairThreads * std::max(nsig, 1);
@@ -937,8 +951,7 @@ TileLds tile_lds(int amax, int max_ni, int j_recs, int j_atoms, int nsig) {
     jb = sizeof(Rec) * (size_t)jr + 32 * (size_t)L.j_atoms + sizeof(uint16_t) * (((size_t)jr + 7) & ~size_t(7));
   }
   L.j_recs = jr;
-  L.bytes = sizeof(double) * kMaxTypes * kMaxTypes + 16 + sizeof(Rec) * (size_t)L.i_recs +
-            32 * (size_t)amax + sizeof(uint16_t) * (((size_t)L.i_recs + 7) & ~size_t(7)) + jb;
+  L.bytes = sizeof(double) * kMaxTypes * kMaxTypes + 16 + 32 * (size_t)amax + jb;
   return L;
 }
 
@@ -1109,7 +1122,8 @@ int fmlh_arad_kernels(int mode, const double *q1, const double *zt1, const int32
 
   // Molecular kernel modes: size-sorted tiles when the molecules fit the LDS
   // stages (every amax up to 29 does).
-  if (mode != FMLH_ARAD_DISTANCE && amax <= 29) {
+  if (mode != FMLH_ARAD_DISTANCE && amax <= 29 &&
+      amax * amax <= kTileRounds * kPairThreads) {
     std::vector<int32_t> h1, h2;
     const int32_t *n1h = n1_host, *n2h = one_set ? n1_host : n2_host;
     if (!n1h || !n2h) {  // fetch the atom counts (synchronises the stream)
@@ -1149,7 +1163,6 @@ int fmlh_arad_kernels(int mode, const double *q1, const double *zt1, const int32
       ta.nsig = pa.nsig;
       ta.triangular = pa.triangular;
       ta.ntiles = (int)plan.tiles.size();
-      ta.stage_i_recs = L.i_recs;
       ta.stage_j_recs = L.j_recs;
       ta.stage_j_atoms = L.j_atoms;
       for (int k = 0; k < kMaxSig; ++k) ta.inv_s2[k] = pa.inv_s2[k];
