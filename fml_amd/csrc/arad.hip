@@ -39,11 +39,9 @@
 //
 // Inner exp: its argument (d2-d1)^2 * (-1/(4w^2)) lies in (-16, 0] whenever
 // the term is taken, because (8w)^2 * (-1/(4w^2)) = -16.  exp_neg() evaluates
-// exp there by Cody-Waite reduction x = k ln2 + r (two-part ln2,
-// |r| <= ln2/2) and the degree-13 Taylor polynomial of e^r (truncation below
-// 5e-18 relative), scaled by 2^k; it has no overflow/underflow/NaN handling
-// because none can occur on that domain.  Measured against exact exp over
-// (-16, 0]: at most 0.82 ulp.
+// exp there with a 32-entry 2^(j/32) table held in LDS and a degree-6
+// polynomial (at most 0.99 ulp on that domain); it has no
+// overflow/underflow/NaN handling because none can occur there.
 #include <math.h>
 #include <string.h>
 
@@ -111,28 +109,38 @@ __device__ __forceinline__ double stoch(double dr, double dc, double rw2, double
   return rw2 / (rw2 + dr * dr) * cw2 / (cw2 + dc * dc);
 }
 
-__device__ __forceinline__ double exp_neg(double x) {
-  const double inv_ln2 = 0x1.71547652b82fep+0;
-  const double ln2_hi = 0x1.62e42fee00000p-1;
-  const double ln2_lo = 0x1.a39ef35793c76p-33;
-  const double k = rint(x * inv_ln2);
-  double r = fma(-k, ln2_hi, x);
-  r = fma(-k, ln2_lo, r);
-  double p = 0x1.6124613a86d09p-33;      // 1/13!
-  p = fma(p, r, 0x1.1eed8eff8d898p-29);  // 1/12!
-  p = fma(p, r, 0x1.ae64567f544e4p-26);  // 1/11!
-  p = fma(p, r, 0x1.27e4fb7789f5cp-22);  // 1/10!
-  p = fma(p, r, 0x1.71de3a556c734p-19);  // 1/9!
-  p = fma(p, r, 0x1.a01a01a01a01ap-16);  // 1/8!
-  p = fma(p, r, 0x1.a01a01a01a01ap-13);  // 1/7!
-  p = fma(p, r, 0x1.6c16c16c16c17p-10);  // 1/6!
-  p = fma(p, r, 0x1.1111111111111p-7);   // 1/5!
-  p = fma(p, r, 0x1.5555555555555p-5);   // 1/4!
-  p = fma(p, r, 0x1.5555555555555p-3);   // 1/3!
-  p = fma(p, r, 0.5);
-  p = fma(p, r, 1.0);
-  p = fma(p, r, 1.0);
-  return ldexp(p, (int)k);
+// 2^(j/32), j = 0..31, correctly rounded.
+__constant__ double c_exp2_32[32] = {
+    0x1.0000000000000p+0, 0x1.059b0d3158574p+0, 0x1.0b5586cf9890fp+0, 0x1.11301d0125b51p+0,
+    0x1.172b83c7d517bp+0, 0x1.1d4873168b9aap+0, 0x1.2387a6e756238p+0, 0x1.29e9df51fdee1p+0,
+    0x1.306fe0a31b715p+0, 0x1.371a7373aa9cbp+0, 0x1.3dea64c123422p+0, 0x1.44e086061892dp+0,
+    0x1.4bfdad5362a27p+0, 0x1.5342b569d4f82p+0, 0x1.5ab07dd485429p+0, 0x1.6247eb03a5585p+0,
+    0x1.6a09e667f3bcdp+0, 0x1.71f75e8ec5f74p+0, 0x1.7a11473eb0187p+0, 0x1.82589994cce13p+0,
+    0x1.8ace5422aa0dbp+0, 0x1.93737b0cdc5e5p+0, 0x1.9c49182a3f090p+0, 0x1.a5503b23e255dp+0,
+    0x1.ae89f995ad3adp+0, 0x1.b7f76f2fb5e47p+0, 0x1.c199bdd85529cp+0, 0x1.cb720dcef9069p+0,
+    0x1.d5818dcfba487p+0, 0x1.dfc97337b9b5fp+0, 0x1.ea4afa2a490dap+0, 0x1.f50765b6e4540p+0,
+};
+
+// exp(x) for x in (-16, 0] with tab = c_exp2_32 (copied to LDS by the
+// caller).  x = (32 k + j) ln2/32 + r, |r| <= ln2/64, from a two-part
+// ln2/32 whose high part times k is exact; e^r = 1 + q(r) with q the
+// degree-6 Taylor polynomial minus 1 (truncation below 4e-18); the result
+// 2^k (t + t q), t = 2^(j/32), rounds once after the table value's own
+// rounding.  Measured against a 40-digit exp on 85,000 points of (-16, 0]:
+// at most 0.99 ulp.
+__device__ __forceinline__ double exp_neg(double x, const double *__restrict__ tab) {
+  const double k32 = rint(x * 0x1.71547652b82fep+5);  // x * 32/ln2
+  double r = fma(-k32, 0x1.62e42fee00000p-6, x);
+  r = fma(-k32, 0x1.a39ef35793c76p-38, r);
+  double q = fma(r, 0x1.6c16c16c16c17p-10, 0x1.1111111111111p-7);  // 1/6!, 1/5!
+  q = fma(q, r, 0x1.5555555555555p-5);                             // 1/4!
+  q = fma(q, r, 0x1.5555555555555p-3);                             // 1/3!
+  q = fma(q, r, 0.5);
+  q = fma(q, r, 1.0);
+  q = q * r;
+  const int k = (int)k32;
+  const double t = tab[k & 31];
+  return ldexp(fma(t, q, t), k >> 5);
 }
 
 // ---- preparation ------------------------------------------------------
@@ -262,26 +270,34 @@ struct NbView {
 };
 
 template <typename TC>
-__device__ __forceinline__ TC texp(TC x);
+__device__ __forceinline__ TC texp(TC x, const double *tab);
 template <>
-__device__ __forceinline__ double texp<double>(double x) { return exp_neg(x); }
+__device__ __forceinline__ double texp<double>(double x, const double *tab) {
+  return exp_neg(x, tab);
+}
 template <>
-__device__ __forceinline__ float texp<float>(float x) { return expf(x); }
+__device__ __forceinline__ float texp<float>(float x, const double *) { return expf(x); }
+
+__device__ __forceinline__ double tfma(double a, double b, double c) { return fma(a, b, c); }
+__device__ __forceinline__ float tfma(float a, float b, float c) { return fmaf(a, b, c); }
 
 // atomic_distl2 for atoms x (list 1) and y (list 2).  kGeneric evaluates T
 // from the raw channels; otherwise ttab is the compact type-pair table.
-template <typename TC, bool kGeneric>
+// etab: the 2^(j/32) table of exp_neg.  kPadY: list 2 may be read up to two
+// records past its end (a padded LDS stage), so the band loop's read-ahead
+// needs no clamp.
+template <typename TC, bool kGeneric, bool kPadY = false>
 __device__ TC arad_distl2(const NbView &x, int cnt1, bool asc1, const NbView &y, int cnt2,
-                          bool asc2, TC inv_w, TC maxg, const TC *__restrict__ ttab, double rw2,
-                          double cw2) {
+                          bool asc2, TC inv_w, TC maxg, const TC *__restrict__ ttab,
+                          const double *__restrict__ etab, double rw2, double cw2) {
   TC acc = TC(0);
-  // d * ((1 + a1 a2) + b1 b2), the quantity farad_kernels.f90:83 adds
-  // value of the term for record yr (index m2) of the second list
+  // d * ((1 + a1 a2) + b1 b2), the quantity farad_kernels.f90:83 adds (the
+  // angular factor with fused multiply-adds); record yr (index m2) of list 2
   auto rec_value = [&](const Rec &yr, uint16_t ty2, int m2, TC d1, TC s1, TC a1, TC b1, int trow,
                        double2 rc1) -> TC {
     const TC dr = (TC)yr.d - d1;
     const TC r2 = dr * dr;
-    TC v = texp<TC>(r2 * inv_w) * s1;
+    TC v = texp<TC>(r2 * inv_w, etab) * s1;
     v = v * (TC)yr.s;
     TC t;
     if (kGeneric) {
@@ -291,30 +307,11 @@ __device__ TC arad_distl2(const NbView &x, int cnt1, bool asc1, const NbView &y,
       t = ttab[trow + ty2];
     }
     v = v * t;
-    TC ang = TC(1) + a1 * (TC)yr.a;
-    ang = ang + b1 * (TC)yr.b;
-    return v * ang;
-  };
-  auto term_value = [&](int m2, TC d1, TC s1, TC a1, TC b1, int trow, double2 rc1) -> TC {
-    const Rec yr = y.rec[m2];
-    const TC dr = (TC)yr.d - d1;
-    const TC r2 = dr * dr;
-    TC v = texp<TC>(r2 * inv_w) * s1;
-    v = v * (TC)yr.s;
-    TC t;
-    if (kGeneric) {
-      const double2 rc2 = y.rc[m2];
-      t = (TC)stoch(rc1.x - rc2.x, rc1.y - rc2.y, rw2, cw2);
-    } else {
-      t = ttab[trow + y.ty[m2]];
-    }
-    v = v * t;
-    TC ang = TC(1) + a1 * (TC)yr.a;
-    ang = ang + b1 * (TC)yr.b;
+    const TC ang = tfma(b1, (TC)yr.b, tfma(a1, (TC)yr.a, TC(1)));
     return v * ang;
   };
   auto term = [&](int m2, TC d1, TC s1, TC a1, TC b1, int trow, double2 rc1) {
-    acc = acc + term_value(m2, d1, s1, a1, b1, trow, rc1);
+    acc = acc + rec_value(y.rec[m2], kGeneric ? uint16_t(0) : y.ty[m2], m2, d1, s1, a1, b1, trow, rc1);
   };
   if (asc1 && asc2) {
     int lo = 0, hi = 0;
@@ -346,12 +343,13 @@ __device__ TC arad_distl2(const NbView &x, int cnt1, bool asc1, const NbView &y,
       // records are read before this trip's arithmetic (indices clamped to
       // the list, the clamped reads are not used)
       const TC s1 = (TC)xr.s, a1 = (TC)xr.a, b1 = (TC)xr.b;
-      const int last = cnt2 - 1;
+      const int last = kPadY ? cnt2 + 1 : cnt2 - 1;
       Rec r0 = y.rec[lo], r1 = y.rec[min(lo + 1, last)];
       uint16_t t0 = y.ty[lo], t1 = y.ty[min(lo + 1, last)];
       int m2 = lo;
       for (; m2 + 1 < hi; m2 += 2) {
-        const int p0 = min(m2 + 2, last), p1 = min(m2 + 3, last);
+        const int p0 = kPadY ? m2 + 2 : min(m2 + 2, last);
+        const int p1 = kPadY ? m2 + 3 : min(m2 + 3, last);
         const Rec n0 = y.rec[p0], n1 = y.rec[p1];
         const uint16_t u0 = y.ty[p0], u1 = y.ty[p1];
         const TC v0 = rec_value(r0, t0, m2, d1, s1, a1, b1, trow, rc1);
@@ -388,17 +386,23 @@ __device__ __forceinline__ NbView global_view(const NbSet &s, int64_t slot, int 
   return NbView{s.rec + o, s.ty + o, s.rc + o};
 }
 
+// LDS copies of the type-pair table and of exp_neg's 2^(j/32) table.
 template <typename TC>
-__device__ void load_ttab(TC *ttab, const Ctl *ctl) {
+__device__ void load_tables(TC *ttab, double *etab, const Ctl *ctl) {
   for (int t = threadIdx.x; t < kMaxTypes * kMaxTypes; t += blockDim.x) ttab[t] = (TC)ctl->ttab[t];
+  for (int t = threadIdx.x; t < 32; t += blockDim.x) etab[t] = c_exp2_32[t];
 }
+
+// bytes of the two tables at the start of a kernel's dynamic LDS
+constexpr size_t kTabBytes = sizeof(double) * (kMaxTypes * kMaxTypes + 32);
 
 // Self overlaps, one thread per slot (global-memory operands).
 template <typename TC>
 __global__ __launch_bounds__(256) void k_arad_self(NbSet set, int nslots, int amax, AradParams p,
                                                    const Ctl *__restrict__ ctl) {
   __shared__ TC ttab[kMaxTypes * kMaxTypes];
-  load_ttab(ttab, ctl);
+  __shared__ double etab[32];
+  load_tables(ttab, etab, ctl);
   __syncthreads();
   const int slot = blockIdx.x * blockDim.x + threadIdx.x;
   if (slot >= nslots) return;
@@ -407,9 +411,11 @@ __global__ __launch_bounds__(256) void k_arad_self(NbSet set, int nslots, int am
   NbView v = global_view(set, slot, amax);
   TC r;
   if (ctl->generic)
-    r = arad_distl2<TC, true>(v, c, asc, v, c, asc, (TC)p.inv_w, (TC)p.maxg, ttab, p.rw2, p.cw2);
+    r = arad_distl2<TC, true>(v, c, asc, v, c, asc, (TC)p.inv_w, (TC)p.maxg, ttab, etab, p.rw2,
+                              p.cw2);
   else
-    r = arad_distl2<TC, false>(v, c, asc, v, c, asc, (TC)p.inv_w, (TC)p.maxg, ttab, p.rw2, p.cw2);
+    r = arad_distl2<TC, false>(v, c, asc, v, c, asc, (TC)p.inv_w, (TC)p.maxg, ttab, etab, p.rw2,
+                               p.cw2);
   if (sizeof(TC) == 8)
     set.self[slot] = (double)r;
   else
@@ -510,12 +516,13 @@ __global__ __launch_bounds__(kPairThreads) void k_arad_pairs(NbSet set1, NbSet s
   extern __shared__ __attribute__((aligned(16))) char smem[];
   char *ptr = smem;
   TC *ttab = (TC *)ptr;
-  ptr += sizeof(double) * kMaxTypes * kMaxTypes;
+  double *etab = (double *)(ptr + sizeof(double) * kMaxTypes * kMaxTypes);
+  ptr += kTabBytes;
   double *red = (double *)ptr;
   ptr += sizeof(double) * kMaxSig * (kPairThreads / 64);
   MolStage mi = carve_stage(ptr, pa.amax, kStage);
   MolStage mj = carve_stage(ptr, pa.amax, kStage);
-  load_ttab(ttab, ctl);
+  load_tables(ttab, etab, ctl);
 
   const bool generic = ctl->generic != 0;
   const int amax = pa.amax;
@@ -572,9 +579,11 @@ __global__ __launch_bounds__(kPairThreads) void k_arad_pairs(NbSet set1, NbSet s
         y = NbView{set2.rec + gy, set2.ty + gy, set2.rc + gy};
       }
       const TC S = generic ? arad_distl2<TC, true>(x, mi.cnt[a], mi.asc[a] != 0, y, mj.cnt[b],
-                                                   mj.asc[b] != 0, inv_w, maxg, ttab, p.rw2, p.cw2)
+                                                   mj.asc[b] != 0, inv_w, maxg, ttab, etab, p.rw2,
+                                                   p.cw2)
                            : arad_distl2<TC, false>(x, mi.cnt[a], mi.asc[a] != 0, y, mj.cnt[b],
-                                                    mj.asc[b] != 0, inv_w, maxg, ttab, p.rw2, p.cw2);
+                                                    mj.asc[b] != 0, inv_w, maxg, ttab, etab, p.rw2,
+                                                    p.cw2);
       // farad_kernels.f90:590-594
       const double T = stoch(mi.zr[a] - mj.zr[b], mi.zc[a] - mj.zc[b], p.rw2, p.cw2);
       double D = mi.self[a] + mj.self[b] - 2.0 * (double)S * T;
@@ -679,7 +688,8 @@ __global__ __launch_bounds__(kPairThreads, 4) void k_arad_tiles(NbSet set1, NbSe
   extern __shared__ __attribute__((aligned(16))) char smem[];
   char *ptr = smem;
   TC *ttab = (TC *)ptr;
-  ptr += sizeof(double) * kMaxTypes * kMaxTypes;
+  double *etab = (double *)(ptr + sizeof(double) * kMaxTypes * kMaxTypes);
+  ptr += kTabBytes;
   int *tile_slot = (int *)ptr;
   ptr += 16;
   // molecule-i stage: per-atom values only; its neighbour records are read
@@ -710,7 +720,7 @@ __global__ __launch_bounds__(kPairThreads, 4) void k_arad_tiles(NbSet set1, NbSe
   ptr += sizeof(int) * ta.stage_j_atoms;
   uint16_t *tyj = (uint16_t *)ptr;
   double *red = (double *)jbase;
-  load_ttab(ttab, ctl);
+  load_tables(ttab, etab, ctl);
 
   const int amax = ta.amax;
   const TC inv_w = (TC)p.inv_w, maxg = (TC)p.maxg;
@@ -733,6 +743,10 @@ __global__ __launch_bounds__(kPairThreads, 4) void k_arad_tiles(NbSet set1, NbSe
     for (int jj = 0; jj < J; ++jj)
       stage_compact<sizeof(TC) == 4, true>(recj, tyj, cntj, ascj, selfj, zrj, zcj, set2,
                                            ta.members[tl.off + jj], s, amax, jj * s);
+    if (threadIdx.x < 2) {  // the two padding records after the stage
+      recj[J * s * s + threadIdx.x] = Rec{0.0, 0.0, 0.0, 0.0};
+      tyj[J * s * s + threadIdx.x] = 0;
+    }
     __syncthreads();
 
     // a thread's atom pairs (rounds of kPairThreads; at most kTileRounds,
@@ -750,8 +764,9 @@ __global__ __launch_bounds__(kPairThreads, 4) void k_arad_tiles(NbSet set1, NbSe
       const int64_t xs = ((int64_t)i * amax + a) * amax;
       NbView x{set1.rec + xs, set1.ty + xs, set1.rc + xs};
       NbView y{recj + bj * s, tyj + bj * s, set2.rc + ((int64_t)j * amax + b) * amax};
-      const TC S = arad_distl2<TC, kGeneric>(x, cnti[a], asci[a] != 0, y, cntj[bj], ascj[bj] != 0,
-                                             inv_w, maxg, ttab, p.rw2, p.cw2);
+      const TC S = arad_distl2<TC, kGeneric, true>(x, cnti[a], asci[a] != 0, y, cntj[bj],
+                                                   ascj[bj] != 0, inv_w, maxg, ttab, etab, p.rw2,
+                                                   p.cw2);
       const double T = stoch(zri[a] - zrj[bj], zci[a] - zcj[bj], p.rw2, p.cw2);
       double D = selfi[a] + selfj[bj] - 2.0 * (double)S * T;  // farad_kernels.f90:590-592
       if (fabs(D) < p.eps) D = 0.0;                            // :594
@@ -800,7 +815,8 @@ __global__ __launch_bounds__(256) void k_arad_atomic(NbSet set1, NbSet set2, int
                                                      int amax, int mode, PairArgs pa, AradParams p,
                                                      const Ctl *__restrict__ ctl) {
   __shared__ TC ttab[kMaxTypes * kMaxTypes];
-  load_ttab(ttab, ctl);
+  __shared__ double etab[32];
+  load_tables(ttab, etab, ctl);
   __syncthreads();
   const bool generic = ctl->generic != 0;
   const int i = blockIdx.x * 16 + (threadIdx.x & 15);
@@ -810,9 +826,11 @@ __global__ __launch_bounds__(256) void k_arad_atomic(NbSet set1, NbSet set2, int
   NbView y = global_view(set2, j, amax);
   const TC inv_w = (TC)p.inv_w, maxg = (TC)p.maxg;
   const TC S = generic ? arad_distl2<TC, true>(x, set1.cnt[i], set1.asc[i] != 0, y, set2.cnt[j],
-                                               set2.asc[j] != 0, inv_w, maxg, ttab, p.rw2, p.cw2)
+                                               set2.asc[j] != 0, inv_w, maxg, ttab, etab, p.rw2,
+                                               p.cw2)
                        : arad_distl2<TC, false>(x, set1.cnt[i], set1.asc[i] != 0, y, set2.cnt[j],
-                                                set2.asc[j] != 0, inv_w, maxg, ttab, p.rw2, p.cw2);
+                                                set2.asc[j] != 0, inv_w, maxg, ttab, etab, p.rw2,
+                                                p.cw2);
   const double T = stoch(set1.zr[i] - set2.zr[j], set1.zc[i] - set2.zc[j], p.rw2, p.cw2);
   const double s1 = sizeof(TC) == 4 ? (double)set1.self32[i] : set1.self[i];
   const double s2 = sizeof(TC) == 4 ? (double)set2.self32[j] : set2.self[j];
@@ -879,8 +897,7 @@ int device_cus() {
 }
 
 size_t pair_lds_bytes(int amax, bool stage) {
-  return sizeof(double) * (kMaxTypes * kMaxTypes + kMaxSig * (kPairThreads / 64)) +
-         2 * stage_bytes(amax, stage);
+  return kTabBytes + sizeof(double) * kMaxSig * (kPairThreads / 64) + 2 * stage_bytes(amax, stage);
 }
 
 constexpr size_t kLdsLimit = 65536;
@@ -943,15 +960,15 @@ TileLds tile_lds(int amax, int max_ni, int j_recs, int j_atoms, int nsig) {
   L.j_atoms = std::max(j_atoms, 1);
   // the J stage doubles as the reduction buffer (nsig x 256 doubles)
   const size_t red = sizeof(double) * kPairThreads * std::max(nsig, 1);
-  size_t jb = sizeof(Rec) * (size_t)std::max(j_recs, 1) + 32 * (size_t)L.j_atoms +
-              sizeof(uint16_t) * (((size_t)std::max(j_recs, 1) + 7) & ~size_t(7));
-  int jr = std::max(j_recs, 1);
+  int jr = std::max(j_recs, 1) + 2;  // two padding records for the band loop's read-ahead
+  size_t jb = sizeof(Rec) * (size_t)jr + 32 * (size_t)L.j_atoms +
+              sizeof(uint16_t) * (((size_t)jr + 7) & ~size_t(7));
   while (jb < red) {  // grow the record area until the stage covers the buffer
     jr += 64;
     jb = sizeof(Rec) * (size_t)jr + 32 * (size_t)L.j_atoms + sizeof(uint16_t) * (((size_t)jr + 7) & ~size_t(7));
   }
   L.j_recs = jr;
-  L.bytes = sizeof(double) * kMaxTypes * kMaxTypes + 16 + 32 * (size_t)amax + jb;
+  L.bytes = kTabBytes + 16 + 32 * (size_t)amax + jb;
   return L;
 }
 
