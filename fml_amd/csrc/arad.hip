@@ -679,11 +679,13 @@ __device__ void stage_compact(Rec *rec, uint16_t *ty, int *cnt, int *asc, double
 }
 
 // Instantiated per type path; the host launches both and the one that does
-// not match the set's path (decided on the device) exits at once.
-template <typename TC, bool kGeneric>
-__global__ __launch_bounds__(kPairThreads, 4) void k_arad_tiles(NbSet set1, NbSet set2, TileArgs ta,
-                                                              AradParams p,
-                                                              const Ctl *__restrict__ ctl) {
+// not match the set's path (decided on the device) exits at once.  kWaves:
+// waves per SIMD the register budget is cut for (5 where the LDS footprint
+// lets five workgroups share a CU, else 4).
+template <typename TC, bool kGeneric, int kWaves>
+__global__ __launch_bounds__(kPairThreads, kWaves) void k_arad_tiles(NbSet set1, NbSet set2,
+                                                                   TileArgs ta, AradParams p,
+                                                                   const Ctl *__restrict__ ctl) {
   if ((ctl->generic != 0) != kGeneric) return;
   extern __shared__ __attribute__((aligned(16))) char smem[];
   char *ptr = smem;
@@ -901,6 +903,16 @@ size_t pair_lds_bytes(int amax, bool stage) {
 }
 
 constexpr size_t kLdsLimit = 65536;
+constexpr size_t kLdsPerCu = 160 * 1024;
+
+template <typename TC, int kWaves>
+void launch_tiles(int grid, size_t lds, hipStream_t st, const NbSet &s1, const NbSet &s2,
+                  const TileArgs &ta, const AradParams &p, const Ctl *ctl) {
+  hipLaunchKernelGGL((k_arad_tiles<TC, false, kWaves>), dim3(grid), dim3(kPairThreads), lds, st, s1,
+                     s2, ta, p, ctl);
+  hipLaunchKernelGGL((k_arad_tiles<TC, true, kWaves>), dim3(grid), dim3(kPairThreads), lds, st, s1,
+                     s2, ta, p, ctl);
+}
 
 // ---- tile planning (host) --------------------------------------------
 
@@ -1189,17 +1201,18 @@ int fmlh_arad_kernels(int mode, const double *q1, const double *zt1, const int32
       ta.counter = counter;
       ta.out = out;
       const int grid = (int)std::min<int64_t>(std::max<int64_t>(ta.ntiles, 1), cap);
+      const bool five = L.bytes * 5 <= kLdsPerCu;
       timing_begin(st);
       if (precision) {
-        hipLaunchKernelGGL((k_arad_tiles<float, false>), dim3(grid), dim3(kPairThreads), L.bytes, st,
-                           s1, s2, ta, p, ctl);
-        hipLaunchKernelGGL((k_arad_tiles<float, true>), dim3(grid), dim3(kPairThreads), L.bytes, st,
-                           s1, s2, ta, p, ctl);
+        if (five)
+          launch_tiles<float, 5>(grid, L.bytes, st, s1, s2, ta, p, ctl);
+        else
+          launch_tiles<float, 4>(grid, L.bytes, st, s1, s2, ta, p, ctl);
       } else {
-        hipLaunchKernelGGL((k_arad_tiles<double, false>), dim3(grid), dim3(kPairThreads), L.bytes, st,
-                           s1, s2, ta, p, ctl);
-        hipLaunchKernelGGL((k_arad_tiles<double, true>), dim3(grid), dim3(kPairThreads), L.bytes, st,
-                           s1, s2, ta, p, ctl);
+        if (five)
+          launch_tiles<double, 5>(grid, L.bytes, st, s1, s2, ta, p, ctl);
+        else
+          launch_tiles<double, 4>(grid, L.bytes, st, s1, s2, ta, p, ctl);
       }
       timing_end(st);
       FMLH_LAUNCH_CHECK("k_arad_tiles");
