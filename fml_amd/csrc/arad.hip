@@ -65,6 +65,7 @@ constexpr uint16_t kTypeNone = 0xFFFF;
 constexpr int kMaxSig = 8;
 constexpr int kPairThreads = 256;
 constexpr int kTileRounds = 4;  // atom-pair rounds per thread in a tile
+constexpr int kMaxAmax = 29;    // largest amax the tile path takes
 
 struct AradParams {
   double width, cut, rw, cw;
@@ -655,6 +656,67 @@ struct TileArgs {
   double *out;
 };
 
+// J (set-2 molecules per tile) for a set-1 molecule of ni atoms against the
+// size class s: fill `lanes` atom pairs, fit the J stage (j_recs_cap
+// records) and the reduction (nsig sums per molecule pair).
+__host__ __device__ inline int tile_j(int ni, int s, int nsig, int lanes, int j_recs_cap) {
+  const int jmax_red = kPairThreads / (nsig > 1 ? nsig : 1);
+  const int pairs = ni * s;
+  int J = pairs > 0 ? (lanes / pairs > 1 ? lanes / pairs : 1) : jmax_red;
+  if (s > 0) {
+    const int jc = j_recs_cap / (s * s);
+    J = J < (jc > 1 ? jc : 1) ? J : (jc > 1 ? jc : 1);
+  }
+  return J < jmax_red ? J : jmax_red;
+}
+
+// Tile list on the device, one wave per set-1 molecule i: its tiles start at
+// tile_start[i]; size classes largest first, each class's member run (for
+// the triangular form only members j >= i) chunked into groups of J.
+__global__ __launch_bounds__(64) void k_plan_emit(const int32_t *__restrict__ n1, int nm1, int amax,
+                                                  int tri, int nsig, int lanes, int j_recs_cap,
+                                                  const int32_t *__restrict__ cls_off,
+                                                  const int32_t *__restrict__ members,
+                                                  const int64_t *__restrict__ tile_start,
+                                                  Tile *__restrict__ tiles) {
+  __shared__ int cnt_s[kMaxAmax + 1], lo_s[kMaxAmax + 1], j_s[kMaxAmax + 1];
+  __shared__ int64_t base_s[kMaxAmax + 1];
+  const int i = blockIdx.x;
+  if (i >= nm1) return;
+  const int ni = min(max(n1[i], 0), amax);
+  const int s = threadIdx.x;
+  if (s <= amax) {
+    int lo = cls_off[s];
+    const int hi = cls_off[s + 1];
+    if (tri) {  // first member >= i (members ascending within a class)
+      int a = lo, b = hi;
+      while (a < b) {
+        const int m = (a + b) >> 1;
+        if (members[m] < i) a = m + 1; else b = m;
+      }
+      lo = a;
+    }
+    const int J = tile_j(ni, s, nsig, lanes, j_recs_cap);
+    lo_s[s] = lo;
+    j_s[s] = J;
+    cnt_s[s] = lo < hi ? (hi - lo + J - 1) / J : 0;
+  }
+  __syncthreads();
+  if (s <= amax) {
+    int64_t b = tile_start[i];
+    for (int t = amax; t > s; --t) b += cnt_s[t];
+    base_s[s] = b;
+  }
+  __syncthreads();
+  for (int c = amax; c >= 0; --c) {
+    const int lo = lo_s[c], J = j_s[c], hi = cls_off[c + 1];
+    for (int k = threadIdx.x; k < cnt_s[c]; k += blockDim.x) {
+      const int o = lo + k * J;
+      tiles[base_s[c] + k] = Tile{i, o, min(J, hi - o), c};
+    }
+  }
+}
+
 template <bool kF32Self, bool kRecs>
 __device__ void stage_compact(Rec *rec, uint16_t *ty, int *cnt, int *asc, double *self,
                               double *zr, double *zc, const NbSet &set, int mol, int natoms,
@@ -914,51 +976,52 @@ void launch_tiles(int grid, size_t lds, hipStream_t st, const NbSet &s1, const N
                      s2, ta, p, ctl);
 }
 
-// ---- tile planning (host) --------------------------------------------
+// ---- tile planning (host side of it) ---------------------------------
 
+// What the host derives from the atom counts alone: size classes of set 2,
+// the first tile of each set-1 molecule, the LDS the largest tile needs.
+// The tile list itself is written on the device by k_plan_emit.
 struct TilePlan {
-  std::vector<Tile> tiles;
-  std::vector<int32_t> members;  // set-2 indices grouped by atom count
-  int max_ni = 0, max_j_recs = 0, max_j_atoms = 0;
+  std::vector<int32_t> cls_off;     // amax + 2
+  std::vector<int32_t> members;     // set-2 indices grouped by atom count
+  std::vector<int64_t> tile_start;  // nm1 + 1
+  int max_j_recs = 0, max_j_atoms = 0;
 };
 
-// lanes: atom pairs one tile may hold.  j_recs_cap: LDS record capacity of
-// the J stage.  Tiles are emitted per molecule i, size classes largest
-// first, each class member run chunked into groups of J.
 void plan_tiles(const int32_t *n1h, int nm1, const int32_t *n2h, int nm2, int amax, bool tri,
                 int nsig, int lanes, int j_recs_cap, TilePlan &P) {
-  std::vector<int> cls_off(amax + 2, 0);
-  for (int j = 0; j < nm2; ++j) cls_off[std::min(std::max(n2h[j], 0), amax) + 1]++;
-  for (int s = 0; s <= amax; ++s) cls_off[s + 1] += cls_off[s];
+  P.cls_off.assign(amax + 2, 0);
+  for (int j = 0; j < nm2; ++j) P.cls_off[std::min(std::max(n2h[j], 0), amax) + 1]++;
+  for (int s = 0; s <= amax; ++s) P.cls_off[s + 1] += P.cls_off[s];
   P.members.assign(nm2, 0);
   {
-    std::vector<int> fill(cls_off.begin(), cls_off.end() - 1);
+    std::vector<int> fill(P.cls_off.begin(), P.cls_off.end() - 1);
     for (int j = 0; j < nm2; ++j) P.members[fill[std::min(std::max(n2h[j], 0), amax)]++] = j;
   }
-  P.tiles.clear();
-  const int jmax_red = std::max(1, kPairThreads / std::max(nsig, 1));
+  // per class, the first member >= i (moves forward as i grows)
+  std::vector<int> first(P.cls_off.begin(), P.cls_off.end() - 1);
+  P.tile_start.assign((size_t)nm1 + 1, 0);
+  P.max_j_recs = P.max_j_atoms = 0;
   for (int i = 0; i < nm1; ++i) {
     const int ni = std::min(std::max(n1h[i], 0), amax);
-    P.max_ni = std::max(P.max_ni, ni);
+    int64_t t = 0;
     for (int s = amax; s >= 0; --s) {
-      int lo = cls_off[s];
-      const int hi = cls_off[s + 1];
-      if (tri) lo = (int)(std::lower_bound(P.members.begin() + lo, P.members.begin() + hi, i) -
-                          P.members.begin());
+      const int hi = P.cls_off[s + 1];
+      int lo = P.cls_off[s];
+      if (tri) {
+        while (first[s] < hi && P.members[first[s]] < i) ++first[s];
+        lo = first[s];
+      }
       if (lo >= hi) continue;
-      const int pairs = ni * s;
-      int J = pairs > 0 ? std::max(1, lanes / pairs) : jmax_red;
-      if (s > 0) J = std::min(J, std::max(1, j_recs_cap / (s * s)));
-      J = std::min(J, jmax_red);
-      for (int o = lo; o < hi; o += J) {
-        const int c = std::min(J, hi - o);
-        P.tiles.push_back(Tile{i, o, c, s});
-        if (pairs > 0) {
-          P.max_j_recs = std::max(P.max_j_recs, c * s * s);
-          P.max_j_atoms = std::max(P.max_j_atoms, c * s);
-        }
+      const int J = tile_j(ni, s, nsig, lanes, j_recs_cap);
+      t += (hi - lo + J - 1) / J;
+      if (ni * s > 0) {
+        const int c = std::min(J, hi - lo);
+        P.max_j_recs = std::max(P.max_j_recs, c * s * s);
+        P.max_j_atoms = std::max(P.max_j_atoms, c * s);
       }
     }
+    P.tile_start[(size_t)i + 1] = P.tile_start[i] + t;
   }
 }
 
@@ -991,9 +1054,11 @@ void *g_plan_host = nullptr;
 size_t g_plan_cap = 0;
 hipEvent_t g_plan_done = nullptr;
 
-int upload_plan(const TilePlan &P, Tile *dtiles, int32_t *dmembers, hipStream_t st) {
+int upload_plan(const TilePlan &P, int32_t *dcls, int32_t *dmembers, int64_t *dstart,
+                hipStream_t st) {
   std::lock_guard<std::mutex> lk(g_plan_mu);
-  const size_t tb = sizeof(Tile) * P.tiles.size(), mb = sizeof(int32_t) * P.members.size();
+  const size_t sb = sizeof(int64_t) * P.tile_start.size();
+  const size_t cb = sizeof(int32_t) * P.cls_off.size(), mb = sizeof(int32_t) * P.members.size();
   if (g_plan_done) {
     hipError_t e = hipEventSynchronize(g_plan_done);
     if (e != hipSuccess) return record_error(e, "hipEventSynchronize(plan)");
@@ -1001,9 +1066,9 @@ int upload_plan(const TilePlan &P, Tile *dtiles, int32_t *dmembers, hipStream_t 
     hipError_t e = hipEventCreateWithFlags(&g_plan_done, hipEventDisableTiming);
     if (e != hipSuccess) return record_error(e, "hipEventCreate(plan)");
   }
-  if (tb + mb > g_plan_cap) {
+  if (sb + cb + mb > g_plan_cap) {
     if (g_plan_host) hipHostFree(g_plan_host);
-    g_plan_cap = (tb + mb) * 2 + 4096;
+    g_plan_cap = (sb + cb + mb) * 2 + 4096;
     hipError_t e = hipHostMalloc(&g_plan_host, g_plan_cap, hipHostMallocDefault);
     if (e != hipSuccess) {
       g_plan_host = nullptr;
@@ -1012,11 +1077,12 @@ int upload_plan(const TilePlan &P, Tile *dtiles, int32_t *dmembers, hipStream_t 
     }
   }
   char *h = static_cast<char *>(g_plan_host);
-  if (tb) memcpy(h, P.tiles.data(), tb);
-  if (mb) memcpy(h + tb, P.members.data(), mb);
-  hipError_t e = hipSuccess;
-  if (tb) e = hipMemcpyAsync(dtiles, h, tb, hipMemcpyHostToDevice, st);
-  if (e == hipSuccess && mb) e = hipMemcpyAsync(dmembers, h + tb, mb, hipMemcpyHostToDevice, st);
+  memcpy(h, P.tile_start.data(), sb);
+  memcpy(h + sb, P.cls_off.data(), cb);
+  if (mb) memcpy(h + sb + cb, P.members.data(), mb);
+  hipError_t e = hipMemcpyAsync(dstart, h, sb, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(dcls, h + sb, cb, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess && mb) e = hipMemcpyAsync(dmembers, h + sb + cb, mb, hipMemcpyHostToDevice, st);
   if (e == hipSuccess) e = hipEventRecord(g_plan_done, st);
   if (e != hipSuccess) return record_error(e, "hipMemcpyAsync(plan)");
   return 0;
@@ -1082,9 +1148,11 @@ size_t fmlh_arad_ws_bytes(int nm1, int nm2, int amax, int mode) {
   const bool one_set = (mode == FMLH_ARAD_SYMMETRIC || mode == FMLH_ARAD_KERNEL);
   size_t b = WsCarver::need(sizeof(Ctl)) + set_bytes((int64_t)nm1 * amax, amax);
   if (!one_set) b += set_bytes((int64_t)nm2 * amax, amax);
-  if (mode != FMLH_ARAD_DISTANCE)  // tile plan: tiles, class member list, counter
+  if (mode != FMLH_ARAD_DISTANCE)  // tile plan: tiles, members, classes, starts, counter
     b += WsCarver::need(sizeof(Tile) * (size_t)max_tiles(nm1, one_set ? nm1 : nm2, one_set)) +
-         WsCarver::need(sizeof(int32_t) * (size_t)(one_set ? nm1 : nm2)) + WsCarver::need(64);
+         WsCarver::need(sizeof(int32_t) * (size_t)(one_set ? nm1 : nm2)) +
+         WsCarver::need(sizeof(int32_t) * (size_t)(amax + 2)) +
+         WsCarver::need(sizeof(int64_t) * ((size_t)nm1 + 1)) + WsCarver::need(64);
   return b;
 }
 
@@ -1170,8 +1238,9 @@ int fmlh_arad_kernels(int mode, const double *q1, const double *zt1, const int32
     const int nsig_eff = (mode == FMLH_ARAD_KERNEL) ? 1 : nsig;
     TilePlan plan;
     plan_tiles(n1h, nm1, n2h, nm2, amax, one_set, nsig_eff, kPairThreads, amax * amax, plan);
-    TileLds L = tile_lds(amax, plan.max_ni, plan.max_j_recs, plan.max_j_atoms, nsig_eff);
-    if (L.bytes <= kLdsLimit) {
+    const int64_t ntiles = plan.tile_start[(size_t)nm1];
+    TileLds L = tile_lds(amax, 0, plan.max_j_recs, plan.max_j_atoms, nsig_eff);
+    if (L.bytes <= kLdsLimit && ntiles < INT32_MAX) {
       WsCarver w(ws, ws_bytes);
       w.take<Ctl>(1);
       NbSet d1, d2;
@@ -1179,10 +1248,17 @@ int fmlh_arad_kernels(int mode, const double *q1, const double *zt1, const int32
       if (!one_set) carve_set(w, (int64_t)nm2 * amax, amax, d2);
       Tile *dt = w.take<Tile>((size_t)max_tiles(nm1, one_set ? nm1 : nm2, one_set));
       int32_t *dm = w.take<int32_t>((size_t)(one_set ? nm1 : nm2));
+      int32_t *dcls = w.take<int32_t>((size_t)(amax + 2));
+      int64_t *dstart = w.take<int64_t>((size_t)nm1 + 1);
       int32_t *counter = w.take<int32_t>(16);
-      if (!dt || !dm || !counter) return -19;
-      rc = upload_plan(plan, dt, dm, st);
+      if (!dt || !dm || !dcls || !dstart || !counter) return -19;
+      rc = upload_plan(plan, dcls, dm, dstart, st);
       if (rc) return rc;
+      if (nm1 > 0) {
+        hipLaunchKernelGGL(k_plan_emit, dim3(nm1), dim3(64), 0, st, n1, nm1, amax, one_set ? 1 : 0,
+                           nsig_eff, kPairThreads, amax * amax, dcls, dm, dstart, dt);
+        FMLH_LAUNCH_CHECK("k_plan_emit");
+      }
       hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, st, counter);
       FMLH_LAUNCH_CHECK("k_zero_counter");
       TileArgs ta;
@@ -1191,7 +1267,7 @@ int fmlh_arad_kernels(int mode, const double *q1, const double *zt1, const int32
       ta.amax = amax;
       ta.nsig = pa.nsig;
       ta.triangular = pa.triangular;
-      ta.ntiles = (int)plan.tiles.size();
+      ta.ntiles = (int)ntiles;
       ta.stage_j_recs = L.j_recs;
       ta.stage_j_atoms = L.j_atoms;
       for (int k = 0; k < kMaxSig; ++k) ta.inv_s2[k] = pa.inv_s2[k];
