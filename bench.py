@@ -5,16 +5,21 @@ Default workload (the global ARAD kernel case): 500 synthetic molecules from
 seed 0 (5-23 atoms, H/C/N/O/S), ARAD descriptors (500, 23, 5, 23) FP64,
 width 0.2, cut 5.0, r_width 1.0, c_width 0.5, the full 500 x 500 molecular
 kernel for sigma in {0.1, 1, 10} in one launch.  One step = one call of the
-GPU entry point (descriptor preparation, self terms, molecule-pair kernel)
-on inputs already resident in HBM.  Metric: kernel entries per second
+GPU entry point (descriptor preparation, self terms, molecule-pair tiles) on
+inputs already resident in HBM.  Metric: kernel entries per second
 (3 sigmas x 500 x 500 entries per step).
 
 Printed: one JSON line with the metric, the roofline of the dominant kernel
-(k_arad_pairs, timed with HIP events on its stream) and a CPU baseline (the
+(k_arad_tiles, timed with HIP events on its stream) and a CPU baseline (the
 reference Fortran built by oracle/Makefile, on a bounded row block, all
 host threads).
 
-    python bench.py [--steps K] [--warmup W] [--workload arad|gaussian|laplacian|cholesky]
+    python bench.py [--steps K] [--warmup W] [--workload NAME]
+
+Other workloads: arad4096 / arad4096_fp32 (4096 x 4096 uneven molecules),
+krr (1000-molecule FP64 KRR), gaussian64 / gaussian32 / gaussianbf16x3 /
+laplacian64 / laplacian32 (36k x 36k kernels), cholesky64 / cholesky32,
+krr36k_fp64 / krr36k_mixed, serving, copy (counter calibration).
 """
 from __future__ import annotations
 
@@ -34,11 +39,18 @@ sys.path.insert(0, ROOT)
 PEAK_HBM_GBS = 8000.0
 PEAK_FP64_TFLOPS = 78.6            # vector and matrix, FMA = 2 flop
 PEAK_FP32_TFLOPS = 157.3
-PEAK_FP64_VALU_TOPS = 39.3         # FP64 VALU lane-instructions/s (256 CU x 64 lanes x 2.4 GHz)
+PEAK_FP64_VALU_TOPS = 39.3         # FP64 VALU lane-operations/s: the FP64 vector peak / 2 (FMA = 1 op)
+PEAK_FP32_VALU_TOPS = 78.6         # FP32 VALU lane-operations/s: the FP32 vector peak / 2
 
 # Algorithmic FP64 operations per inner (m1, m2) evaluation of the overlap
 # term: 13 arithmetic operations plus the exponential counted as 20.
 ARAD_OPS_PER_EVAL = 33
+
+# HBM bytes per launch of k_arad_tiles (FP64), from rocprofv3 --pmc FETCH_SIZE
+# and WRITE_SIZE runs of this script, each scaled by the counter's reading
+# for a device copy of known size (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0):
+# profiles/hbm_traffic_counters.md.
+ARAD_TRAFFIC_BYTES = {"arad500": 2.533e9, "arad4096": 1.898e11}
 
 
 def parse():
@@ -100,6 +112,7 @@ def arad_workload(args, big=False, precision=0):
     from fml_amd.kernels.kernels import arad_types
 
     width, cut, rw, cw = 0.2, 5.0, 1.0, 0.5
+    workload_key = None if precision else ("arad4096" if big else "arad500")
     if big:
         nm, amax, sig = args.n or 4096, 29, [1.0, 4.0]
         ms1 = synth.generate(nm, seed=0, nmin=3, nmax=29, dist="geometric")
@@ -149,9 +162,11 @@ def arad_workload(args, big=False, precision=0):
     def roofline(t_main_s):
         ops = evals * ARAD_OPS_PER_EVAL
         ach = ops / t_main_s / 1e12
-        return {"bound": "valu", "achieved": round(ach, 3), "peak": PEAK_FP64_VALU_TOPS,
-                "unit": "T FP64-VALU-ops/s", "frac": round(ach / PEAK_FP64_VALU_TOPS, 4),
-                "traffic": None,
+        peak = PEAK_FP32_VALU_TOPS if precision else PEAK_FP64_VALU_TOPS
+        return {"bound": "valu", "achieved": round(ach, 3), "peak": peak,
+                "unit": "T FP32-VALU-ops/s" if precision else "T FP64-VALU-ops/s",
+                "frac": round(ach / peak, 4),
+                "traffic": ARAD_TRAFFIC_BYTES.get(workload_key),
                 "inner_evaluations_per_s": round(evals / t_main_s, 1),
                 "ops_per_evaluation": ARAD_OPS_PER_EVAL}
 
@@ -274,8 +289,9 @@ def dense36k_workload(args, kind):
             return {"bound": "mfma", "achieved": round(ach, 2), "peak": round(peak, 1), "unit": "TFLOP/s",
                     "frac": round(ach / peak, 4), "traffic": None}
         ach = (2.0 * n * n * d) / t / 1e12   # sub + abs-add per feature
-        return {"bound": "valu", "achieved": round(ach, 2), "peak": PEAK_FP64_VALU_TOPS,
-                "unit": "T VALU-ops/s", "frac": round(ach / PEAK_FP64_VALU_TOPS, 4), "traffic": None}
+        peak = PEAK_FP64_VALU_TOPS if kind == "laplacian64" else PEAK_FP32_VALU_TOPS
+        return {"bound": "valu", "achieved": round(ach, 2), "peak": peak,
+                "unit": "T VALU-ops/s", "frac": round(ach / peak, 4), "traffic": None}
 
     return info, step, units, roofline, None, None
 
