@@ -12,7 +12,7 @@
 // Factorisation (right-looking, two block levels):
 //   for each 256-column outer panel:
 //     for each 64-column inner block inside it:
-//       k_potf2   diagonal 64x64 block in LDS (one workgroup)
+//       k_potf2   diagonal 64x64 block in LDS (one wave)
 //       k_trsm    rows below: M[r][k] <- M[r][k] L_kk^{-T} (one thread/row)
 //       k_update  panel columns right of the block: C -= P Q^T on MFMA
 //     k_update    trailing matrix right of the outer panel, K = 256 on MFMA
@@ -35,46 +35,44 @@ using namespace fmlh;
 constexpr int kNB = 64;     // inner block
 constexpr int kNBO = 256;   // outer panel
 
+// One wave factors the diagonal block in LDS: lane r owns row r.  Step j
+// takes the square root of the pivot (every lane computes the same value),
+// scales column j below it, then each lane r > j updates its row
+// M[r][c] -= L[r][j] L[c][j] for j < c <= r (L[c][j] read as a broadcast).
 template <typename T>
-__global__ __launch_bounds__(256) void k_potf2(T *__restrict__ a, int64_t lda, int k0, int b,
-                                               int32_t *__restrict__ info) {
+__global__ __launch_bounds__(64) void k_potf2(T *__restrict__ a, int64_t lda, int k0, int b,
+                                              int32_t *__restrict__ info) {
   __shared__ T S[kNB][kNB + 1];
-  __shared__ int fail;
+  __shared__ T colj[kNB];  // column j of L, apart from S so its reads need not wait on S's writes
   if (*info != 0) return;
-  const int tid = threadIdx.x;
-  for (int idx = tid; idx < b * b; idx += 256) {
-    const int r = idx / b, c = idx % b;
-    if (c <= r) S[r][c] = a[(int64_t)(k0 + r) * lda + k0 + c];
-  }
-  if (tid == 0) fail = 0;
+  const int r = threadIdx.x;
+  T *grow = a + (int64_t)(k0 + r) * lda + k0;
+  if (r < b)
+    for (int c = 0; c <= r; ++c) S[r][c] = grow[c];
   __syncthreads();
   for (int j = 0; j < b; ++j) {
-    if (tid == 0) {
-      const T d = S[j][j];
-      if (!(d > T(0)))
-        fail = j + 1;
-      else
-        S[j][j] = sqrt(d);
-    }
-    __syncthreads();
-    if (fail) {
-      if (tid == 0) *info = k0 + fail;
+    const T d = S[j][j];
+    if (!(d > T(0))) {  // uniform: every lane read the same pivot
+      if (r == 0) *info = k0 + j + 1;
       return;
     }
-    const T piv = S[j][j];
-    for (int r = j + 1 + tid; r < b; r += 256) S[r][j] = S[r][j] / piv;
+    const T piv = sqrt(d);
+    T lrj = T(0);
+    if (r == j) S[j][j] = piv;
+    if (r > j && r < b) {
+      lrj = S[r][j] / piv;
+      S[r][j] = lrj;
+    }
+    colj[r] = lrj;
     __syncthreads();
-    const int w = b - j - 1;
-    for (int idx = tid; idx < w * w; idx += 256) {
-      const int r = j + 1 + idx / w, c = j + 1 + idx % w;
-      if (c <= r) S[r][c] -= S[r][j] * S[c][j];
+    if (r > j && r < b) {
+      T *sr = S[r];
+      for (int c = j + 1; c <= r; ++c) sr[c] -= lrj * colj[c];
     }
     __syncthreads();
   }
-  for (int idx = tid; idx < b * b; idx += 256) {
-    const int r = idx / b, c = idx % b;
-    if (c <= r) a[(int64_t)(k0 + r) * lda + k0 + c] = S[r][c];
-  }
+  if (r < b)
+    for (int c = 0; c <= r; ++c) grow[c] = S[r][c];
 }
 
 // Rows r in [r0, n): x = M[r][k0:k0+b] solves x L^T = m_r (forward
@@ -145,7 +143,7 @@ int potrf_impl(T *a, int64_t lda, int n, int32_t *info, hipStream_t st) {
     const int kend = min(K0 + kNBO, n);
     for (int k0 = K0; k0 < kend; k0 += kNB) {
       const int b = min(kNB, kend - k0);
-      hipLaunchKernelGGL(k_potf2<T>, dim3(1), dim3(256), 0, st, a, lda, k0, b, info);
+      hipLaunchKernelGGL(k_potf2<T>, dim3(1), dim3(64), 0, st, a, lda, k0, b, info);
       FMLH_LAUNCH_CHECK("k_potf2");
       const int r0 = k0 + b;
       if (r0 < n) {
