@@ -186,7 +186,25 @@ def arad_workload(args, big=False, precision=0):
                  "sample": f"rows 0..{rows - 1} x all {nm} molecules ({len(sig) * rows * nm} entries), "
                            f"{dt:.2f} s"}, err)
 
-    return info, step, units_per_step, roofline, cpu_baseline, None
+    def finish(res):
+        # the same call through the Python API with host (numpy) inputs and
+        # output: PCIe transfers included (outside the timed region)
+        from fml_amd.kernels import fget_kernels_arad
+        reps = 1 if big else 3
+        fget_kernels_arad(X1, X2, z1, z2, n1, n2, sig, width=width, cut_distance=cut, r_width=rw,
+                          c_width=cw, precision=precision)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fget_kernels_arad(X1, X2, z1, z2, n1, n2, sig, width=width, cut_distance=cut, r_width=rw,
+                              c_width=cw, precision=precision)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / reps
+        res["host_buffers"] = {"ms_per_call": round(dt * 1e3, 3),
+                               "entries_per_s": round(units_per_step / dt, 1),
+                               "note": "numpy in, numpy out: descriptor upload and kernel download included"}
+
+    return info, step, units_per_step, roofline, cpu_baseline, finish
 
 
 def _cm_set(nmol, seed, nmin, nmax_atoms, nmax):
