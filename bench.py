@@ -65,6 +65,7 @@ def parse():
     ap.add_argument("--duration", type=float, default=5.0, help="serving run length in seconds")
     ap.add_argument("--n", type=int, default=0, help="problem size override for the 36k workloads")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--eager", action="store_true", help="plain calls instead of HIP-graph replays")
     ap.add_argument("--cpu-rows", type=int, default=100, help="row block of the CPU baseline sample")
     return ap.parse_args()
 
@@ -108,7 +109,6 @@ def arad_workload(args, big=False, precision=0):
     import fml_amd._native as nat
     from fml_amd import synth
     from fml_amd.arad import ARAD
-    from fml_amd.device import empty, to_device, workspace
     from fml_amd.kernels.kernels import arad_types
 
     width, cut, rw, cw = 0.2, 5.0, 1.0, 0.5
@@ -128,20 +128,20 @@ def arad_workload(args, big=False, precision=0):
     z2, n2 = (z1, n1) if ms2 is ms1 else arad_types([ms2.molecule(m)[0] for m in range(nm)], amax)
     evals = arad_inner_evaluations(X1, n1, X2, n2, cut, width)
 
-    lib = nat.load()
-    dq1, dz1, dn1 = to_device(X1), to_device(z1), to_device(n1, np.int32)
-    dq2, dz2, dn2 = to_device(X2), to_device(z2), to_device(n2, np.int32)
-    out = empty((nm, nm, len(sig)), np.float64)
-    wsb = lib.fmlh_arad_ws_bytes(nm, nm, amax, nat.ARAD_GLOBAL)
-    ws = workspace(wsb)
-    sarr = (ctypes.c_double * len(sig))(*sig)
-    h1, hp1 = nat.host_i32(n1)
-    h2, hp2 = nat.host_i32(n2)
+    from fml_amd.batch import AradKernelCall
+    call = AradKernelCall(X1, z1, n1, X2, z2, n2, sig, mode="global", width=width, cut_distance=cut,
+                          r_width=rw, c_width=cw, precision=precision)
+    out = call.out
+    # tile shape: the autotuner times its candidates on the device (on the
+    # first 512 set-1 molecules for the 4096 x 4096 set) before the timed steps
+    t0 = time.perf_counter()
+    tiling, tuned_ms = call.autotune(sample_rows=512 if big else None, reps=3)
+    tune_s = time.perf_counter() - t0
+    torch.cuda.synchronize()
 
     def step():
-        nat.call("fmlh_arad_kernels", nat.ARAD_GLOBAL, nat.ptr(dq1), nat.ptr(dz1), nat.ptr(dn1), nm,
-                 nat.ptr(dq2), nat.ptr(dz2), nat.ptr(dn2), nm, amax, sarr, len(sig), width, cut, rw, cw,
-                 precision, nat.ptr(out), nat.ptr(ws), wsb, hp1, hp2, nat.stream_ptr())
+        call.run()
+    step.graph_capable = True
 
     units_per_step = len(sig) * nm * nm
     if big:
@@ -155,7 +155,13 @@ def arad_workload(args, big=False, precision=0):
         "unit": "entries/s",
         "dtype": "f32" if precision else "f64",
         "config": {"workload": desc, "molecules": nm, "amax": amax,
-                   "sigmas": sig, "inner_evaluations_per_step": evals},
+                   "sigmas": sig, "inner_evaluations_per_step": evals,
+                   "tiling": {"pairs_per_tile": tiling.pairs_per_tile,
+                              "waves_per_simd": tiling.waves_per_simd,
+                              "groups_per_cu": tiling.groups_per_cu,
+                              "autotuned_on": f"{512 if big else nm} x {nm} molecules",
+                              "autotune_s": round(tune_s, 2),
+                              "tuned_ms_per_call": round(tuned_ms, 3)}},
         "kernel": "k_arad_tiles",
     }
 
@@ -476,16 +482,23 @@ def serving_workload(args):
             fit_futs.append((nf, srv.submit_fit(X[:nf].cpu().numpy().astype(np.float32), y[:nf], 1000.0, 1e-6)))
     res = [f.result(timeout=300) for f in futs]
     t_end = time.perf_counter()
+    t_done = [r[2] for r in res]
     fits = [(nf, f.result(timeout=600)) for nf, f in fit_futs]
-    srv.stop()
+    srv.close()
     lat = latency_percentiles([r[1] for r in res])
+    # requests whose time in the server overlapped a fit, and the rest
+    fit_iv = [(a, b) for _, a, b in srv.fit_log]
+    during = [any(d - l < b and d > a for a, b in fit_iv) for (_, l, d) in res]
+    lat_fit = latency_percentiles([r[1] for r, f in zip(res, during) if f]) if any(during) else None
+    lat_nofit = latency_percentiles([r[1] for r, f in zip(res, during) if not f])
     wall = t_end - t_start
     return {"metric": "serving p50/p99 predict latency", "value": round(lat["p99_ms"], 3), "unit": "ms (p99)",
             "p50_ms": round(lat["p50_ms"], 3), "p99_ms": round(lat["p99_ms"], 3), "mean_ms": round(lat["mean_ms"], 3),
             "requests": len(res), "molecules": nmol_total, "wall_s": round(wall, 3),
             "throughput_req_s": round(len(res) / wall, 1), "throughput_mol_s": round(nmol_total / wall, 1),
             "batches": len(srv.batches), "mean_batch_requests": round(float(np.mean([b[0] for b in srv.batches])), 2),
-            "fits": [{"n": nf, "seconds": round(s, 3)} for nf, s in srv.fit_log],
+            "fits": [{"n": nf, "seconds": round(b - a, 3)} for nf, a, b in srv.fit_log],
+            "overlapping_fits": lat_fit, "outside_fits": lat_nofit,
             "higher_is_better": False, "dtype": "f64", "data": "synthetic",
             "config": {"workload": f"serving: {ntr}-molecule Gaussian KRR model resident, Poisson 2000 req/s, "
                                    "1-256 molecules per request, fit requests 1k-16k admitted by free HBM",
@@ -508,7 +521,10 @@ def main():
         sys.exit(2)
     import torch
     import fml_amd._native as nat
+    from fml_amd.graphs import Graph
     torch.cuda.set_device(0)
+    # a side stream: HIP graphs cannot be captured on the null stream
+    torch.cuda.set_stream(torch.cuda.Stream())
     lib = nat.load()
     lib.fmlh_timing_enable.argtypes = [ctypes.c_int]
     lib.fmlh_timing_enable.restype = None
@@ -556,6 +572,20 @@ def main():
     t_kernel = kms.value / 1e3 / kcnt.value if kcnt.value else t_step
     print(f"[bench] {args.steps} steps: {ms_total:.3f} ms (wall {wall * 1e3:.1f} ms), "
           f"dominant kernel avg {t_kernel * 1e3:.3f} ms over {kcnt.value}", file=sys.stderr, flush=True)
+    graph_ms = None
+    if getattr(step, "graph_capable", False) and not args.eager:
+        # the same step captured once into a HIP graph, replayed K times
+        g = Graph.capture(step, stream=torch.cuda.current_stream())
+        g.launch()
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(args.steps):
+            g.launch()
+        e1.record()
+        torch.cuda.synchronize()
+        graph_ms = e0.elapsed_time(e1) / args.steps
+        g.destroy()
+        print(f"[bench] {args.steps} graph replays: {graph_ms:.3f} ms per step", file=sys.stderr, flush=True)
 
     res = {
         "metric": info["metric"],
@@ -574,6 +604,10 @@ def main():
         "roofline": roofline(t_kernel) if roofline else None,
         "dominant_kernel_ms": round(t_kernel * 1e3, 4),
     }
+    if graph_ms is not None:
+        res["graph_replay"] = {"ms_per_step": round(graph_ms, 4),
+                               "value": round(units / (graph_ms / 1e3), 1) if units else None,
+                               "note": "the step captured once with fmlh_graph_begin/end, replayed per step"}
     if finish:
         finish(res)
     if not args.no_cpu and cpu_baseline:

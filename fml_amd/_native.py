@@ -28,6 +28,13 @@ SIGNATURES = {
     "fmlh_abi_version": (c_int, []),
     "fmlh_last_error": (ctypes.c_char_p, []),
     "fmlh_device_cu_count": (c_int, []),
+    "fmlh_init": (c_int, []),
+    "fmlh_stream_create": (c_int, [c_int, c_void_p]),
+    "fmlh_stream_destroy": (c_int, [c_void_p]),
+    "fmlh_graph_begin": (c_int, [c_void_p]),
+    "fmlh_graph_end": (c_int, [c_void_p, c_void_p]),
+    "fmlh_graph_launch": (c_int, [c_void_p, c_void_p]),
+    "fmlh_graph_destroy": (c_int, [c_void_p]),
     "fmlh_coulomb_sorted": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p,
                                     c_void_p, c_void_p, c_void_p]),
     "fmlh_gaussian_kernel": (c_int, [c_void_p, c_int64, c_int, c_void_p, c_int64, c_int, c_int, c_double,
@@ -41,7 +48,12 @@ SIGNATURES = {
     "fmlh_arad_ws_bytes": (c_size_t, [c_int, c_int, c_int, c_int]),
     "fmlh_arad_kernels": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
                                   c_int, c_int, c_dbl_p, c_int, c_double, c_double, c_double, c_double,
-                                  c_int, c_void_p, c_void_p, c_size_t, c_void_p, c_void_p, c_void_p]),
+                                  c_int, c_void_p, c_void_p, c_size_t, c_void_p, c_void_p]),
+    "fmlh_arad_autotune": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
+                                   c_int, c_int, c_dbl_p, c_int, c_double, c_double, c_double, c_double,
+                                   c_int, c_void_p, c_void_p, c_size_t, c_int, c_void_p, c_dbl_p, c_void_p]),
+    "fmlh_arad_tiling_get": (c_int, [c_int, c_int, c_int, c_int, c_void_p]),
+    "fmlh_arad_tiling_set": (c_int, [c_int, c_int, c_int, c_int, c_void_p]),
     "fmlh_arad_atomic_ws_bytes": (c_size_t, [c_int, c_int, c_int]),
     "fmlh_arad_atomic": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
                                  c_int, c_int, c_dbl_p, c_int, c_double, c_double, c_double, c_double,
@@ -61,6 +73,15 @@ SIGNATURES = {
                                   c_void_p]),
     "fmlh_gemv_t": (c_int, [c_void_p, c_int64, c_int, c_int, c_void_p, c_void_p, c_void_p]),
 }
+
+
+class AradTiling(ctypes.Structure):
+    """fmlh_arad_tiling: tile shape and launch shape of the ARAD tile kernel."""
+    _fields_ = [("pairs_per_tile", c_int), ("waves_per_simd", c_int), ("groups_per_cu", c_int)]
+
+    def as_tuple(self):
+        return (self.pairs_per_tile, self.waves_per_simd, self.groups_per_cu)
+
 
 ARAD_GLOBAL, ARAD_SYMMETRIC, ARAD_KERNEL, ARAD_DISTANCE = 0, 1, 2, 3
 ARAD_ATOMIC_KERNELS, ARAD_ATOMIC_DISTANCE = 4, 5
@@ -129,13 +150,3 @@ def ptr(t):
     if t is None:
         return c_void_p(0)
     return c_void_p(t.data_ptr())
-
-
-def host_i32(a):
-    """(array, pointer) of a contiguous host int32 copy of a; (None, NULL) for
-    device tensors or None."""
-    if a is None or hasattr(a, "is_cuda"):
-        return None, c_void_p(0)
-    import numpy as np
-    h = np.ascontiguousarray(np.asarray(a), dtype=np.int32)
-    return h, c_void_p(h.ctypes.data)

@@ -1,0 +1,143 @@
+"""Prepared, HBM-resident calls for batching and HIP-graph capture (no
+reference counterpart; the batching form of the hot path).
+
+Each object owns its device inputs, output and workspace, so its ``run()``
+can be enqueued any number of times, captured into a ``fml_amd.graphs.Graph``
+together with other prepared calls, and replayed with one launch.  New input
+values of the same shape are copied in with ``set_*`` between runs.
+
+* ``CoulombCall``   sorted Coulomb matrices of a molecule batch
+                    (frepresentations.f90:23-110), one workgroup per molecule.
+* ``AradKernelCall`` a molecular ARAD kernel (fget_kernels_arad /
+                    fget_symmetric_kernels_arad / fget_kernel_arad,
+                    farad_kernels.f90:95-755) over size-sorted molecule-pair
+                    tiles planned on the device; ``autotune()`` picks its
+                    tile shape by timing candidates on the device.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _native, tuning
+from .device import empty, to_device, workspace
+
+_MODES = {"global": _native.ARAD_GLOBAL, "symmetric": _native.ARAD_SYMMETRIC,
+          "kernel": _native.ARAD_KERNEL}
+
+
+class CoulombCall:
+    """Sorted Coulomb matrices of a fixed batch of molecules.
+
+    charges (total,), coords (total, 3), offsets (nmol+1,).  ``cm`` (nmol,
+    nmax(nmax+1)/2) f64 (and ``cm32`` when float32) are resident outputs."""
+
+    def __init__(self, charges, coords, offsets, nmax=23, float32=False):
+        self.z = to_device(charges, np.float64)
+        self.r = to_device(np.asarray(coords).reshape(-1, 3), np.float64)
+        self.off = to_device(offsets, np.int32)
+        self.nmol = int(self.off.shape[0]) - 1
+        self.nmax = int(nmax)
+        counts = np.diff(np.asarray(offsets, dtype=np.int64))
+        if self.nmol and counts.max() > self.nmax:
+            raise ValueError(f"molecule {int(np.argmax(counts > self.nmax))} has more than "
+                             f"nmax={self.nmax} atoms")
+        packed = self.nmax * (self.nmax + 1) // 2
+        self.cm = empty((self.nmol, packed), np.float64)
+        self.cm32 = empty((self.nmol, packed), np.float32) if float32 else None
+        _native.call("fmlh_init")
+
+    def set_coords(self, coords):
+        """New coordinates for the same molecules (same atom order)."""
+        self.r.copy_(to_device(np.asarray(coords).reshape(-1, 3), np.float64))
+
+    def run(self):
+        _native.call("fmlh_coulomb_sorted", _native.ptr(self.z), _native.ptr(self.r),
+                     _native.ptr(self.off), self.nmol, self.nmax, _native.ptr(self.cm),
+                     _native.ptr(self.cm32), None, None, _native.stream_ptr())
+
+
+class AradKernelCall:
+    """One molecular ARAD kernel call on resident descriptor sets.
+
+    q (nm, amax, 5, amax) f64, z (nm, amax, 2) PTP types, n (nm,) atom
+    counts, as ``fget_kernels_arad``.  mode "global" (two sets), "symmetric"
+    or "kernel" (set 1 only).  ``out`` holds the result in C order
+    (nm2, nm1, nsig) -- i.e. the reference's Fortran (nsig, nm1, nm2);
+    ``result()`` returns that Fortran-order view."""
+
+    def __init__(self, q1, z1, n1, q2=None, z2=None, n2=None, sigmas=(1.0,), mode="global",
+                 width=0.2, cut_distance=6.0, r_width=1.0, c_width=0.5, precision=0, tiling=None):
+        self.mode = _MODES[mode]
+        self.q1, self.z1, self.n1 = to_device(q1), to_device(z1), to_device(n1, np.int32)
+        self.nm1, self.amax = int(self.q1.shape[0]), int(self.q1.shape[1])
+        if self.mode == _native.ARAD_GLOBAL:
+            if q2 is None:
+                raise ValueError("the global mode needs set 2")
+            self.q2, self.z2, self.n2 = to_device(q2), to_device(z2), to_device(n2, np.int32)
+            self.nm2 = int(self.q2.shape[0])
+        else:
+            self.q2 = self.z2 = self.n2 = None
+            self.nm2 = self.nm1
+        sig = np.asarray(sigmas, dtype=np.float64).reshape(-1)
+        if self.mode == _native.ARAD_KERNEL:
+            sig = sig[:1]
+            width, cut_distance, r_width, c_width = 0.2, 6.0, 1.0, 0.5
+        if not 1 <= sig.size <= 8:
+            raise ValueError("one prepared call takes 1..8 sigmas")
+        self.nsig = int(sig.size)
+        self.sig = (ctypes.c_double * self.nsig)(*sig.tolist())
+        self.width, self.cut, self.rw, self.cw = float(width), float(cut_distance), float(r_width), float(c_width)
+        self.precision = int(precision)
+        self.tiling = tiling
+        lib = _native.load()
+        self.wsb = lib.fmlh_arad_ws_bytes(self.nm1, self.nm2, self.amax, self.mode)
+        self.ws = workspace(self.wsb)
+        self.out = (empty((self.nm1, self.nm1), np.float64) if self.mode == _native.ARAD_KERNEL
+                    else empty((self.nm2, self.nm1, self.nsig), np.float64))
+
+    def _args(self, nm1=None):
+        p = _native.ptr
+        return (self.mode, p(self.q1), p(self.z1), p(self.n1), self.nm1 if nm1 is None else nm1,
+                p(self.q2), p(self.z2), p(self.n2), self.nm2, self.amax, self.sig, self.nsig,
+                self.width, self.cut, self.rw, self.cw, self.precision, p(self.out), p(self.ws),
+                self.wsb)
+
+    def set_descriptors(self, q1=None, q2=None):
+        """New descriptor values with the same atom counts."""
+        if q1 is not None:
+            self.q1.copy_(to_device(q1))
+        if q2 is not None:
+            if self.q2 is None:
+                raise ValueError("this call has no set 2")
+            self.q2.copy_(to_device(q2))
+
+    def run(self):
+        tp = ctypes.byref(self.tiling) if self.tiling is not None else None
+        _native.call("fmlh_arad_kernels", *self._args(), tp, _native.stream_ptr())
+
+    def autotune(self, sample_rows=None, reps=3):
+        """Time the candidate tilings on the first `sample_rows` set-1
+        molecules (all of them by default) against the whole set 2; the
+        fastest becomes this call's tiling and the library's choice for
+        (mode, amax, precision, nsig).  Returns (tiling, ms per call on the
+        sample)."""
+        rows = self.nm1 if sample_rows is None else max(1, min(int(sample_rows), self.nm1))
+        if self.mode != _native.ARAD_GLOBAL:
+            rows = self.nm1  # the triangular forms need the whole set
+        best = _native.AradTiling()
+        ms = ctypes.c_double(0.0)
+        _native.call("fmlh_arad_autotune", *self._args(rows), int(reps), ctypes.byref(best),
+                     ctypes.byref(ms), _native.stream_ptr())
+        self.tiling = best
+        tuning.record(self.mode, self.amax, self.precision, self.nsig, best)
+        if rows != self.nm1:
+            self.run()  # out then holds the full result again
+        return best, ms.value
+
+    def result(self):
+        """Fortran-order view: (nsig, nm1, nm2), or (nm1, nm1) for "kernel"."""
+        if self.mode == _native.ARAD_KERNEL:
+            return self.out.t()
+        return self.out.permute(2, 1, 0)

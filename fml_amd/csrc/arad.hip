@@ -46,7 +46,9 @@
 #include <string.h>
 
 #include <algorithm>
+#include <map>
 #include <mutex>
+#include <tuple>
 #include <vector>
 
 #include "common.h"
@@ -645,7 +647,7 @@ struct Tile {
 struct TileArgs {
   int mode;
   int nm1, amax, nsig, triangular;
-  int ntiles;
+  const int64_t *tile_start;  // tile_start[nm1] = number of tiles (written by the planner)
   int stage_j_recs;  // LDS capacity (records) of the J-molecule stage
   int stage_j_atoms;
   double inv_s2[kMaxSig];
@@ -668,6 +670,118 @@ __host__ __device__ inline int tile_j(int ni, int s, int nsig, int lanes, int j_
     J = J < (jc > 1 ? jc : 1) ? J : (jc > 1 ? jc : 1);
   }
   return J < jmax_red ? J : jmax_red;
+}
+
+__device__ __forceinline__ int size_class(int n, int amax) { return min(max(n, 0), amax); }
+
+// Size classes of set 2: cls_off[s] = offset of atom count s in members
+// (s = 0..amax, cls_off[amax + 1] = nm2), members = set-2 indices grouped by
+// atom count, ascending within a class.  One workgroup of 1024 threads:
+// counts, a scan, then one wave per class appends its members in index
+// order (a ballot prefix per 64 indices), so the grouping is stable.
+__global__ __launch_bounds__(1024) void k_plan_classes(const int32_t *__restrict__ n2, int nm2,
+                                                       int amax, int32_t *__restrict__ cls_off,
+                                                       int32_t *__restrict__ members) {
+  __shared__ int cnt[kMaxAmax + 2];
+  __shared__ int off[kMaxAmax + 2];
+  const int tid = threadIdx.x;
+  if (tid <= amax + 1) cnt[tid] = 0;
+  __syncthreads();
+  for (int j = tid; j < nm2; j += blockDim.x) atomicAdd(&cnt[size_class(n2[j], amax)], 1);
+  __syncthreads();
+  if (tid == 0) {
+    int a = 0;
+    for (int s = 0; s <= amax; ++s) {
+      off[s] = a;
+      cls_off[s] = a;
+      a += cnt[s];
+    }
+    cls_off[amax + 1] = a;
+  }
+  __syncthreads();
+  const int lane = tid & 63, wave = tid >> 6, nwaves = blockDim.x >> 6;
+  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  for (int s = wave; s <= amax; s += nwaves) {
+    int base = off[s];
+    for (int j0 = 0; j0 < nm2; j0 += 64) {
+      const int j = j0 + lane;
+      const bool hit = j < nm2 && size_class(n2[j], amax) == s;
+      const uint64_t m = __ballot(hit);
+      if (hit) members[base + __popcll(m & below)] = j;
+      base += __popcll(m);
+    }
+  }
+}
+
+// Number of tiles of each set-1 molecule i (one thread per i): per size
+// class, the class's member run (for the triangular form only members
+// j >= i, found by bisection; members ascend within a class) in groups of J.
+__global__ __launch_bounds__(256) void k_plan_counts(const int32_t *__restrict__ n1, int nm1,
+                                                     int amax, int tri, int nsig, int lanes,
+                                                     int j_recs_cap,
+                                                     const int32_t *__restrict__ cls_off,
+                                                     const int32_t *__restrict__ members,
+                                                     int32_t *__restrict__ counts) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nm1) return;
+  const int ni = size_class(n1[i], amax);
+  int t = 0;
+  for (int s = amax; s >= 0; --s) {
+    int lo = cls_off[s];
+    const int hi = cls_off[s + 1];
+    if (tri) {
+      int a = lo, b = hi;
+      while (a < b) {
+        const int m = (a + b) >> 1;
+        if (members[m] < i) a = m + 1; else b = m;
+      }
+      lo = a;
+    }
+    if (lo < hi) {
+      const int J = tile_j(ni, s, nsig, lanes, j_recs_cap);
+      t += (hi - lo + J - 1) / J;
+    }
+  }
+  counts[i] = t;
+}
+
+// tile_start[i] = sum of counts[0..i) for i = 0..nm1 (tile_start[nm1] = the
+// number of tiles).  One workgroup of 1024 threads; a wave-shuffle scan per
+// 1024-entry chunk, chunks chained through a 64-bit carry.
+__global__ __launch_bounds__(1024) void k_plan_scan(const int32_t *__restrict__ counts, int nm1,
+                                                    int64_t *__restrict__ tile_start) {
+  __shared__ int wsum[16];
+  __shared__ int wpre[16];
+  __shared__ int64_t carry;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (tid == 0) carry = 0;
+  __syncthreads();
+  for (int base = 0; base < nm1; base += 1024) {
+    const int i = base + tid;
+    const int v = i < nm1 ? counts[i] : 0;
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int y = __shfl_up(x, o, 64);
+      if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[wave] = x;
+    __syncthreads();
+    if (tid == 0) {
+      int a = 0;
+      for (int w = 0; w < 16; ++w) {
+        wpre[w] = a;
+        a += wsum[w];
+      }
+      wsum[0] = a;  // chunk total, read below before the carry moves
+    }
+    __syncthreads();
+    if (i < nm1) tile_start[i] = carry + (int64_t)(wpre[wave] + x - v);
+    __syncthreads();
+    if (tid == 0) carry += wsum[0];
+    __syncthreads();
+  }
+  if (tid == 0) tile_start[nm1] = carry;
 }
 
 // Tile list on the device, one wave per set-1 molecule i: its tiles start at
@@ -789,13 +903,14 @@ __global__ __launch_bounds__(kPairThreads, kWaves) void k_arad_tiles(NbSet set1,
   const int amax = ta.amax;
   const TC inv_w = (TC)p.inv_w, maxg = (TC)p.maxg;
   const int nsig = (ta.mode == FMLH_ARAD_KERNEL) ? 1 : ta.nsig;
+  const int64_t ntiles = ta.tile_start[ta.nm1];
   int cur_i = -1;
   for (;;) {
     __syncthreads();  // previous tile's readers are done with stages/red
     if (threadIdx.x == 0) tile_slot[0] = atomicAdd(ta.counter, 1);
     __syncthreads();
     const int tix = tile_slot[0];
-    if (tix >= ta.ntiles) break;
+    if (tix >= ntiles) break;
     const Tile tl = ta.tiles[tix];
     const int i = tl.i, J = tl.cnt, s = tl.s;
     const int ni = min(max(ta.n1[i], 0), amax);
@@ -840,24 +955,25 @@ __global__ __launch_bounds__(kPairThreads, kWaves) void k_arad_tiles(NbSet set1,
       D3 = nr == 3 ? D : D3;
     }
     __syncthreads();  // the J stage becomes the reduction buffer
-    // per sigma: the sum over this thread's pairs in round order
-    // (farad_kernels.f90:603; :211 for the single-sigma form)
+    // per sigma, one exponential per atom pair t = jj P + a + n_i b of the
+    // tile, stored at red[k total + t]
     for (int k = 0; k < nsig; ++k) {
       const double is2 = ta.inv_s2[k];
-      double v = 0.0;
-      if (nr > 0) v += exp(D0 * is2);
-      if (nr > 1) v += exp(D1 * is2);
-      if (nr > 2) v += exp(D2 * is2);
-      if (nr > 3) v += exp(D3 * is2);
-      red[k * kPairThreads + threadIdx.x] = v;
+      double *rk = red + k * total + threadIdx.x;
+      if (nr > 0) rk[0] = exp(D0 * is2);
+      if (nr > 1) rk[kPairThreads] = exp(D1 * is2);
+      if (nr > 2) rk[2 * kPairThreads] = exp(D2 * is2);
+      if (nr > 3) rk[3 * kPairThreads] = exp(D3 * is2);
     }
     __syncthreads();
+    // a molecule pair's sum runs over its atom pairs in the reference's
+    // order, a (set-1 atom) inner and b outer (farad_kernels.f90:602-604;
+    // :211 for the single-sigma form), whatever the tile shape
     if ((int)threadIdx.x < J * nsig) {
       const int jj = threadIdx.x / nsig, k = threadIdx.x % nsig;
-      const int l0 = (J == 1) ? 0 : jj * P;
-      const int l1 = (J == 1) ? min(P, kPairThreads) : (jj + 1) * P;
+      const double *rk = red + k * total;
       double v = 0.0;
-      for (int l = l0; l < l1; ++l) v += red[k * kPairThreads + l];
+      for (int l = jj * P; l < (jj + 1) * P; ++l) v += rk[l];
       const int j = ta.members[tl.off + jj];
       if (ta.mode == FMLH_ARAD_KERNEL) {
         ta.out[(int64_t)i + (int64_t)ta.nm1 * j] = v;
@@ -976,116 +1092,65 @@ void launch_tiles(int grid, size_t lds, hipStream_t st, const NbSet &s1, const N
                      s2, ta, p, ctl);
 }
 
-// ---- tile planning (host side of it) ---------------------------------
-
-// What the host derives from the atom counts alone: size classes of set 2,
-// the first tile of each set-1 molecule, the LDS the largest tile needs.
-// The tile list itself is written on the device by k_plan_emit.
-struct TilePlan {
-  std::vector<int32_t> cls_off;     // amax + 2
-  std::vector<int32_t> members;     // set-2 indices grouped by atom count
-  std::vector<int64_t> tile_start;  // nm1 + 1
-  int max_j_recs = 0, max_j_atoms = 0;
-};
-
-void plan_tiles(const int32_t *n1h, int nm1, const int32_t *n2h, int nm2, int amax, bool tri,
-                int nsig, int lanes, int j_recs_cap, TilePlan &P) {
-  P.cls_off.assign(amax + 2, 0);
-  for (int j = 0; j < nm2; ++j) P.cls_off[std::min(std::max(n2h[j], 0), amax) + 1]++;
-  for (int s = 0; s <= amax; ++s) P.cls_off[s + 1] += P.cls_off[s];
-  P.members.assign(nm2, 0);
-  {
-    std::vector<int> fill(P.cls_off.begin(), P.cls_off.end() - 1);
-    for (int j = 0; j < nm2; ++j) P.members[fill[std::min(std::max(n2h[j], 0), amax)]++] = j;
-  }
-  // per class, the first member >= i (moves forward as i grows)
-  std::vector<int> first(P.cls_off.begin(), P.cls_off.end() - 1);
-  P.tile_start.assign((size_t)nm1 + 1, 0);
-  P.max_j_recs = P.max_j_atoms = 0;
-  for (int i = 0; i < nm1; ++i) {
-    const int ni = std::min(std::max(n1h[i], 0), amax);
-    int64_t t = 0;
-    for (int s = amax; s >= 0; --s) {
-      const int hi = P.cls_off[s + 1];
-      int lo = P.cls_off[s];
-      if (tri) {
-        while (first[s] < hi && P.members[first[s]] < i) ++first[s];
-        lo = first[s];
-      }
-      if (lo >= hi) continue;
-      const int J = tile_j(ni, s, nsig, lanes, j_recs_cap);
-      t += (hi - lo + J - 1) / J;
-      if (ni * s > 0) {
-        const int c = std::min(J, hi - lo);
-        P.max_j_recs = std::max(P.max_j_recs, c * s * s);
-        P.max_j_atoms = std::max(P.max_j_atoms, c * s);
-      }
-    }
-    P.tile_start[(size_t)i + 1] = P.tile_start[i] + t;
-  }
-}
+// ---- tile shapes ------------------------------------------------------
 
 struct TileLds {
   int j_recs, j_atoms;
   size_t bytes;
 };
 
-TileLds tile_lds(int amax, int max_ni, int j_recs, int j_atoms, int nsig) {
+// LDS of the tile kernel for the largest J stage any tile can need: the
+// planner runs on the device, so the host bounds J * s^2 records and J * s
+// atoms over every (n_i, s) the tile shape allows.
+TileLds tile_lds(int amax, int nsig, int lanes) {
+  int j_recs = 1, j_atoms = 1;
+  for (int ni = 0; ni <= amax; ++ni)
+    for (int s = 1; s <= amax; ++s) {
+      const int J = tile_j(ni, s, nsig, lanes, amax * amax);
+      j_recs = std::max(j_recs, J * s * s);
+      j_atoms = std::max(j_atoms, J * s);
+    }
+  // largest atom-pair count J n_i s of a tile (at most kTileRounds x 256)
+  int pairs = 1;
+  for (int ni = 1; ni <= amax; ++ni)
+    for (int s = 1; s <= amax; ++s) pairs = std::max(pairs, tile_j(ni, s, nsig, lanes, amax * amax) * ni * s);
   TileLds L;
-  L.j_atoms = std::max(j_atoms, 1);
-  // the J stage doubles as the reduction buffer (nsig x 256 doubles)
-  const size_t red = sizeof(double) * kPairThreads * std::max(nsig, 1);
-  int jr = std::max(j_recs, 1) + 2;  // two padding records for the band loop's read-ahead
-  size_t jb = sizeof(Rec) * (size_t)jr + 32 * (size_t)L.j_atoms +
-              sizeof(uint16_t) * (((size_t)jr + 7) & ~size_t(7));
-  while (jb < red) {  // grow the record area until the stage covers the buffer
-    jr += 64;
-    jb = sizeof(Rec) * (size_t)jr + 32 * (size_t)L.j_atoms + sizeof(uint16_t) * (((size_t)jr + 7) & ~size_t(7));
-  }
+  L.j_atoms = j_atoms;
+  // the J stage doubles as the reduction buffer (nsig x pairs doubles)
+  const size_t red = sizeof(double) * (size_t)pairs * std::max(nsig, 1);
+  int jr = j_recs + 2;  // two padding records for the band loop's read-ahead
+  auto stage = [&](int r) {
+    return sizeof(Rec) * (size_t)r + 32 * (size_t)L.j_atoms +
+           sizeof(uint16_t) * (((size_t)r + 7) & ~size_t(7));
+  };
+  while (stage(jr) < red) jr += 64;  // grow the record area until the stage covers the buffer
   L.j_recs = jr;
-  L.bytes = kTabBytes + 16 + 32 * (size_t)amax + jb;
+  L.bytes = kTabBytes + 16 + 32 * (size_t)amax + stage(jr);
   return L;
 }
 
-// Pinned staging of the plan for the asynchronous upload; one upload is in
-// flight at a time (the next call waits on the event of the previous one).
-std::mutex g_plan_mu;
-void *g_plan_host = nullptr;
-size_t g_plan_cap = 0;
-hipEvent_t g_plan_done = nullptr;
+// Process-wide tiling table: (mode kind, amax, precision, nsig) -> tiling,
+// filled by the autotuner or by fmlh_arad_tiling_set.
+std::mutex g_tune_mu;
+std::map<std::tuple<int, int, int, int>, fmlh_arad_tiling> g_tune;
 
-int upload_plan(const TilePlan &P, int32_t *dcls, int32_t *dmembers, int64_t *dstart,
-                hipStream_t st) {
-  std::lock_guard<std::mutex> lk(g_plan_mu);
-  const size_t sb = sizeof(int64_t) * P.tile_start.size();
-  const size_t cb = sizeof(int32_t) * P.cls_off.size(), mb = sizeof(int32_t) * P.members.size();
-  if (g_plan_done) {
-    hipError_t e = hipEventSynchronize(g_plan_done);
-    if (e != hipSuccess) return record_error(e, "hipEventSynchronize(plan)");
-  } else {
-    hipError_t e = hipEventCreateWithFlags(&g_plan_done, hipEventDisableTiming);
-    if (e != hipSuccess) return record_error(e, "hipEventCreate(plan)");
-  }
-  if (sb + cb + mb > g_plan_cap) {
-    if (g_plan_host) hipHostFree(g_plan_host);
-    g_plan_cap = (sb + cb + mb) * 2 + 4096;
-    hipError_t e = hipHostMalloc(&g_plan_host, g_plan_cap, hipHostMallocDefault);
-    if (e != hipSuccess) {
-      g_plan_host = nullptr;
-      g_plan_cap = 0;
-      return record_error(e, "hipHostMalloc(plan)");
-    }
-  }
-  char *h = static_cast<char *>(g_plan_host);
-  memcpy(h, P.tile_start.data(), sb);
-  memcpy(h + sb, P.cls_off.data(), cb);
-  if (mb) memcpy(h + sb + cb, P.members.data(), mb);
-  hipError_t e = hipMemcpyAsync(dstart, h, sb, hipMemcpyHostToDevice, st);
-  if (e == hipSuccess) e = hipMemcpyAsync(dcls, h + sb, cb, hipMemcpyHostToDevice, st);
-  if (e == hipSuccess && mb) e = hipMemcpyAsync(dmembers, h + sb + cb, mb, hipMemcpyHostToDevice, st);
-  if (e == hipSuccess) e = hipEventRecord(g_plan_done, st);
-  if (e != hipSuccess) return record_error(e, "hipMemcpyAsync(plan)");
-  return 0;
+std::tuple<int, int, int, int> tune_key(int mode, int amax, int precision, int nsig) {
+  const int kind = (mode == FMLH_ARAD_GLOBAL) ? 0 : 1;  // rectangular / triangular tile list
+  return std::make_tuple(kind, amax, precision, mode == FMLH_ARAD_KERNEL ? 1 : nsig);
+}
+
+fmlh_arad_tiling default_tiling() { return fmlh_arad_tiling{kPairThreads, 0, 16}; }
+
+fmlh_arad_tiling lookup_tiling(int mode, int amax, int precision, int nsig) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  auto it = g_tune.find(tune_key(mode, amax, precision, nsig));
+  return it == g_tune.end() ? default_tiling() : it->second;
+}
+
+bool tiling_ok(const fmlh_arad_tiling &t) {
+  return t.pairs_per_tile >= 64 && t.pairs_per_tile <= kTileRounds * kPairThreads &&
+         (t.waves_per_simd == 0 || t.waves_per_simd == 4 || t.waves_per_simd == 5) &&
+         t.groups_per_cu >= 1 && t.groups_per_cu <= 64;
 }
 
 int64_t max_tiles(int nm1, int nm2, bool tri) {
@@ -1148,11 +1213,12 @@ size_t fmlh_arad_ws_bytes(int nm1, int nm2, int amax, int mode) {
   const bool one_set = (mode == FMLH_ARAD_SYMMETRIC || mode == FMLH_ARAD_KERNEL);
   size_t b = WsCarver::need(sizeof(Ctl)) + set_bytes((int64_t)nm1 * amax, amax);
   if (!one_set) b += set_bytes((int64_t)nm2 * amax, amax);
-  if (mode != FMLH_ARAD_DISTANCE)  // tile plan: tiles, members, classes, starts, counter
+  if (mode != FMLH_ARAD_DISTANCE)  // tiles, members, classes, starts, counts, counter
     b += WsCarver::need(sizeof(Tile) * (size_t)max_tiles(nm1, one_set ? nm1 : nm2, one_set)) +
          WsCarver::need(sizeof(int32_t) * (size_t)(one_set ? nm1 : nm2)) +
          WsCarver::need(sizeof(int32_t) * (size_t)(amax + 2)) +
-         WsCarver::need(sizeof(int64_t) * ((size_t)nm1 + 1)) + WsCarver::need(64);
+         WsCarver::need(sizeof(int64_t) * ((size_t)nm1 + 1)) +
+         WsCarver::need(sizeof(int32_t) * (size_t)nm1) + WsCarver::need(64);
   return b;
 }
 
@@ -1161,12 +1227,12 @@ size_t fmlh_arad_atomic_ws_bytes(int na1, int na2, int amax) {
   return WsCarver::need(sizeof(Ctl)) + set_bytes(na1, amax) + set_bytes(na2, amax);
 }
 
-int fmlh_arad_kernels(int mode, const double *q1, const double *zt1, const int32_t *n1, int nm1,
-                      const double *q2, const double *zt2, const int32_t *n2, int nm2, int amax,
-                      const double *sigmas_host, int nsig, double width, double cut,
-                      double r_width, double c_width, int precision, double *out, void *ws,
-                      size_t ws_bytes, const int32_t *n1_host, const int32_t *n2_host,
-                      void *stream) {
+}  // extern "C"
+
+namespace {
+
+int arad_check(int mode, int nm1, int &nm2, int amax, const double *sigmas_host, int nsig,
+               double width, double cut, int precision) {
   FMLH_ARG_CHECK(mode == FMLH_ARAD_GLOBAL || mode == FMLH_ARAD_SYMMETRIC ||
                      mode == FMLH_ARAD_KERNEL || mode == FMLH_ARAD_DISTANCE,
                  1);
@@ -1184,15 +1250,22 @@ int fmlh_arad_kernels(int mode, const double *q1, const double *zt1, const int32
   FMLH_ARG_CHECK(width > 0.0 || mode == FMLH_ARAD_KERNEL, 13);
   FMLH_ARG_CHECK(cut > 0.0 || mode == FMLH_ARAD_KERNEL, 14);
   FMLH_ARG_CHECK(precision == 0 || precision == 1, 17);
-  if (nm1 == 0 || nm2 == 0) return 0;
-  FMLH_ARG_CHECK(q1 && zt1 && n1, 2);
-  if (!one_set) FMLH_ARG_CHECK(q2 && zt2 && n2, 6);
-  FMLH_ARG_CHECK(out != nullptr, 18);
-  FMLH_ARG_CHECK(ws != nullptr && ws_bytes >= fmlh_arad_ws_bytes(nm1, nm2, amax, mode), 19);
+  return 0;
+}
 
+bool uses_tiles(int mode, int amax) {
+  return mode != FMLH_ARAD_DISTANCE && amax <= kMaxAmax && amax * amax <= kTileRounds * kPairThreads;
+}
+
+// Body of fmlh_arad_kernels after argument checks; t is a valid tiling.
+int arad_run(int mode, const double *q1, const double *zt1, const int32_t *n1, int nm1,
+             const double *q2, const double *zt2, const int32_t *n2, int nm2, int amax,
+             const double *sigmas_host, int nsig, double width, double cut, double r_width,
+             double c_width, int precision, double *out, void *ws, size_t ws_bytes,
+             const fmlh_arad_tiling &t, hipStream_t st) {
+  const bool one_set = (mode == FMLH_ARAD_SYMMETRIC || mode == FMLH_ARAD_KERNEL);
   const AradParams p = (mode == FMLH_ARAD_KERNEL) ? make_params(0.2, 6.0, 1.0, 0.5, true)
                                                   : make_params(width, cut, r_width, c_width, false);
-  hipStream_t st = as_stream(stream);
   NbSet s1, s2;
   Ctl *ctl = nullptr;
   const int guard = (mode == FMLH_ARAD_GLOBAL) ? 1 : 0;  // farad_kernels.f90:526,539
@@ -1215,87 +1288,74 @@ int fmlh_arad_kernels(int mode, const double *q1, const double *zt1, const int32
   pa.n2 = one_set ? n1 : n2;
   pa.out = out;
 
-  const int64_t cap = (int64_t)device_cus() * 16;
-
-  // Molecular kernel modes: size-sorted tiles when the molecules fit the LDS
-  // stages (every amax up to 29 does).
-  if (mode != FMLH_ARAD_DISTANCE && amax <= 29 &&
-      amax * amax <= kTileRounds * kPairThreads) {
-    std::vector<int32_t> h1, h2;
-    const int32_t *n1h = n1_host, *n2h = one_set ? n1_host : n2_host;
-    if (!n1h || !n2h) {  // fetch the atom counts (synchronises the stream)
-      h1.resize(nm1);
-      hipError_t e = hipMemcpyAsync(h1.data(), n1, sizeof(int32_t) * nm1, hipMemcpyDeviceToHost, st);
-      if (e == hipSuccess && !one_set) {
-        h2.resize(nm2);
-        e = hipMemcpyAsync(h2.data(), n2, sizeof(int32_t) * nm2, hipMemcpyDeviceToHost, st);
-      }
-      if (e == hipSuccess) e = hipStreamSynchronize(st);
-      if (e != hipSuccess) return record_error(e, "atom counts to host");
-      n1h = h1.data();
-      n2h = one_set ? h1.data() : h2.data();
-    }
+  if (uses_tiles(mode, amax)) {
     const int nsig_eff = (mode == FMLH_ARAD_KERNEL) ? 1 : nsig;
-    TilePlan plan;
-    plan_tiles(n1h, nm1, n2h, nm2, amax, one_set, nsig_eff, kPairThreads, amax * amax, plan);
-    const int64_t ntiles = plan.tile_start[(size_t)nm1];
-    TileLds L = tile_lds(amax, 0, plan.max_j_recs, plan.max_j_atoms, nsig_eff);
-    if (L.bytes <= kLdsLimit && ntiles < INT32_MAX) {
-      WsCarver w(ws, ws_bytes);
-      w.take<Ctl>(1);
-      NbSet d1, d2;
-      carve_set(w, (int64_t)nm1 * amax, amax, d1);
-      if (!one_set) carve_set(w, (int64_t)nm2 * amax, amax, d2);
-      Tile *dt = w.take<Tile>((size_t)max_tiles(nm1, one_set ? nm1 : nm2, one_set));
-      int32_t *dm = w.take<int32_t>((size_t)(one_set ? nm1 : nm2));
-      int32_t *dcls = w.take<int32_t>((size_t)(amax + 2));
-      int64_t *dstart = w.take<int64_t>((size_t)nm1 + 1);
-      int32_t *counter = w.take<int32_t>(16);
-      if (!dt || !dm || !dcls || !dstart || !counter) return -19;
-      rc = upload_plan(plan, dcls, dm, dstart, st);
-      if (rc) return rc;
-      if (nm1 > 0) {
-        hipLaunchKernelGGL(k_plan_emit, dim3(nm1), dim3(64), 0, st, n1, nm1, amax, one_set ? 1 : 0,
-                           nsig_eff, kPairThreads, amax * amax, dcls, dm, dstart, dt);
-        FMLH_LAUNCH_CHECK("k_plan_emit");
-      }
-      hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, st, counter);
-      FMLH_LAUNCH_CHECK("k_zero_counter");
-      TileArgs ta;
-      ta.mode = mode;
-      ta.nm1 = nm1;
-      ta.amax = amax;
-      ta.nsig = pa.nsig;
-      ta.triangular = pa.triangular;
-      ta.ntiles = (int)ntiles;
-      ta.stage_j_recs = L.j_recs;
-      ta.stage_j_atoms = L.j_atoms;
-      for (int k = 0; k < kMaxSig; ++k) ta.inv_s2[k] = pa.inv_s2[k];
-      ta.n1 = n1;
-      ta.tiles = dt;
-      ta.members = dm;
-      ta.counter = counter;
-      ta.out = out;
-      const int grid = (int)std::min<int64_t>(std::max<int64_t>(ta.ntiles, 1), cap);
-      const bool five = L.bytes * 5 <= kLdsPerCu;
-      timing_begin(st);
-      if (precision) {
-        if (five)
-          launch_tiles<float, 5>(grid, L.bytes, st, s1, s2, ta, p, ctl);
-        else
-          launch_tiles<float, 4>(grid, L.bytes, st, s1, s2, ta, p, ctl);
-      } else {
-        if (five)
-          launch_tiles<double, 5>(grid, L.bytes, st, s1, s2, ta, p, ctl);
-        else
-          launch_tiles<double, 4>(grid, L.bytes, st, s1, s2, ta, p, ctl);
-      }
-      timing_end(st);
-      FMLH_LAUNCH_CHECK("k_arad_tiles");
-      return 0;
+    const int lanes = t.pairs_per_tile, jcap = amax * amax;
+    const TileLds L = tile_lds(amax, nsig_eff, lanes);
+    if (L.bytes > kLdsLimit) return -10;
+    WsCarver w(ws, ws_bytes);
+    w.take<Ctl>(1);
+    NbSet d1, d2;
+    carve_set(w, (int64_t)nm1 * amax, amax, d1);
+    if (!one_set) carve_set(w, (int64_t)nm2 * amax, amax, d2);
+    const int nm2e = one_set ? nm1 : nm2;
+    Tile *dt = w.take<Tile>((size_t)max_tiles(nm1, nm2e, one_set));
+    int32_t *dm = w.take<int32_t>((size_t)nm2e);
+    int32_t *dcls = w.take<int32_t>((size_t)(amax + 2));
+    int64_t *dstart = w.take<int64_t>((size_t)nm1 + 1);
+    int32_t *dcount = w.take<int32_t>((size_t)nm1);
+    int32_t *counter = w.take<int32_t>(16);
+    if (!dt || !dm || !dcls || !dstart || !dcount || !counter) return -19;
+    const int32_t *n2e = one_set ? n1 : n2;
+    // the plan: size classes, tiles per set-1 molecule, their prefix, the list
+    hipLaunchKernelGGL(k_plan_classes, dim3(1), dim3(1024), 0, st, n2e, nm2e, amax, dcls, dm);
+    FMLH_LAUNCH_CHECK("k_plan_classes");
+    hipLaunchKernelGGL(k_plan_counts, dim3(ceil_div(nm1, 256)), dim3(256), 0, st, n1, nm1, amax,
+                       one_set ? 1 : 0, nsig_eff, lanes, jcap, dcls, dm, dcount);
+    FMLH_LAUNCH_CHECK("k_plan_counts");
+    hipLaunchKernelGGL(k_plan_scan, dim3(1), dim3(1024), 0, st, dcount, nm1, dstart);
+    FMLH_LAUNCH_CHECK("k_plan_scan");
+    hipLaunchKernelGGL(k_plan_emit, dim3(nm1), dim3(64), 0, st, n1, nm1, amax, one_set ? 1 : 0,
+                       nsig_eff, lanes, jcap, dcls, dm, dstart, dt);
+    FMLH_LAUNCH_CHECK("k_plan_emit");
+    hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, st, counter);
+    FMLH_LAUNCH_CHECK("k_zero_counter");
+    TileArgs ta;
+    ta.mode = mode;
+    ta.nm1 = nm1;
+    ta.amax = amax;
+    ta.nsig = pa.nsig;
+    ta.triangular = pa.triangular;
+    ta.tile_start = dstart;
+    ta.stage_j_recs = L.j_recs;
+    ta.stage_j_atoms = L.j_atoms;
+    for (int k = 0; k < kMaxSig; ++k) ta.inv_s2[k] = pa.inv_s2[k];
+    ta.n1 = n1;
+    ta.tiles = dt;
+    ta.members = dm;
+    ta.counter = counter;
+    ta.out = out;
+    const int64_t tmax = std::max<int64_t>(max_tiles(nm1, nm2e, one_set), 1);
+    const int grid = (int)std::min<int64_t>(tmax, (int64_t)device_cus() * t.groups_per_cu);
+    const bool five = t.waves_per_simd == 5 || (t.waves_per_simd == 0 && L.bytes * 5 <= kLdsPerCu);
+    timing_begin(st);
+    if (precision) {
+      if (five)
+        launch_tiles<float, 5>(grid, L.bytes, st, s1, s2, ta, p, ctl);
+      else
+        launch_tiles<float, 4>(grid, L.bytes, st, s1, s2, ta, p, ctl);
+    } else {
+      if (five)
+        launch_tiles<double, 5>(grid, L.bytes, st, s1, s2, ta, p, ctl);
+      else
+        launch_tiles<double, 4>(grid, L.bytes, st, s1, s2, ta, p, ctl);
     }
+    timing_end(st);
+    FMLH_LAUNCH_CHECK("k_arad_tiles");
+    return 0;
   }
 
+  const int64_t cap = (int64_t)device_cus() * 16;
   const bool stage = pair_lds_bytes(amax, true) <= kLdsLimit;
   const size_t lds = pair_lds_bytes(amax, stage);
   if (lds > kLdsLimit) return -10;
@@ -1318,6 +1378,116 @@ int fmlh_arad_kernels(int mode, const double *q1, const double *zt1, const int32
   }
   timing_end(st);
   FMLH_LAUNCH_CHECK("k_arad_pairs");
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fmlh_arad_kernels(int mode, const double *q1, const double *zt1, const int32_t *n1, int nm1,
+                      const double *q2, const double *zt2, const int32_t *n2, int nm2, int amax,
+                      const double *sigmas_host, int nsig, double width, double cut,
+                      double r_width, double c_width, int precision, double *out, void *ws,
+                      size_t ws_bytes, const fmlh_arad_tiling *tiling, void *stream) {
+  int rc = arad_check(mode, nm1, nm2, amax, sigmas_host, nsig, width, cut, precision);
+  if (rc) return rc;
+  const bool one_set = (mode == FMLH_ARAD_SYMMETRIC || mode == FMLH_ARAD_KERNEL);
+  if (tiling) FMLH_ARG_CHECK(tiling_ok(*tiling), 21);
+  if (nm1 == 0 || nm2 == 0) return 0;
+  FMLH_ARG_CHECK(q1 && zt1 && n1, 2);
+  if (!one_set) FMLH_ARG_CHECK(q2 && zt2 && n2, 6);
+  FMLH_ARG_CHECK(out != nullptr, 18);
+  FMLH_ARG_CHECK(ws != nullptr && ws_bytes >= fmlh_arad_ws_bytes(nm1, nm2, amax, mode), 19);
+  const fmlh_arad_tiling t = tiling ? *tiling : lookup_tiling(mode, amax, precision, nsig);
+  return arad_run(mode, q1, zt1, n1, nm1, q2, zt2, n2, nm2, amax, sigmas_host, nsig, width, cut,
+                  r_width, c_width, precision, out, ws, ws_bytes, t, as_stream(stream));
+}
+
+int fmlh_arad_autotune(int mode, const double *q1, const double *zt1, const int32_t *n1, int nm1,
+                       const double *q2, const double *zt2, const int32_t *n2, int nm2, int amax,
+                       const double *sigmas_host, int nsig, double width, double cut,
+                       double r_width, double c_width, int precision, double *out, void *ws,
+                       size_t ws_bytes, int reps, fmlh_arad_tiling *best, double *best_ms,
+                       void *stream) {
+  int rc = arad_check(mode, nm1, nm2, amax, sigmas_host, nsig, width, cut, precision);
+  if (rc) return rc;
+  const bool one_set = (mode == FMLH_ARAD_SYMMETRIC || mode == FMLH_ARAD_KERNEL);
+  FMLH_ARG_CHECK(reps >= 1 && reps <= 1000, 21);
+  FMLH_ARG_CHECK(best != nullptr, 22);
+  FMLH_ARG_CHECK(uses_tiles(mode, amax), 1);
+  FMLH_ARG_CHECK(nm1 > 0 && nm2 > 0, 5);
+  FMLH_ARG_CHECK(q1 && zt1 && n1, 2);
+  if (!one_set) FMLH_ARG_CHECK(q2 && zt2 && n2, 6);
+  FMLH_ARG_CHECK(out != nullptr, 18);
+  FMLH_ARG_CHECK(ws != nullptr && ws_bytes >= fmlh_arad_ws_bytes(nm1, nm2, amax, mode), 19);
+  hipStream_t st = as_stream(stream);
+  const int nsig_eff = (mode == FMLH_ARAD_KERNEL) ? 1 : nsig;
+  std::vector<fmlh_arad_tiling> cand;
+  for (int lanes : {128, 256, 384, 512, 768, 1024}) {
+    const TileLds L = tile_lds(amax, nsig_eff, lanes);
+    if (L.bytes > kLdsLimit) continue;
+    cand.push_back(fmlh_arad_tiling{lanes, 4, 16});
+    if (L.bytes * 5 <= kLdsPerCu) cand.push_back(fmlh_arad_tiling{lanes, 5, 16});
+  }
+  if (cand.empty()) return -10;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e != hipSuccess) return record_error(e, "hipEventCreate(autotune)");
+  float best_t = 0.f;
+  int best_k = -1;
+  for (size_t k = 0; k < cand.size() && rc == 0; ++k) {
+    rc = arad_run(mode, q1, zt1, n1, nm1, q2, zt2, n2, nm2, amax, sigmas_host, nsig, width, cut,
+                  r_width, c_width, precision, out, ws, ws_bytes, cand[k], st);
+    if (rc) break;
+    e = hipEventRecord(e0, st);
+    for (int r = 0; r < reps && rc == 0 && e == hipSuccess; ++r)
+      rc = arad_run(mode, q1, zt1, n1, nm1, q2, zt2, n2, nm2, amax, sigmas_host, nsig, width, cut,
+                    r_width, c_width, precision, out, ws, ws_bytes, cand[k], st);
+    if (e == hipSuccess) e = hipEventRecord(e1, st);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e != hipSuccess) {
+      rc = record_error(e, "autotune timing");
+      break;
+    }
+    if (best_k < 0 || ms < best_t) {
+      best_t = ms;
+      best_k = (int)k;
+    }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (rc) return rc;
+  // leave `out` computed with the chosen tiling
+  rc = arad_run(mode, q1, zt1, n1, nm1, q2, zt2, n2, nm2, amax, sigmas_host, nsig, width, cut,
+                r_width, c_width, precision, out, ws, ws_bytes, cand[best_k], st);
+  if (rc) return rc;
+  *best = cand[best_k];
+  if (best_ms) *best_ms = best_t / reps;
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune[tune_key(mode, amax, precision, nsig)] = cand[best_k];
+  }
+  return 0;
+}
+
+int fmlh_arad_tiling_get(int mode, int amax, int precision, int nsig, fmlh_arad_tiling *out) {
+  FMLH_ARG_CHECK(out != nullptr, 5);
+  *out = lookup_tiling(mode, amax, precision, nsig);
+  return 0;
+}
+
+int fmlh_arad_tiling_set(int mode, int amax, int precision, int nsig, const fmlh_arad_tiling *t) {
+  FMLH_ARG_CHECK(mode >= FMLH_ARAD_GLOBAL && mode <= FMLH_ARAD_KERNEL, 1);
+  FMLH_ARG_CHECK(amax >= 1 && amax <= kMaxAmax, 2);
+  FMLH_ARG_CHECK(precision == 0 || precision == 1, 3);
+  FMLH_ARG_CHECK(nsig >= 1 && nsig <= kMaxSig, 4);
+  FMLH_ARG_CHECK(t != nullptr && tiling_ok(*t), 5);
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune[tune_key(mode, amax, precision, nsig)] = *t;
   return 0;
 }
 

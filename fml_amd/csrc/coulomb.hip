@@ -159,6 +159,8 @@ __global__ __launch_bounds__(64) void k_coulomb_sorted(
 
 }  // namespace
 
+extern "C" int fmlh_init(void) { return ensure_tables(); }
+
 extern "C" int fmlh_coulomb_sorted(const double *z, const double *r, const int32_t *offsets,
                                    int nmol, int nmax, double *cm, float *cm32, int32_t *perm,
                                    int32_t *status, void *stream) {

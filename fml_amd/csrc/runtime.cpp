@@ -12,13 +12,17 @@ thread_local std::string g_last_error;
 namespace fmlh {
 int record_error(hipError_t e, const char *where) {
   g_last_error = std::string(where) + ": " + hipGetErrorString(e);
+  // the error is reported through the return code: clear the runtime's
+  // per-thread last-error state so it does not surface in a later,
+  // unrelated call of the caller (torch checks it after its own calls)
+  (void)hipGetLastError();
   return FMLH_EHIP;
 }
 }  // namespace fmlh
 
 extern "C" {
 
-int fmlh_abi_version(void) { return 1; }
+int fmlh_abi_version(void) { return 2; }
 
 const char *fmlh_last_error(void) { return g_last_error.c_str(); }
 
@@ -53,6 +57,11 @@ namespace fmlh {
 void timing_begin(hipStream_t st) {
   std::lock_guard<std::mutex> lk(g_tmu);
   if (!g_timing) return;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) {
+    (void)hipGetLastError();
+    return;  // no timing events inside a graph being captured
+  }
   hipEvent_t a = nullptr, b = nullptr;
   if (!g_free.empty()) {
     a = g_free.back().first;
@@ -85,17 +94,108 @@ int fmlh_timing_collect(double *total_ms, int *count) {
   double tot = 0.0;
   int n = 0;
   for (auto &pr : g_pairs) {
-    if (hipEventSynchronize(pr.second) != hipSuccess) continue;
     float ms = 0.f;
-    if (hipEventElapsedTime(&ms, pr.first, pr.second) == hipSuccess) {
+    if (hipEventSynchronize(pr.second) == hipSuccess &&
+        hipEventElapsedTime(&ms, pr.first, pr.second) == hipSuccess) {
       tot += ms;
       ++n;
+    } else {
+      (void)hipGetLastError();  // a pair that never ran: skipped, not reported
     }
     g_free.push_back(pr);
   }
   g_pairs.clear();
   if (total_ms) *total_ms = tot;
   if (count) *count = n;
+  return 0;
+}
+
+}  // extern "C"
+
+// ---- HIP graphs ------------------------------------------------------------
+// Thread-local stream capture: only this thread's calls on the capturing
+// stream are recorded, and other threads (a serving loop's second stream,
+// torch's own work) keep running eagerly.
+
+struct fmlh_graph {
+  hipGraph_t graph;
+  hipGraphExec_t exec;
+};
+
+extern "C" {
+
+int fmlh_graph_begin(void *stream) {
+  int rc = fmlh_init();
+  if (rc) return rc;
+  hipError_t e = hipStreamBeginCapture(fmlh::as_stream(stream), hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) return fmlh::record_error(e, "hipStreamBeginCapture");
+  return 0;
+}
+
+int fmlh_graph_end(void *stream, fmlh_graph **out) {
+  FMLH_ARG_CHECK(out != nullptr, 2);
+  *out = nullptr;
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(fmlh::as_stream(stream), &g);
+  if (e != hipSuccess) return fmlh::record_error(e, "hipStreamEndCapture");
+  hipGraphExec_t x = nullptr;
+  e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    hipGraphDestroy(g);
+    return fmlh::record_error(e, "hipGraphInstantiate");
+  }
+  *out = new fmlh_graph{g, x};
+  return 0;
+}
+
+int fmlh_graph_launch(fmlh_graph *graph, void *stream) {
+  FMLH_ARG_CHECK(graph != nullptr, 1);
+  hipError_t e = hipGraphLaunch(graph->exec, fmlh::as_stream(stream));
+  if (e != hipSuccess) return fmlh::record_error(e, "hipGraphLaunch");
+  return 0;
+}
+
+int fmlh_graph_destroy(fmlh_graph *graph) {
+  if (!graph) return 0;
+  hipError_t e1 = hipGraphExecDestroy(graph->exec);
+  hipError_t e2 = hipGraphDestroy(graph->graph);
+  delete graph;
+  if (e1 != hipSuccess) return fmlh::record_error(e1, "hipGraphExecDestroy");
+  if (e2 != hipSuccess) return fmlh::record_error(e2, "hipGraphDestroy");
+  return 0;
+}
+
+}  // extern "C"
+
+// ---- CU-partitioned streams ---------------------------------------------------
+// A stream whose kernels may only occupy the first `cu_count` compute units
+// of the device (all of them for cu_count <= 0).  Each such stream owns its
+// hardware queue, so work on two of them never waits in one queue; a serving
+// loop gives its fit work a stream that leaves some CUs to its predictions.
+
+#include <vector>
+
+extern "C" {
+
+int fmlh_stream_create(int cu_count, void **stream) {
+  FMLH_ARG_CHECK(stream != nullptr, 2);
+  *stream = nullptr;
+  const int cus = fmlh_device_cu_count();
+  if (cus <= 0) return fmlh::record_error(hipErrorInvalidDevice, "fmlh_stream_create(device)");
+  const int n = (cu_count <= 0 || cu_count > cus) ? cus : cu_count;
+  std::vector<uint32_t> mask((size_t)(cus + 31) / 32, 0u);
+  for (int i = 0; i < n; ++i) mask[(size_t)i / 32] |= 1u << (i % 32);
+  hipStream_t s = nullptr;
+  hipError_t e = hipExtStreamCreateWithCUMask(&s, (uint32_t)mask.size(), mask.data());
+  if (e != hipSuccess) return fmlh::record_error(e, "hipExtStreamCreateWithCUMask");
+  *stream = s;
+  return 0;
+}
+
+int fmlh_stream_destroy(void *stream) {
+  FMLH_ARG_CHECK(stream != nullptr, 1);
+  hipError_t e = hipStreamDestroy(fmlh::as_stream(stream));
+  if (e != hipSuccess) return fmlh::record_error(e, "hipStreamDestroy");
   return 0;
 }
 

@@ -122,9 +122,11 @@ def _sig_host(sigmas):
     return s
 
 
-def _arad_call(mode, q1, z1, n1, q2, z2, n2, sigmas, width, cut, rw, cw, precision=0):
+def _arad_call(mode, q1, z1, n1, q2, z2, n2, sigmas, width, cut, rw, cw, precision=0, tiling=None):
     """Runs one ARAD molecular-mode launch per chunk of <= 8 sigmas.
-    Returns the device output tensor in its C-order storage shape."""
+    Returns the device output tensor in its C-order storage shape.
+    tiling: an _native.AradTiling, or None for the library's current choice
+    (autotuned when fml_amd.tuning has run for this shape)."""
     dq1, dz1, dn1 = to_device(q1), to_device(z1), to_device(n1, np.int32)
     nm1, amax = int(dq1.shape[0]), int(dq1.shape[1])
     if q2 is None:
@@ -136,13 +138,12 @@ def _arad_call(mode, q1, z1, n1, q2, z2, n2, sigmas, width, cut, rw, cw, precisi
     lib = _native.load()
     wsb = lib.fmlh_arad_ws_bytes(nm1, nm2, amax, mode)
     ws = workspace(wsb)
-    h1, hp1 = _native.host_i32(n1)
-    h2, hp2 = _native.host_i32(n2)
+    tp = ctypes.byref(tiling) if tiling is not None else None
     if mode == _native.ARAD_DISTANCE:
         out = empty((amax, amax, nm2, nm1), np.float64)
         _native.call("fmlh_arad_kernels", mode, _native.ptr(dq1), _native.ptr(dz1), _native.ptr(dn1),
                      nm1, _native.ptr(dq2), _native.ptr(dz2), _native.ptr(dn2), nm2, amax, None, 0,
-                     width, cut, rw, cw, precision, _native.ptr(out), _native.ptr(ws), wsb, hp1, hp2,
+                     width, cut, rw, cw, precision, _native.ptr(out), _native.ptr(ws), wsb, tp,
                      _native.stream_ptr())
         return out
     sig = _sig_host(sigmas)
@@ -151,7 +152,7 @@ def _arad_call(mode, q1, z1, n1, q2, z2, n2, sigmas, width, cut, rw, cw, precisi
         arr = (ctypes.c_double * 1)(float(sig[0]))
         _native.call("fmlh_arad_kernels", mode, _native.ptr(dq1), _native.ptr(dz1), _native.ptr(dn1),
                      nm1, None, None, None, nm1, amax, arr, 1, width, cut, rw, cw, precision,
-                     _native.ptr(out), _native.ptr(ws), wsb, hp1, hp1, _native.stream_ptr())
+                     _native.ptr(out), _native.ptr(ws), wsb, tp, _native.stream_ptr())
         return out
     ns = sig.shape[0]
     out = empty((nm2, nm1, ns), np.float64)
@@ -162,7 +163,7 @@ def _arad_call(mode, q1, z1, n1, q2, z2, n2, sigmas, width, cut, rw, cw, precisi
         _native.call("fmlh_arad_kernels", mode, _native.ptr(dq1), _native.ptr(dz1), _native.ptr(dn1),
                      nm1, _native.ptr(dq2), _native.ptr(dz2), _native.ptr(dn2), nm2, amax, arr,
                      len(chunk), width, cut, rw, cw, precision, _native.ptr(tmp), _native.ptr(ws),
-                     wsb, hp1, hp2 if q2 is not None else hp1, _native.stream_ptr())
+                     wsb, tp, _native.stream_ptr())
         if tmp is not out:
             out[:, :, c0:c0 + len(chunk)] = tmp
     return out

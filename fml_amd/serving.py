@@ -15,7 +15,9 @@ the serving form built from the hot-path pieces:
 * fit requests (new training sets) run on a second stream, admitted only
   when the free-HBM budget covers their kernel matrix and factor; predict
   batches keep flowing on the first stream meanwhile, so the two overlap on
-  the device.
+  the device.  The fit stream's kernels are confined to all but
+  ``reserved_cus`` compute units (a CU-masked stream with a hardware queue of
+  its own), so a prediction never waits for a fit's workgroups to drain.
 
 Batch invariance: every number a request gets back depends only on its own
 molecules -- per-molecule Coulomb matrices, per-entry kernel values (MFMA dot
@@ -25,6 +27,7 @@ result is bitwise the same alone and in any mix.
 """
 from __future__ import annotations
 
+import ctypes
 import queue
 import threading
 import time
@@ -33,6 +36,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
+from . import _native
 from .device import to_device, torch
 from .krr import KRR, kernel_matrix, predict_from_kernel
 from .representations import generate_coulomb_matrices
@@ -69,7 +73,7 @@ class KRRServer:
     Coulomb-matrix size the rows were built with."""
 
     def __init__(self, X_train, alpha, sigma, nmax=29, max_batch_molecules=4096,
-                 hbm_budget_bytes=None):
+                 hbm_budget_bytes=None, reserved_cus=32):
         t = torch()
         self.X = to_device(X_train, np.float64)
         self.alpha = to_device(alpha, np.float64)
@@ -77,14 +81,16 @@ class KRRServer:
         self.nmax = int(nmax)
         self.max_batch = int(max_batch_molecules)
         self.hbm_budget = hbm_budget_bytes
-        self.predict_stream = t.cuda.Stream()
-        self.fit_stream = t.cuda.Stream()
+        cus = _native.load().fmlh_device_cu_count()
+        self._raw_streams = []
+        self.predict_stream = self._stream(t, 0)
+        self.fit_stream = self._stream(t, max(cus - int(reserved_cus), 1) if reserved_cus else 0)
         self._q: "queue.Queue[PredictRequest]" = queue.Queue()
         self._fits: "queue.Queue[FitRequest]" = queue.Queue()
         self._stop = threading.Event()
         self._threads = []
         self.batches = []        # (n_requests, n_molecules, seconds) per batch
-        self.fit_log = []
+        self.fit_log = []        # (n, start, end) perf_counter seconds per admitted fit
 
     # -- client API ----------------------------------------------------------
     def submit(self, charges, coords) -> Future:
@@ -104,10 +110,25 @@ class KRRServer:
             th.start()
         return self
 
+    def _stream(self, t, cu_count):
+        p = ctypes.c_void_p(0)
+        _native.call("fmlh_stream_create", int(cu_count), ctypes.byref(p))
+        self._raw_streams.append(p)
+        return t.cuda.ExternalStream(p.value)
+
     def stop(self):
         self._stop.set()
         for th in self._threads:
             th.join(timeout=30)
+        self.predict_stream.synchronize()
+        self.fit_stream.synchronize()
+
+    def close(self):
+        """Stop the loops and release the two streams."""
+        self.stop()
+        for p in self._raw_streams:
+            _native.call("fmlh_stream_destroy", p)
+        self._raw_streams = []
 
     # -- batching ------------------------------------------------------------
     def _drain(self):
@@ -168,7 +189,7 @@ class KRRServer:
             t1 = time.perf_counter()
             self.batches.append((len(batch), sum(len(r.charges) for r in batch), t1 - t0))
             for r, y in zip(batch, res):
-                r.future.set_result((y, t1 - r.t_arrival))
+                r.future.set_result((y, t1 - r.t_arrival, t1))
 
     # -- fits ----------------------------------------------------------------
     def free_hbm(self) -> int:
@@ -194,7 +215,7 @@ class KRRServer:
                     with t.cuda.stream(self.fit_stream):
                         m = KRR(r.sigma, r.lam, precision=r.precision).fit(r.X.astype(np.float64), r.y)
                         self.fit_stream.synchronize()
-                    self.fit_log.append((int(r.X.shape[0]), time.perf_counter() - t0))
+                    self.fit_log.append((int(r.X.shape[0]), t0, time.perf_counter()))
                     r.future.set_result(m)
                 except Exception as e:
                     r.future.set_exception(e)

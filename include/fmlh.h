@@ -44,6 +44,33 @@ int fmlh_device_cu_count(void);
 void fmlh_timing_enable(int on);
 int fmlh_timing_collect(double *total_ms, int *count);
 
+/* One-time per-device set-up (the Coulomb diagonal table).  Entry points
+ * call it themselves; call it before capturing a graph on a device the
+ * library has not used yet, since the set-up copies synchronously. */
+int fmlh_init(void);
+
+/* A stream (returned as hipStream_t in *stream) whose kernels run only on
+ * the first cu_count compute units of the current device (all of them when
+ * cu_count <= 0 or >= the device's count), with a hardware queue of its
+ * own.  No reference counterpart: the serving loop runs fits on such a
+ * stream so that predictions always find free CUs.  Default priority. */
+int fmlh_stream_create(int cu_count, void **stream);
+int fmlh_stream_destroy(void *stream);
+
+/* HIP graphs (no reference counterpart).  fmlh_graph_begin starts capturing
+ * the calling thread's work on `stream`; every entry point called on that
+ * stream until fmlh_graph_end is recorded instead of run (Coulomb matrices,
+ * dense kernels, the ARAD kernel modes, potrf/potrs and the helpers are all
+ * capture-safe; fmlh_arad_autotune is not).  fmlh_graph_end instantiates
+ * the recording; fmlh_graph_launch replays it on any stream, reading the
+ * same device buffers the captured calls were given (their contents may
+ * change between replays, their sizes may not). */
+typedef struct fmlh_graph fmlh_graph;
+int fmlh_graph_begin(void *stream);
+int fmlh_graph_end(void *stream, fmlh_graph **graph);
+int fmlh_graph_launch(fmlh_graph *graph, void *stream);
+int fmlh_graph_destroy(fmlh_graph *graph);
+
 /* ------------------------------------------------------------------------
  * (a) Sorted Coulomb matrices, one workgroup per molecule.
  * Replaces fgenerate_coulomb_matrix
@@ -129,9 +156,24 @@ int fmlh_laplacian_kernel_f32(const float *a, int64_t lda, int na,
 /* precision: 0 = FP64 throughout (the plain form); 1 = FP32 inner overlap
  * (exp, products and accumulation in float), FP64 self terms, distance
  * combination and kernel sums.
- * n1_host / n2_host: optional host copies of n1 / n2.  The kernel modes plan
- * size-sorted molecule-pair tiles from the atom counts on the host; without
- * the host copies the call fetches them, which synchronises the stream. */
+ *
+ * The kernel modes (GLOBAL, SYMMETRIC, KERNEL; amax <= 29) run over
+ * size-sorted molecule-pair tiles: one set-1 molecule i against J set-2
+ * molecules of one atom count s.  The tile list is planned on the device
+ * from the device atom counts, so a call never waits on the host and can
+ * be captured into a HIP graph (fmlh_graph_begin / fmlh_graph_end).
+ * tiling: tile shape and launch shape, or NULL for the library's current
+ * choice for (mode, amax, precision, nsig): an autotuned entry when
+ * fmlh_arad_autotune / fmlh_arad_tiling_set provided one, else the
+ * built-in default {256, 0, 16}. */
+typedef struct fmlh_arad_tiling {
+  int pairs_per_tile; /* target atom pairs J * n_i * s per tile, 64..1024 */
+  int waves_per_simd; /* register budget of the tile kernel: 4 or 5 waves
+                         per SIMD; 0 = 5 where five workgroups fit a CU's
+                         LDS, else 4 */
+  int groups_per_cu;  /* persistent workgroups launched per CU, 1..64 */
+} fmlh_arad_tiling;
+
 size_t fmlh_arad_ws_bytes(int nm1, int nm2, int amax, int mode);
 int fmlh_arad_kernels(int mode, const double *q1, const double *zt1,
                       const int32_t *n1, int nm1, const double *q2,
@@ -139,8 +181,29 @@ int fmlh_arad_kernels(int mode, const double *q1, const double *zt1,
                       const double *sigmas_host, int nsig, double width,
                       double cut, double r_width, double c_width,
                       int precision, double *out, void *ws, size_t ws_bytes,
-                      const int32_t *n1_host, const int32_t *n2_host,
-                      void *stream);
+                      const fmlh_arad_tiling *tiling, void *stream);
+
+/* Autotuner (no reference counterpart): runs fmlh_arad_kernels on the given
+ * inputs once per candidate tiling (pairs_per_tile in {128, 256, 384, 512,
+ * 768, 1024} x waves_per_simd in {4, 5} where five fit), `reps` timed calls
+ * each after one warm-up, timed with HIP events on `stream`; stores the
+ * fastest as the library's choice for (mode, amax, precision, nsig) and
+ * returns it with its mean time per call.  Synchronises the stream; out is
+ * left holding a valid result.  Not capturable. */
+int fmlh_arad_autotune(int mode, const double *q1, const double *zt1,
+                       const int32_t *n1, int nm1, const double *q2,
+                       const double *zt2, const int32_t *n2, int nm2, int amax,
+                       const double *sigmas_host, int nsig, double width,
+                       double cut, double r_width, double c_width,
+                       int precision, double *out, void *ws, size_t ws_bytes,
+                       int reps, fmlh_arad_tiling *best, double *best_ms,
+                       void *stream);
+/* The library's current tiling for (mode, amax, precision, nsig) (default
+ * or tuned), and a setter for loading persisted tuning results. */
+int fmlh_arad_tiling_get(int mode, int amax, int precision, int nsig,
+                         fmlh_arad_tiling *out);
+int fmlh_arad_tiling_set(int mode, int amax, int precision, int nsig,
+                         const fmlh_arad_tiling *t);
 
 /* Atomic (per-atom) forms.  q (na, 5, amax), zt (na, 2), n (na) int32 =
  * atom count of the molecule each atom belongs to.

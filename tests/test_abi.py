@@ -42,13 +42,29 @@ def test_binding_covers_header():
 
 def test_version_and_errors_without_gpu():
     lib = nat.load()
-    assert lib.fmlh_abi_version() == 1
+    assert lib.fmlh_abi_version() == 2
     # argument validation happens before any device call
     assert lib.fmlh_potrf(None, 1, -1, None, None, 0, None) == -3
     assert lib.fmlh_gaussian_kernel(None, 0, 5, None, 0, 5, 3, 1.0, None, 5, None) == -2
     assert lib.fmlh_coulomb_sorted(None, None, None, 1, 0, None, None, None, None, None) == -5
     assert lib.fmlh_arad_kernels(9, None, None, None, 1, None, None, None, 1, 5, None, 1, 0.2, 5.0,
-                                 1.0, 0.5, 0, None, None, 0, None, None, None) == -1
+                                 1.0, 0.5, 0, None, None, 0, None, None) == -1
+    # an out-of-range tiling is rejected before any device work
+    bad = nat.AradTiling(16, 4, 16)
+    import ctypes
+    sig = (ctypes.c_double * 1)(1.0)
+    assert lib.fmlh_arad_kernels(0, None, None, None, 1, None, None, None, 1, 5, sig, 1, 0.2, 5.0,
+                                 1.0, 0.5, 0, None, None, 0, ctypes.byref(bad), None) == -21
+    assert lib.fmlh_arad_tiling_set(0, 23, 0, 3, ctypes.byref(bad)) == -5
+    good = nat.AradTiling(512, 5, 8)
+    assert lib.fmlh_arad_tiling_set(0, 23, 1, 3, ctypes.byref(good)) == 0
+    got = nat.AradTiling()
+    assert lib.fmlh_arad_tiling_get(0, 23, 1, 3, ctypes.byref(got)) == 0
+    assert got.as_tuple() == (512, 5, 8)
+    assert lib.fmlh_arad_tiling_get(0, 23, 1, 2, ctypes.byref(got)) == 0
+    assert got.as_tuple() == (256, 0, 16)  # untuned key: the built-in default
+    assert lib.fmlh_graph_launch(None, None) == -1
+    assert lib.fmlh_graph_destroy(None) == 0
     # empty problems are no-ops
     assert lib.fmlh_gaussian_kernel(None, 3, 0, None, 3, 0, 3, 1.0, None, 1, None) == 0
     assert lib.fmlh_arad_ws_bytes(10, 10, 23, 0) > 0

@@ -42,7 +42,7 @@ def test_serving_batch_invariance(gpu):
     assert len(srv.batches) < len(reqs)  # requests were batched together
     # each request alone
     from fml_amd.serving import PredictRequest
-    for (ch, co), (y, lat) in zip(reqs, results):
+    for (ch, co), (y, lat, _) in zip(reqs, results):
         alone = srv.predict_batch([PredictRequest(ch, co)])[0]
         assert np.array_equal(alone, y)
         assert lat > 0
@@ -54,6 +54,7 @@ def test_serving_batch_invariance(gpu):
     assert np.max(np.abs(results[3][0] - po)) <= 1e-9 * np.max(np.abs(po))
     st = latency_percentiles([r[1] for r in results])
     assert st["p99_ms"] >= st["p50_ms"] > 0
+    srv.close()
 
 
 def test_serving_fit_admission(gpu):
@@ -66,9 +67,9 @@ def test_serving_fit_admission(gpu):
     time.sleep(0.5)
     assert not fut.done()          # waiting for HBM
     # predictions keep flowing while the fit waits
-    y, _ = srv.submit([ms.molecule(0)[0]], [ms.molecule(0)[1]]).result(timeout=60)
+    y, _, _ = srv.submit([ms.molecule(0)[0]], [ms.molecule(0)[1]]).result(timeout=60)
     assert y.shape == (1,)
     srv.hbm_budget = 1 << 40
     model = fut.result(timeout=120)
-    srv.stop()
+    srv.close()
     assert model.alpha.shape[0] == 200
