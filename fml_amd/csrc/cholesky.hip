@@ -12,10 +12,11 @@
 // Factorisation (right-looking, two block levels):
 //   for each 256-column outer panel:
 //     for each 64-column inner block inside it:
-//       k_potf2   diagonal 64x64 block in LDS (one wave)
-//       k_trsm    rows below: M[r][k] <- M[r][k] L_kk^{-T} (one thread/row)
-//       k_update  panel columns right of the block: C -= P Q^T on MFMA
-//     k_update    trailing matrix right of the outer panel, K = 256 on MFMA
+//       k_potf2   diagonal 64x64 block in registers (one wave, lane = row)
+//       k_trsm    rows below: M[r][k] <- M[r][k] L_kk^{-T} (one lane/row)
+//       k_update  panel columns right of the block: C -= P Q^T, 64 tiles
+//     k_update    trailing matrix right of the outer panel, K = 256, on
+//                 128x128 MFMA tiles over its lower triangle only
 // A non-positive (or NaN) pivot sets *info to its 1-based index, as LAPACK;
 // every later kernel sees *info != 0 and returns, so the factor is left
 // undefined but no kernel waits on anything.
@@ -35,71 +36,80 @@ using namespace fmlh;
 constexpr int kNB = 64;     // inner block
 constexpr int kNBO = 256;   // outer panel
 
-// One wave factors the diagonal block in LDS: lane r owns row r.  Step j
-// takes the square root of the pivot (every lane computes the same value),
-// scales column j below it, then each lane r > j updates its row
-// M[r][c] -= L[r][j] L[c][j] for j < c <= r (L[c][j] read as a broadcast).
+// Value of v in lane l (l uniform) for every lane.
+__device__ __forceinline__ double lane_value(double v, int l) {
+  const long long bits = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)bits, l);
+  const int hi = __builtin_amdgcn_readlane((int)(bits >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ float lane_value(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+// One wave factors the diagonal block: lane r holds row r in registers.
+// Step j takes the square root of the pivot (lane j's diagonal, read by
+// every lane), scales column j below it, then every lane r updates its row
+// M[r][c] -= L[r][j] L[c][j] for c > j, L[c][j] read from lane c (entries
+// above the diagonal are updated too and never stored).  Fully unrolled,
+// so the row stays in registers.
 template <typename T>
 __global__ __launch_bounds__(64) void k_potf2(T *__restrict__ a, int64_t lda, int k0, int b,
                                               int32_t *__restrict__ info) {
-  __shared__ T S[kNB][kNB + 1];
-  __shared__ T colj[kNB];  // column j of L, apart from S so its reads need not wait on S's writes
   if (*info != 0) return;
   const int r = threadIdx.x;
   T *grow = a + (int64_t)(k0 + r) * lda + k0;
-  if (r < b)
-    for (int c = 0; c <= r; ++c) S[r][c] = grow[c];
-  __syncthreads();
-  for (int j = 0; j < b; ++j) {
-    const T d = S[j][j];
-    if (!(d > T(0))) {  // uniform: every lane read the same pivot
-      if (r == 0) *info = k0 + j + 1;
-      return;
+  T row[kNB];
+#pragma unroll
+  for (int c = 0; c < kNB; ++c) row[c] = (r < b && c <= r) ? grow[c] : T(0);
+#pragma unroll
+  for (int j = 0; j < kNB; ++j) {
+    if (j < b) {
+      const T d = lane_value(row[j], j);
+      if (!(d > T(0))) {  // uniform: every lane read the same pivot
+        if (r == 0) *info = k0 + j + 1;
+        return;
+      }
+      const T piv = sqrt(d);
+      const T lrj = (r > j && r < b) ? row[j] / piv : T(0);
+      row[j] = (r == j) ? piv : (r > j ? lrj : row[j]);
+#pragma unroll
+      for (int c = j + 1; c < kNB; ++c) row[c] -= lrj * lane_value(lrj, c);
     }
-    const T piv = sqrt(d);
-    T lrj = T(0);
-    if (r == j) S[j][j] = piv;
-    if (r > j && r < b) {
-      lrj = S[r][j] / piv;
-      S[r][j] = lrj;
-    }
-    colj[r] = lrj;
-    __syncthreads();
-    if (r > j && r < b) {
-      T *sr = S[r];
-      for (int c = j + 1; c <= r; ++c) sr[c] -= lrj * colj[c];
-    }
-    __syncthreads();
   }
-  if (r < b)
-    for (int c = 0; c <= r; ++c) grow[c] = S[r][c];
+  if (r < b) {
+#pragma unroll
+    for (int c = 0; c < kNB; ++c)
+      if (c <= r) grow[c] = row[c];
+  }
 }
 
-// Rows r in [r0, n): x = M[r][k0:k0+b] solves x L^T = m_r (forward
-// substitution against the factored diagonal block, held in LDS).
+// Rows r in [r0, n): x = M[r][k0:k0+b] solves x L^T = m_r against the
+// factored diagonal block (LDS), one wave per 64 rows, one row per lane held
+// in registers; column by column, x_c = m_c / L[c][c] then m_t -= x_c L[t][c]
+// for t > c (independent updates, so the row's dependency chain is short).
 template <typename T>
-__global__ __launch_bounds__(256) void k_trsm(T *__restrict__ a, int64_t lda, int n, int k0, int b,
-                                              int r0, const int32_t *__restrict__ info) {
+__global__ __launch_bounds__(64) void k_trsm(T *__restrict__ a, int64_t lda, int n, int k0, int b,
+                                             int r0, const int32_t *__restrict__ info) {
   __shared__ T L[kNB][kNB + 1];
   if (*info != 0) return;
-  for (int idx = threadIdx.x; idx < b * b; idx += 256) {
+  for (int idx = threadIdx.x; idx < b * b; idx += 64) {
     const int r = idx / b, c = idx % b;
     L[r][c] = (c <= r) ? a[(int64_t)(k0 + r) * lda + k0 + c] : T(0);
   }
   __syncthreads();
-  const int r = r0 + blockIdx.x * 256 + threadIdx.x;
+  const int r = r0 + blockIdx.x * 64 + threadIdx.x;
   if (r >= n) return;
   T *row = a + (int64_t)r * lda + k0;
   T x[kNB];
 #pragma unroll
+  for (int c = 0; c < kNB; ++c) x[c] = (c < b) ? row[c] : T(0);
+#pragma unroll
   for (int c = 0; c < kNB; ++c) {
     if (c < b) {
-      T v = row[c];
+      x[c] = x[c] / L[c][c];
 #pragma unroll
-      for (int t = 0; t < c; ++t) v -= x[t] * L[c][t];
-      x[c] = v / L[c][c];
-    } else {
-      x[c] = T(0);
+      for (int t = c + 1; t < kNB; ++t) x[t] -= x[c] * L[t][c];
     }
   }
 #pragma unroll
@@ -108,27 +118,46 @@ __global__ __launch_bounds__(256) void k_trsm(T *__restrict__ a, int64_t lda, in
 }
 
 // C[r][c] -= sum_{t in [kb, ke)} M[r][t] M[c][t] for r in [r0, r1),
-// c in [c0, c1), c <= r.  Grid x over column tiles, y over row tiles.
-template <typename T>
-__global__ __launch_bounds__(256) void k_update(T *__restrict__ a, int64_t lda, int r0, int r1,
-                                                int c0, int c1, int kb, int ke,
+// c in [c0, c1), c <= r, on MT x MT tiles.  kTri: the region is the square
+// [r0, r1)^2 and only its lower-triangle tiles are launched (tile index ->
+// (row, col <= row), rows in order, in XCD-contiguous runs); otherwise a
+// rectangular gx x gy grid in which tiles above the diagonal exit.
+template <typename T, int MT, bool kTri>
+__global__ __launch_bounds__(256, (MT == 128 ? 2 : 1)) void k_update(T *__restrict__ a, int64_t lda, int r0, int r1,
+                                                int c0, int c1, int kb, int ke, int gx, int gy,
                                                 const int32_t *__restrict__ info) {
-  __shared__ TileSmem<T> s;
+  __shared__ TileSmem<T, MT> s;
   if (*info != 0) return;
-  const int tr = r0 + blockIdx.y * kTile;
-  const int tc = c0 + blockIdx.x * kTile;
-  if (tc > tr + kTile - 1) return;  // tile entirely above the diagonal
-  TileAcc<T> acc;
-  tile_nt<T, false>(a + (int64_t)tr * lda + kb, lda, r1 - tr, a + (int64_t)tc * lda + kb, lda,
-                    c1 - tc, ke - kb, s, acc);
+  int tx, ty;
+  if (kTri) {
+    const int64_t t = (int64_t)gy * (gy + 1) / 2;
+    const int64_t per = (t + kXcds - 1) / kXcds;
+    const int64_t L = blockIdx.x;
+    const int64_t local = (L % kXcds) * per + L / kXcds;
+    if (local >= t) return;
+    int row = (int)((sqrt(8.0 * (double)local + 1.0) - 1.0) * 0.5);
+    while ((int64_t)(row + 1) * (row + 2) / 2 <= local) ++row;
+    while ((int64_t)row * (row + 1) / 2 > local) --row;
+    ty = row;
+    tx = (int)(local - (int64_t)row * (row + 1) / 2);
+  } else if (!xcd_tile(gx, gy, tx, ty)) {
+    return;
+  }
+  const int tr = r0 + ty * MT;
+  const int tc = c0 + tx * MT;
+  if (tc > tr + MT - 1) return;  // tile entirely above the diagonal
+  TileAcc<T, MT> acc;
+  tile_nt<T, false, MT>(a + (int64_t)tr * lda + kb, lda, r1 - tr, a + (int64_t)tc * lda + kb, lda,
+                        c1 - tc, ke - kb, s, acc);
+  constexpr int NB = MT / 32;
 #pragma unroll
-  for (int im = 0; im < 2; ++im)
+  for (int im = 0; im < NB; ++im)
 #pragma unroll
-    for (int in = 0; in < 2; ++in)
+    for (int in = 0; in < NB; ++in)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int gr = tr + tile_row<T>(im, r);
-        const int gc = tc + tile_col(in);
+        const int gr = tr + tile_row<T, MT>(im, r);
+        const int gc = tc + tile_col<MT>(in);
         if (gr < r1 && gc < c1 && gc <= gr) a[(int64_t)gr * lda + gc] -= acc.v[im][in][r];
       }
 }
@@ -147,21 +176,30 @@ int potrf_impl(T *a, int64_t lda, int n, int32_t *info, hipStream_t st) {
       FMLH_LAUNCH_CHECK("k_potf2");
       const int r0 = k0 + b;
       if (r0 < n) {
-        hipLaunchKernelGGL(k_trsm<T>, dim3(ceil_div(n - r0, 256)), dim3(256), 0, st, a, lda, n, k0,
+        hipLaunchKernelGGL(k_trsm<T>, dim3(ceil_div(n - r0, 64)), dim3(64), 0, st, a, lda, n, k0,
                            b, r0, info);
         FMLH_LAUNCH_CHECK("k_trsm");
         if (r0 < kend) {  // rest of the outer panel
-          dim3 g(ceil_div(kend - r0, kTile), ceil_div(n - r0, kTile));
-          hipLaunchKernelGGL(k_update<T>, g, dim3(256), 0, st, a, lda, r0, n, r0, kend, k0,
-                             k0 + b, info);
+          const int gx = ceil_div(kend - r0, kTile), gy = ceil_div(n - r0, kTile);
+          hipLaunchKernelGGL((k_update<T, kTile, false>), dim3((unsigned)xcd_grid(gx, gy)),
+                             dim3(256), 0, st, a, lda, r0, n, r0, kend, k0, k0 + b, gx, gy, info);
           FMLH_LAUNCH_CHECK("k_update(panel)");
         }
       }
     }
-    if (kend < n) {
-      dim3 g(ceil_div(n - kend, kTile), ceil_div(n - kend, kTile));
-      hipLaunchKernelGGL(k_update<T>, g, dim3(256), 0, st, a, lda, kend, n, kend, n, K0, kend,
-                         info);
+    if (kend < n) {  // trailing matrix: lower-triangle tiles only
+      const int m = n - kend;
+      if (m >= 4 * 128) {
+        const int g = ceil_div(m, 128);
+        const int64_t t = (int64_t)g * (g + 1) / 2;
+        hipLaunchKernelGGL((k_update<T, 128, true>), dim3((unsigned)xcd_grid((int)t, 1)), dim3(256),
+                           0, st, a, lda, kend, n, kend, n, K0, kend, g, g, info);
+      } else {
+        const int g = ceil_div(m, kTile);
+        const int64_t t = (int64_t)g * (g + 1) / 2;
+        hipLaunchKernelGGL((k_update<T, kTile, true>), dim3((unsigned)xcd_grid((int)t, 1)),
+                           dim3(256), 0, st, a, lda, kend, n, kend, n, K0, kend, g, g, info);
+      }
       FMLH_LAUNCH_CHECK("k_update(trailing)");
     }
   }

@@ -3,7 +3,7 @@
 //
 // Gaussian  K(j,i) = exp(-0.5/sigma^2 * ||a_j - b_i||^2)
 //   fml/kernels/fkernels.f90:317-349.  ||a||^2 + ||b||^2 - 2 a.b with the dot
-//   products on MFMA (64x64 tiles, see gemm_nt.h) and exp fused in the
+//   products on MFMA (128x128 or 64x64 tiles, see gemm_nt.h) and exp fused in the
 //   epilogue; an entry whose GEMM-form distance is within the cancellation
 //   band (d2 <= tau*(||a||^2+||b||^2)) is recomputed from direct differences
 //   by its wavefront (near_pairs.h).
@@ -32,32 +32,38 @@ __device__ __forceinline__ void kstore(O *p, double v) {
   __builtin_nontemporal_store((O)v, p);
 }
 
-// Gaussian on MFMA.  Grid: x over A (na) tiles, y over B (nb) tiles.
-// OutT is float or double; the exponent is formed in double for double
-// output.
-template <typename T, typename OutT>
-__global__ __launch_bounds__(256) void k_gaussian(const T *__restrict__ A, int64_t lda, int na,
+// Gaussian on MFMA, MT x MT output tiles (MT = 128 for large kernels, 64
+// for small ones).  A 1-D grid in XCD-contiguous tile order (gemm_nt.h):
+// gx tiles over A (na), gy over B (nb).  OutT is float or double; the
+// exponent is formed in double for double output.  The distance of an
+// entry does not depend on MT: the dot product runs over k in the same
+// 4-wide MFMA steps and the row norms are sequential fused sums.
+template <typename T, typename OutT, int MT>
+__global__ __launch_bounds__(256, (MT == 128 ? 2 : 1)) void k_gaussian(const T *__restrict__ A, int64_t lda, int na,
                                                   const T *__restrict__ B, int64_t ldb, int nb,
                                                   int d, double inv, OutT *__restrict__ K,
-                                                  int64_t ldk, T tau) {
-  __shared__ TileSmem<T> s;
-  const int n0 = blockIdx.x * kTile;
-  const int m0 = blockIdx.y * kTile;
-  TileAcc<T> acc;
-  tile_nt<T, true>(B + (int64_t)m0 * ldb, ldb, nb - m0, A + (int64_t)n0 * lda, lda, na - n0, d, s,
-                   acc);
+                                                  int64_t ldk, T tau, int gx, int gy) {
+  __shared__ TileSmem<T, MT> s;
+  int tx, ty;
+  if (!xcd_tile(gx, gy, tx, ty)) return;
+  const int n0 = tx * MT;
+  const int m0 = ty * MT;
+  TileAcc<T, MT> acc;
+  tile_nt<T, true, MT>(B + (int64_t)m0 * ldb, ldb, nb - m0, A + (int64_t)n0 * lda, lda, na - n0, d,
+                       s, acc);
+  constexpr int NB = MT / 32;
 #pragma unroll
-  for (int im = 0; im < 2; ++im)
+  for (int im = 0; im < NB; ++im)
 #pragma unroll
-    for (int in = 0; in < 2; ++in)
+    for (int in = 0; in < NB; ++in)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int m = tile_row<T>(im, r);
-        const int n = tile_col(in);
+        const int m = tile_row<T, MT>(im, r);
+        const int n = tile_col<MT>(in);
         const int gm = m0 + m, gn = n0 + n;
         const bool valid = gm < nb && gn < na;
         const T pn = s.pn[m], qn = s.qn[n];
-        T d2 = pn + qn - T(2) * acc.v[im][in][r];
+        T d2 = fma(T(-2), acc.v[im][in][r], pn + qn);
         const bool near = valid && d2 <= tau * (pn + qn);
         d2 = recompute_near(near, d2, A, lda, gn, B, ldb, gm, d);
         if (valid) {
@@ -70,6 +76,27 @@ __global__ __launch_bounds__(256) void k_gaussian(const T *__restrict__ A, int64
           kstore(K + (int64_t)gn + (int64_t)gm * ldk, v);
         }
       }
+}
+
+// Tile size for an na x nb kernel: the 128 tile once both sides span a few
+// of them and the grid still covers the chip several times.
+inline int gauss_tile(int na, int nb) {
+  return (na >= 512 && nb >= 512 && (int64_t)ceil_div(na, 128) * ceil_div(nb, 128) >= 1024) ? 128
+                                                                                          : kTile;
+}
+
+template <typename T, typename OutT>
+void launch_gaussian(const T *a, int64_t lda, int na, const T *b, int64_t ldb, int nb, int d,
+                     double inv, OutT *k, int64_t ldk, T tau, hipStream_t st) {
+  if (gauss_tile(na, nb) == 128) {
+    const int gx = ceil_div(na, 128), gy = ceil_div(nb, 128);
+    hipLaunchKernelGGL((k_gaussian<T, OutT, 128>), dim3((unsigned)xcd_grid(gx, gy)), dim3(256), 0,
+                       st, a, lda, na, b, ldb, nb, d, inv, k, ldk, tau, gx, gy);
+  } else {
+    const int gx = ceil_div(na, kTile), gy = ceil_div(nb, kTile);
+    hipLaunchKernelGGL((k_gaussian<T, OutT, kTile>), dim3((unsigned)xcd_grid(gx, gy)), dim3(256),
+                       0, st, a, lda, na, b, ldb, nb, d, inv, k, ldk, tau, gx, gy);
+  }
 }
 
 // Laplacian on VALU.  Thread (tx, ty) owns A samples n = tx*4+j and B
@@ -237,10 +264,8 @@ int fmlh_gaussian_kernel(const double *a, int64_t lda, int na, const double *b, 
   if (rc) return rc;
   if (na == 0 || nb == 0) return 0;
   const double inv = -0.5 / (sigma * sigma);
-  dim3 grid(ceil_div(na, kTile), ceil_div(nb, kTile));
   timing_begin(as_stream(stream));
-  hipLaunchKernelGGL((k_gaussian<double, double>), grid, dim3(256), 0, as_stream(stream), a, lda,
-                     na, b, ldb, nb, d, inv, k, ldk, 1e-3);
+  launch_gaussian<double, double>(a, lda, na, b, ldb, nb, d, inv, k, ldk, 1e-3, as_stream(stream));
   timing_end(as_stream(stream));
   FMLH_LAUNCH_CHECK("k_gaussian<f64>");
   return 0;
@@ -260,14 +285,11 @@ int fmlh_gaussian_kernel_f32(const float *a, int64_t lda, int na, const float *b
   if (na == 0 || nb == 0) return 0;
   if (mode == 1) return fmlh_gaussian_kernel_bf16x3(a, lda, na, b, ldb, nb, d, sigma, k32, k64, ldk, stream);
   const double inv = -0.5 / (sigma * sigma);
-  dim3 grid(ceil_div(na, kTile), ceil_div(nb, kTile));
   timing_begin(as_stream(stream));
   if (k32)
-    hipLaunchKernelGGL((k_gaussian<float, float>), grid, dim3(256), 0, as_stream(stream), a, lda,
-                       na, b, ldb, nb, d, inv, k32, ldk, 1e-2f);
+    launch_gaussian<float, float>(a, lda, na, b, ldb, nb, d, inv, k32, ldk, 1e-2f, as_stream(stream));
   else
-    hipLaunchKernelGGL((k_gaussian<float, double>), grid, dim3(256), 0, as_stream(stream), a, lda,
-                       na, b, ldb, nb, d, inv, k64, ldk, 1e-2f);
+    launch_gaussian<float, double>(a, lda, na, b, ldb, nb, d, inv, k64, ldk, 1e-2f, as_stream(stream));
   timing_end(as_stream(stream));
   FMLH_LAUNCH_CHECK("k_gaussian<f32>");
   return 0;

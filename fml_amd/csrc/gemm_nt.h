@@ -1,11 +1,16 @@
-// 64x64 "NT" MFMA tile: C[m][n] = sum_k P[m][k] * Q[n][k] for row-major
-// P (rows m, k contiguous) and Q (rows n, k contiguous), 256 threads.
+// "NT" MFMA tiles: C[m][n] = sum_k P[m][k] * Q[n][k] for row-major P (rows m,
+// k contiguous) and Q (rows n, k contiguous), 256 threads per tile.
 //
-// Waves are laid out 2x2; each wave owns a 32x32 sub-tile built from 2x2
-// 16x16 MFMA blocks (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32).
-// Operand tiles of BK=16 columns are staged global -> registers -> LDS with
-// two LDS buffers (one barrier per K step).  LDS rows are padded to 17
-// elements so the 16 rows a 16-lane group reads fall in distinct banks.
+// Tile MT x MT with MT = 64 or 128.  Waves are laid out 2x2; each wave owns
+// an (MT/2) x (MT/2) sub-tile built from NB x NB 16x16 MFMA blocks
+// (NB = MT/32; v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32).  Operand
+// tiles of BK columns (16; 8 for the FP64 128 tile, which keeps its two LDS
+// buffers at 37 KB so four workgroups share a CU) are staged global ->
+// registers -> LDS with two LDS buffers (one barrier per K step).  LDS rows
+// are padded to BK + 1 elements so the 16 rows a 16-lane group reads fall in
+// distinct banks.  The 128 tile halves the operand traffic per flop of the
+// 64 tile; it is the shape for large kernels and trailing updates, the 64
+// tile for small ones.
 //
 // MFMA 16x16x4 lane layout on gfx950 (single-block form; checked on the
 // device with an in-bounds probe):
@@ -43,59 +48,71 @@ struct MfmaOp<float> {
   }
 };
 
-constexpr int kTile = 64;
-constexpr int kBK = 16;
-constexpr int kLdsStride = kBK + 1;
+constexpr int kTile = 64;  // the small tile (also the Cholesky block size)
 constexpr int kTileThreads = 256;
 
-template <typename T>
+template <typename T, int MT>
+constexpr int tile_bk() { return (sizeof(T) == 8 && MT == 128) ? 8 : 16; }
+
+template <typename T, int MT>
 struct TileSmem {
-  T p[2][kTile * kLdsStride];
-  T q[2][kTile * kLdsStride];
-  T pn[kTile];  // optional squared row norms of the P tile
-  T qn[kTile];  // optional squared row norms of the Q tile
+  static constexpr int BK = tile_bk<T, MT>();
+  static constexpr int LS = BK + 1;  // padded LDS row
+  T p[2][MT * LS];
+  T q[2][MT * LS];
+  T pn[MT];  // optional squared row norms of the P tile
+  T qn[MT];  // optional squared row norms of the Q tile
 };
 
-// Result of one tile: acc[im][in][r] is C at m = tile_row<T>(im, r),
-// n = tile_col(in).
-template <typename T>
+// Result of one tile: v[im][in][r] is C at m = tile_row<T, MT>(im, r),
+// n = tile_col<MT>(in).
+template <typename T, int MT>
 struct TileAcc {
-  typename MfmaOp<T>::acc_t v[2][2];
+  static constexpr int NB = MT / 32;
+  typename MfmaOp<T>::acc_t v[NB][NB];
 };
 
 __device__ __forceinline__ int tile_lane() { return threadIdx.x & 63; }
 __device__ __forceinline__ int tile_wm() { return (threadIdx.x >> 6) >> 1; }
 __device__ __forceinline__ int tile_wn() { return (threadIdx.x >> 6) & 1; }
-template <typename T>
+template <typename T, int MT = kTile>
 __device__ __forceinline__ int tile_row(int im, int r) {
   const int q = tile_lane() >> 4;
-  return tile_wm() * 32 + im * 16 + (sizeof(T) == 8 ? q + 4 * r : 4 * q + r);
+  return tile_wm() * (MT / 2) + im * 16 + (sizeof(T) == 8 ? q + 4 * r : 4 * q + r);
 }
+template <int MT = kTile>
 __device__ __forceinline__ int tile_col(int in) {
-  return tile_wn() * 32 + in * 16 + (tile_lane() & 15);
+  return tile_wn() * (MT / 2) + in * 16 + (tile_lane() & 15);
 }
 
-// Computes the 64x64 tile for P rows [0, mvalid) and Q rows [0, nvalid)
+// Computes the MT x MT tile for P rows [0, mvalid) and Q rows [0, nvalid)
 // (pointers already offset to the tile origin and to column kbeg), over
-// columns [0, klen).  Out-of-range rows/columns read as zero.  When
-// kNorms is set, s.pn / s.qn receive sequential sums of squares of the
-// staged rows (ascending k).
-template <typename T, bool kNorms>
+// columns [0, klen).  Out-of-range rows/columns read as zero.  When kNorms
+// is set, s.pn / s.qn receive sequential fused sums of squares of the staged
+// rows (ascending k), the same for either tile size.
+template <typename T, bool kNorms, int MT = kTile>
 __device__ __forceinline__ void tile_nt(const T *__restrict__ P, int64_t ldp, int mvalid,
                                         const T *__restrict__ Q, int64_t ldq, int nvalid,
-                                        int klen, TileSmem<T> &s, TileAcc<T> &acc) {
+                                        int klen, TileSmem<T, MT> &s, TileAcc<T, MT> &acc) {
   typedef MfmaOp<T> Op;
+  constexpr int NB = MT / 32;           // MFMA blocks per wave and dimension
+  constexpr int kBK = TileSmem<T, MT>::BK;
+  constexpr int kLdsStride = TileSmem<T, MT>::LS;
+  constexpr int EPT = MT * kBK / 256;   // staged elements per thread and operand
+  constexpr int TPR = kBK / EPT;        // threads per staged row
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wm = tile_wm(), wn = tile_wn();
-  const int lrow = tid >> 2;        // 0..63
-  const int lcol = (tid & 3) * 4;   // 0,4,8,12
+  const int lrow = tid / TPR;
+  const int lcol = (tid % TPR) * EPT;
 
-  for (int i = 0; i < 2; ++i)
-    for (int j = 0; j < 2; ++j) acc.v[i][j] = Op::zero();
+#pragma unroll
+  for (int i = 0; i < NB; ++i)
+#pragma unroll
+    for (int j = 0; j < NB; ++j) acc.v[i][j] = Op::zero();
 
   T pn_acc = T(0), qn_acc = T(0);
-  T rp[4], rq[4];
+  T rp[EPT], rq[EPT];
   const bool prow_ok = lrow < mvalid;
   const bool qrow_ok = lrow < nvalid;
   const T *prow = P + (int64_t)lrow * ldp;
@@ -103,7 +120,7 @@ __device__ __forceinline__ void tile_nt(const T *__restrict__ P, int64_t ldp, in
 
   auto gload = [&](int k0) {
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
+    for (int c = 0; c < EPT; ++c) {
       const int k = k0 + lcol + c;
       rp[c] = (prow_ok && k < klen) ? prow[k] : T(0);
       rq[c] = (qrow_ok && k < klen) ? qrow[k] : T(0);
@@ -111,7 +128,7 @@ __device__ __forceinline__ void tile_nt(const T *__restrict__ P, int64_t ldp, in
   };
   auto lstore = [&](int buf) {
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
+    for (int c = 0; c < EPT; ++c) {
       s.p[buf][lrow * kLdsStride + lcol + c] = rp[c];
       s.q[buf][lrow * kLdsStride + lcol + c] = rq[c];
     }
@@ -129,27 +146,29 @@ __device__ __forceinline__ void tile_nt(const T *__restrict__ P, int64_t ldp, in
 #pragma unroll
     for (int kq = 0; kq < kBK / 4; ++kq) {
       const int kk = kq * 4 + (lane >> 4);
-      T a0 = sp[(wm * 32 + 0 + (lane & 15)) * kLdsStride + kk];
-      T a1 = sp[(wm * 32 + 16 + (lane & 15)) * kLdsStride + kk];
-      T b0 = sq[(wn * 32 + 0 + (lane & 15)) * kLdsStride + kk];
-      T b1 = sq[(wn * 32 + 16 + (lane & 15)) * kLdsStride + kk];
-      acc.v[0][0] = Op::mma(a0, b0, acc.v[0][0]);
-      acc.v[0][1] = Op::mma(a0, b1, acc.v[0][1]);
-      acc.v[1][0] = Op::mma(a1, b0, acc.v[1][0]);
-      acc.v[1][1] = Op::mma(a1, b1, acc.v[1][1]);
+      T a[NB], b[NB];
+#pragma unroll
+      for (int i = 0; i < NB; ++i) {
+        a[i] = sp[(wm * (MT / 2) + i * 16 + (lane & 15)) * kLdsStride + kk];
+        b[i] = sq[(wn * (MT / 2) + i * 16 + (lane & 15)) * kLdsStride + kk];
+      }
+#pragma unroll
+      for (int i = 0; i < NB; ++i)
+#pragma unroll
+        for (int j = 0; j < NB; ++j) acc.v[i][j] = Op::mma(a[i], b[j], acc.v[i][j]);
     }
     if (kNorms) {
-      if (tid < 64) {
+      if (tid < MT) {
 #pragma unroll
         for (int k = 0; k < kBK; ++k) {
           T v = sp[tid * kLdsStride + k];
-          pn_acc = pn_acc + v * v;
+          pn_acc = fma(v, v, pn_acc);
         }
-      } else if (tid < 128) {
+      } else if (tid < 2 * MT) {
 #pragma unroll
         for (int k = 0; k < kBK; ++k) {
-          T v = sq[(tid - 64) * kLdsStride + k];
-          qn_acc = qn_acc + v * v;
+          T v = sq[(tid - MT) * kLdsStride + k];
+          qn_acc = fma(v, v, qn_acc);
         }
       }
     }
@@ -157,12 +176,43 @@ __device__ __forceinline__ void tile_nt(const T *__restrict__ P, int64_t ldp, in
     __syncthreads();
   }
   if (kNorms) {
-    if (tid < 64)
+    if (tid < MT)
       s.pn[tid] = pn_acc;
-    else if (tid < 128)
-      s.qn[tid - 64] = qn_acc;
+    else if (tid < 2 * MT)
+      s.qn[tid - MT] = qn_acc;
     __syncthreads();
   }
+}
+
+// XCD-aware tile order.  Workgroups are dispatched round-robin over the
+// chip's 8 XCDs (workgroup L runs on XCD L % 8), each XCD with its own L2.
+// The grid is padded to a multiple of 8; workgroup L takes tile
+// (L % 8) * per + L / 8 of a grouped raster order, so the workgroups of one
+// XCD cover a contiguous run of tiles that share operand tiles, and within
+// the run kGroup consecutive tile rows are walked column by column.  Returns
+// false for the padding workgroups (their tile index is past the end).
+constexpr int kXcds = 8;
+constexpr int kGroup = 8;
+
+__host__ __device__ inline int64_t xcd_grid(int gx, int gy) {
+  const int64_t t = (int64_t)gx * gy;
+  return ((t + kXcds - 1) / kXcds) * kXcds;
+}
+
+__device__ __forceinline__ bool xcd_tile(int gx, int gy, int &tx, int &ty) {
+  const int64_t t = (int64_t)gx * gy;
+  const int64_t per = (t + kXcds - 1) / kXcds;
+  const int64_t L = blockIdx.x;
+  const int64_t local = (L % kXcds) * per + L / kXcds;
+  if (local >= t) return false;
+  const int64_t span = (int64_t)kGroup * gx;
+  const int64_t g = local / span;
+  const int first = (int)(g * kGroup);
+  const int rows = min(kGroup, gy - first);
+  const int64_t in = local - g * span;
+  ty = first + (int)(in % rows);
+  tx = (int)(in / rows);
+  return true;
 }
 
 }  // namespace fmlh
