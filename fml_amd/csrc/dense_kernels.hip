@@ -181,6 +181,11 @@ __global__ __launch_bounds__(256) void k_laplacian(const T *__restrict__ A, int6
   }
 }
 
+__global__ void k_fill(double *p, int n, double v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
 __global__ void k_add_diagonal(double *a, int64_t lda, int n, double lambda) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) a[(int64_t)i + (int64_t)i * lda] += lambda;
@@ -371,4 +376,52 @@ int fmlh_gemv_t(const double *k, int64_t ldk, int na, int nb, const double *alph
   return 0;
 }
 
+size_t fmlh_krr_predict_ws_bytes(int nmol, int ntrain, int nmax) {
+  if (nmol < 0 || ntrain < 0 || nmax < 1) return 0;
+  const size_t packed = (size_t)nmax * (nmax + 1) / 2;
+  return WsCarver::need(sizeof(double) * (size_t)nmol * packed) +
+         WsCarver::need(sizeof(double) * (size_t)nmol * (size_t)ntrain);
+}
+
+int fmlh_coulomb_sorted(const double *z, const double *r, const int32_t *offsets, int nmol,
+                        int nmax, double *cm, float *cm32, int32_t *perm, int32_t *status,
+                        void *stream);
+
+int fmlh_krr_predict(const double *z, const double *r, const int32_t *offsets, int nmol, int nmax,
+                     const double *xtrain, int64_t ldx, int ntrain, const double *alpha,
+                     double sigma, double *y, int32_t *status, void *ws, size_t ws_bytes,
+                     void *stream) {
+  FMLH_ARG_CHECK(nmol >= 0, 4);
+  FMLH_ARG_CHECK(nmax >= 1, 5);
+  FMLH_ARG_CHECK(ntrain >= 0, 8);
+  const int packed = nmax * (nmax + 1) / 2;
+  FMLH_ARG_CHECK(ldx >= packed, 7);
+  FMLH_ARG_CHECK(sigma != 0.0, 10);
+  if (nmol == 0) return 0;
+  FMLH_ARG_CHECK(z && r && offsets, 1);
+  FMLH_ARG_CHECK(y != nullptr, 11);
+  if (ntrain > 0) FMLH_ARG_CHECK(xtrain && alpha, 6);
+  FMLH_ARG_CHECK(ws != nullptr && ws_bytes >= fmlh_krr_predict_ws_bytes(nmol, ntrain, nmax), 13);
+  WsCarver w(ws, ws_bytes);
+  double *cm = w.take<double>((size_t)nmol * packed);
+  double *k = w.take<double>((size_t)nmol * (size_t)(ntrain > 0 ? ntrain : 1));
+  if (!cm || !k) return -13;
+  int rc = fmlh_coulomb_sorted(z, r, offsets, nmol, nmax, cm, nullptr, nullptr, status, stream);
+  if (rc) return rc;
+  hipStream_t st = as_stream(stream);
+  if (ntrain == 0) {
+    hipLaunchKernelGGL(k_fill, dim3(ceil_div(nmol, 256)), dim3(256), 0, st, y, nmol, 0.0);
+    FMLH_LAUNCH_CHECK("k_fill");
+    return 0;
+  }
+  const double inv = -0.5 / (sigma * sigma);
+  launch_gaussian<double, double>(xtrain, ldx, ntrain, cm, packed, nmol, packed, inv, k, ntrain,
+                                  1e-3, st);
+  FMLH_LAUNCH_CHECK("k_gaussian<f64>(predict)");
+  hipLaunchKernelGGL(k_gemv_t, dim3(nmol), dim3(256), 0, st, k, (int64_t)ntrain, ntrain, alpha, y);
+  FMLH_LAUNCH_CHECK("k_gemv_t(predict)");
+  return 0;
+}
+
 }  // extern "C"
+

@@ -9,9 +9,11 @@ the serving form built from the hot-path pieces:
   everything pending into one batch, so requests that arrive while a batch
   runs are admitted into the next one -- continuous batching, not a queue
   served one request at a time;
-* a batch is one pass of the GPU path: sorted Coulomb matrices for all its
-  molecules (one workgroup per molecule), the Gaussian kernel block against
-  the training set, and K^T alpha with one workgroup per test molecule;
+* a batch is one pass of the GPU path, one native call (fmlh_krr_predict):
+  sorted Coulomb matrices for all its molecules (one workgroup per
+  molecule), the Gaussian kernel block against the training set, and
+  K^T alpha with one workgroup per test molecule; inputs and outputs move
+  through reused pinned buffers, one event wait per batch;
 * fit requests (new training sets) run on a second stream, admitted only
   when the free-HBM budget covers their kernel matrix and factor; predict
   batches keep flowing on the first stream meanwhile, so the two overlap on
@@ -38,8 +40,7 @@ import numpy as np
 
 from . import _native
 from .device import to_device, torch
-from .krr import KRR, kernel_matrix, predict_from_kernel
-from .representations import generate_coulomb_matrices
+from .krr import KRR
 
 
 @dataclass
@@ -66,6 +67,60 @@ class FitRequest:
         return n * n * 8 + (n * n * 4 if self.precision == "mixed" else 0) + 16 * n * self.X.shape[1]
 
 
+# CU-partitioned streams, one per (device, CU count), created on first use
+# and never destroyed: torch's pinned-memory allocator records an event on
+# every stream a pinned block was used on when the block is freed, possibly
+# long after the server that used the stream has stopped, so the stream has
+# to outlive every pinned buffer it ever touched.
+_CU_STREAMS: dict = {}
+_CU_STREAMS_LOCK = threading.Lock()
+
+
+def _cu_stream(t, cu_count):
+    key = (t.cuda.current_device(), int(cu_count))
+    with _CU_STREAMS_LOCK:
+        st = _CU_STREAMS.get(key)
+        if st is None:
+            p = ctypes.c_void_p(0)
+            _native.call("fmlh_stream_create", int(cu_count), ctypes.byref(p))
+            st = t.cuda.ExternalStream(p.value)
+            _CU_STREAMS[key] = st
+        return st
+
+
+class _Staging:
+    """Pinned host and device buffers of one predict batch (atoms, molecules,
+    the kernel block's workspace), grown on demand and reused, so a batch
+    costs three asynchronous uploads, one native call, one download and one
+    event wait."""
+
+    def __init__(self, t, ntrain, nmax):
+        self.t, self.ntrain, self.nmax = t, ntrain, nmax
+        self.cap_atoms = self.cap_mols = 0
+        self.done = t.cuda.Event()
+
+    def ensure(self, natoms, nmol):
+        if natoms <= self.cap_atoms and nmol <= self.cap_mols:
+            return
+        t = self.t
+        ca = max(natoms, 2 * self.cap_atoms)
+        cm = max(nmol, 2 * self.cap_mols)
+        f64, i32 = t.float64, t.int32
+        self.h_z = t.empty(ca, dtype=f64, pin_memory=True)
+        self.h_r = t.empty((ca, 3), dtype=f64, pin_memory=True)
+        self.h_off = t.empty(cm + 1, dtype=i32, pin_memory=True)
+        self.h_y = t.empty(cm, dtype=f64, pin_memory=True)
+        self.hz, self.hr, self.hoff, self.hy = (self.h_z.numpy(), self.h_r.numpy(), self.h_off.numpy(),
+                                                self.h_y.numpy())
+        self.d_z = t.empty(ca, dtype=f64, device="cuda")
+        self.d_r = t.empty((ca, 3), dtype=f64, device="cuda")
+        self.d_off = t.empty(cm + 1, dtype=i32, device="cuda")
+        self.d_y = t.empty(cm, dtype=f64, device="cuda")
+        self.wsb = _native.load().fmlh_krr_predict_ws_bytes(cm, self.ntrain, self.nmax)
+        self.ws = t.empty(max(self.wsb, 1), dtype=t.uint8, device="cuda")
+        self.cap_atoms, self.cap_mols = ca, cm
+
+
 class KRRServer:
     """Serve predictions of a trained Gaussian-kernel KRR model.
 
@@ -82,13 +137,15 @@ class KRRServer:
         self.max_batch = int(max_batch_molecules)
         self.hbm_budget = hbm_budget_bytes
         cus = _native.load().fmlh_device_cu_count()
-        self._raw_streams = []
         self.predict_stream = self._stream(t, 0)
         self.fit_stream = self._stream(t, max(cus - int(reserved_cus), 1) if reserved_cus else 0)
         self._q: "queue.Queue[PredictRequest]" = queue.Queue()
         self._fits: "queue.Queue[FitRequest]" = queue.Queue()
         self._stop = threading.Event()
         self._threads = []
+        self._lock = threading.Lock()
+        self._staging = _Staging(t, int(self.X.shape[0]), self.nmax)
+        self._staging.ensure(64 * self.nmax, 1024)
         self.batches = []        # (n_requests, n_molecules, seconds) per batch
         self.fit_log = []        # (n, start, end) perf_counter seconds per admitted fit
 
@@ -111,10 +168,7 @@ class KRRServer:
         return self
 
     def _stream(self, t, cu_count):
-        p = ctypes.c_void_p(0)
-        _native.call("fmlh_stream_create", int(cu_count), ctypes.byref(p))
-        self._raw_streams.append(p)
-        return t.cuda.ExternalStream(p.value)
+        return _cu_stream(t, cu_count)
 
     def stop(self):
         self._stop.set()
@@ -124,11 +178,10 @@ class KRRServer:
         self.fit_stream.synchronize()
 
     def close(self):
-        """Stop the loops and release the two streams."""
+        """Stop the loops and drop the staging buffers.  The two streams are
+        shared process-wide and stay alive (see _cu_stream)."""
         self.stop()
-        for p in self._raw_streams:
-            _native.call("fmlh_stream_destroy", p)
-        self._raw_streams = []
+        self._staging = None
 
     # -- batching ------------------------------------------------------------
     def _drain(self):
@@ -150,28 +203,50 @@ class KRRServer:
         return batch
 
     def predict_batch(self, requests):
-        """Run one batch synchronously on the predict stream; returns the
-        per-request prediction arrays."""
+        """Run one batch on the predict stream and wait for it; returns the
+        per-request prediction arrays (a ValueError in place of the array for
+        a request holding a molecule with more than nmax atoms)."""
         t = torch()
-        zs, rs, counts = [], [], []
-        for r in requests:
+        zs, rs, counts, owner = [], [], [], []
+        out = [None] * len(requests)
+        for k, r in enumerate(requests):
+            n_r = [len(z) for z in r.charges]
+            if any(n > self.nmax for n in n_r):
+                out[k] = ValueError(f"a molecule has more than nmax={self.nmax} atoms")
+                continue
             for z, c in zip(r.charges, r.coords):
                 zs.append(np.asarray(z, dtype=np.float64))
                 rs.append(np.asarray(c, dtype=np.float64).reshape(-1, 3))
-                counts.append(len(z))
-        off = np.zeros(len(counts) + 1, dtype=np.int32)
-        off[1:] = np.cumsum(counts)
-        with t.cuda.stream(self.predict_stream):
-            Xq = generate_coulomb_matrices(np.concatenate(zs), np.concatenate(rs), off,
-                                           nmax=self.nmax, on_device=True)
-            Kt = kernel_matrix(self.X, Xq, self.sigma)
-            y = predict_from_kernel(Kt, self.alpha)
-            yh = y.cpu().numpy()
-        out, k = [], 0
-        for r in requests:
-            m = len(r.charges)
-            out.append(yh[k:k + m])
-            k += m
+            counts.extend(n_r)
+            owner.append((k, len(n_r)))
+        nmol = len(counts)
+        if nmol == 0:
+            return out
+        with self._lock:
+            st = self._staging
+            natoms = int(sum(counts))
+            st.ensure(natoms, nmol)
+            st.hz[:natoms] = np.concatenate(zs)
+            st.hr[:natoms] = np.concatenate(rs)
+            st.hoff[0] = 0
+            np.cumsum(counts, out=st.hoff[1:nmol + 1])
+            ps = self.predict_stream
+            with t.cuda.stream(ps):
+                st.d_z[:natoms].copy_(st.h_z[:natoms], non_blocking=True)
+                st.d_r[:natoms].copy_(st.h_r[:natoms], non_blocking=True)
+                st.d_off[:nmol + 1].copy_(st.h_off[:nmol + 1], non_blocking=True)
+                _native.call("fmlh_krr_predict", _native.ptr(st.d_z), _native.ptr(st.d_r),
+                             _native.ptr(st.d_off), nmol, self.nmax, _native.ptr(self.X), self.X.stride(0),
+                             int(self.X.shape[0]), _native.ptr(self.alpha), self.sigma, _native.ptr(st.d_y),
+                             None, _native.ptr(st.ws), st.wsb, ctypes.c_void_p(ps.cuda_stream))
+                st.h_y[:nmol].copy_(st.d_y[:nmol], non_blocking=True)
+                st.done.record(ps)
+            st.done.synchronize()
+            yh = st.hy[:nmol].copy()
+        i = 0
+        for k, m in owner:
+            out[k] = yh[i:i + m]
+            i += m
         return out
 
     def _predict_loop(self):
@@ -189,7 +264,10 @@ class KRRServer:
             t1 = time.perf_counter()
             self.batches.append((len(batch), sum(len(r.charges) for r in batch), t1 - t0))
             for r, y in zip(batch, res):
-                r.future.set_result((y, t1 - r.t_arrival, t1))
+                if isinstance(y, Exception):
+                    r.future.set_exception(y)
+                else:
+                    r.future.set_result((y, t1 - r.t_arrival, t1))
 
     # -- fits ----------------------------------------------------------------
     def free_hbm(self) -> int:
@@ -213,7 +291,10 @@ class KRRServer:
                 try:
                     t = torch()
                     with t.cuda.stream(self.fit_stream):
-                        m = KRR(r.sigma, r.lam, precision=r.precision).fit(r.X.astype(np.float64), r.y)
+                        # widen on the device: torch calls release the GIL, so the
+                        # predict thread keeps running meanwhile
+                        Xd = t.from_numpy(np.ascontiguousarray(r.X)).to("cuda").to(t.float64)
+                        m = KRR(r.sigma, r.lam, precision=r.precision).fit(Xd, r.y)
                         self.fit_stream.synchronize()
                     self.fit_log.append((int(r.X.shape[0]), t0, time.perf_counter()))
                     r.future.set_result(m)

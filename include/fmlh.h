@@ -53,7 +53,11 @@ int fmlh_init(void);
  * the first cu_count compute units of the current device (all of them when
  * cu_count <= 0 or >= the device's count), with a hardware queue of its
  * own.  No reference counterpart: the serving loop runs fits on such a
- * stream so that predictions always find free CUs.  Default priority. */
+ * stream so that predictions always find free CUs.  Default priority.
+ * Destroy a stream only when nothing can still record work or events on it:
+ * a stream handed to torch (ExternalStream) is recorded on by torch's
+ * pinned-memory allocator whenever a pinned block used on it is freed, so
+ * the serving loop keeps its streams for the life of the process. */
 int fmlh_stream_create(int cu_count, void **stream);
 int fmlh_stream_destroy(void *stream);
 
@@ -270,6 +274,21 @@ int fmlh_sym_residual(const double *a, int64_t lda, int n, double lambda,
  * prediction K_test^T alpha. */
 int fmlh_gemv_t(const double *k, int64_t ldk, int na, int nb,
                 const double *alpha, double *y, void *stream);
+
+/* KRR prediction of a molecule batch against a resident Gaussian model in
+ * one call (the serving loop's batch; no single reference counterpart: the
+ * README's predict pattern, README.md:33-69): sorted Coulomb matrices of the
+ * batch (as fmlh_coulomb_sorted, z / r / offsets / status alike), the
+ * Gaussian kernel block K(j, i) between training row j of xtrain (ntrain
+ * rows of nmax(nmax+1)/2 features, leading dimension ldx) and batch
+ * molecule i, and y(i) = sum_j K(j, i) alpha(j).  Every y(i) depends only on
+ * molecule i (fixed per-entry and per-output reduction orders).
+ * ws: fmlh_krr_predict_ws_bytes(nmol, ntrain, nmax) bytes. */
+size_t fmlh_krr_predict_ws_bytes(int nmol, int ntrain, int nmax);
+int fmlh_krr_predict(const double *z, const double *r, const int32_t *offsets,
+                     int nmol, int nmax, const double *xtrain, int64_t ldx,
+                     int ntrain, const double *alpha, double sigma, double *y,
+                     int32_t *status, void *ws, size_t ws_bytes, void *stream);
 
 #ifdef __cplusplus
 }

@@ -73,3 +73,34 @@ def test_serving_fit_admission(gpu):
     model = fut.result(timeout=120)
     srv.close()
     assert model.alpha.shape[0] == 200
+
+
+def test_serving_rejects_oversize_molecule_only(gpu):
+    """A request holding a molecule larger than nmax fails alone; requests
+    batched with it are served, and a batch larger than the staging buffers
+    grows them."""
+    from fml_amd.krr import KRR
+    from fml_amd.serving import KRRServer, PredictRequest
+    ms = synth.generate(300, seed=12, nmin=3, nmax=23)
+    X = oracle.coulomb_matrices(ms.charges, ms.coords, ms.offsets, 23)[0]
+    m = KRR(700.0, 1e-8).fit(X, ms.energies)
+    srv = KRRServer(X, m.alpha, 700.0, nmax=23)
+    big = synth.generate(1, seed=13, nmin=24, nmax=24)
+    ok = synth.generate(1500, seed=14, nmin=3, nmax=23)
+    reqs = [PredictRequest([ok.molecule(i)[0] for i in range(700)], [ok.molecule(i)[1] for i in range(700)]),
+            PredictRequest([big.molecule(0)[0]], [big.molecule(0)[1]]),
+            PredictRequest([ok.molecule(i)[0] for i in range(700, 1500)],
+                           [ok.molecule(i)[1] for i in range(700, 1500)])]
+    out = srv.predict_batch(reqs)
+    assert isinstance(out[1], ValueError)
+    assert out[0].shape == (700,) and out[2].shape == (800,)
+    # same values as the FP64 oracle kernel times alpha
+    Xq = oracle.coulomb_matrices(ok.charges, ok.coords, ok.offsets, 23)[0]
+    Kq = oracle.gaussian_kernel(X.T, Xq.T, 700.0)  # (ntrain, nq), the reference's Fortran K
+    po = Kq.T @ m.alpha.cpu().numpy()
+    got = np.concatenate([out[0], out[2]])
+    assert np.max(np.abs(got - po)) <= 1e-9 * np.max(np.abs(po))
+    # batch invariance across the buffer growth
+    alone = srv.predict_batch([reqs[2]])[0]
+    assert np.array_equal(alone, out[2])
+    srv.close()
