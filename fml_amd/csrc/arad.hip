@@ -38,10 +38,16 @@
 // unchanged.  Lists that are not ascending use the full loop.
 //
 // Inner exp: its argument (d2-d1)^2 * (-1/(4w^2)) lies in (-16, 0] whenever
-// the term is taken, because (8w)^2 * (-1/(4w^2)) = -16.  exp_neg() evaluates
-// exp there with a 32-entry 2^(j/32) table held in LDS and a degree-6
-// polynomial (at most 0.99 ulp on that domain); it has no
+// the term is taken, because (8w)^2 * (-1/(4w^2)) = -16.  exp_neg()
+// (exp_neg.h) evaluates exp there with a 128-entry 2^(j/128) table held in
+// LDS and a degree-5 polynomial (at most one ulp on that domain); it has no
 // overflow/underflow/NaN handling because none can occur there.
+//
+// Per term the products are grouped as exp * s2 * T * (s1 + (s1 a1) a2 +
+// (s1 b1) b2), with s1 a1 and s1 b1 formed once per m1: the reference's
+// terms (farad_kernels.f90:78-83), one multiply fewer per term, rounding
+// different at the last-ulp level; the accumulation over m1 and m2 keeps the
+// reference's order.
 #include <math.h>
 #include <string.h>
 
@@ -52,6 +58,7 @@
 #include <vector>
 
 #include "common.h"
+#include "exp_neg.h"
 
 #pragma clang fp contract(off)
 
@@ -112,39 +119,8 @@ __device__ __forceinline__ double stoch(double dr, double dc, double rw2, double
   return rw2 / (rw2 + dr * dr) * cw2 / (cw2 + dc * dc);
 }
 
-// 2^(j/32), j = 0..31, correctly rounded.
-__constant__ double c_exp2_32[32] = {
-    0x1.0000000000000p+0, 0x1.059b0d3158574p+0, 0x1.0b5586cf9890fp+0, 0x1.11301d0125b51p+0,
-    0x1.172b83c7d517bp+0, 0x1.1d4873168b9aap+0, 0x1.2387a6e756238p+0, 0x1.29e9df51fdee1p+0,
-    0x1.306fe0a31b715p+0, 0x1.371a7373aa9cbp+0, 0x1.3dea64c123422p+0, 0x1.44e086061892dp+0,
-    0x1.4bfdad5362a27p+0, 0x1.5342b569d4f82p+0, 0x1.5ab07dd485429p+0, 0x1.6247eb03a5585p+0,
-    0x1.6a09e667f3bcdp+0, 0x1.71f75e8ec5f74p+0, 0x1.7a11473eb0187p+0, 0x1.82589994cce13p+0,
-    0x1.8ace5422aa0dbp+0, 0x1.93737b0cdc5e5p+0, 0x1.9c49182a3f090p+0, 0x1.a5503b23e255dp+0,
-    0x1.ae89f995ad3adp+0, 0x1.b7f76f2fb5e47p+0, 0x1.c199bdd85529cp+0, 0x1.cb720dcef9069p+0,
-    0x1.d5818dcfba487p+0, 0x1.dfc97337b9b5fp+0, 0x1.ea4afa2a490dap+0, 0x1.f50765b6e4540p+0,
-};
-
-// exp(x) for x in (-16, 0] with tab = c_exp2_32 (copied to LDS by the
-// caller).  x = (32 k + j) ln2/32 + r, |r| <= ln2/64, from a two-part
-// ln2/32 whose high part times k is exact; e^r = 1 + q(r) with q the
-// degree-6 Taylor polynomial minus 1 (truncation below 4e-18); the result
-// 2^k (t + t q), t = 2^(j/32), rounds once after the table value's own
-// rounding.  Measured against a 40-digit exp on 85,000 points of (-16, 0]:
-// at most 0.99 ulp.
-__device__ __forceinline__ double exp_neg(double x, const double *__restrict__ tab) {
-  const double k32 = rint(x * 0x1.71547652b82fep+5);  // x * 32/ln2
-  double r = fma(-k32, 0x1.62e42fee00000p-6, x);
-  r = fma(-k32, 0x1.a39ef35793c76p-38, r);
-  double q = fma(r, 0x1.6c16c16c16c17p-10, 0x1.1111111111111p-7);  // 1/6!, 1/5!
-  q = fma(q, r, 0x1.5555555555555p-5);                             // 1/4!
-  q = fma(q, r, 0x1.5555555555555p-3);                             // 1/3!
-  q = fma(q, r, 0.5);
-  q = fma(q, r, 1.0);
-  q = q * r;
-  const int k = (int)k32;
-  const double t = tab[k & 31];
-  return ldexp(fma(t, q, t), k >> 5);
-}
+// 2^(j/128), j = 0..127 (exp_neg's table; copied to LDS by the kernels).
+__constant__ double c_exp2_128[128] = {FMLH_EXP2_128_VALUES};
 
 // ---- preparation ------------------------------------------------------
 
@@ -296,12 +272,13 @@ __device__ TC arad_distl2(const NbView &x, int cnt1, bool asc1, const NbView &y,
   TC acc = TC(0);
   // d * ((1 + a1 a2) + b1 b2), the quantity farad_kernels.f90:83 adds (the
   // angular factor with fused multiply-adds); record yr (index m2) of list 2
-  auto rec_value = [&](const Rec &yr, uint16_t ty2, int m2, TC d1, TC s1, TC a1, TC b1, int trow,
+  // exp * s2 * T * (s1 + sa1 a2 + sb1 b2) with sa1 = s1 a1, sb1 = s1 b1: the
+  // quantity farad_kernels.f90:78-83 adds, for record yr (index m2) of list 2
+  auto rec_value = [&](const Rec &yr, uint16_t ty2, int m2, TC d1, TC s1, TC sa1, TC sb1, int trow,
                        double2 rc1) -> TC {
     const TC dr = (TC)yr.d - d1;
     const TC r2 = dr * dr;
-    TC v = texp<TC>(r2 * inv_w, etab) * s1;
-    v = v * (TC)yr.s;
+    TC v = texp<TC>(r2 * inv_w, etab) * (TC)yr.s;
     TC t;
     if (kGeneric) {
       const double2 rc2 = y.rc[m2];
@@ -310,11 +287,11 @@ __device__ TC arad_distl2(const NbView &x, int cnt1, bool asc1, const NbView &y,
       t = ttab[trow + ty2];
     }
     v = v * t;
-    const TC ang = tfma(b1, (TC)yr.b, tfma(a1, (TC)yr.a, TC(1)));
+    const TC ang = tfma(sb1, (TC)yr.b, tfma(sa1, (TC)yr.a, s1));
     return v * ang;
   };
-  auto term = [&](int m2, TC d1, TC s1, TC a1, TC b1, int trow, double2 rc1) {
-    acc = acc + rec_value(y.rec[m2], kGeneric ? uint16_t(0) : y.ty[m2], m2, d1, s1, a1, b1, trow, rc1);
+  auto term = [&](int m2, TC d1, TC s1, TC sa1, TC sb1, int trow, double2 rc1) {
+    acc = acc + rec_value(y.rec[m2], kGeneric ? uint16_t(0) : y.ty[m2], m2, d1, s1, sa1, sb1, trow, rc1);
   };
   if (asc1 && asc2) {
     int lo = 0, hi = 0;
@@ -345,7 +322,7 @@ __device__ TC arad_distl2(const NbView &x, int cnt1, bool asc1, const NbView &y,
       // added to acc one after the other in m2 order; the next trip's two
       // records are read before this trip's arithmetic (indices clamped to
       // the list, the clamped reads are not used)
-      const TC s1 = (TC)xr.s, a1 = (TC)xr.a, b1 = (TC)xr.b;
+      const TC s1 = (TC)xr.s, sa1 = s1 * (TC)xr.a, sb1 = s1 * (TC)xr.b;
       const int last = kPadY ? cnt2 + 1 : cnt2 - 1;
       Rec r0 = y.rec[lo], r1 = y.rec[min(lo + 1, last)];
       uint16_t t0 = y.ty[lo], t1 = y.ty[min(lo + 1, last)];
@@ -355,8 +332,8 @@ __device__ TC arad_distl2(const NbView &x, int cnt1, bool asc1, const NbView &y,
         const int p1 = kPadY ? m2 + 3 : min(m2 + 3, last);
         const Rec n0 = y.rec[p0], n1 = y.rec[p1];
         const uint16_t u0 = y.ty[p0], u1 = y.ty[p1];
-        const TC v0 = rec_value(r0, t0, m2, d1, s1, a1, b1, trow, rc1);
-        const TC v1 = rec_value(r1, t1, m2 + 1, d1, s1, a1, b1, trow, rc1);
+        const TC v0 = rec_value(r0, t0, m2, d1, s1, sa1, sb1, trow, rc1);
+        const TC v1 = rec_value(r1, t1, m2 + 1, d1, s1, sa1, sb1, trow, rc1);
         acc = acc + v0;
         acc = acc + v1;
         r0 = n0;
@@ -364,7 +341,7 @@ __device__ TC arad_distl2(const NbView &x, int cnt1, bool asc1, const NbView &y,
         t0 = u0;
         t1 = u1;
       }
-      if (m2 < hi) acc = acc + rec_value(r0, t0, m2, d1, s1, a1, b1, trow, rc1);
+      if (m2 < hi) acc = acc + rec_value(r0, t0, m2, d1, s1, sa1, sb1, trow, rc1);
       }
       xr = xnext;
       xt = xtnext;
@@ -375,9 +352,10 @@ __device__ TC arad_distl2(const NbView &x, int cnt1, bool asc1, const NbView &y,
       const TC d1 = (TC)xr.d;
       const int trow = kGeneric ? 0 : (int)x.ty[m1] * kMaxTypes;
       const double2 rc1 = kGeneric ? x.rc[m1] : make_double2(0.0, 0.0);
+      const TC s1 = (TC)xr.s, sa1 = s1 * (TC)xr.a, sb1 = s1 * (TC)xr.b;
       for (int m2 = 0; m2 < cnt2; ++m2) {
         const TC dr = (TC)y.rec[m2].d - d1;
-        if (dr * dr < maxg) term(m2, d1, (TC)xr.s, (TC)xr.a, (TC)xr.b, trow, rc1);
+        if (dr * dr < maxg) term(m2, d1, s1, sa1, sb1, trow, rc1);
       }
     }
   }
@@ -393,18 +371,18 @@ __device__ __forceinline__ NbView global_view(const NbSet &s, int64_t slot, int 
 template <typename TC>
 __device__ void load_tables(TC *ttab, double *etab, const Ctl *ctl) {
   for (int t = threadIdx.x; t < kMaxTypes * kMaxTypes; t += blockDim.x) ttab[t] = (TC)ctl->ttab[t];
-  for (int t = threadIdx.x; t < 32; t += blockDim.x) etab[t] = c_exp2_32[t];
+  for (int t = threadIdx.x; t < 128; t += blockDim.x) etab[t] = c_exp2_128[t];
 }
 
 // bytes of the two tables at the start of a kernel's dynamic LDS
-constexpr size_t kTabBytes = sizeof(double) * (kMaxTypes * kMaxTypes + 32);
+constexpr size_t kTabBytes = sizeof(double) * (kMaxTypes * kMaxTypes + 128);
 
 // Self overlaps, one thread per slot (global-memory operands).
 template <typename TC>
 __global__ __launch_bounds__(256) void k_arad_self(NbSet set, int nslots, int amax, AradParams p,
                                                    const Ctl *__restrict__ ctl) {
   __shared__ TC ttab[kMaxTypes * kMaxTypes];
-  __shared__ double etab[32];
+  __shared__ double etab[128];
   load_tables(ttab, etab, ctl);
   __syncthreads();
   const int slot = blockIdx.x * blockDim.x + threadIdx.x;
@@ -995,7 +973,7 @@ __global__ __launch_bounds__(256) void k_arad_atomic(NbSet set1, NbSet set2, int
                                                      int amax, int mode, PairArgs pa, AradParams p,
                                                      const Ctl *__restrict__ ctl) {
   __shared__ TC ttab[kMaxTypes * kMaxTypes];
-  __shared__ double etab[32];
+  __shared__ double etab[128];
   load_tables(ttab, etab, ctl);
   __syncthreads();
   const bool generic = ctl->generic != 0;

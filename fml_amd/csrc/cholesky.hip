@@ -85,20 +85,21 @@ __global__ __launch_bounds__(64) void k_potf2(T *__restrict__ a, int64_t lda, in
 }
 
 // Rows r in [r0, n): x = M[r][k0:k0+b] solves x L^T = m_r against the
-// factored diagonal block (LDS), one wave per 64 rows, one row per lane held
-// in registers; column by column, x_c = m_c / L[c][c] then m_t -= x_c L[t][c]
-// for t > c (independent updates, so the row's dependency chain is short).
+// factored diagonal block (LDS, loaded by all 256 threads), one row per
+// thread held in registers; column by column, x_c = m_c / L[c][c] then
+// m_t -= x_c L[t][c] for t > c (independent updates, so the row's
+// dependency chain is short).
 template <typename T>
-__global__ __launch_bounds__(64) void k_trsm(T *__restrict__ a, int64_t lda, int n, int k0, int b,
-                                             int r0, const int32_t *__restrict__ info) {
+__global__ __launch_bounds__(256) void k_trsm(T *__restrict__ a, int64_t lda, int n, int k0, int b,
+                                              int r0, const int32_t *__restrict__ info) {
   __shared__ T L[kNB][kNB + 1];
   if (*info != 0) return;
-  for (int idx = threadIdx.x; idx < b * b; idx += 64) {
+  for (int idx = threadIdx.x; idx < b * b; idx += 256) {
     const int r = idx / b, c = idx % b;
     L[r][c] = (c <= r) ? a[(int64_t)(k0 + r) * lda + k0 + c] : T(0);
   }
   __syncthreads();
-  const int r = r0 + blockIdx.x * 64 + threadIdx.x;
+  const int r = r0 + blockIdx.x * 256 + threadIdx.x;
   if (r >= n) return;
   T *row = a + (int64_t)r * lda + k0;
   T x[kNB];
@@ -117,11 +118,44 @@ __global__ __launch_bounds__(64) void k_trsm(T *__restrict__ a, int64_t lda, int
     if (c < b) row[c] = x[c];
 }
 
+// Tile (tx, ty), tx <= ty < g, of the lower triangle of a g x g tile grid
+// at position `local` of this order: groups of kGroup tile rows, each group
+// walked column by column, so a group's row tiles stay in L2 while its
+// column tiles stream past once (the row-major triangle would stream every
+// column tile once per row).  Returns false past the end.
+__device__ __forceinline__ bool tri_tile(int g, int64_t local, int &tx, int &ty) {
+  if (local >= (int64_t)g * (g + 1) / 2) return false;
+  // row of `local` in row-major order: T(r) <= local < T(r + 1), T(r) = r(r+1)/2
+  int r = (int)((sqrt(8.0 * (double)local + 1.0) - 1.0) * 0.5);
+  while ((int64_t)(r + 1) * (r + 2) / 2 <= local) ++r;
+  while ((int64_t)r * (r + 1) / 2 > local) --r;
+  const int r0 = (r / kGroup) * kGroup;  // first row of the group (it starts at T(r0))
+  const int R = min(kGroup, g - r0);
+  int64_t o = local - (int64_t)r0 * (r0 + 1) / 2;
+  const int64_t full = (int64_t)(r0 + 1) * R;  // columns 0..r0 hold R tiles each
+  if (o < full) {
+    tx = (int)(o / R);
+    ty = r0 + (int)(o % R);
+    return true;
+  }
+  o -= full;
+  for (int c = r0 + 1; c < r0 + R; ++c) {  // column c holds rows c..r0+R-1
+    const int cnt = r0 + R - c;
+    if (o < cnt) {
+      tx = c;
+      ty = c + (int)o;
+      return true;
+    }
+    o -= cnt;
+  }
+  return false;
+}
+
 // C[r][c] -= sum_{t in [kb, ke)} M[r][t] M[c][t] for r in [r0, r1),
 // c in [c0, c1), c <= r, on MT x MT tiles.  kTri: the region is the square
-// [r0, r1)^2 and only its lower-triangle tiles are launched (tile index ->
-// (row, col <= row), rows in order, in XCD-contiguous runs); otherwise a
-// rectangular gx x gy grid in which tiles above the diagonal exit.
+// [r0, r1)^2 and only its lower-triangle tiles are launched (tri_tile order,
+// in XCD-contiguous runs); otherwise a rectangular gx x gy grid in which
+// tiles above the diagonal exit.
 template <typename T, int MT, bool kTri>
 __global__ __launch_bounds__(256, (MT == 128 ? 2 : 1)) void k_update(T *__restrict__ a, int64_t lda, int r0, int r1,
                                                 int c0, int c1, int kb, int ke, int gx, int gy,
@@ -133,13 +167,7 @@ __global__ __launch_bounds__(256, (MT == 128 ? 2 : 1)) void k_update(T *__restri
     const int64_t t = (int64_t)gy * (gy + 1) / 2;
     const int64_t per = (t + kXcds - 1) / kXcds;
     const int64_t L = blockIdx.x;
-    const int64_t local = (L % kXcds) * per + L / kXcds;
-    if (local >= t) return;
-    int row = (int)((sqrt(8.0 * (double)local + 1.0) - 1.0) * 0.5);
-    while ((int64_t)(row + 1) * (row + 2) / 2 <= local) ++row;
-    while ((int64_t)row * (row + 1) / 2 > local) --row;
-    ty = row;
-    tx = (int)(local - (int64_t)row * (row + 1) / 2);
+    if (!tri_tile(gy, (L % kXcds) * per + L / kXcds, tx, ty)) return;
   } else if (!xcd_tile(gx, gy, tx, ty)) {
     return;
   }
@@ -176,7 +204,7 @@ int potrf_impl(T *a, int64_t lda, int n, int32_t *info, hipStream_t st) {
       FMLH_LAUNCH_CHECK("k_potf2");
       const int r0 = k0 + b;
       if (r0 < n) {
-        hipLaunchKernelGGL(k_trsm<T>, dim3(ceil_div(n - r0, 64)), dim3(64), 0, st, a, lda, n, k0,
+        hipLaunchKernelGGL(k_trsm<T>, dim3(ceil_div(n - r0, 256)), dim3(256), 0, st, a, lda, n, k0,
                            b, r0, info);
         FMLH_LAUNCH_CHECK("k_trsm");
         if (r0 < kend) {  // rest of the outer panel

@@ -30,6 +30,7 @@ result is bitwise the same alone and in any mix.
 from __future__ import annotations
 
 import ctypes
+import gc
 import queue
 import threading
 import time
@@ -160,7 +161,16 @@ class KRRServer:
         self._fits.put(r)
         return r.future
 
-    def start(self):
+    def start(self, freeze_gc=True):
+        """Start the predict and fit loops.  freeze_gc: collect, then move
+        every object alive now to the collector's permanent generation
+        (gc.freeze), so the periodic full collections during serving scan
+        only what serving allocates instead of pausing all threads for the
+        whole heap (torch's modules alone hold hundreds of thousands of
+        objects)."""
+        if freeze_gc:
+            gc.collect()
+            gc.freeze()
         self._threads = [threading.Thread(target=self._predict_loop, daemon=True),
                          threading.Thread(target=self._fit_loop, daemon=True)]
         for th in self._threads:
