@@ -75,6 +75,7 @@ constexpr int kMaxSig = 8;
 constexpr int kPairThreads = 256;
 constexpr int kTileRounds = 4;  // atom-pair rounds per thread in a tile
 constexpr int kMaxAmax = 29;    // largest amax the tile path takes
+constexpr int kStageAtoms = 128;  // most set-2 atoms one tile stages
 
 struct AradParams {
   double width, cut, rw, cw;
@@ -646,6 +647,8 @@ __host__ __device__ inline int tile_j(int ni, int s, int nsig, int lanes, int j_
   if (s > 0) {
     const int jc = j_recs_cap / (s * s);
     J = J < (jc > 1 ? jc : 1) ? J : (jc > 1 ? jc : 1);
+    const int ja = kStageAtoms / s;  // per-atom stage (32 bytes an atom) stays small
+    J = J < (ja > 1 ? ja : 1) ? J : (ja > 1 ? ja : 1);
   }
   return J < jmax_red ? J : jmax_red;
 }
@@ -944,14 +947,19 @@ __global__ __launch_bounds__(kPairThreads, kWaves) void k_arad_tiles(NbSet set1,
       if (nr > 3) rk[3 * kPairThreads] = exp(D3 * is2);
     }
     __syncthreads();
-    // a molecule pair's sum runs over its atom pairs in the reference's
-    // order, a (set-1 atom) inner and b outer (farad_kernels.f90:602-604;
-    // :211 for the single-sigma form), whatever the tile shape
-    if ((int)threadIdx.x < J * nsig) {
-      const int jj = threadIdx.x / nsig, k = threadIdx.x % nsig;
-      const double *rk = red + k * total;
+    // one wave per (molecule pair, sigma) in turn: lane l adds the atom pairs
+    // t = l, l + 64, ... (t = a + n_i b, the reference's enumeration,
+    // farad_kernels.f90:602-604 and :211) in ascending order, then a
+    // butterfly adds the 64 lanes -- a fixed order that depends only on the
+    // molecule pair, whatever the tile shape
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int cidx = wave; cidx < J * nsig; cidx += kPairThreads / 64) {
+      const int jj = cidx / nsig, k = cidx % nsig;
+      const double *rk = red + k * total + jj * P;
       double v = 0.0;
-      for (int l = jj * P; l < (jj + 1) * P; ++l) v += rk[l];
+      for (int l = lane; l < P; l += 64) v += rk[l];
+      v = wave_sum(v);
+      if (lane != 0) continue;
       const int j = ta.members[tl.off + jj];
       if (ta.mode == FMLH_ARAD_KERNEL) {
         ta.out[(int64_t)i + (int64_t)ta.nm1 * j] = v;

@@ -22,8 +22,10 @@
 // undefined but no kernel waits on anything.
 //
 // Solve: blocked forward (L w = b) and backward (L^T x = w) substitution
-// over 64-row blocks: a multi-workgroup partial product of the already
-// solved part followed by a one-workgroup triangular solve of the block.
+// over 64-row blocks.  One right-hand side: one launch per direction, the
+// blocks chained through ready flags (k_trsv_chain).  Several: per block a
+// multi-workgroup partial product of the already solved part followed by a
+// one-workgroup triangular solve of the block.
 #include <math.h>
 
 #include "common.h"
@@ -234,6 +236,147 @@ int potrf_impl(T *a, int64_t lda, int n, int32_t *info, hipStream_t st) {
   return 0;
 }
 
+// ---- inverse from the factor (LAPACK dpotri('U') semantics) ------------
+//
+// inv(M) = L^{-T} L^{-1}.  X = L^{-1} is formed by blocked forward
+// substitution with the identity as right-hand side, stored transposed in
+// the workspace (XT[j][k] = X[k][j], row-major), so that both of its
+// products are "NT" MFMA tiles over contiguous rows:
+//   block row b (rows r0..r0+bs): C = L[b rows][0:r0] X[0:r0][j] for the
+//     columns j < r0 (k_potri_gemm; X is lower triangular, so columns
+//     j >= r0 have no entries above block b), then every column j < r0+bs
+//     solves L_bb x = e_j - C (k_potri_diag, one column per thread);
+//   inv(M)[r][c] = sum_k XT[r][k] XT[c][k] over the lower-triangle tiles
+//     (k_potri_syrk), written over the factor.
+
+// XT[j][r0 + m] = -sum_{k < r0} M[r0 + m][k] XT[j][k], m < bs, j < r0.
+template <typename T>
+__global__ __launch_bounds__(256) void k_potri_gemm(const T *__restrict__ a, int64_t lda, int r0,
+                                                    int bs, T *__restrict__ xt, int64_t ldx,
+                                                    const int32_t *__restrict__ info) {
+  __shared__ TileSmem<T, kTile> s;
+  if (*info != 0) return;
+  const int j0 = blockIdx.x * kTile;
+  TileAcc<T, kTile> acc;
+  tile_nt<T, false, kTile>(a + (int64_t)r0 * lda, lda, bs, xt + (int64_t)j0 * ldx, ldx, r0 - j0, r0,
+                           s, acc);
+#pragma unroll
+  for (int im = 0; im < 2; ++im)
+#pragma unroll
+    for (int in = 0; in < 2; ++in)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = tile_row<T, kTile>(im, r);
+        const int j = j0 + tile_col<kTile>(in);
+        if (m < bs && j < r0) xt[(int64_t)j * ldx + r0 + m] = -acc.v[im][in][r];
+      }
+}
+
+// Column j < r0 + bs: x = L_bb^{-1} (e_j - C) for the block rows r0..r0+bs,
+// forward substitution in registers, L_bb in LDS.  A zero pivot sets *info.
+template <typename T>
+__global__ __launch_bounds__(256) void k_potri_diag(const T *__restrict__ a, int64_t lda, int r0,
+                                                    int bs, T *__restrict__ xt, int64_t ldx,
+                                                    int32_t *__restrict__ info) {
+  __shared__ T L[kNB][kNB + 1];
+  if (*info != 0) return;
+  for (int idx = threadIdx.x; idx < bs * bs; idx += 256) {
+    const int r = idx / bs, c = idx % bs;
+    L[r][c] = (c <= r) ? a[(int64_t)(r0 + r) * lda + r0 + c] : T(0);
+  }
+  __syncthreads();
+  if (threadIdx.x < bs && L[threadIdx.x][threadIdx.x] == T(0)) atomicCAS(info, 0, r0 + threadIdx.x + 1);
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  if (j >= r0 + bs) return;
+  T *row = xt + (int64_t)j * ldx + r0;
+  T x[kNB];
+#pragma unroll
+  for (int c = 0; c < kNB; ++c) {
+    T v = T(0);
+    if (c < bs) v = (j < r0) ? row[c] : T(j - r0 == c ? 1 : 0);
+    x[c] = v;
+  }
+#pragma unroll
+  for (int c = 0; c < kNB; ++c) {
+    if (c < bs) {
+      x[c] = x[c] / L[c][c];
+#pragma unroll
+      for (int t = c + 1; t < kNB; ++t) x[t] -= x[c] * L[t][c];
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < kNB; ++c)
+    if (c < bs) row[c] = x[c];
+}
+
+// M[r][c] = sum_{k >= tile row start} XT[r][k] XT[c][k] for c <= r < n,
+// lower-triangle tiles in tri_tile order (XT[r][k] = 0 for k < r).
+template <typename T, int MT>
+__global__ __launch_bounds__(256, (MT == 128 ? 2 : 1)) void k_potri_syrk(
+    const T *__restrict__ xt, int64_t ldx, int n, int g, T *__restrict__ a, int64_t lda,
+    const int32_t *__restrict__ info) {
+  __shared__ TileSmem<T, MT> s;
+  if (*info != 0) return;
+  int tx, ty;
+  const int64_t t = (int64_t)g * (g + 1) / 2;
+  const int64_t per = (t + kXcds - 1) / kXcds;
+  const int64_t L = blockIdx.x;
+  if (!tri_tile(g, (L % kXcds) * per + L / kXcds, tx, ty)) return;
+  const int tr = ty * MT, tc = tx * MT;
+  TileAcc<T, MT> acc;
+  tile_nt<T, false, MT>(xt + (int64_t)tr * ldx + tr, ldx, n - tr, xt + (int64_t)tc * ldx + tr, ldx,
+                        n - tc, n - tr, s, acc);
+  constexpr int NB = MT / 32;
+#pragma unroll
+  for (int im = 0; im < NB; ++im)
+#pragma unroll
+    for (int in = 0; in < NB; ++in)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gr = tr + tile_row<T, MT>(im, r);
+        const int gc = tc + tile_col<MT>(in);
+        if (gr < n && gc <= gr) a[(int64_t)gr * lda + gc] = acc.v[im][in][r];
+      }
+}
+
+__global__ void k_fill_rows(double *p, int64_t count) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) p[i] = 0.0;
+}
+
+int potri_impl(double *a, int64_t lda, int n, int32_t *info, double *xt, hipStream_t st) {
+  const int64_t ldx = n;
+  hipLaunchKernelGGL(k_set_int_impl, dim3(1), dim3(1), 0, st, info, 0);
+  FMLH_LAUNCH_CHECK("k_set_int");
+  const int64_t cells = (int64_t)n * ldx;
+  hipLaunchKernelGGL(k_fill_rows, dim3((unsigned)ceil_div64(cells, 256)), dim3(256), 0, st, xt, cells);
+  FMLH_LAUNCH_CHECK("k_fill_rows");
+  for (int r0 = 0; r0 < n; r0 += kNB) {
+    const int bs = min(kNB, n - r0);
+    if (r0 > 0) {
+      hipLaunchKernelGGL(k_potri_gemm<double>, dim3(ceil_div(r0, kTile)), dim3(256), 0, st, a, lda,
+                         r0, bs, xt, ldx, info);
+      FMLH_LAUNCH_CHECK("k_potri_gemm");
+    }
+    hipLaunchKernelGGL(k_potri_diag<double>, dim3(ceil_div(r0 + bs, 256)), dim3(256), 0, st, a, lda,
+                       r0, bs, xt, ldx, info);
+    FMLH_LAUNCH_CHECK("k_potri_diag");
+  }
+  if (n >= 4 * 128) {
+    const int g = ceil_div(n, 128);
+    const int64_t t = (int64_t)g * (g + 1) / 2;
+    hipLaunchKernelGGL((k_potri_syrk<double, 128>), dim3((unsigned)xcd_grid((int)t, 1)), dim3(256), 0,
+                       st, xt, ldx, n, g, a, lda, info);
+  } else {
+    const int g = ceil_div(n, kTile);
+    const int64_t t = (int64_t)g * (g + 1) / 2;
+    hipLaunchKernelGGL((k_potri_syrk<double, kTile>), dim3((unsigned)xcd_grid((int)t, 1)), dim3(256),
+                       0, st, xt, ldx, n, g, a, lda, info);
+  }
+  FMLH_LAUNCH_CHECK("k_potri_syrk");
+  return 0;
+}
+
 // ---- triangular solves -------------------------------------------------
 
 constexpr int kChunk = 256;
@@ -330,17 +473,133 @@ __global__ __launch_bounds__(256) void k_block_solve(const T *__restrict__ a, in
   }
 }
 
+// One-launch triangular solves for a single right-hand side.  Workgroups
+// take 64-row blocks in ticket order (forward: block = ticket; backward:
+// block = last - ticket), accumulate the products of their rows with each
+// solved block as its ready flag appears (blocks in solve order), solve
+// their diagonal block in one wave and publish it: the values, a release
+// fence at device scope, then the flag.  A ticket waits only on smaller
+// tickets, whose workgroups are already running, so every wait ends.
+// ctl: [0] ticket counter, [1 + b] ready flag of block b; zeroed before.
+
+__device__ __forceinline__ void await_flag(const int32_t *flag) {
+  if (threadIdx.x == 0)
+    while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
+      __builtin_amdgcn_s_sleep(1);
+  __syncthreads();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
+template <typename T, bool kForward>
+__global__ __launch_bounds__(256) void k_trsv_chain(const T *__restrict__ a, int64_t lda, int n,
+                                                    T *x, int32_t *ctl) {
+  __shared__ T L[kNB][kNB + 1];
+  __shared__ T part[4][kNB];
+  __shared__ T rhs[kNB];
+  __shared__ int s_ticket;
+  const int nblk = (n + kNB - 1) / kNB;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (tid == 0) s_ticket = atomicAdd(&ctl[0], 1);
+  __syncthreads();
+  const int ticket = s_ticket;
+  if (ticket >= nblk) return;
+  const int b = kForward ? ticket : nblk - 1 - ticket;
+  const int r0 = b * kNB, bs = min(kNB, n - r0);
+  for (int idx = tid; idx < bs * bs; idx += 256) {
+    const int r = idx / bs, c = idx % bs;
+    L[r][c] = (c <= r) ? a[(int64_t)(r0 + r) * lda + r0 + c] : T(0);
+  }
+  if (kForward) {
+    // rows wave*16 .. wave*16+15 of the block, lanes over each solved block's columns
+    T acc[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) acc[i] = T(0);
+    for (int j = 0; j < b; ++j) {
+      await_flag(&ctl[1 + j]);
+      const T xv = x[j * kNB + lane];
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        const int r = wave * 16 + i;
+        if (r < bs) acc[i] = fma(a[(int64_t)(r0 + r) * lda + j * kNB + lane], xv, acc[i]);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      T v = acc[i];
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+      if (lane == 0) part[0][wave * 16 + i] = v;
+    }
+    __syncthreads();
+    if (tid < bs) rhs[tid] = x[r0 + tid] - part[0][tid];
+  } else {
+    // lanes over the block's columns, waves over the rows of each solved block
+    T acc = T(0);
+    for (int j = nblk - 1; j > b; --j) {
+      await_flag(&ctl[1 + j]);
+      const int jb = min(kNB, n - j * kNB);
+      if (lane < bs)
+        for (int tt = wave; tt < jb; tt += 4) {
+          const int row = j * kNB + tt;
+          acc = fma(a[(int64_t)row * lda + r0 + lane], x[row], acc);
+        }
+    }
+    part[wave][lane] = acc;
+    __syncthreads();
+    if (tid < bs) rhs[tid] = x[r0 + tid] - (((part[0][tid] + part[1][tid]) + part[2][tid]) + part[3][tid]);
+  }
+  __syncthreads();
+  if (wave == 0) {
+    T v = lane < bs ? rhs[lane] : T(0);
+    if (kForward) {
+      for (int c = 0; c < bs; ++c) {
+        const T xc = __shfl(v, c, 64) / L[c][c];
+        if (lane == c) v = xc;
+        if (lane > c) v -= L[lane][c] * xc;
+      }
+    } else {
+      for (int c = bs - 1; c >= 0; --c) {
+        const T xc = __shfl(v, c, 64) / L[c][c];
+        if (lane == c) v = xc;
+        if (lane < c) v -= L[c][lane] * xc;
+      }
+    }
+    if (lane < bs) x[r0 + lane] = v;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    if (lane == 0) __hip_atomic_store(&ctl[1 + b], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+__global__ void k_zero_i32(int32_t *p, int count) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) p[i] = 0;
+}
+
 template <typename T>
 size_t potrs_ws(int n, int nrhs) {
-  return WsCarver::need(sizeof(T) * (size_t)ceil_div(n > 0 ? n : 1, kChunk) * kNB *
-                        (nrhs > 0 ? nrhs : 1));
+  const size_t blocked = WsCarver::need(sizeof(T) * (size_t)ceil_div(n > 0 ? n : 1, kChunk) * kNB *
+                                        (nrhs > 0 ? nrhs : 1));
+  const size_t chain = WsCarver::need(sizeof(int32_t) * 2 * ((size_t)ceil_div(n > 0 ? n : 1, kNB) + 1));
+  return blocked > chain ? blocked : chain;
 }
 
 template <typename T>
 int potrs_impl(const T *a, int64_t lda, int n, T *x, int64_t ldx, int nrhs, void *ws,
                hipStream_t st) {
-  T *part = static_cast<T *>(ws);
   const int nblk = ceil_div(n, kNB);
+  if (nrhs == 1) {  // one launch per direction
+    int32_t *ctl = static_cast<int32_t *>(ws);
+    hipLaunchKernelGGL(k_zero_i32, dim3(ceil_div(2 * (nblk + 1), 256)), dim3(256), 0, st, ctl,
+                       2 * (nblk + 1));
+    FMLH_LAUNCH_CHECK("k_zero_i32");
+    hipLaunchKernelGGL((k_trsv_chain<T, true>), dim3(nblk), dim3(256), 0, st, a, lda, n, x, ctl);
+    FMLH_LAUNCH_CHECK("k_trsv_chain<fwd>");
+    hipLaunchKernelGGL((k_trsv_chain<T, false>), dim3(nblk), dim3(256), 0, st, a, lda, n, x,
+                       ctl + nblk + 1);
+    FMLH_LAUNCH_CHECK("k_trsv_chain<bwd>");
+    return 0;
+  }
+  T *part = static_cast<T *>(ws);
   for (int kb = 0; kb < nblk; ++kb) {
     const int k0 = kb * kNB, b = min(kNB, n - k0);
     const int nch = ceil_div(k0, kChunk);
@@ -413,6 +672,25 @@ int fmlh_spotrf(float *a, int64_t lda, int n, int32_t *info, void *ws, size_t ws
   int rc = potrf_impl<float>(a, lda, n, info, as_stream(stream));
   timing_end(as_stream(stream));
   return rc;
+}
+
+size_t fmlh_potri_ws_bytes(int n) {
+  return n > 0 ? WsCarver::need(sizeof(double) * (size_t)n * (size_t)n) : 0;
+}
+
+int fmlh_potri(double *a, int64_t lda, int n, int32_t *info, void *ws, size_t ws_bytes,
+               void *stream) {
+  FMLH_ARG_CHECK(n >= 0, 3);
+  FMLH_ARG_CHECK(lda >= (n > 1 ? n : 1), 2);
+  FMLH_ARG_CHECK(info != nullptr, 4);
+  if (n == 0) {
+    hipLaunchKernelGGL(k_set_int_impl, dim3(1), dim3(1), 0, as_stream(stream), info, 0);
+    FMLH_LAUNCH_CHECK("k_set_int");
+    return 0;
+  }
+  FMLH_ARG_CHECK(a != nullptr, 1);
+  FMLH_ARG_CHECK(ws != nullptr && ws_bytes >= fmlh_potri_ws_bytes(n), 5);
+  return potri_impl(a, lda, n, info, static_cast<double *>(ws), as_stream(stream));
 }
 
 int fmlh_potrs(const double *a, int64_t lda, int n, double *x, int64_t ldx, int nrhs, void *ws,

@@ -169,6 +169,50 @@ def _arad_call(mode, q1, z1, n1, q2, z2, n2, sigmas, width, cut, rw, cw, precisi
     return out
 
 
+def fget_alpha_arad(q, z, n, y, sigma, llambda, alpha=None):
+    """farad_kernels.f90:231-436 -- regression coefficients of molecular ARAD
+    KRR, one per atom: the molecular kernel K of the training set (the
+    kernel of fget_kernel_arad: width .2, cut 6.0, r_width 1, c_width .5),
+    lambda added to its diagonal, (K + lambda I)^-1 y from dpotrf('U') +
+    dpotri('U') + dsymv('U') (here: fmlh_potrf, fmlh_potri and the
+    upper-triangle product), and molecule i's value repeated for each of its
+    n(i) atoms.
+
+    q (nm, amax, 5, amax), z (nm, amax, 2), n (nm,), y (nm,).  alpha: the
+    caller's (sum(n),) float64 array, filled in place as F2PY fills the
+    reference's intent(inout) argument; a new array when None.  Returns
+    alpha.  A failed factorisation or inversion prints the reference's
+    warning line (the coefficients are then NaN)."""
+    from ..krr import add_diagonal, sym_residual
+    from ..math.cho_solve import potrf_inplace, potri_inplace
+    dev = is_device_tensor(q)
+    counts = to_host(n).astype(np.int64) if is_device_tensor(n) else np.asarray(n, dtype=np.int64)
+    K = _arad_call(_native.ARAD_KERNEL, q, z, n, None, None, None, [sigma], 0.2, 6.0, 1.0, 0.5)
+    nm = int(K.shape[0])
+    add_diagonal(K, float(llambda))
+    info = potrf_inplace(K)
+    if info > 0:
+        print(" WARNING: Cholesky decomposition DPOTRF() exited with error code:", info)
+        K.fill_(float("nan"))
+    else:
+        info = potri_inplace(K)
+        if info > 0:
+            print(" WARNING: Cholesky inversion DPOTRI() exited with error code:", info)
+    yd = to_device(y, np.float64)
+    zero = torch().zeros(nm, dtype=torch().float64, device="cuda")
+    ly = empty((nm,), np.float64)
+    sym_residual(K, 0.0, yd, zero, ly)   # ly = 0 - K^-1 y
+    ly.neg_()
+    per_atom = torch().repeat_interleave(ly, torch().as_tensor(counts, device="cuda"))
+    if alpha is None:
+        return per_atom if dev else to_host(per_atom)
+    if is_device_tensor(alpha):
+        alpha.copy_(per_atom)
+    else:
+        alpha[...] = to_host(per_atom)
+    return alpha
+
+
 def _ret(t, dev, perm):
     return t.permute(*perm) if dev else to_host(t).transpose(perm)
 

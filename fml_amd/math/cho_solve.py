@@ -88,6 +88,52 @@ def cho_solve_factored(F: CholeskyFactor, y):
     return x if dev else to_host(x)
 
 
+def potri_inplace(m):
+    """Inverse from the factor held in m (as left by potrf_inplace): m's
+    row-major lower triangle (the upper triangle of the Fortran matrix)
+    receives inv(A)'s; returns info."""
+    t = torch()
+    n = int(m.shape[0])
+    info = t.zeros(1, dtype=t.int32, device="cuda")
+    lib = _native.load()
+    wsb = lib.fmlh_potri_ws_bytes(n)
+    ws = workspace(wsb)
+    _native.call("fmlh_potri", _native.ptr(m), m.stride(0), n, _native.ptr(info), _native.ptr(ws), wsb,
+                 _native.stream_ptr())
+    return int(info.item())
+
+
+def cho_invert(A):
+    """Inverse of a symmetric positive definite matrix through its Cholesky
+    factor (cho_solve.py:27-30 -> fcho_solve.f90:49-68: dpotrf('U') then
+    dpotri('U'), both on the upper triangle).  Returns the Fortran-ordered
+    float64 result the reference's call leaves: inv(A) in the upper triangle,
+    A's own values in the strictly lower triangle.
+
+    Deviations, by design: the caller's A is never overwritten (the
+    reference's asfortranarray + in-place call clobbers a Fortran-ordered
+    float64 A); after a failed factorisation the reference's warning line is
+    printed and the upper triangle is NaN instead of undefined."""
+    if len(A.shape) != 2 or A.shape[0] != A.shape[1]:
+        raise ValueError('expected square matrix')
+    dev = is_device_tensor(A)
+    m = _fortran_memory(A)
+    if m.dtype != torch().float64:
+        m = m.double()
+    n = int(m.shape[0])
+    if n:
+        info = potrf_inplace(m)
+        if info != 0:
+            print(" WARNING: Cholesky decomposition DPOTRF() exited with error code:", info)
+            tri = torch().tril(torch().ones(n, n, dtype=torch().bool, device="cuda"))
+            m.masked_fill_(tri, float("nan"))
+        else:
+            info = potri_inplace(m)
+            if info != 0:
+                print(" WARNING: Cholesky Inversion DPOTRI() exited with error code:", info)
+    return m.t() if dev else to_host(m).T
+
+
 def cho_solve(A, y):
     """Solves A x = y for x with a Cholesky factorisation (cho_solve.py:33-58)."""
     if len(A.shape) != 2 or A.shape[0] != A.shape[1]:

@@ -249,6 +249,18 @@ size_t fmlh_potrs_ws_bytes(int n, int nrhs);
 int fmlh_potrs(const double *a, int64_t lda, int n, double *x, int64_t ldx,
                int nrhs, void *ws, size_t ws_bytes, void *stream);
 
+/* Inverse from the factor, as LAPACK dpotri('U'): a holds U from fmlh_potrf
+ * (upper triangle, Fortran order) and receives the upper triangle of
+ * inv(A) = inv(U) inv(U)^T; the strictly lower part is not touched.
+ * info (device int32): 0, or k > 0 when U(k,k) is zero.  Replaces the
+ * dpotri call of fcho_invert (fml/math/fcho_solve.f90:49-68, Python entry
+ * cho_invert, fml/math/cho_solve.py:27-30) and of fget_alpha_arad
+ * (fml/kernels/farad_kernels.f90:414).  ws: fmlh_potri_ws_bytes(n) bytes
+ * (an n x n scratch matrix). */
+size_t fmlh_potri_ws_bytes(int n);
+int fmlh_potri(double *a, int64_t lda, int n, int32_t *info, void *ws,
+               size_t ws_bytes, void *stream);
+
 /* Single-precision factor for the mixed-precision form (same layout and
  * semantics on float data). */
 size_t fmlh_spotrf_ws_bytes(int n);

@@ -113,6 +113,8 @@ def ref(variant: str = "libfml_ref"):
         r.ref_arad_l2_distance_all.argtypes = [_c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_int_p, _c_int_p,
                                                I, I, I, D, D, D, D, _c_dbl_p]
         r.ref_cho_solve.argtypes = [_c_dbl_p, I, _c_dbl_p, _c_dbl_p]
+        r.ref_cho_invert.argtypes = [_c_dbl_p, I]
+        r.ref_alpha_arad.argtypes = [_c_dbl_p, _c_dbl_p, _c_int_p, _c_dbl_p, D, D, I, I, I, _c_dbl_p]
         _ref[variant] = r
     return _ref[variant]
 
@@ -344,6 +346,26 @@ def ref_cho_solve(A, y, variant="libfml_ref"):
     return x
 
 
+def ref_cho_invert(A, variant="libfml_ref"):
+    """fcho_invert on a private Fortran copy of A: upper triangle inv(A),
+    strictly lower triangle A's."""
+    Af = np.array(A, dtype=np.float64, order="F", copy=True)
+    ref(variant).ref_cho_invert(Af.ctypes.data_as(_c_dbl_p), Af.shape[0])
+    return Af
+
+
+def ref_alpha_arad(q, z, n, y, sigma, lam, variant="libfml_ref"):
+    """fget_alpha_arad (farad_kernels.f90:231-436): alpha per atom."""
+    q, z = _f(q), _f(z)
+    n = np.asarray(n, dtype=np.int32)
+    nm, amax = q.shape[0], q.shape[1]
+    na = int(n.sum())
+    alpha = np.zeros(na)
+    ref(variant).ref_alpha_arad(_dp(q), _dp(z), _ip(n), _dp(_c(y)), float(sigma), float(lam), nm, amax,
+                                na, alpha.ctypes.data_as(_c_dbl_p))
+    return alpha
+
+
 # ---------------------------------------------------------------------------
 # cho_solve: LAPACK contract through scipy (fml/math/fcho_solve.f90:23-47)
 # ---------------------------------------------------------------------------
@@ -357,6 +379,34 @@ def cho_solve(A, y):
         return np.full(A.shape[0], np.nan), info
     x, info2 = lapack.dpotrs(c, np.asarray(y, dtype=np.float64), lower=0)
     return x, info2
+
+
+def cho_invert(A):
+    """dpotrf('U') + dpotri('U') on the upper triangle of A, as fcho_invert
+    (fcho_solve.f90:49-68).  Returns (Fortran array: upper inv(A), strictly
+    lower A's), info of the first failing call."""
+    from scipy.linalg import lapack
+    Af = np.array(A, dtype=np.float64, order="F", copy=True)
+    c, info = lapack.dpotrf(Af, lower=0, clean=0, overwrite_a=0)
+    if info != 0:
+        return c, info
+    inv, info2 = lapack.dpotri(c, lower=0)
+    return inv, info2
+
+
+def alpha_arad(q, z, n, y, sigma, lam):
+    """fget_alpha_arad (farad_kernels.f90:231-436) restated: the molecular
+    kernel of kernel_arad (fget_kernel_arad's, the same loops), lambda on the
+    diagonal, dpotrf + dpotri + dsymv on the upper triangle, each molecule's
+    value repeated for its atoms.  Returns (alpha, info)."""
+    from scipy.linalg import blas, lapack
+    K = kernel_arad(q, z, n, sigma)
+    K = np.array(K, dtype=np.float64, order="F", copy=True)
+    K[np.diag_indices_from(K)] += lam
+    c, info = lapack.dpotrf(K, lower=0, clean=0)
+    inv, info2 = lapack.dpotri(c, lower=0)
+    ly = blas.dsymv(1.0, inv, np.asarray(y, dtype=np.float64), lower=0)
+    return np.repeat(ly, np.asarray(n, dtype=np.int64)), (info or info2)
 
 
 def cho_solve_c(A, y):

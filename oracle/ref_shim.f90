@@ -137,6 +137,20 @@ module fml_ref_shim
       double precision, dimension(:), intent(in) :: y
       double precision, dimension(:), intent(inout) :: x
     end subroutine
+    ! fml/math/fcho_solve.f90:49-53
+    subroutine fcho_invert(A)
+      double precision, dimension(:,:), intent(inout) :: A
+    end subroutine
+    ! fml/kernels/farad_kernels.f90:231-266
+    subroutine fget_alpha_arad(q, z, n, y, sigma, lambda, alpha)
+      double precision, dimension(:,:,:,:), intent(in) :: q
+      double precision, dimension(:,:,:), intent(in) :: z
+      integer, dimension(:), intent(in) :: n
+      double precision, dimension(:), intent(in) :: y
+      double precision, intent(in) :: sigma
+      double precision, intent(in) :: lambda
+      double precision, dimension(:), intent(inout) :: alpha
+    end subroutine
   end interface
 
 contains
@@ -247,6 +261,24 @@ contains
     real(c_double), intent(in) :: y(n)
     real(c_double), intent(inout) :: x(n)
     call fcho_solve(a, y, x)
+  end subroutine
+
+  subroutine ref_cho_invert(a, n) bind(C, name="ref_cho_invert")
+    integer(c_int), value :: n
+    real(c_double), intent(inout) :: a(n, n)
+    call fcho_invert(a)
+  end subroutine
+
+  subroutine ref_alpha_arad(q, z, n, y, sigma, lambda, nm, amax, na, alpha) &
+      & bind(C, name="ref_alpha_arad")
+    integer(c_int), value :: nm, amax, na
+    real(c_double), intent(in) :: q(nm, amax, 5, amax)
+    real(c_double), intent(in) :: z(nm, amax, 2)
+    integer(c_int), intent(in) :: n(nm)
+    real(c_double), intent(in) :: y(nm)
+    real(c_double), value :: sigma, lambda
+    real(c_double), intent(inout) :: alpha(na)
+    call fget_alpha_arad(q, z, n, y, sigma, lambda, alpha)
   end subroutine
 
 end module fml_ref_shim

@@ -3,7 +3,8 @@ Fortran (compiled unchanged by oracle/Makefile into oracle/_ref/libfml_ref.so,
 IEEE order) for the hot-path routines.  Run in a tree where that library is
 built (make -C oracle ref):
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py            # every fixture
+    python tests/golden/make_golden.py inverse    # inverse.npz only
 
 Inputs: the reference's test molecules tests/xyz/01..10.xyz (copied to
 tests/golden/xyz) and small seeded synthetic sets.  Outputs go next to this
@@ -75,8 +76,25 @@ def main():
     Kc[np.diag_indices_from(Kc)] += 1e-6
     y = ms.energies.copy()
     np.savez_compressed(os.path.join(HERE, "cho_solve.npz"), A=Kc, y=y, x=oracle.ref_cho_solve(Kc, y))
+    inverse_fixtures(Kc, X, z, n)
     print("golden fixtures written")
 
 
+def inverse_fixtures(Kc=None, X=None, z=None, n=None):
+    """fcho_invert on the cho_solve system, fget_alpha_arad on the xyz
+    molecules (sigma 10, lambda 1e-3, y = their atom counts times -0.5)."""
+    if Kc is None:
+        Kc = np.load(os.path.join(HERE, "cho_solve.npz"))["A"]
+        d = np.load(os.path.join(HERE, "arad.npz"))
+        X, z, n = d["X"], d["z"], d["n"]
+    y = -0.5 * n.astype(np.float64) + 0.01 * np.arange(len(n))
+    np.savez_compressed(os.path.join(HERE, "inverse.npz"), A=Kc, inv=oracle.ref_cho_invert(Kc),
+                        y_arad=y, alpha_arad=oracle.ref_alpha_arad(X, z, n, y, 10.0, 1e-3))
+
+
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["inverse"]:
+        assert oracle.ref_available(), "build oracle/_ref first (make -C oracle ref)"
+        inverse_fixtures()
+    else:
+        main()

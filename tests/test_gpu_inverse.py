@@ -1,0 +1,94 @@
+"""GPU inverse from the Cholesky factor (fmlh_potri: cho_invert,
+fml/math/fcho_solve.f90:49-68) and the all-in-one ARAD regression
+coefficients (fget_alpha_arad, fml/kernels/farad_kernels.f90:231-436),
+against the oracle (scipy's dpotrf/dpotri/dsymv, bitwise equal to the
+reference's Fortran -- tests/test_oracle.py) and the golden vectors.
+
+Tolerances, relative to the largest entry, from kappa * n * u (two
+backward-stable algorithms agree to about that): random SPD matrices with
+kappa <= 6: 1e-13; the golden Gaussian system (kappa 1.5e6, n 40): 1e-7;
+the golden ARAD system (sigma 10, lambda 1e-3; kappa 2.9e4, n 10): 1e-10;
+50 uneven molecules (sigma 20, lambda 1e-2; kappa 1e6): 1e-7."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from fml_amd import synth
+from fml_amd.arad import ARAD
+from fml_amd.kernels.kernels import arad_types
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _spd(n, seed):
+    rng = np.random.default_rng(seed)
+    B = rng.normal(size=(n, n))
+    return B @ B.T / n + np.eye(n)
+
+
+@pytest.mark.parametrize("n", [1, 5, 63, 64, 65, 130, 600])
+def test_cho_invert_matches_lapack(gpu, n):
+    from fml_amd.math import cho_invert
+    A = np.asfortranarray(_spd(n, 50 + n))
+    A_before = A.copy()
+    got = cho_invert(A)
+    assert np.array_equal(A, A_before)  # the caller's matrix is not overwritten
+    ref, info = oracle.cho_invert(A)
+    assert info == 0
+    up = np.triu_indices(n)
+    scale = np.max(np.abs(ref[up]))
+    assert np.max(np.abs(got[up] - ref[up])) <= 1e-13 * scale
+    assert np.array_equal(np.tril(got, -1), np.tril(A, -1))  # strictly lower part: A's own
+    full = np.triu(got) + np.triu(got, 1).T
+    assert np.max(np.abs(full @ A - np.eye(n))) < 1e-12
+
+
+def test_cho_invert_golden_and_failure(gpu, capsys):
+    from fml_amd.math import cho_invert
+    d = np.load(os.path.join(GOLD, "inverse.npz"))
+    got = cho_invert(d["A"])
+    up = np.triu_indices(d["A"].shape[0])
+    ref = d["inv"][up]
+    # the golden system is the 40-molecule Gaussian kernel + 1e-6 I: kappa 1.5e6
+    assert np.max(np.abs(got[up] - ref)) <= 1e-7 * np.max(np.abs(ref))
+    A = np.eye(6)
+    A[3, 3] = -1.0
+    bad = cho_invert(A)
+    assert "DPOTRF() exited with error code: 4" in capsys.readouterr().out
+    assert np.isnan(bad[np.triu_indices(6)]).all()
+
+
+def test_fget_alpha_arad(gpu):
+    from fml_amd.kernels import fget_alpha_arad
+    a = np.load(os.path.join(GOLD, "arad.npz"))
+    d = np.load(os.path.join(GOLD, "inverse.npz"))
+    got = fget_alpha_arad(a["X"], a["z"], a["n"], d["y_arad"], 10.0, 1e-3)
+    ref = d["alpha_arad"]
+    assert got.shape == ref.shape == (int(a["n"].sum()),)
+    assert np.max(np.abs(got - ref)) <= 1e-10 * np.max(np.abs(ref))
+    # the reference fills the caller's alpha in place
+    alpha = np.zeros_like(ref)
+    out = fget_alpha_arad(a["X"], a["z"], a["n"], d["y_arad"], 10.0, 1e-3, alpha)
+    assert out is alpha and np.array_equal(alpha, got)
+    # per molecule it equals the cho_solve of the same system
+    from fml_amd.kernels import fget_kernel_arad
+    from fml_amd.math import cho_solve
+    K = fget_kernel_arad(a["X"], a["z"], a["n"], 10.0) + 1e-3 * np.eye(len(a["n"]))
+    x = cho_solve(K, d["y_arad"])
+    per_mol = got[np.cumsum(a["n"]) - 1]
+    assert np.max(np.abs(per_mol - x)) <= 1e-9 * np.max(np.abs(x))
+
+
+def test_fget_alpha_arad_uneven_molecules(gpu):
+    from fml_amd.kernels import fget_alpha_arad
+    ms = synth.generate(50, seed=51, nmin=1, nmax=29, dist="geometric")
+    ar = ARAD(maxMolSize=29, maxAts=29)
+    X = np.array([ar.describe(ms.molecule(m)[1], ms.molecule(m)[0]) for m in range(ms.nmol)])
+    z, n = arad_types([ms.molecule(m)[0] for m in range(ms.nmol)], 29)
+    got = fget_alpha_arad(X, z, n, ms.energies, 20.0, 1e-2)
+    ref, info = oracle.alpha_arad(X, z, n, ms.energies, 20.0, 1e-2)
+    assert info == 0
+    assert np.max(np.abs(got - ref)) <= 1e-7 * np.max(np.abs(ref))

@@ -303,6 +303,40 @@ def test_cho_solve_closed_forms():
     assert np.allclose(np.array([[4, 2], [2, 3.0]]) @ x, [1, 2])
 
 
+def test_cho_invert_and_alpha_arad_golden():
+    """scipy's dpotrf + dpotri (+ dsymv) equal the reference's fcho_invert
+    and fget_alpha_arad bit for bit (same LAPACK), the latter on the C
+    oracle's molecular kernel."""
+    d = g("inverse.npz")
+    inv, info = oracle.cho_invert(d["A"])
+    assert info == 0
+    assert np.array_equal(np.triu(inv), np.triu(d["inv"]))
+    assert np.array_equal(np.tril(d["inv"], -1), np.tril(d["A"], -1))  # dpotri('U') leaves the rest
+    a = g("arad.npz")
+    alpha, info = oracle.alpha_arad(a["X"], a["z"], a["n"], d["y_arad"], 10.0, 1e-3)
+    assert info == 0
+    assert np.array_equal(alpha, d["alpha_arad"])
+
+
+def test_cho_invert_and_alpha_arad_closed_forms():
+    # inverse of L L^T with a small-integer unit-diagonal L is an integer matrix
+    rng = np.random.default_rng(5)
+    for n in (1, 4, 9):
+        L = np.tril(rng.integers(-2, 3, size=(n, n)), -1).astype(float) + np.eye(n)
+        A = L @ L.T
+        inv, info = oracle.cho_invert(A)
+        Li = np.linalg.inv(L)
+        assert info == 0 and np.allclose(np.triu(inv), np.triu(np.round(Li.T @ Li)), atol=1e-9)
+    # one molecule: alpha = y / (K + lambda) for each of its atoms
+    ms = synth.generate(1, seed=6, nmin=7, nmax=7)
+    X = ARAD(maxMolSize=7, maxAts=7).describe(ms.molecule(0)[1], ms.molecule(0)[0])[None]
+    z, n = arad_types([ms.molecule(0)[0]], 7)
+    k = oracle.kernel_arad(X, z, n, 3.0)[0, 0]
+    alpha, info = oracle.alpha_arad(X, z, n, [2.5], 3.0, 0.25)
+    assert info == 0 and alpha.shape == (7,)
+    assert np.allclose(alpha, 2.5 / (k + 0.25), rtol=1e-14)
+
+
 def test_oracle_ptp_and_charges():
     from oracle.pyref import NUCLEAR_CHARGE_ORACLE, ptp_types
     from fml_amd.data import NUCLEAR_CHARGE
