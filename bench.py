@@ -46,11 +46,11 @@ PEAK_FP32_VALU_TOPS = 78.6         # FP32 VALU lane-operations/s: the FP32 vecto
 # term: 13 arithmetic operations plus the exponential counted as 20.
 ARAD_OPS_PER_EVAL = 33
 
-# HBM bytes per launch of k_arad_tiles (FP64), from rocprofv3 --pmc FETCH_SIZE
-# and WRITE_SIZE runs of this script, each scaled by the counter's reading
-# for a device copy of known size (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0):
-# profiles/hbm_traffic_counters.md.
-ARAD_TRAFFIC_BYTES = {"arad500": 2.533e9, "arad4096": 1.898e11}
+# HBM bytes per launch of k_arad_tiles (FP64, the tuned tile shape), from
+# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this script, each scaled by
+# the counter's reading for a device copy of known size (FETCH_SIZE x 2.0,
+# WRITE_SIZE x 1.0): profiles/hbm_traffic_counters.md.
+ARAD_TRAFFIC_BYTES = {"arad500": 3.762e9}
 
 
 def parse():

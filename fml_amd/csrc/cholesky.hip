@@ -13,7 +13,7 @@
 //   for each 256-column outer panel:
 //     for each 64-column inner block inside it:
 //       k_potf2   diagonal 64x64 block in registers (one wave, lane = row)
-//       k_trsm    rows below: M[r][k] <- M[r][k] L_kk^{-T} (one lane/row)
+//       k_trsm    rows below: M[r][k] <- M[r][k] L_kk^{-T} (one thread/row)
 //       k_update  panel columns right of the block: C -= P Q^T, 64 tiles
 //     k_update    trailing matrix right of the outer panel, K = 256, on
 //                 128x128 MFMA tiles over its lower triangle only
@@ -86,11 +86,8 @@ __global__ __launch_bounds__(64) void k_potf2(T *__restrict__ a, int64_t lda, in
   }
 }
 
-// Rows r in [r0, n): x = M[r][k0:k0+b] solves x L^T = m_r against the
-// factored diagonal block (LDS, loaded by all 256 threads), one row per
-// thread held in registers; column by column, x_c = m_c / L[c][c] then
-// m_t -= x_c L[t][c] for t > c (independent updates, so the row's
-// dependency chain is short).
+// Rows r in [r0, n): x = M[r][k0:k0+b] solves x L^T = m_r (forward
+// substitution against the factored diagonal block, held in LDS).
 template <typename T>
 __global__ __launch_bounds__(256) void k_trsm(T *__restrict__ a, int64_t lda, int n, int k0, int b,
                                               int r0, const int32_t *__restrict__ info) {
@@ -106,13 +103,14 @@ __global__ __launch_bounds__(256) void k_trsm(T *__restrict__ a, int64_t lda, in
   T *row = a + (int64_t)r * lda + k0;
   T x[kNB];
 #pragma unroll
-  for (int c = 0; c < kNB; ++c) x[c] = (c < b) ? row[c] : T(0);
-#pragma unroll
   for (int c = 0; c < kNB; ++c) {
     if (c < b) {
-      x[c] = x[c] / L[c][c];
+      T v = row[c];
 #pragma unroll
-      for (int t = c + 1; t < kNB; ++t) x[t] -= x[c] * L[t][c];
+      for (int t = 0; t < c; ++t) v -= x[t] * L[c][t];
+      x[c] = v / L[c][c];
+    } else {
+      x[c] = T(0);
     }
   }
 #pragma unroll
