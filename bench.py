@@ -121,9 +121,11 @@ def arad_workload(args, big=False, precision=0):
         nm, amax, sig = 500, 23, [0.1, 1.0, 10.0]
         ms1 = ms2 = synth.generate(nm, seed=0, nmin=5, nmax=23)
     ar = ARAD(maxMolSize=amax, maxAts=amax, cut=5.0)
+    t0 = time.perf_counter()
     X1 = np.array([ar.describe(ms1.molecule(m)[1], ms1.molecule(m)[0]) for m in range(nm)])
     X2 = X1 if ms2 is ms1 else np.array([ar.describe(ms2.molecule(m)[1], ms2.molecule(m)[0])
                                          for m in range(nm)])
+    host_describe_s = time.perf_counter() - t0
     z1, n1 = arad_types([ms1.molecule(m)[0] for m in range(nm)], amax)
     z2, n2 = (z1, n1) if ms2 is ms1 else arad_types([ms2.molecule(m)[0] for m in range(nm)], amax)
     evals = arad_inner_evaluations(X1, n1, X2, n2, cut, width)
@@ -209,6 +211,36 @@ def arad_workload(args, big=False, precision=0):
         res["host_buffers"] = {"ms_per_call": round(dt * 1e3, 3),
                                "entries_per_s": round(units_per_step / dt, 1),
                                "note": "numpy in, numpy out: descriptor upload and kernel download included"}
+        # the descriptors themselves on the device (fmlh_arad_describe), both
+        # sets, against the host numpy ARAD.describe loop that made X1/X2
+        from fml_amd.batch import AradDescribeCall
+        sets = [(ms1, X1)] if ms2 is ms1 else [(ms1, X1), (ms2, X2)]
+        calls = [AradDescribeCall(m.charges, m.coords, m.offsets, amax=amax, cut=5.0) for m, _ in sets]
+        for c in calls:
+            c.run()
+        torch.cuda.synchronize()
+        diff = 0.0
+        for c, (_, X) in zip(calls, sets):
+            q = c.q.cpu().numpy()
+            if not np.array_equal(q[:, :, 0:3], X[:, :, 0:3]):
+                raise RuntimeError("device ARAD descriptors differ from the host's in channels 0-2")
+            a, b = q[:, :, 3:5], X[:, :, 3:5]
+            fin = ~np.isnan(b)
+            diff = max(diff, float(np.max(np.abs(a[fin] - b[fin]), initial=0.0)))
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(5):
+            for c in calls:
+                c.run()
+        e1.record()
+        torch.cuda.synchronize()
+        res["descriptors_on_device"] = {
+            "ms_per_call": round(e0.elapsed_time(e1) / 5, 3),
+            "molecules": nm * len(sets),
+            "host_numpy_s": round(host_describe_s, 3),
+            "max_abs_diff_channels_3_4": diff,
+            "note": "fmlh_arad_describe of every set, one wavefront per molecule; channels 0-2 "
+                    "bit-identical to the host ARAD.describe"}
 
     return info, step, units_per_step, roofline, cpu_baseline, finish
 

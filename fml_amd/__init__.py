@@ -12,9 +12,9 @@ Compute runs in hand-written HIP kernels for gfx950 (libfmlhip.so, C ABI in
 include/fmlh.h) called through ctypes; there is no CPU fallback.
 """
 from .data import NUCLEAR_CHARGE
-from .arad import ARAD
+from .arad import ARAD, generate_arad_descriptors
 from .fml import Molecule
 from . import kernels, math, representations
 
 __version__ = "0.1.0"
-__all__ = ["Molecule", "ARAD", "NUCLEAR_CHARGE", "kernels", "math", "representations"]
+__all__ = ["Molecule", "ARAD", "generate_arad_descriptors", "NUCLEAR_CHARGE", "kernels", "math", "representations"]

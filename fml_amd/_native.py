@@ -45,6 +45,8 @@ SIGNATURES = {
                                       c_void_p, c_int64, c_void_p]),
     "fmlh_laplacian_kernel_f32": (c_int, [c_void_p, c_int64, c_int, c_void_p, c_int64, c_int, c_int,
                                           c_double, c_void_p, c_void_p, c_int64, c_void_p]),
+    "fmlh_arad_describe": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_double, c_void_p, c_int,
+                                   c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "fmlh_arad_ws_bytes": (c_size_t, [c_int, c_int, c_int, c_int]),
     "fmlh_arad_kernels": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
                                   c_int, c_int, c_dbl_p, c_int, c_double, c_double, c_double, c_double,

@@ -8,6 +8,10 @@ values of the same shape are copied in with ``set_*`` between runs.
 
 * ``CoulombCall``   sorted Coulomb matrices of a molecule batch
                     (frepresentations.f90:23-110), one workgroup per molecule.
+* ``AradDescribeCall`` ARAD descriptors of a molecule batch
+                    (arad.py:70-145), one wavefront per molecule; its
+                    outputs are a ready AradKernelCall input, so descriptor
+                    and kernel can share one graph.
 * ``AradKernelCall`` a molecular ARAD kernel (fget_kernels_arad /
                     fget_symmetric_kernels_arad / fget_kernel_arad,
                     farad_kernels.f90:95-755) over size-sorted molecule-pair
@@ -21,7 +25,8 @@ import ctypes
 import numpy as np
 
 from . import _native, tuning
-from .device import empty, to_device, workspace
+from .arad import MAX_DEVICE_ATOMS, describe_into
+from .device import check_offsets, empty, to_device, workspace
 
 _MODES = {"global": _native.ARAD_GLOBAL, "symmetric": _native.ARAD_SYMMETRIC,
           "kernel": _native.ARAD_KERNEL}
@@ -36,6 +41,7 @@ class CoulombCall:
     def __init__(self, charges, coords, offsets, nmax=23, float32=False):
         self.z = to_device(charges, np.float64)
         self.r = to_device(np.asarray(coords).reshape(-1, 3), np.float64)
+        check_offsets(offsets, int(self.z.shape[0]))
         self.off = to_device(offsets, np.int32)
         self.nmol = int(self.off.shape[0]) - 1
         self.nmax = int(nmax)
@@ -56,6 +62,45 @@ class CoulombCall:
         _native.call("fmlh_coulomb_sorted", _native.ptr(self.z), _native.ptr(self.r),
                      _native.ptr(self.off), self.nmol, self.nmax, _native.ptr(self.cm),
                      _native.ptr(self.cm32), None, None, _native.stream_ptr())
+
+
+class AradDescribeCall:
+    """ARAD descriptors of a fixed batch of molecules.
+
+    charges (total,), coords (total, 3), offsets (nmol+1,).  Resident
+    outputs ``q`` (nmol, amax, 5, amax), ``zt`` (nmol, amax, 2) and ``n``
+    (nmol,) int32 in the layouts ``AradKernelCall`` takes (pass them
+    straight in: no copy is made), and ``status`` (nmol,) int32, 1 for a
+    molecule with a charge without a PTP entry."""
+
+    def __init__(self, charges, coords, offsets, amax=23, cut=5.0):
+        self.z = to_device(charges, np.float64)
+        self.r = to_device(np.asarray(coords).reshape(-1, 3), np.float64)
+        check_offsets(offsets, int(self.z.shape[0]))
+        self.off = to_device(offsets, np.int32)
+        self.nmol = int(self.off.shape[0]) - 1
+        self.amax = int(amax)
+        self.cut = float(cut)
+        if not 1 <= self.amax <= MAX_DEVICE_ATOMS:
+            raise ValueError(f"amax must be in 1..{MAX_DEVICE_ATOMS}, got {amax}")
+        counts = np.diff(np.asarray(offsets, dtype=np.int64))
+        if self.nmol and counts.max() > self.amax:
+            raise ValueError(f"molecule {int(np.argmax(counts > self.amax))} has more than "
+                             f"amax={self.amax} atoms")
+        self.q = empty((self.nmol, self.amax, 5, self.amax), np.float64)
+        self.zt = empty((self.nmol, self.amax, 2), np.float64)
+        self.n = empty((self.nmol,), np.int32)
+        self.status = empty((max(self.nmol, 1),), np.int32)
+        _native.call("fmlh_init")
+
+    def set_coords(self, coords):
+        """New coordinates for the same molecules (same atom order)."""
+        self.r.copy_(to_device(np.asarray(coords).reshape(-1, 3), np.float64))
+
+    def run(self):
+        if self.nmol:
+            describe_into(self.z, self.r, self.off, self.nmol, self.amax, self.cut, self.q,
+                          self.zt, self.n, self.status)
 
 
 class AradKernelCall:

@@ -33,6 +33,19 @@ def to_device(x, dtype=np.float64):
     return t.from_numpy(a).to("cuda", non_blocking=False)
 
 
+def check_offsets(offsets, natoms: int) -> None:
+    """Molecule m owns atoms offsets[m]:offsets[m+1]: the offsets must start
+    at 0 or more, never decrease and end within the natoms atoms.  Device
+    tensors are taken as given (checking them would wait on the GPU)."""
+    if is_device_tensor(offsets):
+        return
+    off = np.asarray(offsets).reshape(-1)
+    if off.size == 0:
+        raise ValueError("offsets needs nmol + 1 entries")
+    if off[0] < 0 or np.any(np.diff(off) < 0) or off[-1] > natoms:
+        raise ValueError(f"offsets must be non-decreasing within 0..{natoms}")
+
+
 def empty(shape, dtype=np.float64):
     t = torch()
     tdt = {np.float64: t.float64, np.float32: t.float32, np.int32: t.int32, np.uint8: t.uint8}[dtype]

@@ -9,7 +9,7 @@ from __future__ import annotations
 import numpy as np
 
 from .. import _native
-from ..device import empty, is_device_tensor, to_device, to_host
+from ..device import check_offsets, empty, is_device_tensor, to_device, to_host
 
 
 def generate_coulomb_matrices(charges, coords, offsets, nmax: int = 23, float32: bool = False,
@@ -26,6 +26,9 @@ def generate_coulomb_matrices(charges, coords, offsets, nmax: int = 23, float32:
     z = to_device(charges, np.float64)
     r = to_device(np.asarray(coords).reshape(-1, 3) if not is_device_tensor(coords) else coords,
                   np.float64)
+    if int(r.shape[0]) != int(z.shape[0]):
+        raise ValueError(f"{int(r.shape[0])} coordinates for {int(z.shape[0])} charges")
+    check_offsets(offsets, int(z.shape[0]))
     off = to_device(offsets, np.int32)
     nmol = int(off.shape[0]) - 1
     packed = nmax * (nmax + 1) // 2

@@ -157,6 +157,24 @@ int fmlh_laplacian_kernel_f32(const float *a, int64_t lda, int na,
 #define FMLH_ARAD_KERNEL 2
 #define FMLH_ARAD_DISTANCE 3
 
+/* ARAD descriptors on the device: ARAD.describe (fml/arad.py:70-145, the
+ * non-periodic form; Python entry Molecule.generate_arad_descriptor,
+ * fml/fml.py:80-86), one wavefront per molecule.
+ * z, r, offsets: molecules as for fmlh_coulomb_sorted (at most 64 atoms
+ * each); ptp: (ptp_rows, 2) f64 PTP (row, column) per nuclear charge, a row
+ * with a non-positive period marks a charge without an entry.
+ * Outputs, in the layouts the kernel modes read: q (nmol, amax, 5, amax),
+ * zt (nmol, amax, 2), n (nmol) atom counts; status (may be NULL) (nmol):
+ * 1 where a molecule has more than amax atoms or a charge that is not an
+ * integer with a PTP entry (its block is all padding and n = 0).
+ * Distances, types and the neighbour order equal the host computation's
+ * bit for bit; the angular channels differ only by the device sin/cos/acos
+ * rounding. */
+int fmlh_arad_describe(const double *z, const double *r, const int32_t *offsets,
+                       int nmol, int amax, double cut, const double *ptp,
+                       int ptp_rows, double *q, double *zt, int32_t *n,
+                       int32_t *status, void *stream);
+
 /* precision: 0 = FP64 throughout (the plain form); 1 = FP32 inner overlap
  * (exp, products and accumulation in float), FP64 self terms, distance
  * combination and kernel sums.

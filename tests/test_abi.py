@@ -65,6 +65,14 @@ def test_version_and_errors_without_gpu():
     assert got.as_tuple() == (256, 0, 16)  # untuned key: the built-in default
     assert lib.fmlh_graph_launch(None, None) == -1
     assert lib.fmlh_graph_destroy(None) == 0
+    describe = lib.fmlh_arad_describe
+    assert describe(None, None, None, -1, 23, 5.0, None, 119, None, None, None, None, None) == -4
+    assert describe(None, None, None, 1, 0, 5.0, None, 119, None, None, None, None, None) == -5
+    assert describe(None, None, None, 1, 65, 5.0, None, 119, None, None, None, None, None) == -5
+    assert describe(None, None, None, 1, 23, 0.0, None, 119, None, None, None, None, None) == -6
+    assert describe(None, None, None, 1, 23, 5.0, None, 0, None, None, None, None, None) == -8
+    assert describe(None, None, None, 1, 23, 5.0, None, 119, None, None, None, None, None) == -1
+    assert describe(None, None, None, 0, 23, 5.0, None, 119, None, None, None, None, None) == 0
     # empty problems are no-ops
     assert lib.fmlh_gaussian_kernel(None, 3, 0, None, 3, 0, 3, 1.0, None, 1, None) == 0
     assert lib.fmlh_arad_ws_bytes(10, 10, 23, 0) > 0

@@ -66,3 +66,32 @@ def test_arad_shape_asserts():
     X = np.zeros((2, 5, 5, 4))
     with pytest.raises(AssertionError):
         get_atomic_kernels_arad(X, X, [[1], [1]], [[1], [1]], [1.0])
+
+
+def test_ptp_table_matches_dictionary():
+    from fml_amd.arad import ptp_table
+    t = ptp_table()
+    assert t.shape == (max(PTP) + 1, 2)
+    for z in range(t.shape[0]):
+        if z in PTP:
+            assert list(t[z]) == PTP[z] and t[z, 0] > 0
+        else:
+            assert list(t[z]) == [0.0, 0.0]
+
+
+def test_check_offsets():
+    from fml_amd.device import check_offsets
+    check_offsets(np.array([0, 3, 3, 7]), 7)
+    check_offsets(np.array([0]), 0)
+    for bad, n in (([0, 4, 2], 5), ([-1, 2], 5), ([0, 6], 5), ([], 0)):
+        with pytest.raises(ValueError):
+            check_offsets(np.array(bad, dtype=np.int64), n)
+
+
+def test_describe_argument_checks():
+    """Rejected on the host before any device work."""
+    from fml_amd.arad import generate_arad_descriptors
+    z, r, off = np.array([6.0, 1.0]), np.zeros((2, 3)), np.array([0, 2])
+    for kw in ({"amax": 0}, {"amax": 65}, {"cut": 0.0}, {"cut": -1.0}):
+        with pytest.raises(ValueError):
+            generate_arad_descriptors(z, r, off, **kw)
