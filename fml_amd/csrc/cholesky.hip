@@ -36,7 +36,7 @@ namespace {
 using namespace fmlh;
 
 constexpr int kNB = 64;     // inner block
-constexpr int kNBO = 512;   // outer panel
+constexpr int kNBO = 1024;  // outer panel
 
 // Value of v in lane l (l uniform) for every lane.
 __device__ __forceinline__ double lane_value(double v, int l) {

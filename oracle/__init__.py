@@ -79,6 +79,19 @@ def lib():
         _lib.oracle_arad_l2_distance_all.argtypes = [_c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_int_p,
                                                      _c_int_p, I, I, I, D, D, D, D, _c_dbl_p]
         _lib.oracle_cho_solve.argtypes = [_c_dbl_p, I, _c_dbl_p, _c_dbl_p]
+        _lib.oracle_coulomb_form.argtypes = [I, _c_dbl_p, _c_dbl_p, I, I, _c_dbl_p]
+        _lib.oracle_coulomb_forms.argtypes = [I, _c_dbl_p, _c_dbl_p, _c_int_p, I, I, _c_dbl_p]
+        _lib.oracle_vector_kernels.argtypes = [I, _c_dbl_p, _c_dbl_p, _c_int_p, _c_int_p, _c_dbl_p,
+                                               I, I, I, I, I, I, _c_dbl_p]
+        _lib.oracle_alpha_kernel.argtypes = [_c_dbl_p, _c_int_p, I, I, D, _c_dbl_p]
+        _lib.oracle_prediction.argtypes = [_c_dbl_p, I, _c_dbl_p, I, _c_int_p, I, I, _c_dbl_p, D,
+                                           _c_dbl_p]
+        _lib.oracle_alpha_kernel_from_distance.argtypes = [_c_dbl_p, _c_int_p, I, D, _c_dbl_p]
+        _lib.oracle_prediction_from_distance.argtypes = [_c_dbl_p, I, I, _c_int_p, I, _c_dbl_p, D,
+                                                         _c_dbl_p]
+        for f in ("oracle_vector_kernels", "oracle_alpha_kernel", "oracle_prediction",
+                  "oracle_alpha_kernel_from_distance", "oracle_prediction_from_distance"):
+            getattr(_lib, f).restype = None
         for f in ("oracle_kernels_arad", "oracle_symmetric_kernels_arad", "oracle_kernel_arad",
                   "oracle_atomic_kernels_arad", "oracle_atomic_distance_arad",
                   "oracle_arad_l2_distance_all"):
@@ -115,6 +128,14 @@ def ref(variant: str = "libfml_ref"):
         r.ref_cho_solve.argtypes = [_c_dbl_p, I, _c_dbl_p, _c_dbl_p]
         r.ref_cho_invert.argtypes = [_c_dbl_p, I]
         r.ref_alpha_arad.argtypes = [_c_dbl_p, _c_dbl_p, _c_int_p, _c_dbl_p, D, D, I, I, I, _c_dbl_p]
+        r.ref_coulomb_form.argtypes = [I, _c_dbl_p, _c_dbl_p, I, I, _c_dbl_p]
+        r.ref_vector_kernels.argtypes = [I, _c_dbl_p, _c_dbl_p, _c_int_p, _c_int_p, _c_dbl_p,
+                                         I, I, I, I, I, I, _c_dbl_p]
+        r.ref_get_alpha.argtypes = [_c_dbl_p, _c_int_p, _c_dbl_p, D, D, I, I, I, _c_dbl_p]
+        r.ref_get_prediction.argtypes = [_c_dbl_p, _c_dbl_p, _c_int_p, _c_dbl_p, D, I, I, I, I,
+                                         _c_dbl_p]
+        r.ref_alpha_from_distance.argtypes = [_c_dbl_p, _c_int_p, _c_dbl_p, D, D, I, I, _c_dbl_p]
+        r.ref_prediction_from_distance.argtypes = [_c_dbl_p, _c_int_p, _c_dbl_p, D, I, I, I, _c_dbl_p]
         _ref[variant] = r
     return _ref[variant]
 
@@ -416,6 +437,177 @@ def cho_solve_c(A, y):
     x = np.zeros_like(y)
     info = lib().oracle_cho_solve(Af.ctypes.data_as(_c_dbl_p), Af.shape[0], _dp(y), _dp(x))
     return x, info
+
+
+# ---------------------------------------------------------------------------
+# Atomic KRR path: per-atom Coulomb matrices, vector kernels, get_alpha /
+# get_prediction (frepresentations.f90:112-390, fkernels.f90:23-162, 380-702)
+# ---------------------------------------------------------------------------
+
+UNSORTED, LOCAL, ATOMIC = 0, 1, 2
+
+
+def coulomb_form(mode, Z, R, nmax):
+    """One molecule: mode 0 (unsorted) -> (packed,), 1/2 (local/atomic) ->
+    (natoms, packed) C order."""
+    Z = np.ascontiguousarray(Z, dtype=np.float64)
+    R = np.ascontiguousarray(R, dtype=np.float64).reshape(-1, 3)
+    n = Z.shape[0]
+    packed = nmax * (nmax + 1) // 2
+    cm = np.zeros(packed) if mode == UNSORTED else np.zeros((n, packed))
+    if lib().oracle_coulomb_form(mode, _dp(Z), _dp(R), n, nmax, _dp(cm)) != 0:
+        raise ValueError("natoms > nmax")
+    return cm
+
+
+def coulomb_forms(mode, Zcat, Rcat, offsets, nmax):
+    """Concatenated molecules: (nmol, packed) for mode 0, (atoms, packed)
+    for modes 1/2."""
+    Zcat = np.ascontiguousarray(Zcat, dtype=np.float64)
+    Rcat = np.ascontiguousarray(Rcat, dtype=np.float64).reshape(-1, 3)
+    offsets = np.ascontiguousarray(offsets, dtype=np.int32)
+    nmol = offsets.shape[0] - 1
+    rows = nmol if mode == UNSORTED else int(offsets[-1])
+    cm = np.zeros((rows, nmax * (nmax + 1) // 2))
+    if lib().oracle_coulomb_forms(mode, _dp(Zcat), _dp(Rcat), _ip(offsets), nmol, nmax, _dp(cm)) != 0:
+        raise ValueError("natoms > nmax")
+    return cm
+
+
+def ref_coulomb_form(mode, Z, R, nmax, variant="libfml_ref"):
+    """The reference's fgenerate_{unsorted,local,atomic}_coulomb_matrix."""
+    Z = np.ascontiguousarray(Z, dtype=np.float64)
+    R = np.asfortranarray(np.asarray(R, dtype=np.float64).reshape(-1, 3))
+    n = Z.shape[0]
+    packed = nmax * (nmax + 1) // 2
+    cm = np.zeros(packed) if mode == UNSORTED else np.zeros((n, packed), order="F")
+    ref(variant).ref_coulomb_form(mode, _dp(Z), _dp(R), n, nmax, cm.ctypes.data_as(_c_dbl_p))
+    return np.ascontiguousarray(cm)
+
+
+def _atoms3(q):
+    """(nm, amax, d) float64 C order (atom rows of each molecule)."""
+    return np.ascontiguousarray(q, dtype=np.float64)
+
+
+def vector_kernels(kind, q1, q2, n1, n2, sigmas):
+    """fget_vector_kernels_{gaussian (kind 0), laplacian (kind 1)} on
+    q1 (nm1, amax1, d), q2 (nm2, amax2, d) -> (nsig, nm1, nm2)."""
+    q1, q2 = _atoms3(q1), _atoms3(q2)
+    n1, n2 = _c(n1, np.int32), _c(n2, np.int32)
+    sig = _c(sigmas)
+    nm1, amax1, d = q1.shape
+    nm2, amax2, _ = q2.shape
+    out = np.zeros((len(sig), nm1, nm2), order="F")
+    lib().oracle_vector_kernels(kind, _dp(q1), _dp(q2), _ip(n1), _ip(n2), _dp(sig), nm1, nm2,
+                                len(sig), amax1, amax2, d, out.ctypes.data_as(_c_dbl_p))
+    return out
+
+
+def ref_vector_kernels(kind, q1, q2, n1, n2, sigmas, variant="libfml_ref"):
+    q1, q2 = _atoms3(q1), _atoms3(q2)
+    n1, n2 = _c(n1, np.int32), _c(n2, np.int32)
+    sig = _c(sigmas)
+    nm1, amax1, d = q1.shape
+    nm2, amax2, _ = q2.shape
+    out = np.zeros((len(sig), nm1, nm2), order="F")
+    ref(variant).ref_vector_kernels(kind, _dp(q1), _dp(q2), _ip(n1), _ip(n2), _dp(sig), nm1, nm2,
+                                    len(sig), amax1, amax2, d, out.ctypes.data_as(_c_dbl_p))
+    return out
+
+
+def _solve_expand(K, n, y, lam):
+    """K + lam I, dpotrf + dpotri + dsymv on the upper triangle, each
+    molecule's value repeated for its atoms (fkernels.f90:437-476)."""
+    from scipy.linalg import blas, lapack
+    K = np.array(K, dtype=np.float64, order="F", copy=True)
+    K[np.diag_indices_from(K)] += lam
+    c, info = lapack.dpotrf(K, lower=0, clean=0)
+    inv, info2 = lapack.dpotri(c, lower=0)
+    ly = blas.dsymv(1.0, inv, np.asarray(y, dtype=np.float64), lower=0)
+    return np.repeat(ly, np.asarray(n, dtype=np.int64)), (info or info2)
+
+
+def get_alpha(Q, n, y, sigma, lam):
+    """fget_alpha (fkernels.f90:380-478): Q (d, na) atom columns, n (nm,).
+    Returns (alpha (na,), info)."""
+    q = np.ascontiguousarray(np.asarray(Q, dtype=np.float64).T)
+    n = _c(n, np.int32)
+    nm, d = n.shape[0], q.shape[1]
+    K = np.zeros((nm, nm), order="F")
+    lib().oracle_alpha_kernel(_dp(q), _ip(n), nm, d, float(sigma), K.ctypes.data_as(_c_dbl_p))
+    return _solve_expand(K, n, y, lam)
+
+
+def ref_get_alpha(Q, n, y, sigma, lam, variant="libfml_ref"):
+    q = np.ascontiguousarray(np.asarray(Q, dtype=np.float64).T)
+    n = _c(n, np.int32)
+    na, d = q.shape
+    alpha = np.zeros(na)
+    ref(variant).ref_get_alpha(_dp(q), _ip(n), _dp(_c(y)), float(sigma), float(lam), na, n.shape[0],
+                               d, _dp(alpha))
+    return alpha
+
+
+def get_prediction(Q, Q2, n2, alpha, sigma):
+    """fget_prediction (fkernels.f90:480-539): Q (d, na), Q2 (d, na2)."""
+    q = np.ascontiguousarray(np.asarray(Q, dtype=np.float64).T)
+    q2 = np.ascontiguousarray(np.asarray(Q2, dtype=np.float64).T)
+    n2 = _c(n2, np.int32)
+    y = np.zeros(n2.shape[0])
+    lib().oracle_prediction(_dp(q), q.shape[0], _dp(q2), q2.shape[0], _ip(n2), n2.shape[0], q.shape[1],
+                            _dp(_c(alpha)), float(sigma), _dp(y))
+    return y
+
+
+def ref_get_prediction(Q, Q2, n2, alpha, sigma, variant="libfml_ref"):
+    q = np.ascontiguousarray(np.asarray(Q, dtype=np.float64).T)
+    q2 = np.ascontiguousarray(np.asarray(Q2, dtype=np.float64).T)
+    n2 = _c(n2, np.int32)
+    y = np.zeros(n2.shape[0])
+    ref(variant).ref_get_prediction(_dp(q), _dp(q2), _ip(n2), _dp(_c(alpha)), float(sigma), q.shape[0],
+                                    q2.shape[0], n2.shape[0], q.shape[1], _dp(y))
+    return y
+
+
+def get_alpha_from_distance(D, n, y, gamma, lam):
+    """fget_alpha_from_distance (fkernels.f90:541-641): D (na, na)."""
+    Df = np.asfortranarray(D, dtype=np.float64)
+    n = _c(n, np.int32)
+    nm = n.shape[0]
+    K = np.zeros((nm, nm), order="F")
+    lib().oracle_alpha_kernel_from_distance(Df.ctypes.data_as(_c_dbl_p), _ip(n), nm, float(gamma),
+                                            K.ctypes.data_as(_c_dbl_p))
+    return _solve_expand(K, n, y, lam)
+
+
+def ref_get_alpha_from_distance(D, n, y, gamma, lam, variant="libfml_ref"):
+    Df = np.asfortranarray(D, dtype=np.float64)
+    n = _c(n, np.int32)
+    alpha = np.zeros(Df.shape[0])
+    ref(variant).ref_alpha_from_distance(Df.ctypes.data_as(_c_dbl_p), _ip(n), _dp(_c(y)), float(gamma),
+                                         float(lam), Df.shape[0], n.shape[0], _dp(alpha))
+    return alpha
+
+
+def get_prediction_from_distance(D2, n2, alpha, gamma):
+    """fget_prediction_from_distance (fkernels.f90:645-702): D2 (na, na2)."""
+    Df = np.asfortranarray(D2, dtype=np.float64)
+    n2 = _c(n2, np.int32)
+    y = np.zeros(n2.shape[0])
+    lib().oracle_prediction_from_distance(Df.ctypes.data_as(_c_dbl_p), Df.shape[0], Df.shape[1], _ip(n2),
+                                          n2.shape[0], _dp(_c(alpha)), float(gamma), _dp(y))
+    return y
+
+
+def ref_get_prediction_from_distance(D2, n2, alpha, gamma, variant="libfml_ref"):
+    Df = np.asfortranarray(D2, dtype=np.float64)
+    n2 = _c(n2, np.int32)
+    y = np.zeros(n2.shape[0])
+    ref(variant).ref_prediction_from_distance(Df.ctypes.data_as(_c_dbl_p), _ip(n2), _dp(_c(alpha)),
+                                              float(gamma), Df.shape[0], Df.shape[1], n2.shape[0],
+                                              _dp(y))
+    return y
 
 
 from .pyref import (arad_describe, arad_distl2_py, ptp_types, NUCLEAR_CHARGE_ORACLE)  # noqa: E402,F401

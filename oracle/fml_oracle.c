@@ -607,3 +607,354 @@ EXPORT int oracle_cho_solve(const double *a_in, int n, const double *y,
   free(u);
   return 0;
 }
+
+/* ------------------------------------------------------------------ */
+/* Atomic KRR path: per-atom Coulomb matrices, vector kernels,          */
+/* get_alpha / get_prediction                                          */
+/* ------------------------------------------------------------------ */
+
+/* Pair matrix and row norms as frepresentations.f90:152-172 (identical in
+ * :238-258): diag 0.5*Z**2.4, pair Zi*Zj/sqrt(sum((Rj-Ri)**2)) for i<j;
+ * norms in the single-thread order of the two REDUCTION loops. */
+static void cm_pairs(const double *z, const double *r, int n, double *pdm,
+                     double *norms) {
+  double *priv = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+  for (int i = 0; i < n; ++i) {
+    double p = 0.5 * pow(z[i], 2.4);
+    norms[i] = 0.0 + p;
+    pdm[i * n + i] = p;
+    priv[i] = 0.0;
+  }
+  for (int i = 0; i < n; ++i) {
+    for (int j = i + 1; j < n; ++j) {
+      double dx = r[j * 3 + 0] - r[i * 3 + 0];
+      double dy = r[j * 3 + 1] - r[i * 3 + 1];
+      double dz = r[j * 3 + 2] - r[i * 3 + 2];
+      double s = dx * dx;
+      s = s + dy * dy;
+      s = s + dz * dz;
+      double p = z[i] * z[j] / sqrt(s);
+      priv[j] = priv[j] + p;
+      priv[i] = priv[i] + p;
+      pdm[i * n + j] = p;
+      pdm[j * n + i] = p;
+    }
+  }
+  for (int i = 0; i < n; ++i) norms[i] = norms[i] + priv[i];
+  free(priv);
+}
+
+/* Writes row `row` of a packed per-atom matrix: entries (m, q), q <= m,
+ * of the pair matrix taken in the order ord[] (frepresentations.f90:280-291,
+ * :376-387), zero padded. */
+static void cm_pack_row(const double *pdm, int n, const int *ord, int packed,
+                        double *out) {
+  for (int t = 0; t < packed; ++t) out[t] = 0.0;
+  for (int m = 0; m < n; ++m) {
+    int i = ord[m];
+    int idx = m * (m + 1) / 2;
+    for (int q = 0; q <= m; ++q) out[idx + q] = pdm[i * n + ord[q]];
+  }
+}
+
+/*
+ * Per-molecule Coulomb matrix forms; each returns 0, or -1 when
+ * natoms > nmax (the reference writes out of bounds there) or natoms < 0.
+ *
+ * mode 0: fgenerate_unsorted_coulomb_matrix (frepresentations.f90:112-201):
+ *   the packed lower triangle in input atom order, cm[packed].
+ * mode 1: fgenerate_local_coulomb_matrix (:204-294): atoms sorted once by
+ *   descending row norm (repeated minloc placed last, :262-266: equal norms
+ *   with the higher index first); row k puts atom k first and then the
+ *   sorted list without k (:268-277); cm[natoms][packed].
+ * mode 2: fgenerate_atomic_coulomb_matrix (:297-390): row k orders the atoms
+ *   by ascending distance from atom k (repeated minloc over distance
+ *   column k, :366-372: equal distances lowest index first); distances
+ *   sqrt(sum((Rj-Ri)**2)) for i<j, pair values Zi*Zj/distance (:339-357);
+ *   cm[natoms][packed].
+ * Parity: pinned bitwise by the flang build of the reference (oracle/_ref)
+ * and by hand-computed molecules (tests/test_oracle.py).
+ */
+EXPORT int oracle_coulomb_form(int mode, const double *z, const double *r,
+                               int natoms, int nmax, double *cm) {
+  if (natoms < 0 || natoms > nmax) return -1;
+  const int n = natoms;
+  const int packed = nmax * (nmax + 1) / 2;
+  const size_t nn = (size_t)(n > 0 ? n : 1);
+  double *pdm = (double *)malloc(sizeof(double) * nn * nn);
+  double *norms = (double *)malloc(sizeof(double) * nn);
+  int *sorted = (int *)malloc(sizeof(int) * nn);
+  int *ord = (int *)malloc(sizeof(int) * nn);
+  cm_pairs(z, r, n, pdm, norms);
+  if (mode == 0) {
+    for (int i = 0; i < n; ++i) ord[i] = i;
+    cm_pack_row(pdm, n, ord, packed, cm);
+  } else if (mode == 1) {
+    for (int i = 0; i < n; ++i) {
+      int jmin = 0;
+      for (int j = 1; j < n; ++j)
+        if (norms[j] < norms[jmin]) jmin = j;
+      sorted[n - 1 - i] = jmin;
+      norms[jmin] = 1.7976931348623157e308;
+    }
+    for (int k = 0; k < n; ++k) {
+      ord[0] = k;
+      int m = 0;
+      for (int i = 0; i < n - 1; ++i) {
+        if (sorted[i] == k) m = 1;
+        ord[i + 1] = sorted[i + m];
+      }
+      cm_pack_row(pdm, n, ord, packed, cm + (size_t)k * packed);
+    }
+  } else {
+    double *dist = (double *)malloc(sizeof(double) * nn * nn);
+    for (int i = 0; i < n; ++i) dist[i * n + i] = 0.0;
+    for (int i = 0; i < n; ++i)
+      for (int j = i + 1; j < n; ++j) {
+        double dx = r[j * 3 + 0] - r[i * 3 + 0];
+        double dy = r[j * 3 + 1] - r[i * 3 + 1];
+        double dz = r[j * 3 + 2] - r[i * 3 + 2];
+        double s = dx * dx;
+        s = s + dy * dy;
+        s = s + dz * dz;
+        double d = sqrt(s);
+        dist[i * n + j] = d;
+        dist[j * n + i] = d;
+      }
+    /* pair values from the distance: Zi*Zj/distance_matrix(j,i) (:355-356) */
+    for (int i = 0; i < n; ++i)
+      for (int j = i + 1; j < n; ++j) {
+        double p = z[i] * z[j] / dist[j * n + i];
+        pdm[i * n + j] = p;
+        pdm[j * n + i] = p;
+      }
+    double *col = (double *)malloc(sizeof(double) * nn);
+    for (int k = 0; k < n; ++k) {
+      for (int i = 0; i < n; ++i) col[i] = dist[i * n + k];
+      for (int i = 0; i < n; ++i) {
+        int jmin = 0;
+        for (int j = 1; j < n; ++j)
+          if (col[j] < col[jmin]) jmin = j;
+        ord[i] = jmin;
+        col[jmin] = 1.7976931348623157e308;
+      }
+      cm_pack_row(pdm, n, ord, packed, cm + (size_t)k * packed);
+    }
+    free(col);
+    free(dist);
+  }
+  free(pdm);
+  free(norms);
+  free(sorted);
+  free(ord);
+  return 0;
+}
+
+/* Batched form over concatenated molecules (offsets: nmol+1 entries).
+ * Mode 0 writes one packed row per molecule, modes 1-2 one per atom (row
+ * offsets[m] + k for atom k of molecule m). */
+EXPORT int oracle_coulomb_forms(int mode, const double *z, const double *r,
+                                const int *offsets, int nmol, int nmax,
+                                double *cm) {
+  const int packed = nmax * (nmax + 1) / 2;
+  int rc = 0;
+#pragma omp parallel for schedule(dynamic, 16) reduction(| : rc)
+  for (int m = 0; m < nmol; ++m) {
+    int a0 = offsets[m];
+    int na = offsets[m + 1] - a0;
+    size_t row = mode == 0 ? (size_t)m : (size_t)a0;
+    rc |= oracle_coulomb_form(mode, z + a0, r + 3 * (size_t)a0, na, nmax,
+                              cm + row * packed);
+  }
+  return rc ? -1 : 0;
+}
+
+/*
+ * fml/kernels/fkernels.f90:23-162 (fget_vector_kernels_laplacian, kind 1;
+ * fget_vector_kernels_gaussian, kind 0, at :93).
+ * q1[nm1][amax1][d], q2[nm2][amax2][d]: the D-vector of atom ia of molecule
+ * i is q1(:, ia, i) in the reference.  Atom-pair distance
+ * sum(abs(q1-q2)) (:76) or sum((q1-q2)**2) (:147), sequential over d;
+ * inv = -1/sigma (:58) or -0.5/sigma**2 (:128);
+ * kernels(k, i, j) = sum(exp(dist(:ni, :nj) * inv(k))) (:82, :153): one
+ * accumulator over the section in array element order (ia fastest).
+ * out: Fortran (nsig, nm1, nm2) -> out[k + nsig*(i + nm1*j)].
+ * Parity: pinned by the flang build of the reference (oracle/_ref) and by
+ * closed forms (tests/test_oracle.py).
+ */
+EXPORT void oracle_vector_kernels(int kind, const double *q1, const double *q2,
+                                  const int *n1, const int *n2,
+                                  const double *sigmas, int nm1, int nm2,
+                                  int nsig, int amax1, int amax2, int d,
+                                  double *out) {
+  double *inv = (double *)malloc(sizeof(double) * (size_t)(nsig > 0 ? nsig : 1));
+  for (int k = 0; k < nsig; ++k)
+    inv[k] = kind == 1 ? -1.0 / sigmas[k] : -0.5 / (sigmas[k] * sigmas[k]);
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int j = 0; j < nm2; ++j) {
+    double *dist = (double *)malloc(sizeof(double) * (size_t)(amax1 * amax2 > 0 ? amax1 * amax2 : 1));
+    const int nj = n2[j];
+    for (int i = 0; i < nm1; ++i) {
+      const int ni = n1[i];
+      for (int ja = 0; ja < nj; ++ja) {
+        const double *b = q2 + ((size_t)j * amax2 + ja) * d;
+        for (int ia = 0; ia < ni; ++ia) {
+          const double *a = q1 + ((size_t)i * amax1 + ia) * d;
+          double s = 0.0;
+          if (kind == 1)
+            for (int t = 0; t < d; ++t) s = s + fabs(a[t] - b[t]);
+          else
+            for (int t = 0; t < d; ++t) {
+              double df = a[t] - b[t];
+              s = s + df * df;
+            }
+          dist[ja * ni + ia] = s;
+        }
+      }
+      for (int k = 0; k < nsig; ++k) {
+        double acc = 0.0;
+        for (int t = 0; t < ni * nj; ++t) acc = acc + exp(dist[t] * inv[k]);
+        out[k + (size_t)nsig * (i + (size_t)nm1 * j)] = acc;
+      }
+    }
+    free(dist);
+  }
+  free(inv);
+}
+
+/*
+ * fml/kernels/fkernels.f90:380-478 (fget_alpha), the molecular kernel part:
+ * q[na][d] atom rows (q(:, k) in the reference), n[nm] atoms per molecule
+ * in order.  For each molecule j: ltc(i) = sum over atoms k of j, in order,
+ * of exp(inv * sum(abs(q(:,k) - q(:,i)))) with inv = -1/sigma (:407, :430);
+ * ltcl(i, j) = sum(ltc(atoms of i)) for i <= j (:433-435).  Writes the
+ * upper triangle of ltcl, Fortran (nm, nm): ltcl[i + nm*j].
+ * Parity: flang build of the reference (oracle/_ref) through the full
+ * fget_alpha, and closed forms.
+ */
+EXPORT void oracle_alpha_kernel(const double *q, const int *n, int nm, int d,
+                                double sigma, double *ltcl) {
+  const double inv = -1.0 / sigma;
+  int *start = (int *)malloc(sizeof(int) * (size_t)(nm + 1));
+  start[0] = 0;
+  for (int i = 0; i < nm; ++i) start[i + 1] = start[i] + n[i];
+  const int na = start[nm];
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int j = 0; j < nm; ++j) {
+    double *ltc = (double *)malloc(sizeof(double) * (size_t)(na > 0 ? na : 1));
+    for (int i = 0; i < na; ++i) {
+      double acc = 0.0;
+      const double *qi = q + (size_t)i * d;
+      for (int k = start[j]; k < start[j + 1]; ++k) {
+        const double *qk = q + (size_t)k * d;
+        double s = 0.0;
+        for (int t = 0; t < d; ++t) s = s + fabs(qk[t] - qi[t]);
+        acc = acc + exp(inv * s);
+      }
+      ltc[i] = acc;
+    }
+    for (int i = 0; i <= j; ++i) {
+      double s = 0.0;
+      for (int a = start[i]; a < start[i + 1]; ++a) s = s + ltc[a];
+      ltcl[i + (size_t)nm * j] = s;
+    }
+    free(ltc);
+  }
+  free(start);
+}
+
+/*
+ * fml/kernels/fkernels.f90:480-539 (fget_prediction): for each test atom i,
+ * csa(i) = sum over training atoms j, in order, of
+ * exp(inv * sum(abs(q(:,j) - q2(:,i)))) * alpha(j), inv = -1/sigma (:504,
+ * :522); Y(m) = sum(csa(atoms of test molecule m)) (:530).
+ * q[na][d], q2[na2][d] atom rows; n2[nm2].
+ * Parity: flang build of the reference (oracle/_ref) and closed forms.
+ */
+EXPORT void oracle_prediction(const double *q, int na, const double *q2, int na2,
+                              const int *n2, int nm2, int d, const double *alpha,
+                              double sigma, double *y) {
+  const double inv = -1.0 / sigma;
+  double *csa = (double *)malloc(sizeof(double) * (size_t)(na2 > 0 ? na2 : 1));
+#pragma omp parallel for schedule(dynamic, 8)
+  for (int i = 0; i < na2; ++i) {
+    const double *b = q2 + (size_t)i * d;
+    double acc = 0.0;
+    for (int j = 0; j < na; ++j) {
+      const double *a = q + (size_t)j * d;
+      double s = 0.0;
+      for (int t = 0; t < d; ++t) s = s + fabs(a[t] - b[t]);
+      acc = acc + exp(inv * s) * alpha[j];
+    }
+    csa[i] = acc;
+  }
+  int a0 = 0;
+  for (int m = 0; m < nm2; ++m) {
+    double s = 0.0;
+    for (int a = a0; a < a0 + n2[m]; ++a) s = s + csa[a];
+    y[m] = s;
+    a0 += n2[m];
+  }
+  free(csa);
+}
+
+/*
+ * fml/kernels/fkernels.f90:541-641 (fget_alpha_from_distance), the molecular
+ * kernel part: dmat Fortran (na, na), d(k, i) at dmat[k + na*i];
+ * ltc(j, i) = sum over atoms k of j of exp(inv * d(k, i)), inv = -1/dgamma
+ * (:568, :588-590); ltcl(i, j) = sum(ltc(j, atoms of i)) for all i, j
+ * (:597-599).  Writes ltcl Fortran (nm, nm).
+ */
+EXPORT void oracle_alpha_kernel_from_distance(const double *dmat, const int *n, int nm,
+                                              double dgamma, double *ltcl) {
+  const double inv = -1.0 / dgamma;
+  int *start = (int *)malloc(sizeof(int) * (size_t)(nm + 1));
+  start[0] = 0;
+  for (int i = 0; i < nm; ++i) start[i + 1] = start[i] + n[i];
+  const int na = start[nm];
+  double *ltc = (double *)malloc(sizeof(double) * (size_t)(nm > 0 ? nm : 1) * (size_t)(na > 0 ? na : 1));
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < na; ++i)
+    for (int j = 0; j < nm; ++j) {
+      double acc = 0.0;
+      for (int k = start[j]; k < start[j + 1]; ++k) acc = acc + exp(inv * dmat[k + (size_t)na * i]);
+      ltc[j + (size_t)nm * i] = acc;
+    }
+#pragma omp parallel for schedule(static)
+  for (int j = 0; j < nm; ++j)
+    for (int i = 0; i < nm; ++i) {
+      double s = 0.0;
+      for (int a = start[i]; a < start[i + 1]; ++a) s = s + ltc[j + (size_t)nm * a];
+      ltcl[i + (size_t)nm * j] = s;
+    }
+  free(ltc);
+  free(start);
+}
+
+/*
+ * fml/kernels/fkernels.f90:645-702 (fget_prediction_from_distance):
+ * d2 Fortran (na, na2), D2(j, i) at d2[j + na*i];
+ * csa(i) = sum_j exp(inv * D2(j, i)) * alpha(j), inv = -1/dgamma (:668,
+ * :685); Y(m) = sum(csa(atoms of test molecule m)) (:693).
+ * The reference never zeroes csa before accumulating (allocated at :680,
+ * unlike fget_prediction's :517); it is taken as starting from zero here.
+ */
+EXPORT void oracle_prediction_from_distance(const double *d2, int na, int na2,
+                                            const int *n2, int nm2, const double *alpha,
+                                            double dgamma, double *y) {
+  const double inv = -1.0 / dgamma;
+  double *csa = (double *)malloc(sizeof(double) * (size_t)(na2 > 0 ? na2 : 1));
+  for (int i = 0; i < na2; ++i) {
+    double acc = 0.0;
+    for (int j = 0; j < na; ++j) acc = acc + exp(inv * d2[j + (size_t)na * i]) * alpha[j];
+    csa[i] = acc;
+  }
+  int a0 = 0;
+  for (int m = 0; m < nm2; ++m) {
+    double s = 0.0;
+    for (int a = a0; a < a0 + n2[m]; ++a) s = s + csa[a];
+    y[m] = s;
+    a0 += n2[m];
+  }
+  free(csa);
+}

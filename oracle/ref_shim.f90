@@ -151,6 +151,89 @@ module fml_ref_shim
       double precision, intent(in) :: lambda
       double precision, dimension(:), intent(inout) :: alpha
     end subroutine
+    ! fml/representations/frepresentations.f90:112-131
+    subroutine fgenerate_unsorted_coulomb_matrix(nuclear_charges, coordinates, natoms, nmax, cm)
+      double precision, dimension(:), intent(in) :: nuclear_charges
+      double precision, dimension(:,:), intent(in) :: coordinates
+      integer, intent(in) :: nmax
+      double precision, dimension((nmax + 1) * nmax / 2), intent(out) :: cm
+      integer :: natoms
+    end subroutine
+    ! fml/representations/frepresentations.f90:204-214
+    subroutine fgenerate_local_coulomb_matrix(nuclear_charges, coordinates, natoms, nmax, cm)
+      double precision, dimension(:), intent(in) :: nuclear_charges
+      double precision, dimension(:,:), intent(in) :: coordinates
+      integer, intent(in) :: natoms
+      integer, intent(in) :: nmax
+      double precision, dimension(natoms, (nmax + 1) * nmax / 2), intent(out) :: cm
+    end subroutine
+    ! fml/representations/frepresentations.f90:297-307
+    subroutine fgenerate_atomic_coulomb_matrix(nuclear_charges, coordinates, natoms, nmax, cm)
+      double precision, dimension(:), intent(in) :: nuclear_charges
+      double precision, dimension(:,:), intent(in) :: coordinates
+      integer, intent(in) :: natoms
+      integer, intent(in) :: nmax
+      double precision, dimension(natoms, (nmax + 1) * nmax / 2), intent(out) :: cm
+    end subroutine
+    ! fml/kernels/fkernels.f90:23-51
+    subroutine fget_vector_kernels_laplacian(q1, q2, n1, n2, sigmas, nm1, nm2, nsigmas, kernels)
+      double precision, dimension(:,:,:), intent(in) :: q1
+      double precision, dimension(:,:,:), intent(in) :: q2
+      integer, dimension(:), intent(in) :: n1
+      integer, dimension(:), intent(in) :: n2
+      double precision, dimension(:), intent(in) :: sigmas
+      integer, intent(in) :: nm1
+      integer, intent(in) :: nm2
+      integer, intent(in) :: nsigmas
+      double precision, dimension(nsigmas,nm1,nm2), intent(out) :: kernels
+    end subroutine
+    ! fml/kernels/fkernels.f90:93-122
+    subroutine fget_vector_kernels_gaussian(q1, q2, n1, n2, sigmas, nm1, nm2, nsigmas, kernels)
+      double precision, dimension(:,:,:), intent(in) :: q1
+      double precision, dimension(:,:,:), intent(in) :: q2
+      integer, dimension(:), intent(in) :: n1
+      integer, dimension(:), intent(in) :: n2
+      double precision, dimension(:), intent(in) :: sigmas
+      integer, intent(in) :: nm1
+      integer, intent(in) :: nm2
+      integer, intent(in) :: nsigmas
+      double precision, dimension(nsigmas,nm1,nm2), intent(out) :: kernels
+    end subroutine
+    ! fml/kernels/fkernels.f90:380-391
+    subroutine fget_alpha(q, n, y, sigma, lambda, alpha)
+      double precision, dimension(:,:), intent(in) :: q
+      integer, dimension(:), intent(in) :: n
+      double precision, dimension(:), intent(in) :: y
+      double precision, intent(in) :: lambda
+      double precision, intent(in) :: sigma
+      double precision, dimension(:), intent(inout) :: alpha
+    end subroutine
+    ! fml/kernels/fkernels.f90:480-490
+    subroutine fget_prediction(Q, Q2, N2, alpha, sigma, Y)
+      double precision, dimension(:,:), intent(in) :: Q
+      double precision, dimension(:,:), intent(in) :: Q2
+      integer, dimension(:), intent(in) :: N2
+      double precision, dimension(:), intent(in) :: alpha
+      double precision, intent(in) :: sigma
+      double precision, dimension(:), intent(inout) :: Y
+    end subroutine
+    ! fml/kernels/fkernels.f90:541-551
+    subroutine fget_alpha_from_distance(d, n, y, dgamma, lambda, alpha)
+      double precision, dimension(:,:), intent(in) :: d
+      integer, dimension(:), intent(in) :: n
+      double precision, dimension(:), intent(in) :: y
+      double precision, intent(in) :: dgamma
+      double precision, intent(in) :: lambda
+      double precision, dimension(:), intent(inout) :: alpha
+    end subroutine
+    ! fml/kernels/fkernels.f90:645-654
+    subroutine fget_prediction_from_distance(D2, N2, alpha, dgamma, Y)
+      double precision, dimension(:,:), intent(in) :: D2
+      integer, dimension(:), intent(in) :: N2
+      double precision, dimension(:), intent(in) :: alpha
+      double precision, intent(in) :: dgamma
+      double precision, dimension(:), intent(inout) :: Y
+    end subroutine
   end interface
 
 contains
@@ -279,6 +362,80 @@ contains
     real(c_double), value :: sigma, lambda
     real(c_double), intent(inout) :: alpha(na)
     call fget_alpha_arad(q, z, n, y, sigma, lambda, alpha)
+  end subroutine
+
+  ! mode 0: unsorted (cm(packed)); 1: local, 2: atomic (cm(natoms, packed))
+  subroutine ref_coulomb_form(mode, z, r, natoms, nmax, cm) bind(C, name="ref_coulomb_form")
+    integer(c_int), value :: mode, natoms, nmax
+    real(c_double), intent(in) :: z(natoms), r(natoms, 3)
+    real(c_double), intent(out) :: cm(*)
+    integer :: n
+    n = natoms
+    if (mode == 0) then
+      call fgenerate_unsorted_coulomb_matrix(z, r, n, nmax, cm)
+    else if (mode == 1) then
+      call fgenerate_local_coulomb_matrix(z, r, n, nmax, cm)
+    else
+      call fgenerate_atomic_coulomb_matrix(z, r, n, nmax, cm)
+    end if
+  end subroutine
+
+  ! kind 0: Gaussian, 1: Laplacian
+  subroutine ref_vector_kernels(kind, q1, q2, n1, n2, sigmas, nm1, nm2, nsig, amax1, amax2, d, &
+      & kernels) bind(C, name="ref_vector_kernels")
+    integer(c_int), value :: kind, nm1, nm2, nsig, amax1, amax2, d
+    real(c_double), intent(in) :: q1(d, amax1, nm1), q2(d, amax2, nm2)
+    integer(c_int), intent(in) :: n1(nm1), n2(nm2)
+    real(c_double), intent(in) :: sigmas(nsig)
+    real(c_double), intent(out) :: kernels(nsig, nm1, nm2)
+    if (kind == 1) then
+      call fget_vector_kernels_laplacian(q1, q2, n1, n2, sigmas, nm1, nm2, nsig, kernels)
+    else
+      call fget_vector_kernels_gaussian(q1, q2, n1, n2, sigmas, nm1, nm2, nsig, kernels)
+    end if
+  end subroutine
+
+  subroutine ref_get_alpha(q, n, y, sigma, lambda, na, nm, d, alpha) bind(C, name="ref_get_alpha")
+    integer(c_int), value :: na, nm, d
+    real(c_double), intent(in) :: q(d, na)
+    integer(c_int), intent(in) :: n(nm)
+    real(c_double), intent(in) :: y(nm)
+    real(c_double), value :: sigma, lambda
+    real(c_double), intent(inout) :: alpha(na)
+    call fget_alpha(q, n, y, sigma, lambda, alpha)
+  end subroutine
+
+  subroutine ref_get_prediction(q, q2, n2, alpha, sigma, na, na2, nm2, d, y) &
+      & bind(C, name="ref_get_prediction")
+    integer(c_int), value :: na, na2, nm2, d
+    real(c_double), intent(in) :: q(d, na), q2(d, na2)
+    integer(c_int), intent(in) :: n2(nm2)
+    real(c_double), intent(in) :: alpha(na)
+    real(c_double), value :: sigma
+    real(c_double), intent(inout) :: y(nm2)
+    call fget_prediction(q, q2, n2, alpha, sigma, y)
+  end subroutine
+
+  subroutine ref_alpha_from_distance(dmat, n, y, dgamma, lambda, na, nm, alpha) &
+      & bind(C, name="ref_alpha_from_distance")
+    integer(c_int), value :: na, nm
+    real(c_double), intent(in) :: dmat(na, na)
+    integer(c_int), intent(in) :: n(nm)
+    real(c_double), intent(in) :: y(nm)
+    real(c_double), value :: dgamma, lambda
+    real(c_double), intent(inout) :: alpha(na)
+    call fget_alpha_from_distance(dmat, n, y, dgamma, lambda, alpha)
+  end subroutine
+
+  subroutine ref_prediction_from_distance(d2, n2, alpha, dgamma, na, na2, nm2, y) &
+      & bind(C, name="ref_prediction_from_distance")
+    integer(c_int), value :: na, na2, nm2
+    real(c_double), intent(in) :: d2(na, na2)
+    integer(c_int), intent(in) :: n2(nm2)
+    real(c_double), intent(in) :: alpha(na)
+    real(c_double), value :: dgamma
+    real(c_double), intent(inout) :: y(nm2)
+    call fget_prediction_from_distance(d2, n2, alpha, dgamma, y)
   end subroutine
 
 end module fml_ref_shim

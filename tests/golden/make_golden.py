@@ -5,6 +5,7 @@ built (make -C oracle ref):
 
     python tests/golden/make_golden.py            # every fixture
     python tests/golden/make_golden.py inverse    # inverse.npz only
+    python tests/golden/make_golden.py atomic     # atomic.npz only
 
 Inputs: the reference's test molecules tests/xyz/01..10.xyz (copied to
 tests/golden/xyz) and small seeded synthetic sets.  Outputs go next to this
@@ -92,9 +93,56 @@ def inverse_fixtures(Kc=None, X=None, z=None, n=None):
                         y_arad=y, alpha_arad=oracle.ref_alpha_arad(X, z, n, y, 10.0, 1e-3))
 
 
+def atomic_fixtures():
+    """atomic.npz: the atomic-KRR path -- unsorted/local/atomic Coulomb
+    matrices of the xyz molecules (nmax 23), vector Gaussian/Laplacian
+    kernels, get_alpha/get_prediction and get_alpha_from_distance on local
+    Coulomb matrices of a synthetic set (fget_prediction_from_distance is
+    left out: the reference accumulates into an uninitialised array,
+    fkernels.f90:680-686)."""
+    mols = xyz_molecules()
+    z = np.concatenate([np.asarray(m.nuclear_charges, dtype=np.float64) for m in mols])
+    r = np.concatenate([m.coordinates for m in mols])
+    off = np.concatenate([[0], np.cumsum([len(m.nuclear_charges) for m in mols])]).astype(np.int32)
+    forms = {}
+    for mode, name in ((oracle.UNSORTED, "unsorted"), (oracle.LOCAL, "local"), (oracle.ATOMIC, "atomic")):
+        rows = [np.atleast_2d(oracle.ref_coulomb_form(mode, m.nuclear_charges, m.coordinates, 23))
+                for m in mols]
+        forms["cm_" + name] = np.concatenate(rows)
+    ms1 = synth.generate(9, seed=8, nmin=1, nmax=12)
+    ms2 = synth.generate(6, seed=9, nmin=1, nmax=12)
+    def local(ms):
+        return np.concatenate([oracle.ref_coulomb_form(oracle.LOCAL, *ms.molecule(i), 12)
+                               for i in range(ms.nmol)])
+    x1, x2 = local(ms1), local(ms2)            # atom rows (na, 78)
+    def padded(ms, x):
+        q = np.zeros((ms.nmol, 12, x.shape[1]))
+        for i in range(ms.nmol):
+            q[i, :ms.natoms[i]] = x[ms.offsets[i]:ms.offsets[i + 1]]
+        return q
+    q1, q2 = padded(ms1, x1), padded(ms2, x2)
+    sg, sl = np.array([5.0, 40.0, 300.0]), np.array([20.0, 200.0, 2000.0])
+    rng = np.random.default_rng(10)
+    y1 = rng.standard_normal(ms1.nmol)
+    alpha = oracle.ref_get_alpha(x1.T, ms1.natoms, y1, 200.0, 1e-4)
+    pred = oracle.ref_get_prediction(x1.T, x2.T, ms2.natoms, alpha, 200.0)
+    dmat = np.abs(x1[:, None, :] - x1[None, :, :]).sum(axis=2)
+    alpha_d = oracle.ref_get_alpha_from_distance(dmat, ms1.natoms, y1, 200.0, 1e-4)
+    np.savez_compressed(os.path.join(HERE, "atomic.npz"), charges=z, coords=r, offsets=off, **forms,
+                        x1=x1, x2=x2, n1=ms1.natoms, n2=ms2.natoms, q1=q1, q2=q2,
+                        sigmas_gauss=sg, sigmas_lap=sl,
+                        vec_gauss=oracle.ref_vector_kernels(0, q1, q2, ms1.natoms, ms2.natoms, sg),
+                        vec_lap=oracle.ref_vector_kernels(1, q1, q2, ms1.natoms, ms2.natoms, sl),
+                        y1=y1, alpha=alpha, pred=pred, dmat=dmat, alpha_d=alpha_d)
+
+
 if __name__ == "__main__":
     if sys.argv[1:] == ["inverse"]:
         assert oracle.ref_available(), "build oracle/_ref first (make -C oracle ref)"
         inverse_fixtures()
+    elif sys.argv[1:] == ["atomic"]:
+        assert oracle.ref_available(), "build oracle/_ref first (make -C oracle ref)"
+        atomic_fixtures()
     else:
         main()
+        atomic_fixtures()

@@ -349,3 +349,157 @@ def test_oracle_ptp_and_charges():
     for zz in range(1, 119):
         assert tuple(PTP[zz]) == ptp_types(zz)
     assert PTP[101] == [7, 32] and PTP[102] == [7, 14]
+
+
+# ------------------------------------------------------- atomic KRR path
+
+def test_coulomb_forms_golden_bitwise():
+    """Unsorted, local and atomic Coulomb matrices of the reference's xyz
+    molecules against the reference Fortran's outputs."""
+    d = g("atomic.npz")
+    z, r, off = d["charges"], d["coords"], d["offsets"]
+    assert np.array_equal(oracle.coulomb_forms(oracle.UNSORTED, z, r, off, 23), d["cm_unsorted"])
+    assert np.array_equal(oracle.coulomb_forms(oracle.LOCAL, z, r, off, 23), d["cm_local"])
+    assert np.array_equal(oracle.coulomb_forms(oracle.ATOMIC, z, r, off, 23), d["cm_atomic"])
+
+
+def test_coulomb_forms_closed_form():
+    """O at the origin, H at (1,0,0) and (0,2,0): pair values 8, 4, 1/sqrt5;
+    row norms order O, H1, H2; distances from H1 order H1, O, H2."""
+    Z = np.array([8.0, 1.0, 1.0])
+    R = np.array([[0.0, 0, 0], [1.0, 0, 0], [0.0, 2.0, 0]])
+    d0, dh = 0.5 * 8.0 ** 2.4, 0.5
+    h12 = 1.0 / math.sqrt(5.0)
+    un = oracle.coulomb_form(oracle.UNSORTED, Z, R, 4)
+    assert un[:6] == pytest.approx([d0, 8.0, dh, 4.0, h12, dh], rel=1e-15) and np.all(un[6:] == 0)
+    loc = oracle.coulomb_form(oracle.LOCAL, Z, R, 3)
+    # row of H2 (k=2): order H2, O, H1
+    assert loc[2] == pytest.approx([dh, 4.0, d0, h12, 8.0, dh], rel=1e-15)
+    # row of O (k=0): order O, H1, H2 (the sorted list itself)
+    assert loc[0] == pytest.approx([d0, 8.0, dh, 4.0, h12, dh], rel=1e-15)
+    at = oracle.coulomb_form(oracle.ATOMIC, Z, R, 3)
+    # row of H1: order H1 (0), O (1), H2 (sqrt5)
+    assert at[1] == pytest.approx([dh, 8.0, d0, h12, 4.0, dh], rel=1e-15)
+    # row of H2: order H2 (0), O (2), H1 (sqrt5)
+    assert at[2] == pytest.approx([dh, 4.0, d0, h12, 8.0, dh], rel=1e-15)
+    with pytest.raises(ValueError):
+        oracle.coulomb_form(oracle.LOCAL, Z, R, 2)
+
+
+def test_coulomb_forms_invariants():
+    """Row k of the local and atomic forms starts with atom k's diagonal; the
+    atomic form's first column lists pair values by ascending distance."""
+    ms = synth.generate(12, seed=21, nmin=2, nmax=16)
+    for i in range(ms.nmol):
+        Z, R = ms.molecule(i)
+        loc = oracle.coulomb_form(oracle.LOCAL, Z, R, 16)
+        at = oracle.coulomb_form(oracle.ATOMIC, Z, R, 16)
+        diag = 0.5 * Z ** 2.4
+        assert np.allclose(loc[:, 0], diag, rtol=1e-15) and np.allclose(at[:, 0], diag, rtol=1e-15)
+        n = len(Z)
+        tri = [m * (m + 1) // 2 for m in range(n)]   # (m, 0) entries: pair with atom k
+        for k in range(n):
+            dk = np.sqrt(((R - R[k]) ** 2).sum(axis=1))
+            order = np.argsort(dk, kind="stable")
+            expect = [diag[k]] + [Z[k] * Z[j] / dk[j] for j in order[1:]]
+            assert np.allclose(at[k, tri], expect, rtol=1e-14)
+
+
+def test_vector_kernels_golden_bitwise():
+    d = g("atomic.npz")
+    assert np.array_equal(oracle.vector_kernels(0, d["q1"], d["q2"], d["n1"], d["n2"], d["sigmas_gauss"]),
+                          d["vec_gauss"])
+    assert np.array_equal(oracle.vector_kernels(1, d["q1"], d["q2"], d["n1"], d["n2"], d["sigmas_lap"]),
+                          d["vec_lap"])
+
+
+def test_vector_kernels_closed_form():
+    """Brute force with numpy, and one-atom molecules = the dense kernels."""
+    rng = np.random.default_rng(22)
+    nm1, nm2, amax, d = 5, 4, 6, 9
+    q1, q2 = rng.standard_normal((nm1, amax, d)), rng.standard_normal((nm2, amax, d))
+    n1, n2 = rng.integers(1, amax + 1, nm1), rng.integers(1, amax + 1, nm2)
+    sig = [0.7, 3.0]
+    for kind in (0, 1):
+        got = oracle.vector_kernels(kind, q1, q2, n1, n2, sig)
+        for s, sg in enumerate(sig):
+            for i in range(nm1):
+                for j in range(nm2):
+                    a, b = q1[i, :n1[i]], q2[j, :n2[j]]
+                    diff = a[:, None, :] - b[None, :, :]
+                    dist = (diff ** 2).sum(-1) if kind == 0 else np.abs(diff).sum(-1)
+                    inv = -0.5 / sg ** 2 if kind == 0 else -1.0 / sg
+                    assert got[s, i, j] == pytest.approx(np.exp(dist * inv).sum(), rel=1e-13)
+    ones1, ones2 = np.ones(nm1, np.int32), np.ones(nm2, np.int32)
+    A, B = q1[:, 0, :].T, q2[:, 0, :].T
+    assert np.array_equal(oracle.vector_kernels(0, q1, q2, ones1, ones2, [2.0])[0],
+                          oracle.gaussian_kernel(A, B, 2.0))
+    assert np.array_equal(oracle.vector_kernels(1, q1, q2, ones1, ones2, [2.0])[0],
+                          oracle.laplacian_kernel(A, B, 2.0))
+
+
+def _laplacian_mol_kernel(x, n, sigma):
+    E = np.exp(-np.abs(x[:, None, :] - x[None, :, :]).sum(-1) / sigma)
+    idx = np.concatenate([[0], np.cumsum(n)[:-1]])
+    return np.add.reduceat(np.add.reduceat(E, idx, axis=0), idx, axis=1)
+
+
+def test_get_alpha_prediction_golden_bitwise():
+    d = g("atomic.npz")
+    alpha, info = oracle.get_alpha(d["x1"].T, d["n1"], d["y1"], 200.0, 1e-4)
+    assert info == 0 and np.array_equal(alpha, d["alpha"])
+    assert np.array_equal(oracle.get_prediction(d["x1"].T, d["x2"].T, d["n2"], d["alpha"], 200.0),
+                          d["pred"])
+    alpha_d, info = oracle.get_alpha_from_distance(d["dmat"], d["n1"], d["y1"], 200.0, 1e-4)
+    assert info == 0 and np.array_equal(alpha_d, d["alpha_d"])
+
+
+def test_get_alpha_prediction_closed_form():
+    """alpha = (K + lambda I)^-1 y per molecule, repeated over its atoms,
+    K the molecular Laplacian kernel; predictions = segment sums of the
+    atom kernel times alpha.  numpy's solve differs from dpotri + dsymv by
+    a few ulps times the condition number."""
+    d = g("atomic.npz")
+    x1, x2, n1, n2, y = d["x1"], d["x2"], d["n1"], d["n2"], d["y1"]
+    K = _laplacian_mol_kernel(x1, n1, 200.0)
+    ref = np.repeat(np.linalg.solve(K + 1e-4 * np.eye(len(n1)), y), n1)
+    alpha, _ = oracle.get_alpha(x1.T, n1, y, 200.0, 1e-4)
+    cond = np.linalg.cond(K + 1e-4 * np.eye(len(n1)))
+    assert np.max(np.abs(alpha - ref)) <= 1e-15 * cond * np.max(np.abs(ref)) * 10
+    E = np.exp(-np.abs(x2[:, None, :] - x1[None, :, :]).sum(-1) / 200.0)
+    pred = np.add.reduceat(E @ alpha, np.concatenate([[0], np.cumsum(n2)[:-1]]))
+    assert np.allclose(oracle.get_prediction(x1.T, x2.T, n2, alpha, 200.0), pred, rtol=1e-13, atol=1e-14)
+    # the distance forms on the L1 distance matrix are the same model
+    D2 = np.abs(x1[:, None, :] - x2[None, :, :]).sum(-1)
+    assert np.allclose(oracle.get_prediction_from_distance(D2, n2, alpha, 200.0), pred, rtol=1e-13,
+                       atol=1e-14)
+
+
+@pytest.mark.skipif(not oracle.ref_available(), reason="reference build absent")
+def test_atomic_path_vs_reference_build():
+    """Fresh random cases against the reference Fortran, bit for bit
+    (fget_prediction_from_distance excluded: the reference accumulates
+    into an uninitialised array, fkernels.f90:680-686)."""
+    rng = np.random.default_rng(23)
+    ms = synth.generate(15, seed=24, nmin=1, nmax=20)
+    for i in range(ms.nmol):
+        Z, R = ms.molecule(i)
+        for mode in (oracle.UNSORTED, oracle.LOCAL, oracle.ATOMIC):
+            assert np.array_equal(oracle.coulomb_form(mode, Z, R, 20), oracle.ref_coulomb_form(mode, Z, R, 20))
+    nm1, nm2, amax, d = 6, 5, 7, 13
+    q1, q2 = rng.standard_normal((nm1, amax, d)), rng.standard_normal((nm2, amax, d))
+    n1, n2 = rng.integers(0, amax + 1, nm1), rng.integers(1, amax + 1, nm2)
+    for kind in (0, 1):
+        assert np.array_equal(oracle.vector_kernels(kind, q1, q2, n1, n2, [0.4, 5.0]),
+                              oracle.ref_vector_kernels(kind, q1, q2, n1, n2, [0.4, 5.0]))
+    n1 = np.maximum(n1, 1)
+    Q = rng.standard_normal((d, int(n1.sum())))
+    Q2 = rng.standard_normal((d, int(n2.sum())))
+    y = rng.standard_normal(nm1)
+    a, info = oracle.get_alpha(Q, n1, y, 6.0, 1e-2)
+    assert info == 0 and np.array_equal(a, oracle.ref_get_alpha(Q, n1, y, 6.0, 1e-2))
+    assert np.array_equal(oracle.get_prediction(Q, Q2, n2, a, 6.0), oracle.ref_get_prediction(Q, Q2, n2, a, 6.0))
+    P = rng.standard_normal((int(n1.sum()), 3))
+    D = np.sqrt(((P[:, None] - P[None]) ** 2).sum(-1))
+    a, info = oracle.get_alpha_from_distance(D, n1, y, 1.0, 1e-2)
+    assert info == 0 and np.array_equal(a, oracle.ref_get_alpha_from_distance(D, n1, y, 1.0, 1e-2))
