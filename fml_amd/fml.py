@@ -5,7 +5,8 @@ import numpy as np
 
 from .arad import ARAD
 from .data import NUCLEAR_CHARGE
-from .representations import fgenerate_coulomb_matrix
+from .representations import (fgenerate_atomic_coulomb_matrix, fgenerate_coulomb_matrix,
+                              fgenerate_local_coulomb_matrix, fgenerate_unsorted_coulomb_matrix)
 
 
 class Molecule:
@@ -27,6 +28,21 @@ class Molecule:
         """Sorted Coulomb matrix, fml.py:64-66 (computed on the GPU)."""
         self.coulomb_matrix = fgenerate_coulomb_matrix(self.nuclear_charges, self.coordinates,
                                                        self.natoms, size)
+
+    def generate_unsorted_coulomb_matrix(self, size=23):
+        """Unsorted Coulomb matrix, fml.py:68-70 (computed on the GPU)."""
+        self.unsorted_coulomb_matrix = fgenerate_unsorted_coulomb_matrix(
+            self.nuclear_charges, self.coordinates, self.natoms, size)
+
+    def generate_local_coulomb_matrix(self, calc="all", size=23):
+        """Local Coulomb matrices, one row per atom, fml.py:72-74 (GPU)."""
+        self.local_coulomb_matrix = fgenerate_local_coulomb_matrix(
+            self.nuclear_charges, self.coordinates, self.natoms, size)
+
+    def generate_atomic_coulomb_matrix(self, calc="all", size=23):
+        """Atomic Coulomb matrices, one row per atom, fml.py:76-78 (GPU)."""
+        self.atomic_coulomb_matrix = fgenerate_atomic_coulomb_matrix(
+            self.nuclear_charges, self.coordinates, self.natoms, size)
 
     def generate_arad_descriptor(self, size=23):
         """ARAD descriptor, fml.py:80-86."""

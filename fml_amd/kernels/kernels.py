@@ -183,11 +183,23 @@ def fget_alpha_arad(q, z, n, y, sigma, llambda, alpha=None):
     reference's intent(inout) argument; a new array when None.  Returns
     alpha.  A failed factorisation or inversion prints the reference's
     warning line (the coefficients are then NaN)."""
-    from ..krr import add_diagonal, sym_residual
-    from ..math.cho_solve import potrf_inplace, potri_inplace
     dev = is_device_tensor(q)
     counts = to_host(n).astype(np.int64) if is_device_tensor(n) else np.asarray(n, dtype=np.int64)
     K = _arad_call(_native.ARAD_KERNEL, q, z, n, None, None, None, [sigma], 0.2, 6.0, 1.0, 0.5)
+    return solve_expand(K, counts, y, llambda, alpha, dev)
+
+
+def solve_expand(K, counts, y, llambda, alpha, dev):
+    """The regression step shared by fget_alpha, fget_alpha_from_distance
+    (fkernels.f90:446-476) and fget_alpha_arad (farad_kernels.f90:408-434):
+    lambda added to the diagonal of the (nm, nm) device kernel K (Fortran
+    memory; overwritten), dpotrf('U') + dpotri('U') (fmlh_potrf, fmlh_potri),
+    ly = inv(K + lambda I) y from the upper triangle (dsymv('U')), and
+    molecule i's value repeated for its counts[i] atoms.  A failed
+    factorisation or inversion prints the reference's warning line (the
+    coefficients are then NaN).  alpha: filled in place when given."""
+    from ..krr import add_diagonal, sym_residual
+    from ..math.cho_solve import potrf_inplace, potri_inplace
     nm = int(K.shape[0])
     add_diagonal(K, float(llambda))
     info = potrf_inplace(K)
@@ -203,7 +215,8 @@ def fget_alpha_arad(q, z, n, y, sigma, llambda, alpha=None):
     ly = empty((nm,), np.float64)
     sym_residual(K, 0.0, yd, zero, ly)   # ly = 0 - K^-1 y
     ly.neg_()
-    per_atom = torch().repeat_interleave(ly, torch().as_tensor(counts, device="cuda"))
+    per_atom = torch().repeat_interleave(ly, torch().as_tensor(np.asarray(counts, dtype=np.int64),
+                                                                device="cuda"))
     if alpha is None:
         return per_atom if dev else to_host(per_atom)
     if is_device_tensor(alpha):

@@ -96,6 +96,24 @@ int fmlh_coulomb_sorted(const double *z, const double *r, const int32_t *offsets
                         int nmol, int nmax, double *cm, float *cm32,
                         int32_t *perm, int32_t *status, void *stream);
 
+/* Unsorted, local and atomic Coulomb matrices:
+ *   mode 0: fgenerate_unsorted_coulomb_matrix (frepresentations.f90:112-201;
+ *           Molecule.generate_unsorted_coulomb_matrix, fml/fml.py:68-70):
+ *           cm (nmol, nmax(nmax+1)/2), the packed lower triangle in input
+ *           atom order;
+ *   mode 1: fgenerate_local_coulomb_matrix (:204-294; fml/fml.py:72-74);
+ *   mode 2: fgenerate_atomic_coulomb_matrix (:297-390; fml/fml.py:76-78):
+ *           cm (offsets[nmol], nmax(nmax+1)/2), one row per atom k of each
+ *           molecule (row offsets[m] + k): atom k first, then the others by
+ *           descending row norm (mode 1), or all atoms by ascending distance
+ *           from k (mode 2).
+ * z, r, offsets as for fmlh_coulomb_sorted; status as there (a molecule
+ * with more than nmax atoms gets zero rows; the reference writes out of
+ * bounds).  Bitwise equal to the reference for finite coordinates. */
+int fmlh_coulomb_form(int mode, const double *z, const double *r,
+                      const int32_t *offsets, int nmol, int nmax, double *cm,
+                      int32_t *status, void *stream);
+
 /* ------------------------------------------------------------------------
  * (b) Gaussian kernel K(j,i) = exp(-0.5/sigma^2 * ||a_j - b_i||^2)
  * Replaces fgaussian_kernel (fml/kernels/fkernels.f90:317-349; Python entry
@@ -156,6 +174,73 @@ int fmlh_laplacian_kernel_f32(const float *a, int64_t lda, int na,
 #define FMLH_ARAD_SYMMETRIC 1
 #define FMLH_ARAD_KERNEL 2
 #define FMLH_ARAD_DISTANCE 3
+
+/* ------------------------------------------------------------------------
+ * Atomic KRR path (molecule sums over atom-pair kernels)
+ *
+ * Atoms are rows: x (na, d) f64 row-major with leading dimension ld; off
+ * (nm+1) int32 prefix offsets, molecule m owns rows off[m]..off[m+1]-1
+ * (off[0] = 0, off[nm] = na); amax (1..64) bounds every molecule's atom
+ * count.  status (may be NULL) (1) int32: set to 1 when a molecule is
+ * larger than amax (its block is skipped).  Sums have a fixed order: a
+ * molecule pair's value does not depend on the rest of the batch.
+ * ---------------------------------------------------------------------- */
+
+/* fget_vector_kernels_gaussian (kind 0, fkernels.f90:93-162) and
+ * fget_vector_kernels_laplacian (kind 1, :23-91); Python entries
+ * get_atomic_kernels_gaussian / get_atomic_kernels_laplacian
+ * (fml/kernels/kernels.py:351-362, 380-406).
+ * K_s(i, j) = sum over atoms a of molecule i (set 1), b of molecule j
+ * (set 2) of exp(inv_s dist(a, b)); dist = sum_t (x1_a,t - x2_b,t)^2 with
+ * inv = -0.5/sigma^2 (kind 0), or sum_t |x1_a,t - x2_b,t| with
+ * inv = -1/sigma (kind 1).  sigmas: host array of nsig (1..8) values.
+ * out (nm2, nm1, nsig) C order = the reference's Fortran (nsig, nm1, nm2). */
+int fmlh_vector_kernels(int kind, const double *x1, int64_t ldx1,
+                        const int32_t *off1, int nm1, int na1,
+                        const double *x2, int64_t ldx2, const int32_t *off2,
+                        int nm2, int na2, int d, int amax,
+                        const double *sigmas, int nsig, double *out,
+                        int32_t *status, void *stream);
+
+/* The molecular kernel of fget_alpha_from_distance (fkernels.f90:541-599;
+ * Python entry get_alpha_from_distance, fml/kernels/kernels.py:127-134):
+ * dmat: the reference's Fortran (na, na) distance array d(k, a), i.e.
+ * row-major rows a with leading dimension ldd;
+ * k(i, j) = sum over atoms a of i, k of j of exp(-d(k, a) / gamma), written
+ * Fortran (nm, nm): k[i + nm j].  (The same sums for d = the L1 distances
+ * of atom rows give fget_alpha's kernel, fkernels.f90:420-435, which is
+ * fmlh_vector_kernels kind 1 with set 2 = set 1.) */
+int fmlh_distance_mol_kernel(const double *dmat, int64_t ldd,
+                             const int32_t *off, int nm, int na, int amax,
+                             double gamma, double *k, int32_t *status,
+                             void *stream);
+
+/* fget_prediction (fkernels.f90:480-539; get_prediction,
+ * fml/kernels/kernels.py:101-108): test atoms xt (na_t, d) in molecules
+ * off_t, training atoms x (na, d), alpha (na):
+ * y(m) = sum over test atoms a of m, training atoms b of
+ * exp(-sum_t |x_b,t - xt_a,t| / sigma) alpha(b).
+ * ws: fmlh_atomic_predict_ws_bytes(nm_t, na) bytes. */
+size_t fmlh_atomic_predict_ws_bytes(int nm_t, int na);
+int fmlh_atomic_predict(const double *xt, int64_t ldt, const int32_t *off_t,
+                        int nm_t, int na_t, const double *x, int64_t ldx,
+                        int na, int d, int amax, const double *alpha,
+                        double sigma, double *y, int32_t *status, void *ws,
+                        size_t ws_bytes, void *stream);
+
+/* fget_prediction_from_distance (fkernels.f90:645-702;
+ * get_prediction_from_distance, fml/kernels/kernels.py:110-117): d2 the
+ * reference's Fortran (na, na_t) array D2(b, a), i.e. row-major rows a
+ * (test atoms) with leading dimension ldd;
+ * y(m) = sum over test atoms a of m, training atoms b of
+ * exp(-D2(b, a) / gamma) alpha(b).  The reference accumulates into an
+ * uninitialised array (fkernels.f90:680-686); here the sums start at 0.
+ * ws: fmlh_atomic_predict_ws_bytes(nm_t, na) bytes. */
+int fmlh_distance_predict(const double *d2, int64_t ldd, const int32_t *off_t,
+                          int nm_t, int na_t, int na, int amax,
+                          const double *alpha, double gamma, double *y,
+                          int32_t *status, void *ws, size_t ws_bytes,
+                          void *stream);
 
 /* ARAD descriptors on the device: ARAD.describe (fml/arad.py:70-145, the
  * non-periodic form; Python entry Molecule.generate_arad_descriptor,
