@@ -61,7 +61,8 @@ def parse():
     ap.add_argument("--workload", default="arad",
                     choices=["arad", "krr", "gaussian64", "gaussian32", "gaussianbf16x3",
                              "laplacian64", "laplacian32", "cholesky64", "cholesky32",
-                             "krr36k_fp64", "krr36k_mixed", "serving", "arad4096", "arad4096_fp32", "copy"])
+                             "krr36k_fp64", "krr36k_mixed", "serving", "arad4096", "arad4096_fp32", "copy",
+                             "atomic", "atomic_lap", "atomic_mfma"])
     ap.add_argument("--duration", type=float, default=5.0, help="serving run length in seconds")
     ap.add_argument("--n", type=int, default=0, help="problem size override for the 36k workloads")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -352,6 +353,103 @@ def dense36k_workload(args, kind):
     return info, step, units, roofline, None, None
 
 
+def atomic_workload(args, kind=0):
+    """The atomic (vector) kernels of the atomic KRR path: kind 0 the
+    Gaussian (fget_vector_kernels_gaussian, fkernels.f90:93-162, sigmas
+    {30, 300, 3000}), kind 1 the Laplacian (fget_vector_kernels_laplacian,
+    :23-91, sigmas {300, 3000, 30000}); local Coulomb matrices (nmax 23, 276
+    features) of two 4000-molecule sets (5-23 atoms, seeds 0 and 1); one
+    step = one fmlh_vector_kernels call on HBM-resident atom rows."""
+    import torch
+    import fml_amd._native as nat
+    from fml_amd import synth
+    from fml_amd.device import empty, to_device
+    from fml_amd.representations import generate_local_coulomb_matrices
+    nm = args.n or 4000
+    nmax = 23
+    sig = [300.0, 3000.0, 30000.0] if kind == 1 else [30.0, 300.0, 3000.0]
+    ms1 = synth.generate(nm, seed=0, nmin=5, nmax=23)
+    ms2 = synth.generate(nm, seed=1, nmin=5, nmax=23)
+    x1 = generate_local_coulomb_matrices(ms1.charges, ms1.coords, ms1.offsets, nmax, on_device=True)
+    x2 = generate_local_coulomb_matrices(ms2.charges, ms2.coords, ms2.offsets, nmax, on_device=True)
+    off1 = to_device(ms1.offsets, np.int32)
+    off2 = to_device(ms2.offsets, np.int32)
+    na1, na2, d = int(x1.shape[0]), int(x2.shape[0]), int(x1.shape[1])
+    amax = int(max(ms1.natoms.max(), ms2.natoms.max()))
+    out = empty((nm, nm, len(sig)), np.float64)
+    status = empty((1,), np.int32)
+    arr = (ctypes.c_double * len(sig))(*sig)
+    # molecules packed greedily into blocks of at most 128 atoms (host plan)
+    from fml_amd.kernels.atomic import plan_blocks
+    b1, b2 = plan_blocks(ms1.natoms), plan_blocks(ms2.natoms)
+    blk1, blk2 = to_device(b1, np.int32), to_device(b2, np.int32)
+    args_c = (kind, nat.ptr(x1), x1.stride(0), nat.ptr(off1), nm, na1, nat.ptr(x2), x2.stride(0),
+              nat.ptr(off2), nm, na2, d, amax, arr, len(sig), nat.ptr(blk1), len(b1) - 1, nat.ptr(blk2),
+              len(b2) - 1, nat.ptr(out), nat.ptr(status))
+
+    def step():
+        nat.call("fmlh_vector_kernels", *args_c, nat.stream_ptr())
+    step.graph_capable = True
+
+    units = nm * nm * len(sig)
+    pairs = float(na1) * float(na2)
+    name = {0: "Gaussian (fget_vector_kernels_gaussian)",
+            1: "Laplacian (fget_vector_kernels_laplacian)",
+            2: "Gaussian (fget_vector_kernels_gaussian, precision mfma)"}[kind]
+    info = {"metric": "kernel entries/s", "unit": "entries/s", "dtype": "f64",
+            "config": {"workload": f"atomic {name} kernel, {nm}x{nm} molecules (5-23 atoms, seeds 0/1), "
+                                   f"local Coulomb matrices nmax {nmax} (D={d}), {na1}x{na2} atoms, "
+                                   f"sigmas {{{','.join(f'{v:g}' for v in sig)}}}",
+                       "molecules": nm, "atoms": [na1, na2], "features": d,
+                       "atom_blocks": [len(b1) - 1, len(b2) - 1]},
+            "kernel": {0: "k_atom_tiles<L2>", 1: "k_atom_tiles<L1>", 2: "k_atom_tiles<L2 MFMA>"}[kind]}
+
+    def roofline(t):
+        # matrix-core form: 2 D flop per atom pair (the dot product) on the
+        # FP64 MFMA; VALU forms: per feature a difference and an abs-add
+        # (Laplacian, 2 ops) or a difference, a product and a sum (Gaussian,
+        # 3 ops, the reference's order)
+        if kind == 2:
+            fl = pairs * d * 2.0
+            ach = fl / t / 1e12
+            return {"bound": "mfma", "achieved": round(ach, 3), "peak": PEAK_FP64_TFLOPS,
+                    "unit": "TFLOP/s", "frac": round(ach / PEAK_FP64_TFLOPS, 4), "traffic": None,
+                    "atom_pairs_per_s": round(pairs / t, 1), "flop_per_atom_pair": 2 * d}
+        per = 3 if kind == 0 else 2
+        ops = pairs * d * per
+        ach = ops / t / 1e12
+        return {"bound": "valu", "achieved": round(ach, 3), "peak": PEAK_FP64_VALU_TOPS,
+                "unit": "T FP64-VALU-ops/s", "frac": round(ach / PEAK_FP64_VALU_TOPS, 4), "traffic": None,
+                "atom_pairs_per_s": round(pairs / t, 1), "ops_per_atom_pair": per * d}
+
+    def cpu_baseline(rows):
+        import oracle
+        if int(status.item()) != 0:
+            raise RuntimeError("atomic kernel reported an oversize block")
+        rows = max(1, min(rows, nm))
+        bkind = "reference" if oracle.ref_available() else "port"
+        fn = oracle.ref_vector_kernels if bkind == "reference" else oracle.vector_kernels
+        r1, r2 = x1.cpu().numpy(), x2.cpu().numpy()
+        q1 = np.zeros((rows, nmax, d))
+        q2 = np.zeros((nm, nmax, d))
+        for i in range(rows):
+            q1[i, :ms1.natoms[i]] = r1[ms1.offsets[i]:ms1.offsets[i + 1]]
+        for j in range(nm):
+            q2[j, :ms2.natoms[j]] = r2[ms2.offsets[j]:ms2.offsets[j + 1]]
+        t0 = time.perf_counter()
+        kc = fn(1 if kind == 1 else 0, q1, q2, ms1.natoms[:rows], ms2.natoms, sig)
+        dt = time.perf_counter() - t0
+        g = np.transpose(out[:, :rows, :].cpu().numpy(), (2, 1, 0))
+        m = kc > 1e-290
+        err = float(np.max(np.abs(g[m] - kc[m]) / kc[m]))
+        return ({"value": round(len(sig) * rows * nm / dt, 1), "unit": "entries/s",
+                 "cores": host_threads(), "kind": bkind,
+                 "sample": f"set-1 molecules 0..{rows - 1} x all {nm} set-2 molecules "
+                           f"({len(sig) * rows * nm} entries), {dt:.2f} s"}, err)
+
+    return info, step, units, roofline, cpu_baseline, None
+
+
 def copy_workload(args):
     """Plain device-to-device copy of 512 MiB (torch's elementwise copy
     kernel): the known-size calibration for the HBM FETCH_SIZE / WRITE_SIZE
@@ -581,6 +679,9 @@ def main():
         info, step, units, roofline, cpu_baseline, finish = cholesky_workload(args, wl)
     elif wl == "copy":
         info, step, units, roofline, cpu_baseline, finish = copy_workload(args)
+    elif wl.startswith("atomic"):
+        info, step, units, roofline, cpu_baseline, finish = atomic_workload(
+            args, {"atomic": 0, "atomic_lap": 1, "atomic_mfma": 2}[wl])
     else:
         info, step, units, roofline, cpu_baseline, finish = dense36k_workload(args, wl)
     print(f"[bench] workload ready: {info['config']['workload']}", file=sys.stderr, flush=True)

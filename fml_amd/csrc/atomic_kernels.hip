@@ -14,13 +14,20 @@
 //            exp(inv * dist(a, b)) * alpha(b).
 //
 // One workgroup owns a tile of up to 128 x 128 atoms.  Row and column
-// blocks are molecule aligned: block b holds the molecules whose first atom
-// index lies in [b S, (b+1) S) with S = 129 - amax, so no molecule crosses
-// a block and a block spans at most S - 1 + amax = 128 atoms.  (Prediction
+// blocks are molecule aligned: the caller's block list (molecules packed
+// greedily into blocks of at most 128 atoms), or by default block b holds
+// the molecules whose first atom index lies in [b S, (b+1) S) with
+// S = 129 - amax, so no molecule crosses a block and a block spans at most
+// S - 1 + amax = 128 atoms.  (Prediction
 // columns are plain 128-atom blocks: they are summed over, not split by
-// molecule.)  The tile's distances come from a VALU loop over the
-// descriptor (each thread an 8 x 8 register block, the sum over t in order)
-// or from the distance matrix itself.  The epilogue passes over the tile in
+// molecule.)  The tile's distances come from a VALU loop (each thread an
+// 8 x 8 register block of rows ty + 16 i and columns tx + 16 j, so a wave's
+// LDS reads are broadcasts or 16 consecutive doubles; the sum over t in the
+// reference's order, so every distance is the reference's bit for bit),
+// from the matrix cores for the fast squared-distance form (|a|^2 + |b|^2 -
+// 2 a.b, 16x16x4 FP64 MFMA tiles, as the dense Gaussian kernel), or from the
+// distance matrix itself.
+// Exponentials use the table-driven exp_neg_wide (<= 1 ulp).  The epilogue passes over the tile in
 // two 64-row halves through LDS: per row, the exponentials of each column
 // molecule are summed in column order and stored over that molecule's first
 // column (step A); per molecule pair the row sums are added in row order
@@ -30,6 +37,7 @@
 #include <math.h>
 
 #include "common.h"
+#include "exp_neg.h"
 #include "gemm_nt.h"
 
 #pragma clang fp contract(off)
@@ -46,12 +54,19 @@ constexpr int kDS = kT + 1;         // distance tile stride (doubles)
 constexpr int kMaxSig = 8;          // sigmas per launch
 constexpr int kMaxAmax = 64;        // largest molecule
 
-enum Source { kL2 = 0, kL1 = 1, kMatrix = 2 };
+// kL2: squared distances summed over t in the reference's order (difference,
+// product, sum; no fused multiply-add), bitwise the reference's; kL1: the
+// L1 distance in the same order; kL2M: squared distances in the norm form
+// on the matrix cores; kMatrix: distances given.
+enum Source { kL2 = 0, kL1 = 1, kL2M = 2, kMatrix = 3 };
+
+__constant__ double c_exp2_128[128] = {FMLH_EXP2_128_VALUES};
 
 struct Side {
   const double *x;        // atom rows (kL1/kL2), ld doubles apart
   int64_t ld;
   const int32_t *off;     // molecule atom offsets (nm + 1); null: plain blocks
+  const int32_t *blk;     // optional block starts (nb + 1 molecule indices)
   int nm, na, span, nb;   // molecules, atoms, block start interval, blocks
 };
 
@@ -66,7 +81,8 @@ struct TileArgs {
   int ostride;
   const double *w;        // weighted: column weights (c.na)
   double *part;           // weighted: part[i * c.nb + column block]
-  int32_t *status;        // status[0] = 1: a block wider than 128 atoms
+  int32_t *status;        // status[0] = 1: a block wider than 128 atoms or
+                          // a block list outside the molecules
 };
 
 // First m in [0, nm) with off[m] >= v, nm if none.
@@ -95,8 +111,13 @@ __device__ Block side_block(const Side &s, int b) {
     g.a1 = min(s.na, g.a0 + kT);
     return g;
   }
-  g.m0 = b == 0 ? 0 : first_at_least(s.off, s.nm, b * s.span);
-  g.m1 = b == s.nb - 1 ? s.nm : first_at_least(s.off, s.nm, (b + 1) * s.span);
+  if (s.blk) {
+    g.m0 = s.blk[b];
+    g.m1 = s.blk[b + 1];
+  } else {
+    g.m0 = b == 0 ? 0 : first_at_least(s.off, s.nm, b * s.span);
+    g.m1 = b == s.nb - 1 ? s.nm : first_at_least(s.off, s.nm, (b + 1) * s.span);
+  }
   g.a0 = g.m0 < s.nm ? s.off[g.m0] : s.na;
   g.a1 = g.m1 < s.nm ? s.off[g.m1] : s.na;
   return g;
@@ -109,9 +130,11 @@ __global__ __launch_bounds__(256, 2) void k_atom_tiles(TileArgs a) {
       double A[2][kBK * kStage];
       double B[2][kBK * kStage];
     } st;
+    TileSmem<double, kT> mf;
     double D[kHalf * kDS];
   } sm;
   __shared__ double carry[kMaxSig * kT];
+  __shared__ double etab[128];
 
   int bx, by;  // column block, row block
   if (!xcd_tile(a.c.nb, a.r.nb, bx, by)) return;
@@ -119,14 +142,35 @@ __global__ __launch_bounds__(256, 2) void k_atom_tiles(TileArgs a) {
   const Block cb = side_block(a.c, bx);
   const int nrows = rb.a1 - rb.a0, ncols = cb.a1 - cb.a0;
   const int tid = threadIdx.x;
-  if (nrows > kT || ncols > kT || nrows < 0 || ncols < 0) {
+  const bool mols_ok = rb.m0 >= 0 && rb.m0 <= rb.m1 && rb.m1 <= a.r.nm &&
+                       cb.m0 >= 0 && cb.m0 <= cb.m1 && cb.m1 <= a.c.nm;
+  if (!mols_ok || nrows > kT || ncols > kT || nrows < 0 || ncols < 0) {
     if (tid == 0 && a.status) a.status[0] = 1;
     return;
   }
   const int tx = tid & 15, ty = tid >> 4;
+  if (tid < 128) etab[tid] = c_exp2_128[tid];
   double acc[8][8];
+  TileAcc<double, kT> mac;
 
-  if constexpr (kSrc != kMatrix) {
+  if constexpr (kSrc == kL2M) {
+    // squared distances in the norm form |a|^2 + |b|^2 - 2 a.b: the dot
+    // products on the matrix cores, the norms sequential fused sums
+    tile_nt<double, true, kT>(a.r.x + (int64_t)rb.a0 * a.r.ld, a.r.ld, nrows,
+                              a.c.x + (int64_t)cb.a0 * a.c.ld, a.c.ld, ncols, a.d, sm.mf, mac);
+    __syncthreads();  // (tile_nt's last LDS reads)
+    constexpr int NB = kT / 32;
+#pragma unroll
+    for (int im = 0; im < NB; ++im)
+#pragma unroll
+      for (int in = 0; in < NB; ++in)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int m = tile_row<double, kT>(im, r), n = tile_col<kT>(in);
+          mac.v[im][in][r] = fma(-2.0, mac.v[im][in][r], sm.mf.pn[m] + sm.mf.qn[n]);
+        }
+    __syncthreads();  // the norms are read before the epilogue reuses the space
+  } else if constexpr (kSrc == kL1 || kSrc == kL2) {
 #pragma unroll
     for (int i = 0; i < 8; ++i)
 #pragma unroll
@@ -151,6 +195,8 @@ __global__ __launch_bounds__(256, 2) void k_atom_tiles(TileArgs a) {
         sm.st.B[buf][(lk + q) * kStage + lrow] = pb[q];
       }
     };
+    // Staged elements past d are zero on both sides and add exactly 0 to
+    // every distance, so each step runs all kBK elements.
     const int nk = (a.d + kBK - 1) / kBK;
     gload(0);
     lstore(0);
@@ -158,22 +204,24 @@ __global__ __launch_bounds__(256, 2) void k_atom_tiles(TileArgs a) {
     for (int kt = 0; kt < nk; ++kt) {
       const int buf = kt & 1;
       if (kt + 1 < nk) gload((kt + 1) * kBK);
-      const int kmax = min(kBK, a.d - kt * kBK);
-      for (int k = 0; k < kmax; ++k) {
+#pragma unroll 2
+      for (int k = 0; k < kBK; ++k) {
         double ra[8], rbv[8];
 #pragma unroll
-        for (int i = 0; i < 8; ++i) ra[i] = sm.st.A[buf][k * kStage + ty * 8 + i];
+        for (int i = 0; i < 8; ++i) ra[i] = sm.st.A[buf][k * kStage + ty + 16 * i];
 #pragma unroll
-        for (int j = 0; j < 8; ++j) rbv[j] = sm.st.B[buf][k * kStage + tx * 8 + j];
+        for (int j = 0; j < 8; ++j) rbv[j] = sm.st.B[buf][k * kStage + tx + 16 * j];
 #pragma unroll
         for (int i = 0; i < 8; ++i)
 #pragma unroll
           for (int j = 0; j < 8; ++j) {
+          {
             const double t = ra[i] - rbv[j];
             if constexpr (kSrc == kL2)
-              acc[i][j] = fma(t, t, acc[i][j]);
+              acc[i][j] = acc[i][j] + t * t;
             else
               acc[i][j] = acc[i][j] + fabs(t);
+          }
           }
       }
       if (kt + 1 < nk) lstore(buf ^ 1);
@@ -201,12 +249,25 @@ __global__ __launch_bounds__(256, 2) void k_atom_tiles(TileArgs a) {
                                   ? a.dmat[(int64_t)(rb.a0 + hr0 + p) * a.ldd + cb.a0 + q]
                                   : 0.0;
         }
+      } else if constexpr (kSrc == kL2M) {
+        // wave (wm, wn) holds rows wm*64 .. wm*64+63: all in half wm
+        if (tile_wm() == h) {
+          constexpr int NB = kT / 32;
+#pragma unroll
+          for (int im = 0; im < NB; ++im)
+#pragma unroll
+            for (int in = 0; in < NB; ++in)
+#pragma unroll
+              for (int r = 0; r < 4; ++r)
+                sm.D[(tile_row<double, kT>(im, r) - hr0) * kDS + tile_col<kT>(in)] = mac.v[im][in][r];
+        }
       } else {
-        if ((ty >> 3) == h) {
+        // thread (tx, ty) holds rows ty + 16 i, columns tx + 16 j
 #pragma unroll
-          for (int i = 0; i < 8; ++i)
+        for (int i = 0; i < 8; ++i) {
+          if ((i >> 2) != h) continue;
 #pragma unroll
-            for (int j = 0; j < 8; ++j) sm.D[(ty * 8 + i - hr0) * kDS + tx * 8 + j] = acc[i][j];
+          for (int j = 0; j < 8; ++j) sm.D[(ty + 16 * i - hr0) * kDS + tx + 16 * j] = acc[i][j];
         }
       }
       __syncthreads();
@@ -219,7 +280,7 @@ __global__ __launch_bounds__(256, 2) void k_atom_tiles(TileArgs a) {
           if constexpr (kWeighted) {
             const int q0 = qtr * 32, q1 = min(ncols, q0 + 32);
             double v = 0.0;
-            for (int q = q0; q < q1; ++q) v = v + exp(inv * Dp[q]) * a.w[cb.a0 + q];
+            for (int q = q0; q < q1; ++q) v = v + exp_neg_wide(inv * Dp[q], etab) * a.w[cb.a0 + q];
             Dp[q0] = v;
           } else {
             const int lo = cb.a0 + qtr * 32;
@@ -230,7 +291,7 @@ __global__ __launch_bounds__(256, 2) void k_atom_tiles(TileArgs a) {
               const int ce = (jl + 1 < Jc ? coff[jl + 1] : cb.a1) - cb.a0;
               if (ce == cs) continue;
               double v = 0.0;
-              for (int q = cs; q < ce; ++q) v = v + exp(inv * Dp[q]);
+              for (int q = cs; q < ce; ++q) v = v + exp_neg_wide(inv * Dp[q], etab);
               Dp[cs] = v;
             }
           }
@@ -291,15 +352,19 @@ __global__ void k_part_rows(const double *__restrict__ part, int nm, int nb, dou
 
 __global__ void k_zero_status(int32_t *s) { s[0] = 0; }
 
-Side mol_side(const double *x, int64_t ld, const int32_t *off, int nm, int na, int amax) {
+// Molecule-aligned blocks: the caller's block starts (blk, nb blocks), or
+// the molecules whose first atom lies in [b S, (b+1) S), S = 129 - amax.
+Side mol_side(const double *x, int64_t ld, const int32_t *off, int nm, int na, int amax,
+              const int32_t *blk, int nb) {
   Side s;
   s.x = x;
   s.ld = ld;
   s.off = off;
+  s.blk = blk;
   s.nm = nm;
   s.na = na;
   s.span = kT + 1 - amax;
-  s.nb = na > 0 ? (na + s.span - 1) / s.span : 1;
+  s.nb = blk ? nb : (na > 0 ? (na + s.span - 1) / s.span : 1);
   return s;
 }
 
@@ -308,6 +373,7 @@ Side plain_side(const double *x, int64_t ld, int na) {
   s.x = x;
   s.ld = ld;
   s.off = nullptr;
+  s.blk = nullptr;
   s.nm = 0;
   s.na = na;
   s.span = kT;
@@ -341,9 +407,10 @@ size_t predict_ws(int nm_test, int na_train) {
 extern "C" int fmlh_vector_kernels(int kind, const double *x1, int64_t ldx1, const int32_t *off1,
                                    int nm1, int na1, const double *x2, int64_t ldx2,
                                    const int32_t *off2, int nm2, int na2, int d, int amax,
-                                   const double *sigmas, int nsig, double *out, int32_t *status,
+                                   const double *sigmas, int nsig, const int32_t *blk1, int nb1,
+                                   const int32_t *blk2, int nb2, double *out, int32_t *status,
                                    void *stream) {
-  FMLH_ARG_CHECK(kind == 0 || kind == 1, 1);
+  FMLH_ARG_CHECK(kind >= 0 && kind <= 2, 1);
   FMLH_ARG_CHECK(nm1 >= 0, 5);
   FMLH_ARG_CHECK(na1 >= 0, 6);
   FMLH_ARG_CHECK(nm2 >= 0, 10);
@@ -356,13 +423,15 @@ extern "C" int fmlh_vector_kernels(int kind, const double *x1, int64_t ldx1, con
   if (nm1 == 0 || nm2 == 0) return 0;
   FMLH_ARG_CHECK(off1 != nullptr && off2 != nullptr, 4);
   FMLH_ARG_CHECK((x1 != nullptr || na1 == 0) && (x2 != nullptr || na2 == 0), 2);
-  FMLH_ARG_CHECK(out != nullptr, 16);
+  FMLH_ARG_CHECK(blk1 == nullptr || nb1 >= 1, 17);
+  FMLH_ARG_CHECK(blk2 == nullptr || nb2 >= 1, 19);
+  FMLH_ARG_CHECK(out != nullptr, 20);
   const hipStream_t st = as_stream(stream);
   int rc = zero_status(status, st);
   if (rc) return rc;
   TileArgs a = {};
-  a.r = mol_side(x1, ldx1, off1, nm1, na1, amax);
-  a.c = mol_side(x2, ldx2, off2, nm2, na2, amax);
+  a.r = mol_side(x1, ldx1, off1, nm1, na1, amax, blk1, nb1);
+  a.c = mol_side(x2, ldx2, off2, nm2, na2, amax, blk2, nb2);
   a.d = d;
   a.nsig = nsig;
   for (int s = 0; s < nsig; ++s)
@@ -371,15 +440,16 @@ extern "C" int fmlh_vector_kernels(int kind, const double *x1, int64_t ldx1, con
   a.ostride = nsig;
   a.status = status;
   timing_begin(st);
-  rc = kind == 1 ? launch_tiles<kL1, false>(a, st, "k_atom_tiles<L1>")
-                 : launch_tiles<kL2, false>(a, st, "k_atom_tiles<L2>");
+  rc = kind == 1   ? launch_tiles<kL1, false>(a, st, "k_atom_tiles<L1>")
+       : kind == 2 ? launch_tiles<kL2M, false>(a, st, "k_atom_tiles<L2 MFMA>")
+                   : launch_tiles<kL2, false>(a, st, "k_atom_tiles<L2>");
   timing_end(st);
   return rc;
 }
 
 extern "C" int fmlh_distance_mol_kernel(const double *dmat, int64_t ldd, const int32_t *off, int nm,
-                                        int na, int amax, double gamma, double *k, int32_t *status,
-                                        void *stream) {
+                                        int na, int amax, double gamma, const int32_t *blk, int nb,
+                                        double *k, int32_t *status, void *stream) {
   FMLH_ARG_CHECK(nm >= 0, 4);
   FMLH_ARG_CHECK(na >= 0, 5);
   FMLH_ARG_CHECK(ldd >= na, 2);
@@ -388,12 +458,13 @@ extern "C" int fmlh_distance_mol_kernel(const double *dmat, int64_t ldd, const i
   if (nm == 0) return 0;
   FMLH_ARG_CHECK(off != nullptr, 3);
   FMLH_ARG_CHECK(dmat != nullptr || na == 0, 1);
-  FMLH_ARG_CHECK(k != nullptr, 8);
+  FMLH_ARG_CHECK(blk == nullptr || nb >= 1, 8);
+  FMLH_ARG_CHECK(k != nullptr, 10);
   const hipStream_t st = as_stream(stream);
   int rc = zero_status(status, st);
   if (rc) return rc;
   TileArgs a = {};
-  a.r = mol_side(nullptr, 0, off, nm, na, amax);
+  a.r = mol_side(nullptr, 0, off, nm, na, amax, blk, nb);
   a.c = a.r;
   a.dmat = dmat;
   a.ldd = ldd;
@@ -427,8 +498,9 @@ static int predict_common(TileArgs &a, int src, double *y, void *ws, size_t ws_b
 
 extern "C" int fmlh_atomic_predict(const double *xt, int64_t ldt, const int32_t *off_t, int nm_t,
                                    int na_t, const double *x, int64_t ldx, int na, int d, int amax,
-                                   const double *alpha, double sigma, double *y, int32_t *status,
-                                   void *ws, size_t ws_bytes, void *stream) {
+                                   const double *alpha, double sigma, const int32_t *blk_t,
+                                   int nb_t, double *y, int32_t *status, void *ws,
+                                   size_t ws_bytes, void *stream) {
   FMLH_ARG_CHECK(nm_t >= 0, 4);
   FMLH_ARG_CHECK(na_t >= 0, 5);
   FMLH_ARG_CHECK(na >= 0, 8);
@@ -438,15 +510,16 @@ extern "C" int fmlh_atomic_predict(const double *xt, int64_t ldt, const int32_t 
   FMLH_ARG_CHECK(sigma > 0.0, 12);
   if (nm_t == 0) return 0;
   FMLH_ARG_CHECK(off_t != nullptr, 3);
-  FMLH_ARG_CHECK(y != nullptr, 13);
+  FMLH_ARG_CHECK(blk_t == nullptr || nb_t >= 1, 13);
+  FMLH_ARG_CHECK(y != nullptr, 15);
   FMLH_ARG_CHECK((xt != nullptr || na_t == 0) && (x != nullptr || na == 0), 1);
   FMLH_ARG_CHECK(alpha != nullptr || na == 0, 11);
-  FMLH_ARG_CHECK(ws != nullptr && ws_bytes >= predict_ws(nm_t, na), 15);
+  FMLH_ARG_CHECK(ws != nullptr && ws_bytes >= predict_ws(nm_t, na), 17);
   const hipStream_t st = as_stream(stream);
   int rc = zero_status(status, st);
   if (rc) return rc;
   TileArgs a = {};
-  a.r = mol_side(xt, ldt, off_t, nm_t, na_t, amax);
+  a.r = mol_side(xt, ldt, off_t, nm_t, na_t, amax, blk_t, nb_t);
   a.c = plain_side(x, ldx, na);
   a.d = d;
   a.nsig = 1;
@@ -463,13 +536,13 @@ extern "C" int fmlh_atomic_predict(const double *xt, int64_t ldt, const int32_t 
   timing_begin(st);
   rc = predict_common(a, kL1, y, ws, ws_bytes, st);
   timing_end(st);
-  return rc == -99 ? -15 : rc;
+  return rc == -99 ? -17 : rc;
 }
 
 extern "C" int fmlh_distance_predict(const double *d2, int64_t ldd, const int32_t *off_t, int nm_t,
                                      int na_t, int na, int amax, const double *alpha, double gamma,
-                                     double *y, int32_t *status, void *ws, size_t ws_bytes,
-                                     void *stream) {
+                                     const int32_t *blk_t, int nb_t, double *y, int32_t *status,
+                                     void *ws, size_t ws_bytes, void *stream) {
   FMLH_ARG_CHECK(nm_t >= 0, 4);
   FMLH_ARG_CHECK(na_t >= 0, 5);
   FMLH_ARG_CHECK(na >= 0, 6);
@@ -478,15 +551,16 @@ extern "C" int fmlh_distance_predict(const double *d2, int64_t ldd, const int32_
   FMLH_ARG_CHECK(gamma > 0.0, 9);
   if (nm_t == 0) return 0;
   FMLH_ARG_CHECK(off_t != nullptr, 3);
-  FMLH_ARG_CHECK(y != nullptr, 10);
+  FMLH_ARG_CHECK(blk_t == nullptr || nb_t >= 1, 10);
+  FMLH_ARG_CHECK(y != nullptr, 12);
   FMLH_ARG_CHECK(d2 != nullptr || na_t == 0 || na == 0, 1);
   FMLH_ARG_CHECK(alpha != nullptr || na == 0, 8);
-  FMLH_ARG_CHECK(ws != nullptr && ws_bytes >= predict_ws(nm_t, na), 12);
+  FMLH_ARG_CHECK(ws != nullptr && ws_bytes >= predict_ws(nm_t, na), 14);
   const hipStream_t st = as_stream(stream);
   int rc = zero_status(status, st);
   if (rc) return rc;
   TileArgs a = {};
-  a.r = mol_side(nullptr, 0, off_t, nm_t, na_t, amax);
+  a.r = mol_side(nullptr, 0, off_t, nm_t, na_t, amax, blk_t, nb_t);
   a.c = plain_side(nullptr, 0, na);
   a.dmat = d2;
   a.ldd = ldd;
@@ -501,5 +575,5 @@ extern "C" int fmlh_distance_predict(const double *d2, int64_t ldd, const int32_
     return 0;
   }
   rc = predict_common(a, kMatrix, y, ws, ws_bytes, st);
-  return rc == -99 ? -12 : rc;
+  return rc == -99 ? -14 : rc;
 }

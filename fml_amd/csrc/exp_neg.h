@@ -1,5 +1,6 @@
-// exp(x) on the ARAD overlap's argument domain, shared by the device code
-// and the host-side accuracy test.
+// exp(x) for non-positive x, shared by the device code and the host-side
+// accuracy test: exp_neg on the ARAD overlap's domain, exp_neg_wide for the
+// atom-pair kernels' exponents.
 //
 // Inside the ARAD band every exponent x = (d2 - d1)^2 * (-1/(4 w^2)) lies in
 // (-16, 0] ((8w)^2 * (-1/(4w^2)) = -16).  exp_neg() evaluates exp there with
@@ -80,6 +81,15 @@ FMLH_HD inline double exp_neg(double x, const double *__restrict__ tab) {
   vb += (long long)(k >> 7) * (1LL << 52);
   __builtin_memcpy(&v, &vb, sizeof(v));
   return v;
+}
+
+// exp(x) for any x <= 0 (and the tiny positive x that rounding can give a
+// squared distance): the table step above while the result stays normal
+// (x >= -707: t + t q >= 2^-1 and k >> 7 >= -1021, so the exponent field
+// stays >= 1), the library exp below, where results are subnormal or zero,
+// and for NaN.  Same error bound as exp_neg on [-707, 0].
+FMLH_HD inline double exp_neg_wide(double x, const double *__restrict__ tab) {
+  return x >= -707.0 ? exp_neg(x, tab) : exp(x);
 }
 
 }  // namespace fmlh

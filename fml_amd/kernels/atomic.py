@@ -32,6 +32,25 @@ def _offsets(counts):
     return np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)
 
 
+def plan_blocks(counts, tile=128):
+    """Molecules packed greedily, in order, into blocks of at most `tile`
+    atoms (the device's tile side): (nb + 1,) int32 block starts."""
+    off = np.concatenate([[0], np.cumsum(np.asarray(counts, dtype=np.int64))])
+    nm = len(off) - 1
+    starts = [0]
+    m = 0
+    while m < nm:
+        m2 = int(np.searchsorted(off, off[m] + tile, side="right")) - 1
+        m = min(max(m2, m + 1), nm)
+        starts.append(m)
+    return np.asarray(starts, dtype=np.int32)
+
+
+def _blocks(counts):
+    b = plan_blocks(counts)
+    return to_device(b, np.int32), int(b.shape[0]) - 1
+
+
 def _amax(*counts):
     m = max([int(c.max()) if c.size else 0 for c in counts] + [1])
     if m > MAX_ATOMS:
@@ -80,7 +99,8 @@ def _check_status(status):
 
 def vector_kernels(kind, x1, counts1, x2, counts2, sigmas):
     """Device (nm2, nm1, nsig) molecular sums of the atom-pair Gaussian
-    (kind 0) or Laplacian (kind 1) kernel over device atom rows x1, x2."""
+    (kind 0; kind 2 with the matrix-core distances) or Laplacian (kind 1)
+    kernel over device atom rows x1, x2."""
     nm1, nm2 = int(counts1.shape[0]), int(counts2.shape[0])
     amax = _amax(counts1, counts2)
     sig = np.ascontiguousarray(np.asarray(sigmas, dtype=np.float64).reshape(-1))
@@ -95,6 +115,8 @@ def vector_kernels(kind, x1, counts1, x2, counts2, sigmas):
         raise ValueError(f"descriptor lengths differ: {d} and {int(x2.shape[1])}")
     off1 = to_device(_offsets(counts1), np.int32)
     off2 = to_device(_offsets(counts2), np.int32)
+    blk1, nb1 = _blocks(counts1)
+    blk2, nb2 = (blk1, nb1) if counts2 is counts1 else _blocks(counts2)
     status = empty((1,), np.int32)
     for c0 in range(0, ns, _MAX_SIG):
         chunk = sig[c0:c0 + _MAX_SIG]
@@ -102,19 +124,30 @@ def vector_kernels(kind, x1, counts1, x2, counts2, sigmas):
         tmp = out if len(chunk) == ns else empty((nm2, nm1, len(chunk)), np.float64)
         _native.call("fmlh_vector_kernels", kind, _native.ptr(x1), x1.stride(0), _native.ptr(off1), nm1,
                      int(x1.shape[0]), _native.ptr(x2), x2.stride(0), _native.ptr(off2), nm2,
-                     int(x2.shape[0]), d, amax, arr, len(chunk), _native.ptr(tmp), _native.ptr(status),
-                     _native.stream_ptr())
+                     int(x2.shape[0]), d, amax, arr, len(chunk), _native.ptr(blk1), nb1, _native.ptr(blk2),
+                     nb2, _native.ptr(tmp), _native.ptr(status), _native.stream_ptr())
         _check_status(status)
         if tmp is not out:
             out[:, :, c0:c0 + len(chunk)] = tmp
     return out
 
 
-def fget_vector_kernels_gaussian(q1, q2, n1, n2, sigmas, nm1=None, nm2=None, nsigmas=None):
+_GAUSS_KIND = {"fp64": 0, "mfma": 2}
+
+
+def fget_vector_kernels_gaussian(q1, q2, n1, n2, sigmas, nm1=None, nm2=None, nsigmas=None,
+                                 precision="fp64"):
     """fkernels.f90:93 -- q1 (d, amax1, nm1), q2 (d, amax2, nm2) ->
-    (nsigmas, nm1, nm2): sum over atom pairs of exp(-|a - b|^2 / (2 sigma^2))."""
+    (nsigmas, nm1, nm2): sum over atom pairs of exp(-|a - b|^2 / (2 sigma^2)).
+
+    precision "fp64": the squared distances summed in the reference's order
+    (the reference's distances bit for bit); "mfma": |a|^2 + |b|^2 - 2 a.b
+    with the products on the matrix cores -- faster, each distance within
+    about 1e-13 (|a|^2 + |b|^2), so an entry K carries a relative error of
+    about 1e-12 + 1e-13 |ln K| (noticeable only for entries far below 1)."""
     c1, c2 = _counts(n1), _counts(n2)
-    out = vector_kernels(0, _rows_from_q(q1, c1), c1, _rows_from_q(q2, c2), c2, sigmas)
+    out = vector_kernels(_GAUSS_KIND[precision], _rows_from_q(q1, c1), c1, _rows_from_q(q2, c2), c2,
+                         sigmas)
     return _ret(out, is_device_tensor(q1), (2, 1, 0))
 
 
@@ -132,16 +165,17 @@ def get_atomic_kernels_laplacian(x1, x2, N1, N2, sigmas):
     return fget_vector_kernels_laplacian(x1, x2, N1, N2, sigmas)
 
 
-def get_atomic_kernels_gaussian(mols1, mols2, sigmas):
+def get_atomic_kernels_gaussian(mols1, mols2, sigmas, precision="fp64"):
     """kernels.py:380-406 -- the molecules' local_coulomb_matrix rows
-    (Molecule.generate_local_coulomb_matrix) -> (nsigmas, nm1, nm2)."""
+    (Molecule.generate_local_coulomb_matrix) -> (nsigmas, nm1, nm2);
+    precision as fget_vector_kernels_gaussian."""
     c1 = np.array([m.natoms for m in mols1], dtype=np.int64)
     c2 = np.array([m.natoms for m in mols2], dtype=np.int64)
     r1 = np.concatenate([np.asarray(m.local_coulomb_matrix, dtype=np.float64) for m in mols1])
     r2 = np.concatenate([np.asarray(m.local_coulomb_matrix, dtype=np.float64) for m in mols2])
     if r1.shape[0] != c1.sum() or r2.shape[0] != c2.sum():
         raise ValueError("local_coulomb_matrix rows differ from natoms")
-    out = vector_kernels(0, to_device(r1), c1, to_device(r2), c2, sigmas)
+    out = vector_kernels(_GAUSS_KIND[precision], to_device(r1), c1, to_device(r2), c2, sigmas)
     return _ret(out, False, (2, 1, 0))
 
 
@@ -189,10 +223,11 @@ def _predict(xt, counts_t, x, alpha, sigma):
     wsb = lib.fmlh_atomic_predict_ws_bytes(nm_t, na)
     ws = workspace(wsb)
     status = empty((1,), np.int32)
+    blk, nb = _blocks(counts_t)
     _native.call("fmlh_atomic_predict", _native.ptr(xt), xt.stride(0), _native.ptr(off), nm_t,
                  int(xt.shape[0]), _native.ptr(x), x.stride(0), na, d, amax, _native.ptr(al),
-                 float(sigma), _native.ptr(y), _native.ptr(status), _native.ptr(ws), wsb,
-                 _native.stream_ptr())
+                 float(sigma), _native.ptr(blk), nb, _native.ptr(y), _native.ptr(status), _native.ptr(ws),
+                 wsb, _native.stream_ptr())
     _check_status(status)
     return y
 
@@ -227,9 +262,11 @@ def _mol_kernel_from_distance(D, counts, gamma):
     if nm == 0:
         return K
     status = empty((1,), np.int32)
+    blk, nb = _blocks(counts)
     _native.call("fmlh_distance_mol_kernel", _native.ptr(dm), dm.stride(0),
                  _native.ptr(to_device(_offsets(counts), np.int32)), nm, na, _amax(counts),
-                 float(gamma), _native.ptr(K), _native.ptr(status), _native.stream_ptr())
+                 float(gamma), _native.ptr(blk), nb, _native.ptr(K), _native.ptr(status),
+                 _native.stream_ptr())
     _check_status(status)
     return K
 
@@ -272,10 +309,11 @@ def get_prediction_from_distance(D2, N2, alpha, sigma):
         wsb = lib.fmlh_atomic_predict_ws_bytes(nm_t, na)
         ws = workspace(wsb)
         status = empty((1,), np.int32)
+        blk, nb = _blocks(c2)
         _native.call("fmlh_distance_predict", _native.ptr(dm), dm.stride(0),
                      _native.ptr(to_device(_offsets(c2), np.int32)), nm_t, na_t, na, _amax(c2),
-                     _native.ptr(al), float(sigma), _native.ptr(y), _native.ptr(status), _native.ptr(ws),
-                     wsb, _native.stream_ptr())
+                     _native.ptr(al), float(sigma), _native.ptr(blk), nb, _native.ptr(y),
+                     _native.ptr(status), _native.ptr(ws), wsb, _native.stream_ptr())
         _check_status(status)
     return y if is_device_tensor(D2) else to_host(y)
 

@@ -181,25 +181,35 @@ int fmlh_laplacian_kernel_f32(const float *a, int64_t lda, int na,
  * Atoms are rows: x (na, d) f64 row-major with leading dimension ld; off
  * (nm+1) int32 prefix offsets, molecule m owns rows off[m]..off[m+1]-1
  * (off[0] = 0, off[nm] = na); amax (1..64) bounds every molecule's atom
- * count.  status (may be NULL) (1) int32: set to 1 when a molecule is
- * larger than amax (its block is skipped).  Sums have a fixed order: a
- * molecule pair's value does not depend on the rest of the batch.
+ * count.  Molecules are processed in blocks of at most 128 atoms: blk
+ * (optional, device, nb + 1 int32 molecule indices, blk[0] = 0,
+ * blk[nb] = nm, non-decreasing) lists the caller's blocks (e.g. molecules
+ * packed greedily); NULL takes the molecules whose first atom lies in
+ * [b S, (b+1) S), S = 129 - amax.  status (may be NULL) (1) int32: set to 1
+ * when a block spans more than 128 atoms (a molecule larger than amax, or a
+ * bad block list; the block is skipped).  Sums have a fixed order: a
+ * molecule pair's value does not depend on the blocks or on the rest of
+ * the batch.
  * ---------------------------------------------------------------------- */
 
-/* fget_vector_kernels_gaussian (kind 0, fkernels.f90:93-162) and
+/* fget_vector_kernels_gaussian (kinds 0 and 2, fkernels.f90:93-162) and
  * fget_vector_kernels_laplacian (kind 1, :23-91); Python entries
  * get_atomic_kernels_gaussian / get_atomic_kernels_laplacian
  * (fml/kernels/kernels.py:351-362, 380-406).
  * K_s(i, j) = sum over atoms a of molecule i (set 1), b of molecule j
  * (set 2) of exp(inv_s dist(a, b)); dist = sum_t (x1_a,t - x2_b,t)^2 with
- * inv = -0.5/sigma^2 (kind 0), or sum_t |x1_a,t - x2_b,t| with
- * inv = -1/sigma (kind 1).  sigmas: host array of nsig (1..8) values.
+ * inv = -0.5/sigma^2 (kind 0: summed over t in the reference's order, the
+ * reference's distances bit for bit; kind 2: |a|^2 + |b|^2 - 2 a.b with the
+ * products on the matrix cores, faster, distances to about 1e-13 of
+ * |a|^2 + |b|^2), or dist = sum_t |x1_a,t - x2_b,t| with inv = -1/sigma
+ * (kind 1, in the reference's order).  sigmas: host array of nsig (1..8) values.
  * out (nm2, nm1, nsig) C order = the reference's Fortran (nsig, nm1, nm2). */
 int fmlh_vector_kernels(int kind, const double *x1, int64_t ldx1,
                         const int32_t *off1, int nm1, int na1,
                         const double *x2, int64_t ldx2, const int32_t *off2,
                         int nm2, int na2, int d, int amax,
-                        const double *sigmas, int nsig, double *out,
+                        const double *sigmas, int nsig, const int32_t *blk1,
+                        int nb1, const int32_t *blk2, int nb2, double *out,
                         int32_t *status, void *stream);
 
 /* The molecular kernel of fget_alpha_from_distance (fkernels.f90:541-599;
@@ -212,8 +222,8 @@ int fmlh_vector_kernels(int kind, const double *x1, int64_t ldx1,
  * fmlh_vector_kernels kind 1 with set 2 = set 1.) */
 int fmlh_distance_mol_kernel(const double *dmat, int64_t ldd,
                              const int32_t *off, int nm, int na, int amax,
-                             double gamma, double *k, int32_t *status,
-                             void *stream);
+                             double gamma, const int32_t *blk, int nb,
+                             double *k, int32_t *status, void *stream);
 
 /* fget_prediction (fkernels.f90:480-539; get_prediction,
  * fml/kernels/kernels.py:101-108): test atoms xt (na_t, d) in molecules
@@ -225,8 +235,9 @@ size_t fmlh_atomic_predict_ws_bytes(int nm_t, int na);
 int fmlh_atomic_predict(const double *xt, int64_t ldt, const int32_t *off_t,
                         int nm_t, int na_t, const double *x, int64_t ldx,
                         int na, int d, int amax, const double *alpha,
-                        double sigma, double *y, int32_t *status, void *ws,
-                        size_t ws_bytes, void *stream);
+                        double sigma, const int32_t *blk_t, int nb_t,
+                        double *y, int32_t *status, void *ws, size_t ws_bytes,
+                        void *stream);
 
 /* fget_prediction_from_distance (fkernels.f90:645-702;
  * get_prediction_from_distance, fml/kernels/kernels.py:110-117): d2 the
@@ -238,7 +249,8 @@ int fmlh_atomic_predict(const double *xt, int64_t ldt, const int32_t *off_t,
  * ws: fmlh_atomic_predict_ws_bytes(nm_t, na) bytes. */
 int fmlh_distance_predict(const double *d2, int64_t ldd, const int32_t *off_t,
                           int nm_t, int na_t, int na, int amax,
-                          const double *alpha, double gamma, double *y,
+                          const double *alpha, double gamma,
+                          const int32_t *blk_t, int nb_t, double *y,
                           int32_t *status, void *ws, size_t ws_bytes,
                           void *stream);
 

@@ -1,9 +1,11 @@
 // Host-side accuracy check of exp_neg (fml_amd/csrc/exp_neg.h, the exp of
-// the ARAD overlap) against
-// long-double expl on (-16, 0].  Prints: points, max ulp error, worst x.
+// the ARAD overlap) against long-double expl on (-16, 0], and of
+// exp_neg_wide on [-760, 1e-9].  Prints two lines: points, max ulp error,
+// worst x.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <initializer_list>
 
 #define FMLH_HD
 #include "exp_neg.h"
@@ -51,5 +53,38 @@ int main(int argc, char **argv) {
   check(0.0);
   check(std::nextafter(-16.0, 0.0));
   std::printf("%ld %.6f %.17g\n", count, worst, wx);
+  // exp_neg_wide on [-760, 1e-9]: grid, both sides of the -707 switch, the
+  // subnormal range and tiny positive arguments
+  double worst_w = 0.0, wxw = 0.0;
+  long count_w = 0;
+  auto check_w = [&](double x) {
+    const double v = fmlh::exp_neg_wide(x, kTab);
+    const long double ref = expl((long double)x);
+    double e;
+    if ((double)ref == 0.0)
+      e = v == 0.0 ? 0.0 : 1e9;
+    else
+      e = ulp_err(v, ref);
+    ++count_w;
+    if (e > worst_w) {
+      worst_w = e;
+      wxw = x;
+    }
+  };
+  for (long i = 0; i < n; ++i) check_w(-760.0 * ((double)i + 0.5) / (double)n);
+  for (double c : {-707.0, -708.0, -708.3964185322641, -745.1332191019411}) {
+    double x = c;
+    for (int s = 0; s < 64; ++s) {
+      check_w(x);
+      x = std::nextafter(x, 1.0);
+    }
+    x = c;
+    for (int s = 0; s < 64; ++s) {
+      x = std::nextafter(x, -1e300);
+      check_w(x);
+    }
+  }
+  for (double x = 1e-300; x < 1e-9; x *= 3.0) check_w(x);
+  std::printf("%ld %.6f %.17g\n", count_w, worst_w, wxw);
   return 0;
 }

@@ -4,13 +4,13 @@ the committed reference outputs (tests/golden/atomic.npz).
 
 Tolerances (stated per check):
   * unsorted / local / atomic Coulomb matrices: bit-exact.
-  * vector kernels: atom distances are the oracle's bit for bit for the
-    Laplacian (same sequential order over the descriptor); the Gaussian's
-    squared differences accumulate with fused multiply-adds (1 rounding per
-    step instead of 2).  Entries differ by the device exp and the order of
-    the per-pair sum: 1e-13 relative (Laplacian), 1e-12 * max(1, |ln K|/30)
-    (Gaussian: a last-bit change of the exponent x moves exp(x) by |x|
-    ulps), with entries below 1e-290 compared as underflow.
+  * vector kernels: atom distances are the oracle's bit for bit (same
+    sequential order over the descriptor, no fused multiply-add); entries
+    differ by the device exp (<= 1 ulp) and the order of the per-pair sum:
+    1e-13 relative, entries below 1e-290 compared as underflow.
+  * the Gaussian's matrix-core form (precision "mfma"): distances from
+    |a|^2 + |b|^2 - 2 a.b carry ~1e-13 (|a|^2 + |b|^2) absolute error, and an
+    exponent error dx moves exp(x) by dx relative: 1e-12 + 1e-13 |ln K|.
   * alpha: kernel entries within 1e-13 move alpha by up to cond(K) times
     that; the bound used is 10 * cond(K + lambda I) * 1e-15 relative.
   * predictions: 1e-12 relative to sum |terms| of each prediction.
@@ -34,16 +34,15 @@ def gold():
     return np.load(os.path.join(GOLD, "atomic.npz"))
 
 
-def _close(got, ref, rel, floor=1e-290, log_scaled=False):
-    """Relative error <= rel; with log_scaled, rel * max(1, |ln ref| / 30):
-    an entry exp(x) with a last-bit change of x moves by |x| ulps."""
+def _close(got, ref, rel, floor=1e-290, per_log=0.0):
+    """Relative error <= rel + per_log * |ln ref|."""
     got, ref = np.asarray(got), np.asarray(ref)
     assert got.shape == ref.shape
     big = np.abs(ref) > floor
     assert np.all(np.abs(got[~big]) <= 1e-280)
     if big.any():
         err = np.abs(got[big] - ref[big]) / np.abs(ref[big])
-        tol = rel * (np.maximum(1.0, np.abs(np.log(np.abs(ref[big]))) / 30.0) if log_scaled else 1.0)
+        tol = rel + per_log * np.abs(np.log(np.abs(ref[big])))
         assert np.all(err <= tol), np.max(err / tol)
 
 
@@ -119,11 +118,13 @@ def test_vector_kernels_golden(gpu):
     q1, q2 = np.transpose(d["q1"], (2, 1, 0)), np.transpose(d["q2"], (2, 1, 0))
     kg = fget_vector_kernels_gaussian(q1, q2, d["n1"], d["n2"], d["sigmas_gauss"])
     kl = fget_vector_kernels_laplacian(q1, q2, d["n1"], d["n2"], d["sigmas_lap"])
-    _close(kg, d["vec_gauss"], 1e-12, log_scaled=True)
+    _close(kg, d["vec_gauss"], 1e-13)
     _close(kl, d["vec_lap"], 1e-13)
+    km = fget_vector_kernels_gaussian(q1, q2, d["n1"], d["n2"], d["sigmas_gauss"], precision="mfma")
+    _close(km, d["vec_gauss"], 1e-12, per_log=1e-13)
 
 
-@pytest.mark.parametrize("kind", [0, 1])
+@pytest.mark.parametrize("kind", [0, 1, 2])
 def test_vector_kernels_multiblock_vs_oracle(gpu, kind):
     """Hundreds of molecules (tens of 128-atom blocks per side), molecules
     of 1-29 atoms, local Coulomb matrices as descriptors."""
@@ -135,11 +136,17 @@ def test_vector_kernels_multiblock_vs_oracle(gpu, kind):
     r2 = generate_local_coulomb_matrices(ms2.charges, ms2.coords, ms2.offsets, 29)
     q1, q1o = _q_layout(r1, ms1.natoms, 29)
     q2, q2o = _q_layout(r2, ms2.natoms, 29)
-    sig = [30.0, 300.0] if kind == 0 else [300.0, 3000.0]
-    fn = fget_vector_kernels_gaussian if kind == 0 else fget_vector_kernels_laplacian
-    got = fn(q1, q2, ms1.natoms, ms2.natoms, sig)
-    ref = oracle.vector_kernels(kind, q1o, q2o, ms1.natoms, ms2.natoms, sig)
-    _close(got, ref, 1e-12 if kind == 0 else 1e-13, log_scaled=kind == 0)
+    sig = [300.0, 3000.0] if kind == 1 else [30.0, 300.0]
+    if kind == 1:
+        got = fget_vector_kernels_laplacian(q1, q2, ms1.natoms, ms2.natoms, sig)
+    else:
+        got = fget_vector_kernels_gaussian(q1, q2, ms1.natoms, ms2.natoms, sig,
+                                           precision="mfma" if kind == 2 else "fp64")
+    ref = oracle.vector_kernels(1 if kind == 1 else 0, q1o, q2o, ms1.natoms, ms2.natoms, sig)
+    if kind == 2:
+        _close(got, ref, 1e-12, per_log=1e-13)
+    else:
+        _close(got, ref, 1e-13)
 
 
 def test_vector_kernels_edges(gpu):
@@ -181,6 +188,46 @@ def test_vector_kernels_batch_invariance(gpu):
     assert np.array_equal(sub, full[:, 7:41, 60:83])
 
 
+def test_block_lists_do_not_change_results(gpu):
+    """Greedy blocks, one molecule per block and the default fixed-window
+    blocks give bitwise the same kernel; a block list wider than 128 atoms
+    is reported through status, not computed."""
+    import ctypes
+    from fml_amd import _native
+    from fml_amd.device import empty, to_device
+    from fml_amd.kernels.atomic import plan_blocks
+    from fml_amd.representations import generate_local_coulomb_matrices
+    ms1 = synth.generate(70, seed=40, nmin=1, nmax=23)
+    ms2 = synth.generate(50, seed=41, nmin=1, nmax=23)
+    x1 = generate_local_coulomb_matrices(ms1.charges, ms1.coords, ms1.offsets, 23, on_device=True)
+    x2 = generate_local_coulomb_matrices(ms2.charges, ms2.coords, ms2.offsets, 23, on_device=True)
+    o1, o2 = to_device(ms1.offsets, np.int32), to_device(ms2.offsets, np.int32)
+    sig = (ctypes.c_double * 2)(20.0, 200.0)
+    status = empty((1,), np.int32)
+
+    def run(kind, b1, b2):
+        out = empty((ms2.nmol, ms1.nmol, 2), np.float64)
+        d1 = to_device(b1, np.int32) if b1 is not None else None
+        d2 = to_device(b2, np.int32) if b2 is not None else None
+        _native.call("fmlh_vector_kernels", kind, _native.ptr(x1), x1.stride(0), _native.ptr(o1), ms1.nmol,
+                     int(x1.shape[0]), _native.ptr(x2), x2.stride(0), _native.ptr(o2), ms2.nmol,
+                     int(x2.shape[0]), int(x1.shape[1]), 23, sig, 2, _native.ptr(d1),
+                     0 if b1 is None else len(b1) - 1, _native.ptr(d2), 0 if b2 is None else len(b2) - 1,
+                     _native.ptr(out), _native.ptr(status), _native.stream_ptr())
+        return out.cpu().numpy(), int(status.item())
+
+    for kind in (0, 1, 2):
+        ref, st = run(kind, None, None)
+        assert st == 0
+        got, st = run(kind, plan_blocks(ms1.natoms), plan_blocks(ms2.natoms))
+        assert st == 0 and np.array_equal(got, ref)
+        one1 = np.arange(ms1.nmol + 1, dtype=np.int32)
+        got, st = run(kind, one1, plan_blocks(ms2.natoms))
+        assert st == 0 and np.array_equal(got, ref)
+    _, st = run(1, np.array([0, ms1.nmol], dtype=np.int32), None)   # one block, > 128 atoms
+    assert st == 1
+
+
 def test_get_atomic_kernels_gaussian_molecules(gpu):
     """kernels.py:380-406 from Molecule objects."""
     from fml_amd.kernels import get_atomic_kernels_gaussian
@@ -195,7 +242,7 @@ def test_get_atomic_kernels_gaussian_molecules(gpu):
     n2 = np.array([m.natoms for m in mols[4:]])
     _, q1 = _q_layout(np.concatenate([m.local_coulomb_matrix for m in mols[:6]]), n1, 23)
     _, q2 = _q_layout(np.concatenate([m.local_coulomb_matrix for m in mols[4:]]), n2, 23)
-    _close(K, oracle.vector_kernels(0, q1, q2, n1, n2, [100.0, 1000.0]), 1e-12, log_scaled=True)
+    _close(K, oracle.vector_kernels(0, q1, q2, n1, n2, [100.0, 1000.0]), 1e-13)
 
 
 # ------------------------------------------------------------ alpha / prediction

@@ -95,3 +95,20 @@ def test_describe_argument_checks():
     for kw in ({"amax": 0}, {"amax": 65}, {"cut": 0.0}, {"cut": -1.0}):
         with pytest.raises(ValueError):
             generate_arad_descriptors(z, r, off, **kw)
+
+
+def test_plan_blocks_greedy():
+    """Molecules in order, whole, at most 128 atoms per block, and a block
+    closes only when the next molecule would not fit."""
+    from fml_amd.kernels.atomic import plan_blocks
+    rng = np.random.default_rng(3)
+    for trial in range(20):
+        counts = rng.integers(0 if trial % 3 == 0 else 1, 65, rng.integers(1, 400))
+        b = plan_blocks(counts)
+        off = np.concatenate([[0], np.cumsum(counts)])
+        assert b[0] == 0 and b[-1] == len(counts) and np.all(np.diff(b) >= 1)
+        sizes = off[b[1:]] - off[b[:-1]]
+        assert np.all(sizes <= 128)
+        for k in range(len(b) - 2):  # every closed block could not take the next molecule
+            assert sizes[k] + counts[b[k + 1]] > 128
+    assert list(plan_blocks(np.zeros(5, dtype=np.int64))) == [0, 5]
