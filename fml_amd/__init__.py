@@ -13,8 +13,9 @@ include/fmlh.h) called through ctypes; there is no CPU fallback.
 """
 from .data import NUCLEAR_CHARGE
 from .arad import ARAD, generate_arad_descriptors
+from .aras import ARAS
 from .fml import Molecule
 from . import kernels, math, representations
 
 __version__ = "0.1.0"
-__all__ = ["Molecule", "ARAD", "generate_arad_descriptors", "NUCLEAR_CHARGE", "kernels", "math", "representations"]
+__all__ = ["Molecule", "ARAD", "ARAS", "generate_arad_descriptors", "NUCLEAR_CHARGE", "kernels", "math", "representations"]

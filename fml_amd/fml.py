@@ -4,6 +4,7 @@ from __future__ import annotations
 import numpy as np
 
 from .arad import ARAD
+from .aras import ARAS
 from .data import NUCLEAR_CHARGE
 from .representations import (fgenerate_atomic_coulomb_matrix, fgenerate_coulomb_matrix,
                               fgenerate_local_coulomb_matrix, fgenerate_unsorted_coulomb_matrix)
@@ -51,6 +52,14 @@ class Molecule:
                                                     np.array(self.nuclear_charges))
         assert (self.arad_descriptor).shape[0] == size, "ERROR: Check ARAD descriptor size!"
         assert (self.arad_descriptor).shape[2] == size, "ERROR: Check ARAD descriptor size!"
+
+    def generate_aras_descriptor(self, size=23):
+        """ARAS descriptor, fml.py:100-106."""
+        aras_object = ARAS(maxMolSize=size, maxAts=size)
+        self.aras_descriptor = aras_object.describe(np.array(self.coordinates),
+                                                    np.array(self.nuclear_charges))
+        assert (self.aras_descriptor).shape[0] == size, "ERROR: Check ARAS descriptor size!"
+        assert (self.aras_descriptor).shape[2] == size, "ERROR: Check ARAS descriptor size!"
 
     def read_xyz(self, filename):
         """XYZ reader, fml.py:109-132: atom count from line 1, atom lines from

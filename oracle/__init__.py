@@ -89,6 +89,10 @@ def lib():
         _lib.oracle_alpha_kernel_from_distance.argtypes = [_c_dbl_p, _c_int_p, I, D, _c_dbl_p]
         _lib.oracle_prediction_from_distance.argtypes = [_c_dbl_p, I, I, _c_int_p, I, _c_dbl_p, D,
                                                          _c_dbl_p]
+        _lib.oracle_kernels_aras.argtypes = [I, _c_dbl_p, _c_dbl_p, _c_int_p, _c_int_p, _c_int_p, _c_int_p,
+                                             _c_dbl_p, I, I, I, I, D, D, D, D, D, I, _c_dbl_p, I, D,
+                                             _c_dbl_p]
+        _lib.oracle_kernels_aras.restype = None
         for f in ("oracle_vector_kernels", "oracle_alpha_kernel", "oracle_prediction",
                   "oracle_alpha_kernel_from_distance", "oracle_prediction_from_distance"):
             getattr(_lib, f).restype = None
@@ -129,6 +133,10 @@ def ref(variant: str = "libfml_ref"):
         r.ref_cho_invert.argtypes = [_c_dbl_p, I]
         r.ref_alpha_arad.argtypes = [_c_dbl_p, _c_dbl_p, _c_int_p, _c_dbl_p, D, D, I, I, I, _c_dbl_p]
         r.ref_coulomb_form.argtypes = [I, _c_dbl_p, _c_dbl_p, I, I, _c_dbl_p]
+        r.ref_kernels_aras.argtypes = [_c_dbl_p, _c_dbl_p, _c_int_p, _c_int_p, _c_int_p, _c_int_p, _c_dbl_p,
+                                       I, I, I, I, D, D, D, D, D, I, _c_dbl_p, I, D, _c_dbl_p]
+        r.ref_symmetric_kernels_aras.argtypes = [_c_dbl_p, _c_int_p, _c_int_p, _c_dbl_p, I, I, I, D, D, D,
+                                                 D, D, I, _c_dbl_p, I, D, _c_dbl_p]
         r.ref_vector_kernels.argtypes = [I, _c_dbl_p, _c_dbl_p, _c_int_p, _c_int_p, _c_dbl_p,
                                          I, I, I, I, I, I, _c_dbl_p]
         r.ref_get_alpha.argtypes = [_c_dbl_p, _c_int_p, _c_dbl_p, D, D, I, I, I, _c_dbl_p]
@@ -608,6 +616,80 @@ def ref_get_prediction_from_distance(D2, n2, alpha, gamma, variant="libfml_ref")
                                               float(gamma), Df.shape[0], Df.shape[1], n2.shape[0],
                                               _dp(y))
     return y
+
+
+# ---------------------------------------------------------------------------
+# ARAS kernels (faras_kernels.f90:150-723)
+# ---------------------------------------------------------------------------
+
+def aras_pd(emax=103, r_width=1.0, c_width=0.5):
+    """kernels.py gen_pd: (emax, emax) element similarity
+    (rw^2 cw^2) / ((rw^2 + drow^2)(cw^2 + dcol^2)) over the PTP table."""
+    from .pyref import ptp_types
+    pd = np.zeros((emax, emax))
+    for i in range(emax):
+        ri, ci = ptp_types(i + 1)
+        for j in range(emax):
+            rj, cj = ptp_types(j + 1)
+            pd[i, j] = (r_width ** 2 * c_width ** 2) / ((r_width ** 2 + (ri - rj) ** 2) *
+                                                       (c_width ** 2 + (ci - cj) ** 2))
+    return pd
+
+
+def aras_neighbors(X, n, cut):
+    """kernels.py:460-472: per atom, the number of descriptor distances < cut."""
+    X = np.asarray(X)
+    nn = np.zeros((X.shape[0], X.shape[1]), dtype=np.int32)
+    for a in range(X.shape[0]):
+        for i in range(int(n[a])):
+            nn[a, i] = int(np.sum(X[a, i, 0] < cut))
+    return nn
+
+
+def _aras_args(X1, X2, n1, n2, sigmas, t_width, d_width, cut, r_width, order, c_width, scale):
+    X1 = _c(X1)
+    X2 = X1 if X2 is None else _c(X2)
+    n1 = _c(n1, np.int32)
+    n2 = n1 if n2 is None else _c(n2, np.int32)
+    nn1, nn2 = aras_neighbors(X1, n1, cut), aras_neighbors(X2, n2, cut)
+    pd = np.asfortranarray(aras_pd(103, r_width, c_width))
+    return X1, X2, n1, n2, nn1, nn2, _c(sigmas), pd
+
+
+def kernels_aras(X1, X2, n1, n2, sigmas, t_width=np.pi, d_width=0.2, cut=5.0, r_width=1.0, order=1,
+                 c_width=0.5, scale=0.1):
+    """fget_kernels_aras (X2 given) or fget_symmetric_kernels_aras (X2 None)
+    on descriptors (nm, amax, 3 + amax, amax) -> (nsig, nm1, nm2)."""
+    sym = X2 is None
+    X1, X2, n1, n2, nn1, nn2, sig, pd = _aras_args(X1, X2, n1, n2, sigmas, t_width, d_width, cut,
+                                                   r_width, order, c_width, scale)
+    nm1, nm2, amax = X1.shape[0], X2.shape[0], X1.shape[1]
+    out = np.zeros((sig.shape[0], nm1, nm2), order="F")
+    lib().oracle_kernels_aras(int(sym), _dp(X1), _dp(X2), _ip(n1), _ip(n2), _ip(nn1), _ip(nn2), _dp(sig),
+                              nm1, nm2, sig.shape[0], amax, t_width, r_width, c_width, d_width, cut,
+                              order, pd.ctypes.data_as(_c_dbl_p), 103, scale, out.ctypes.data_as(_c_dbl_p))
+    return out
+
+
+def ref_kernels_aras(X1, X2, n1, n2, sigmas, t_width=np.pi, d_width=0.2, cut=5.0, r_width=1.0, order=1,
+                     c_width=0.5, scale=0.1, variant="libfml_ref"):
+    sym = X2 is None
+    X1, X2, n1, n2, nn1, nn2, sig, pd = _aras_args(X1, X2, n1, n2, sigmas, t_width, d_width, cut,
+                                                   r_width, order, c_width, scale)
+    nm1, nm2, amax = X1.shape[0], X2.shape[0], X1.shape[1]
+    out = np.zeros((sig.shape[0], nm1, nm2), order="F")
+    f1, f2 = np.asfortranarray(X1), np.asfortranarray(X2)
+    nf1, nf2 = np.asfortranarray(nn1), np.asfortranarray(nn2)
+    P = pd.ctypes.data_as(_c_dbl_p)
+    if sym:
+        ref(variant).ref_symmetric_kernels_aras(_dp(f1), _ip(n1), _ip(nf1), _dp(sig), nm1, sig.shape[0],
+                                                amax, t_width, r_width, c_width, d_width, cut, order, P,
+                                                103, scale, out.ctypes.data_as(_c_dbl_p))
+    else:
+        ref(variant).ref_kernels_aras(_dp(f1), _dp(f2), _ip(n1), _ip(n2), _ip(nf1), _ip(nf2), _dp(sig),
+                                      nm1, nm2, sig.shape[0], amax, t_width, r_width, c_width, d_width,
+                                      cut, order, P, 103, scale, out.ctypes.data_as(_c_dbl_p))
+    return out
 
 
 from .pyref import (arad_describe, arad_distl2_py, ptp_types, NUCLEAR_CHARGE_ORACLE)  # noqa: E402,F401

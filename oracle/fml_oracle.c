@@ -958,3 +958,276 @@ EXPORT void oracle_prediction_from_distance(const double *d2, int na, int na2,
   }
   free(csa);
 }
+
+/* ------------------------------------------------------------------ */
+/* ARAS kernels (fml/kernels/faras_kernels.f90)                        */
+/* ------------------------------------------------------------------ */
+
+typedef struct {
+  double t_width, r_width, c_width, d_width, cut, angular_scale;
+  int order;
+} aras_par;
+
+/* x[nm][amax][3 + amax][amax]: channel 0 distances, 1 nuclear charges,
+ * 3 + m angles to neighbour m (aras.py:123-129). */
+#define ARAS_X(x, amax, a, i, c, m) \
+  (x)[((((size_t)(a) * (amax) + (i)) * (size_t)(3 + (amax)) + (c)) * (amax)) + (m)]
+
+/* Per-atom tables of one set (faras_kernels.f90:258-361 / :540-620):
+ * ksi[a][i][m] (radial, neighbours m >= 1 inside the cut: 1/d^6 q0 qm),
+ * cosp/sinp[a][i][o][j][p] (p = nuclear charge 0..pmax): sum over
+ * neighbours k != j (both >= 1, < nneigh) with charge p of
+ * (cos(o' th) - cos((th + pi) o')) ksi3(j, k), o' = o + 1, th the angle
+ * between j and k, and ksi3 the three-body factor
+ * q0 qj qk (1 + cos(th) cos_i cos_j) / (di dj dij)^3 with dij, cos_i, cos_j
+ * from the law of cosines (:288-306). */
+typedef struct {
+  int nm, amax, order, pmax;
+  double *ksi;
+  double *cosp, *sinp;
+} aras_tables;
+
+static double powi6(double x) { return x * x * x * x * x * x; }
+
+static void aras_build(const double *x, const int *n, const int *nn, int nm, int amax,
+                       const aras_par *pr, int pmax, aras_tables *t) {
+  const double pi = 3.141592653589793;
+  t->nm = nm;
+  t->amax = amax;
+  t->order = pr->order;
+  t->pmax = pmax;
+  const size_t na = (size_t)nm * amax;
+  t->ksi = (double *)calloc(na * amax + 1, sizeof(double));
+  const size_t per = (size_t)pr->order * amax * (pmax + 1);
+  t->cosp = (double *)calloc(na * per + 1, sizeof(double));
+  t->sinp = (double *)calloc(na * per + 1, sizeof(double));
+  double *k3 = (double *)calloc((size_t)amax * amax + 1, sizeof(double));
+  for (int a = 0; a < nm; ++a)
+    for (int i = 0; i < n[a]; ++i) {
+      const int nb = nn[a * amax + i];
+      const double q0 = ARAS_X(x, amax, a, i, 1, 0);
+      double *ks = t->ksi + ((size_t)a * amax + i) * amax;
+      for (int m = 1; m < nb; ++m) {
+        const double dm = ARAS_X(x, amax, a, i, 0, m);
+        if (dm < pr->cut) ks[m] = 1.0 / powi6(dm) * q0 * ARAS_X(x, amax, a, i, 1, m);
+      }
+      for (int m1 = 1; m1 < nb; ++m1)
+        for (int m2 = 1; m2 < nb; ++m2) {
+          if (m1 == m2) continue;
+          const double di = ARAS_X(x, amax, a, i, 0, m1);
+          const double dj = ARAS_X(x, amax, a, i, 0, m2);
+          const double ang = ARAS_X(x, amax, a, i, 3 + m1, m2);
+          const double dij = sqrt(di * di + dj * dj - 2.0 * di * dj * cos(ang));
+          const double cosi = (dj * dj + dij * dij - di * di) / (2.0 * dj * dij);
+          const double cosj = (di * di + dij * dij - dj * dj) / (2.0 * di * dij);
+          const double p3 = di * dj * dij;
+          k3[m1 * amax + m2] = q0 * ARAS_X(x, amax, a, i, 1, m1) * ARAS_X(x, amax, a, i, 1, m2) *
+                               (1.0 + cos(ang) * cosi * cosj) / (p3 * p3 * p3);
+        }
+      double *cp = t->cosp + ((size_t)a * amax + i) * per;
+      double *sp = t->sinp + ((size_t)a * amax + i) * per;
+      for (int o = 0; o < pr->order; ++o) {
+        const double mo = (double)(o + 1);
+        for (int j = 1; j < nb; ++j)
+          for (int k = 1; k < nb; ++k) {
+            if (j == k) continue;
+            const int p = (int)ARAS_X(x, amax, a, i, 1, k);
+            const double th = ARAS_X(x, amax, a, i, 3 + j, k);
+            const size_t at = ((size_t)o * amax + j) * (pmax + 1) + p;
+            cp[at] = cp[at] + (cos(mo * th) - cos((th + pi) * mo)) * k3[j * amax + k];
+            sp[at] = sp[at] + (sin(mo * th) - sin((th + pi) * mo)) * k3[j * amax + k];
+          }
+      }
+    }
+  free(k3);
+}
+
+static void aras_free(aras_tables *t) {
+  free(t->ksi);
+  free(t->cosp);
+  free(t->sinp);
+}
+
+/* faras_kernels.f90:198-207: the angular normalisation. */
+static double aras_ang_norm2(double t_width) {
+  const double pi = 3.141592653589793;
+  double s = 0.0;
+  for (int n = -10000; n <= 10000; ++n) {
+    const double tn = t_width * n;
+    s = s + exp(-(tn * tn)) * (2.0 - 2.0 * cos(n * pi));
+  }
+  return sqrt(s * pi) * 2.0;
+}
+
+/* faras_kernels.f90:7-147 (aras_utils::atomic_distl2) for atom i1 of
+ * molecule a1 (set 1) against atom i2 of molecule a2 (set 2). */
+static double aras_distl2(const double *x1, const aras_tables *t1, int a1, int i1, int nb1,
+                          const double *x2, const aras_tables *t2, int a2, int i2, int nb2,
+                          const aras_par *pr, const double *pd, int emax, const double *s) {
+  const int amax1 = t1->amax, amax2 = t2->amax;
+  int pmax1 = 0, pmax2 = 0;
+  for (int m = 0; m < nb1; ++m) {
+    const int q = (int)ARAS_X(x1, amax1, a1, i1, 1, m);
+    if (q > pmax1) pmax1 = q;
+  }
+  for (int m = 0; m < nb2; ++m) {
+    const int q = (int)ARAS_X(x2, amax2, a2, i2, 1, m);
+    if (q > pmax2) pmax2 = q;
+  }
+  char *use1 = (char *)calloc((size_t)pmax1 + 1, 1);
+  char *use2 = (char *)calloc((size_t)pmax2 + 1, 1);
+  for (int m = 0; m < nb1; ++m) use1[(int)ARAS_X(x1, amax1, a1, i1, 1, m)] = 1;
+  for (int m = 0; m < nb2; ++m) use2[(int)ARAS_X(x2, amax2, a2, i2, 1, m)] = 1;
+  const double inv_width = -1.0 / (4.0 * (pr->d_width * pr->d_width));
+  const double g8 = 8.0 * pr->d_width;
+  const double maxg2 = g8 * g8;
+#define PD(p, q) pd[((size_t)(q) - 1) * emax + ((p) - 1)] /* Fortran pd(p, q) */
+  const int q01 = (int)ARAS_X(x1, amax1, a1, i1, 1, 0), q02 = (int)ARAS_X(x2, amax2, a2, i2, 1, 0);
+  double aad = PD(q01, q02);
+  const double *ks1 = t1->ksi + ((size_t)a1 * amax1 + i1) * amax1;
+  const double *ks2 = t2->ksi + ((size_t)a2 * amax2 + i2) * amax2;
+  const size_t per1 = (size_t)t1->order * amax1 * (t1->pmax + 1);
+  const size_t per2 = (size_t)t2->order * amax2 * (t2->pmax + 1);
+  const double *c1 = t1->cosp + ((size_t)a1 * amax1 + i1) * per1;
+  const double *s1 = t1->sinp + ((size_t)a1 * amax1 + i1) * per1;
+  const double *c2 = t2->cosp + ((size_t)a2 * amax2 + i2) * per2;
+  const double *s2 = t2->sinp + ((size_t)a2 * amax2 + i2) * per2;
+  for (int m1 = 1; m1 < nb1; ++m1) {
+    const double d1 = ARAS_X(x1, amax1, a1, i1, 0, m1);
+    if (d1 > pr->cut) break;
+    for (int m2 = 1; m2 < nb2; ++m2) {
+      const double d2 = ARAS_X(x2, amax2, a2, i2, 0, m2);
+      if (d2 > pr->cut) break;
+      const double dd = d2 - d1;
+      const double r2 = dd * dd;
+      if (!(r2 < maxg2)) continue;
+      const double d = exp(r2 * inv_width) *
+                       PD((int)ARAS_X(x1, amax1, a1, i1, 1, m1), (int)ARAS_X(x2, amax2, a2, i2, 1, m2));
+      double angular = 0.0 * 0.0;
+      for (int o = 0; o < pr->order; ++o) {
+        double temp = 0.0;
+        for (int p1 = 1; p1 <= pmax1; ++p1) {
+          if (!use1[p1]) continue;
+          const double cc = c1[((size_t)o * amax1 + m1) * (t1->pmax + 1) + p1];
+          const double ss = s1[((size_t)o * amax1 + m1) * (t1->pmax + 1) + p1];
+          for (int p2 = 1; p2 <= pmax2; ++p2) {
+            if (!use2[p2]) continue;
+            temp = temp + (cc * c2[((size_t)o * amax2 + m2) * (t2->pmax + 1) + p2] +
+                           ss * s2[((size_t)o * amax2 + m2) * (t2->pmax + 1) + p2]) * PD(p2, p1);
+          }
+        }
+        angular = angular + temp * s[o];
+      }
+      aad = aad + d * (ks1[m1] * ks2[m2] + angular * pr->angular_scale);
+    }
+  }
+#undef PD
+  free(use1);
+  free(use2);
+  return aad;
+}
+
+static int aras_pmax(const double *x, const int *n, const int *nn, int nm, int amax) {
+  int p = 0;
+  for (int a = 0; a < nm; ++a)
+    for (int i = 0; i < n[a]; ++i)
+      for (int m = 0; m < nn[a * amax + i]; ++m) {
+        const int q = (int)ARAS_X(x, amax, a, i, 1, m);
+        if (q > p) p = q;
+      }
+  return p;
+}
+
+/*
+ * fml/kernels/faras_kernels.f90:150-483 (fget_kernels_aras; sym = 0) and
+ * :486-723 (fget_symmetric_kernels_aras; sym = 1, x2 = x1, each pair i >= j
+ * computed with molecule i in the first role and written to both halves).
+ * nn[a][i]: neighbours of atom i inside the cut, self included (the
+ * Python wrapper's neighbors arrays, kernels.py:460-472).  pd: Fortran
+ * (emax, emax) element similarity (kernels.py gen_pd).  The element axis of
+ * the projection tables is sized by the largest charge any atom sees (the
+ * reference sizes it from each molecule's first atom and writes past it
+ * when another atom sees a heavier element, :210-215).  Expression order
+ * as the reference; l2 = self1 + self2 - 2 l2 pd(q1, q2), zeroed below
+ * 5e-12 (:452-455); kernels(s, i, j) = sum over the (ni, nj) atom block,
+ * column-major, of exp(l2 * (-1/sigma^2)).
+ * out: Fortran (nsig, nm1, nm2).
+ * Parity: flang build of the reference (oracle/_ref) and closed forms.
+ */
+EXPORT void oracle_kernels_aras(int sym, const double *x1, const double *x2, const int *n1,
+                                const int *n2, const int *nn1, const int *nn2,
+                                const double *sigmas, int nm1, int nm2, int nsig, int amax,
+                                double t_width, double r_width, double c_width, double d_width,
+                                double cut, int order, const double *pd, int emax,
+                                double angular_scale, double *out) {
+  aras_par pr = {t_width, r_width, c_width, d_width, cut, angular_scale, order};
+  if (sym) {
+    x2 = x1;
+    n2 = n1;
+    nn2 = nn1;
+    nm2 = nm1;
+  }
+  const double pi = 3.141592653589793;
+  const double ang_norm2 = aras_ang_norm2(t_width);
+  const double g1 = sqrt(2.0 * pi) / ang_norm2;
+  double *s = (double *)malloc(sizeof(double) * (size_t)(order > 0 ? order : 1));
+  for (int o = 0; o < order; ++o) {
+    const double tm = t_width * (o + 1);
+    s[o] = g1 * exp(-(tm * tm) / 2.0);
+  }
+  int pmax = aras_pmax(x1, n1, nn1, nm1, amax);
+  const int pm2 = aras_pmax(x2, n2, nn2, nm2, amax);
+  if (pm2 > pmax) pmax = pm2;
+  aras_tables t1, t2;
+  aras_build(x1, n1, nn1, nm1, amax, &pr, pmax, &t1);
+  if (!sym) aras_build(x2, n2, nn2, nm2, amax, &pr, pmax, &t2);
+  const aras_tables *tb2 = sym ? &t1 : &t2;
+  double *self1 = (double *)calloc((size_t)nm1 * amax + 1, sizeof(double));
+  double *self2 = sym ? self1 : (double *)calloc((size_t)nm2 * amax + 1, sizeof(double));
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int a = 0; a < nm1; ++a)
+    for (int i = 0; i < n1[a]; ++i)
+      self1[a * amax + i] = aras_distl2(x1, &t1, a, i, nn1[a * amax + i], x1, &t1, a, i,
+                                        nn1[a * amax + i], &pr, pd, emax, s);
+  if (!sym) {
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int a = 0; a < nm2; ++a)
+      for (int i = 0; i < n2[a]; ++i)
+        self2[a * amax + i] = aras_distl2(x2, tb2, a, i, nn2[a * amax + i], x2, tb2, a, i,
+                                          nn2[a * amax + i], &pr, pd, emax, s);
+  }
+  double *inv = (double *)malloc(sizeof(double) * (size_t)(nsig > 0 ? nsig : 1));
+  for (int k = 0; k < nsig; ++k) inv[k] = -1.0 / (sigmas[k] * sigmas[k]);
+#define PD(p, q) pd[((size_t)(q) - 1) * emax + ((p) - 1)]
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int j = 0; j < nm2; ++j) {
+    double *ad = (double *)calloc((size_t)amax * amax + 1, sizeof(double));
+    const int nj = n2[j];
+    for (int i = sym ? j : 0; i < nm1; ++i) {
+      const int ni = n1[i];
+      for (int i1 = 0; i1 < ni; ++i1)
+        for (int j1 = 0; j1 < nj; ++j1) {
+          double l2 = aras_distl2(x1, &t1, i, i1, nn1[i * amax + i1], x2, tb2, j, j1,
+                                  nn2[j * amax + j1], &pr, pd, emax, s);
+          l2 = self1[i * amax + i1] + self2[j * amax + j1] -
+               2.0 * l2 * PD((int)ARAS_X(x1, amax, i, i1, 1, 0), (int)ARAS_X(x2, amax, j, j1, 1, 0));
+          if (fabs(l2) < 5.0e-12) l2 = 0.0;
+          ad[j1 * ni + i1] = l2;
+        }
+      for (int k = 0; k < nsig; ++k) {
+        double acc = 0.0;
+        for (int t = 0; t < ni * nj; ++t) acc = acc + exp(ad[t] * inv[k]);
+        out[k + (size_t)nsig * (i + (size_t)nm1 * j)] = acc;
+        if (sym) out[k + (size_t)nsig * (j + (size_t)nm1 * i)] = acc;
+      }
+    }
+    free(ad);
+  }
+#undef PD
+  free(inv);
+  if (!sym) free(self2);
+  free(self1);
+  aras_free(&t1);
+  if (!sym) aras_free(&t2);
+  free(s);
+}

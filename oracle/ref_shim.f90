@@ -234,6 +234,48 @@ module fml_ref_shim
       double precision, intent(in) :: dgamma
       double precision, dimension(:), intent(inout) :: Y
     end subroutine
+    ! fml/kernels/faras_kernels.f90:150-175
+    subroutine fget_kernels_aras(x1, x2, n1, n2, nneigh1, nneigh2, sigmas, nm1, nm2, nsigmas, &
+        & t_width, r_width, c_width, d_width, cut_distance, order, pd, angular_scale, kernels)
+      double precision, dimension(:,:,:,:), intent(in) :: x1
+      double precision, dimension(:,:,:,:), intent(in) :: x2
+      integer, dimension(:), intent(in) :: n1
+      integer, dimension(:), intent(in) :: n2
+      integer, dimension(:,:), intent(in) :: nneigh1
+      integer, dimension(:,:), intent(in) :: nneigh2
+      double precision, dimension(:), intent(in) :: sigmas
+      integer, intent(in) :: nm1
+      integer, intent(in) :: nm2
+      integer, intent(in) :: nsigmas
+      double precision, intent(in) :: t_width
+      double precision, intent(in) :: r_width
+      double precision, intent(in) :: c_width
+      double precision, intent(in) :: d_width
+      double precision, intent(in) :: cut_distance
+      integer, intent(in) :: order
+      double precision, dimension(:,:), intent(in) :: pd
+      double precision, intent(in) :: angular_scale
+      double precision, dimension(nsigmas,nm1,nm2), intent(out) :: kernels
+    end subroutine
+    ! fml/kernels/faras_kernels.f90:486-508
+    subroutine fget_symmetric_kernels_aras(x1, n1, nneigh1, sigmas, nm1, nsigmas, &
+        & t_width, r_width, c_width, d_width, cut_distance, order, pd, angular_scale, kernels)
+      double precision, dimension(:,:,:,:), intent(in) :: x1
+      integer, dimension(:), intent(in) :: n1
+      integer, dimension(:,:), intent(in) :: nneigh1
+      double precision, dimension(:), intent(in) :: sigmas
+      integer, intent(in) :: nm1
+      integer, intent(in) :: nsigmas
+      double precision, intent(in) :: t_width
+      double precision, intent(in) :: r_width
+      double precision, intent(in) :: c_width
+      double precision, intent(in) :: d_width
+      double precision, intent(in) :: cut_distance
+      integer, intent(in) :: order
+      double precision, dimension(:,:), intent(in) :: pd
+      double precision, intent(in) :: angular_scale
+      double precision, dimension(nsigmas,nm1,nm1), intent(out) :: kernels
+    end subroutine
   end interface
 
 contains
@@ -436,6 +478,37 @@ contains
     real(c_double), value :: dgamma
     real(c_double), intent(inout) :: y(nm2)
     call fget_prediction_from_distance(d2, n2, alpha, dgamma, y)
+  end subroutine
+
+  ! x: (nm, amax, 3 + amax, amax) Fortran (the numpy C array (amax, 3+amax, amax, nm) reversed)
+  subroutine ref_kernels_aras(x1, x2, n1, n2, nn1, nn2, sigmas, nm1, nm2, nsig, amax, &
+      & t_width, r_width, c_width, d_width, cut, order, pd, emax, angular_scale, kernels) &
+      & bind(C, name="ref_kernels_aras")
+    integer(c_int), value :: nm1, nm2, nsig, amax, order, emax
+    real(c_double), intent(in) :: x1(nm1, amax, 3 + amax, amax), x2(nm2, amax, 3 + amax, amax)
+    integer(c_int), intent(in) :: n1(nm1), n2(nm2), nn1(nm1, amax), nn2(nm2, amax)
+    real(c_double), intent(in) :: sigmas(nsig), pd(emax, emax)
+    real(c_double), value :: t_width, r_width, c_width, d_width, cut, angular_scale
+    real(c_double), intent(out) :: kernels(nsig, nm1, nm2)
+    integer :: ordi
+    ordi = order
+    call fget_kernels_aras(x1, x2, n1, n2, nn1, nn2, sigmas, nm1, nm2, nsig, &
+        & t_width, r_width, c_width, d_width, cut, ordi, pd, angular_scale, kernels)
+  end subroutine
+
+  subroutine ref_symmetric_kernels_aras(x1, n1, nn1, sigmas, nm1, nsig, amax, &
+      & t_width, r_width, c_width, d_width, cut, order, pd, emax, angular_scale, kernels) &
+      & bind(C, name="ref_symmetric_kernels_aras")
+    integer(c_int), value :: nm1, nsig, amax, order, emax
+    real(c_double), intent(in) :: x1(nm1, amax, 3 + amax, amax)
+    integer(c_int), intent(in) :: n1(nm1), nn1(nm1, amax)
+    real(c_double), intent(in) :: sigmas(nsig), pd(emax, emax)
+    real(c_double), value :: t_width, r_width, c_width, d_width, cut, angular_scale
+    real(c_double), intent(out) :: kernels(nsig, nm1, nm1)
+    integer :: ordi
+    ordi = order
+    call fget_symmetric_kernels_aras(x1, n1, nn1, sigmas, nm1, nsig, &
+        & t_width, r_width, c_width, d_width, cut, ordi, pd, angular_scale, kernels)
   end subroutine
 
 end module fml_ref_shim

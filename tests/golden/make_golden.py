@@ -6,6 +6,7 @@ built (make -C oracle ref):
     python tests/golden/make_golden.py            # every fixture
     python tests/golden/make_golden.py inverse    # inverse.npz only
     python tests/golden/make_golden.py atomic     # atomic.npz only
+    python tests/golden/make_golden.py aras       # aras.npz only
 
 Inputs: the reference's test molecules tests/xyz/01..10.xyz (copied to
 tests/golden/xyz) and small seeded synthetic sets.  Outputs go next to this
@@ -136,6 +137,24 @@ def atomic_fixtures():
                         y1=y1, alpha=alpha, pred=pred, dmat=dmat, alpha_d=alpha_d)
 
 
+def aras_fixtures():
+    """aras.npz: ARAS descriptors of the xyz molecules (size 11) and the
+    reference's fget_kernels_aras (molecules 0-4 against 5-9) and
+    fget_symmetric_kernels_aras (all ten), default widths and order 1, and
+    the global kernel again with order 2, t_width 2, d_width 0.3, cut 4."""
+    from fml_amd.aras import ARAS
+    mols = xyz_molecules()
+    ar = ARAS(maxMolSize=11, maxAts=11)
+    X = np.array([ar.describe(m.coordinates, m.nuclear_charges) for m in mols])
+    n = np.array([len(m.nuclear_charges) for m in mols], dtype=np.int32)
+    sig = np.array([0.5, 2.0, 10.0])
+    np.savez_compressed(os.path.join(HERE, "aras.npz"), X=X, n=n, sigmas=sig,
+                        kernels=oracle.ref_kernels_aras(X[:5], X[5:], n[:5], n[5:], sig),
+                        symmetric=oracle.ref_kernels_aras(X, None, n, None, sig),
+                        kernels_o2=oracle.ref_kernels_aras(X[:5], X[5:], n[:5], n[5:], sig, t_width=2.0,
+                                                           d_width=0.3, cut=4.0, order=2))
+
+
 if __name__ == "__main__":
     if sys.argv[1:] == ["inverse"]:
         assert oracle.ref_available(), "build oracle/_ref first (make -C oracle ref)"
@@ -143,6 +162,10 @@ if __name__ == "__main__":
     elif sys.argv[1:] == ["atomic"]:
         assert oracle.ref_available(), "build oracle/_ref first (make -C oracle ref)"
         atomic_fixtures()
+    elif sys.argv[1:] == ["aras"]:
+        assert oracle.ref_available(), "build oracle/_ref first (make -C oracle ref)"
+        aras_fixtures()
     else:
         main()
         atomic_fixtures()
+        aras_fixtures()

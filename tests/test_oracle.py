@@ -503,3 +503,67 @@ def test_atomic_path_vs_reference_build():
     D = np.sqrt(((P[:, None] - P[None]) ** 2).sum(-1))
     a, info = oracle.get_alpha_from_distance(D, n1, y, 1.0, 1e-2)
     assert info == 0 and np.array_equal(a, oracle.ref_get_alpha_from_distance(D, n1, y, 1.0, 1e-2))
+
+
+# ------------------------------------------------------------------- ARAS
+
+def test_aras_golden_bitwise():
+    d = g("aras.npz")
+    X, n, sig = d["X"], d["n"], d["sigmas"]
+    from fml_amd.aras import ARAS
+    ar = ARAS(maxMolSize=11, maxAts=11)
+    Xh = np.array([ar.describe(m.coordinates, m.nuclear_charges) for m in xyz_mols()])
+    assert np.array_equal(Xh, X)
+    assert np.array_equal(oracle.kernels_aras(X[:5], X[5:], n[:5], n[5:], sig), d["kernels"])
+    assert np.array_equal(oracle.kernels_aras(X, None, n, None, sig), d["symmetric"])
+    assert np.array_equal(oracle.kernels_aras(X[:5], X[5:], n[:5], n[5:], sig, t_width=2.0, d_width=0.3,
+                                              cut=4.0, order=2), d["kernels_o2"])
+
+
+def test_aras_closed_forms():
+    """One-atom molecules: l2 = pd(p,p) + pd(q,q) - 2 pd(p,q)^2, so K = 1 for
+    equal elements and exp(-(2 - 2 pd^2)/sigma^2) otherwise.  A diatomic
+    A-B against itself: each atom's self distance is 1 + ksi^2 with
+    ksi = qA qB / d^6 (no angular term without two neighbours), the A-B
+    distance pd(A,B) (1 + ksi^2), so the two cross pairs have
+    l2 = 2 (1 + ksi^2)(1 - pd^2) and K = 2 + 2 exp(-l2 / sigma^2)."""
+    from fml_amd.aras import ARAS
+    ar = ARAS(maxMolSize=2, maxAts=2)
+    Zs = [1, 6, 8]
+    X = np.array([ar.describe(np.zeros((1, 3)), [z]) for z in Zs])
+    n = np.ones(3, dtype=np.int32)
+    pd = oracle.aras_pd()
+    K = oracle.kernels_aras(X, None, n, None, [1.5])[0]
+    for i, zi in enumerate(Zs):
+        for j, zj in enumerate(Zs):
+            p = pd[zi - 1, zj - 1]
+            assert K[i, j] == pytest.approx(math.exp(-(2.0 - 2.0 * p * p) / 2.25), rel=1e-14)
+    co = np.array([[0.0, 0.0, 0.0], [1.1, 0.0, 0.0]])
+    for za, zb, sigma in ((6, 8, 100.0), (1, 1, 0.7)):
+        Xd = ar.describe(co, [za, zb])[None]
+        Kd = oracle.kernels_aras(Xd, None, np.array([2], dtype=np.int32), None, [sigma])[0, 0, 0]
+        ksi = za * zb / 1.1 ** 6
+        p = pd[za - 1, zb - 1]
+        expect = 2.0 + 2.0 * math.exp(-2.0 * (1.0 + ksi * ksi) * (1.0 - p * p) / sigma ** 2)
+        assert Kd == pytest.approx(expect, rel=1e-12)
+    # global against itself equals the symmetric form for i >= j, bitwise
+    ms = synth.generate(6, seed=25, nmin=2, nmax=7)
+    ar = ARAS(maxMolSize=7, maxAts=7)
+    Xs = np.array([ar.describe(ms.molecule(i)[1], ms.molecule(i)[0]) for i in range(ms.nmol)])
+    g_ = oracle.kernels_aras(Xs, Xs, ms.natoms, ms.natoms, [1.0, 3.0])
+    s_ = oracle.kernels_aras(Xs, None, ms.natoms, None, [1.0, 3.0])
+    low = np.tril(np.ones((ms.nmol, ms.nmol), dtype=bool))
+    assert np.array_equal(g_[:, low], s_[:, low])
+
+
+@pytest.mark.skipif(not oracle.ref_available(), reason="reference build absent")
+def test_aras_vs_reference_build():
+    from fml_amd.aras import ARAS
+    ms = synth.generate(8, seed=26, nmin=3, nmax=9)
+    ar = ARAS(maxMolSize=9, maxAts=9, cut=4.0)
+    X = np.array([ar.describe(ms.molecule(i)[1], ms.molecule(i)[0]) for i in range(ms.nmol)])
+    kw = dict(t_width=1.5, d_width=0.25, cut=4.0, order=3, r_width=0.8, c_width=0.6, scale=0.3)
+    assert np.array_equal(oracle.kernels_aras(X[:3], X[3:], ms.natoms[:3], ms.natoms[3:], [0.8, 4.0], **kw),
+                          oracle.ref_kernels_aras(X[:3], X[3:], ms.natoms[:3], ms.natoms[3:], [0.8, 4.0], **kw))
+    assert np.array_equal(oracle.kernels_aras(X, None, ms.natoms, None, [2.0], **kw),
+                          oracle.ref_kernels_aras(X, None, ms.natoms, None, [2.0], **kw))
