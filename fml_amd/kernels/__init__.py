@@ -10,6 +10,8 @@ from .atomic import (get_atomic_kernels_gaussian, get_atomic_kernels_laplacian,
                      get_prediction, get_alpha_from_distance, get_prediction_from_distance,
                      fget_alpha, fget_prediction, fget_alpha_from_distance,
                      fget_prediction_from_distance)
+from .aras_kernels import (get_atomic_kernels_aras, get_atomic_symmetric_kernels_aras,
+                           fget_kernels_aras, fget_symmetric_kernels_aras, gen_pd, periodic_distance)
 
 __all__ = ["gaussian_kernel", "laplacian_kernel", "get_atomic_kernels_arad",
            "get_atomic_symmetric_kernels_arad", "fget_kernels_arad", "fget_symmetric_kernels_arad",
@@ -18,4 +20,6 @@ __all__ = ["gaussian_kernel", "laplacian_kernel", "get_atomic_kernels_arad",
            "fget_vector_kernels_gaussian", "fget_vector_kernels_laplacian", "get_alpha",
            "get_prediction", "get_alpha_from_distance", "get_prediction_from_distance",
            "fget_alpha", "fget_prediction", "fget_alpha_from_distance",
-           "fget_prediction_from_distance"]
+           "fget_prediction_from_distance", "get_atomic_kernels_aras",
+           "get_atomic_symmetric_kernels_aras", "fget_kernels_aras", "fget_symmetric_kernels_aras",
+           "gen_pd", "periodic_distance"]

@@ -272,6 +272,36 @@ int fmlh_arad_describe(const double *z, const double *r, const int32_t *offsets,
                        int ptp_rows, double *q, double *zt, int32_t *n,
                        int32_t *status, void *stream);
 
+/* ------------------------------------------------------------------------
+ * ARAS kernels: fget_kernels_aras (mode 0, fml/kernels/faras_kernels.f90:
+ * 150-483) and fget_symmetric_kernels_aras (mode 1, :486-723; set 2 = set 1,
+ * each pair i >= j computed with molecule i first and written to both
+ * halves); Python entries get_atomic_kernels_aras /
+ * get_atomic_symmetric_kernels_aras (fml/kernels/kernels.py:408-560).
+ * x1, x2: ARAS descriptors (nm, amax, 3 + amax, amax) f64 as ARAS.describe
+ * makes them (fml/aras.py:70-131); n1, n2 (nm) int32 atom counts; nn1, nn2
+ * (optional, (nm, amax) int32): neighbour counts per atom, self included
+ * (NULL: the number of distances < cut, as the reference's wrapper counts
+ * them, kernels.py:460-472).  sigmas: host array of nsig (1..8) values;
+ * pd: device Fortran (emax, emax) element similarity (gen_pd,
+ * kernels.py:68-98, built with r_width and c_width); order 1..8; amax <= 64.
+ * out (nm2, nm1, nsig) C order = the reference's Fortran (nsig, nm1, nm2).
+ * status (may be NULL) (1) int32: 1 when a charge is not an integer in
+ * 1..emax, a neighbourhood holds more than 16 elements, or a molecule more
+ * than amax atoms.  ws: fmlh_aras_ws_bytes(nm1, nm2, amax, order, mode).
+ * Every expression in the reference's order; entries differ from the
+ * reference by the device exp/sin/cos rounding and the order of the final
+ * sum over atom pairs (a fixed order: values do not depend on the batch).
+ * ---------------------------------------------------------------------- */
+size_t fmlh_aras_ws_bytes(int nm1, int nm2, int amax, int order, int mode);
+int fmlh_aras_kernels(int mode, const double *x1, const int32_t *n1,
+                      const int32_t *nn1, int nm1, const double *x2,
+                      const int32_t *n2, const int32_t *nn2, int nm2, int amax,
+                      const double *sigmas, int nsig, double t_width,
+                      double d_width, double cut, int order, const double *pd,
+                      int emax, double angular_scale, double *out,
+                      int32_t *status, void *ws, size_t ws_bytes, void *stream);
+
 /* precision: 0 = FP64 throughout (the plain form); 1 = FP32 inner overlap
  * (exp, products and accumulation in float), FP64 self terms, distance
  * combination and kernel sums.
