@@ -62,7 +62,7 @@ def parse():
                     choices=["arad", "krr", "gaussian64", "gaussian32", "gaussianbf16x3",
                              "laplacian64", "laplacian32", "cholesky64", "cholesky32",
                              "krr36k_fp64", "krr36k_mixed", "serving", "arad4096", "arad4096_fp32", "copy",
-                             "atomic", "atomic_lap", "atomic_mfma"])
+                             "atomic", "atomic_lap", "atomic_mfma", "aras"])
     ap.add_argument("--duration", type=float, default=5.0, help="serving run length in seconds")
     ap.add_argument("--n", type=int, default=0, help="problem size override for the 36k workloads")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -450,6 +450,116 @@ def atomic_workload(args, kind=0):
     return info, step, units, roofline, cpu_baseline, None
 
 
+def aras_workload(args):
+    """The global ARAS kernel (fget_kernels_aras, faras_kernels.f90:150-483):
+    500 x 500 molecules (5-23 atoms, seeds 0 and 1), descriptors
+    (500, 23, 26, 23), default widths (t_width pi, d_width 0.2, cut 5,
+    r_width 1, c_width 0.5, order 1, angular scale 0.1), sigmas {0.5, 2, 10};
+    one step = one fmlh_aras_kernels call (per-set preparation, self
+    distances and the molecule-pair kernel) on HBM-resident descriptors."""
+    import torch
+    import fml_amd._native as nat
+    from fml_amd import synth
+    from fml_amd.aras import ARAS
+    from fml_amd.device import empty, to_device, workspace
+    from fml_amd.kernels.aras_kernels import _neighbors, _pd_cached, aras_blocks
+    nm, amax, sig = args.n or 500, 23, [0.5, 2.0, 10.0]
+    ms1 = synth.generate(nm, seed=0, nmin=5, nmax=23)
+    ms2 = synth.generate(nm, seed=1, nmin=5, nmax=23)
+    ar = ARAS(maxMolSize=amax, maxAts=amax)
+    X1 = np.array([ar.describe(ms1.molecule(m)[1], ms1.molecule(m)[0]) for m in range(nm)])
+    X2 = np.array([ar.describe(ms2.molecule(m)[1], ms2.molecule(m)[0]) for m in range(nm)])
+    nn1, nn2 = _neighbors(X1, ms1.natoms, 5.0), _neighbors(X2, ms2.natoms, 5.0)
+    dX1, dX2 = to_device(X1), to_device(X2)
+    dn1, dn2 = to_device(ms1.natoms, np.int32), to_device(ms2.natoms, np.int32)
+    dnn1, dnn2 = to_device(nn1, np.int32), to_device(nn2, np.int32)
+    pd = _pd_cached(1.0, 0.5)
+    out = empty((nm, nm, len(sig)), np.float64)
+    status = empty((1,), np.int32)
+    lib = nat.load()
+    wsb = lib.fmlh_aras_ws_bytes(nm, nm, amax, 1, 0)
+    ws = workspace(wsb)
+    arr = (ctypes.c_double * len(sig))(*sig)
+    p = nat.ptr
+    blk, nblk = aras_blocks(ms1.natoms, amax)
+    args_c = (0, p(dX1), p(dn1), p(dnn1), nm, p(dX2), p(dn2), p(dnn2), nm, amax, arr, len(sig), np.pi, 0.2,
+              5.0, 1, p(pd), int(pd.shape[0]), 0.1, p(blk), nblk, p(out), p(status), p(ws), wsb)
+
+    def step():
+        nat.call("fmlh_aras_kernels", *args_c, nat.stream_ptr())
+    step.graph_capable = True
+
+    # VALU work of the molecule-pair kernel, counted exactly on set-1 rows
+    # 0..19 against all of set 2 and scaled to the whole set: per neighbour
+    # pair inside the Gaussian window 28 ops (window test, exp counted as
+    # 20, radial products and sums) + 5 per element pair (two products, a
+    # sum, the pd weight, the accumulation) + 2; atom pairs: 4 + 3 per sigma
+    # (the exp of l2 counted as 20 is in the exponent stage, not here).
+    def nt_of(X, n):
+        out_ = np.zeros((X.shape[0], amax), dtype=np.int64)
+        nb_ = np.zeros((X.shape[0], amax), dtype=np.int64)
+        for a in range(X.shape[0]):
+            for i in range(int(n[a])):
+                k = int(np.sum(X[a, i, 0] < 5.0))
+                nb_[a, i] = k
+                out_[a, i] = len(set(X[a, i, 1, :k].astype(int).tolist()))
+        return nb_, out_
+    nb1, nt1 = nt_of(X1[:20], ms1.natoms[:20])
+    nb2, nt2 = nt_of(X2, ms2.natoms)
+    ops_sample = 0.0
+    evals_sample = 0
+    for i in range(20):
+        for i1 in range(int(ms1.natoms[i])):
+            d1 = X1[i, i1, 0, 1:nb1[i, i1]]
+            for j in range(nm):
+                nj = int(ms2.natoms[j])
+                d2 = X2[j, :nj, 0, :]
+                w = ((d2[:, None, :] - d1[None, :, None]) ** 2 < 2.56)
+                mask = np.arange(amax)[None, None, :] >= 1
+                mask = mask & (np.arange(amax)[None, None, :] < nb2[j, :nj, None, None])
+                cnt = np.sum(w & mask, axis=(1, 2))
+                evals_sample += int(cnt.sum())
+                ops_sample += float(np.sum(cnt * (30 + 5 * nt1[i, i1] * nt2[j, :nj])))
+    rows_frac = float(np.sum(ms1.natoms[:20])) / float(np.sum(ms1.natoms))
+    ops = ops_sample / rows_frac
+    evals = evals_sample / rows_frac
+    units = len(sig) * nm * nm
+    info = {"metric": "kernel entries/s", "unit": "entries/s", "dtype": "f64",
+            "config": {"workload": f"global ARAS kernel, {nm}x{nm} molecules (5-23 atoms, seeds 0/1), "
+                                   f"descriptors ({nm},{amax},{3 + amax},{amax}), sigmas {{0.5,2,10}}, "
+                                   "t_width pi, d_width 0.2, cut 5, order 1",
+                       "molecules": nm, "amax": amax,
+                       "inner_evaluations_per_step_est": round(evals, 0)},
+            "kernel": "k_aras_tiles"}
+
+    def roofline(t):
+        ach = ops / t / 1e12
+        return {"bound": "valu", "achieved": round(ach, 3), "peak": PEAK_FP64_VALU_TOPS,
+                "unit": "T FP64-VALU-ops/s", "frac": round(ach / PEAK_FP64_VALU_TOPS, 4), "traffic": None,
+                "ops_per_step_est": round(ops, 0),
+                "note": "ops counted on set-1 rows 0-19 against all of set 2, scaled by atom count"}
+
+    def cpu_baseline(rows):
+        import oracle
+        if int(status.item()) != 0:
+            raise RuntimeError("ARAS kernel reported bad descriptors")
+        rows = max(1, min(rows // 4, nm))
+        bkind = "reference" if oracle.ref_available() else "port"
+        fn = oracle.ref_kernels_aras if bkind == "reference" else oracle.kernels_aras
+        t0 = time.perf_counter()
+        kc = fn(X1[:rows], X2, ms1.natoms[:rows], ms2.natoms, sig)
+        dt = time.perf_counter() - t0
+        g = np.transpose(out[:, :rows, :].cpu().numpy(), (2, 1, 0))
+        m = kc > 1e-290
+        err = float(np.max(np.abs(g[m] - kc[m]) / kc[m]))
+        return ({"value": round(len(sig) * rows * nm / dt, 1), "unit": "entries/s",
+                 "cores": host_threads(), "kind": bkind,
+                 "sample": f"set-1 molecules 0..{rows - 1} x all {nm} set-2 molecules "
+                           f"({len(sig) * rows * nm} entries), {dt:.2f} s"}, err)
+
+    return info, step, units, roofline, cpu_baseline, None
+
+
 def copy_workload(args):
     """Plain device-to-device copy of 512 MiB (torch's elementwise copy
     kernel): the known-size calibration for the HBM FETCH_SIZE / WRITE_SIZE
@@ -679,6 +789,8 @@ def main():
         info, step, units, roofline, cpu_baseline, finish = cholesky_workload(args, wl)
     elif wl == "copy":
         info, step, units, roofline, cpu_baseline, finish = copy_workload(args)
+    elif wl == "aras":
+        info, step, units, roofline, cpu_baseline, finish = aras_workload(args)
     elif wl.startswith("atomic"):
         info, step, units, roofline, cpu_baseline, finish = atomic_workload(
             args, {"atomic": 0, "atomic_lap": 1, "atomic_mfma": 2}[wl])

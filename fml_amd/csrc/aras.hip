@@ -8,22 +8,23 @@
 // last), channel 1 their nuclear charges, channel 3 + m the angles between
 // neighbour m and every other neighbour.
 //
-// Three launches per set and one for the pairs:
-//   k_aras_prep (one wavefront per molecule, atom by atom): the neighbour
-//     count nb (distances < cut, self included), the atom's distinct
-//     neighbour elements in ascending order, ksi_m = 1/d_m^6 q0 q_m, and per
-//     order o and element p the projections
-//       cp[o][j][p] = sum_k (cos(o th) - cos((th + pi) o)) ksi3(j, k),
-//       sp[o][j][p] = sum_k (sin(o th) - sin((th + pi) o)) ksi3(j, k)
-//     over neighbours k != j of charge p (k ascending), th the j-k angle and
-//     ksi3 = q0 qj qk (1 + cos th cos_i cos_j) / (di dj dij)^3 from the law
-//     of cosines (faras_kernels.f90:258-361), each lane a neighbour j;
-//   k_aras_self: atomic_distl2 of every atom with itself;
-//   k_aras_pairs (one wavefront per molecule pair): per atom pair
-//     l2 = self1 + self2 - 2 pd(q1, q2) D(a, b), zeroed below 5e-12, and
-//     K_s(i, j) = sum over the atom pairs of exp(-l2 / sigma_s^2), the sum
-//     in a fixed order (lane partials in atom-pair order, then a fixed
-//     wavefront tree), so a pair's value does not depend on the batch.
+// Launches: k_aras_init; per set k_aras_prep (one wavefront per molecule,
+// atom by atom): the neighbour count nb (distances < cut, self included),
+// the atom's distinct neighbour elements in ascending order, ksi_m =
+// 1/d_m^6 q0 q_m, and per order o and element p the projections
+//   cp[o][j][p] = sum_k (cos(o th) - cos((th + pi) o)) ksi3(j, k),
+//   sp[o][j][p] = sum_k (sin(o th) - sin((th + pi) o)) ksi3(j, k)
+// over neighbours k != j of charge p (k ascending), th the j-k angle and
+// ksi3 = q0 qj qk (1 + cos th cos_i cos_j) / (di dj dij)^3 from the law of
+// cosines (faras_kernels.f90:258-361), each lane a neighbour j;
+// k_aras_elems: the elements of both sets as one ascending list; per set
+// k_aras_proj (element indices into that list, the projections by list
+// index) and k_aras_self (atomic_distl2 of every atom with itself);
+// k_aras_tiles: per atom pair l2 = self1 + self2 - 2 pd(q1, q2) D(a, b),
+// zeroed below 5e-12, and K_s(i, j) = sum over the atom pairs of
+// exp(-l2 / sigma_s^2) in a fixed order (per set-1 atom over the set-2
+// atoms in order, then over the set-1 atoms in order), so a pair's value
+// does not depend on the batch.
 // D(a, b) = pd(q0a, q0b) + sum over neighbours m1 of a, m2 of b (both past
 // self, inside the cut) with (d2 - d1)^2 < (8 d_width)^2 of
 //   exp((d2 - d1)^2 (-1/(4 d_width^2))) pd(q_m1, q_m2)
@@ -69,8 +70,22 @@ struct ArasSet {
   int32_t *nt;      // (nm, amax)
   int32_t *types;   // (nm, amax, kMaxTypes)
   double *ksi;      // (nm, amax, amax)
-  double *cp, *sp;  // (nm, amax, order, amax, kMaxTypes)
+  double *cs;       // (nm, amax, order, amax, kMaxTypes, 2): (cp, sp) per the atom's own elements
+  double *csg;      // the same per element of the data set's list, zero where absent
   double *self;     // (nm, amax)
+  double *dist;     // (nm, amax, amax) neighbour distances
+  int32_t *gq;      // (nm, amax, amax) neighbour element as an index into the
+                    // data set's element list (G: a charge outside it)
+  int32_t *tg;      // (nm, amax, kMaxTypes) the atom's elements as indices into that list
+};
+
+// Elements present in either set (bitmask, then an ascending list).
+struct ArasElems {
+  uint32_t bits[4];
+  int32_t count;
+  int32_t list[kMaxTypes];
+  int32_t gidx[128];   // element -> index into list, count for none
+  int32_t unsorted;    // some atom's neighbour distances are not ascending
 };
 
 __device__ __forceinline__ double xval(const ArasSet &S, int amax, int a, int i, int c, int m) {
@@ -92,7 +107,8 @@ __device__ __forceinline__ int64_t atom_index(const ArasParams &P, int a, int i)
   return (int64_t)a * P.amax + i;
 }
 
-__global__ __launch_bounds__(64) void k_aras_prep(ArasSet S, ArasParams P, int32_t *status) {
+__global__ __launch_bounds__(64) void k_aras_prep(ArasSet S, ArasParams P, ArasElems *E,
+                                                  int32_t *status) {
   __shared__ double d_s[kMaxAmax], q_s[kMaxAmax];
   __shared__ int tix[kMaxAmax];          // element index of each neighbour
   __shared__ int types[kMaxTypes];
@@ -113,6 +129,7 @@ __global__ __launch_bounds__(64) void k_aras_prep(ArasSet S, ArasParams P, int32
       q = xval(S, amax, a, i, 1, lane);
       d_s[lane] = d;
       q_s[lane] = q;
+      S.dist[ai * amax + lane] = d;
     }
     const int nb = S.nn ? max(0, min((int)S.nn[ai], amax))
                         : __popcll(__ballot(lane < amax && d < P.cut));
@@ -138,7 +155,20 @@ __global__ __launch_bounds__(64) void k_aras_prep(ArasSet S, ArasParams P, int32
         for (int t = nt; t > pos; --t) types[t] = types[t - 1];
         types[pos] = p;
         ++nt;
+        if (p < 128)
+          atomicOr(&E->bits[p >> 5], 1u << (p & 31));
+        else
+          bad = true;
       }
+      // the atom's own charge (used even when no neighbour is inside the cut)
+      const int p0 = (int)q_s[0];
+      if ((double)p0 == q_s[0] && p0 >= 1 && p0 <= P.emax && p0 < 128)
+        atomicOr(&E->bits[p0 >> 5], 1u << (p0 & 31));
+      for (int m = 1; m < nb; ++m)
+        if (!(d_s[m - 1] <= d_s[m])) {
+          atomicOr(&E->unsorted, 1);
+          break;
+        }
       if (bad && status) status[0] = 1;
       ntypes_s = nt;
       S.nb[ai] = nb;
@@ -198,8 +228,8 @@ __global__ __launch_bounds__(64) void k_aras_prep(ArasSet S, ArasParams P, int32
       if (lane < amax) {
         const int64_t base = ((ai * P.order + o) * amax + lane) * kMaxTypes;
         for (int t = 0; t < kMaxTypes; ++t) {
-          S.cp[base + t] = cacc[lane * kMaxTypes + t];
-          S.sp[base + t] = sacc[lane * kMaxTypes + t];
+          S.cs[2 * (base + t)] = cacc[lane * kMaxTypes + t];
+          S.cs[2 * (base + t) + 1] = sacc[lane * kMaxTypes + t];
         }
       }
       __syncthreads();
@@ -207,58 +237,163 @@ __global__ __launch_bounds__(64) void k_aras_prep(ArasSet S, ArasParams P, int32
   }
 }
 
+// The data set's element list from the bitmask (one thread).
+__global__ void k_aras_elems(ArasElems *E, int32_t *status) {
+  int c = 0;
+  for (int p = 1; p < 128; ++p)
+    if (E->bits[p >> 5] & (1u << (p & 31))) {
+      if (c < kMaxTypes)
+        E->list[c] = p;
+      else if (status)
+        status[0] = 1;
+      ++c;
+    }
+  E->count = min(c, kMaxTypes);
+  for (int p = 0; p < 128; ++p) E->gidx[p] = E->count;
+  for (int g = 0; g < E->count; ++g) E->gidx[E->list[g]] = g;
+}
+
+// Per atom: each neighbour's element (lane m) and each of the atom's own
+// elements (lane t) as an index into the data set's element list, for the
+// element similarity table that the pair kernels keep in LDS.
+__global__ __launch_bounds__(64) void k_aras_proj(ArasSet S, ArasParams P, const ArasElems *E) {
+  const int a = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int amax = P.amax;
+  const int n = mol_atoms(S.n, a, amax);
+  const int G = E->count;
+  for (int i = 0; i < n; ++i) {
+    const int64_t ai = atom_index(P, a, i);
+    if (lane < amax) {
+      const int nb = S.nb[ai];
+      int g = G;
+      if (lane < nb || lane == 0) {
+        const double qm = xval(S, amax, a, i, 1, lane);
+        const int p = (int)qm;
+        if ((double)p == qm && p >= 1 && p < 128) g = E->gidx[p];
+      }
+      S.gq[ai * amax + lane] = g;
+    }
+    if (lane < kMaxTypes) {
+      const int p = S.types[ai * kMaxTypes + lane];
+      S.tg[ai * kMaxTypes + lane] = (lane < S.nt[ai] && p >= 1 && p < 128) ? E->gidx[p] : G;
+    }
+    // (cp, sp) by the data set's element index, lane m a neighbour
+    if (lane < amax) {
+      const int nt = S.nt[ai];
+      const int32_t *T = S.types + ai * kMaxTypes;
+      for (int o = 0; o < P.order; ++o) {
+        const int64_t base = 2 * (((ai * P.order + o) * amax + lane) * kMaxTypes);
+        double c[kMaxTypes], sn[kMaxTypes];
+#pragma unroll
+        for (int g = 0; g < kMaxTypes; ++g) c[g] = sn[g] = 0.0;
+        for (int t = 0; t < nt; ++t) {
+          const int p = T[t];
+          const int g = (p >= 1 && p < 128) ? E->gidx[p] : G;
+#pragma unroll
+          for (int gg = 0; gg < kMaxTypes; ++gg)
+            if (gg == g) {
+              c[gg] = S.cs[base + 2 * t];
+              sn[gg] = S.cs[base + 2 * t + 1];
+            }
+        }
+#pragma unroll
+        for (int g = 0; g < kMaxTypes; ++g) {
+          S.csg[base + 2 * g] = c[g];
+          S.csg[base + 2 * g + 1] = sn[g];
+        }
+      }
+    }
+  }
+}
+
+// Element similarity between the data set's elements, pdg[a (G + 1) + b] =
+// pd(list[a], list[b]); row and column G (padding, and a charge outside the
+// list, which the status flags) hold 0.
+__device__ void load_pdg(double *pdg, const ArasParams &P, const ArasElems *E) {
+  const int G = E->count;
+  for (int t = threadIdx.x; t < (G + 1) * (G + 1); t += blockDim.x) {
+    const int a = t / (G + 1), b = t % (G + 1);
+    pdg[t] = (a < G && b < G) ? pdv(P, E->list[a], E->list[b]) : 0.0;
+  }
+}
+
 // aras_utils::atomic_distl2 (faras_kernels.f90:7-147) for atom (a1, i1) of
-// set A against atom (a2, i2) of set B.
+// set A against atom (a2, i2) of set B, in the reference's order over the
+// neighbour pairs.  The neighbour distances are ascending, and
+// fl((d2 - d1)^2) < (8 d_width)^2 holds on one interval of m2 whose ends only
+// move forward with m1 (fl(d2 - d1) is monotone in both arguments); the
+// walk visits that interval only, the same pairs in the same order.  With
+// band false (distances not ascending somewhere in the data) every m2 is
+// tested, as in the reference.
 __device__ double aras_distl2(const ArasSet &A, int a1, int i1, const ArasSet &B, int a2, int i2,
-                              const ArasParams &P) {
+                              const ArasParams &P, const double *pdg, int G, bool band) {
   const int amax = P.amax;
   const int64_t x1 = atom_index(P, a1, i1), x2 = atom_index(P, a2, i2);
   const int nb1 = A.nb[x1], nb2 = B.nb[x2];
-  const int nt1 = A.nt[x1], nt2 = B.nt[x2];
-  const int32_t *T1 = A.types + x1 * kMaxTypes;
-  const int32_t *T2 = B.types + x2 * kMaxTypes;
+  const double *dA = A.dist + x1 * amax;
+  const double *dB = B.dist + x2 * amax;
+  const int32_t *gA = A.gq + x1 * amax;
+  const int32_t *gB = B.gq + x2 * amax;
   const double *ks1 = A.ksi + x1 * amax;
   const double *ks2 = B.ksi + x2 * amax;
-  double aad = pdv(P, (int)xval(A, amax, a1, i1, 1, 0), (int)xval(B, amax, a2, i2, 1, 0));
+  const int G1 = G + 1;
+  const int nt1 = A.nt[x1], nt2 = B.nt[x2];
+  const int32_t *tg1 = A.tg + x1 * kMaxTypes;
+  const int32_t *tg2 = B.tg + x2 * kMaxTypes;
+  double aad = pdg[gA[0] * G1 + gB[0]];
+  int lo = 1;
   for (int m1 = 1; m1 < nb1; ++m1) {
-    const double d1 = xval(A, amax, a1, i1, 0, m1);
+    const double d1 = dA[m1];
     if (d1 > P.cut) break;
-    const int q1 = (int)xval(A, amax, a1, i1, 1, m1);
-    for (int m2 = 1; m2 < nb2; ++m2) {
-      const double d2 = xval(B, amax, a2, i2, 0, m2);
+    const int g1 = gA[m1];
+    const double k1 = ks1[m1];
+    while (band && lo < nb2) {
+      const double d2 = dB[lo];
+      const double dd = d2 - d1;
+      if (d2 < d1 && !(dd * dd < P.maxg2))
+        ++lo;
+      else
+        break;
+    }
+    for (int m2 = lo; m2 < nb2; ++m2) {
+      const double d2 = dB[m2];
       if (d2 > P.cut) break;
       const double dd = d2 - d1;
       const double r2 = dd * dd;
-      if (!(r2 < P.maxg2)) continue;
-      const int q2 = (int)xval(B, amax, a2, i2, 1, m2);
-      const double dw = exp(r2 * P.inv_width) * pdv(P, q1, q2);
+      if (!(r2 < P.maxg2)) {
+        if (band) break;   // past the window's upper end
+        continue;
+      }
+      const double dw = exp(r2 * P.inv_width) * pdg[g1 * G1 + gB[m2]];
       double angular = 0.0 * 0.0;
       for (int o = 0; o < P.order; ++o) {
-        const double *c1 = A.cp + ((x1 * P.order + o) * amax + m1) * kMaxTypes;
-        const double *s1 = A.sp + ((x1 * P.order + o) * amax + m1) * kMaxTypes;
-        const double *c2 = B.cp + ((x2 * P.order + o) * amax + m2) * kMaxTypes;
-        const double *s2 = B.sp + ((x2 * P.order + o) * amax + m2) * kMaxTypes;
+        const double *c1 = A.cs + 2 * ((x1 * P.order + o) * amax + m1) * kMaxTypes;
+        const double *c2 = B.cs + 2 * ((x2 * P.order + o) * amax + m2) * kMaxTypes;
         double temp = 0.0;
         for (int t1 = 0; t1 < nt1; ++t1) {
-          const double cc = c1[t1], ss = s1[t1];
-          const int p1 = T1[t1];
+          const double cc = c1[2 * t1], ss = c1[2 * t1 + 1];
+          const double *pw = pdg + tg1[t1];   // pd(p2, p1) = pw[g2 (G + 1)]
           for (int t2 = 0; t2 < nt2; ++t2)
-            temp = temp + (cc * c2[t2] + ss * s2[t2]) * pdv(P, T2[t2], p1);
+            temp = temp + (cc * c2[2 * t2] + ss * c2[2 * t2 + 1]) * pw[tg2[t2] * G1];
         }
         angular = angular + temp * P.s[o];
       }
-      aad = aad + dw * (ks1[m1] * ks2[m2] + angular * P.scale);
+      aad = aad + dw * (k1 * ks2[m2] + angular * P.scale);
     }
   }
   return aad;
 }
 
-__global__ void k_aras_self(ArasSet S, ArasParams P) {
+__global__ __launch_bounds__(128) void k_aras_self(ArasSet S, ArasParams P, const ArasElems *E) {
+  __shared__ double pdg[(kMaxTypes + 1) * (kMaxTypes + 1)];
+  load_pdg(pdg, P, E);
+  __syncthreads();
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (int64_t)S.nm * P.amax) return;
   const int a = (int)(t / P.amax), i = (int)(t % P.amax);
   if (i >= mol_atoms(S.n, a, P.amax)) return;
-  S.self[t] = aras_distl2(S, a, i, S, a, i, P);
+  S.self[t] = aras_distl2(S, a, i, S, a, i, P, pdg, E->count, E->unsorted == 0);
 }
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -266,48 +401,183 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// One wavefront per molecule pair (i of set A, j of set B); symmetric:
-// pairs i >= j of one set, written to both halves.
-__global__ __launch_bounds__(64) void k_aras_pairs(ArasSet A, ArasSet B, ArasParams P, int sym,
-                                                   double *__restrict__ out) {
-  __shared__ double l2s[kMaxAmax * kMaxAmax];
-  int i, j;
-  const int64_t w = blockIdx.x;
-  if (sym) {
-    // w = i (i + 1) / 2 + j, j <= i
-    i = (int)((sqrt(8.0 * (double)w + 1.0) - 1.0) * 0.5);
-    while ((int64_t)(i + 1) * (i + 2) / 2 <= w) ++i;
-    while ((int64_t)i * (i + 1) / 2 > w) --i;
-    j = (int)(w - (int64_t)i * (i + 1) / 2);
-  } else {
-    i = (int)(w % A.nm);
-    j = (int)(w / A.nm);
-  }
+constexpr int kJB = 2;   // set-B molecules per workgroup
+
+// Set-B tables the tile kernel reads at wave-uniform addresses.
+struct ArasUniform {
+  const int32_t *__restrict__ n;
+  const int32_t *__restrict__ nb;
+  const int32_t *__restrict__ nt;
+  const int32_t *__restrict__ tg;
+  const int32_t *__restrict__ gq;
+  const double *__restrict__ dist;
+  const double *__restrict__ ksi;
+  const double *__restrict__ cs;
+  const double *__restrict__ self;
+};
+
+// Molecule-pair kernels over tiles: a workgroup (one wavefront) takes a
+// block of set-A molecules, one atom per lane (at most 64 atoms, block
+// boundaries blk[b] .. blk[b + 1]), against kJB molecules of set B, whose
+// atoms it walks one at a time, so every set-B value is the same for all
+// lanes.  Each lane forms atomic_distl2 of its atom a against the current
+// set-B atom b with the reference's loops -- m1 over a's neighbours, m2 over
+// b's, (cp, sp) products over the elements p1 of a (ascending) and p2 of b
+// (ascending) -- and the same running sums; a's projections are indexed by
+// the data set's element list, GB wide, zero for elements a does not have,
+// which adds exact zeros between the reference's terms.  The lane then adds
+// exp(-l2 / sigma^2) over b in the molecule's atom order; the molecule's
+// value is the sum of its lanes in atom order.  Symmetric: only pairs
+// i >= j, written to both halves.  GB is the smallest of {4, 6, 8, 16} that
+// holds the data set's element count (the other instantiations return).
+template <int GB>
+__global__ __launch_bounds__(64) void k_aras_tiles(ArasSet A, ArasUniform B, int nm2, ArasParams P,
+                                                   const int32_t *__restrict__ blk, int nblk,
+                                                   int per_blk, int sym, int g_lo,
+                                                   const ArasElems *__restrict__ E,
+                                                   double *__restrict__ out,
+                                                   int32_t *__restrict__ status) {
+  const int G = E->count;
+  if (G > GB || G <= g_lo) return;
+  __shared__ double pdg[(kMaxTypes + 1) * (kMaxTypes + 1)];
+  __shared__ double red[64 * kMaxSig];
+  __shared__ int lane_mol[64], lane_atom[64], natoms_s;
   const int lane = threadIdx.x;
-  const int ni = mol_atoms(A.n, i, P.amax), nj = mol_atoms(B.n, j, P.amax);
-  const int np = ni * nj;
-  for (int t = lane; t < np; t += 64) {
-    const int i1 = t % ni, j1 = t / ni;  // the reference's column-major atom block
-    const double l = aras_distl2(A, i, i1, B, j, j1, P);
-    const double q1 = xval(A, P.amax, i, i1, 1, 0), q2 = xval(B, P.amax, j, j1, 1, 0);
-    double l2 = A.self[atom_index(P, i, i1)] + B.self[atom_index(P, j, j1)] -
-                2.0 * l * pdv(P, (int)q1, (int)q2);
-    if (fabs(l2) < 5.0e-12) l2 = 0.0;
-    l2s[t] = l2;
+  const int64_t w = blockIdx.x;
+  const int ib = (int)(w % nblk);
+  const int j0 = (int)(w / nblk) * kJB;
+  const int m0 = blk ? blk[ib] : ib * per_blk;
+  const int m1e = blk ? blk[ib + 1] : min(A.nm, (ib + 1) * per_blk);
+  if (sym && j0 > m1e - 1) return;   // every pair of the tile has i < j
+  load_pdg(pdg, P, E);
+  if (lane == 0) {
+    int c = 0;
+    for (int m = m0; m < m1e; ++m) {
+      const int nm_ = mol_atoms(A.n, m, P.amax);
+      for (int i1 = 0; i1 < nm_; ++i1) {
+        if (c < 64) {
+          lane_mol[c] = m;
+          lane_atom[c] = i1;
+        }
+        ++c;
+      }
+    }
+    if (c > 64 && status) status[0] = 1;
+    natoms_s = min(c, 64);
   }
   __syncthreads();
-  for (int s = 0; s < P.nsig; ++s) {
-    double v = 0.0;
-    for (int t = lane; t < np; t += 64) v = v + exp(l2s[t] * P.inv_sig[s]);
-    v = wave_sum(v);
-    if (lane == 0) {
-      out[((int64_t)j * A.nm + i) * P.nsig + s] = v;
-      if (sym) out[((int64_t)i * A.nm + j) * P.nsig + s] = v;
+  const int amax = P.amax;
+  const int G1 = G + 1;
+  const bool valid = lane < natoms_s;
+  const int mi = valid ? lane_mol[lane] : m0;
+  const int64_t x1 = valid ? atom_index(P, mi, lane_atom[lane]) : 0;
+  const int nb1 = valid ? A.nb[x1] : 0;
+  const int g01 = valid ? A.gq[x1 * amax] : G;
+  const double self1 = valid ? A.self[x1] : 0.0;
+  const bool first = valid && lane_atom[lane] == 0;
+  const int ni = valid ? mol_atoms(A.n, mi, amax) : 0;
+
+  for (int jj = 0; jj < kJB; ++jj) {
+    const int j = j0 + jj;
+    if (j >= nm2) break;
+    if (sym && j > m1e - 1) break;
+    double e[kMaxSig];
+#pragma unroll
+    for (int s = 0; s < kMaxSig; ++s) e[s] = 0.0;
+    const int nj = mol_atoms(B.n, j, amax);
+    for (int j1 = 0; j1 < nj; ++j1) {
+      const int64_t x2 = atom_index(P, j, j1);
+      const int nb2 = B.nb[x2], nt2 = B.nt[x2];
+      int tg2[GB];
+#pragma unroll
+      for (int t = 0; t < GB; ++t) tg2[t] = t < nt2 ? B.tg[x2 * kMaxTypes + t] : G;
+      const double *d2p = B.dist + x2 * amax;
+      const int32_t *g2p = B.gq + x2 * amax;
+      const double *k2p = B.ksi + x2 * amax;
+      double aad = pdg[g01 * G1 + g2p[0]];
+      bool alive = valid;
+      for (int m1 = 1;; ++m1) {
+        bool act = alive && m1 < nb1;
+        if (__ballot(act) == 0) break;
+        double d1 = 0.0;
+        if (act) {
+          d1 = A.dist[x1 * amax + m1];
+          if (d1 > P.cut) {   // the reference leaves the m1 loop here
+            alive = false;
+            act = false;
+          }
+        }
+        int g1 = G;
+        double k1 = 0.0;
+        double c1[GB], s1[GB];
+#pragma unroll
+        for (int g = 0; g < GB; ++g) c1[g] = s1[g] = 0.0;
+        if (act) {
+          g1 = A.gq[x1 * amax + m1];
+          k1 = A.ksi[x1 * amax + m1];
+          const double *cp = A.csg + 2 * ((x1 * P.order) * amax + m1) * kMaxTypes;
+#pragma unroll
+          for (int g = 0; g < GB; ++g) {
+            c1[g] = cp[2 * g];
+            s1[g] = cp[2 * g + 1];
+          }
+        }
+        for (int m2 = 1; m2 < nb2; ++m2) {
+          const double d2 = d2p[m2];
+          if (d2 > P.cut) break;
+          const double dd = d2 - d1;
+          const double r2 = dd * dd;
+          if (act && r2 < P.maxg2) {
+            const double dw = exp(r2 * P.inv_width) * pdg[g1 * G1 + g2p[m2]];
+            double angular = 0.0 * 0.0;
+            for (int o = 0; o < P.order; ++o) {
+              const double *c2 = B.cs + 2 * ((x2 * P.order + o) * amax + m2) * kMaxTypes;
+              const double *cq = A.csg + 2 * ((x1 * P.order + o) * amax + m1) * kMaxTypes;
+              double temp = 0.0;
+#pragma unroll
+              for (int g = 0; g < GB; ++g) {
+                const double cc = o == 0 ? c1[g] : cq[2 * g];
+                const double ss = o == 0 ? s1[g] : cq[2 * g + 1];
+#pragma unroll
+                for (int t = 0; t < GB; ++t) {
+                  if (t < nt2)
+                    temp = temp + (cc * c2[2 * t] + ss * c2[2 * t + 1]) * pdg[tg2[t] * G1 + g];
+                }
+              }
+              angular = angular + temp * P.s[o];
+            }
+            aad = aad + dw * (k1 * k2p[m2] + angular * P.scale);
+          }
+        }
+      }
+      if (valid) {
+        double l2 = self1 + B.self[x2] - 2.0 * aad * pdg[g01 * G1 + g2p[0]];
+        if (fabs(l2) < 5.0e-12) l2 = 0.0;
+#pragma unroll
+        for (int s = 0; s < kMaxSig; ++s)
+          if (s < P.nsig) e[s] = e[s] + exp(l2 * P.inv_sig[s]);
+      }
     }
+#pragma unroll
+    for (int s = 0; s < kMaxSig; ++s) red[lane * kMaxSig + s] = e[s];
+    __syncthreads();
+    if (first && (!sym || mi >= j)) {
+      for (int s = 0; s < P.nsig; ++s) {
+        double v = red[lane * kMaxSig + s];
+        for (int k = 1; k < ni && lane + k < 64; ++k) v = v + red[(lane + k) * kMaxSig + s];
+        out[((int64_t)j * A.nm + mi) * P.nsig + s] = v;
+        if (sym) out[((int64_t)mi * A.nm + j) * P.nsig + s] = v;
+      }
+    }
+    __syncthreads();
   }
 }
 
-__global__ void k_zero_status1(int32_t *s) { s[0] = 0; }
+__global__ void k_aras_init(int32_t *status, ArasElems *E) {
+  if (status) status[0] = 0;
+  for (int k = 0; k < 4; ++k) E->bits[k] = 0u;
+  E->unsorted = 0;
+}
 
 // Workspace of one set's tables.
 size_t set_bytes(int nm, int amax, int order) {
@@ -315,8 +585,10 @@ size_t set_bytes(int nm, int amax, int order) {
   size_t b = 0;
   b += WsCarver::need(sizeof(int32_t) * na) * 2;
   b += WsCarver::need(sizeof(int32_t) * na * kMaxTypes);
-  b += WsCarver::need(sizeof(double) * na * amax);
-  b += WsCarver::need(sizeof(double) * na * order * amax * kMaxTypes) * 2;
+  b += WsCarver::need(sizeof(double) * na * amax) * 2;
+  b += WsCarver::need(sizeof(int32_t) * na * amax);
+  b += WsCarver::need(sizeof(int32_t) * na * kMaxTypes);
+  b += WsCarver::need(sizeof(double) * 2 * na * order * amax * kMaxTypes) * 2;
   b += WsCarver::need(sizeof(double) * na);
   return b;
 }
@@ -327,17 +599,21 @@ bool carve_set(WsCarver &wc, ArasSet &S, int nm, int amax, int order) {
   S.nt = wc.take<int32_t>(na);
   S.types = wc.take<int32_t>(na * kMaxTypes);
   S.ksi = wc.take<double>(na * amax);
-  S.cp = wc.take<double>(na * order * amax * kMaxTypes);
-  S.sp = wc.take<double>(na * order * amax * kMaxTypes);
+  S.cs = wc.take<double>(2 * na * order * amax * kMaxTypes);
+  S.csg = wc.take<double>(2 * na * order * amax * kMaxTypes);
   S.self = wc.take<double>(na);
-  return S.nb && S.nt && S.types && S.ksi && S.cp && S.sp && S.self;
+  S.dist = wc.take<double>(na * amax);
+  S.gq = wc.take<int32_t>(na * amax);
+  S.tg = wc.take<int32_t>(na * kMaxTypes);
+  return S.nb && S.nt && S.types && S.ksi && S.cs && S.csg && S.self && S.dist && S.gq && S.tg;
 }
 
 }  // namespace
 
 extern "C" size_t fmlh_aras_ws_bytes(int nm1, int nm2, int amax, int order, int mode) {
   if (nm1 < 0 || nm2 < 0 || amax < 1 || order < 1) return 0;
-  return set_bytes(nm1, amax, order) + (mode == 1 ? 0 : set_bytes(nm2, amax, order));
+  return WsCarver::need(sizeof(ArasElems)) + set_bytes(nm1, amax, order) +
+         (mode == 1 ? 0 : set_bytes(nm2, amax, order));
 }
 
 extern "C" int fmlh_aras_kernels(int mode, const double *x1, const int32_t *n1,
@@ -345,8 +621,9 @@ extern "C" int fmlh_aras_kernels(int mode, const double *x1, const int32_t *n1,
                                  const int32_t *n2, const int32_t *nn2, int nm2, int amax,
                                  const double *sigmas, int nsig, double t_width, double d_width,
                                  double cut, int order, const double *pd, int emax,
-                                 double angular_scale, double *out, int32_t *status, void *ws,
-                                 size_t ws_bytes, void *stream) {
+                                 double angular_scale, const int32_t *blk1, int nblk1,
+                                 double *out, int32_t *status, void *ws, size_t ws_bytes,
+                                 void *stream) {
   FMLH_ARG_CHECK(mode == 0 || mode == 1, 1);
   FMLH_ARG_CHECK(nm1 >= 0, 5);
   FMLH_ARG_CHECK(mode == 1 || nm2 >= 0, 9);
@@ -363,8 +640,9 @@ extern "C" int fmlh_aras_kernels(int mode, const double *x1, const int32_t *n1,
   FMLH_ARG_CHECK(x1 != nullptr && n1 != nullptr, 2);
   FMLH_ARG_CHECK(mode == 1 || (x2 != nullptr && n2 != nullptr), 6);
   FMLH_ARG_CHECK(pd != nullptr, 17);
-  FMLH_ARG_CHECK(out != nullptr, 20);
-  FMLH_ARG_CHECK(ws != nullptr && ws_bytes >= fmlh_aras_ws_bytes(nm1, nm2, amax, order, mode), 22);
+  FMLH_ARG_CHECK(blk1 == nullptr || nblk1 >= 1, 21);
+  FMLH_ARG_CHECK(out != nullptr, 22);
+  FMLH_ARG_CHECK(ws != nullptr && ws_bytes >= fmlh_aras_ws_bytes(nm1, nm2, amax, order, mode), 24);
 
   // angular normalisation and order weights (faras_kernels.f90:198-207, 84-88)
   const double pi = 3.141592653589793;
@@ -393,41 +671,59 @@ extern "C" int fmlh_aras_kernels(int mode, const double *x1, const int32_t *n1,
   for (int s = 0; s < nsig; ++s) P.inv_sig[s] = -1.0 / (sigmas[s] * sigmas[s]);
 
   WsCarver wc(ws, ws_bytes);
+  ArasElems *E = wc.take<ArasElems>(1);
+  if (E == nullptr) return -24;
   ArasSet A = {}, B = {};
   A.x = x1;
   A.n = n1;
   A.nn = nn1;
   A.nm = nm1;
-  if (!carve_set(wc, A, nm1, amax, order)) return -22;
+  if (!carve_set(wc, A, nm1, amax, order)) return -24;
   if (mode == 0) {
     B.x = x2;
     B.n = n2;
     B.nn = nn2;
     B.nm = nm2;
-    if (!carve_set(wc, B, nm2, amax, order)) return -22;
+    if (!carve_set(wc, B, nm2, amax, order)) return -24;
   } else {
     B = A;
   }
   const hipStream_t st = as_stream(stream);
-  if (status) {
-    hipLaunchKernelGGL(k_zero_status1, dim3(1), dim3(1), 0, st, status);
-    FMLH_LAUNCH_CHECK("k_zero_status1");
-  }
-  hipLaunchKernelGGL(k_aras_prep, dim3(nm1), dim3(64), 0, st, A, P, status);
+  hipLaunchKernelGGL(k_aras_init, dim3(1), dim3(1), 0, st, status, E);
+  FMLH_LAUNCH_CHECK("k_aras_init");
+  hipLaunchKernelGGL(k_aras_prep, dim3(nm1), dim3(64), 0, st, A, P, E, status);
   FMLH_LAUNCH_CHECK("k_aras_prep");
-  hipLaunchKernelGGL(k_aras_self, dim3(ceil_div(nm1 * amax, 128)), dim3(128), 0, st, A, P);
+  if (mode == 0) {
+    hipLaunchKernelGGL(k_aras_prep, dim3(nm2), dim3(64), 0, st, B, P, E, status);
+    FMLH_LAUNCH_CHECK("k_aras_prep");
+  }
+  hipLaunchKernelGGL(k_aras_elems, dim3(1), dim3(1), 0, st, E, status);
+  FMLH_LAUNCH_CHECK("k_aras_elems");
+  hipLaunchKernelGGL(k_aras_proj, dim3(nm1), dim3(64), 0, st, A, P, E);
+  FMLH_LAUNCH_CHECK("k_aras_proj");
+  hipLaunchKernelGGL(k_aras_self, dim3(ceil_div(nm1 * amax, 128)), dim3(128), 0, st, A, P, E);
   FMLH_LAUNCH_CHECK("k_aras_self");
   if (mode == 0) {
-    hipLaunchKernelGGL(k_aras_prep, dim3(nm2), dim3(64), 0, st, B, P, status);
-    FMLH_LAUNCH_CHECK("k_aras_prep");
-    hipLaunchKernelGGL(k_aras_self, dim3(ceil_div(nm2 * amax, 128)), dim3(128), 0, st, B, P);
+    hipLaunchKernelGGL(k_aras_proj, dim3(nm2), dim3(64), 0, st, B, P, E);
+    FMLH_LAUNCH_CHECK("k_aras_proj");
+    hipLaunchKernelGGL(k_aras_self, dim3(ceil_div(nm2 * amax, 128)), dim3(128), 0, st, B, P, E);
     FMLH_LAUNCH_CHECK("k_aras_self");
   }
-  const int64_t nw = mode == 1 ? (int64_t)nm1 * (nm1 + 1) / 2 : (int64_t)nm1 * nm2;
-  if (nw > 0x7fffffff) return -4;
+  const int per_blk = max(1, 64 / amax);
+  const int nblk = blk1 ? nblk1 : ceil_div(nm1, per_blk);
+  const int64_t nw = (int64_t)nblk * ceil_div(nm2, kJB);
+  if (nw > 0x7fffffff) return -5;
+  const ArasUniform U = {B.n, B.nb, B.nt, B.tg, B.gq, B.dist, B.ksi, B.cs, B.self};
   timing_begin(st);
-  hipLaunchKernelGGL(k_aras_pairs, dim3((unsigned)nw), dim3(64), 0, st, A, B, P, mode, out);
+  hipLaunchKernelGGL(k_aras_tiles<4>, dim3((unsigned)nw), dim3(64), 0, st, A, U, nm2, P, blk1, nblk,
+                     per_blk, mode, 0, E, out, status);
+  hipLaunchKernelGGL(k_aras_tiles<6>, dim3((unsigned)nw), dim3(64), 0, st, A, U, nm2, P, blk1, nblk,
+                     per_blk, mode, 4, E, out, status);
+  hipLaunchKernelGGL(k_aras_tiles<8>, dim3((unsigned)nw), dim3(64), 0, st, A, U, nm2, P, blk1, nblk,
+                     per_blk, mode, 6, E, out, status);
+  hipLaunchKernelGGL(k_aras_tiles<16>, dim3((unsigned)nw), dim3(64), 0, st, A, U, nm2, P, blk1, nblk,
+                     per_blk, mode, 8, E, out, status);
   timing_end(st);
-  FMLH_LAUNCH_CHECK("k_aras_pairs");
+  FMLH_LAUNCH_CHECK("k_aras_tiles");
   return 0;
 }

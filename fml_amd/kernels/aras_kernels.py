@@ -70,6 +70,15 @@ def _neighbors(X, n, cut):
     return nn
 
 
+def aras_blocks(n, amax):
+    """Set-1 molecules packed greedily, in order, into blocks of at most 64
+    atoms (one wavefront lane each): device (nblk + 1,) int32 block starts
+    and nblk."""
+    from .atomic import plan_blocks
+    b = plan_blocks(np.clip(np.asarray(n, dtype=np.int64), 0, amax), tile=64)
+    return to_device(b, np.int32), int(b.shape[0]) - 1
+
+
 def _aras_call(mode, x1, n1, nn1, x2, n2, nn2, sigmas, t_width, d_width, cut, order, pd, scale):
     X1 = to_device(x1)
     nm1, amax = int(X1.shape[0]), int(X1.shape[1])
@@ -95,6 +104,7 @@ def _aras_call(mode, x1, n1, nn1, x2, n2, nn2, sigmas, t_width, d_width, cut, or
     if nm1 == 0 or nm2 == 0 or ns == 0:
         return out
     emax = int(pd.shape[0])
+    blk, nblk = aras_blocks(to_host(n1) if is_device_tensor(n1) else n1, amax)
     lib = _native.load()
     wsb = lib.fmlh_aras_ws_bytes(nm1, nm2, amax, int(order), mode)
     ws = workspace(wsb)
@@ -106,10 +116,10 @@ def _aras_call(mode, x1, n1, nn1, x2, n2, nn2, sigmas, t_width, d_width, cut, or
         tmp = out if len(chunk) == ns else empty((nm2, nm1, len(chunk)), np.float64)
         _native.call("fmlh_aras_kernels", mode, p(X1), p(N1), p(NN1), nm1, p(X2), p(N2), p(NN2), nm2, amax,
                      arr, len(chunk), float(t_width), float(d_width), float(cut), int(order), p(pd), emax,
-                     float(scale), p(tmp), p(status), p(ws), wsb, _native.stream_ptr())
+                     float(scale), p(blk), nblk, p(tmp), p(status), p(ws), wsb, _native.stream_ptr())
         if int(status.item()) != 0:
             raise ValueError("ARAS descriptors hold a charge that is not an element of the PTP table, "
-                             "more than 16 elements around one atom, or a molecule larger than amax")
+                             "more than 16 distinct elements over both sets, or a molecule larger than amax")
         if tmp is not out:
             out[:, :, c0:c0 + len(chunk)] = tmp
     return out

@@ -286,12 +286,17 @@ int fmlh_arad_describe(const double *z, const double *r, const int32_t *offsets,
  * pd: device Fortran (emax, emax) element similarity (gen_pd,
  * kernels.py:68-98, built with r_width and c_width); order 1..8; amax <= 64.
  * out (nm2, nm1, nsig) C order = the reference's Fortran (nsig, nm1, nm2).
+ * blk1 (optional, device int32, nblk1 + 1 ascending molecule indices from
+ * 0 to nm1): set-1 molecule blocks of at most 64 atoms each, one workgroup
+ * row per block (NULL: blocks of 64 / amax molecules).
  * status (may be NULL) (1) int32: 1 when a charge is not an integer in
- * 1..emax, a neighbourhood holds more than 16 elements, or a molecule more
- * than amax atoms.  ws: fmlh_aras_ws_bytes(nm1, nm2, amax, order, mode).
- * Every expression in the reference's order; entries differ from the
- * reference by the device exp/sin/cos rounding and the order of the final
- * sum over atom pairs (a fixed order: values do not depend on the batch).
+ * 1..emax (or above 127), the two sets hold more than 16 distinct elements,
+ * a block more than 64 atoms, or a molecule more than amax atoms.
+ * ws: fmlh_aras_ws_bytes(nm1, nm2, amax, order, mode).
+ * Every expression of the atom distance in the reference's order; entries
+ * differ from the reference by the device exp/sin/cos rounding and the
+ * order of the final sum over atom pairs (a fixed order: values do not
+ * depend on the batch or the blocks).
  * ---------------------------------------------------------------------- */
 size_t fmlh_aras_ws_bytes(int nm1, int nm2, int amax, int order, int mode);
 int fmlh_aras_kernels(int mode, const double *x1, const int32_t *n1,
@@ -299,8 +304,9 @@ int fmlh_aras_kernels(int mode, const double *x1, const int32_t *n1,
                       const int32_t *n2, const int32_t *nn2, int nm2, int amax,
                       const double *sigmas, int nsig, double t_width,
                       double d_width, double cut, int order, const double *pd,
-                      int emax, double angular_scale, double *out,
-                      int32_t *status, void *ws, size_t ws_bytes, void *stream);
+                      int emax, double angular_scale, const int32_t *blk1,
+                      int nblk1, double *out, int32_t *status, void *ws,
+                      size_t ws_bytes, void *stream);
 
 /* precision: 0 = FP64 throughout (the plain form); 1 = FP32 inner overlap
  * (exp, products and accumulation in float), FP64 self terms, distance

@@ -109,3 +109,52 @@ def test_aras_edges(gpu):
     bad[0, 0, 1, 0] = 200.0            # no such element
     with pytest.raises(ValueError):
         get_atomic_kernels_aras(bad, X1, Z1, Z1, [1.0])
+
+
+def test_aras_unsorted_neighbours(gpu):
+    """Descriptors whose neighbour distances are not ascending (the
+    reference's loops take them as they come): the device switches from the
+    window walk to testing every neighbour pair and matches the oracle."""
+    from fml_amd.kernels import get_atomic_kernels_aras, get_atomic_symmetric_kernels_aras
+    X, Z, n = _set(10, 54, 4, 10, 10)
+    rng = np.random.default_rng(7)
+    for a in (0, 3):
+        for i in range(int(n[a])):
+            k = int(np.sum(X[a, i, 0, :] < 5.0))
+            perm = np.concatenate([[0], 1 + rng.permutation(k - 1), np.arange(k, X.shape[3])])
+            X[a, i, :2, :] = X[a, i, :2, perm].T
+            X[a, i, 3:, :] = X[a, i, 3 + perm][:, perm]
+    sg = [0.8, 5.0]
+    _close(get_atomic_kernels_aras(X, X[2:7], Z, Z[2:7], sg), oracle.kernels_aras(X, X[2:7], n, n[2:7], sg))
+    _close(get_atomic_symmetric_kernels_aras(X, Z, sg), oracle.kernels_aras(X, None, n, None, sg))
+
+
+def test_aras_block_lists(gpu):
+    """The C-ABI with the library's own molecule blocks (blk1 NULL) and with
+    greedy 64-atom blocks gives the same matrix, bitwise: a pair's value
+    does not depend on how set-1 molecules are grouped."""
+    import ctypes
+    from fml_amd import _native
+    from fml_amd.device import empty, to_device, workspace
+    from fml_amd.kernels.aras_kernels import _pd_cached, aras_blocks
+    X, Z, n = _set(21, 55, 2, 14, 14)
+    nn = oracle.aras_neighbors(X, n, 5.0)
+    dX, dn, dnn = to_device(X), to_device(n, np.int32), to_device(nn, np.int32)
+    pd = _pd_cached(1.0, 0.5)
+    sg = (ctypes.c_double * 2)(0.9, 3.0)
+    lib = _native.load()
+    p = _native.ptr
+    res = []
+    for blocks in (None, aras_blocks(n, 14)):
+        out = empty((21, 21, 2), np.float64)
+        st = empty((1,), np.int32)
+        wsb = lib.fmlh_aras_ws_bytes(21, 21, 14, 1, 0)
+        ws = workspace(wsb)
+        blk, nblk = (None, 0) if blocks is None else blocks
+        _native.call("fmlh_aras_kernels", 0, p(dX), p(dn), p(dnn), 21, p(dX), p(dn), p(dnn), 21, 14, sg, 2,
+                     np.pi, 0.2, 5.0, 1, p(pd), int(pd.shape[0]), 0.1, p(blk), nblk, p(out), p(st), p(ws), wsb,
+                     _native.stream_ptr())
+        assert int(st.item()) == 0
+        res.append(out.cpu().numpy())
+    assert np.array_equal(res[0], res[1])
+    _close(np.transpose(res[0], (2, 1, 0)), oracle.kernels_aras(X, X, n, n, [0.9, 3.0]))
