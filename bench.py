@@ -481,9 +481,9 @@ def aras_workload(args):
     ws = workspace(wsb)
     arr = (ctypes.c_double * len(sig))(*sig)
     p = nat.ptr
-    blk, nblk = aras_blocks(ms1.natoms, amax)
+    blk, ordv, nblk = aras_blocks(ms1.natoms, amax, X1[:, :, 1, 0])
     args_c = (0, p(dX1), p(dn1), p(dnn1), nm, p(dX2), p(dn2), p(dnn2), nm, amax, arr, len(sig), np.pi, 0.2,
-              5.0, 1, p(pd), int(pd.shape[0]), 0.1, p(blk), nblk, p(out), p(status), p(ws), wsb)
+              5.0, 1, p(pd), int(pd.shape[0]), 0.1, p(blk), p(ordv), nblk, p(out), p(status), p(ws), wsb)
 
     def step():
         nat.call("fmlh_aras_kernels", *args_c, nat.stream_ptr())

@@ -64,8 +64,8 @@ SIGNATURES = {
     "fmlh_aras_ws_bytes": (c_size_t, [c_int, c_int, c_int, c_int, c_int]),
     "fmlh_aras_kernels": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
                                   c_int, c_int, c_void_p, c_int, c_double, c_double, c_double, c_int,
-                                  c_void_p, c_int, c_double, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
-                                  c_size_t, c_void_p]),
+                                  c_void_p, c_int, c_double, c_void_p, c_void_p, c_int, c_void_p, c_void_p,
+                                  c_void_p, c_size_t, c_void_p]),
     "fmlh_arad_ws_bytes": (c_size_t, [c_int, c_int, c_int, c_int]),
     "fmlh_arad_kernels": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
                                   c_int, c_int, c_dbl_p, c_int, c_double, c_double, c_double, c_double,

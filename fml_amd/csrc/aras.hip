@@ -309,10 +309,11 @@ __global__ __launch_bounds__(64) void k_aras_proj(ArasSet S, ArasParams P, const
 
 // Element similarity between the data set's elements, pdg[a (G + 1) + b] =
 // pd(list[a], list[b]); row and column G (padding, and a charge outside the
-// list, which the status flags) hold 0.
+// list, which the status flags) and the rest of the kMaxTypes + 1 square
+// hold 0.
 __device__ void load_pdg(double *pdg, const ArasParams &P, const ArasElems *E) {
   const int G = E->count;
-  for (int t = threadIdx.x; t < (G + 1) * (G + 1); t += blockDim.x) {
+  for (int t = threadIdx.x; t < (kMaxTypes + 1) * (kMaxTypes + 1); t += blockDim.x) {
     const int a = t / (G + 1), b = t % (G + 1);
     pdg[t] = (a < G && b < G) ? pdv(P, E->list[a], E->list[b]) : 0.0;
   }
@@ -416,23 +417,125 @@ struct ArasUniform {
   const double *__restrict__ self;
 };
 
+// One lane's set-A atom (per-lane values) for the tile kernel.
+struct LaneAtom {
+  int64_t x1;
+  int nb1;
+  bool valid;
+};
+
+// atomic_distl2 of the lane's set-A atom against set-B atom x2 (the same
+// for every lane), accumulated on aad in the reference's order: m1 over
+// the lane's neighbours, m2 over x2's, and for the angular term p1 over
+// the data set's elements g (ascending; the lane's projections are zero
+// for elements its atom lacks, and elements no lane of the block has --
+// bit clear in gmask -- are skipped: both only drop exact zeros) and p2
+// over x2's own elements t (ascending).  NT2 > 0: x2 has exactly NT2
+// elements, whose similarity row pd(p2_t, g) sits in registers; NT2 == 0:
+// any count nt2, similarities read from LDS.
+template <int GB, int NT2>
+__device__ __forceinline__ double atom_pair(const ArasSet &A, const ArasUniform &B, const ArasParams &P,
+                                            const double *pdg, int G1, const LaneAtom &L,
+                                            unsigned gmask, int64_t x2, int nt2, double aad) {
+  const int amax = P.amax;
+  const int nb2 = B.nb[x2];
+  constexpr int NT = NT2 > 0 ? NT2 : GB;
+  int tg2[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) tg2[t] = (NT2 > 0 || t < nt2) ? B.tg[x2 * kMaxTypes + t] : G1 - 1;
+  double pw[NT2 > 0 ? NT2 : 1][GB];
+  if (NT2 > 0) {
+#pragma unroll
+    for (int t = 0; t < (NT2 > 0 ? NT2 : 1); ++t)
+#pragma unroll
+      for (int g = 0; g < GB; ++g) pw[t][g] = pdg[tg2[t] * G1 + g];
+  }
+  const double *d2p = B.dist + x2 * amax;
+  const int32_t *g2p = B.gq + x2 * amax;
+  const double *k2p = B.ksi + x2 * amax;
+  bool alive = L.valid;
+  for (int m1 = 1;; ++m1) {
+    bool act = alive && m1 < L.nb1;
+    if (__ballot(act) == 0) break;
+    double d1 = 0.0;
+    if (act) {
+      d1 = A.dist[L.x1 * amax + m1];
+      if (d1 > P.cut) {   // the reference leaves the m1 loop here
+        alive = false;
+        act = false;
+      }
+    }
+    int g1 = G1 - 1;
+    double k1 = 0.0;
+    double c1[GB], s1[GB];
+#pragma unroll
+    for (int g = 0; g < GB; ++g) c1[g] = s1[g] = 0.0;
+    if (act) {
+      g1 = A.gq[L.x1 * amax + m1];
+      k1 = A.ksi[L.x1 * amax + m1];
+      const double *cp = A.csg + 2 * ((L.x1 * P.order) * amax + m1) * kMaxTypes;
+#pragma unroll
+      for (int g = 0; g < GB; ++g) {
+        c1[g] = cp[2 * g];
+        s1[g] = cp[2 * g + 1];
+      }
+    }
+    for (int m2 = 1; m2 < nb2; ++m2) {
+      const double d2 = d2p[m2];
+      if (d2 > P.cut) break;
+      const double dd = d2 - d1;
+      const double r2 = dd * dd;
+      if (act && r2 < P.maxg2) {
+        const double dw = exp(r2 * P.inv_width) * pdg[g1 * G1 + g2p[m2]];
+        double angular = 0.0 * 0.0;
+        for (int o = 0; o < P.order; ++o) {
+          const double *c2 = B.cs + 2 * ((x2 * P.order + o) * amax + m2) * kMaxTypes;
+          const double *cq = A.csg + 2 * ((L.x1 * P.order + o) * amax + m1) * kMaxTypes;
+          double c2r[NT], s2r[NT];
+#pragma unroll
+          for (int t = 0; t < NT; ++t) {
+            c2r[t] = (NT2 > 0 || t < nt2) ? c2[2 * t] : 0.0;
+            s2r[t] = (NT2 > 0 || t < nt2) ? c2[2 * t + 1] : 0.0;
+          }
+          double temp = 0.0;
+#pragma unroll
+          for (int g = 0; g < GB; ++g) {
+            if (!((gmask >> g) & 1u)) continue;
+            const double cc = o == 0 ? c1[g] : cq[2 * g];
+            const double ss = o == 0 ? s1[g] : cq[2 * g + 1];
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+              if (NT2 > 0) {
+                temp = temp + (cc * c2r[t] + ss * s2r[t]) * pw[t][g];
+              } else if (t < nt2) {
+                temp = temp + (cc * c2r[t] + ss * s2r[t]) * pdg[tg2[t] * G1 + g];
+              }
+            }
+          }
+          angular = angular + temp * P.s[o];
+        }
+        aad = aad + dw * (k1 * k2p[m2] + angular * P.scale);
+      }
+    }
+  }
+  return aad;
+}
+
 // Molecule-pair kernels over tiles: a workgroup (one wavefront) takes a
-// block of set-A molecules, one atom per lane (at most 64 atoms, block
-// boundaries blk[b] .. blk[b + 1]), against kJB molecules of set B, whose
-// atoms it walks one at a time, so every set-B value is the same for all
-// lanes.  Each lane forms atomic_distl2 of its atom a against the current
-// set-B atom b with the reference's loops -- m1 over a's neighbours, m2 over
-// b's, (cp, sp) products over the elements p1 of a (ascending) and p2 of b
-// (ascending) -- and the same running sums; a's projections are indexed by
-// the data set's element list, GB wide, zero for elements a does not have,
-// which adds exact zeros between the reference's terms.  The lane then adds
-// exp(-l2 / sigma^2) over b in the molecule's atom order; the molecule's
-// value is the sum of its lanes in atom order.  Symmetric: only pairs
-// i >= j, written to both halves.  GB is the smallest of {4, 6, 8, 16} that
-// holds the data set's element count (the other instantiations return).
+// block of set-A molecules, one atom per lane (at most 64 atoms: molecules
+// ord[blk[b]] .. ord[blk[b + 1] - 1], ord the identity when NULL), against
+// kJB molecules of set B, whose atoms it walks one at a time, so every
+// set-B value is the same for all lanes.  Each lane forms atomic_distl2 of
+// its atom against the current set-B atom (atom_pair), then adds
+// exp(-l2 / sigma^2) over the set-B atoms in the molecule's atom order; the
+// molecule pair's value is the sum of its lanes in atom order.  Symmetric:
+// only pairs i >= j, written to both halves.  GB is the smallest of
+// {4, 5, 6, 8, 16} that holds the data set's element count (the other
+// instantiations return at once).
 template <int GB>
 __global__ __launch_bounds__(64) void k_aras_tiles(ArasSet A, ArasUniform B, int nm2, ArasParams P,
-                                                   const int32_t *__restrict__ blk, int nblk,
+                                                   const int32_t *__restrict__ blk,
+                                                   const int32_t *__restrict__ ord, int nblk,
                                                    int per_blk, int sym, int g_lo,
                                                    const ArasElems *__restrict__ E,
                                                    double *__restrict__ out,
@@ -441,18 +544,19 @@ __global__ __launch_bounds__(64) void k_aras_tiles(ArasSet A, ArasUniform B, int
   if (G > GB || G <= g_lo) return;
   __shared__ double pdg[(kMaxTypes + 1) * (kMaxTypes + 1)];
   __shared__ double red[64 * kMaxSig];
-  __shared__ int lane_mol[64], lane_atom[64], natoms_s;
+  __shared__ int lane_mol[64], lane_atom[64], natoms_s, maxmol_s;
   const int lane = threadIdx.x;
   const int64_t w = blockIdx.x;
   const int ib = (int)(w % nblk);
   const int j0 = (int)(w / nblk) * kJB;
-  const int m0 = blk ? blk[ib] : ib * per_blk;
-  const int m1e = blk ? blk[ib + 1] : min(A.nm, (ib + 1) * per_blk);
-  if (sym && j0 > m1e - 1) return;   // every pair of the tile has i < j
+  const int b0 = blk ? blk[ib] : ib * per_blk;
+  const int b1 = blk ? blk[ib + 1] : min(A.nm, (ib + 1) * per_blk);
   load_pdg(pdg, P, E);
   if (lane == 0) {
-    int c = 0;
-    for (int m = m0; m < m1e; ++m) {
+    int c = 0, mx = -1;
+    for (int b = b0; b < b1; ++b) {
+      const int m = ord ? ord[b] : b;
+      mx = max(mx, m);
       const int nm_ = mol_atoms(A.n, m, P.amax);
       for (int i1 = 0; i1 < nm_; ++i1) {
         if (c < 64) {
@@ -464,94 +568,60 @@ __global__ __launch_bounds__(64) void k_aras_tiles(ArasSet A, ArasUniform B, int
     }
     if (c > 64 && status) status[0] = 1;
     natoms_s = min(c, 64);
+    maxmol_s = mx;
   }
   __syncthreads();
+  const int maxmol = maxmol_s;
+  if (sym && j0 > maxmol) return;   // every pair of the tile has i < j
   const int amax = P.amax;
   const int G1 = G + 1;
-  const bool valid = lane < natoms_s;
-  const int mi = valid ? lane_mol[lane] : m0;
-  const int64_t x1 = valid ? atom_index(P, mi, lane_atom[lane]) : 0;
-  const int nb1 = valid ? A.nb[x1] : 0;
-  const int g01 = valid ? A.gq[x1 * amax] : G;
-  const double self1 = valid ? A.self[x1] : 0.0;
-  const bool first = valid && lane_atom[lane] == 0;
-  const int ni = valid ? mol_atoms(A.n, mi, amax) : 0;
+  LaneAtom L;
+  L.valid = lane < natoms_s;
+  const int mi = L.valid ? lane_mol[lane] : 0;
+  L.x1 = L.valid ? atom_index(P, mi, lane_atom[lane]) : 0;
+  L.nb1 = L.valid ? A.nb[L.x1] : 0;
+  const int g01 = L.valid ? A.gq[L.x1 * amax] : G;
+  const double self1 = L.valid ? A.self[L.x1] : 0.0;
+  const bool first = L.valid && lane_atom[lane] == 0;
+  const int ni = L.valid ? mol_atoms(A.n, mi, amax) : 0;
+  // elements held by some lane's atom
+  unsigned lm = 0u;
+  if (L.valid)
+    for (int t = 0; t < A.nt[L.x1]; ++t) lm |= 1u << A.tg[L.x1 * kMaxTypes + t];
+  unsigned gmask = 0u;
+#pragma unroll
+  for (int g = 0; g < GB; ++g)
+    if (__ballot((lm >> g) & 1u) != 0) gmask |= 1u << g;
 
   for (int jj = 0; jj < kJB; ++jj) {
     const int j = j0 + jj;
     if (j >= nm2) break;
-    if (sym && j > m1e - 1) break;
+    if (sym && j > maxmol) break;
+    LaneAtom Lj = L;
+    if (sym && mi < j) Lj.valid = false;   // this pair is the other orientation's
     double e[kMaxSig];
 #pragma unroll
     for (int s = 0; s < kMaxSig; ++s) e[s] = 0.0;
     const int nj = mol_atoms(B.n, j, amax);
     for (int j1 = 0; j1 < nj; ++j1) {
       const int64_t x2 = atom_index(P, j, j1);
-      const int nb2 = B.nb[x2], nt2 = B.nt[x2];
-      int tg2[GB];
-#pragma unroll
-      for (int t = 0; t < GB; ++t) tg2[t] = t < nt2 ? B.tg[x2 * kMaxTypes + t] : G;
-      const double *d2p = B.dist + x2 * amax;
-      const int32_t *g2p = B.gq + x2 * amax;
-      const double *k2p = B.ksi + x2 * amax;
-      double aad = pdg[g01 * G1 + g2p[0]];
-      bool alive = valid;
-      for (int m1 = 1;; ++m1) {
-        bool act = alive && m1 < nb1;
-        if (__ballot(act) == 0) break;
-        double d1 = 0.0;
-        if (act) {
-          d1 = A.dist[x1 * amax + m1];
-          if (d1 > P.cut) {   // the reference leaves the m1 loop here
-            alive = false;
-            act = false;
-          }
+      const int g02 = B.gq[x2 * amax];
+      double aad = pdg[g01 * G1 + g02];
+      const int nt2 = B.nt[x2];
+      if constexpr (GB <= 6) {
+        switch (nt2) {
+          case 1: aad = atom_pair<GB, 1>(A, B, P, pdg, G1, Lj, gmask, x2, nt2, aad); break;
+          case 2: aad = atom_pair<GB, 2>(A, B, P, pdg, G1, Lj, gmask, x2, nt2, aad); break;
+          case 3: aad = atom_pair<GB, 3>(A, B, P, pdg, G1, Lj, gmask, x2, nt2, aad); break;
+          case 4: aad = atom_pair<GB, (GB >= 4 ? 4 : 0)>(A, B, P, pdg, G1, Lj, gmask, x2, nt2, aad); break;
+          case 5: aad = atom_pair<GB, (GB >= 5 ? 5 : 0)>(A, B, P, pdg, G1, Lj, gmask, x2, nt2, aad); break;
+          default: aad = atom_pair<GB, 0>(A, B, P, pdg, G1, Lj, gmask, x2, nt2, aad); break;
         }
-        int g1 = G;
-        double k1 = 0.0;
-        double c1[GB], s1[GB];
-#pragma unroll
-        for (int g = 0; g < GB; ++g) c1[g] = s1[g] = 0.0;
-        if (act) {
-          g1 = A.gq[x1 * amax + m1];
-          k1 = A.ksi[x1 * amax + m1];
-          const double *cp = A.csg + 2 * ((x1 * P.order) * amax + m1) * kMaxTypes;
-#pragma unroll
-          for (int g = 0; g < GB; ++g) {
-            c1[g] = cp[2 * g];
-            s1[g] = cp[2 * g + 1];
-          }
-        }
-        for (int m2 = 1; m2 < nb2; ++m2) {
-          const double d2 = d2p[m2];
-          if (d2 > P.cut) break;
-          const double dd = d2 - d1;
-          const double r2 = dd * dd;
-          if (act && r2 < P.maxg2) {
-            const double dw = exp(r2 * P.inv_width) * pdg[g1 * G1 + g2p[m2]];
-            double angular = 0.0 * 0.0;
-            for (int o = 0; o < P.order; ++o) {
-              const double *c2 = B.cs + 2 * ((x2 * P.order + o) * amax + m2) * kMaxTypes;
-              const double *cq = A.csg + 2 * ((x1 * P.order + o) * amax + m1) * kMaxTypes;
-              double temp = 0.0;
-#pragma unroll
-              for (int g = 0; g < GB; ++g) {
-                const double cc = o == 0 ? c1[g] : cq[2 * g];
-                const double ss = o == 0 ? s1[g] : cq[2 * g + 1];
-#pragma unroll
-                for (int t = 0; t < GB; ++t) {
-                  if (t < nt2)
-                    temp = temp + (cc * c2[2 * t] + ss * c2[2 * t + 1]) * pdg[tg2[t] * G1 + g];
-                }
-              }
-              angular = angular + temp * P.s[o];
-            }
-            aad = aad + dw * (k1 * k2p[m2] + angular * P.scale);
-          }
-        }
+      } else {
+        aad = atom_pair<GB, 0>(A, B, P, pdg, G1, Lj, gmask, x2, nt2, aad);
       }
-      if (valid) {
-        double l2 = self1 + B.self[x2] - 2.0 * aad * pdg[g01 * G1 + g2p[0]];
+      if (Lj.valid) {
+        double l2 = self1 + B.self[x2] - 2.0 * aad * pdg[g01 * G1 + g02];
         if (fabs(l2) < 5.0e-12) l2 = 0.0;
 #pragma unroll
         for (int s = 0; s < kMaxSig; ++s)
@@ -621,8 +691,8 @@ extern "C" int fmlh_aras_kernels(int mode, const double *x1, const int32_t *n1,
                                  const int32_t *n2, const int32_t *nn2, int nm2, int amax,
                                  const double *sigmas, int nsig, double t_width, double d_width,
                                  double cut, int order, const double *pd, int emax,
-                                 double angular_scale, const int32_t *blk1, int nblk1,
-                                 double *out, int32_t *status, void *ws, size_t ws_bytes,
+                                 double angular_scale, const int32_t *blk1, const int32_t *ord1,
+                                 int nblk1, double *out, int32_t *status, void *ws, size_t ws_bytes,
                                  void *stream) {
   FMLH_ARG_CHECK(mode == 0 || mode == 1, 1);
   FMLH_ARG_CHECK(nm1 >= 0, 5);
@@ -640,9 +710,9 @@ extern "C" int fmlh_aras_kernels(int mode, const double *x1, const int32_t *n1,
   FMLH_ARG_CHECK(x1 != nullptr && n1 != nullptr, 2);
   FMLH_ARG_CHECK(mode == 1 || (x2 != nullptr && n2 != nullptr), 6);
   FMLH_ARG_CHECK(pd != nullptr, 17);
-  FMLH_ARG_CHECK(blk1 == nullptr || nblk1 >= 1, 21);
-  FMLH_ARG_CHECK(out != nullptr, 22);
-  FMLH_ARG_CHECK(ws != nullptr && ws_bytes >= fmlh_aras_ws_bytes(nm1, nm2, amax, order, mode), 24);
+  FMLH_ARG_CHECK(blk1 == nullptr || nblk1 >= 1, 22);
+  FMLH_ARG_CHECK(out != nullptr, 23);
+  FMLH_ARG_CHECK(ws != nullptr && ws_bytes >= fmlh_aras_ws_bytes(nm1, nm2, amax, order, mode), 25);
 
   // angular normalisation and order weights (faras_kernels.f90:198-207, 84-88)
   const double pi = 3.141592653589793;
@@ -672,19 +742,19 @@ extern "C" int fmlh_aras_kernels(int mode, const double *x1, const int32_t *n1,
 
   WsCarver wc(ws, ws_bytes);
   ArasElems *E = wc.take<ArasElems>(1);
-  if (E == nullptr) return -24;
+  if (E == nullptr) return -25;
   ArasSet A = {}, B = {};
   A.x = x1;
   A.n = n1;
   A.nn = nn1;
   A.nm = nm1;
-  if (!carve_set(wc, A, nm1, amax, order)) return -24;
+  if (!carve_set(wc, A, nm1, amax, order)) return -25;
   if (mode == 0) {
     B.x = x2;
     B.n = n2;
     B.nn = nn2;
     B.nm = nm2;
-    if (!carve_set(wc, B, nm2, amax, order)) return -24;
+    if (!carve_set(wc, B, nm2, amax, order)) return -25;
   } else {
     B = A;
   }
@@ -715,13 +785,15 @@ extern "C" int fmlh_aras_kernels(int mode, const double *x1, const int32_t *n1,
   if (nw > 0x7fffffff) return -5;
   const ArasUniform U = {B.n, B.nb, B.nt, B.tg, B.gq, B.dist, B.ksi, B.cs, B.self};
   timing_begin(st);
-  hipLaunchKernelGGL(k_aras_tiles<4>, dim3((unsigned)nw), dim3(64), 0, st, A, U, nm2, P, blk1, nblk,
+  hipLaunchKernelGGL(k_aras_tiles<4>, dim3((unsigned)nw), dim3(64), 0, st, A, U, nm2, P, blk1, ord1, nblk,
                      per_blk, mode, 0, E, out, status);
-  hipLaunchKernelGGL(k_aras_tiles<6>, dim3((unsigned)nw), dim3(64), 0, st, A, U, nm2, P, blk1, nblk,
+  hipLaunchKernelGGL(k_aras_tiles<5>, dim3((unsigned)nw), dim3(64), 0, st, A, U, nm2, P, blk1, ord1, nblk,
                      per_blk, mode, 4, E, out, status);
-  hipLaunchKernelGGL(k_aras_tiles<8>, dim3((unsigned)nw), dim3(64), 0, st, A, U, nm2, P, blk1, nblk,
+  hipLaunchKernelGGL(k_aras_tiles<6>, dim3((unsigned)nw), dim3(64), 0, st, A, U, nm2, P, blk1, ord1, nblk,
+                     per_blk, mode, 5, E, out, status);
+  hipLaunchKernelGGL(k_aras_tiles<8>, dim3((unsigned)nw), dim3(64), 0, st, A, U, nm2, P, blk1, ord1, nblk,
                      per_blk, mode, 6, E, out, status);
-  hipLaunchKernelGGL(k_aras_tiles<16>, dim3((unsigned)nw), dim3(64), 0, st, A, U, nm2, P, blk1, nblk,
+  hipLaunchKernelGGL(k_aras_tiles<16>, dim3((unsigned)nw), dim3(64), 0, st, A, U, nm2, P, blk1, ord1, nblk,
                      per_blk, mode, 8, E, out, status);
   timing_end(st);
   FMLH_LAUNCH_CHECK("k_aras_tiles");

@@ -70,13 +70,29 @@ def _neighbors(X, n, cut):
     return nn
 
 
-def aras_blocks(n, amax):
-    """Set-1 molecules packed greedily, in order, into blocks of at most 64
-    atoms (one wavefront lane each): device (nblk + 1,) int32 block starts
-    and nblk."""
+def aras_blocks(n, amax, own=None):
+    """Set-1 molecule blocks of at most 64 atoms (one wavefront lane each):
+    device int32 block boundaries (nblk + 1,), device int32 molecule order
+    (or None for the identity) and nblk.  own: (nm, amax) nuclear charge of
+    each atom (descriptor row 1, column 0) -- when given, molecules are
+    ordered by their set of elements, then by size, so that a block's atoms
+    share elements and neighbour counts; the matrix does not depend on the
+    order."""
     from .atomic import plan_blocks
-    b = plan_blocks(np.clip(np.asarray(n, dtype=np.int64), 0, amax), tile=64)
-    return to_device(b, np.int32), int(b.shape[0]) - 1
+    n = np.clip(np.asarray(n, dtype=np.int64), 0, amax)
+    order = None
+    if own is not None and len(n):
+        own = np.asarray(own)
+        mask = np.arange(own.shape[1])[None, :] < n[:, None]
+        elems = np.unique(own[mask])
+        key = np.zeros(len(n), dtype=np.int64)
+        for b, z in enumerate(elems[:62]):
+            key |= np.any((own == z) & mask, axis=1).astype(np.int64) << b
+        order = np.lexsort((n, key)).astype(np.int32)
+        n = n[order]
+    b = plan_blocks(n, tile=64)
+    return (to_device(b, np.int32), None if order is None else to_device(order, np.int32),
+            int(b.shape[0]) - 1)
 
 
 def _aras_call(mode, x1, n1, nn1, x2, n2, nn2, sigmas, t_width, d_width, cut, order, pd, scale):
@@ -104,7 +120,10 @@ def _aras_call(mode, x1, n1, nn1, x2, n2, nn2, sigmas, t_width, d_width, cut, or
     if nm1 == 0 or nm2 == 0 or ns == 0:
         return out
     emax = int(pd.shape[0])
-    blk, nblk = aras_blocks(to_host(n1) if is_device_tensor(n1) else n1, amax)
+    own = None
+    if mode == 0:   # the symmetric form keeps the identity: it needs i >= j inside a block
+        own = to_host(X1[:, :, 1, 0]) if is_device_tensor(X1) else np.asarray(X1)[:, :, 1, 0]
+    blk, ordv, nblk = aras_blocks(to_host(n1) if is_device_tensor(n1) else n1, amax, own)
     lib = _native.load()
     wsb = lib.fmlh_aras_ws_bytes(nm1, nm2, amax, int(order), mode)
     ws = workspace(wsb)
@@ -116,7 +135,7 @@ def _aras_call(mode, x1, n1, nn1, x2, n2, nn2, sigmas, t_width, d_width, cut, or
         tmp = out if len(chunk) == ns else empty((nm2, nm1, len(chunk)), np.float64)
         _native.call("fmlh_aras_kernels", mode, p(X1), p(N1), p(NN1), nm1, p(X2), p(N2), p(NN2), nm2, amax,
                      arr, len(chunk), float(t_width), float(d_width), float(cut), int(order), p(pd), emax,
-                     float(scale), p(blk), nblk, p(tmp), p(status), p(ws), wsb, _native.stream_ptr())
+                     float(scale), p(blk), p(ordv), nblk, p(tmp), p(status), p(ws), wsb, _native.stream_ptr())
         if int(status.item()) != 0:
             raise ValueError("ARAS descriptors hold a charge that is not an element of the PTP table, "
                              "more than 16 distinct elements over both sets, or a molecule larger than amax")

@@ -286,9 +286,12 @@ int fmlh_arad_describe(const double *z, const double *r, const int32_t *offsets,
  * pd: device Fortran (emax, emax) element similarity (gen_pd,
  * kernels.py:68-98, built with r_width and c_width); order 1..8; amax <= 64.
  * out (nm2, nm1, nsig) C order = the reference's Fortran (nsig, nm1, nm2).
- * blk1 (optional, device int32, nblk1 + 1 ascending molecule indices from
- * 0 to nm1): set-1 molecule blocks of at most 64 atoms each, one workgroup
- * row per block (NULL: blocks of 64 / amax molecules).
+ * blk1 (optional, device int32, nblk1 + 1 ascending positions from 0 to
+ * nm1) and ord1 (optional, device int32 permutation of the nm1 set-1
+ * molecules, NULL: the identity): block b holds molecules ord1[blk1[b]] ..
+ * ord1[blk1[b + 1] - 1], at most 64 atoms, one workgroup row per block
+ * (blk1 NULL: blocks of 64 / amax consecutive positions).  Blocks of
+ * molecules with the same elements and similar sizes run fastest.
  * status (may be NULL) (1) int32: 1 when a charge is not an integer in
  * 1..emax (or above 127), the two sets hold more than 16 distinct elements,
  * a block more than 64 atoms, or a molecule more than amax atoms.
@@ -305,8 +308,8 @@ int fmlh_aras_kernels(int mode, const double *x1, const int32_t *n1,
                       const double *sigmas, int nsig, double t_width,
                       double d_width, double cut, int order, const double *pd,
                       int emax, double angular_scale, const int32_t *blk1,
-                      int nblk1, double *out, int32_t *status, void *ws,
-                      size_t ws_bytes, void *stream);
+                      const int32_t *ord1, int nblk1, double *out,
+                      int32_t *status, void *ws, size_t ws_bytes, void *stream);
 
 /* precision: 0 = FP64 throughout (the plain form); 1 = FP32 inner overlap
  * (exp, products and accumulation in float), FP64 self terms, distance

@@ -130,8 +130,9 @@ def test_aras_unsorted_neighbours(gpu):
 
 
 def test_aras_block_lists(gpu):
-    """The C-ABI with the library's own molecule blocks (blk1 NULL) and with
-    greedy 64-atom blocks gives the same matrix, bitwise: a pair's value
+    """The C-ABI with the library's own molecule blocks (blk1 NULL), with
+    greedy 64-atom blocks in input order and with molecules ordered by
+    element set and size gives the same matrix, bitwise: a pair's value
     does not depend on how set-1 molecules are grouped."""
     import ctypes
     from fml_amd import _native
@@ -145,16 +146,16 @@ def test_aras_block_lists(gpu):
     lib = _native.load()
     p = _native.ptr
     res = []
-    for blocks in (None, aras_blocks(n, 14)):
+    for blocks in (None, aras_blocks(n, 14), aras_blocks(n, 14, X[:, :, 1, 0])):
         out = empty((21, 21, 2), np.float64)
         st = empty((1,), np.int32)
         wsb = lib.fmlh_aras_ws_bytes(21, 21, 14, 1, 0)
         ws = workspace(wsb)
-        blk, nblk = (None, 0) if blocks is None else blocks
+        blk, ordv, nblk = (None, None, 0) if blocks is None else blocks
         _native.call("fmlh_aras_kernels", 0, p(dX), p(dn), p(dnn), 21, p(dX), p(dn), p(dnn), 21, 14, sg, 2,
-                     np.pi, 0.2, 5.0, 1, p(pd), int(pd.shape[0]), 0.1, p(blk), nblk, p(out), p(st), p(ws), wsb,
-                     _native.stream_ptr())
+                     np.pi, 0.2, 5.0, 1, p(pd), int(pd.shape[0]), 0.1, p(blk), p(ordv), nblk, p(out), p(st),
+                     p(ws), wsb, _native.stream_ptr())
         assert int(st.item()) == 0
         res.append(out.cpu().numpy())
-    assert np.array_equal(res[0], res[1])
+    assert np.array_equal(res[0], res[1]) and np.array_equal(res[0], res[2])
     _close(np.transpose(res[0], (2, 1, 0)), oracle.kernels_aras(X, X, n, n, [0.9, 3.0]))
