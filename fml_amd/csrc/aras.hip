@@ -402,7 +402,7 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-constexpr int kJB = 2;   // set-B molecules per workgroup
+constexpr int kJB = 1;   // set-B molecules per workgroup
 
 // Set-B tables the tile kernel reads at wave-uniform addresses.
 struct ArasUniform {
@@ -533,7 +533,7 @@ __device__ __forceinline__ double atom_pair(const ArasSet &A, const ArasUniform 
 // {4, 5, 6, 8, 16} that holds the data set's element count (the other
 // instantiations return at once).
 template <int GB>
-__global__ __launch_bounds__(64) void k_aras_tiles(ArasSet A, ArasUniform B, int nm2, ArasParams P,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_aras_tiles(ArasSet A, ArasUniform B, int nm2, ArasParams P,
                                                    const int32_t *__restrict__ blk,
                                                    const int32_t *__restrict__ ord, int nblk,
                                                    int per_blk, int sym, int g_lo,
@@ -599,9 +599,10 @@ __global__ __launch_bounds__(64) void k_aras_tiles(ArasSet A, ArasUniform B, int
     if (sym && j > maxmol) break;
     LaneAtom Lj = L;
     if (sym && mi < j) Lj.valid = false;   // this pair is the other orientation's
-    double e[kMaxSig];
-#pragma unroll
-    for (int s = 0; s < kMaxSig; ++s) e[s] = 0.0;
+    // the lane's sums over the set-B atoms, per sigma, in LDS (registers go
+    // to the atom-pair walk)
+    double *e = red + lane * kMaxSig;
+    for (int s = 0; s < P.nsig; ++s) e[s] = 0.0;
     const int nj = mol_atoms(B.n, j, amax);
     for (int j1 = 0; j1 < nj; ++j1) {
       const int64_t x2 = atom_index(P, j, j1);
@@ -623,13 +624,9 @@ __global__ __launch_bounds__(64) void k_aras_tiles(ArasSet A, ArasUniform B, int
       if (Lj.valid) {
         double l2 = self1 + B.self[x2] - 2.0 * aad * pdg[g01 * G1 + g02];
         if (fabs(l2) < 5.0e-12) l2 = 0.0;
-#pragma unroll
-        for (int s = 0; s < kMaxSig; ++s)
-          if (s < P.nsig) e[s] = e[s] + exp(l2 * P.inv_sig[s]);
+        for (int s = 0; s < P.nsig; ++s) e[s] = e[s] + exp(l2 * P.inv_sig[s]);
       }
     }
-#pragma unroll
-    for (int s = 0; s < kMaxSig; ++s) red[lane * kMaxSig + s] = e[s];
     __syncthreads();
     if (first && (!sym || mi >= j)) {
       for (int s = 0; s < P.nsig; ++s) {
