@@ -8,8 +8,8 @@
 // last), channel 1 their nuclear charges, channel 3 + m the angles between
 // neighbour m and every other neighbour.
 //
-// Launches: k_aras_init; per set k_aras_prep (one wavefront per molecule,
-// atom by atom): the neighbour count nb (distances < cut, self included),
+// Launches: k_aras_init; per set k_aras_prep (one wavefront per atom): the
+// neighbour count nb (distances < cut, self included),
 // the atom's distinct neighbour elements in ascending order, ksi_m =
 // 1/d_m^6 q0 q_m, and per order o and element p the projections
 //   cp[o][j][p] = sum_k (cos(o th) - cos((th + pi) o)) ksi3(j, k),
@@ -85,7 +85,6 @@ struct ArasElems {
   int32_t count;
   int32_t list[kMaxTypes];
   int32_t gidx[128];   // element -> index into list, count for none
-  int32_t unsorted;    // some atom's neighbour distances are not ascending
 };
 
 __device__ __forceinline__ double xval(const ArasSet &S, int amax, int a, int i, int c, int m) {
@@ -114,13 +113,14 @@ __global__ __launch_bounds__(64) void k_aras_prep(ArasSet S, ArasParams P, ArasE
   __shared__ int types[kMaxTypes];
   __shared__ int ntypes_s;
   __shared__ double cacc[kMaxAmax * kMaxTypes], sacc[kMaxAmax * kMaxTypes];
-  const int a = blockIdx.x;
   const int lane = threadIdx.x;
   const int amax = P.amax;
+  const int a = (int)(blockIdx.x / amax), i = (int)(blockIdx.x % amax);
   const int n = mol_atoms(S.n, a, amax);
-  if (lane == 0 && status && (S.n[a] > amax || S.n[a] < 0)) status[0] = 1;
+  if (i == 0 && lane == 0 && status && (S.n[a] > amax || S.n[a] < 0)) status[0] = 1;
+  if (i >= n) return;
   const double pi = 3.141592653589793;
-  for (int i = 0; i < n; ++i) {
+  {
     const int64_t ai = atom_index(P, a, i);
     double d = 1e300;
     double q = 0.0;
@@ -164,11 +164,6 @@ __global__ __launch_bounds__(64) void k_aras_prep(ArasSet S, ArasParams P, ArasE
       const int p0 = (int)q_s[0];
       if ((double)p0 == q_s[0] && p0 >= 1 && p0 <= P.emax && p0 < 128)
         atomicOr(&E->bits[p0 >> 5], 1u << (p0 & 31));
-      for (int m = 1; m < nb; ++m)
-        if (!(d_s[m - 1] <= d_s[m])) {
-          atomicOr(&E->unsorted, 1);
-          break;
-        }
       if (bad && status) status[0] = 1;
       ntypes_s = nt;
       S.nb[ai] = nb;
@@ -319,82 +314,72 @@ __device__ void load_pdg(double *pdg, const ArasParams &P, const ArasElems *E) {
   }
 }
 
-// aras_utils::atomic_distl2 (faras_kernels.f90:7-147) for atom (a1, i1) of
-// set A against atom (a2, i2) of set B, in the reference's order over the
-// neighbour pairs.  The neighbour distances are ascending, and
-// fl((d2 - d1)^2) < (8 d_width)^2 holds on one interval of m2 whose ends only
-// move forward with m1 (fl(d2 - d1) is monotone in both arguments); the
-// walk visits that interval only, the same pairs in the same order.  With
-// band false (distances not ascending somewhere in the data) every m2 is
-// tested, as in the reference.
-__device__ double aras_distl2(const ArasSet &A, int a1, int i1, const ArasSet &B, int a2, int i2,
-                              const ArasParams &P, const double *pdg, int G, bool band) {
-  const int amax = P.amax;
-  const int64_t x1 = atom_index(P, a1, i1), x2 = atom_index(P, a2, i2);
-  const int nb1 = A.nb[x1], nb2 = B.nb[x2];
-  const double *dA = A.dist + x1 * amax;
-  const double *dB = B.dist + x2 * amax;
-  const int32_t *gA = A.gq + x1 * amax;
-  const int32_t *gB = B.gq + x2 * amax;
-  const double *ks1 = A.ksi + x1 * amax;
-  const double *ks2 = B.ksi + x2 * amax;
-  const int G1 = G + 1;
-  const int nt1 = A.nt[x1], nt2 = B.nt[x2];
-  const int32_t *tg1 = A.tg + x1 * kMaxTypes;
-  const int32_t *tg2 = B.tg + x2 * kMaxTypes;
-  double aad = pdg[gA[0] * G1 + gB[0]];
-  int lo = 1;
-  for (int m1 = 1; m1 < nb1; ++m1) {
-    const double d1 = dA[m1];
-    if (d1 > P.cut) break;
-    const int g1 = gA[m1];
-    const double k1 = ks1[m1];
-    while (band && lo < nb2) {
-      const double d2 = dB[lo];
-      const double dd = d2 - d1;
-      if (d2 < d1 && !(dd * dd < P.maxg2))
-        ++lo;
-      else
-        break;
-    }
-    for (int m2 = lo; m2 < nb2; ++m2) {
-      const double d2 = dB[m2];
-      if (d2 > P.cut) break;
-      const double dd = d2 - d1;
-      const double r2 = dd * dd;
-      if (!(r2 < P.maxg2)) {
-        if (band) break;   // past the window's upper end
-        continue;
-      }
-      const double dw = exp(r2 * P.inv_width) * pdg[g1 * G1 + gB[m2]];
-      double angular = 0.0 * 0.0;
-      for (int o = 0; o < P.order; ++o) {
-        const double *c1 = A.cs + 2 * ((x1 * P.order + o) * amax + m1) * kMaxTypes;
-        const double *c2 = B.cs + 2 * ((x2 * P.order + o) * amax + m2) * kMaxTypes;
-        double temp = 0.0;
-        for (int t1 = 0; t1 < nt1; ++t1) {
-          const double cc = c1[2 * t1], ss = c1[2 * t1 + 1];
-          const double *pw = pdg + tg1[t1];   // pd(p2, p1) = pw[g2 (G + 1)]
-          for (int t2 = 0; t2 < nt2; ++t2)
-            temp = temp + (cc * c2[2 * t2] + ss * c2[2 * t2 + 1]) * pw[tg2[t2] * G1];
-        }
-        angular = angular + temp * P.s[o];
-      }
-      aad = aad + dw * (k1 * ks2[m2] + angular * P.scale);
-    }
-  }
-  return aad;
-}
-
-__global__ __launch_bounds__(128) void k_aras_self(ArasSet S, ArasParams P, const ArasElems *E) {
+// aras_utils::atomic_distl2 (faras_kernels.f90:7-147) of every atom with
+// itself, one wavefront per atom.  The reference's loops: m1 and m2 over the
+// neighbours past self, each loop left at its first distance above the cut
+// (the same index mc for both, one list), pairs with (d2 - d1)^2 <
+// (8 d_width)^2 adding dw (ksi_m1 ksi_m2 + scale sum_o s_o sum_{p1} sum_{p2}
+// (cp[p1] cp[p2] + sp[p1] sp[p2]) pd(p2, p1)) in m1-major order.  Lanes
+// evaluate 64 consecutive (m1, m2) pairs of that order at a time (0 for a
+// pair outside the window: adding +0 leaves the sum unchanged), then lane 0
+// adds them in order.
+__global__ __launch_bounds__(64) void k_aras_self(ArasSet S, ArasParams P, const ArasElems *E) {
   __shared__ double pdg[(kMaxTypes + 1) * (kMaxTypes + 1)];
+  __shared__ double term[64];
   load_pdg(pdg, P, E);
   __syncthreads();
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (int64_t)S.nm * P.amax) return;
-  const int a = (int)(t / P.amax), i = (int)(t % P.amax);
-  if (i >= mol_atoms(S.n, a, P.amax)) return;
-  S.self[t] = aras_distl2(S, a, i, S, a, i, P, pdg, E->count, E->unsorted == 0);
+  const int amax = P.amax;
+  const int a = (int)(blockIdx.x / amax), i = (int)(blockIdx.x % amax);
+  if (i >= mol_atoms(S.n, a, amax)) return;
+  const int lane = threadIdx.x;
+  const int G1 = E->count + 1;
+  const int64_t x = atom_index(P, a, i);
+  const int nb = S.nb[x];
+  const double *dx = S.dist + x * amax;
+  const int32_t *gx = S.gq + x * amax;
+  const double *kx = S.ksi + x * amax;
+  const int nt = S.nt[x];
+  const int32_t *tg = S.tg + x * kMaxTypes;
+  // first neighbour past self beyond the cut
+  const bool beyond = lane >= 1 && lane < nb && dx[lane] > P.cut;
+  const uint64_t bb = __ballot(beyond);
+  const int mc = bb ? (int)__ffsll((long long)bb) - 1 : max(nb, 1);
+  const int K = max(mc - 1, 0);
+  double aad = pdg[gx[0] * G1 + gx[0]];
+  for (int base = 0; base < K * K; base += 64) {
+    const int k = base + lane;
+    double v = 0.0;
+    if (k < K * K) {
+      const int m1 = 1 + k / K, m2 = 1 + k % K;
+      const double d1 = dx[m1], d2 = dx[m2];
+      const double dd = d2 - d1;
+      const double r2 = dd * dd;
+      if (r2 < P.maxg2) {
+        const double dw = exp(r2 * P.inv_width) * pdg[gx[m1] * G1 + gx[m2]];
+        double angular = 0.0 * 0.0;
+        for (int o = 0; o < P.order; ++o) {
+          const double *c1 = S.cs + 2 * ((x * P.order + o) * amax + m1) * kMaxTypes;
+          const double *c2 = S.cs + 2 * ((x * P.order + o) * amax + m2) * kMaxTypes;
+          double temp = 0.0;
+          for (int t1 = 0; t1 < nt; ++t1) {
+            const double cc = c1[2 * t1], ss = c1[2 * t1 + 1];
+            for (int t2 = 0; t2 < nt; ++t2)
+              temp = temp + (cc * c2[2 * t2] + ss * c2[2 * t2 + 1]) * pdg[tg[t2] * G1 + tg[t1]];
+          }
+          angular = angular + temp * P.s[o];
+        }
+        v = dw * (kx[m1] * kx[m2] + angular * P.scale);
+      }
+    }
+    term[lane] = v;
+    __syncthreads();
+    if (lane == 0) {
+      const int cnt = min(64, K * K - base);
+      for (int l = 0; l < cnt; ++l) aad = aad + term[l];
+    }
+    __syncthreads();
+  }
+  if (lane == 0) S.self[x] = aad;
 }
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -643,7 +628,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
 __global__ void k_aras_init(int32_t *status, ArasElems *E) {
   if (status) status[0] = 0;
   for (int k = 0; k < 4; ++k) E->bits[k] = 0u;
-  E->unsorted = 0;
 }
 
 // Workspace of one set's tables.
@@ -755,25 +739,26 @@ extern "C" int fmlh_aras_kernels(int mode, const double *x1, const int32_t *n1,
   } else {
     B = A;
   }
+  if ((int64_t)max(nm1, nm2) * amax > 0x7fffffff) return -5;
   const hipStream_t st = as_stream(stream);
   hipLaunchKernelGGL(k_aras_init, dim3(1), dim3(1), 0, st, status, E);
   FMLH_LAUNCH_CHECK("k_aras_init");
-  hipLaunchKernelGGL(k_aras_prep, dim3(nm1), dim3(64), 0, st, A, P, E, status);
+  hipLaunchKernelGGL(k_aras_prep, dim3(nm1 * amax), dim3(64), 0, st, A, P, E, status);
   FMLH_LAUNCH_CHECK("k_aras_prep");
   if (mode == 0) {
-    hipLaunchKernelGGL(k_aras_prep, dim3(nm2), dim3(64), 0, st, B, P, E, status);
+    hipLaunchKernelGGL(k_aras_prep, dim3(nm2 * amax), dim3(64), 0, st, B, P, E, status);
     FMLH_LAUNCH_CHECK("k_aras_prep");
   }
   hipLaunchKernelGGL(k_aras_elems, dim3(1), dim3(1), 0, st, E, status);
   FMLH_LAUNCH_CHECK("k_aras_elems");
   hipLaunchKernelGGL(k_aras_proj, dim3(nm1), dim3(64), 0, st, A, P, E);
   FMLH_LAUNCH_CHECK("k_aras_proj");
-  hipLaunchKernelGGL(k_aras_self, dim3(ceil_div(nm1 * amax, 128)), dim3(128), 0, st, A, P, E);
+  hipLaunchKernelGGL(k_aras_self, dim3(nm1 * amax), dim3(64), 0, st, A, P, E);
   FMLH_LAUNCH_CHECK("k_aras_self");
   if (mode == 0) {
     hipLaunchKernelGGL(k_aras_proj, dim3(nm2), dim3(64), 0, st, B, P, E);
     FMLH_LAUNCH_CHECK("k_aras_proj");
-    hipLaunchKernelGGL(k_aras_self, dim3(ceil_div(nm2 * amax, 128)), dim3(128), 0, st, B, P, E);
+    hipLaunchKernelGGL(k_aras_self, dim3(nm2 * amax), dim3(64), 0, st, B, P, E);
     FMLH_LAUNCH_CHECK("k_aras_self");
   }
   const int per_blk = max(1, 64 / amax);
