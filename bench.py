@@ -240,7 +240,7 @@ def arad_workload(args, big=False, precision=0):
             "molecules": nm * len(sets),
             "host_numpy_s": round(host_describe_s, 3),
             "max_abs_diff_channels_3_4": diff,
-            "note": "fmlh_arad_describe of every set, one wavefront per molecule; channels 0-2 "
+            "note": "fmlh_arad_describe of every set, one wavefront per atom row; channels 0-2 "
                     "bit-identical to the host ARAD.describe"}
 
     return info, step, units_per_step, roofline, cpu_baseline, finish

@@ -1,6 +1,6 @@
 // ARAD descriptors on the device: ARAD.describe (fml/arad.py:70-145, the
 // non-periodic form; Python entry Molecule.generate_arad_descriptor,
-// fml/fml.py:80-86), one wavefront per molecule.
+// fml/fml.py:80-86), one wavefront per atom row (molecule, atom i).
 //
 // For atom i of a molecule with L atoms, every atom k (i itself included)
 // is a neighbour: v_k = R_k - R_i, d_k = sqrt((x^2 + y^2) + z^2), weight
@@ -49,6 +49,7 @@ __global__ __launch_bounds__(64) void k_arad_describe(const double *__restrict__
   __shared__ double trow[kDescMaxAtoms], tcol[kDescMaxAtoms];
   __shared__ int order[kDescMaxAtoms];
   const int mol = blockIdx.x;
+  const int i = blockIdx.y;   // the atom whose row this workgroup writes
   const int lane = threadIdx.x;
   const int a0 = offsets[mol];
   const int L = offsets[mol + 1] - a0;
@@ -63,15 +64,18 @@ __global__ __launch_bounds__(64) void k_arad_describe(const double *__restrict__
     if (!bad) bad = (double)(int)zc != zc || !(ptp[2 * (int)zc] > 0.0);
   }
   bad = __any(bad);
-  if (status && lane == 0) status[mol] = bad ? 1 : 0;
-  if (lane == 0) n_out[mol] = bad ? 0 : L;
-  for (int a = lane; a < amax; a += 64) {
-    const int zi = (!bad && a < L) ? (int)z[a0 + a] : 0;
-    zt[((int64_t)mol * amax + a) * 2 + 0] = (!bad && a < L) ? ptp[2 * zi] : 0.0;
-    zt[((int64_t)mol * amax + a) * 2 + 1] = (!bad && a < L) ? ptp[2 * zi + 1] : 0.0;
+  if (i == 0) {
+    if (status && lane == 0) status[mol] = bad ? 1 : 0;
+    if (lane == 0) n_out[mol] = bad ? 0 : L;
+    for (int a = lane; a < amax; a += 64) {
+      const int zi = (!bad && a < L) ? (int)z[a0 + a] : 0;
+      zt[((int64_t)mol * amax + a) * 2 + 0] = (!bad && a < L) ? ptp[2 * zi] : 0.0;
+      zt[((int64_t)mol * amax + a) * 2 + 1] = (!bad && a < L) ? ptp[2 * zi + 1] : 0.0;
+    }
   }
-  if (bad) {  // an all-padding block, flagged in status
-    for (int64_t t = lane; t < (int64_t)amax * blk; t += 64) qm[t] = ((t % blk) < amax) ? 1e100 : 0.0;
+  double *Mi = qm + (int64_t)i * blk;
+  if (bad || i >= L) {  // a padding row (all rows of a molecule flagged in status)
+    for (int t = lane; t < (int)blk; t += 64) Mi[t] = t < amax ? 1e100 : 0.0;
     return;
   }
   if (lane < L) {
@@ -85,18 +89,7 @@ __global__ __launch_bounds__(64) void k_arad_describe(const double *__restrict__
   __syncthreads();
   const double eps = 100.0 * DBL_EPSILON;
   const double pi = 3.141592653589793;
-  for (int i = 0; i < amax; ++i) {
-    double *Mi = qm + (int64_t)i * blk;
-    if (i >= L) {
-      for (int p = lane; p < amax; p += 64) {
-        Mi[p] = 1e100;
-        Mi[amax + p] = 0.0;
-        Mi[2 * amax + p] = 0.0;
-        Mi[3 * amax + p] = 0.0;
-        Mi[4 * amax + p] = 0.0;
-      }
-      continue;
-    }
+  {
     double d = 0.0;
     if (lane < L) {
       const double x = xs[lane] - xs[i], y = ys[lane] - ys[i], w = zs[lane] - zs[i];
@@ -187,7 +180,6 @@ __global__ __launch_bounds__(64) void k_arad_describe(const double *__restrict__
       Mi[3 * amax + p] = kept ? sc / W : 0.0;
       Mi[4 * amax + p] = kept ? ss / W : 0.0;
     }
-    __syncthreads();  // the next atom overwrites vx/dist/order
   }
 }
 
@@ -205,7 +197,7 @@ extern "C" int fmlh_arad_describe(const double *z, const double *r, const int32_
   FMLH_ARG_CHECK(z && r && offsets, 1);
   FMLH_ARG_CHECK(ptp != nullptr, 7);
   FMLH_ARG_CHECK(q && zt && n, 9);
-  hipLaunchKernelGGL(k_arad_describe, dim3(nmol), dim3(64), 0, as_stream(stream), z, r, offsets,
+  hipLaunchKernelGGL(k_arad_describe, dim3(nmol, amax), dim3(64), 0, as_stream(stream), z, r, offsets,
                      nmol, amax, cut, ptp, ptp_rows, q, zt, n, status);
   FMLH_LAUNCH_CHECK("k_arad_describe");
   return 0;
