@@ -70,14 +70,14 @@ def _neighbors(X, n, cut):
     return nn
 
 
-def aras_blocks(n, amax, own=None):
-    """Set-1 molecule blocks of at most 64 atoms (one wavefront lane each):
-    device int32 block boundaries (nblk + 1,), device int32 molecule order
-    (or None for the identity) and nblk.  own: (nm, amax) nuclear charge of
-    each atom (descriptor row 1, column 0) -- when given, molecules are
-    ordered by their set of elements, then by size, so that a block's atoms
-    share elements and neighbour counts; the matrix does not depend on the
-    order."""
+def aras_block_plan(n, amax, own=None):
+    """Set-1 molecule blocks of at most 64 atoms (one wavefront lane each),
+    on the host: (nblk + 1,) int32 block boundaries into the molecule order
+    and that order ((nm,) int32, or None for the identity).  own: (nm, amax)
+    nuclear charge of each atom (descriptor row 1, column 0) -- when given,
+    molecules are ordered by their set of elements, then by size, so that a
+    block's atoms share elements and neighbour counts; the matrix does not
+    depend on the order."""
     from .atomic import plan_blocks
     n = np.clip(np.asarray(n, dtype=np.int64), 0, amax)
     order = None
@@ -90,7 +90,12 @@ def aras_blocks(n, amax, own=None):
             key |= np.any((own == z) & mask, axis=1).astype(np.int64) << b
         order = np.lexsort((n, key)).astype(np.int32)
         n = n[order]
-    b = plan_blocks(n, tile=64)
+    return plan_blocks(n, tile=64), order
+
+
+def aras_blocks(n, amax, own=None):
+    """aras_block_plan on the device: (blk, order or None, nblk)."""
+    b, order = aras_block_plan(n, amax, own)
     return (to_device(b, np.int32), None if order is None else to_device(order, np.int32),
             int(b.shape[0]) - 1)
 

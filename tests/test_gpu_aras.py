@@ -159,3 +159,22 @@ def test_aras_block_lists(gpu):
         res.append(out.cpu().numpy())
     assert np.array_equal(res[0], res[1]) and np.array_equal(res[0], res[2])
     _close(np.transpose(res[0], (2, 1, 0)), oracle.kernels_aras(X, X, n, n, [0.9, 3.0]))
+
+
+@pytest.mark.parametrize("elements", [[1, 6, 7, 8, 9, 16], [1, 6, 7, 8, 9, 15, 16, 17],
+                                      [1, 5, 6, 7, 8, 9, 14, 15, 16, 17, 35]])
+def test_aras_many_elements(gpu, elements):
+    """Data sets of 6, 8 and 11 elements (the 6-, 8- and 16-wide element
+    lists, atoms with more than five neighbour elements) against the oracle,
+    global and symmetric."""
+    from fml_amd.kernels import get_atomic_kernels_aras, get_atomic_symmetric_kernels_aras
+    rng = np.random.default_rng(len(elements))
+    ms = synth.generate(14, seed=56, nmin=6, nmax=16)
+    ar = ARAS(maxMolSize=16, maxAts=16)
+    Z = [rng.choice(elements, size=int(k)).tolist() for k in ms.natoms]
+    X = np.array([ar.describe(ms.molecule(i)[1], Z[i]) for i in range(ms.nmol)])
+    n = ms.natoms
+    sg = [0.7, 4.0]
+    _close(get_atomic_kernels_aras(X[:9], X[4:], Z[:9], Z[4:], sg),
+           oracle.kernels_aras(X[:9], X[4:], n[:9], n[4:], sg))
+    _close(get_atomic_symmetric_kernels_aras(X, Z, sg), oracle.kernels_aras(X, None, n, None, sg))

@@ -129,3 +129,33 @@ def test_aras_argument_checks():
     X = np.zeros((2, 3, 6, 3))
     with pytest.raises(ValueError):
         get_atomic_kernels_aras(X, np.zeros((2, 4, 7, 4)), [[1, 1], [6]], [[1], [8]], [1.0])
+
+
+def test_aras_block_plan():
+    """Set-1 blocks for the ARAS tiles: at most 64 atoms, every molecule in
+    exactly one block, molecules grouped by element set, then by size."""
+    from fml_amd.kernels.aras_kernels import aras_block_plan
+    rng = np.random.default_rng(3)
+    nm, amax = 200, 23
+    n = rng.integers(1, amax + 1, size=nm)
+    own = np.zeros((nm, amax))
+    sets = [[1, 6], [1, 6, 8], [1, 6, 7, 8]]
+    choice = rng.integers(0, 3, size=nm)
+    for m in range(nm):
+        own[m, :n[m]] = rng.choice(sets[choice[m]], size=n[m])
+        own[m, 0] = 1
+        own[m, n[m]:] = 99.0          # padding past the atom count is ignored
+    for o in (None, own):
+        blk, order = aras_block_plan(n, amax, o)
+        order = np.arange(nm) if order is None else order
+        assert sorted(order.tolist()) == list(range(nm))
+        assert blk[0] == 0 and blk[-1] == nm and np.all(np.diff(blk) >= 1)
+        for b in range(len(blk) - 1):
+            assert n[order[blk[b]:blk[b + 1]]].sum() <= 64
+    blk, order = aras_block_plan(n, amax, own)
+    key = [frozenset(own[m, :n[m]].astype(int).tolist()) for m in order]
+    runs = [k for i, k in enumerate(key) if i == 0 or k != key[i - 1]]
+    assert len(runs) == len(set(key))                       # each element set is one run
+    for k in set(key):
+        sizes = [n[m] for m, kk in zip(order, key) if kk == k]
+        assert sizes == sorted(sizes)
