@@ -10,13 +10,15 @@
 // (the strictly lower part of the caller's array) is never touched.
 //
 // Factorisation (right-looking, two block levels):
-//   for each 256-column outer panel:
+//   for each 1024-column outer panel:
 //     for each 64-column inner block inside it:
 //       k_potf2   diagonal 64x64 block in registers (one wave, lane = row)
 //       k_trsm    rows below: M[r][k] <- M[r][k] L_kk^{-T} (one thread/row)
 //       k_update  panel columns right of the block: C -= P Q^T, 64 tiles
-//     k_update    trailing matrix right of the outer panel, K = 256, on
-//                 128x128 MFMA tiles over its lower triangle only
+//     k_update    the next outer panel's columns, K = 1024, 64 tiles; then
+//                 the rest of the trailing matrix on 128x128 MFMA tiles over
+//                 its lower triangle only, while the next panel is factored
+//                 on an auxiliary stream (one panel of look-ahead)
 // A non-positive (or NaN) pivot sets *info to its 1-based index, as LAPACK;
 // every later kernel sees *info != 0 and returns, so the factor is left
 // undefined but no kernel waits on anything.
@@ -27,6 +29,8 @@
 // multi-workgroup partial product of the already solved part followed by a
 // one-workgroup triangular solve of the block.
 #include <math.h>
+
+#include <mutex>
 
 #include "common.h"
 #include "gemm_nt.h"
@@ -192,43 +196,138 @@ __global__ __launch_bounds__(256, (MT == 128 ? 2 : 1)) void k_update(T *__restri
 
 __global__ void k_set_int_impl(int32_t *p, int v) { *p = v; }
 
+// The inner blocks of outer panel [K0, kend): each 64-column block is
+// factored (k_potf2), the rows below it solved (k_trsm) and the rest of the
+// panel's columns updated with it (k_update, 64 tiles).
+template <typename T>
+int panel_factor(T *a, int64_t lda, int n, int K0, int kend, int32_t *info, hipStream_t st) {
+  for (int k0 = K0; k0 < kend; k0 += kNB) {
+    const int b = min(kNB, kend - k0);
+    hipLaunchKernelGGL(k_potf2<T>, dim3(1), dim3(64), 0, st, a, lda, k0, b, info);
+    FMLH_LAUNCH_CHECK("k_potf2");
+    const int r0 = k0 + b;
+    if (r0 < n) {
+      hipLaunchKernelGGL(k_trsm<T>, dim3(ceil_div(n - r0, 256)), dim3(256), 0, st, a, lda, n, k0, b,
+                         r0, info);
+      FMLH_LAUNCH_CHECK("k_trsm");
+      if (r0 < kend) {  // rest of the outer panel
+        const int gx = ceil_div(kend - r0, kTile), gy = ceil_div(n - r0, kTile);
+        hipLaunchKernelGGL((k_update<T, kTile, false>), dim3((unsigned)xcd_grid(gx, gy)), dim3(256),
+                           0, st, a, lda, r0, n, r0, kend, k0, k0 + b, gx, gy, info);
+        FMLH_LAUNCH_CHECK("k_update(panel)");
+      }
+    }
+  }
+  return 0;
+}
+
+// C -= P P^T over the lower triangle of the square [r0, n)^2, P the columns
+// [kb, ke) of those rows.
+template <typename T>
+int trailing_update(T *a, int64_t lda, int n, int r0, int kb, int ke, int32_t *info, hipStream_t st) {
+  const int m = n - r0;
+  if (m <= 0) return 0;
+  if (m >= 4 * 128) {
+    const int g = ceil_div(m, 128);
+    const int64_t t = (int64_t)g * (g + 1) / 2;
+    hipLaunchKernelGGL((k_update<T, 128, true>), dim3((unsigned)xcd_grid((int)t, 1)), dim3(256), 0,
+                       st, a, lda, r0, n, r0, n, kb, ke, g, g, info);
+  } else {
+    const int g = ceil_div(m, kTile);
+    const int64_t t = (int64_t)g * (g + 1) / 2;
+    hipLaunchKernelGGL((k_update<T, kTile, true>), dim3((unsigned)xcd_grid((int)t, 1)), dim3(256),
+                       0, st, a, lda, r0, n, r0, n, kb, ke, g, g, info);
+  }
+  FMLH_LAUNCH_CHECK("k_update(trailing)");
+  return 0;
+}
+
+// One auxiliary stream (non-blocking, default priority) and two events per
+// device, created on first use, for the factorisation's look-ahead.
+struct Aux {
+  hipStream_t s = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
+};
+
+bool aux_for_device(Aux &out) {
+  static std::mutex mu;
+  static Aux table[64];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) {
+    (void)hipGetLastError();
+    return false;
+  }
+  std::lock_guard<std::mutex> lk(mu);
+  Aux &x = table[dev];
+  if (x.s == nullptr) {
+    Aux y;
+    if (hipStreamCreateWithFlags(&y.s, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&y.fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&y.join, hipEventDisableTiming) != hipSuccess) {
+      (void)hipGetLastError();
+      return false;
+    }
+    x = y;
+  }
+  out = x;
+  return true;
+}
+
+// Right-looking factorisation over 1024-column outer panels with one panel
+// of look-ahead: after panel p is factored, the columns of panel p + 1 are
+// updated with it first; panel p + 1 is then factored on the auxiliary
+// stream while the caller's stream updates the rest of the trailing matrix
+// (disjoint columns), and the next update waits for that factor.  Inside a
+// graph capture, or without the auxiliary stream, everything runs on the
+// caller's stream in the same order.
 template <typename T>
 int potrf_impl(T *a, int64_t lda, int n, int32_t *info, hipStream_t st) {
   hipLaunchKernelGGL(k_set_int_impl, dim3(1), dim3(1), 0, st, info, 0);
   FMLH_LAUNCH_CHECK("k_set_int");
+  Aux aux;
+  bool ahead = true;
+  {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) {
+      (void)hipGetLastError();
+      ahead = false;   // nothing is created or forked while a graph is captured
+    }
+  }
+  if (ahead) ahead = aux_for_device(aux);
+  int rc = panel_factor(a, lda, n, 0, min(kNBO, n), info, st);
+  if (rc) return rc;
   for (int K0 = 0; K0 < n; K0 += kNBO) {
     const int kend = min(K0 + kNBO, n);
-    for (int k0 = K0; k0 < kend; k0 += kNB) {
-      const int b = min(kNB, kend - k0);
-      hipLaunchKernelGGL(k_potf2<T>, dim3(1), dim3(64), 0, st, a, lda, k0, b, info);
-      FMLH_LAUNCH_CHECK("k_potf2");
-      const int r0 = k0 + b;
-      if (r0 < n) {
-        hipLaunchKernelGGL(k_trsm<T>, dim3(ceil_div(n - r0, 256)), dim3(256), 0, st, a, lda, n, k0,
-                           b, r0, info);
-        FMLH_LAUNCH_CHECK("k_trsm");
-        if (r0 < kend) {  // rest of the outer panel
-          const int gx = ceil_div(kend - r0, kTile), gy = ceil_div(n - r0, kTile);
-          hipLaunchKernelGGL((k_update<T, kTile, false>), dim3((unsigned)xcd_grid(gx, gy)),
-                             dim3(256), 0, st, a, lda, r0, n, r0, kend, k0, k0 + b, gx, gy, info);
-          FMLH_LAUNCH_CHECK("k_update(panel)");
-        }
-      }
+    if (kend >= n) break;
+    const int nend = min(kend + kNBO, n);  // the next panel is [kend, nend)
+    // the next panel's columns, all rows below kend, with this panel
+    if (n - kend >= 4 * 128) {
+      const int gx = ceil_div(nend - kend, 128), gy = ceil_div(n - kend, 128);
+      hipLaunchKernelGGL((k_update<T, 128, false>), dim3((unsigned)xcd_grid(gx, gy)), dim3(256), 0,
+                         st, a, lda, kend, n, kend, nend, K0, kend, gx, gy, info);
+    } else {
+      const int gx = ceil_div(nend - kend, kTile), gy = ceil_div(n - kend, kTile);
+      hipLaunchKernelGGL((k_update<T, kTile, false>), dim3((unsigned)xcd_grid(gx, gy)), dim3(256), 0,
+                         st, a, lda, kend, n, kend, nend, K0, kend, gx, gy, info);
     }
-    if (kend < n) {  // trailing matrix: lower-triangle tiles only
-      const int m = n - kend;
-      if (m >= 4 * 128) {
-        const int g = ceil_div(m, 128);
-        const int64_t t = (int64_t)g * (g + 1) / 2;
-        hipLaunchKernelGGL((k_update<T, 128, true>), dim3((unsigned)xcd_grid((int)t, 1)), dim3(256),
-                           0, st, a, lda, kend, n, kend, n, K0, kend, g, g, info);
-      } else {
-        const int g = ceil_div(m, kTile);
-        const int64_t t = (int64_t)g * (g + 1) / 2;
-        hipLaunchKernelGGL((k_update<T, kTile, true>), dim3((unsigned)xcd_grid((int)t, 1)),
-                           dim3(256), 0, st, a, lda, kend, n, kend, n, K0, kend, g, g, info);
-      }
-      FMLH_LAUNCH_CHECK("k_update(trailing)");
+    FMLH_LAUNCH_CHECK("k_update(next panel)");
+    if (ahead) {
+      hipError_t e = hipEventRecord(aux.fork, st);
+      if (e == hipSuccess) e = hipStreamWaitEvent(aux.s, aux.fork, 0);
+      if (e != hipSuccess) return record_error(e, "potrf fork");
+      rc = panel_factor(a, lda, n, kend, nend, info, aux.s);
+      if (rc) return rc;
+      e = hipEventRecord(aux.join, aux.s);
+      if (e != hipSuccess) return record_error(e, "potrf join record");
+      rc = trailing_update(a, lda, n, nend, K0, kend, info, st);
+      if (rc) return rc;
+      e = hipStreamWaitEvent(st, aux.join, 0);
+      if (e != hipSuccess) return record_error(e, "potrf join");
+    } else {
+      rc = panel_factor(a, lda, n, kend, nend, info, st);
+      if (rc) return rc;
+      rc = trailing_update(a, lda, n, nend, K0, kend, info, st);
+      if (rc) return rc;
     }
   }
   return 0;
