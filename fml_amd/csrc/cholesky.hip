@@ -161,7 +161,7 @@ __device__ __forceinline__ bool tri_tile(int g, int64_t local, int &tx, int &ty)
 // in XCD-contiguous runs); otherwise a rectangular gx x gy grid in which
 // tiles above the diagonal exit.
 template <typename T, int MT, bool kTri>
-__global__ __launch_bounds__(256, (MT == 128 ? 2 : 1)) void k_update(T *__restrict__ a, int64_t lda, int r0, int r1,
+__global__ __launch_bounds__(MT == 128 ? 512 : 256, (MT == 128 ? 2 : 1)) void k_update(T *__restrict__ a, int64_t lda, int r0, int r1,
                                                 int c0, int c1, int kb, int ke, int gx, int gy,
                                                 const int32_t *__restrict__ info) {
   __shared__ TileSmem<T, MT> s;
@@ -178,17 +178,17 @@ __global__ __launch_bounds__(256, (MT == 128 ? 2 : 1)) void k_update(T *__restri
   const int tr = r0 + ty * MT;
   const int tc = c0 + tx * MT;
   if (tc > tr + MT - 1) return;  // tile entirely above the diagonal
-  TileAcc<T, MT> acc;
-  tile_nt<T, false, MT>(a + (int64_t)tr * lda + kb, lda, r1 - tr, a + (int64_t)tc * lda + kb, lda,
-                        c1 - tc, ke - kb, s, acc);
-  constexpr int NB = MT / 32;
+  constexpr int WR = MT == 128 ? 4 : 2;
+  TileAcc<T, MT, WR> acc;
+  tile_nt<T, false, MT, WR>(a + (int64_t)tr * lda + kb, lda, r1 - tr, a + (int64_t)tc * lda + kb,
+                            lda, c1 - tc, ke - kb, s, acc);
 #pragma unroll
-  for (int im = 0; im < NB; ++im)
+  for (int im = 0; im < TileAcc<T, MT, WR>::NBM; ++im)
 #pragma unroll
-    for (int in = 0; in < NB; ++in)
+    for (int in = 0; in < TileAcc<T, MT, WR>::NBN; ++in)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int gr = tr + tile_row<T, MT>(im, r);
+        const int gr = tr + tile_row<T, MT, WR>(im, r);
         const int gc = tc + tile_col<MT>(in);
         if (gr < r1 && gc < c1 && gc <= gr) a[(int64_t)gr * lda + gc] -= acc.v[im][in][r];
       }
@@ -230,7 +230,7 @@ int trailing_update(T *a, int64_t lda, int n, int r0, int kb, int ke, int32_t *i
   if (m >= 4 * 128) {
     const int g = ceil_div(m, 128);
     const int64_t t = (int64_t)g * (g + 1) / 2;
-    hipLaunchKernelGGL((k_update<T, 128, true>), dim3((unsigned)xcd_grid((int)t, 1)), dim3(256), 0,
+    hipLaunchKernelGGL((k_update<T, 128, true>), dim3((unsigned)xcd_grid((int)t, 1)), dim3(512), 0,
                        st, a, lda, r0, n, r0, n, kb, ke, g, g, info);
   } else {
     const int g = ceil_div(m, kTile);
@@ -303,7 +303,7 @@ int potrf_impl(T *a, int64_t lda, int n, int32_t *info, hipStream_t st) {
     // the next panel's columns, all rows below kend, with this panel
     if (n - kend >= 4 * 128) {
       const int gx = ceil_div(nend - kend, 128), gy = ceil_div(n - kend, 128);
-      hipLaunchKernelGGL((k_update<T, 128, false>), dim3((unsigned)xcd_grid(gx, gy)), dim3(256), 0,
+      hipLaunchKernelGGL((k_update<T, 128, false>), dim3((unsigned)xcd_grid(gx, gy)), dim3(512), 0,
                          st, a, lda, kend, n, kend, nend, K0, kend, gx, gy, info);
     } else {
       const int gx = ceil_div(nend - kend, kTile), gy = ceil_div(n - kend, kTile);
@@ -409,7 +409,7 @@ __global__ __launch_bounds__(256) void k_potri_diag(const T *__restrict__ a, int
 // M[r][c] = sum_{k >= tile row start} XT[r][k] XT[c][k] for c <= r < n,
 // lower-triangle tiles in tri_tile order (XT[r][k] = 0 for k < r).
 template <typename T, int MT>
-__global__ __launch_bounds__(256, (MT == 128 ? 2 : 1)) void k_potri_syrk(
+__global__ __launch_bounds__(MT == 128 ? 512 : 256, (MT == 128 ? 2 : 1)) void k_potri_syrk(
     const T *__restrict__ xt, int64_t ldx, int n, int g, T *__restrict__ a, int64_t lda,
     const int32_t *__restrict__ info) {
   __shared__ TileSmem<T, MT> s;
@@ -420,17 +420,17 @@ __global__ __launch_bounds__(256, (MT == 128 ? 2 : 1)) void k_potri_syrk(
   const int64_t L = blockIdx.x;
   if (!tri_tile(g, (L % kXcds) * per + L / kXcds, tx, ty)) return;
   const int tr = ty * MT, tc = tx * MT;
-  TileAcc<T, MT> acc;
-  tile_nt<T, false, MT>(xt + (int64_t)tr * ldx + tr, ldx, n - tr, xt + (int64_t)tc * ldx + tr, ldx,
-                        n - tc, n - tr, s, acc);
-  constexpr int NB = MT / 32;
+  constexpr int WR = MT == 128 ? 4 : 2;
+  TileAcc<T, MT, WR> acc;
+  tile_nt<T, false, MT, WR>(xt + (int64_t)tr * ldx + tr, ldx, n - tr, xt + (int64_t)tc * ldx + tr,
+                            ldx, n - tc, n - tr, s, acc);
 #pragma unroll
-  for (int im = 0; im < NB; ++im)
+  for (int im = 0; im < TileAcc<T, MT, WR>::NBM; ++im)
 #pragma unroll
-    for (int in = 0; in < NB; ++in)
+    for (int in = 0; in < TileAcc<T, MT, WR>::NBN; ++in)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int gr = tr + tile_row<T, MT>(im, r);
+        const int gr = tr + tile_row<T, MT, WR>(im, r);
         const int gc = tc + tile_col<MT>(in);
         if (gr < n && gc <= gr) a[(int64_t)gr * lda + gc] = acc.v[im][in][r];
       }
@@ -462,7 +462,7 @@ int potri_impl(double *a, int64_t lda, int n, int32_t *info, double *xt, hipStre
   if (n >= 4 * 128) {
     const int g = ceil_div(n, 128);
     const int64_t t = (int64_t)g * (g + 1) / 2;
-    hipLaunchKernelGGL((k_potri_syrk<double, 128>), dim3((unsigned)xcd_grid((int)t, 1)), dim3(256), 0,
+    hipLaunchKernelGGL((k_potri_syrk<double, 128>), dim3((unsigned)xcd_grid((int)t, 1)), dim3(512), 0,
                        st, xt, ldx, n, g, a, lda, info);
   } else {
     const int g = ceil_div(n, kTile);

@@ -39,7 +39,7 @@ __device__ __forceinline__ void kstore(O *p, double v) {
 // entry does not depend on MT: the dot product runs over k in the same
 // 4-wide MFMA steps and the row norms are sequential fused sums.
 template <typename T, typename OutT, int MT>
-__global__ __launch_bounds__(256, (MT == 128 ? 2 : 1)) void k_gaussian(const T *__restrict__ A, int64_t lda, int na,
+__global__ __launch_bounds__(MT == 128 ? 512 : 256, (MT == 128 ? 2 : 1)) void k_gaussian(const T *__restrict__ A, int64_t lda, int na,
                                                   const T *__restrict__ B, int64_t ldb, int nb,
                                                   int d, double inv, OutT *__restrict__ K,
                                                   int64_t ldk, T tau, int gx, int gy) {
@@ -48,17 +48,17 @@ __global__ __launch_bounds__(256, (MT == 128 ? 2 : 1)) void k_gaussian(const T *
   if (!xcd_tile(gx, gy, tx, ty)) return;
   const int n0 = tx * MT;
   const int m0 = ty * MT;
-  TileAcc<T, MT> acc;
-  tile_nt<T, true, MT>(B + (int64_t)m0 * ldb, ldb, nb - m0, A + (int64_t)n0 * lda, lda, na - n0, d,
-                       s, acc);
-  constexpr int NB = MT / 32;
+  constexpr int WR = MT == 128 ? 4 : 2;
+  TileAcc<T, MT, WR> acc;
+  tile_nt<T, true, MT, WR>(B + (int64_t)m0 * ldb, ldb, nb - m0, A + (int64_t)n0 * lda, lda, na - n0,
+                           d, s, acc);
 #pragma unroll
-  for (int im = 0; im < NB; ++im)
+  for (int im = 0; im < TileAcc<T, MT, WR>::NBM; ++im)
 #pragma unroll
-    for (int in = 0; in < NB; ++in)
+    for (int in = 0; in < TileAcc<T, MT, WR>::NBN; ++in)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int m = tile_row<T, MT>(im, r);
+        const int m = tile_row<T, MT, WR>(im, r);
         const int n = tile_col<MT>(in);
         const int gm = m0 + m, gn = n0 + n;
         const bool valid = gm < nb && gn < na;
@@ -90,7 +90,7 @@ void launch_gaussian(const T *a, int64_t lda, int na, const T *b, int64_t ldb, i
                      double inv, OutT *k, int64_t ldk, T tau, hipStream_t st) {
   if (gauss_tile(na, nb) == 128) {
     const int gx = ceil_div(na, 128), gy = ceil_div(nb, 128);
-    hipLaunchKernelGGL((k_gaussian<T, OutT, 128>), dim3((unsigned)xcd_grid(gx, gy)), dim3(256), 0,
+    hipLaunchKernelGGL((k_gaussian<T, OutT, 128>), dim3((unsigned)xcd_grid(gx, gy)), dim3(512), 0,
                        st, a, lda, na, b, ldb, nb, d, inv, k, ldk, tau, gx, gy);
   } else {
     const int gx = ceil_div(na, kTile), gy = ceil_div(nb, kTile);

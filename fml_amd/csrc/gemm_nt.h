@@ -64,21 +64,25 @@ struct TileSmem {
   T qn[MT];  // optional squared row norms of the Q tile
 };
 
-// Result of one tile: v[im][in][r] is C at m = tile_row<T, MT>(im, r),
-// n = tile_col<MT>(in).
-template <typename T, int MT>
+// Result of one tile: v[im][in][r] is C at m = tile_row<T, MT, WR>(im, r),
+// n = tile_col<MT>(in).  WR: wave rows (2: 2 x 2 waves, 256 threads, each
+// wave (MT/2) x (MT/2); 4: 4 x 2 waves, 512 threads, each wave
+// (MT/4) x (MT/2) -- half the accumulator registers per wave, so twice the
+// waves per SIMD).
+template <typename T, int MT, int WR = 2>
 struct TileAcc {
-  static constexpr int NB = MT / 32;
-  typename MfmaOp<T>::acc_t v[NB][NB];
+  static constexpr int NBM = MT / (16 * WR);
+  static constexpr int NBN = MT / 32;
+  typename MfmaOp<T>::acc_t v[NBM][NBN];
 };
 
 __device__ __forceinline__ int tile_lane() { return threadIdx.x & 63; }
 __device__ __forceinline__ int tile_wm() { return (threadIdx.x >> 6) >> 1; }
 __device__ __forceinline__ int tile_wn() { return (threadIdx.x >> 6) & 1; }
-template <typename T, int MT = kTile>
+template <typename T, int MT = kTile, int WR = 2>
 __device__ __forceinline__ int tile_row(int im, int r) {
   const int q = tile_lane() >> 4;
-  return tile_wm() * (MT / 2) + im * 16 + (sizeof(T) == 8 ? q + 4 * r : 4 * q + r);
+  return tile_wm() * (MT / WR) + im * 16 + (sizeof(T) == 8 ? q + 4 * r : 4 * q + r);
 }
 template <int MT = kTile>
 __device__ __forceinline__ int tile_col(int in) {
@@ -90,15 +94,17 @@ __device__ __forceinline__ int tile_col(int in) {
 // columns [0, klen).  Out-of-range rows/columns read as zero.  When kNorms
 // is set, s.pn / s.qn receive sequential fused sums of squares of the staged
 // rows (ascending k), the same for either tile size.
-template <typename T, bool kNorms, int MT = kTile>
+template <typename T, bool kNorms, int MT = kTile, int WR = 2>
 __device__ __forceinline__ void tile_nt(const T *__restrict__ P, int64_t ldp, int mvalid,
                                         const T *__restrict__ Q, int64_t ldq, int nvalid,
-                                        int klen, TileSmem<T, MT> &s, TileAcc<T, MT> &acc) {
+                                        int klen, TileSmem<T, MT> &s, TileAcc<T, MT, WR> &acc) {
   typedef MfmaOp<T> Op;
-  constexpr int NB = MT / 32;           // MFMA blocks per wave and dimension
+  constexpr int NBM = TileAcc<T, MT, WR>::NBM;  // MFMA blocks per wave, rows
+  constexpr int NBN = TileAcc<T, MT, WR>::NBN;  // and columns
+  constexpr int kThreads = 128 * WR;
   constexpr int kBK = TileSmem<T, MT>::BK;
   constexpr int kLdsStride = TileSmem<T, MT>::LS;
-  constexpr int EPT = MT * kBK / 256;   // staged elements per thread and operand
+  constexpr int EPT = MT * kBK / kThreads;  // staged elements per thread and operand
   constexpr int TPR = kBK / EPT;        // threads per staged row
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -107,9 +113,9 @@ __device__ __forceinline__ void tile_nt(const T *__restrict__ P, int64_t ldp, in
   const int lcol = (tid % TPR) * EPT;
 
 #pragma unroll
-  for (int i = 0; i < NB; ++i)
+  for (int i = 0; i < NBM; ++i)
 #pragma unroll
-    for (int j = 0; j < NB; ++j) acc.v[i][j] = Op::zero();
+    for (int j = 0; j < NBN; ++j) acc.v[i][j] = Op::zero();
 
   T pn_acc = T(0), qn_acc = T(0);
   T rp[EPT], rq[EPT];
@@ -146,16 +152,15 @@ __device__ __forceinline__ void tile_nt(const T *__restrict__ P, int64_t ldp, in
 #pragma unroll
     for (int kq = 0; kq < kBK / 4; ++kq) {
       const int kk = kq * 4 + (lane >> 4);
-      T a[NB], b[NB];
+      T a[NBM], b[NBN];
 #pragma unroll
-      for (int i = 0; i < NB; ++i) {
-        a[i] = sp[(wm * (MT / 2) + i * 16 + (lane & 15)) * kLdsStride + kk];
-        b[i] = sq[(wn * (MT / 2) + i * 16 + (lane & 15)) * kLdsStride + kk];
-      }
+      for (int i = 0; i < NBM; ++i) a[i] = sp[(wm * (MT / WR) + i * 16 + (lane & 15)) * kLdsStride + kk];
 #pragma unroll
-      for (int i = 0; i < NB; ++i)
+      for (int j = 0; j < NBN; ++j) b[j] = sq[(wn * (MT / 2) + j * 16 + (lane & 15)) * kLdsStride + kk];
 #pragma unroll
-        for (int j = 0; j < NB; ++j) acc.v[i][j] = Op::mma(a[i], b[j], acc.v[i][j]);
+      for (int i = 0; i < NBM; ++i)
+#pragma unroll
+        for (int j = 0; j < NBN; ++j) acc.v[i][j] = Op::mma(a[i], b[j], acc.v[i][j]);
     }
     if (kNorms) {
       if (tid < MT) {
