@@ -99,36 +99,46 @@ void launch_gaussian(const T *a, int64_t lda, int na, const T *b, int64_t ldb, i
   }
 }
 
-// Laplacian on VALU.  Thread (tx, ty) owns A samples n = tx*4+j and B
-// samples m = ty*4+i of the 64x64 tile.
+// Laplacian on VALU over MT x MT tiles (64 or 128), 256 threads: thread
+// (tx, ty) owns A samples n = tx+16j and B samples m = ty+16i, R = MT/16
+// (strided, so the 16 lanes of a row group read 16 consecutive LDS words),
+// and sums |a - b| over k in ascending order (the same order for either
+// tile).  Operands are staged k-major through two LDS buffers, kLapBK
+// columns at a time.  The launcher uses the 64 tile: the 128 tile (8 x 8
+// outputs per thread) needs 210 registers, runs two waves per SIMD and is
+// no faster.
 constexpr int kLapBK = 16;
-constexpr int kLapStride = 64 + 2;
 
-template <typename T, typename OutT>
-__global__ __launch_bounds__(256) void k_laplacian(const T *__restrict__ A, int64_t lda, int na,
-                                                   const T *__restrict__ B, int64_t ldb, int nb,
-                                                   int d, double inv, OutT *__restrict__ K,
-                                                   int64_t ldk) {
+template <typename T, typename OutT, int MT>
+__global__ __launch_bounds__(256, (MT == 128 ? 2 : 1)) void k_laplacian(
+    const T *__restrict__ A, int64_t lda, int na, const T *__restrict__ B, int64_t ldb, int nb,
+    int d, double inv, OutT *__restrict__ K, int64_t ldk, int gx, int gy) {
 #pragma clang fp contract(off)
-  __shared__ T As[2][kLapBK * kLapStride];
-  __shared__ T Bs[2][kLapBK * kLapStride];
+  constexpr int R = MT / 16;
+  constexpr int kStride = MT + 2;
+  constexpr int EPT = MT * kLapBK / 256;  // staged elements per thread and operand
+  constexpr int TPR = kLapBK / EPT;       // threads per staged row
+  __shared__ T As[2][kLapBK * kStride];
+  __shared__ T Bs[2][kLapBK * kStride];
+  int bx, by;
+  if (!xcd_tile(gx, gy, bx, by)) return;
   const int tid = threadIdx.x;
   const int tx = tid & 15, ty = tid >> 4;
-  const int n0 = blockIdx.x * 64, m0 = blockIdx.y * 64;
-  const int lrow = tid >> 2, lk = (tid & 3) * 4;
+  const int n0 = bx * MT, m0 = by * MT;
+  const int lrow = tid / TPR, lk = (tid % TPR) * EPT;
   const bool arow_ok = n0 + lrow < na, brow_ok = m0 + lrow < nb;
   const T *arow = A + (int64_t)(n0 + lrow) * lda;
   const T *brow = B + (int64_t)(m0 + lrow) * ldb;
-  T ra[4], rb[4];
-  T acc[4][4];
+  T ra[EPT], rb[EPT];
+  T acc[R][R];
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < R; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = T(0);
+    for (int j = 0; j < R; ++j) acc[i][j] = T(0);
 
   auto gload = [&](int k0) {
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
+    for (int c = 0; c < EPT; ++c) {
       const int k = k0 + lk + c;
       ra[c] = (arow_ok && k < d) ? arow[k] : T(0);
       rb[c] = (brow_ok && k < d) ? brow[k] : T(0);
@@ -136,9 +146,9 @@ __global__ __launch_bounds__(256) void k_laplacian(const T *__restrict__ A, int6
   };
   auto lstore = [&](int buf) {
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      As[buf][(lk + c) * kLapStride + lrow] = ra[c];
-      Bs[buf][(lk + c) * kLapStride + lrow] = rb[c];
+    for (int c = 0; c < EPT; ++c) {
+      As[buf][(lk + c) * kStride + lrow] = ra[c];
+      Bs[buf][(lk + c) * kStride + lrow] = rb[c];
     }
   };
   const int nk = (d + kLapBK - 1) / kLapBK;
@@ -150,26 +160,26 @@ __global__ __launch_bounds__(256) void k_laplacian(const T *__restrict__ A, int6
     if (kt + 1 < nk) gload((kt + 1) * kLapBK);
     const int kmax = min(kLapBK, d - kt * kLapBK);
     for (int k = 0; k < kmax; ++k) {
-      T a[4], b[4];
+      T a[R], b[R];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) a[j] = As[buf][k * kLapStride + tx * 4 + j];
+      for (int j = 0; j < R; ++j) a[j] = As[buf][k * kStride + tx + 16 * j];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) b[i] = Bs[buf][k * kLapStride + ty * 4 + i];
+      for (int i = 0; i < R; ++i) b[i] = Bs[buf][k * kStride + ty + 16 * i];
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+      for (int i = 0; i < R; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = acc[i][j] + fabs(a[j] - b[i]);
+        for (int j = 0; j < R; ++j) acc[i][j] = acc[i][j] + fabs(a[j] - b[i]);
     }
     if (kt + 1 < nk) lstore(buf ^ 1);
     __syncthreads();
   }
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int gm = m0 + ty * 4 + i;
+  for (int i = 0; i < R; ++i) {
+    const int gm = m0 + ty + 16 * i;
     if (gm >= nb) continue;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int gn = n0 + tx * 4 + j;
+    for (int j = 0; j < R; ++j) {
+      const int gn = n0 + tx + 16 * j;
       if (gn >= na) continue;
       double v;
       if (sizeof(OutT) == 8)
@@ -179,6 +189,14 @@ __global__ __launch_bounds__(256) void k_laplacian(const T *__restrict__ A, int6
       kstore(K + (int64_t)gn + (int64_t)gm * ldk, v);
     }
   }
+}
+
+template <typename T, typename OutT>
+void launch_laplacian(const T *a, int64_t lda, int na, const T *b, int64_t ldb, int nb, int d,
+                      double inv, OutT *k, int64_t ldk, hipStream_t st) {
+  const int gx = ceil_div(na, 64), gy = ceil_div(nb, 64);
+  hipLaunchKernelGGL((k_laplacian<T, OutT, 64>), dim3((unsigned)xcd_grid(gx, gy)), dim3(256), 0, st,
+                     a, lda, na, b, ldb, nb, d, inv, k, ldk, gx, gy);
 }
 
 __global__ void k_fill(double *p, int n, double v) {
@@ -306,10 +324,8 @@ int fmlh_laplacian_kernel(const double *a, int64_t lda, int na, const double *b,
   if (rc) return rc;
   if (na == 0 || nb == 0) return 0;
   const double inv = -1.0 / sigma;
-  dim3 grid(ceil_div(na, 64), ceil_div(nb, 64));
   timing_begin(as_stream(stream));
-  hipLaunchKernelGGL((k_laplacian<double, double>), grid, dim3(256), 0, as_stream(stream), a, lda,
-                     na, b, ldb, nb, d, inv, k, ldk);
+  launch_laplacian<double, double>(a, lda, na, b, ldb, nb, d, inv, k, ldk, as_stream(stream));
   timing_end(as_stream(stream));
   FMLH_LAUNCH_CHECK("k_laplacian<f64>");
   return 0;
@@ -323,14 +339,11 @@ int fmlh_laplacian_kernel_f32(const float *a, int64_t lda, int na, const float *
   FMLH_ARG_CHECK((k32 == nullptr) != (k64 == nullptr), 9);
   if (na == 0 || nb == 0) return 0;
   const double inv = -1.0 / sigma;
-  dim3 grid(ceil_div(na, 64), ceil_div(nb, 64));
   timing_begin(as_stream(stream));
   if (k32)
-    hipLaunchKernelGGL((k_laplacian<float, float>), grid, dim3(256), 0, as_stream(stream), a, lda,
-                       na, b, ldb, nb, d, inv, k32, ldk);
+    launch_laplacian<float, float>(a, lda, na, b, ldb, nb, d, inv, k32, ldk, as_stream(stream));
   else
-    hipLaunchKernelGGL((k_laplacian<float, double>), grid, dim3(256), 0, as_stream(stream), a, lda,
-                       na, b, ldb, nb, d, inv, k64, ldk);
+    launch_laplacian<float, double>(a, lda, na, b, ldb, nb, d, inv, k64, ldk, as_stream(stream));
   timing_end(as_stream(stream));
   FMLH_LAUNCH_CHECK("k_laplacian<f32>");
   return 0;
