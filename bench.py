@@ -62,7 +62,7 @@ def parse():
                     choices=["arad", "krr", "gaussian64", "gaussian32", "gaussianbf16x3",
                              "laplacian64", "laplacian32", "cholesky64", "cholesky32",
                              "krr36k_fp64", "krr36k_mixed", "serving", "arad4096", "arad4096_fp32", "copy",
-                             "atomic", "atomic_lap", "atomic_mfma", "aras"])
+                             "atomic", "atomic_lap", "atomic_mfma", "aras", "inverse64"])
     ap.add_argument("--duration", type=float, default=5.0, help="serving run length in seconds")
     ap.add_argument("--n", type=int, default=0, help="problem size override for the 36k workloads")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -624,6 +624,63 @@ def cholesky_workload(args, kind):
     return info, step, None, roofline, None, finish
 
 
+def inverse_workload(args):
+    """cho_invert's second half at 36k (cho_solve.py:27-30 -> fcho_solve.f90:
+    49-68, dpotri('U')): one step = one fmlh_potri of the FP64 Cholesky
+    factor of the Gaussian K + lambda I (factored once, copied in each step):
+    L^{-1} by blocked substitution, then L^{-T} L^{-1} on the lower triangle."""
+    import torch
+    import fml_amd._native as nat
+    from fml_amd.device import to_device, workspace
+    from fml_amd.krr import add_diagonal, kernel_matrix
+    from fml_amd.math.cho_solve import potrf_inplace
+    from fml_amd.representations import generate_coulomb_matrices
+    n = args.n or 36000
+    ms = _cm_set(n, 0, 3, 29, 29)
+    X = generate_coulomb_matrices(to_device(ms.charges), to_device(ms.coords),
+                                  to_device(ms.offsets, np.int32), nmax=29, on_device=True)
+    A = kernel_matrix(X, X, 1000.0)
+    add_diagonal(A, 1e-6)
+    F = A.clone()
+    info0 = potrf_inplace(F, sync=True)
+    del X
+    work = torch.empty_like(F)
+    lib = nat.load()
+    wsb = lib.fmlh_potri_ws_bytes(n)
+    ws = workspace(wsb)
+    infot = torch.zeros(1, dtype=torch.int32, device="cuda")
+
+    def step():
+        work.copy_(F)
+        nat.call("fmlh_potri", nat.ptr(work), work.stride(0), n, nat.ptr(infot), nat.ptr(ws), wsb,
+                 nat.stream_ptr())
+
+    info = {"metric": "inverse TFLOP/s", "unit": "TFLOP/s", "dtype": "f64",
+            "config": {"workload": f"inverse64: potri from the FP64 factor of the {n}x{n} Gaussian K + "
+                                   "lambda I (sigma 1000, lambda 1e-6)"}}
+    flops = 2.0 * n ** 3 / 3.0   # triangular inverse n^3/3 + L^-T L^-1 n^3/3
+
+    def roofline(t):
+        ach = flops / t / 1e12
+        return {"bound": "mfma", "achieved": round(ach, 2), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(ach / PEAK_FP64_TFLOPS, 4), "traffic": None}
+
+    def finish(res):
+        res["potrf_info"] = int(info0 if isinstance(info0, int) else info0.item())
+        res["potri_info"] = int(infot.item())
+        res["value"] = round(flops / (res["dominant_kernel_ms"] / 1e3) / 1e12, 3)
+        # a sample of rows of inv(A) A - I; inv(A) sits in work's row-major
+        # lower triangle, so row r is work[r, :r+1] followed by work[r+1:, r]
+        rows = torch.arange(0, n, max(1, n // 64), device="cuda")
+        cols = torch.arange(n, device="cuda")
+        inv_rows = torch.where(cols[None, :] <= rows[:, None], work[rows], work[:, rows].t())
+        eye = (cols[None, :] == rows[:, None]).to(torch.float64)
+        r = inv_rows @ A - eye
+        res["sampled_rows_max_abs_inv_a_minus_i"] = float(r.abs().max().item())
+
+    return info, step, None, roofline, None, finish
+
+
 def _train_36k(n_total, n_train, precision="fp64", sigma=1000.0, lam=1e-6):
     """Coulomb features (nmax 29, 40k molecules seed 0 3-29 atoms) and a fitted model."""
     import torch
@@ -785,6 +842,8 @@ def main():
         info, step, units, roofline, cpu_baseline, finish = krr_workload(args)
     elif wl.startswith("krr36k"):
         info, step, units, roofline, cpu_baseline, finish = krr36k_workload(args, wl.split("_")[1])
+    elif wl == "inverse64":
+        info, step, units, roofline, cpu_baseline, finish = inverse_workload(args)
     elif wl.startswith("cholesky"):
         info, step, units, roofline, cpu_baseline, finish = cholesky_workload(args, wl)
     elif wl == "copy":

@@ -127,14 +127,14 @@ __global__ __launch_bounds__(256) void k_trsm(T *__restrict__ a, int64_t lda, in
 // walked column by column, so a group's row tiles stay in L2 while its
 // column tiles stream past once (the row-major triangle would stream every
 // column tile once per row).  Returns false past the end.
-__device__ __forceinline__ bool tri_tile(int g, int64_t local, int &tx, int &ty) {
+__host__ __device__ inline bool tri_tile(int g, int64_t local, int &tx, int &ty) {
   if (local >= (int64_t)g * (g + 1) / 2) return false;
   // row of `local` in row-major order: T(r) <= local < T(r + 1), T(r) = r(r+1)/2
   int r = (int)((sqrt(8.0 * (double)local + 1.0) - 1.0) * 0.5);
   while ((int64_t)(r + 1) * (r + 2) / 2 <= local) ++r;
   while ((int64_t)r * (r + 1) / 2 > local) --r;
   const int r0 = (r / kGroup) * kGroup;  // first row of the group (it starts at T(r0))
-  const int R = min(kGroup, g - r0);
+  const int R = kGroup < g - r0 ? kGroup : g - r0;
   int64_t o = local - (int64_t)r0 * (r0 + 1) / 2;
   const int64_t full = (int64_t)(r0 + 1) * R;  // columns 0..r0 hold R tiles each
   if (o < full) {
@@ -408,17 +408,52 @@ __global__ __launch_bounds__(256) void k_potri_diag(const T *__restrict__ a, int
 
 // M[r][c] = sum_{k >= tile row start} XT[r][k] XT[c][k] for c <= r < n,
 // lower-triangle tiles in tri_tile order (XT[r][k] = 0 for k < r).
+// Runs of consecutive tri_tile tiles, one per XCD (workgroup L runs on XCD
+// L % 8 and takes tile runs.start[L % 8] + L / 8), cut so that each run
+// holds about the same work.  For the inverse product a tile's depth is
+// n - ty*MT, so equal tile counts would give the first XCD the deepest
+// tiles and the last the shallowest.
+struct XcdRuns {
+  int64_t start[kXcds + 1];
+};
+
+inline XcdRuns balanced_runs(int g, int n, int mt, int64_t *max_len) {
+  const int64_t t = (int64_t)g * (g + 1) / 2;
+  const int64_t overhead = 4 * mt;  // staging and epilogue, in k columns
+  double total = 0.0;
+  for (int64_t l = 0; l < t; ++l) {
+    int tx = 0, ty = 0;
+    tri_tile(g, l, tx, ty);
+    total += (double)((n - ty * mt > 0 ? n - ty * mt : 0) + overhead);
+  }
+  XcdRuns r;
+  r.start[0] = 0;
+  int x = 1;
+  double acc = 0.0;
+  for (int64_t l = 0; l < t && x < kXcds; ++l) {
+    int tx = 0, ty = 0;
+    tri_tile(g, l, tx, ty);
+    acc += (double)((n - ty * mt > 0 ? n - ty * mt : 0) + overhead);
+    while (x < kXcds && acc >= total * x / kXcds) r.start[x++] = l + 1;
+  }
+  while (x <= kXcds) r.start[x++] = t;
+  int64_t m = 0;
+  for (int k = 0; k < kXcds; ++k) m = r.start[k + 1] - r.start[k] > m ? r.start[k + 1] - r.start[k] : m;
+  *max_len = m;
+  return r;
+}
+
 template <typename T, int MT>
 __global__ __launch_bounds__(MT == 128 ? 512 : 256, (MT == 128 ? 2 : 1)) void k_potri_syrk(
     const T *__restrict__ xt, int64_t ldx, int n, int g, T *__restrict__ a, int64_t lda,
-    const int32_t *__restrict__ info) {
+    XcdRuns runs, const int32_t *__restrict__ info) {
   __shared__ TileSmem<T, MT> s;
   if (*info != 0) return;
   int tx, ty;
-  const int64_t t = (int64_t)g * (g + 1) / 2;
-  const int64_t per = (t + kXcds - 1) / kXcds;
   const int64_t L = blockIdx.x;
-  if (!tri_tile(g, (L % kXcds) * per + L / kXcds, tx, ty)) return;
+  const int x = (int)(L % kXcds);
+  const int64_t local = runs.start[x] + L / kXcds;
+  if (local >= runs.start[x + 1] || !tri_tile(g, local, tx, ty)) return;
   const int tr = ty * MT, tc = tx * MT;
   constexpr int WR = MT == 128 ? 4 : 2;
   TileAcc<T, MT, WR> acc;
@@ -461,14 +496,16 @@ int potri_impl(double *a, int64_t lda, int n, int32_t *info, double *xt, hipStre
   }
   if (n >= 4 * 128) {
     const int g = ceil_div(n, 128);
-    const int64_t t = (int64_t)g * (g + 1) / 2;
-    hipLaunchKernelGGL((k_potri_syrk<double, 128>), dim3((unsigned)xcd_grid((int)t, 1)), dim3(512), 0,
-                       st, xt, ldx, n, g, a, lda, info);
+    int64_t len = 0;
+    const XcdRuns runs = balanced_runs(g, n, 128, &len);
+    hipLaunchKernelGGL((k_potri_syrk<double, 128>), dim3((unsigned)(len * kXcds)), dim3(512), 0, st,
+                       xt, ldx, n, g, a, lda, runs, info);
   } else {
     const int g = ceil_div(n, kTile);
-    const int64_t t = (int64_t)g * (g + 1) / 2;
-    hipLaunchKernelGGL((k_potri_syrk<double, kTile>), dim3((unsigned)xcd_grid((int)t, 1)), dim3(256),
-                       0, st, xt, ldx, n, g, a, lda, info);
+    int64_t len = 0;
+    const XcdRuns runs = balanced_runs(g, n, kTile, &len);
+    hipLaunchKernelGGL((k_potri_syrk<double, kTile>), dim3((unsigned)(len * kXcds)), dim3(256), 0, st,
+                       xt, ldx, n, g, a, lda, runs, info);
   }
   FMLH_LAUNCH_CHECK("k_potri_syrk");
   return 0;
@@ -787,7 +824,10 @@ int fmlh_potri(double *a, int64_t lda, int n, int32_t *info, void *ws, size_t ws
   }
   FMLH_ARG_CHECK(a != nullptr, 1);
   FMLH_ARG_CHECK(ws != nullptr && ws_bytes >= fmlh_potri_ws_bytes(n), 5);
-  return potri_impl(a, lda, n, info, static_cast<double *>(ws), as_stream(stream));
+  timing_begin(as_stream(stream));
+  const int rc = potri_impl(a, lda, n, info, static_cast<double *>(ws), as_stream(stream));
+  timing_end(as_stream(stream));
+  return rc;
 }
 
 int fmlh_potrs(const double *a, int64_t lda, int n, double *x, int64_t ldx, int nrhs, void *ws,
