@@ -347,6 +347,10 @@ int potrf_impl(T *a, int64_t lda, int n, int32_t *info, hipStream_t st) {
 //     (k_potri_syrk), written over the factor.
 
 // XT[j][r0 + m] = -sum_{k < r0} M[r0 + m][k] XT[j][k], m < bs, j < r0.
+// XT[j][k] is 0 for k < j (X = L^{-1} is lower triangular; the workspace is
+// zero-filled and the diagonal solves leave exact zeros there), so the tile
+// of columns j0.. sums k from j0: the products dropped are exact zeros, on
+// the same 4-aligned k steps.
 template <typename T>
 __global__ __launch_bounds__(256) void k_potri_gemm(const T *__restrict__ a, int64_t lda, int r0,
                                                     int bs, T *__restrict__ xt, int64_t ldx,
@@ -355,8 +359,8 @@ __global__ __launch_bounds__(256) void k_potri_gemm(const T *__restrict__ a, int
   if (*info != 0) return;
   const int j0 = blockIdx.x * kTile;
   TileAcc<T, kTile> acc;
-  tile_nt<T, false, kTile>(a + (int64_t)r0 * lda, lda, bs, xt + (int64_t)j0 * ldx, ldx, r0 - j0, r0,
-                           s, acc);
+  tile_nt<T, false, kTile>(a + (int64_t)r0 * lda + j0, lda, bs, xt + (int64_t)j0 * ldx + j0, ldx,
+                           r0 - j0, r0 - j0, s, acc);
 #pragma unroll
   for (int im = 0; im < 2; ++im)
 #pragma unroll
@@ -371,6 +375,57 @@ __global__ __launch_bounds__(256) void k_potri_gemm(const T *__restrict__ a, int
 
 // Column j < r0 + bs: x = L_bb^{-1} (e_j - C) for the block rows r0..r0+bs,
 // forward substitution in registers, L_bb in LDS.  A zero pivot sets *info.
+// The same product split over k: block (tile, c) of a 2-D grid sums
+// k in [j0 + c*kPotriKC, min(j0 + (c+1)*kPotriKC, r0)) into its own 64 x 64
+// partial; k_potri_sum adds a tile's partials in chunk order.  Used when
+// the deepest tile (j0 = 0, depth r0) spans several chunks, so that a
+// block row's few long tiles become many short ones.
+constexpr int kPotriKC = 2048;
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_potri_gemm_part(const T *__restrict__ a, int64_t lda,
+                                                         int r0, int bs, const T *__restrict__ xt,
+                                                         int64_t ldx, int cmax,
+                                                         T *__restrict__ part,
+                                                         const int32_t *__restrict__ info) {
+  __shared__ TileSmem<T, kTile> s;
+  if (*info != 0) return;
+  const int tile = blockIdx.x, c = blockIdx.y;
+  const int j0 = tile * kTile;
+  const int kb = j0 + c * kPotriKC;
+  if (kb >= r0) return;
+  const int ke = min(kb + kPotriKC, r0);
+  TileAcc<T, kTile> acc;
+  tile_nt<T, false, kTile>(a + (int64_t)r0 * lda + kb, lda, bs, xt + (int64_t)j0 * ldx + kb, ldx,
+                           r0 - j0, ke - kb, s, acc);
+  T *pt = part + ((int64_t)tile * cmax + c) * kTile * kTile;
+#pragma unroll
+  for (int im = 0; im < 2; ++im)
+#pragma unroll
+    for (int in = 0; in < 2; ++in)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        pt[tile_row<T, kTile>(im, r) * kTile + tile_col<kTile>(in)] = acc.v[im][in][r];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_potri_sum(int r0, int bs, T *__restrict__ xt, int64_t ldx,
+                                                   int cmax, const T *__restrict__ part,
+                                                   const int32_t *__restrict__ info) {
+  if (*info != 0) return;
+  const int tile = blockIdx.x;
+  const int j0 = tile * kTile;
+  const int nc = (r0 - j0 + kPotriKC - 1) / kPotriKC;
+  for (int e = threadIdx.x; e < kTile * kTile; e += 256) {
+    const int m = e / kTile, jj = e % kTile;
+    const int j = j0 + jj;
+    if (m >= bs || j >= r0) continue;
+    T v = T(0);
+    for (int c = 0; c < nc; ++c) v = v + part[((int64_t)tile * cmax + c) * kTile * kTile + e];
+    xt[(int64_t)j * ldx + r0 + m] = -v;
+  }
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_potri_diag(const T *__restrict__ a, int64_t lda, int r0,
                                                     int bs, T *__restrict__ xt, int64_t ldx,
@@ -476,7 +531,8 @@ __global__ void k_fill_rows(double *p, int64_t count) {
   if (i < count) p[i] = 0.0;
 }
 
-int potri_impl(double *a, int64_t lda, int n, int32_t *info, double *xt, hipStream_t st) {
+int potri_impl(double *a, int64_t lda, int n, int32_t *info, double *xt, double *part,
+               hipStream_t st) {
   const int64_t ldx = n;
   hipLaunchKernelGGL(k_set_int_impl, dim3(1), dim3(1), 0, st, info, 0);
   FMLH_LAUNCH_CHECK("k_set_int");
@@ -485,7 +541,15 @@ int potri_impl(double *a, int64_t lda, int n, int32_t *info, double *xt, hipStre
   FMLH_LAUNCH_CHECK("k_fill_rows");
   for (int r0 = 0; r0 < n; r0 += kNB) {
     const int bs = min(kNB, n - r0);
-    if (r0 > 0) {
+    if (r0 > kPotriKC) {
+      const int tiles = ceil_div(r0, kTile), cmax = ceil_div(r0, kPotriKC);
+      hipLaunchKernelGGL(k_potri_gemm_part<double>, dim3(tiles, cmax), dim3(256), 0, st, a, lda, r0,
+                         bs, xt, ldx, cmax, part, info);
+      FMLH_LAUNCH_CHECK("k_potri_gemm_part");
+      hipLaunchKernelGGL(k_potri_sum<double>, dim3(tiles), dim3(256), 0, st, r0, bs, xt, ldx, cmax,
+                         part, info);
+      FMLH_LAUNCH_CHECK("k_potri_sum");
+    } else if (r0 > 0) {
       hipLaunchKernelGGL(k_potri_gemm<double>, dim3(ceil_div(r0, kTile)), dim3(256), 0, st, a, lda,
                          r0, bs, xt, ldx, info);
       FMLH_LAUNCH_CHECK("k_potri_gemm");
@@ -808,8 +872,13 @@ int fmlh_spotrf(float *a, int64_t lda, int n, int32_t *info, void *ws, size_t ws
   return rc;
 }
 
+// X^T (n x n) and the split-k partials of the largest block row:
+// ceil(n/64) tiles x ceil(n/2048) chunks of 64 x 64.
 size_t fmlh_potri_ws_bytes(int n) {
-  return n > 0 ? WsCarver::need(sizeof(double) * (size_t)n * (size_t)n) : 0;
+  if (n <= 0) return 0;
+  const size_t tiles = (size_t)ceil_div(n, kTile), chunks = (size_t)ceil_div(n, kPotriKC);
+  return WsCarver::need(sizeof(double) * (size_t)n * (size_t)n) +
+         WsCarver::need(sizeof(double) * tiles * chunks * kTile * kTile);
 }
 
 int fmlh_potri(double *a, int64_t lda, int n, int32_t *info, void *ws, size_t ws_bytes,
@@ -825,7 +894,11 @@ int fmlh_potri(double *a, int64_t lda, int n, int32_t *info, void *ws, size_t ws
   FMLH_ARG_CHECK(a != nullptr, 1);
   FMLH_ARG_CHECK(ws != nullptr && ws_bytes >= fmlh_potri_ws_bytes(n), 5);
   timing_begin(as_stream(stream));
-  const int rc = potri_impl(a, lda, n, info, static_cast<double *>(ws), as_stream(stream));
+  WsCarver wc(ws, ws_bytes);
+  double *xt = wc.take<double>((size_t)n * (size_t)n);
+  double *part = wc.take<double>((size_t)ceil_div(n, kTile) * ceil_div(n, kPotriKC) * kTile * kTile);
+  if (xt == nullptr || part == nullptr) return -5;
+  const int rc = potri_impl(a, lda, n, info, xt, part, as_stream(stream));
   timing_end(as_stream(stream));
   return rc;
 }

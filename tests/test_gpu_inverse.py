@@ -46,6 +46,20 @@ def test_cho_invert_matches_lapack(gpu, n):
     assert np.max(np.abs(full @ A - np.eye(n))) < 1e-12
 
 
+@pytest.mark.parametrize("n", [2200, 4300])
+def test_cho_invert_split_product(gpu, n):
+    """Sizes whose triangular-inverse products run split over k (block rows
+    deeper than 2048 columns; 4300 gives three chunks): against LAPACK's
+    inverse (numpy) and through inv(A) A = I."""
+    from fml_amd.math import cho_invert
+    A = np.asfortranarray(_spd(n, 7 + n))
+    got = cho_invert(A)
+    full = np.triu(got) + np.triu(got, 1).T
+    ref = np.linalg.inv(A)
+    assert np.max(np.abs(full - ref)) <= 1e-12 * np.max(np.abs(ref))
+    assert np.max(np.abs(full @ A - np.eye(n))) < 1e-11
+
+
 def test_cho_invert_golden_and_failure(gpu, capsys):
     from fml_amd.math import cho_invert
     d = np.load(os.path.join(GOLD, "inverse.npz"))
