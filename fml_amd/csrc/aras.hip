@@ -35,12 +35,15 @@
 #include <math.h>
 
 #include "common.h"
+#include "exp_neg.h"
 
 #pragma clang fp contract(off)
 
 namespace {
 
 using namespace fmlh;
+
+__constant__ double c_exp2_128[128] = {FMLH_EXP2_128_VALUES};
 
 constexpr int kMaxAmax = 64;    // atoms per molecule (one lane each in the prep)
 constexpr int kMaxTypes = 16;   // distinct elements within one atom's cut
@@ -326,7 +329,9 @@ __device__ void load_pdg(double *pdg, const ArasParams &P, const ArasElems *E) {
 __global__ __launch_bounds__(64) void k_aras_self(ArasSet S, ArasParams P, const ArasElems *E) {
   __shared__ double pdg[(kMaxTypes + 1) * (kMaxTypes + 1)];
   __shared__ double term[64];
+  __shared__ double etab[128];
   load_pdg(pdg, P, E);
+  for (int t = threadIdx.x; t < 128; t += blockDim.x) etab[t] = c_exp2_128[t];
   __syncthreads();
   const int amax = P.amax;
   const int a = (int)(blockIdx.x / amax), i = (int)(blockIdx.x % amax);
@@ -355,7 +360,7 @@ __global__ __launch_bounds__(64) void k_aras_self(ArasSet S, ArasParams P, const
       const double dd = d2 - d1;
       const double r2 = dd * dd;
       if (r2 < P.maxg2) {
-        const double dw = exp(r2 * P.inv_width) * pdg[gx[m1] * G1 + gx[m2]];
+        const double dw = exp_neg(r2 * P.inv_width, etab) * pdg[gx[m1] * G1 + gx[m2]];
         double angular = 0.0 * 0.0;
         for (int o = 0; o < P.order; ++o) {
           const double *c1 = S.cs + 2 * ((x * P.order + o) * amax + m1) * kMaxTypes;
@@ -417,11 +422,15 @@ struct LaneAtom {
 // bit clear in gmask -- are skipped: both only drop exact zeros) and p2
 // over x2's own elements t (ascending).  NT2 > 0: x2 has exactly NT2
 // elements, whose similarity row pd(p2_t, g) sits in registers; NT2 == 0:
-// any count nt2, similarities read from LDS.
+// any count nt2, similarities read from LDS.  The window factor's exponent
+// lies in (-16, 0] ((8 d_width)^2 (-1/(4 d_width^2)) = -16), where exp_neg
+// (exp_neg.h: a 2^(j/128) table in LDS and a degree-5 polynomial, within
+// about one ulp) evaluates it.
 template <int GB, int NT2>
 __device__ __forceinline__ double atom_pair(const ArasSet &A, const ArasUniform &B, const ArasParams &P,
-                                            const double *pdg, int G1, const LaneAtom &L,
-                                            unsigned gmask, int64_t x2, int nt2, double aad) {
+                                            const double *pdg, const double *etab, int G1,
+                                            const LaneAtom &L, unsigned gmask, int64_t x2,
+                                            int nt2, double aad) {
   const int amax = P.amax;
   const int nb2 = B.nb[x2];
   constexpr int NT = NT2 > 0 ? NT2 : GB;
@@ -471,7 +480,7 @@ __device__ __forceinline__ double atom_pair(const ArasSet &A, const ArasUniform 
       const double dd = d2 - d1;
       const double r2 = dd * dd;
       if (act && r2 < P.maxg2) {
-        const double dw = exp(r2 * P.inv_width) * pdg[g1 * G1 + g2p[m2]];
+        const double dw = exp_neg(r2 * P.inv_width, etab) * pdg[g1 * G1 + g2p[m2]];
         double angular = 0.0 * 0.0;
         for (int o = 0; o < P.order; ++o) {
           const double *c2 = B.cs + 2 * ((x2 * P.order + o) * amax + m2) * kMaxTypes;
@@ -530,6 +539,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
   __shared__ double pdg[(kMaxTypes + 1) * (kMaxTypes + 1)];
   __shared__ double red[64 * kMaxSig];
   __shared__ int lane_mol[64], lane_atom[64], natoms_s, maxmol_s;
+  __shared__ double etab[128];
   const int lane = threadIdx.x;
   const int64_t w = blockIdx.x;
   const int ib = (int)(w % nblk);
@@ -537,6 +547,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
   const int b0 = blk ? blk[ib] : ib * per_blk;
   const int b1 = blk ? blk[ib + 1] : min(A.nm, (ib + 1) * per_blk);
   load_pdg(pdg, P, E);
+  for (int t = lane; t < 128; t += 64) etab[t] = c_exp2_128[t];
   if (lane == 0) {
     int c = 0, mx = -1;
     for (int b = b0; b < b1; ++b) {
@@ -596,15 +607,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
       const int nt2 = B.nt[x2];
       if constexpr (GB <= 6) {
         switch (nt2) {
-          case 1: aad = atom_pair<GB, 1>(A, B, P, pdg, G1, Lj, gmask, x2, nt2, aad); break;
-          case 2: aad = atom_pair<GB, 2>(A, B, P, pdg, G1, Lj, gmask, x2, nt2, aad); break;
-          case 3: aad = atom_pair<GB, 3>(A, B, P, pdg, G1, Lj, gmask, x2, nt2, aad); break;
-          case 4: aad = atom_pair<GB, (GB >= 4 ? 4 : 0)>(A, B, P, pdg, G1, Lj, gmask, x2, nt2, aad); break;
-          case 5: aad = atom_pair<GB, (GB >= 5 ? 5 : 0)>(A, B, P, pdg, G1, Lj, gmask, x2, nt2, aad); break;
-          default: aad = atom_pair<GB, 0>(A, B, P, pdg, G1, Lj, gmask, x2, nt2, aad); break;
+          case 1: aad = atom_pair<GB, 1>(A, B, P, pdg, etab, G1, Lj, gmask, x2, nt2, aad); break;
+          case 2: aad = atom_pair<GB, 2>(A, B, P, pdg, etab, G1, Lj, gmask, x2, nt2, aad); break;
+          case 3: aad = atom_pair<GB, 3>(A, B, P, pdg, etab, G1, Lj, gmask, x2, nt2, aad); break;
+          case 4: aad = atom_pair<GB, (GB >= 4 ? 4 : 0)>(A, B, P, pdg, etab, G1, Lj, gmask, x2, nt2, aad); break;
+          case 5: aad = atom_pair<GB, (GB >= 5 ? 5 : 0)>(A, B, P, pdg, etab, G1, Lj, gmask, x2, nt2, aad); break;
+          default: aad = atom_pair<GB, 0>(A, B, P, pdg, etab, G1, Lj, gmask, x2, nt2, aad); break;
         }
       } else {
-        aad = atom_pair<GB, 0>(A, B, P, pdg, G1, Lj, gmask, x2, nt2, aad);
+        aad = atom_pair<GB, 0>(A, B, P, pdg, etab, G1, Lj, gmask, x2, nt2, aad);
       }
       if (Lj.valid) {
         double l2 = self1 + B.self[x2] - 2.0 * aad * pdg[g01 * G1 + g02];
