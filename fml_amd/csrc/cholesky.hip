@@ -247,10 +247,12 @@ int trailing_update(T *a, int64_t lda, int n, int r0, int kb, int ke, int32_t *i
 struct Aux {
   hipStream_t s = nullptr;
   hipEvent_t fork = nullptr, join = nullptr;
+  std::mutex *enqueue = nullptr;  // held while one factorisation enqueues on s
 };
 
 bool aux_for_device(Aux &out) {
   static std::mutex mu;
+  static std::mutex enqueue_mu[64];
   static Aux table[64];
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) {
@@ -267,6 +269,7 @@ bool aux_for_device(Aux &out) {
       (void)hipGetLastError();
       return false;
     }
+    y.enqueue = &enqueue_mu[dev];
     x = y;
   }
   out = x;
@@ -294,6 +297,12 @@ int potrf_impl(T *a, int64_t lda, int n, int32_t *info, hipStream_t st) {
     }
   }
   if (ahead) ahead = aux_for_device(aux);
+  // Calls from several host threads share the device's auxiliary stream and
+  // events: one factorisation enqueues at a time (each event wait takes the
+  // event's latest record, so serialised enqueues keep every fork and join
+  // paired; the device work of the calls still overlaps on their streams).
+  std::unique_lock<std::mutex> lk;
+  if (ahead) lk = std::unique_lock<std::mutex>(*aux.enqueue);
   int rc = panel_factor(a, lda, n, 0, min(kNBO, n), info, st);
   if (rc) return rc;
   for (int K0 = 0; K0 < n; K0 += kNBO) {

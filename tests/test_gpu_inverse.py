@@ -106,3 +106,36 @@ def test_fget_alpha_arad_uneven_molecules(gpu):
     ref, info = oracle.alpha_arad(X, z, n, ms.energies, 20.0, 1e-2)
     assert info == 0
     assert np.max(np.abs(got - ref)) <= 1e-7 * np.max(np.abs(ref))
+
+
+def test_potrf_lookahead_concurrent_threads(gpu):
+    """Factorisations large enough for the panel look-ahead (3 outer panels),
+    enqueued at the same time from two host threads on two streams, give
+    the same factors bit for bit as one at a time."""
+    import threading
+    import torch
+    from fml_amd.math.cho_solve import potrf_inplace
+    mats = [torch.from_numpy(_spd(3000, 90 + k)).cuda() for k in range(2)]
+    ref = []
+    for m in mats:
+        f = m.clone()
+        assert potrf_inplace(f) == 0
+        ref.append(f)
+    out = [m.clone() for m in mats]
+    streams = [torch.cuda.Stream() for _ in mats]
+    infos = [None, None]
+    barrier = threading.Barrier(2)
+
+    def run(k):
+        with torch.cuda.stream(streams[k]):
+            barrier.wait()
+            infos[k] = potrf_inplace(out[k], sync=False)
+    ths = [threading.Thread(target=run, args=(k,)) for k in range(2)]
+    for t in ths:
+        t.start()
+    for t in ths:
+        t.join()
+    torch.cuda.synchronize()
+    for k in range(2):
+        assert int(infos[k].item()) == 0
+        assert torch.equal(torch.tril(out[k]), torch.tril(ref[k]))
