@@ -56,12 +56,13 @@ __device__ __forceinline__ float lane_value(float v, int l) {
 // One wave factors the diagonal block: lane r holds row r in registers.
 // Step j takes the square root of the pivot (lane j's diagonal, read by
 // every lane), scales column j below it, then every lane r updates its row
-// M[r][c] -= L[r][j] L[c][j] for c > j, L[c][j] read from lane c (entries
-// above the diagonal are updated too and never stored).  Fully unrolled,
+// M[r][c] -= L[r][j] L[c][j] for c > j, L[c][j] read from an LDS copy of
+// column j (entries above the diagonal are updated too and never stored).  Fully unrolled,
 // so the row stays in registers.
 template <typename T>
 __global__ __launch_bounds__(64) void k_potf2(T *__restrict__ a, int64_t lda, int k0, int b,
                                               int32_t *__restrict__ info) {
+  __shared__ T col[kNB];
   if (*info != 0) return;
   const int r = threadIdx.x;
   T *grow = a + (int64_t)(k0 + r) * lda + k0;
@@ -79,8 +80,13 @@ __global__ __launch_bounds__(64) void k_potf2(T *__restrict__ a, int64_t lda, in
       const T piv = sqrt(d);
       const T lrj = (r > j && r < b) ? row[j] / piv : T(0);
       row[j] = (r == j) ? piv : (r > j ? lrj : row[j]);
+      // column j of L through LDS: every lane reads L[c][j] at the same
+      // address (a broadcast) instead of one readlane pair per element
+      col[r] = lrj;
+      __syncthreads();
 #pragma unroll
-      for (int c = j + 1; c < kNB; ++c) row[c] -= lrj * lane_value(lrj, c);
+      for (int c = j + 1; c < kNB; ++c) row[c] -= lrj * col[c];
+      __syncthreads();
     }
   }
   if (r < b) {
