@@ -103,10 +103,11 @@ SIGNATURES = {
 
 class AradTiling(ctypes.Structure):
     """fmlh_arad_tiling: tile shape and launch shape of the ARAD tile kernel."""
-    _fields_ = [("pairs_per_tile", c_int), ("waves_per_simd", c_int), ("groups_per_cu", c_int)]
+    _fields_ = [("pairs_per_tile", c_int), ("waves_per_simd", c_int), ("groups_per_cu", c_int),
+                ("pair_order", c_int)]
 
     def as_tuple(self):
-        return (self.pairs_per_tile, self.waves_per_simd, self.groups_per_cu)
+        return (self.pairs_per_tile, self.waves_per_simd, self.groups_per_cu, self.pair_order)
 
 
 ARAD_GLOBAL, ARAD_SYMMETRIC, ARAD_KERNEL, ARAD_DISTANCE = 0, 1, 2, 3

@@ -38,16 +38,17 @@
 // unchanged.  Lists that are not ascending use the full loop.
 //
 // Inner exp: its argument (d2-d1)^2 * (-1/(4w^2)) lies in (-16, 0] whenever
-// the term is taken, because (8w)^2 * (-1/(4w^2)) = -16.  exp_neg()
-// (exp_neg.h) evaluates exp there with a 128-entry 2^(j/128) table held in
-// LDS and a degree-5 polynomial (at most one ulp on that domain); it has no
-// overflow/underflow/NaN handling because none can occur there.
+// the term is taken, because (8w)^2 * (-1/(4w^2)) = -16.  exp_neg() and the
+// sine terms' sin_libm() (exp_neg.h) round exactly as the host C library's
+// exp and sin that the reference calls (the same table-driven algorithms;
+// exp_neg's table is held in LDS).
 //
-// Per term the products are grouped as exp * s2 * T * (s1 + (s1 a1) a2 +
-// (s1 b1) b2), with s1 a1 and s1 b1 formed once per m1: the reference's
-// terms (farad_kernels.f90:78-83), one multiply fewer per term, rounding
-// different at the last-ulp level; the accumulation over m1 and m2 keeps the
-// reference's order.
+// Every term is formed with the reference's operations in the reference's
+// order, FP contraction off: ((exp * s1) * s2) * T, times (1 + a1 a2) + b1 b2
+// (farad_kernels.f90:78-83), and the per-sigma kernel sums run sequentially
+// over the atom pairs in the reference's column-major order (:602-604,
+// :216-217), so overlaps, distances and kernel entries are the reference's
+// bit for bit.
 #include <math.h>
 #include <string.h>
 
@@ -120,8 +121,10 @@ __device__ __forceinline__ double stoch(double dr, double dc, double rw2, double
   return rw2 / (rw2 + dr * dr) * cw2 / (cw2 + dc * dc);
 }
 
-// 2^(j/128), j = 0..127 (exp_neg's table; copied to LDS by the kernels).
-__constant__ double c_exp2_128[128] = {FMLH_EXP2_128_VALUES};
+// exp_neg's 2^(j/128) table (copied to LDS by the kernels) and sin_libm's
+// sin/cos table.
+__constant__ ExpPair c_exp_pairs[128] = {FMLH_EXP2_128_PAIRS};
+__constant__ SinCosEntry c_sincos[116] = {FMLH_SINCOS_128};
 
 // ---- preparation ------------------------------------------------------
 
@@ -154,7 +157,7 @@ __global__ __launch_bounds__(256) void k_arad_prep(const double *__restrict__ q,
       if (d > p.cut) break;  // farad_kernels.f90:68,72
       const double rowv = x[amax + m], colv = x[2 * amax + m];
       // global kernel: sine only where d < cut (farad_kernels.f90:526,539)
-      const double sv = (!sin_guard || d < p.cut) ? 1.0 - sin(d * p.inv_cut) : 0.0;
+      const double sv = (!sin_guard || d < p.cut) ? 1.0 - sin_libm(d * p.inv_cut, c_sincos) : 0.0;
       const int64_t o = (int64_t)slot * amax + m;
       set.rec[o] = Rec{d, sv, x[3 * amax + m], x[4 * amax + m]};
       set.rc[o] = make_double2(rowv, colv);
@@ -250,36 +253,32 @@ struct NbView {
 };
 
 template <typename TC>
-__device__ __forceinline__ TC texp(TC x, const double *tab);
+__device__ __forceinline__ TC texp(TC x, const ExpPair *tab);
 template <>
-__device__ __forceinline__ double texp<double>(double x, const double *tab) {
+__device__ __forceinline__ double texp<double>(double x, const ExpPair *tab) {
   return exp_neg(x, tab);
 }
 template <>
-__device__ __forceinline__ float texp<float>(float x, const double *) { return expf(x); }
-
-__device__ __forceinline__ double tfma(double a, double b, double c) { return fma(a, b, c); }
-__device__ __forceinline__ float tfma(float a, float b, float c) { return fmaf(a, b, c); }
+__device__ __forceinline__ float texp<float>(float x, const ExpPair *) { return expf(x); }
 
 // atomic_distl2 for atoms x (list 1) and y (list 2).  kGeneric evaluates T
 // from the raw channels; otherwise ttab is the compact type-pair table.
-// etab: the 2^(j/32) table of exp_neg.  kPadY: list 2 may be read up to two
-// records past its end (a padded LDS stage), so the band loop's read-ahead
-// needs no clamp.
+// etab: exp_neg's table.  kPadY: list 2 may be read up to two records past
+// its end (a padded LDS stage), so the band loop's read-ahead needs no clamp.
 template <typename TC, bool kGeneric, bool kPadY = false>
 __device__ TC arad_distl2(const NbView &x, int cnt1, bool asc1, const NbView &y, int cnt2,
                           bool asc2, TC inv_w, TC maxg, const TC *__restrict__ ttab,
-                          const double *__restrict__ etab, double rw2, double cw2) {
+                          const ExpPair *__restrict__ etab, double rw2, double cw2) {
   TC acc = TC(0);
-  // d * ((1 + a1 a2) + b1 b2), the quantity farad_kernels.f90:83 adds (the
-  // angular factor with fused multiply-adds); record yr (index m2) of list 2
-  // exp * s2 * T * (s1 + sa1 a2 + sb1 b2) with sa1 = s1 a1, sb1 = s1 b1: the
-  // quantity farad_kernels.f90:78-83 adds, for record yr (index m2) of list 2
-  auto rec_value = [&](const Rec &yr, uint16_t ty2, int m2, TC d1, TC s1, TC sa1, TC sb1, int trow,
+  // the quantity farad_kernels.f90:78-83 adds for record yr (index m2) of
+  // list 2, in the reference's order: d = (exp(r2 inv_w) s1) s2; d = d T;
+  // d ((1 + a1 a2) + b1 b2)
+  auto rec_value = [&](const Rec &yr, uint16_t ty2, int m2, TC d1, TC s1, TC a1, TC b1, int trow,
                        double2 rc1) -> TC {
     const TC dr = (TC)yr.d - d1;
     const TC r2 = dr * dr;
-    TC v = texp<TC>(r2 * inv_w, etab) * (TC)yr.s;
+    TC v = texp<TC>(r2 * inv_w, etab) * s1;
+    v = v * (TC)yr.s;
     TC t;
     if (kGeneric) {
       const double2 rc2 = y.rc[m2];
@@ -288,11 +287,12 @@ __device__ TC arad_distl2(const NbView &x, int cnt1, bool asc1, const NbView &y,
       t = ttab[trow + ty2];
     }
     v = v * t;
-    const TC ang = tfma(sb1, (TC)yr.b, tfma(sa1, (TC)yr.a, s1));
+    TC ang = TC(1) + a1 * (TC)yr.a;
+    ang = ang + b1 * (TC)yr.b;
     return v * ang;
   };
-  auto term = [&](int m2, TC d1, TC s1, TC sa1, TC sb1, int trow, double2 rc1) {
-    acc = acc + rec_value(y.rec[m2], kGeneric ? uint16_t(0) : y.ty[m2], m2, d1, s1, sa1, sb1, trow, rc1);
+  auto term = [&](int m2, TC d1, TC s1, TC a1, TC b1, int trow, double2 rc1) {
+    acc = acc + rec_value(y.rec[m2], kGeneric ? uint16_t(0) : y.ty[m2], m2, d1, s1, a1, b1, trow, rc1);
   };
   if (asc1 && asc2) {
     int lo = 0, hi = 0;
@@ -323,7 +323,7 @@ __device__ TC arad_distl2(const NbView &x, int cnt1, bool asc1, const NbView &y,
       // added to acc one after the other in m2 order; the next trip's two
       // records are read before this trip's arithmetic (indices clamped to
       // the list, the clamped reads are not used)
-      const TC s1 = (TC)xr.s, sa1 = s1 * (TC)xr.a, sb1 = s1 * (TC)xr.b;
+      const TC s1 = (TC)xr.s, a1 = (TC)xr.a, b1 = (TC)xr.b;
       const int last = kPadY ? cnt2 + 1 : cnt2 - 1;
       Rec r0 = y.rec[lo], r1 = y.rec[min(lo + 1, last)];
       uint16_t t0 = y.ty[lo], t1 = y.ty[min(lo + 1, last)];
@@ -333,8 +333,8 @@ __device__ TC arad_distl2(const NbView &x, int cnt1, bool asc1, const NbView &y,
         const int p1 = kPadY ? m2 + 3 : min(m2 + 3, last);
         const Rec n0 = y.rec[p0], n1 = y.rec[p1];
         const uint16_t u0 = y.ty[p0], u1 = y.ty[p1];
-        const TC v0 = rec_value(r0, t0, m2, d1, s1, sa1, sb1, trow, rc1);
-        const TC v1 = rec_value(r1, t1, m2 + 1, d1, s1, sa1, sb1, trow, rc1);
+        const TC v0 = rec_value(r0, t0, m2, d1, s1, a1, b1, trow, rc1);
+        const TC v1 = rec_value(r1, t1, m2 + 1, d1, s1, a1, b1, trow, rc1);
         acc = acc + v0;
         acc = acc + v1;
         r0 = n0;
@@ -342,7 +342,7 @@ __device__ TC arad_distl2(const NbView &x, int cnt1, bool asc1, const NbView &y,
         t0 = u0;
         t1 = u1;
       }
-      if (m2 < hi) acc = acc + rec_value(r0, t0, m2, d1, s1, sa1, sb1, trow, rc1);
+      if (m2 < hi) acc = acc + rec_value(r0, t0, m2, d1, s1, a1, b1, trow, rc1);
       }
       xr = xnext;
       xt = xtnext;
@@ -353,10 +353,10 @@ __device__ TC arad_distl2(const NbView &x, int cnt1, bool asc1, const NbView &y,
       const TC d1 = (TC)xr.d;
       const int trow = kGeneric ? 0 : (int)x.ty[m1] * kMaxTypes;
       const double2 rc1 = kGeneric ? x.rc[m1] : make_double2(0.0, 0.0);
-      const TC s1 = (TC)xr.s, sa1 = s1 * (TC)xr.a, sb1 = s1 * (TC)xr.b;
+      const TC s1 = (TC)xr.s, a1 = (TC)xr.a, b1 = (TC)xr.b;
       for (int m2 = 0; m2 < cnt2; ++m2) {
         const TC dr = (TC)y.rec[m2].d - d1;
-        if (dr * dr < maxg) term(m2, d1, s1, sa1, sb1, trow, rc1);
+        if (dr * dr < maxg) term(m2, d1, s1, a1, b1, trow, rc1);
       }
     }
   }
@@ -368,22 +368,23 @@ __device__ __forceinline__ NbView global_view(const NbSet &s, int64_t slot, int 
   return NbView{s.rec + o, s.ty + o, s.rc + o};
 }
 
-// LDS copies of the type-pair table and of exp_neg's 2^(j/32) table.
+// LDS copies of the type-pair table and of exp_neg's table.
 template <typename TC>
-__device__ void load_tables(TC *ttab, double *etab, const Ctl *ctl) {
+__device__ void load_tables(TC *ttab, ExpPair *etab, const Ctl *ctl) {
   for (int t = threadIdx.x; t < kMaxTypes * kMaxTypes; t += blockDim.x) ttab[t] = (TC)ctl->ttab[t];
-  for (int t = threadIdx.x; t < 128; t += blockDim.x) etab[t] = c_exp2_128[t];
+  for (int t = threadIdx.x; t < 128; t += blockDim.x) etab[t] = c_exp_pairs[t];
 }
 
-// bytes of the two tables at the start of a kernel's dynamic LDS
-constexpr size_t kTabBytes = sizeof(double) * (kMaxTypes * kMaxTypes + 128);
+// bytes of the two tables at the start of a kernel's dynamic LDS (the
+// type-pair table, then exp_neg's, 16-byte aligned)
+constexpr size_t kTabBytes = sizeof(double) * kMaxTypes * kMaxTypes + sizeof(ExpPair) * 128;
 
 // Self overlaps, one thread per slot (global-memory operands).
 template <typename TC>
 __global__ __launch_bounds__(256) void k_arad_self(NbSet set, int nslots, int amax, AradParams p,
                                                    const Ctl *__restrict__ ctl) {
   __shared__ TC ttab[kMaxTypes * kMaxTypes];
-  __shared__ double etab[128];
+  __shared__ ExpPair etab[128];
   load_tables(ttab, etab, ctl);
   __syncthreads();
   const int slot = blockIdx.x * blockDim.x + threadIdx.x;
@@ -467,12 +468,6 @@ __device__ void stage_molecule(MolStage &m, const NbSet &set, int mol, int natom
   }
 }
 
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
-
 struct PairArgs {
   int mode;  // FMLH_ARAD_*
   int nm1, nm2, amax, nsig;
@@ -498,17 +493,16 @@ __global__ __launch_bounds__(kPairThreads) void k_arad_pairs(NbSet set1, NbSet s
   extern __shared__ __attribute__((aligned(16))) char smem[];
   char *ptr = smem;
   TC *ttab = (TC *)ptr;
-  double *etab = (double *)(ptr + sizeof(double) * kMaxTypes * kMaxTypes);
+  ExpPair *etab = (ExpPair *)(ptr + sizeof(double) * kMaxTypes * kMaxTypes);
   ptr += kTabBytes;
-  double *red = (double *)ptr;
-  ptr += sizeof(double) * kMaxSig * (kPairThreads / 64);
+  double *chunk = (double *)ptr;  // kMaxSig x kPairThreads exponentials
+  ptr += sizeof(double) * kMaxSig * kPairThreads;
   MolStage mi = carve_stage(ptr, pa.amax, kStage);
   MolStage mj = carve_stage(ptr, pa.amax, kStage);
   load_tables(ttab, etab, ctl);
 
   const bool generic = ctl->generic != 0;
   const int amax = pa.amax;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const TC inv_w = (TC)p.inv_w, maxg = (TC)p.maxg;
   int cur_i = -1, cur_j = -1;
 
@@ -538,69 +532,67 @@ __global__ __launch_bounds__(kPairThreads) void k_arad_pairs(NbSet set1, NbSet s
     }
     __syncthreads();
 
-    double part[kMaxSig];
-#pragma unroll
-    for (int k = 0; k < kMaxSig; ++k) part[k] = 0.0;
-
+    // atom pairs t = a + arow b in chunks of kPairThreads (one per thread);
+    // each chunk's exponentials go to LDS and thread k adds them to its
+    // running sum for sigma k in t order: the reference's column-major sum
+    // (farad_kernels.f90:602-604, :216-217)
     const bool dist_mode = (pa.mode == FMLH_ARAD_DISTANCE);
+    const int nsig = (pa.mode == FMLH_ARAD_KERNEL) ? 1 : pa.nsig;
     const int arow = dist_mode ? amax : ni;
     const int npair_atoms = dist_mode ? amax * amax : ni * nj;
-    for (int t = threadIdx.x; t < npair_atoms; t += kPairThreads) {
-      const int a = t % arow, b = t / arow;
-      if (a >= ni || b >= nj) {  // DISTANCE mode zero padding (farad.f90:565)
-        pa.out[dist_index(i, j, a, b, pa.nm1, pa.nm2, amax)] = 0.0;
-        continue;
+    double acc = 0.0;  // thread k < nsig: the sum for sigma k
+    for (int t0 = 0; t0 < npair_atoms; t0 += kPairThreads) {
+      const int t = t0 + (int)threadIdx.x;
+      if (t < npair_atoms) {
+        const int a = t % arow, b = t / arow;
+        if (a >= ni || b >= nj) {  // DISTANCE mode zero padding (farad.f90:565)
+          pa.out[dist_index(i, j, a, b, pa.nm1, pa.nm2, amax)] = 0.0;
+        } else {
+          const int64_t gx = ((int64_t)i * amax + a) * amax, gy = ((int64_t)j * amax + b) * amax;
+          NbView x, y;
+          if (kStage) {
+            x = NbView{mi.rec + a * amax, mi.ty + a * amax, set1.rc + gx};
+            y = NbView{mj.rec + b * amax, mj.ty + b * amax, set2.rc + gy};
+          } else {
+            x = NbView{set1.rec + gx, set1.ty + gx, set1.rc + gx};
+            y = NbView{set2.rec + gy, set2.ty + gy, set2.rc + gy};
+          }
+          const TC S = generic ? arad_distl2<TC, true>(x, mi.cnt[a], mi.asc[a] != 0, y, mj.cnt[b],
+                                                       mj.asc[b] != 0, inv_w, maxg, ttab, etab,
+                                                       p.rw2, p.cw2)
+                               : arad_distl2<TC, false>(x, mi.cnt[a], mi.asc[a] != 0, y, mj.cnt[b],
+                                                        mj.asc[b] != 0, inv_w, maxg, ttab, etab,
+                                                        p.rw2, p.cw2);
+          // farad_kernels.f90:590-594
+          const double T = stoch(mi.zr[a] - mj.zr[b], mi.zc[a] - mj.zc[b], p.rw2, p.cw2);
+          double D = mi.self[a] + mj.self[b] - 2.0 * (double)S * T;
+          if (fabs(D) < p.eps) D = 0.0;
+          if (dist_mode) {
+            pa.out[dist_index(i, j, a, b, pa.nm1, pa.nm2, amax)] = D;
+          } else {
+            for (int k = 0; k < nsig; ++k)  // :211, :603
+              chunk[k * kPairThreads + threadIdx.x] = exp_neg_wide(D * pa.inv_s2[k], etab);
+          }
+        }
       }
-      const int64_t gx = ((int64_t)i * amax + a) * amax, gy = ((int64_t)j * amax + b) * amax;
-      NbView x, y;
-      if (kStage) {
-        x = NbView{mi.rec + a * amax, mi.ty + a * amax, set1.rc + gx};
-        y = NbView{mj.rec + b * amax, mj.ty + b * amax, set2.rc + gy};
-      } else {
-        x = NbView{set1.rec + gx, set1.ty + gx, set1.rc + gx};
-        y = NbView{set2.rec + gy, set2.ty + gy, set2.rc + gy};
+      if (dist_mode) continue;
+      __syncthreads();
+      if ((int)threadIdx.x < nsig) {
+        const double *ck = chunk + threadIdx.x * kPairThreads;
+        const int m = min(kPairThreads, npair_atoms - t0);
+        for (int u = 0; u < m; ++u) acc = acc + ck[u];
       }
-      const TC S = generic ? arad_distl2<TC, true>(x, mi.cnt[a], mi.asc[a] != 0, y, mj.cnt[b],
-                                                   mj.asc[b] != 0, inv_w, maxg, ttab, etab, p.rw2,
-                                                   p.cw2)
-                           : arad_distl2<TC, false>(x, mi.cnt[a], mi.asc[a] != 0, y, mj.cnt[b],
-                                                    mj.asc[b] != 0, inv_w, maxg, ttab, etab, p.rw2,
-                                                    p.cw2);
-      // farad_kernels.f90:590-594
-      const double T = stoch(mi.zr[a] - mj.zr[b], mi.zc[a] - mj.zc[b], p.rw2, p.cw2);
-      double D = mi.self[a] + mj.self[b] - 2.0 * (double)S * T;
-      if (fabs(D) < p.eps) D = 0.0;
-      if (dist_mode) {
-        pa.out[dist_index(i, j, a, b, pa.nm1, pa.nm2, amax)] = D;
-      } else if (pa.mode == FMLH_ARAD_KERNEL) {
-        part[0] += exp(pa.inv_s2[0] * D);  // farad_kernels.f90:211
-      } else {
-#pragma unroll
-        for (int k = 0; k < kMaxSig; ++k)
-          if (k < pa.nsig) part[k] += exp(D * pa.inv_s2[k]);  // farad_kernels.f90:603
-      }
+      __syncthreads();
     }
     if (dist_mode) continue;
-
-    const int nsig = (pa.mode == FMLH_ARAD_KERNEL) ? 1 : pa.nsig;
-#pragma unroll
-    for (int k = 0; k < kMaxSig; ++k) {
-      if (k < nsig) {
-        const double w = wave_sum(part[k]);
-        if (lane == 0) red[k * (kPairThreads / 64) + wave] = w;
-      }
-    }
-    __syncthreads();
     if ((int)threadIdx.x < nsig) {
       const int k = threadIdx.x;
-      double v = 0.0;
-      for (int w = 0; w < kPairThreads / 64; ++w) v += red[k * (kPairThreads / 64) + w];
       if (pa.mode == FMLH_ARAD_KERNEL) {
-        pa.out[(int64_t)i + (int64_t)pa.nm1 * j] = v;
-        pa.out[(int64_t)j + (int64_t)pa.nm1 * i] = v;
+        pa.out[(int64_t)i + (int64_t)pa.nm1 * j] = acc;
+        pa.out[(int64_t)j + (int64_t)pa.nm1 * i] = acc;
       } else {
-        pa.out[k + (int64_t)pa.nsig * ((int64_t)i + (int64_t)pa.nm1 * j)] = v;
-        if (pa.triangular) pa.out[k + (int64_t)pa.nsig * ((int64_t)j + (int64_t)pa.nm1 * i)] = v;
+        pa.out[k + (int64_t)pa.nsig * ((int64_t)i + (int64_t)pa.nm1 * j)] = acc;
+        if (pa.triangular) pa.out[k + (int64_t)pa.nsig * ((int64_t)j + (int64_t)pa.nm1 * i)] = acc;
       }
     }
   }
@@ -635,6 +627,7 @@ struct TileArgs {
   const int32_t *members;
   int32_t *counter;
   double *out;
+  int bfast;  // atom-pair order of a tile's threads: 0 a fastest, 1 set-2 atom fastest
 };
 
 // J (set-2 molecules per tile) for a set-1 molecule of ni atoms against the
@@ -847,7 +840,7 @@ __global__ __launch_bounds__(kPairThreads, kWaves) void k_arad_tiles(NbSet set1,
   extern __shared__ __attribute__((aligned(16))) char smem[];
   char *ptr = smem;
   TC *ttab = (TC *)ptr;
-  double *etab = (double *)(ptr + sizeof(double) * kMaxTypes * kMaxTypes);
+  ExpPair *etab = (ExpPair *)(ptr + sizeof(double) * kMaxTypes * kMaxTypes);
   ptr += kTabBytes;
   int *tile_slot = (int *)ptr;
   ptr += 16;
@@ -911,14 +904,31 @@ __global__ __launch_bounds__(kPairThreads, kWaves) void k_arad_tiles(NbSet set1,
 
     // a thread's atom pairs (rounds of kPairThreads; at most kTileRounds,
     // the host uses tiles only where amax^2 fits) keep their D values in
-    // registers until the per-sigma exponentials below
+    // registers until the per-sigma exponentials below.  Pair u of the tile
+    // is (jj, a, b): with bfast = 0, u = jj P + a + n_i b (a fastest); with
+    // bfast = 1, u = a (J s) + jj s + b, so a wave's lanes share the set-1
+    // atom a (one neighbour list, one m1 loop) and differ in the set-2 atom.
     double D0 = 0.0, D1 = 0.0, D2 = 0.0, D3 = 0.0;
     const int P = ni * s;
     const int total = J * P;
+    const int Js = J * s;
+    auto decode = [&](int u, int &jj, int &a, int &b) {
+      if (ta.bfast) {
+        a = u / Js;
+        const int bj = u - a * Js;
+        jj = bj / s;
+        b = bj - jj * s;
+      } else {
+        jj = u / P;
+        const int rem = u - jj * P;
+        a = rem % ni;
+        b = rem / ni;
+      }
+    };
     int nr = 0;
     for (int t = threadIdx.x; t < total; t += kPairThreads, ++nr) {
-      const int jj = t / P, rem = t - jj * P;
-      const int a = rem % ni, b = rem / ni;
+      int jj, a, b;
+      decode(t, jj, a, b);
       const int bj = jj * s + b;  // staged atom index of b
       const int j = ta.members[tl.off + jj];
       const int64_t xs = ((int64_t)i * amax + a) * amax;
@@ -936,30 +946,25 @@ __global__ __launch_bounds__(kPairThreads, kWaves) void k_arad_tiles(NbSet set1,
       D3 = nr == 3 ? D : D3;
     }
     __syncthreads();  // the J stage becomes the reduction buffer
-    // per sigma, one exponential per atom pair t = jj P + a + n_i b of the
-    // tile, stored at red[k total + t]
-    for (int k = 0; k < nsig; ++k) {
-      const double is2 = ta.inv_s2[k];
-      double *rk = red + k * total + threadIdx.x;
-      if (nr > 0) rk[0] = exp(D0 * is2);
-      if (nr > 1) rk[kPairThreads] = exp(D1 * is2);
-      if (nr > 2) rk[2 * kPairThreads] = exp(D2 * is2);
-      if (nr > 3) rk[3 * kPairThreads] = exp(D3 * is2);
+    // per sigma k, the exponential of atom pair (jj, a, b) goes to
+    // red[(a + n_i b) JK + jj nsig + k], JK = J nsig: column c = jj nsig + k
+    // of the buffer lists one molecule pair's terms in the reference's
+    // column-major order (a fastest, farad_kernels.f90:602-604, :216-217)
+    const int JK = J * nsig;
+    for (int r = 0; r < nr; ++r) {
+      const double D = r == 0 ? D0 : r == 1 ? D1 : r == 2 ? D2 : D3;
+      int jj, a, b;
+      decode(threadIdx.x + r * kPairThreads, jj, a, b);
+      double *rk = red + (int64_t)(a + ni * b) * JK + jj * nsig;
+      for (int k = 0; k < nsig; ++k) rk[k] = exp_neg_wide(D * ta.inv_s2[k], etab);
     }
     __syncthreads();
-    // one wave per (molecule pair, sigma) in turn: lane l adds the atom pairs
-    // t = l, l + 64, ... (t = a + n_i b, the reference's enumeration,
-    // farad_kernels.f90:602-604 and :211) in ascending order, then a
-    // butterfly adds the 64 lanes -- a fixed order that depends only on the
-    // molecule pair, whatever the tile shape
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int cidx = wave; cidx < J * nsig; cidx += kPairThreads / 64) {
-      const int jj = cidx / nsig, k = cidx % nsig;
-      const double *rk = red + k * total + jj * P;
+    // one thread per (molecule pair, sigma) adds its column sequentially: the
+    // reference's sum, in its order, whatever the tile shape
+    for (int c = threadIdx.x; c < JK; c += kPairThreads) {
+      const int jj = c / nsig, k = c - jj * nsig;
       double v = 0.0;
-      for (int l = lane; l < P; l += 64) v += rk[l];
-      v = wave_sum(v);
-      if (lane != 0) continue;
+      for (int u = 0; u < P; ++u) v = v + red[(int64_t)u * JK + c];
       const int j = ta.members[tl.off + jj];
       if (ta.mode == FMLH_ARAD_KERNEL) {
         ta.out[(int64_t)i + (int64_t)ta.nm1 * j] = v;
@@ -981,7 +986,7 @@ __global__ __launch_bounds__(256) void k_arad_atomic(NbSet set1, NbSet set2, int
                                                      int amax, int mode, PairArgs pa, AradParams p,
                                                      const Ctl *__restrict__ ctl) {
   __shared__ TC ttab[kMaxTypes * kMaxTypes];
-  __shared__ double etab[128];
+  __shared__ ExpPair etab[128];
   load_tables(ttab, etab, ctl);
   __syncthreads();
   const bool generic = ctl->generic != 0;
@@ -1008,7 +1013,8 @@ __global__ __launch_bounds__(256) void k_arad_atomic(NbSet set1, NbSet set2, int
 #pragma unroll
     for (int k = 0; k < kMaxSig; ++k)
       if (k < pa.nsig)
-        pa.out[k + (int64_t)pa.nsig * ((int64_t)i + (int64_t)na1 * j)] = exp(D * pa.inv_s2[k]);
+        pa.out[k + (int64_t)pa.nsig * ((int64_t)i + (int64_t)na1 * j)] =
+            exp_neg_wide(D * pa.inv_s2[k], etab);  // farad_kernels.f90:887
   }
 }
 
@@ -1063,7 +1069,7 @@ int device_cus() {
 }
 
 size_t pair_lds_bytes(int amax, bool stage) {
-  return kTabBytes + sizeof(double) * kMaxSig * (kPairThreads / 64) + 2 * stage_bytes(amax, stage);
+  return kTabBytes + sizeof(double) * kMaxSig * kPairThreads + 2 * stage_bytes(amax, stage);
 }
 
 constexpr size_t kLdsLimit = 65536;
@@ -1125,7 +1131,7 @@ std::tuple<int, int, int, int> tune_key(int mode, int amax, int precision, int n
   return std::make_tuple(kind, amax, precision, mode == FMLH_ARAD_KERNEL ? 1 : nsig);
 }
 
-fmlh_arad_tiling default_tiling() { return fmlh_arad_tiling{kPairThreads, 0, 16}; }
+fmlh_arad_tiling default_tiling() { return fmlh_arad_tiling{kPairThreads, 0, 16, 0}; }
 
 fmlh_arad_tiling lookup_tiling(int mode, int amax, int precision, int nsig) {
   std::lock_guard<std::mutex> lk(g_tune_mu);
@@ -1136,7 +1142,7 @@ fmlh_arad_tiling lookup_tiling(int mode, int amax, int precision, int nsig) {
 bool tiling_ok(const fmlh_arad_tiling &t) {
   return t.pairs_per_tile >= 64 && t.pairs_per_tile <= kTileRounds * kPairThreads &&
          (t.waves_per_simd == 0 || t.waves_per_simd == 4 || t.waves_per_simd == 5) &&
-         t.groups_per_cu >= 1 && t.groups_per_cu <= 64;
+         t.groups_per_cu >= 1 && t.groups_per_cu <= 64 && (t.pair_order == 0 || t.pair_order == 1);
 }
 
 int64_t max_tiles(int nm1, int nm2, bool tri) {
@@ -1321,6 +1327,7 @@ int arad_run(int mode, const double *q1, const double *zt1, const int32_t *n1, i
     ta.members = dm;
     ta.counter = counter;
     ta.out = out;
+    ta.bfast = t.pair_order;
     const int64_t tmax = std::max<int64_t>(max_tiles(nm1, nm2e, one_set), 1);
     const int grid = (int)std::min<int64_t>(tmax, (int64_t)device_cus() * t.groups_per_cu);
     const bool five = t.waves_per_simd == 5 || (t.waves_per_simd == 0 && L.bytes * 5 <= kLdsPerCu);
@@ -1413,8 +1420,10 @@ int fmlh_arad_autotune(int mode, const double *q1, const double *zt1, const int3
   for (int lanes : {128, 256, 384, 512, 768, 1024}) {
     const TileLds L = tile_lds(amax, nsig_eff, lanes);
     if (L.bytes > kLdsLimit) continue;
-    cand.push_back(fmlh_arad_tiling{lanes, 4, 16});
-    if (L.bytes * 5 <= kLdsPerCu) cand.push_back(fmlh_arad_tiling{lanes, 5, 16});
+    for (int order = 0; order < 2; ++order) {
+      cand.push_back(fmlh_arad_tiling{lanes, 4, 16, order});
+      if (L.bytes * 5 <= kLdsPerCu) cand.push_back(fmlh_arad_tiling{lanes, 5, 16, order});
+    }
   }
   if (cand.empty()) return -10;
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1444,8 +1453,8 @@ int fmlh_arad_autotune(int mode, const double *q1, const double *zt1, const int3
       best_k = (int)k;
     }
   }
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   if (rc) return rc;
   // leave `out` computed with the chosen tiling
   rc = arad_run(mode, q1, zt1, n1, nm1, q2, zt2, n2, nm2, amax, sigmas_host, nsig, width, cut,

@@ -43,7 +43,7 @@ namespace {
 
 using namespace fmlh;
 
-__constant__ double c_exp2_128[128] = {FMLH_EXP2_128_VALUES};
+__constant__ ExpPair c_exp_pairs[128] = {FMLH_EXP2_128_PAIRS};
 
 constexpr int kMaxAmax = 64;    // atoms per molecule (one lane each in the prep)
 constexpr int kMaxTypes = 16;   // distinct elements within one atom's cut
@@ -329,9 +329,9 @@ __device__ void load_pdg(double *pdg, const ArasParams &P, const ArasElems *E) {
 __global__ __launch_bounds__(64) void k_aras_self(ArasSet S, ArasParams P, const ArasElems *E) {
   __shared__ double pdg[(kMaxTypes + 1) * (kMaxTypes + 1)];
   __shared__ double term[64];
-  __shared__ double etab[128];
+  __shared__ ExpPair etab[128];
   load_pdg(pdg, P, E);
-  for (int t = threadIdx.x; t < 128; t += blockDim.x) etab[t] = c_exp2_128[t];
+  for (int t = threadIdx.x; t < 128; t += blockDim.x) etab[t] = c_exp_pairs[t];
   __syncthreads();
   const int amax = P.amax;
   const int a = (int)(blockIdx.x / amax), i = (int)(blockIdx.x % amax);
@@ -424,11 +424,12 @@ struct LaneAtom {
 // elements, whose similarity row pd(p2_t, g) sits in registers; NT2 == 0:
 // any count nt2, similarities read from LDS.  The window factor's exponent
 // lies in (-16, 0] ((8 d_width)^2 (-1/(4 d_width^2)) = -16), where exp_neg
-// (exp_neg.h: a 2^(j/128) table in LDS and a degree-5 polynomial, within
-// about one ulp) evaluates it.
+// (exp_neg.h: Tang's table-driven method, a head/tail 2^(j/128) table in
+// LDS, rounding like a correctly rounded exp but for ~0.004 ulp around
+// midpoints) evaluates it.
 template <int GB, int NT2>
 __device__ __forceinline__ double atom_pair(const ArasSet &A, const ArasUniform &B, const ArasParams &P,
-                                            const double *pdg, const double *etab, int G1,
+                                            const double *pdg, const ExpPair *etab, int G1,
                                             const LaneAtom &L, unsigned gmask, int64_t x2,
                                             int nt2, double aad) {
   const int amax = P.amax;
@@ -539,7 +540,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
   __shared__ double pdg[(kMaxTypes + 1) * (kMaxTypes + 1)];
   __shared__ double red[64 * kMaxSig];
   __shared__ int lane_mol[64], lane_atom[64], natoms_s, maxmol_s;
-  __shared__ double etab[128];
+  __shared__ ExpPair etab[128];
   const int lane = threadIdx.x;
   const int64_t w = blockIdx.x;
   const int ib = (int)(w % nblk);
@@ -547,7 +548,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
   const int b0 = blk ? blk[ib] : ib * per_blk;
   const int b1 = blk ? blk[ib + 1] : min(A.nm, (ib + 1) * per_blk);
   load_pdg(pdg, P, E);
-  for (int t = lane; t < 128; t += 64) etab[t] = c_exp2_128[t];
+  for (int t = lane; t < 128; t += 64) etab[t] = c_exp_pairs[t];
   if (lane == 0) {
     int c = 0, mx = -1;
     for (int b = b0; b < b1; ++b) {

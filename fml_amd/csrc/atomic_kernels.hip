@@ -27,7 +27,7 @@
 // from the matrix cores for the fast squared-distance form (|a|^2 + |b|^2 -
 // 2 a.b, 16x16x4 FP64 MFMA tiles, as the dense Gaussian kernel), or from the
 // distance matrix itself.
-// Exponentials use the table-driven exp_neg_wide (<= 1 ulp).  The epilogue passes over the tile in
+// Exponentials use the table-driven exp_neg_wide (Tang's method, rounding like a correctly rounded exp).  The epilogue passes over the tile in
 // two 64-row halves through LDS: per row, the exponentials of each column
 // molecule are summed in column order and stored over that molecule's first
 // column (step A); per molecule pair the row sums are added in row order
@@ -60,7 +60,7 @@ constexpr int kMaxAmax = 64;        // largest molecule
 // on the matrix cores; kMatrix: distances given.
 enum Source { kL2 = 0, kL1 = 1, kL2M = 2, kMatrix = 3 };
 
-__constant__ double c_exp2_128[128] = {FMLH_EXP2_128_VALUES};
+__constant__ ExpPair c_exp_pairs[128] = {FMLH_EXP2_128_PAIRS};
 
 struct Side {
   const double *x;        // atom rows (kL1/kL2), ld doubles apart
@@ -134,7 +134,7 @@ __global__ __launch_bounds__(256, 2) void k_atom_tiles(TileArgs a) {
     double D[kHalf * kDS];
   } sm;
   __shared__ double carry[kMaxSig * kT];
-  __shared__ double etab[128];
+  __shared__ ExpPair etab[128];
 
   int bx, by;  // column block, row block
   if (!xcd_tile(a.c.nb, a.r.nb, bx, by)) return;
@@ -149,7 +149,7 @@ __global__ __launch_bounds__(256, 2) void k_atom_tiles(TileArgs a) {
     return;
   }
   const int tx = tid & 15, ty = tid >> 4;
-  if (tid < 128) etab[tid] = c_exp2_128[tid];
+  if (tid < 128) etab[tid] = c_exp_pairs[tid];
   double acc[8][8];
   TileAcc<double, kT> mac;
 

@@ -16,7 +16,7 @@ import json
 
 from . import _native
 
-#: (mode, amax, precision, nsig) -> (pairs_per_tile, waves_per_simd, groups_per_cu)
+#: (mode, amax, precision, nsig) -> (pairs_per_tile, waves_per_simd, groups_per_cu, pair_order)
 RESULTS: dict = {}
 
 
@@ -33,7 +33,8 @@ def current(mode: int, amax: int, precision: int, nsig: int):
 
 def save(path: str) -> None:
     rows = [{"mode": k[0], "amax": k[1], "precision": k[2], "nsig": k[3],
-             "pairs_per_tile": v[0], "waves_per_simd": v[1], "groups_per_cu": v[2]}
+             "pairs_per_tile": v[0], "waves_per_simd": v[1], "groups_per_cu": v[2],
+             "pair_order": v[3]}
             for k, v in sorted(RESULTS.items())]
     with open(path, "w") as f:
         json.dump(rows, f, indent=1)
@@ -44,7 +45,8 @@ def load(path: str) -> int:
     with open(path) as f:
         rows = json.load(f)
     for r in rows:
-        t = _native.AradTiling(int(r["pairs_per_tile"]), int(r["waves_per_simd"]), int(r["groups_per_cu"]))
+        t = _native.AradTiling(int(r["pairs_per_tile"]), int(r["waves_per_simd"]), int(r["groups_per_cu"]),
+                               int(r.get("pair_order", 0)))
         _native.call("fmlh_arad_tiling_set", int(r["mode"]), int(r["amax"]), int(r["precision"]),
                      int(r["nsig"]), ctypes.byref(t))
         record(r["mode"], r["amax"], r["precision"], r["nsig"], t)

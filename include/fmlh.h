@@ -323,13 +323,17 @@ int fmlh_aras_kernels(int mode, const double *x1, const int32_t *n1,
  * tiling: tile shape and launch shape, or NULL for the library's current
  * choice for (mode, amax, precision, nsig): an autotuned entry when
  * fmlh_arad_autotune / fmlh_arad_tiling_set provided one, else the
- * built-in default {256, 0, 16}. */
+ * built-in default {256, 0, 16, 0}. */
 typedef struct fmlh_arad_tiling {
   int pairs_per_tile; /* target atom pairs J * n_i * s per tile, 64..1024 */
   int waves_per_simd; /* register budget of the tile kernel: 4 or 5 waves
                          per SIMD; 0 = 5 where five workgroups fit a CU's
                          LDS, else 4 */
   int groups_per_cu;  /* persistent workgroups launched per CU, 1..64 */
+  int pair_order;     /* order of a tile's atom pairs over its threads:
+                         0 = set-1 atom fastest, 1 = set-2 atom fastest
+                         (a wave's lanes share one set-1 atom); results are
+                         bitwise the same for every tiling */
 } fmlh_arad_tiling;
 
 size_t fmlh_arad_ws_bytes(int nm1, int nm2, int amax, int mode);
@@ -343,7 +347,8 @@ int fmlh_arad_kernels(int mode, const double *q1, const double *zt1,
 
 /* Autotuner (no reference counterpart): runs fmlh_arad_kernels on the given
  * inputs once per candidate tiling (pairs_per_tile in {128, 256, 384, 512,
- * 768, 1024} x waves_per_simd in {4, 5} where five fit), `reps` timed calls
+ * 768, 1024} x waves_per_simd in {4, 5} where five fit x pair_order in
+ * {0, 1}), `reps` timed calls
  * each after one warm-up, timed with HIP events on `stream`; stores the
  * fastest as the library's choice for (mode, amax, precision, nsig) and
  * returns it with its mean time per call.  Synchronises the stream; out is

@@ -7,6 +7,7 @@ built (make -C oracle ref):
     python tests/golden/make_golden.py inverse    # inverse.npz only
     python tests/golden/make_golden.py atomic     # atomic.npz only
     python tests/golden/make_golden.py aras       # aras.npz only
+    python tests/golden/make_golden.py arad500    # arad500.npz only
 
 Inputs: the reference's test molecules tests/xyz/01..10.xyz (copied to
 tests/golden/xyz) and small seeded synthetic sets.  Outputs go next to this
@@ -155,8 +156,27 @@ def aras_fixtures():
                                                            d_width=0.3, cut=4.0, order=2))
 
 
+def arad500_fixtures(rows=8):
+    """The global ARAD kernel workload: 500 molecules (5-23 atoms, seed 0),
+    descriptors (500, 23, 5, 23), width 0.2, cut 5.0, r_width 1.0, c_width
+    0.5, sigmas {0.1, 1, 10}.  Stored: the reference's kernel rows
+    0..rows-1 against all 500 molecules, and the atom counts as a check that
+    the generator reproduces the set."""
+    ms = synth.generate(500, seed=0, nmin=5, nmax=23)
+    ar = ARAD(maxMolSize=23, maxAts=23, cut=5.0)
+    X = np.array([ar.describe(ms.molecule(m)[1], ms.molecule(m)[0]) for m in range(ms.nmol)])
+    z, n = arad_types([ms.molecule(m)[0] for m in range(ms.nmol)], 23)
+    sig = np.array([0.1, 1.0, 10.0])
+    k = oracle.ref_kernels_arad(X[:rows], X, z[:rows], z, n[:rows], n, sig, 0.2, 5.0, 1.0, 0.5)
+    np.savez_compressed(os.path.join(HERE, "arad500.npz"), natoms=n, sigmas=sig, rows=np.arange(rows),
+                        kernels=k)
+
+
 if __name__ == "__main__":
-    if sys.argv[1:] == ["inverse"]:
+    if sys.argv[1:] == ["arad500"]:
+        assert oracle.ref_available(), "build oracle/_ref first (make -C oracle ref)"
+        arad500_fixtures()
+    elif sys.argv[1:] == ["inverse"]:
         assert oracle.ref_available(), "build oracle/_ref first (make -C oracle ref)"
         inverse_fixtures()
     elif sys.argv[1:] == ["atomic"]:
@@ -169,3 +189,4 @@ if __name__ == "__main__":
         main()
         atomic_fixtures()
         aras_fixtures()
+        arad500_fixtures()

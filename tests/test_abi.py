@@ -50,19 +50,20 @@ def test_version_and_errors_without_gpu():
     assert lib.fmlh_arad_kernels(9, None, None, None, 1, None, None, None, 1, 5, None, 1, 0.2, 5.0,
                                  1.0, 0.5, 0, None, None, 0, None, None) == -1
     # an out-of-range tiling is rejected before any device work
-    bad = nat.AradTiling(16, 4, 16)
+    bad = nat.AradTiling(16, 4, 16, 0)
     import ctypes
     sig = (ctypes.c_double * 1)(1.0)
     assert lib.fmlh_arad_kernels(0, None, None, None, 1, None, None, None, 1, 5, sig, 1, 0.2, 5.0,
                                  1.0, 0.5, 0, None, None, 0, ctypes.byref(bad), None) == -21
     assert lib.fmlh_arad_tiling_set(0, 23, 0, 3, ctypes.byref(bad)) == -5
-    good = nat.AradTiling(512, 5, 8)
+    assert lib.fmlh_arad_tiling_set(0, 23, 0, 3, ctypes.byref(nat.AradTiling(512, 5, 8, 2))) == -5
+    good = nat.AradTiling(512, 5, 8, 1)
     assert lib.fmlh_arad_tiling_set(0, 23, 1, 3, ctypes.byref(good)) == 0
     got = nat.AradTiling()
     assert lib.fmlh_arad_tiling_get(0, 23, 1, 3, ctypes.byref(got)) == 0
-    assert got.as_tuple() == (512, 5, 8)
+    assert got.as_tuple() == (512, 5, 8, 1)
     assert lib.fmlh_arad_tiling_get(0, 23, 1, 2, ctypes.byref(got)) == 0
-    assert got.as_tuple() == (256, 0, 16)  # untuned key: the built-in default
+    assert got.as_tuple() == (256, 0, 16, 0)  # untuned key: the built-in default
     assert lib.fmlh_graph_launch(None, None) == -1
     assert lib.fmlh_graph_destroy(None) == 0
     describe = lib.fmlh_arad_describe

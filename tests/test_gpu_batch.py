@@ -48,10 +48,12 @@ def test_tile_shapes_bitwise_equal(gpu, uneven, mode):
     (X1, z1, n1), (X2, z2, n2) = uneven
     sig = [1.0, 4.0, 9.0]
     ref = None
-    for pairs in (64, 128, 256, 384, 512, 768, 1024):
-        for waves in (4, 5):
-            for gpc in (1, 16):
-                t = nat.AradTiling(pairs, waves, gpc)
+    shapes = [(p, w, g, o) for p in (64, 128, 256, 384, 512, 768, 1024) for w in (4, 5)
+              for g in (1, 16) for o in (0, 1)]
+    for pairs, waves, gpc, order in shapes:
+        if True:
+            if True:
+                t = nat.AradTiling(pairs, waves, gpc, order)
                 c = AradKernelCall(X1, z1, n1, X2, z2, n2, sig, mode=mode, cut_distance=5.0, tiling=t)
                 c.run()
                 r = c.result().cpu().numpy()
@@ -65,7 +67,7 @@ def test_tile_shapes_bitwise_equal(gpu, uneven, mode):
                         ko = oracle.kernel_arad(X1, z1, n1, 1.0)
                     assert _rel(r, ko) < 1e-12, mode
                 else:
-                    assert np.array_equal(r, ref), (mode, pairs, waves, gpc)
+                    assert np.array_equal(r, ref), (mode, pairs, waves, gpc, order)
 
 
 def test_autotune_picks_a_candidate_and_keeps_results(gpu, uneven, tmp_path):
@@ -79,13 +81,13 @@ def test_autotune_picks_a_candidate_and_keeps_results(gpu, uneven, tmp_path):
     before = c.result().cpu().numpy()
     best, ms = c.autotune(sample_rows=16, reps=2)
     assert best.pairs_per_tile in (128, 256, 384, 512, 768, 1024)
-    assert best.waves_per_simd in (4, 5) and ms > 0
+    assert best.waves_per_simd in (4, 5) and best.pair_order in (0, 1) and ms > 0
     assert np.array_equal(c.result().cpu().numpy(), before)
     # the library now uses the tuned tiling for this key by default
     assert tuning.current(nat.ARAD_GLOBAL, 29, 0, 2).as_tuple() == best.as_tuple()
     path = str(tmp_path / "tiling.json")
     tuning.save(path)
-    nat.call("fmlh_arad_tiling_set", nat.ARAD_GLOBAL, 29, 0, 2, ctypes.byref(nat.AradTiling(256, 0, 16)))
+    nat.call("fmlh_arad_tiling_set", nat.ARAD_GLOBAL, 29, 0, 2, ctypes.byref(nat.AradTiling(256, 0, 16, 0)))
     assert tuning.load(path) >= 1
     assert tuning.current(nat.ARAD_GLOBAL, 29, 0, 2).as_tuple() == best.as_tuple()
 

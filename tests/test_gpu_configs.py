@@ -1,0 +1,63 @@
+"""GPU parity at the benchmark configurations' own sizes.
+
+  * global ARAD, 500 x 500 molecules (5-23 atoms, seed 0), sigmas
+    {0.1, 1, 10}: the full kernel on the device; rows 0-7 against the
+    reference's own output (tests/golden/arad500.npz, written by the flang
+    build), rows 0-63 against the oracle; the target is 1e-12 relative on
+    every entry, at every sigma.  The device overlap rounds every operation
+    as the reference does (its exp and sin as the host libm's), so entries
+    are expected bit-identical; the test states the bitwise fraction.
+
+Each test prints the figures it checks (-s shows them).
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from fml_amd import synth
+from fml_amd.arad import ARAD
+from fml_amd.kernels.kernels import arad_types
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _rel(a, b, floor=1e-300):
+    m = np.abs(b) > floor
+    return float(np.max(np.abs(a[m] - b[m]) / np.abs(b[m]))) if m.any() else 0.0
+
+
+@pytest.fixture(scope="module")
+def arad500():
+    ms = synth.generate(500, seed=0, nmin=5, nmax=23)
+    ar = ARAD(maxMolSize=23, maxAts=23, cut=5.0)
+    X = np.array([ar.describe(ms.molecule(m)[1], ms.molecule(m)[0]) for m in range(ms.nmol)])
+    Z = [ms.molecule(m)[0] for m in range(ms.nmol)]
+    z, n = arad_types(Z, 23)
+    return X, Z, z, n
+
+
+def test_global_arad_500x500(gpu, arad500):
+    from fml_amd.kernels import get_atomic_kernels_arad
+    X, Z, z, n = arad500
+    g = np.load(os.path.join(GOLDEN, "arad500.npz"))
+    assert np.array_equal(n, g["natoms"]), "the generator no longer reproduces the 500-molecule set"
+    sig = [0.1, 1.0, 10.0]
+    k = get_atomic_kernels_arad(X, X, Z, Z, sig, cut_distance=5.0)
+    assert k.shape == (3, 500, 500)
+    assert np.all(np.isfinite(k)) and np.all(k > 0)
+    rows = g["rows"]
+    kr = g["kernels"]
+    rel_ref = [_rel(k[s][rows], kr[s]) for s in range(3)]
+    same_ref = float(np.mean(k[:, rows] == kr))
+    ko = oracle.kernels_arad(X[:64], X, z[:64], z, n[:64], n, sig, 0.2, 5.0, 1.0, 0.5)
+    rel_or = [_rel(k[s][:64], ko[s]) for s in range(3)]
+    same_or = float(np.mean(k[:, :64] == ko))
+    print(f"\n500x500 global ARAD: vs reference rows 0-7 max rel {rel_ref}, bitwise {same_ref:.6f}; "
+          f"vs oracle rows 0-63 max rel {rel_or}, bitwise {same_or:.6f}")
+    for s in range(3):
+        assert rel_ref[s] <= 1e-12, (sig[s], rel_ref[s])
+        assert rel_or[s] <= 1e-12, (sig[s], rel_or[s])
