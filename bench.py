@@ -42,9 +42,12 @@ PEAK_FP32_TFLOPS = 157.3
 PEAK_FP64_VALU_TOPS = 39.3         # FP64 VALU lane-operations/s: the FP64 vector peak / 2 (FMA = 1 op)
 PEAK_FP32_VALU_TOPS = 78.6         # FP32 VALU lane-operations/s: the FP32 vector peak / 2
 
-# Algorithmic FP64 operations per inner (m1, m2) evaluation of the overlap
-# term: 13 arithmetic operations plus the exponential counted as 20.
-ARAD_OPS_PER_EVAL = 33
+# FP64 operations per inner (m1, m2) evaluation of the overlap term: the
+# reference's 12 arithmetic operations (difference, square, scale, the
+# products by s1, s2 and the type factor T, the angular (1 + a1 a2) + b1 b2,
+# the product and the accumulation) plus the band test, and the exponential
+# at the 12 FP64 instructions its compiled form issues.
+ARAD_OPS_PER_EVAL = 25
 
 # HBM bytes per launch of k_arad_tiles (FP64, the tuned tile shape), from
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this script, each scaled by
@@ -162,6 +165,7 @@ def arad_workload(args, big=False, precision=0):
                    "tiling": {"pairs_per_tile": tiling.pairs_per_tile,
                               "waves_per_simd": tiling.waves_per_simd,
                               "groups_per_cu": tiling.groups_per_cu,
+                              "pair_order": tiling.pair_order,
                               "autotuned_on": f"{512 if big else nm} x {nm} molecules",
                               "autotune_s": round(tune_s, 2),
                               "tuned_ms_per_call": round(tuned_ms, 3)}},

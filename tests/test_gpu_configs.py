@@ -48,7 +48,7 @@ def test_global_arad_500x500(gpu, arad500):
     sig = [0.1, 1.0, 10.0]
     k = get_atomic_kernels_arad(X, X, Z, Z, sig, cut_distance=5.0)
     assert k.shape == (3, 500, 500)
-    assert np.all(np.isfinite(k)) and np.all(k > 0)
+    assert np.all(np.isfinite(k)) and np.all(k >= 0)
     rows = g["rows"]
     kr = g["kernels"]
     rel_ref = [_rel(k[s][rows], kr[s]) for s in range(3)]
@@ -61,3 +61,5 @@ def test_global_arad_500x500(gpu, arad500):
     for s in range(3):
         assert rel_ref[s] <= 1e-12, (sig[s], rel_ref[s])
         assert rel_or[s] <= 1e-12, (sig[s], rel_or[s])
+    # measured on MI355X: every entry equal to the reference's
+    assert same_ref == 1.0 and same_or == 1.0
