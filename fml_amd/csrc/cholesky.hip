@@ -30,6 +30,8 @@
 // one-workgroup triangular solve of the block.
 #include <math.h>
 
+#include <map>
+#include <memory>
 #include <mutex>
 
 #include "common.h"
@@ -248,37 +250,47 @@ int trailing_update(T *a, int64_t lda, int n, int r0, int kb, int ke, int32_t *i
   return 0;
 }
 
-// One auxiliary stream (non-blocking, default priority) and two events per
-// device, created on first use, for the factorisation's look-ahead.
+// Auxiliary streams (non-blocking, default priority) with two events each,
+// created on first use, for the factorisation's look-ahead: one per device,
+// and one per device and CU mask for callers on a CU-masked stream made by
+// fmlh_stream_create, so the look-ahead's panel work stays on the CUs the
+// caller's stream may use.
 struct Aux {
   hipStream_t s = nullptr;
   hipEvent_t fork = nullptr, join = nullptr;
   std::mutex *enqueue = nullptr;  // held while one factorisation enqueues on s
 };
 
-bool aux_for_device(Aux &out) {
+bool aux_for_stream(hipStream_t caller, Aux &out) {
   static std::mutex mu;
-  static std::mutex enqueue_mu[64];
-  static Aux table[64];
+  static std::map<std::pair<int, std::vector<uint32_t>>, Aux> table;
+  static std::map<std::pair<int, std::vector<uint32_t>>, std::unique_ptr<std::mutex>> locks;
   int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) {
+  if (hipGetDevice(&dev) != hipSuccess) {
     (void)hipGetLastError();
     return false;
   }
+  std::vector<uint32_t> mask;  // empty: every CU
+  const bool masked = stream_cu_mask(caller, mask);
+  const auto key = std::make_pair(dev, mask);
   std::lock_guard<std::mutex> lk(mu);
-  Aux &x = table[dev];
-  if (x.s == nullptr) {
+  auto it = table.find(key);
+  if (it == table.end()) {
     Aux y;
-    if (hipStreamCreateWithFlags(&y.s, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&y.fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&y.join, hipEventDisableTiming) != hipSuccess) {
+    hipError_t e = masked ? hipExtStreamCreateWithCUMask(&y.s, (uint32_t)mask.size(), mask.data())
+                          : hipStreamCreateWithFlags(&y.s, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&y.fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&y.join, hipEventDisableTiming);
+    if (e != hipSuccess) {
       (void)hipGetLastError();
       return false;
     }
-    y.enqueue = &enqueue_mu[dev];
-    x = y;
+    auto &m = locks[key];
+    m.reset(new std::mutex);
+    y.enqueue = m.get();
+    it = table.emplace(key, y).first;
   }
-  out = x;
+  out = it->second;
   return true;
 }
 
@@ -302,7 +314,7 @@ int potrf_impl(T *a, int64_t lda, int n, int32_t *info, hipStream_t st) {
       ahead = false;   // nothing is created or forked while a graph is captured
     }
   }
-  if (ahead) ahead = aux_for_device(aux);
+  if (ahead) ahead = aux_for_stream(st, aux);
   // Calls from several host threads share the device's auxiliary stream and
   // events: one factorisation enqueues at a time (each event wait takes the
   // event's latest record, so serialised enqueues keep every fork and join

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "fmlh.h"
 
 namespace fmlh {
@@ -15,6 +17,11 @@ int record_error(hipError_t e, const char *where);
 // unless fmlh_timing_enable(1) was called).
 void timing_begin(hipStream_t st);
 void timing_end(hipStream_t st);
+
+// The CU mask of a stream made by fmlh_stream_create (false for any other
+// stream).  Work the library forks off such a stream onto a stream of its
+// own uses a stream with the same mask.
+bool stream_cu_mask(hipStream_t st, std::vector<uint32_t> &mask);
 
 // Checks the launch that was just queued.
 inline int check_launch(const char *where) {

@@ -392,8 +392,10 @@ int fmlh_gemv_t(const double *k, int64_t ldk, int na, int nb, const double *alph
 size_t fmlh_krr_predict_ws_bytes(int nmol, int ntrain, int nmax) {
   if (nmol < 0 || ntrain < 0 || nmax < 1) return 0;
   const size_t packed = (size_t)nmax * (nmax + 1) / 2;
+  // the kernel block is taken with at least one column, also for an empty
+  // training set (whose predictions are zeros)
   return WsCarver::need(sizeof(double) * (size_t)nmol * packed) +
-         WsCarver::need(sizeof(double) * (size_t)nmol * (size_t)ntrain);
+         WsCarver::need(sizeof(double) * (size_t)nmol * (size_t)(ntrain > 0 ? ntrain : 1));
 }
 
 int fmlh_coulomb_sorted(const double *z, const double *r, const int32_t *offsets, int nmol,

@@ -22,7 +22,7 @@ int record_error(hipError_t e, const char *where) {
 
 extern "C" {
 
-int fmlh_abi_version(void) { return 2; }
+int fmlh_abi_version(void) { return 3; }
 
 const char *fmlh_last_error(void) { return g_last_error.c_str(); }
 
@@ -177,6 +177,28 @@ int fmlh_graph_destroy(fmlh_graph *graph) {
 
 extern "C" {
 
+}  // extern "C"
+
+namespace {
+std::mutex g_mask_mu;
+std::vector<std::pair<hipStream_t, std::vector<uint32_t>>> g_masked;  // streams made by fmlh_stream_create
+}  // namespace
+
+namespace fmlh {
+bool stream_cu_mask(hipStream_t st, std::vector<uint32_t> &mask) {
+  if (st == nullptr) return false;
+  std::lock_guard<std::mutex> lk(g_mask_mu);
+  for (const auto &e : g_masked)
+    if (e.first == st) {
+      mask = e.second;
+      return true;
+    }
+  return false;
+}
+}  // namespace fmlh
+
+extern "C" {
+
 int fmlh_stream_create(int cu_count, void **stream) {
   FMLH_ARG_CHECK(stream != nullptr, 2);
   *stream = nullptr;
@@ -188,12 +210,24 @@ int fmlh_stream_create(int cu_count, void **stream) {
   hipStream_t s = nullptr;
   hipError_t e = hipExtStreamCreateWithCUMask(&s, (uint32_t)mask.size(), mask.data());
   if (e != hipSuccess) return fmlh::record_error(e, "hipExtStreamCreateWithCUMask");
+  {
+    std::lock_guard<std::mutex> lk(g_mask_mu);
+    g_masked.emplace_back(s, mask);
+  }
   *stream = s;
   return 0;
 }
 
 int fmlh_stream_destroy(void *stream) {
   FMLH_ARG_CHECK(stream != nullptr, 1);
+  {
+    std::lock_guard<std::mutex> lk(g_mask_mu);
+    for (size_t i = 0; i < g_masked.size(); ++i)
+      if (g_masked[i].first == fmlh::as_stream(stream)) {
+        g_masked.erase(g_masked.begin() + (long)i);
+        break;
+      }
+  }
   hipError_t e = hipStreamDestroy(fmlh::as_stream(stream));
   if (e != hipSuccess) return fmlh::record_error(e, "hipStreamDestroy");
   return 0;

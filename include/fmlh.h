@@ -57,7 +57,10 @@ int fmlh_init(void);
  * Destroy a stream only when nothing can still record work or events on it:
  * a stream handed to torch (ExternalStream) is recorded on by torch's
  * pinned-memory allocator whenever a pinned block used on it is freed, so
- * the serving loop keeps its streams for the life of the process. */
+ * the serving loop keeps its streams for the life of the process.
+ * Work an entry point forks onto a stream of its own (fmlh_potrf's panel
+ * look-ahead) runs on a stream with the same CU mask when the caller's
+ * stream was made here. */
 int fmlh_stream_create(int cu_count, void **stream);
 int fmlh_stream_destroy(void *stream);
 

@@ -42,7 +42,7 @@ def test_binding_covers_header():
 
 def test_version_and_errors_without_gpu():
     lib = nat.load()
-    assert lib.fmlh_abi_version() == 2
+    assert lib.fmlh_abi_version() == 3
     # argument validation happens before any device call
     assert lib.fmlh_potrf(None, 1, -1, None, None, 0, None) == -3
     assert lib.fmlh_gaussian_kernel(None, 0, 5, None, 0, 5, 3, 1.0, None, 5, None) == -2
@@ -84,3 +84,13 @@ def test_no_cpu_fallback(monkeypatch):
     monkeypatch.setattr(nat, "_lib", None)
     with pytest.raises(nat.NativeError):
         nat.load("/nonexistent/libfmlhip.so")
+
+
+def test_predict_workspace_covers_empty_training_set():
+    """fmlh_krr_predict takes a kernel block of at least one column, also when
+    the training set is empty (its predictions are then zeros): the size
+    query must include it."""
+    lib = nat.load()
+    for nmol in (1, 7, 300):
+        assert lib.fmlh_krr_predict_ws_bytes(nmol, 0, 23) == lib.fmlh_krr_predict_ws_bytes(nmol, 1, 23)
+        assert lib.fmlh_krr_predict_ws_bytes(nmol, 0, 23) > 0

@@ -123,6 +123,8 @@ def test_potrf_lookahead_concurrent_threads(gpu):
         ref.append(f)
     out = [m.clone() for m in mats]
     streams = [torch.cuda.Stream() for _ in mats]
+    for st in streams:  # the clones (on the current stream) come first
+        st.wait_stream(torch.cuda.current_stream())
     infos = [None, None]
     barrier = threading.Barrier(2)
 

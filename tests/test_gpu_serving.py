@@ -104,3 +104,23 @@ def test_serving_rejects_oversize_molecule_only(gpu):
     alone = srv.predict_batch([reqs[2]])[0]
     assert np.array_equal(alone, out[2])
     srv.close()
+
+
+def test_predict_empty_training_set(gpu):
+    """fmlh_krr_predict against an empty training set: zeros, through the C
+    ABI with exactly the workspace the size query reports."""
+    import torch
+    import fml_amd._native as nat
+    ms = synth.generate(5, seed=15, nmin=3, nmax=23)
+    z = torch.tensor(ms.charges, dtype=torch.float64, device="cuda")
+    r = torch.tensor(ms.coords, dtype=torch.float64, device="cuda")
+    off = torch.tensor(ms.offsets, dtype=torch.int32, device="cuda")
+    y = torch.full((5,), 7.0, dtype=torch.float64, device="cuda")
+    status = torch.zeros(5, dtype=torch.int32, device="cuda")
+    wsb = nat.load().fmlh_krr_predict_ws_bytes(5, 0, 23)
+    ws = torch.empty(wsb, dtype=torch.uint8, device="cuda")
+    nat.call("fmlh_krr_predict", nat.ptr(z), nat.ptr(r), nat.ptr(off), 5, 23, None, 276, 0, None, 700.0,
+             nat.ptr(y), nat.ptr(status), nat.ptr(ws), wsb, nat.stream_ptr())
+    torch.cuda.synchronize()
+    assert torch.equal(y, torch.zeros_like(y))
+    assert int(status.sum().item()) == 0

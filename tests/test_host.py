@@ -159,3 +159,25 @@ def test_aras_block_plan():
     for k in set(key):
         sizes = [n[m] for m, kk in zip(order, key) if kk == k]
         assert sizes == sorted(sizes)
+
+
+def test_fml_import_path():
+    """The reference's own import lines resolve through the ``fml`` name
+    (tests/test_arad.py:8-11, examples/example_arad.py:9-12)."""
+    import fml
+    from fml.kernels import get_atomic_kernels_arad
+    from fml.math.distance import get_l2_distance_arad
+    import fml_amd
+    assert fml.kernels is fml_amd.kernels
+    assert get_atomic_kernels_arad is fml_amd.kernels.get_atomic_kernels_arad
+    assert get_l2_distance_arad is fml_amd.math.distance.get_l2_distance_arad
+    from fml.math import cho_solve
+    from fml.kernels import gaussian_kernel, laplacian_kernel
+    assert cho_solve is fml_amd.math.cho_solve and gaussian_kernel and laplacian_kernel
+    import fml.fml
+    assert fml.fml.Molecule is fml.Molecule is fml_amd.Molecule
+    from fml.arad import ARAD
+    assert ARAD is fml_amd.ARAD
+    m = fml.Molecule()
+    m.read_xyz(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "xyz", "01.xyz"))
+    assert m.natoms > 0
