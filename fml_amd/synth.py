@@ -54,22 +54,23 @@ class MoleculeSet:
 
 
 def _place(n: int, rng: np.random.Generator, min_dist: float, max_tries: int) -> np.ndarray:
+    # the minimum-distance test in plain floats: (dx*dx + dy*dy) + dz*dz is
+    # exactly numpy's sum((a - b)**2) over three values
     edge = 1.2 * n ** (1.0 / 3.0)
     md2 = min_dist * min_dist
     while True:
-        pts = np.empty((n, 3))
-        ok = True
+        pts = []
         for k in range(n):
             for _ in range(max_tries):
-                p = rng.uniform(0.0, edge, 3)
-                if k == 0 or np.min(np.sum((pts[:k] - p) ** 2, axis=1)) >= md2:
-                    pts[k] = p
+                x, y, z = rng.uniform(0.0, edge, 3).tolist()
+                if all((x - a) * (x - a) + (y - b) * (y - b) + (z - c) * (z - c) >= md2
+                       for a, b, c in pts):
+                    pts.append((x, y, z))
                     break
             else:
-                ok = False
                 break
-        if ok:
-            return pts
+        if len(pts) == n:
+            return np.array(pts, dtype=np.float64).reshape(n, 3)
 
 
 def atom_counts(nmol: int, rng: np.random.Generator, nmin: int, nmax: int,

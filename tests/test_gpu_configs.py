@@ -1,5 +1,21 @@
 """GPU parity at the benchmark configurations' own sizes.
 
+  * plain FP64 KRR, 1000 molecules (5-23 atoms, seed 0), Gaussian sigma 700
+    on 800, lambda 1e-8, predict 200: Coulomb matrices bit-exact, kernels
+    1e-12, alpha within 10 kappa u of LAPACK's (kappa ~ 7e10 makes any two
+    backward-stable solvers differ there), backward error 1e-14,
+    predictions and MAE within 1e-6;
+  * 36,000 training molecules (3-29 atoms, nmax 29): the full Gaussian
+    (sigma 1000) and Laplacian (sigma 10,000) kernels in FP64, FP32 and
+    split-BF16x3, row blocks against the oracle (FP64 Gaussian 1e-12,
+    Laplacian 4e-16, FP32 and BF16x3 Gaussian 1e-5, FP32 Laplacian 5e-6),
+    unit diagonal, symmetry; the FP64 Cholesky of K + 1e-6 I (info 0,
+    sampled |L L^T - A|) and the solve's backward error;
+  * uneven ARAD, 4096 x 4096 molecules (3-29 atoms, truncated geometric),
+    sigmas {1, 4}: FP64 row blocks bit-exact against the oracle, the FP32
+    overlap at its stated bound;
+  * serving with the 36,000-molecule model: results alone and in a mix of
+    concurrent requests bitwise equal, and within 1e-9 of the FP64 oracle;
   * global ARAD, 500 x 500 molecules (5-23 atoms, seed 0), sigmas
     {0.1, 1, 10}: the full kernel on the device; rows 0-7 against the
     reference's own output (tests/golden/arad500.npz, written by the flang
@@ -63,3 +79,219 @@ def test_global_arad_500x500(gpu, arad500):
         assert rel_or[s] <= 1e-12, (sig[s], rel_or[s])
     # measured on MI355X: every entry equal to the reference's
     assert same_ref == 1.0 and same_or == 1.0
+
+
+@pytest.fixture(scope="module")
+def krr1000():
+    ms = synth.generate(1000, seed=0, nmin=5, nmax=23)
+    return ms
+
+
+def test_krr_1000_fp64(gpu, krr1000):
+    from fml_amd.kernels import gaussian_kernel
+    from fml_amd.math import cho_solve
+    from fml_amd.representations import generate_coulomb_matrices
+    from fml_amd.krr import KRR
+    ms = krr1000
+    X = generate_coulomb_matrices(ms.charges, ms.coords, ms.offsets, nmax=23)
+    Xo = oracle.coulomb_matrices(ms.charges, ms.coords, ms.offsets, 23)[0]
+    assert X.shape == (1000, 276) and np.array_equal(X, Xo)
+    y = ms.energies
+    A, B = X[:800].T, X[800:].T
+    K = gaussian_kernel(A, A, 700.0)
+    Ko = oracle.gaussian_kernel(A, A, 700.0)
+    assert _rel(K, Ko) <= 1e-12 and np.all(np.diag(K) == 1.0)
+    Kl = Ko + 1e-8 * np.eye(800)
+    Kd = K + 1e-8 * np.eye(800)
+    alpha = cho_solve(Kd, y[:800])
+    ao, info = oracle.cho_solve(Kl, y[:800])
+    assert info == 0
+    kappa = np.linalg.cond(Kl)
+    da = np.linalg.norm(alpha - ao) / np.linalg.norm(ao)
+    berr = np.linalg.norm(Kl @ alpha - y[:800]) / (np.linalg.norm(Kl, 2) * np.linalg.norm(alpha))
+    Kt = gaussian_kernel(A, B, 700.0)
+    Kto = oracle.gaussian_kernel(A, B, 700.0)
+    assert _rel(Kt, Kto) <= 1e-12
+    pred, po = Kt.T @ alpha, Kto.T @ ao
+    mae, maeo = np.mean(np.abs(pred - y[800:])), np.mean(np.abs(po - y[800:]))
+    # the device-resident composition (bench's krr step) gives the same model
+    m = KRR(700.0, 1e-8).fit(X[:800], y[:800])
+    pd = m.predict(X[800:]).cpu().numpy()
+    print(f"\n1000-molecule KRR: kappa {kappa:.3g}, alpha rel diff {da:.3g} (bound {10 * kappa * 1.1e-16:.3g}), "
+          f"backward error {berr:.3g}, MAE {mae:.6f} vs LAPACK {maeo:.6f}, "
+          f"prediction diff {np.max(np.abs(pred - po)) / np.max(np.abs(po)):.3g}")
+    assert da <= 10 * kappa * 1.1e-16
+    assert berr <= 1e-14
+    assert np.max(np.abs(pred - po)) <= 1e-6 * np.max(np.abs(po))
+    assert abs(mae - maeo) <= 1e-6 * maeo
+    assert np.max(np.abs(pd - po)) <= 1e-6 * np.max(np.abs(po))
+
+
+@pytest.fixture(scope="module")
+def set40k():
+    import torch
+    from fml_amd.device import to_device
+    from fml_amd.representations import generate_coulomb_matrices
+    ms = synth.generate(40000, seed=0, nmin=3, nmax=29)
+    X = generate_coulomb_matrices(to_device(ms.charges), to_device(ms.coords),
+                                  to_device(ms.offsets, np.int32), nmax=29, on_device=True)
+    torch.cuda.synchronize()
+    return ms, X
+
+
+def test_kernels_36k(gpu, set40k):
+    import torch
+    from fml_amd.krr import kernel_matrix
+    ms, X = set40k
+    n, rows = 36000, 96
+    Xtr = X[:n]
+    Xh = Xtr.cpu().numpy()
+    X32 = Xtr.to(torch.float32)
+    # oracle columns: K(x_j, x_i) for i < rows, all j (the oracle's (n, rows) array)
+    go = oracle.gaussian_kernel(Xh.T, Xh[:rows].T, 1000.0)
+    lo = oracle.laplacian_kernel(Xh.T, Xh[:rows].T, 1e4)
+    out = torch.empty((n, n), dtype=torch.float64, device="cuda")
+    figures = {}
+    kernel_matrix(Xtr, Xtr, 1000.0, "gaussian", precision="fp64", out=out)
+    blk = out[:rows].cpu().numpy()
+    figures["gaussian64"] = _rel(blk, go.T)
+    assert torch.all(torch.diagonal(out) == 1.0)
+    sub = out[:512, :512]
+    figures["gaussian64_asym"] = float((sub - sub.T).abs().max().item())
+    kernel_matrix(Xtr, Xtr, 1e4, "laplacian", out=out)
+    figures["laplacian64"] = _rel(out[:rows].cpu().numpy(), lo.T)
+    assert torch.all(torch.diagonal(out) == 1.0)
+    sub = out[:512, :512]
+    assert torch.equal(sub, sub.T)  # sequential L1 sums: exactly symmetric
+    del out
+    out32 = torch.empty((n, n), dtype=torch.float32, device="cuda")
+    for prec in ("fp32", "bf16x3"):
+        kernel_matrix(X32, X32, 1000.0, "gaussian", precision=prec, out=out32)
+        figures["gaussian_" + prec] = _rel(out32[:rows].double().cpu().numpy(), go.T)
+        assert torch.all(torch.diagonal(out32) == 1.0)
+    kernel_matrix(X32, X32, 1e4, "laplacian", out=out32)
+    figures["laplacian32"] = _rel(out32[:rows].double().cpu().numpy(), lo.T)
+    del out32
+    torch.cuda.empty_cache()
+    print("\n36k kernels, max relative error of rows 0-95 against the FP64 oracle:", figures)
+    assert figures["gaussian64"] <= 1e-12
+    assert figures["gaussian64_asym"] <= 1e-15
+    assert figures["laplacian64"] <= 4e-16
+    # measured on MI355X: 1.2e-6, 8.7e-7, 9.6e-7
+    assert figures["gaussian_fp32"] <= 1e-5
+    assert figures["gaussian_bf16x3"] <= 1e-5
+    assert figures["laplacian32"] <= 5e-6
+
+
+def test_cholesky_36k(gpu, set40k):
+    import torch
+    from fml_amd.krr import add_diagonal, kernel_matrix, sym_residual
+    from fml_amd.math.cho_solve import potrf_inplace, potrs_inplace
+    ms, X = set40k
+    n = 36000
+    K = kernel_matrix(X[:n], X[:n], 1000.0, "gaussian")
+    add_diagonal(K, 1e-6)
+    rng = np.random.default_rng(0)
+    rows = torch.from_numpy(np.sort(rng.choice(n, 48, replace=False))).cuda()
+    A_s = K[rows][:, rows].clone()
+    info = potrf_inplace(K)
+    assert info == 0
+    L_rows = torch.tril(K[rows]) if False else K[rows].clone()
+    # the factor is the lower triangle of the row-major view: zero the rest
+    cols = torch.arange(n, device="cuda")
+    L_rows[cols[None, :] > rows[:, None]] = 0.0
+    LLt = L_rows @ L_rows.T
+    fact_err = float((LLt - A_s).abs().max().item() / A_s.abs().max().item())
+    y = torch.from_numpy(ms.energies[:n].copy()).cuda()
+    alpha = y.clone()
+    potrs_inplace(K, alpha)
+    del K, L_rows
+    torch.cuda.empty_cache()
+    K2 = kernel_matrix(X[:n], X[:n], 1000.0, "gaussian")
+    r = torch.empty_like(y)
+    sym_residual(K2, 1e-6, alpha, y, r)
+    knorm = float(torch.linalg.matrix_norm(K2, ord="fro").item())
+    berr = float(torch.linalg.norm(r).item()) / (knorm * float(torch.linalg.norm(alpha).item()))
+    print(f"\n36k Cholesky: sampled |LL^T - A| / |A| {fact_err:.3g}, solve backward error {berr:.3g}")
+    assert fact_err <= 1e-13
+    assert berr <= 1e-15
+
+
+@pytest.fixture(scope="module")
+def arad4096():
+    from fml_amd import synth as sy
+    ms1 = sy.generate(4096, seed=0, nmin=3, nmax=29, dist="geometric")
+    ms2 = sy.generate(4096, seed=1, nmin=3, nmax=29, dist="geometric")
+    ar = ARAD(maxMolSize=29, maxAts=29, cut=5.0)
+    out = []
+    for ms in (ms1, ms2):
+        X = np.array([ar.describe(ms.molecule(m)[1], ms.molecule(m)[0]) for m in range(ms.nmol)])
+        Z = [ms.molecule(m)[0] for m in range(ms.nmol)]
+        z, n = arad_types(Z, 29)
+        out.append((X, Z, z, n))
+    return out
+
+
+def test_arad_4096x4096(gpu, arad4096):
+    from fml_amd.kernels import get_atomic_kernels_arad
+    (X1, Z1, z1, n1), (X2, Z2, z2, n2) = arad4096
+    assert 13.0 < n1.mean() < 15.5 and n1.min() >= 3 and n1.max() <= 29
+    sig = [1.0, 4.0]
+    rows = np.array([0, 1, 2, 3, 1024, 2047, 3000, 4095])
+    ko = oracle.kernels_arad(X1[rows], X2, z1[rows], z2, n1[rows], n2, sig, 0.2, 5.0, 1.0, 0.5)
+    k = get_atomic_kernels_arad(X1, X2, Z1, Z2, sig, cut_distance=5.0)
+    assert k.shape == (2, 4096, 4096) and np.all(np.isfinite(k)) and np.all(k >= 0)
+    same = float(np.mean(k[:, rows] == ko))
+    rel = _rel(k[:, rows], ko)
+    del k
+    k32 = get_atomic_kernels_arad(X1, X2, Z1, Z2, sig, cut_distance=5.0, precision=1)
+    rel32 = [_rel(k32[s][rows], ko[s]) for s in range(2)]
+    print(f"\n4096x4096 ARAD: FP64 rows vs oracle max rel {rel:.3g}, bitwise {same:.6f}; FP32 overlap {rel32}")
+    assert rel <= 1e-12 and same == 1.0
+    assert rel32[0] <= 1e-3 and rel32[1] <= 1e-4
+
+
+def test_serving_36k_model(gpu, set40k):
+    import concurrent.futures as cf
+    import torch
+    from fml_amd.krr import KRR
+    from fml_amd.serving import KRRServer
+    ms, X = set40k
+    n = 36000
+    m = KRR(1000.0, 1e-6).fit(X[:n], ms.energies[:n])
+    srv = KRRServer(X[:n], m.alpha, 1000.0, nmax=29).start()
+    pool = synth.generate(600, seed=2, nmin=3, nmax=29)
+    rng = np.random.default_rng(2)
+    sizes = np.exp(rng.uniform(0.0, np.log(256.0), 24)).astype(int).clip(1, 256)
+    starts = np.concatenate([[0], np.cumsum(sizes)]) % (pool.nmol - 256)
+    reqs = [([pool.molecule(int(i))[0] for i in range(a, a + b)], [pool.molecule(int(i))[1] for i in range(a, a + b)])
+            for a, b in zip(starts[:-1], sizes)]
+    # in a mix: a 4,000-molecule fit (above the factor's look-ahead size, on
+    # the CU-masked fit stream) and all requests submitted at once from
+    # several threads
+    Xf = X[n:].cpu().numpy()
+    fit = srv.submit_fit(Xf, ms.energies[n:], 1000.0, 1e-6)
+    with cf.ThreadPoolExecutor(8) as ex:
+        futs = list(ex.map(lambda q: srv.submit(*q), reqs))
+    mixed = [f.result(timeout=120)[0] for f in futs]
+    fitted = fit.result(timeout=300)
+    ref_fit = KRR(1000.0, 1e-6).fit(Xf, ms.energies[n:])
+    assert torch.equal(fitted.alpha, ref_fit.alpha)
+    alone = [srv.submit(*q).result(timeout=120)[0] for q in reqs]
+    srv.close()
+    for a, b in zip(alone, mixed):
+        assert np.array_equal(a, b)
+    # against the FP64 oracle kernel and the device alpha
+    Xh = X[:n].cpu().numpy()
+    al = m.alpha.cpu().numpy()
+    worst = 0.0
+    for q, got in zip(reqs[:6], alone[:6]):
+        zq = np.concatenate(q[0])
+        rq = np.concatenate(q[1])
+        off = np.concatenate([[0], np.cumsum([len(c) for c in q[0]])]).astype(np.int32)
+        Xq = oracle.coulomb_matrices(zq, rq, off, 29)[0]
+        po = oracle.gaussian_kernel(Xh.T, Xq.T, 1000.0).T @ al
+        worst = max(worst, float(np.max(np.abs(got - po)) / np.max(np.abs(po))))
+    print(f"\nserving, 36k model: {len(reqs)} requests bitwise equal alone and mixed (with a 4000-molecule "
+          f"fit running); max rel vs oracle {worst:.3g}")
+    assert worst <= 1e-9
