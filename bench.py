@@ -49,11 +49,11 @@ PEAK_FP32_VALU_TOPS = 78.6         # FP32 VALU lane-operations/s: the FP32 vecto
 # at the 12 FP64 instructions its compiled form issues.
 ARAD_OPS_PER_EVAL = 25
 
-# HBM bytes per launch of k_arad_tiles (FP64, the tuned tile shape), from
-# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this script, each scaled by
-# the counter's reading for a device copy of known size (FETCH_SIZE x 2.0,
-# WRITE_SIZE x 1.0): profiles/hbm_traffic_counters.md.
-ARAD_TRAFFIC_BYTES = {"arad500": 3.762e9}
+# HBM bytes per launch of k_arad_tiles (FP64, tile shape 1024 pairs, 5 waves,
+# set-1 atom fastest), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of
+# this script, each scaled by the counter's reading for a device copy of known
+# size (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0): profiles/hbm_traffic_counters.md.
+ARAD_TRAFFIC_BYTES = {"arad500": 2.0 * 550153.2 * 1024 + 17096.3 * 1024}
 
 
 def parse():
@@ -69,8 +69,11 @@ def parse():
     ap.add_argument("--duration", type=float, default=5.0, help="serving run length in seconds")
     ap.add_argument("--n", type=int, default=0, help="problem size override for the 36k workloads")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--tiling", default=None,
+                    help="ARAD tile shape pairs,waves,groups_per_cu,pair_order instead of autotuning")
     ap.add_argument("--eager", action="store_true", help="plain calls instead of HIP-graph replays")
-    ap.add_argument("--cpu-rows", type=int, default=100, help="row block of the CPU baseline sample")
+    ap.add_argument("--cpu-rows", type=int, default=0,
+                    help="row block of the ARAD CPU baseline sample (0: the whole 500-molecule set, 100 rows at 4096)")
     return ap.parse_args()
 
 
@@ -80,6 +83,37 @@ def host_threads():
         return int(v) if v else (os.cpu_count() or 1)
     except ValueError:
         return os.cpu_count() or 1
+
+
+def host_cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or "unknown"
+
+
+def median_time(fn, runs=5, warmup=1):
+    """Median wall seconds of `runs` calls after `warmup` untimed ones; the
+    last call's result."""
+    out = None
+    for _ in range(warmup):
+        out = fn()
+    ts = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        out = fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts)), ts, out
+
+
+def cpu_record(value, unit, kind, sample, ts):
+    return {"value": value, "unit": unit, "cores": host_threads(), "kind": kind, "sample": sample,
+            "host_cpu": host_cpu_model(), "runs_s": [round(t, 4) for t in ts]}
 
 
 def arad_inner_evaluations(X1, n1, X2, n2, cut, width):
@@ -141,7 +175,13 @@ def arad_workload(args, big=False, precision=0):
     # tile shape: the autotuner times its candidates on the device (on the
     # first 512 set-1 molecules for the 4096 x 4096 set) before the timed steps
     t0 = time.perf_counter()
-    tiling, tuned_ms = call.autotune(sample_rows=512 if big else None, reps=3)
+    if args.tiling:
+        tiling = nat.AradTiling(*[int(v) for v in args.tiling.split(",")])
+        call.tiling = tiling
+        call.run()
+        tuned_ms = float("nan")
+    else:
+        tiling, tuned_ms = call.autotune(sample_rows=512 if big else None, reps=3)
     tune_s = time.perf_counter() - t0
     torch.cuda.synchronize()
 
@@ -166,7 +206,8 @@ def arad_workload(args, big=False, precision=0):
                               "waves_per_simd": tiling.waves_per_simd,
                               "groups_per_cu": tiling.groups_per_cu,
                               "pair_order": tiling.pair_order,
-                              "autotuned_on": f"{512 if big else nm} x {nm} molecules",
+                              "autotuned_on": (f"{512 if big else nm} x {nm} molecules" if not args.tiling
+                                               else "fixed by --tiling"),
                               "autotune_s": round(tune_s, 2),
                               "tuned_ms_per_call": round(tuned_ms, 3)}},
         "kernel": "k_arad_tiles",
@@ -176,10 +217,15 @@ def arad_workload(args, big=False, precision=0):
         ops = evals * ARAD_OPS_PER_EVAL
         ach = ops / t_main_s / 1e12
         peak = PEAK_FP32_VALU_TOPS if precision else PEAK_FP64_VALU_TOPS
+        # the call's compulsory bytes: both descriptor sets and type channels
+        # read, the kernel written
+        alg = (X1.nbytes + X2.nbytes + z1.nbytes + z2.nbytes + n1.nbytes + n2.nbytes
+               + 8 * len(sig) * X1.shape[0] * X2.shape[0])
         return {"bound": "valu", "achieved": round(ach, 3), "peak": peak,
                 "unit": "T FP32-VALU-ops/s" if precision else "T FP64-VALU-ops/s",
                 "frac": round(ach / peak, 4),
                 "traffic": ARAD_TRAFFIC_BYTES.get(workload_key),
+                "algorithmic_bytes": int(alg),
                 "inner_evaluations_per_s": round(evals / t_main_s, 1),
                 "ops_per_evaluation": ARAD_OPS_PER_EVAL}
 
@@ -187,17 +233,19 @@ def arad_workload(args, big=False, precision=0):
         import oracle
         kind = "reference" if oracle.ref_available() else "port"
         fn = oracle.ref_kernels_arad if kind == "reference" else oracle.kernels_arad
-        t0 = time.perf_counter()
-        kc = fn(X1[:rows], X2, z1[:rows], z2, n1[:rows], n2, sig, width, cut, rw, cw)
-        dt = time.perf_counter() - t0
+        rows = min(rows or (100 if big else nm), nm)
+        runs, warm = (5, 1) if not big else (1, 0)
+        dt, ts, kc = median_time(lambda: fn(X1[:rows], X2, z1[:rows], z2, n1[:rows], n2, sig, width, cut,
+                                            rw, cw), runs=runs, warmup=warm)
         g = torch.stack([out[:, :rows, k] for k in range(len(sig))]).cpu().numpy()  # (s, nm, rows)
         g = np.transpose(g, (0, 2, 1))
         m = kc > 1e-300
         err = float(np.max(np.abs(g[m] - kc[m]) / kc[m]))
-        return ({"value": round(len(sig) * rows * nm / dt, 1), "unit": "entries/s",
-                 "cores": host_threads(), "kind": kind,
-                 "sample": f"rows 0..{rows - 1} x all {nm} molecules ({len(sig) * rows * nm} entries), "
-                           f"{dt:.2f} s"}, err)
+        what = "the full kernel" if rows == nm else f"rows 0..{rows - 1} x all {nm} molecules"
+        return (cpu_record(round(len(sig) * rows * nm / dt, 1), "entries/s", kind,
+                           f"{what} ({len(sig) * rows * nm} entries), median of {runs} runs"
+                           f"{' after a warm-up' if warm else ''}: {dt:.2f} s", ts),
+                {"max_rel": err, "bitwise_equal_fraction": float(np.mean(g == kc))})
 
     def finish(res):
         # the same call through the Python API with host (numpy) inputs and
@@ -290,6 +338,37 @@ def krr_workload(args):
             "config": {"workload": "plain FP64 KRR: 1000 molecules (seed 0, 5-23 atoms), sorted CM "
                                    "nmax 23, Gaussian sigma 700, 800 train / 200 test, lambda 1e-8"}}
 
+    # FP64 flop per step: two Gaussian blocks (2 D per entry), the factor
+    # (n^3/3), two triangular solves and the prediction
+    flop = 2.0 * 276 * (800 * 800 + 800 * 200) + 800 ** 3 / 3.0 + 2 * 800 ** 2 + 2 * 800 * 200
+
+    def roofline(t):
+        ach = flop / t / 1e12
+        return {"bound": "mfma", "achieved": round(ach, 4), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(ach / PEAK_FP64_TFLOPS, 5), "traffic": None, "flop_per_step": flop,
+                "note": "latency-bound: ~30 dependent launches on an 800-row system; the whole step is timed"}
+
+    def cpu_baseline(rows):
+        import oracle
+        kind = "reference" if oracle.ref_available() else "port"
+        cm = oracle.ref_coulomb_matrix if kind == "reference" else oracle.coulomb_matrix
+        gk = oracle.ref_gaussian_kernel if kind == "reference" else oracle.gaussian_kernel
+        cs = oracle.ref_cho_solve if kind == "reference" else (lambda A, y: oracle.cho_solve(A, y)[0])
+
+        def run():
+            X = np.array([cm(*ms.molecule(i), 23) for i in range(ms.nmol)])
+            K = gk(X[:800].T, X[:800].T, 700.0)
+            K[np.diag_indices_from(K)] += 1e-8
+            a = cs(K, ms.energies[:800])
+            return gk(X[:800].T, X[800:].T, 700.0).T @ a
+        dt, ts, pred_cpu = median_time(run)
+        pred = state["pred"].cpu().numpy()
+        err = float(np.max(np.abs(pred - pred_cpu)) / np.max(np.abs(pred_cpu)))
+        return cpu_record(round(dt * 1e3, 3), "ms", kind,
+                          "the whole fit+predict: Coulomb matrices (one call per molecule), Gaussian "
+                          "800x800 and 800x200, fcho_solve, predictions; median of 5 runs after a "
+                          f"warm-up: {dt * 1e3:.1f} ms", ts), err
+
     def finish(res):
         import oracle
         pred = state["pred"].cpu().numpy()
@@ -304,7 +383,7 @@ def krr_workload(args):
         res["potrf_info"] = int(state["info"].item())
         res["cond_estimate"] = float(np.linalg.cond(K))
 
-    return info, step, None, None, None, finish
+    return info, step, None, roofline, cpu_baseline, finish
 
 
 def dense36k_workload(args, kind):
@@ -430,7 +509,7 @@ def atomic_workload(args, kind=0):
         import oracle
         if int(status.item()) != 0:
             raise RuntimeError("atomic kernel reported an oversize block")
-        rows = max(1, min(rows, nm))
+        rows = max(1, min(rows or 100, nm))
         bkind = "reference" if oracle.ref_available() else "port"
         fn = oracle.ref_vector_kernels if bkind == "reference" else oracle.vector_kernels
         r1, r2 = x1.cpu().numpy(), x2.cpu().numpy()
@@ -547,7 +626,7 @@ def aras_workload(args):
         import oracle
         if int(status.item()) != 0:
             raise RuntimeError("ARAS kernel reported bad descriptors")
-        rows = max(1, min(rows // 4, nm))
+        rows = max(1, min((rows or 100) // 4, nm))
         bkind = "reference" if oracle.ref_available() else "port"
         fn = oracle.ref_kernels_aras if bkind == "reference" else oracle.kernels_aras
         t0 = time.perf_counter()
@@ -735,11 +814,39 @@ def krr36k_workload(args, precision):
                                   "solve": round(rep.solve_s, 4), "total": round(rep.seconds, 4)}
         res["iterations"] = rep.iterations
         res["shift"] = rep.shift
-        res["alpha"] = m.alpha.cpu().numpy()[:8].tolist()
-        np.save(f"gpurun_out/alpha_{precision}.npy", m.alpha.cpu().numpy())
-        np.save(f"gpurun_out/pred_{precision}.npy", pred)
+        res["alpha_head"] = m.alpha.cpu().numpy()[:8].tolist()
+        # the dominant kernel work: the Cholesky factorisation, n^3/3 flop
+        fl = ntr ** 3 / 3.0
+        peak = PEAK_FP64_TFLOPS if precision == "fp64" else PEAK_FP32_TFLOPS
+        ach = fl / rep.factor_s / 1e12
+        res["roofline"] = {"bound": "mfma", "achieved": round(ach, 2), "peak": peak, "unit": "TFLOP/s",
+                           "frac": round(ach / peak, 4), "traffic": None,
+                           "note": f"the {'FP64' if precision == 'fp64' else 'FP32'} factorisation of "
+                                   f"K + lambda I (n^3/3 flop), timed inside the fit"}
 
-    return info, step, None, None, None, finish
+    def cpu_baseline(rows):
+        # the reference on the host: Gaussian kernel and fcho_solve on a 12,000-molecule
+        # block, extrapolated to 36k (kernel by entries, factorisation by n^3)
+        import oracle
+        kind = "reference" if oracle.ref_available() else "port"
+        gk = oracle.ref_gaussian_kernel if kind == "reference" else oracle.gaussian_kernel
+        cs = oracle.ref_cho_solve if kind == "reference" else (lambda A, b: oracle.cho_solve(A, b)[0])
+        nb = 12000
+        Xh = X[:nb].cpu().numpy()
+        t0 = time.perf_counter()
+        K = gk(Xh.T, Xh.T, 1000.0)
+        tk = time.perf_counter() - t0
+        K[np.diag_indices_from(K)] += 1e-6
+        t0 = time.perf_counter()
+        cs(K, y[:nb])
+        tc = time.perf_counter() - t0
+        est = tk * (ntr * ntr + ntr * (n - ntr)) / (nb * nb) + tc * (ntr / nb) ** 3
+        return cpu_record(round(est * 1e3, 1), "ms", kind,
+                          f"Gaussian kernel {nb}x{nb} ({tk:.2f} s) and fcho_solve n={nb} ({tc:.2f} s), "
+                          f"one run each, extrapolated to fit {ntr} + predict {n - ntr} (kernel by entries, "
+                          "factorisation by n^3)", [tk, tc]), None
+
+    return info, step, None, None, cpu_baseline, finish
 
 
 def serving_workload(args):
@@ -798,6 +905,37 @@ def serving_workload(args):
     lat_fit = latency_percentiles([r[1] for r, f in zip(res, during) if f]) if any(during) else None
     lat_nofit = latency_percentiles([r[1] for r, f in zip(res, during) if not f])
     wall = t_end - t_start
+    # the dominant device work per batch is reading the resident training
+    # rows (ntrain x 435 FP64) for the Gaussian block: HBM-bound
+    d = int(X.shape[1])
+    bsec = float(np.sum([b[2] for b in srv.batches]))
+    bbytes = float(len(srv.batches)) * ntr * d * 8 + float(np.sum([b[1] for b in srv.batches])) * ntr * 8 * 2
+    ach = bbytes / bsec / 1e9
+    roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": None,
+            "note": "per batch: the training rows (ntrain x D x 8 B) plus the kernel block written and read "
+                    "(2 x b x ntrain x 8 B), over the batches' host-measured wall time (staging, launch and "
+                    "one event wait included)"}
+    # CPU: the reference's routines predicting a sample of the same requests
+    import oracle
+    kind = "reference" if oracle.ref_available() else "port"
+    cm = oracle.ref_coulomb_matrix if kind == "reference" else oracle.coulomb_matrix
+    gk = oracle.ref_gaussian_kernel if kind == "reference" else oracle.gaussian_kernel
+    Xh = X[:ntr].cpu().numpy()
+    al = m.alpha.cpu().numpy()
+    crng = np.random.default_rng(4)
+    cts, cmols = [], 0
+    for _ in range(12):
+        k = min(max(int(np.floor(np.exp(crng.uniform(0, np.log(257))))), 1), 256)
+        idx = crng.integers(0, len(pool_mols), k)
+        t0 = time.perf_counter()
+        Xq = np.array([cm(*pool_mols[i], 29) for i in idx])
+        gk(Xh.T, Xq.T, 1000.0).T @ al
+        cts.append(time.perf_counter() - t0)
+        cmols += k
+    cpu = cpu_record(round(float(np.median(cts)) * 1e3, 3), "ms (median per request)", kind,
+                     f"12 requests of the same size law ({cmols} molecules), one at a time: Coulomb "
+                     "matrices and the Gaussian block against the 36k training rows, then K^T alpha", cts)
     return {"metric": "serving p50/p99 predict latency", "value": round(lat["p99_ms"], 3), "unit": "ms (p99)",
             "p50_ms": round(lat["p50_ms"], 3), "p99_ms": round(lat["p99_ms"], 3), "mean_ms": round(lat["mean_ms"], 3),
             "requests": len(res), "molecules": nmol_total, "wall_s": round(wall, 3),
@@ -805,6 +943,7 @@ def serving_workload(args):
             "batches": len(srv.batches), "mean_batch_requests": round(float(np.mean([b[0] for b in srv.batches])), 2),
             "fits": [{"n": nf, "seconds": round(b - a, 3)} for nf, a, b in srv.fit_log],
             "overlapping_fits": lat_fit, "outside_fits": lat_nofit,
+            "roofline": roof, "cpu_baseline": cpu,
             "higher_is_better": False, "dtype": "f64", "data": "synthetic",
             "config": {"workload": f"serving: {ntr}-molecule Gaussian KRR model resident, Poisson 2000 req/s, "
                                    "1-256 molecules per request, fit requests 1k-16k admitted by free HBM",
@@ -818,7 +957,7 @@ def main():
         torch.cuda.set_device(0)
         out = serving_workload(args)
         out.update({"n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": out["p50_ms"], "scaling": "weak",
-                    "vs_baseline": None, "roofline": None})
+                    "vs_baseline": None})
         print(json.dumps(out), flush=True)
         return
     if args.gpus != 1:
@@ -927,7 +1066,11 @@ def main():
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
         cb, err = cpu_baseline(args.cpu_rows)
         res["cpu_baseline"] = cb
-        res["parity_vs_cpu_baseline_max_rel"] = err
+        if isinstance(err, dict):
+            res["parity_vs_cpu_baseline_max_rel"] = err["max_rel"]
+            res["parity_vs_cpu_baseline"] = err
+        elif err is not None:
+            res["parity_vs_cpu_baseline_max_rel"] = err
     print(json.dumps(res), flush=True)
 
 

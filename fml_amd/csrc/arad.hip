@@ -23,10 +23,11 @@
 //      generic path, which evaluates T from the raw channels per term.
 //   3. k_arad_self: self overlap S(a, a) per atom with the same device
 //      function, so D(x, x) = 0 exactly.
-//   4. k_arad_pairs: persistent workgroups walk molecule pairs; both
-//      molecules' records are staged in LDS, each lane owns atom pairs, the
-//      per-sigma exp sums are reduced with wave shuffles in a fixed order.
-//      A molecule pair's result depends only on that pair.
+//   4. k_arad_tiles (amax <= 29): persistent workgroups take size-sorted
+//      molecule-pair tiles; k_arad_pairs (larger amax, distance form) walks
+//      molecule pairs.  Each lane owns atom pairs; the per-sigma sums run
+//      sequentially in the reference's order, so a molecule pair's result
+//      depends only on that pair.
 //   5. k_arad_atomic: 16 x 16 atom tiles for the per-atom forms.
 //
 // Band walk: when both neighbour lists are ascending in distance (always
@@ -71,7 +72,7 @@ constexpr int kMaxTypes = 16;            // compact type ids
 constexpr int kTypeRows = 8;             // PTP rows 0..7 representable
 constexpr int kTypeCols = 64;            // PTP columns 0..63 representable
 constexpr int kTypeBitmap = kTypeRows * kTypeCols;
-constexpr uint16_t kTypeNone = 0xFFFF;
+constexpr int32_t kTypeNone = 0xFFFF;
 constexpr int kMaxSig = 8;
 constexpr int kPairThreads = 256;
 constexpr int kTileRounds = 4;  // atom-pair rounds per thread in a tile
@@ -87,15 +88,15 @@ struct AradParams {
   double eps;      // real(5e-12) or real(1e-11), widened
 };
 
-struct Rec {  // one neighbour: distance, sine term, two angular channels
+struct Rec {  // one neighbour: distance, sine term, two angular channels, compact type id
   double d, s, a, b;
+  int32_t ty, pad;
 };
 
 // Per-set prepared neighbour data (workspace).
 struct NbSet {
   Rec *rec;         // [slot*amax + m]
   double2 *rc;      // [slot*amax + m] raw (row, col) type channels
-  uint16_t *ty;     // [slot*amax + m] compact type id
   int32_t *cnt;     // [slot]
   int32_t *asc;     // [slot] 1 when d[0..cnt) is ascending
   double *self;     // [slot] self overlap (FP64 form)
@@ -159,7 +160,7 @@ __global__ __launch_bounds__(256) void k_arad_prep(const double *__restrict__ q,
       // global kernel: sine only where d < cut (farad_kernels.f90:526,539)
       const double sv = (!sin_guard || d < p.cut) ? 1.0 - sin_libm(d * p.inv_cut, c_sincos) : 0.0;
       const int64_t o = (int64_t)slot * amax + m;
-      set.rec[o] = Rec{d, sv, x[3 * amax + m], x[4 * amax + m]};
+      set.rec[o] = Rec{d, sv, x[3 * amax + m], x[4 * amax + m], kTypeNone, 0};
       set.rc[o] = make_double2(rowv, colv);
       if (small_int(rowv, kTypeRows) && small_int(colv, kTypeCols)) {
         // a handful of types cover every neighbour: set each bit once
@@ -233,58 +234,65 @@ __global__ __launch_bounds__(256) void k_arad_type_ids(NbSet set, int nslots, in
   const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (o >= (int64_t)nslots * amax) return;
   const int slot = (int)(o / amax), m = (int)(o % amax);
-  uint16_t id = kTypeNone;
+  int32_t id = kTypeNone;
   if (m < set.cnt[slot]) {
     const double2 v = set.rc[o];
     if (small_int(v.x, kTypeRows) && small_int(v.y, kTypeCols)) {
       const int c = ctl->id[(int)v.x * kTypeCols + (int)v.y];
-      if (c >= 0) id = (uint16_t)c;
+      if (c >= 0) id = c;
     }
   }
-  set.ty[o] = id;
+  if (m < set.cnt[slot]) set.rec[o].ty = id;
 }
 
 // ---- the overlap ------------------------------------------------------
 
 struct NbView {
   const Rec *rec;
-  const uint16_t *ty;
   const double2 *rc;  // raw channels, generic path only
 };
 
 template <typename TC>
-__device__ __forceinline__ TC texp(TC x, const ExpPair *tab);
+__device__ __forceinline__ TC texp(TC x, const ExpPair *tab, double c2, double c4);
 template <>
-__device__ __forceinline__ double texp<double>(double x, const ExpPair *tab) {
-  return exp_neg(x, tab);
+__device__ __forceinline__ double texp<double>(double x, const ExpPair *tab, double c2, double c4) {
+  return exp_neg(x, tab, c2, c4);
 }
 template <>
-__device__ __forceinline__ float texp<float>(float x, const ExpPair *) { return expf(x); }
+__device__ __forceinline__ float texp<float>(float x, const ExpPair *, double, double) {
+  return expf(x);
+}
 
 // atomic_distl2 for atoms x (list 1) and y (list 2).  kGeneric evaluates T
 // from the raw channels; otherwise ttab is the compact type-pair table.
-// etab: exp_neg's table.  kPadY: list 2 may be read up to two records past
-// its end (a padded LDS stage), so the band loop's read-ahead needs no clamp.
-template <typename TC, bool kGeneric, bool kPadY = false>
+// etab: exp_neg's table.
+template <typename TC, bool kGeneric>
 __device__ TC arad_distl2(const NbView &x, int cnt1, bool asc1, const NbView &y, int cnt2,
                           bool asc2, TC inv_w, TC maxg, const TC *__restrict__ ttab,
                           const ExpPair *__restrict__ etab, double rw2, double cw2) {
   TC acc = TC(0);
+  // exp_neg's constants C2 and C4 held in vector registers for the whole
+  // call (the empty asm makes them opaque values), so its two FMAs with a
+  // second constant need no register copy per term
+  double c2 = FMLH_EXP_C2, c4 = FMLH_EXP_C4;
+#if __HIP_DEVICE_COMPILE__
+  asm volatile("" : "+v"(c2), "+v"(c4));
+#endif
   // the quantity farad_kernels.f90:78-83 adds for record yr (index m2) of
   // list 2, in the reference's order: d = (exp(r2 inv_w) s1) s2; d = d T;
   // d ((1 + a1 a2) + b1 b2)
-  auto rec_value = [&](const Rec &yr, uint16_t ty2, int m2, TC d1, TC s1, TC a1, TC b1, int trow,
+  auto rec_value = [&](const Rec &yr, int m2, TC d1, TC s1, TC a1, TC b1, int trow,
                        double2 rc1) -> TC {
     const TC dr = (TC)yr.d - d1;
     const TC r2 = dr * dr;
-    TC v = texp<TC>(r2 * inv_w, etab) * s1;
+    TC v = texp<TC>(r2 * inv_w, etab, c2, c4) * s1;
     v = v * (TC)yr.s;
     TC t;
     if (kGeneric) {
       const double2 rc2 = y.rc[m2];
       t = (TC)stoch(rc1.x - rc2.x, rc1.y - rc2.y, rw2, cw2);
     } else {
-      t = ttab[trow + ty2];
+      t = ttab[trow + yr.ty];
     }
     v = v * t;
     TC ang = TC(1) + a1 * (TC)yr.a;
@@ -292,17 +300,12 @@ __device__ TC arad_distl2(const NbView &x, int cnt1, bool asc1, const NbView &y,
     return v * ang;
   };
   auto term = [&](int m2, TC d1, TC s1, TC a1, TC b1, int trow, double2 rc1) {
-    acc = acc + rec_value(y.rec[m2], kGeneric ? uint16_t(0) : y.ty[m2], m2, d1, s1, a1, b1, trow, rc1);
+    acc = acc + rec_value(y.rec[m2], m2, d1, s1, a1, b1, trow, rc1);
   };
   if (asc1 && asc2) {
     int lo = 0, hi = 0;
-    // the next m1's record and type are read one m1 ahead (index clamped)
-    Rec xr = x.rec[0];
-    uint16_t xt = x.ty[0];
     for (int m1 = 0; m1 < cnt1; ++m1) {
-      const int mn = min(m1 + 1, cnt1 - 1);
-      const Rec xnext = x.rec[mn];
-      const uint16_t xtnext = x.ty[mn];
+      const Rec xr = x.rec[m1];
       const TC d1 = (TC)xr.d;
       while (lo < cnt2) {
         const TC d2 = (TC)y.rec[lo].d;
@@ -317,41 +320,29 @@ __device__ TC arad_distl2(const NbView &x, int cnt1, bool asc1, const NbView &y,
         ++hi;
       }
       if (lo < hi) {
-      const int trow = kGeneric ? 0 : (int)xt * kMaxTypes;
-      const double2 rc1 = kGeneric ? x.rc[m1] : make_double2(0.0, 0.0);
-      // two independent terms per trip for instruction-level parallelism,
-      // added to acc one after the other in m2 order; the next trip's two
-      // records are read before this trip's arithmetic (indices clamped to
-      // the list, the clamped reads are not used)
-      const TC s1 = (TC)xr.s, a1 = (TC)xr.a, b1 = (TC)xr.b;
-      const int last = kPadY ? cnt2 + 1 : cnt2 - 1;
-      Rec r0 = y.rec[lo], r1 = y.rec[min(lo + 1, last)];
-      uint16_t t0 = y.ty[lo], t1 = y.ty[min(lo + 1, last)];
-      int m2 = lo;
-      for (; m2 + 1 < hi; m2 += 2) {
-        const int p0 = kPadY ? m2 + 2 : min(m2 + 2, last);
-        const int p1 = kPadY ? m2 + 3 : min(m2 + 3, last);
-        const Rec n0 = y.rec[p0], n1 = y.rec[p1];
-        const uint16_t u0 = y.ty[p0], u1 = y.ty[p1];
-        const TC v0 = rec_value(r0, t0, m2, d1, s1, a1, b1, trow, rc1);
-        const TC v1 = rec_value(r1, t1, m2 + 1, d1, s1, a1, b1, trow, rc1);
-        acc = acc + v0;
-        acc = acc + v1;
-        r0 = n0;
-        r1 = n1;
-        t0 = u0;
-        t1 = u1;
+        const int trow = kGeneric ? 0 : xr.ty * kMaxTypes;
+        const double2 rc1 = kGeneric ? x.rc[m1] : make_double2(0.0, 0.0);
+        const TC s1 = (TC)xr.s, a1 = (TC)xr.a, b1 = (TC)xr.b;
+        // two terms per trip, added to acc one after the other in m2 order;
+        // their records are read at the top of the trip (the other waves of
+        // the SIMD cover the LDS latency)
+        const Rec *yp = y.rec + lo;
+        int m2 = lo;
+        for (int k = (hi - lo) >> 1; k > 0; --k, m2 += 2, yp += 2) {
+          const Rec r0 = yp[0], r1 = yp[1];
+          const TC v0 = rec_value(r0, m2, d1, s1, a1, b1, trow, rc1);
+          const TC v1 = rec_value(r1, m2 + 1, d1, s1, a1, b1, trow, rc1);
+          acc = acc + v0;
+          acc = acc + v1;
+        }
+        if ((hi - lo) & 1) acc = acc + rec_value(yp[0], m2, d1, s1, a1, b1, trow, rc1);
       }
-      if (m2 < hi) acc = acc + rec_value(r0, t0, m2, d1, s1, a1, b1, trow, rc1);
-      }
-      xr = xnext;
-      xt = xtnext;
     }
   } else {
     for (int m1 = 0; m1 < cnt1; ++m1) {
       const Rec xr = x.rec[m1];
       const TC d1 = (TC)xr.d;
-      const int trow = kGeneric ? 0 : (int)x.ty[m1] * kMaxTypes;
+      const int trow = kGeneric ? 0 : xr.ty * kMaxTypes;
       const double2 rc1 = kGeneric ? x.rc[m1] : make_double2(0.0, 0.0);
       const TC s1 = (TC)xr.s, a1 = (TC)xr.a, b1 = (TC)xr.b;
       for (int m2 = 0; m2 < cnt2; ++m2) {
@@ -365,7 +356,7 @@ __device__ TC arad_distl2(const NbView &x, int cnt1, bool asc1, const NbView &y,
 
 __device__ __forceinline__ NbView global_view(const NbSet &s, int64_t slot, int amax) {
   const int64_t o = slot * amax;
-  return NbView{s.rec + o, s.ty + o, s.rc + o};
+  return NbView{s.rec + o, s.rc + o};
 }
 
 // LDS copies of the type-pair table and of exp_neg's table.
@@ -410,7 +401,6 @@ __global__ __launch_bounds__(256) void k_arad_self(NbSet set, int nslots, int am
 // LDS image of one molecule.
 struct MolStage {
   Rec *rec;
-  uint16_t *ty;
   int *cnt, *asc;
   double *self, *zr, *zc;
 };
@@ -419,7 +409,6 @@ __device__ __forceinline__ MolStage carve_stage(char *&p, int amax, bool records
   MolStage m;
   const int nn = amax * amax;
   m.rec = nullptr;
-  m.ty = nullptr;
   if (records) {
     m.rec = (Rec *)p;
     p += sizeof(Rec) * nn;
@@ -434,17 +423,13 @@ __device__ __forceinline__ MolStage carve_stage(char *&p, int amax, bool records
   p += sizeof(int) * amax;
   m.asc = (int *)p;
   p += sizeof(int) * amax;
-  if (records) {
-    m.ty = (uint16_t *)p;
-    p += sizeof(uint16_t) * ((nn + 7) & ~7);
-  }
   return m;
 }
 
 size_t stage_bytes(int amax, bool records) {
   const size_t nn = (size_t)amax * amax;
   size_t b = (sizeof(double) * 3 + sizeof(int) * 2) * (size_t)amax;
-  if (records) b += sizeof(Rec) * nn + sizeof(uint16_t) * ((nn + 7) & ~size_t(7));
+  if (records) b += sizeof(Rec) * nn;
   return b;
 }
 
@@ -464,7 +449,6 @@ __device__ void stage_molecule(MolStage &m, const NbSet &set, int mol, int natom
   for (int t = threadIdx.x; t < total; t += blockDim.x) {
     const int64_t o = slot0 * amax + t;
     m.rec[t] = set.rec[o];
-    m.ty[t] = set.ty[o];
   }
 }
 
@@ -551,11 +535,11 @@ __global__ __launch_bounds__(kPairThreads) void k_arad_pairs(NbSet set1, NbSet s
           const int64_t gx = ((int64_t)i * amax + a) * amax, gy = ((int64_t)j * amax + b) * amax;
           NbView x, y;
           if (kStage) {
-            x = NbView{mi.rec + a * amax, mi.ty + a * amax, set1.rc + gx};
-            y = NbView{mj.rec + b * amax, mj.ty + b * amax, set2.rc + gy};
+            x = NbView{mi.rec + a * amax, set1.rc + gx};
+            y = NbView{mj.rec + b * amax, set2.rc + gy};
           } else {
-            x = NbView{set1.rec + gx, set1.ty + gx, set1.rc + gx};
-            y = NbView{set2.rec + gy, set2.ty + gy, set2.rc + gy};
+            x = NbView{set1.rec + gx, set1.rc + gx};
+            y = NbView{set2.rec + gy, set2.rc + gy};
           }
           const TC S = generic ? arad_distl2<TC, true>(x, mi.cnt[a], mi.asc[a] != 0, y, mj.cnt[b],
                                                        mj.asc[b] != 0, inv_w, maxg, ttab, etab,
@@ -806,7 +790,7 @@ __global__ __launch_bounds__(64) void k_plan_emit(const int32_t *__restrict__ n1
 }
 
 template <bool kF32Self, bool kRecs>
-__device__ void stage_compact(Rec *rec, uint16_t *ty, int *cnt, int *asc, double *self,
+__device__ void stage_compact(Rec *rec, int *cnt, int *asc, double *self,
                               double *zr, double *zc, const NbSet &set, int mol, int natoms,
                               int amax, int atom0) {
   const int64_t slot0 = (int64_t)mol * amax;
@@ -824,7 +808,6 @@ __device__ void stage_compact(Rec *rec, uint16_t *ty, int *cnt, int *asc, double
     const int a = t / natoms, m = t % natoms;
     const int64_t o = (slot0 + a) * amax + m;
     rec[(atom0 + a) * natoms + m] = set.rec[o];
-    ty[(atom0 + a) * natoms + m] = set.ty[o];
   }
 }
 
@@ -870,7 +853,6 @@ __global__ __launch_bounds__(kPairThreads, kWaves) void k_arad_tiles(NbSet set1,
   ptr += sizeof(int) * ta.stage_j_atoms;
   int *ascj = (int *)ptr;
   ptr += sizeof(int) * ta.stage_j_atoms;
-  uint16_t *tyj = (uint16_t *)ptr;
   double *red = (double *)jbase;
   load_tables(ttab, etab, ctl);
 
@@ -889,17 +871,13 @@ __global__ __launch_bounds__(kPairThreads, kWaves) void k_arad_tiles(NbSet set1,
     const int i = tl.i, J = tl.cnt, s = tl.s;
     const int ni = min(max(ta.n1[i], 0), amax);
     if (i != cur_i) {
-      stage_compact<sizeof(TC) == 4, false>(nullptr, nullptr, cnti, asci, selfi, zri, zci, set1, i,
+      stage_compact<sizeof(TC) == 4, false>(nullptr, cnti, asci, selfi, zri, zci, set1, i,
                                             ni, amax, 0);
       cur_i = i;
     }
     for (int jj = 0; jj < J; ++jj)
-      stage_compact<sizeof(TC) == 4, true>(recj, tyj, cntj, ascj, selfj, zrj, zcj, set2,
+      stage_compact<sizeof(TC) == 4, true>(recj, cntj, ascj, selfj, zrj, zcj, set2,
                                            ta.members[tl.off + jj], s, amax, jj * s);
-    if (threadIdx.x < 2) {  // the two padding records after the stage
-      recj[J * s * s + threadIdx.x] = Rec{0.0, 0.0, 0.0, 0.0};
-      tyj[J * s * s + threadIdx.x] = 0;
-    }
     __syncthreads();
 
     // a thread's atom pairs (rounds of kPairThreads; at most kTileRounds,
@@ -932,9 +910,9 @@ __global__ __launch_bounds__(kPairThreads, kWaves) void k_arad_tiles(NbSet set1,
       const int bj = jj * s + b;  // staged atom index of b
       const int j = ta.members[tl.off + jj];
       const int64_t xs = ((int64_t)i * amax + a) * amax;
-      NbView x{set1.rec + xs, set1.ty + xs, set1.rc + xs};
-      NbView y{recj + bj * s, tyj + bj * s, set2.rc + ((int64_t)j * amax + b) * amax};
-      const TC S = arad_distl2<TC, kGeneric, true>(x, cnti[a], asci[a] != 0, y, cntj[bj],
+      NbView x{set1.rec + xs, set1.rc + xs};
+      NbView y{recj + bj * s, set2.rc + ((int64_t)j * amax + b) * amax};
+      const TC S = arad_distl2<TC, kGeneric>(x, cnti[a], asci[a] != 0, y, cntj[bj],
                                                    ascj[bj] != 0, inv_w, maxg, ttab, etab, p.rw2,
                                                    p.cw2);
       const double T = stoch(zri[a] - zrj[bj], zci[a] - zcj[bj], p.rw2, p.cw2);
@@ -1025,7 +1003,6 @@ size_t set_bytes(int64_t nslots, int amax) {
   size_t b = 0;
   b += WsCarver::need(sizeof(Rec) * nn);
   b += WsCarver::need(sizeof(double2) * nn);
-  b += WsCarver::need(sizeof(uint16_t) * nn);
   b += WsCarver::need(sizeof(int32_t) * nslots) * 2;
   b += WsCarver::need(sizeof(double) * nslots) * 3;
   b += WsCarver::need(sizeof(float) * nslots);
@@ -1036,14 +1013,13 @@ bool carve_set(WsCarver &w, int64_t nslots, int amax, NbSet &s) {
   const size_t nn = (size_t)nslots * amax;
   s.rec = w.take<Rec>(nn);
   s.rc = w.take<double2>(nn);
-  s.ty = w.take<uint16_t>(nn);
   s.cnt = w.take<int32_t>(nslots);
   s.asc = w.take<int32_t>(nslots);
   s.self = w.take<double>(nslots);
   s.zr = w.take<double>(nslots);
   s.zc = w.take<double>(nslots);
   s.self32 = w.take<float>(nslots);
-  return s.rec && s.rc && s.ty && s.cnt && s.asc && s.self && s.zr && s.zc && s.self32;
+  return s.rec && s.rc && s.cnt && s.asc && s.self && s.zr && s.zc && s.self32;
 }
 
 AradParams make_params(double width, double cut, double rw, double cw, bool eps_1e11) {
@@ -1110,10 +1086,9 @@ TileLds tile_lds(int amax, int nsig, int lanes) {
   L.j_atoms = j_atoms;
   // the J stage doubles as the reduction buffer (nsig x pairs doubles)
   const size_t red = sizeof(double) * (size_t)pairs * std::max(nsig, 1);
-  int jr = j_recs + 2;  // two padding records for the band loop's read-ahead
+  int jr = j_recs;
   auto stage = [&](int r) {
-    return sizeof(Rec) * (size_t)r + 32 * (size_t)L.j_atoms +
-           sizeof(uint16_t) * (((size_t)r + 7) & ~size_t(7));
+    return sizeof(Rec) * (size_t)r + 32 * (size_t)L.j_atoms;
   };
   while (stage(jr) < red) jr += 64;  // grow the record area until the stage covers the buffer
   L.j_recs = jr;
