@@ -29,6 +29,7 @@
 // multi-workgroup partial product of the already solved part followed by a
 // one-workgroup triangular solve of the block.
 #include <math.h>
+#include <stdio.h>
 
 #include <map>
 #include <memory>
@@ -852,9 +853,86 @@ int potrs_impl(const T *a, int64_t lda, int n, T *x, int64_t ldx, int nrhs, void
   return 0;
 }
 
+// ---- the allocation-free solve ----------------------------------------
+//
+// fmlh_potrs_noalloc takes its scratch (the chained solve's ready flags and
+// tickets, or the blocked solve's partial products) from this file-scope
+// device array instead of a caller workspace.  Host code never passes the
+// array's name to a kernel: on each device the address comes from
+// hipGetSymbolAddress, taken once per device (every device holds its own
+// copy of the array) and kept below.
+constexpr size_t kStaticScratchBytes = size_t(1) << 20;
+__device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// The address of this device's copy of scratch (nullptr and the HIP error
+// recorded on failure).
+void *device_scratch(int &rc) {
+  rc = 0;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) {
+    rc = record_error(e, "hipGetDevice(potrs scratch)");
+    return nullptr;
+  }
+  if (dev < 0 || dev >= kMaxDevices) {
+    rc = record_message(FMLH_EHIP, "fmlh_potrs_noalloc: device index beyond the library's table");
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr) {
+      rc = record_error(e != hipSuccess ? e : hipErrorInvalidSymbol, "hipGetSymbolAddress(scratch)");
+      return nullptr;
+    }
+    g_scratch_ptr[dev] = p;
+  }
+  return g_scratch_ptr[dev];
+}
+
 }  // namespace
 
+namespace fmlh {
+// Takes this device's scratch address ahead of the first solve (fmlh_init).
+int potrs_scratch_init() {
+  int rc = 0;
+  device_scratch(rc);
+  return rc;
+}
+}  // namespace fmlh
+
 extern "C" {
+
+size_t fmlh_potrs_noalloc_max_bytes(void) { return kStaticScratchBytes; }
+
+int fmlh_potrs_noalloc(const double *a, int64_t lda, int n, double *x, int64_t ldx, int nrhs,
+                       void *stream) {
+  FMLH_ARG_CHECK(n >= 0, 3);
+  FMLH_ARG_CHECK(lda >= (n > 1 ? n : 1), 2);
+  FMLH_ARG_CHECK(nrhs >= 0, 6);
+  FMLH_ARG_CHECK(ldx >= (n > 1 ? n : 1), 5);
+  if (n == 0 || nrhs == 0) return 0;
+  FMLH_ARG_CHECK(a != nullptr, 1);
+  FMLH_ARG_CHECK(x != nullptr, 4);
+  const size_t need = potrs_ws<double>(n, nrhs);
+  if (need > kStaticScratchBytes) {
+    char msg[200];
+    snprintf(msg, sizeof(msg),
+             "fmlh_potrs_noalloc: n = %d with nrhs = %d needs %zu bytes of scratch, more than its "
+             "%zu-byte static array (use fmlh_potrs with a workspace)",
+             n, nrhs, need, kStaticScratchBytes);
+    return record_message(-3, msg);
+  }
+  int rc = 0;
+  void *ws = device_scratch(rc);
+  if (!ws) return rc;
+  return potrs_impl<double>(a, lda, n, x, ldx, nrhs, ws, as_stream(stream));
+}
 
 size_t fmlh_potrf_ws_bytes(int n) { return n >= 0 ? 0 : 0; }
 size_t fmlh_spotrf_ws_bytes(int n) { return n >= 0 ? 0 : 0; }

@@ -13,6 +13,9 @@ namespace fmlh {
 // Records a HIP error for fmlh_last_error() and returns FMLH_EHIP.
 int record_error(hipError_t e, const char *where);
 
+// Records a message for fmlh_last_error() and returns rc.
+int record_message(int rc, const char *text);
+
 // Brackets an entry point's dominant kernel with timing events (no-op
 // unless fmlh_timing_enable(1) was called).
 void timing_begin(hipStream_t st);

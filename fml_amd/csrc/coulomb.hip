@@ -316,7 +316,15 @@ __global__ __launch_bounds__(64) void k_coulomb_forms(
 
 }  // namespace
 
-extern "C" int fmlh_init(void) { return ensure_tables(); }
+namespace fmlh {
+int potrs_scratch_init();
+}
+
+extern "C" int fmlh_init(void) {
+  int rc = ensure_tables();
+  if (rc) return rc;
+  return fmlh::potrs_scratch_init();
+}
 
 extern "C" int fmlh_coulomb_form(int mode, const double *z, const double *r,
                                  const int32_t *offsets, int nmol, int nmax, double *cm,

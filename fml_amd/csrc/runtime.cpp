@@ -18,6 +18,11 @@ int record_error(hipError_t e, const char *where) {
   (void)hipGetLastError();
   return FMLH_EHIP;
 }
+
+int record_message(int rc, const char *text) {
+  g_last_error = text;
+  return rc;
+}
 }  // namespace fmlh
 
 extern "C" {

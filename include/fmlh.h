@@ -44,9 +44,10 @@ int fmlh_device_cu_count(void);
 void fmlh_timing_enable(int on);
 int fmlh_timing_collect(double *total_ms, int *count);
 
-/* One-time per-device set-up (the Coulomb diagonal table).  Entry points
- * call it themselves; call it before capturing a graph on a device the
- * library has not used yet, since the set-up copies synchronously. */
+/* One-time per-device set-up (the Coulomb diagonal table, the address of
+ * fmlh_potrs_noalloc's scratch array).  Entry points call it themselves;
+ * call it before capturing a graph on a device the library has not used
+ * yet, since the set-up copies synchronously. */
 int fmlh_init(void);
 
 /* A stream (returned as hipStream_t in *stream) whose kernels run only on
@@ -410,6 +411,22 @@ int fmlh_potrf(double *a, int64_t lda, int n, int32_t *info, void *ws,
 size_t fmlh_potrs_ws_bytes(int n, int nrhs);
 int fmlh_potrs(const double *a, int64_t lda, int n, double *x, int64_t ldx,
                int nrhs, void *ws, size_t ws_bytes, void *stream);
+
+/* fmlh_potrs without a workspace argument: no reference counterpart.  For
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc): the same kernels,
+ * with their scratch (ready flags and tickets of the one-launch solve, or
+ * the blocked solve's partial products; potrs_ws(n, nrhs) bytes) in a
+ * file-scope device array of fmlh_potrs_noalloc_max_bytes() = 1 MiB that
+ * each device holds until the process ends.  A call whose scratch would
+ * exceed it returns -3 (n) with the reason in fmlh_last_error(); nothing is
+ * allocated or freed.  Every call on a device shares that device's array,
+ * which keeps the last call's contents (the solve clears what must start at
+ * zero itself): calls that overlap on one device -- two streams, or two
+ * host threads -- race and must not be issued. */
+size_t fmlh_potrs_noalloc_max_bytes(void);
+int fmlh_potrs_noalloc(const double *a, int64_t lda, int n, double *x, int64_t ldx,
+                       int nrhs, void *stream);
 
 /* Inverse from the factor, as LAPACK dpotri('U'): a holds U from fmlh_potrf
  * (upper triangle, Fortran order) and receives the upper triangle of
