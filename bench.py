@@ -65,7 +65,7 @@ def parse():
                     choices=["arad", "krr", "gaussian64", "gaussian32", "gaussianbf16x3",
                              "laplacian64", "laplacian32", "cholesky64", "cholesky32",
                              "krr36k_fp64", "krr36k_mixed", "serving", "arad4096", "arad4096_fp32", "copy",
-                             "atomic", "atomic_lap", "atomic_mfma", "aras", "inverse64"])
+                             "atomic", "atomic_lap", "atomic_mfma", "aras", "inverse64", "solve64", "solve32"])
     ap.add_argument("--duration", type=float, default=5.0, help="serving run length in seconds")
     ap.add_argument("--n", type=int, default=0, help="problem size override for the 36k workloads")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -707,6 +707,58 @@ def cholesky_workload(args, kind):
     return info, step, None, roofline, None, finish
 
 
+def solve_workload(args, kind):
+    """Triangular solves of cho_solve (dpotrs) with the factor of the 36k
+    Gaussian K + lambda I: one right-hand side, forward and backward (solve64:
+    FP64 factor; solve32: the FP32 factor of the shifted matrix, the mixed
+    fit's preconditioner).  HBM-bound: each direction reads the factor's
+    triangle once."""
+    import torch
+    from fml_amd.device import to_device
+    from fml_amd.krr import add_diagonal, kernel_matrix
+    from fml_amd.math.cho_solve import potrf_inplace
+    from fml_amd.representations import generate_coulomb_matrices
+    import fml_amd._native as nat
+    n = args.n or 36000
+    ms = _cm_set(n, 0, 3, 29, 29)
+    X = generate_coulomb_matrices(to_device(ms.charges), to_device(ms.coords),
+                                  to_device(ms.offsets, np.int32), nmax=29, on_device=True)
+    K = kernel_matrix(X, X, 1000.0)
+    add_diagonal(K, 1e-6)
+    f32 = kind == "solve32"
+    if f32:
+        K = K.to(torch.float32)
+        idx = torch.arange(n, device="cuda")
+        K[idx, idx] += 1e-3
+    assert potrf_inplace(K) == 0
+    y = torch.from_numpy(ms.energies.astype(np.float32 if f32 else np.float64)).cuda()
+    x = torch.empty_like(y)
+    fn = "fmlh_spotrs" if f32 else "fmlh_potrs"
+    wsb = getattr(nat.load(), fn + "_ws_bytes")(n, 1)
+    ws = torch.empty(max(wsb, 1), dtype=torch.uint8, device="cuda")
+
+    def step():
+        x.copy_(y)
+        nat.call(fn, nat.ptr(K), n, n, nat.ptr(x), n, 1, nat.ptr(ws), wsb, nat.stream_ptr())
+    step.graph_capable = True
+    esz = 4 if f32 else 8
+    nbytes = 2 * (n * (n + 1) // 2) * esz  # the triangle, once per direction
+    info = {"metric": "triangular solve GB/s", "unit": "GB/s", "dtype": "f32" if f32 else "f64",
+            "config": {"workload": f"{kind}: forward + backward solve with the {n}x{n} Cholesky factor, "
+                                   "one right-hand side"}}
+
+    def roofline(t):
+        ach = nbytes / t / 1e9
+        return {"bound": "hbm", "achieved": round(ach, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": None, "algorithmic_bytes": nbytes}
+
+    def finish(res):
+        res["value"] = round(nbytes / (res["ms_per_step"] / 1e3) / 1e9, 1)
+        res["solution_head"] = x[:4].double().cpu().numpy().tolist()
+
+    return info, step, None, roofline, None, finish
+
+
 def inverse_workload(args):
     """cho_invert's second half at 36k (cho_solve.py:27-30 -> fcho_solve.f90:
     49-68, dpotri('U')): one step = one fmlh_potri of the FP64 Cholesky
@@ -985,6 +1037,8 @@ def main():
         info, step, units, roofline, cpu_baseline, finish = krr_workload(args)
     elif wl.startswith("krr36k"):
         info, step, units, roofline, cpu_baseline, finish = krr36k_workload(args, wl.split("_")[1])
+    elif wl.startswith("solve"):
+        info, step, units, roofline, cpu_baseline, finish = solve_workload(args, wl)
     elif wl == "inverse64":
         info, step, units, roofline, cpu_baseline, finish = inverse_workload(args)
     elif wl.startswith("cholesky"):

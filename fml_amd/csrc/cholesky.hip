@@ -702,26 +702,69 @@ __global__ __launch_bounds__(256) void k_block_solve(const T *__restrict__ a, in
 // One-launch triangular solves for a single right-hand side.  Workgroups
 // take 64-row blocks in ticket order (forward: block = ticket; backward:
 // block = last - ticket), accumulate the products of their rows with each
-// solved block as its ready flag appears (blocks in solve order), solve
-// their diagonal block in one wave and publish it: the values, a release
-// fence at device scope, then the flag.  A ticket waits only on smaller
-// tickets, whose workgroups are already running, so every wait ends.
-// ctl: [0] ticket counter, [1 + b] ready flag of block b; zeroed before.
+// solved block as its values appear in the publish buffer (blocks in solve
+// order, each tile loaded before its values are awaited), solve their
+// diagonal block in one wave (multiplying by the pivots' reciprocals) and
+// publish it.  A ticket waits only on smaller tickets, whose workgroups are
+// already running, so every wait ends.  ctl[0]: the ticket counter (zero
+// before the launch); pub: n values, the sentinel before the launch.
 
-__device__ __forceinline__ void await_flag(const int32_t *flag) {
-  if (threadIdx.x == 0)
-    while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
-      __builtin_amdgcn_s_sleep(1);
-  __syncthreads();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+// Publication of solved values between workgroups without fences: the
+// publish buffer starts filled with a signalling-NaN pattern that no
+// arithmetic produces; each solved value is written once with a relaxed
+// device-scope atomic store, and a reader polls its value with relaxed
+// device-scope atomic loads until the pattern is gone.  Each value carries
+// its own readiness, so no ordering between values (and no cache
+// write-back or invalidation) is needed.
+template <typename T> struct PubBits;
+template <> struct PubBits<double> {
+  typedef unsigned long long U;
+  static constexpr U kSentinel = 0x7FF4000000000DEAull;
+};
+template <> struct PubBits<float> {
+  typedef unsigned int U;
+  static constexpr U kSentinel = 0x7FA00DEAu;
+};
+
+template <typename T>
+__device__ __forceinline__ void publish(T *pub, int i, T v) {
+  typedef typename PubBits<T>::U U;
+  U u;
+  __builtin_memcpy(&u, &v, sizeof(u));
+  __hip_atomic_store(reinterpret_cast<U *>(pub) + i, u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <typename T>
+__device__ __forceinline__ T await_value(const T *pub, int i) {
+  typedef typename PubBits<T>::U U;
+  const U *p = reinterpret_cast<const U *>(pub) + i;
+  U u = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  while (u == PubBits<T>::kSentinel) {
+    __builtin_amdgcn_s_sleep(1);
+    u = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  T v;
+  __builtin_memcpy(&v, &u, sizeof(v));
+  return v;
+}
+
+// Before a chained solve: the ticket counter to zero, the publish buffer to
+// the sentinel (a kernel, not a memset, so the call can be captured).
+template <typename T>
+__global__ void k_trsv_prep(int32_t *ticket, T *pub, int n) {
+  typedef typename PubBits<T>::U U;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) *ticket = 0;
+  if (i < n) reinterpret_cast<U *>(pub)[i] = PubBits<T>::kSentinel;
 }
 
 template <typename T, bool kForward>
 __global__ __launch_bounds__(256) void k_trsv_chain(const T *__restrict__ a, int64_t lda, int n,
-                                                    T *x, int32_t *ctl) {
+                                                    T *x, int32_t *ctl, T *pub) {
   __shared__ T L[kNB][kNB + 1];
   __shared__ T part[4][kNB];
   __shared__ T rhs[kNB];
+  __shared__ T rinv[kNB];
   __shared__ int s_ticket;
   const int nblk = (n + kNB - 1) / kNB;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -735,20 +778,41 @@ __global__ __launch_bounds__(256) void k_trsv_chain(const T *__restrict__ a, int
     const int r = idx / bs, c = idx % bs;
     L[r][c] = (c <= r) ? a[(int64_t)(r0 + r) * lda + r0 + c] : T(0);
   }
+  // the block's own right-hand side, and the reciprocals of its pivots (the
+  // substitution multiplies by them instead of dividing), before any wait
+  const T xown = tid < bs ? x[r0 + tid] : T(0);
+  __syncthreads();
+  if (tid < bs) rinv[tid] = T(1) / L[tid][tid];
+  // The products with the solved blocks: the tile of block j is loaded one
+  // step ahead (it does not depend on x), so each step waits only for the
+  // flag and the 64 values of x_j.
+  T ca[16], cb[16];
   if (kForward) {
-    // rows wave*16 .. wave*16+15 of the block, lanes over each solved block's columns
+    // rows wave*16 .. wave*16+15 of the block, lanes over block j's columns
     T acc[16];
 #pragma unroll
     for (int i = 0; i < 16; ++i) acc[i] = T(0);
-    for (int j = 0; j < b; ++j) {
-      await_flag(&ctl[1 + j]);
-      const T xv = x[j * kNB + lane];
+    auto load_tile = [&](int j, T *dst) {
 #pragma unroll
       for (int i = 0; i < 16; ++i) {
         const int r = wave * 16 + i;
-        if (r < bs) acc[i] = fma(a[(int64_t)(r0 + r) * lda + j * kNB + lane], xv, acc[i]);
+        dst[i] = r < bs ? a[(int64_t)(r0 + r) * lda + j * kNB + lane] : T(0);
       }
+    };
+    auto step = [&](int j, const T *t) {
+      const T xv = await_value(pub, j * kNB + lane);  // blocks before the last are full
+#pragma unroll
+      for (int i = 0; i < 16; ++i) acc[i] = fma(t[i], xv, acc[i]);
+    };
+    if (b > 0) load_tile(0, ca);
+    int j = 0;
+    for (; j + 1 < b; j += 2) {
+      load_tile(j + 1, cb);
+      step(j, ca);
+      if (j + 2 < b) load_tile(j + 2, ca);
+      step(j + 1, cb);
     }
+    if (j < b) step(j, ca);
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
       T v = acc[i];
@@ -757,55 +821,79 @@ __global__ __launch_bounds__(256) void k_trsv_chain(const T *__restrict__ a, int
       if (lane == 0) part[0][wave * 16 + i] = v;
     }
     __syncthreads();
-    if (tid < bs) rhs[tid] = x[r0 + tid] - part[0][tid];
+    if (tid < bs) rhs[tid] = xown - part[0][tid];
   } else {
-    // lanes over the block's columns, waves over the rows of each solved block
+    // lanes over the block's columns, waves over the rows of each solved
+    // block (row wave + 4 i of block j), in ascending order
     T acc = T(0);
-    for (int j = nblk - 1; j > b; --j) {
-      await_flag(&ctl[1 + j]);
+    auto load_tile = [&](int j, T *dst) {
       const int jb = min(kNB, n - j * kNB);
-      if (lane < bs)
-        for (int tt = wave; tt < jb; tt += 4) {
-          const int row = j * kNB + tt;
-          acc = fma(a[(int64_t)row * lda + r0 + lane], x[row], acc);
-        }
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        const int tt = wave + 4 * i;
+        dst[i] = (lane < bs && tt < jb) ? a[(int64_t)(j * kNB + tt) * lda + r0 + lane] : T(0);
+      }
+    };
+    auto step = [&](int j, const T *t) {
+      const int jb = min(kNB, n - j * kNB);
+      const T xl = lane < jb ? await_value(pub, j * kNB + lane) : T(0);
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        const int tt = wave + 4 * i;
+        const T xv = __shfl(xl, tt, 64);
+        if (tt < jb) acc = fma(t[i], xv, acc);
+      }
+    };
+    if (b < nblk - 1) load_tile(nblk - 1, ca);
+    int j = nblk - 1;
+    for (; j - 1 > b; j -= 2) {
+      load_tile(j - 1, cb);
+      step(j, ca);
+      if (j - 2 > b) load_tile(j - 2, ca);
+      step(j - 1, cb);
     }
+    if (j > b) step(j, ca);
     part[wave][lane] = acc;
     __syncthreads();
-    if (tid < bs) rhs[tid] = x[r0 + tid] - (((part[0][tid] + part[1][tid]) + part[2][tid]) + part[3][tid]);
+    if (tid < bs) rhs[tid] = xown - (((part[0][tid] + part[1][tid]) + part[2][tid]) + part[3][tid]);
   }
   __syncthreads();
   if (wave == 0) {
+    // the block's substitution in one wave: lane r holds row r; each pivot
+    // value is broadcast with readlane (a register read, no LDS round trip)
     T v = lane < bs ? rhs[lane] : T(0);
     if (kForward) {
+      T lnext = L[lane][0];
       for (int c = 0; c < bs; ++c) {
-        const T xc = __shfl(v, c, 64) / L[c][c];
+        const T lc = lnext;
+        if (c + 1 < bs) lnext = L[lane][c + 1];
+        const T xc = lane_value(v, c) * rinv[c];
         if (lane == c) v = xc;
-        if (lane > c) v -= L[lane][c] * xc;
+        if (lane > c) v -= lc * xc;
       }
     } else {
+      T lnext = L[bs - 1][lane];
       for (int c = bs - 1; c >= 0; --c) {
-        const T xc = __shfl(v, c, 64) / L[c][c];
+        const T lc = lnext;
+        if (c > 0) lnext = L[c - 1][lane];
+        const T xc = lane_value(v, c) * rinv[c];
         if (lane == c) v = xc;
-        if (lane < c) v -= L[c][lane] * xc;
+        if (lane < c) v -= lc * xc;
       }
     }
-    if (lane < bs) x[r0 + lane] = v;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    if (lane == 0) __hip_atomic_store(&ctl[1 + b], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (lane < bs) {
+      x[r0 + lane] = v;
+      publish(pub, r0 + lane, v);
+    }
   }
-}
-
-__global__ void k_zero_i32(int32_t *p, int count) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < count) p[i] = 0;
 }
 
 template <typename T>
 size_t potrs_ws(int n, int nrhs) {
   const size_t blocked = WsCarver::need(sizeof(T) * (size_t)ceil_div(n > 0 ? n : 1, kChunk) * kNB *
                                         (nrhs > 0 ? nrhs : 1));
-  const size_t chain = WsCarver::need(sizeof(int32_t) * 2 * ((size_t)ceil_div(n > 0 ? n : 1, kNB) + 1));
+  // ticket counters, then the publish buffer of solved values
+  const size_t chain = WsCarver::need(sizeof(int32_t) * 2) + WsCarver::need(sizeof(T) * (size_t)(n > 0 ? n : 1));
   return blocked > chain ? blocked : chain;
 }
 
@@ -814,14 +902,19 @@ int potrs_impl(const T *a, int64_t lda, int n, T *x, int64_t ldx, int nrhs, void
                hipStream_t st) {
   const int nblk = ceil_div(n, kNB);
   if (nrhs == 1) {  // one launch per direction
-    int32_t *ctl = static_cast<int32_t *>(ws);
-    hipLaunchKernelGGL(k_zero_i32, dim3(ceil_div(2 * (nblk + 1), 256)), dim3(256), 0, st, ctl,
-                       2 * (nblk + 1));
-    FMLH_LAUNCH_CHECK("k_zero_i32");
-    hipLaunchKernelGGL((k_trsv_chain<T, true>), dim3(nblk), dim3(256), 0, st, a, lda, n, x, ctl);
+    WsCarver w(ws, potrs_ws<T>(n, nrhs));
+    int32_t *ctl = w.take<int32_t>(2);
+    T *pub = w.take<T>((size_t)n);
+    if (!ctl || !pub) return -7;
+    const unsigned g = (unsigned)ceil_div(n, 256);
+    hipLaunchKernelGGL(k_trsv_prep<T>, dim3(g), dim3(256), 0, st, ctl, pub, n);
+    FMLH_LAUNCH_CHECK("k_trsv_prep");
+    hipLaunchKernelGGL((k_trsv_chain<T, true>), dim3(nblk), dim3(256), 0, st, a, lda, n, x, ctl, pub);
     FMLH_LAUNCH_CHECK("k_trsv_chain<fwd>");
+    hipLaunchKernelGGL(k_trsv_prep<T>, dim3(g), dim3(256), 0, st, ctl + 1, pub, n);
+    FMLH_LAUNCH_CHECK("k_trsv_prep");
     hipLaunchKernelGGL((k_trsv_chain<T, false>), dim3(nblk), dim3(256), 0, st, a, lda, n, x,
-                       ctl + nblk + 1);
+                       ctl + 1, pub);
     FMLH_LAUNCH_CHECK("k_trsv_chain<bwd>");
     return 0;
   }
@@ -1018,7 +1111,10 @@ int fmlh_potrs(const double *a, int64_t lda, int n, double *x, int64_t ldx, int 
   FMLH_ARG_CHECK(a != nullptr, 1);
   FMLH_ARG_CHECK(x != nullptr, 4);
   FMLH_ARG_CHECK(ws != nullptr && ws_bytes >= potrs_ws<double>(n, nrhs), 7);
-  return potrs_impl<double>(a, lda, n, x, ldx, nrhs, ws, as_stream(stream));
+  timing_begin(as_stream(stream));
+  const int rc = potrs_impl<double>(a, lda, n, x, ldx, nrhs, ws, as_stream(stream));
+  timing_end(as_stream(stream));
+  return rc;
 }
 
 int fmlh_spotrs(const float *a, int64_t lda, int n, float *x, int64_t ldx, int nrhs, void *ws,
@@ -1031,7 +1127,10 @@ int fmlh_spotrs(const float *a, int64_t lda, int n, float *x, int64_t ldx, int n
   FMLH_ARG_CHECK(a != nullptr, 1);
   FMLH_ARG_CHECK(x != nullptr, 4);
   FMLH_ARG_CHECK(ws != nullptr && ws_bytes >= potrs_ws<float>(n, nrhs), 7);
-  return potrs_impl<float>(a, lda, n, x, ldx, nrhs, ws, as_stream(stream));
+  timing_begin(as_stream(stream));
+  const int rc = potrs_impl<float>(a, lda, n, x, ldx, nrhs, ws, as_stream(stream));
+  timing_end(as_stream(stream));
+  return rc;
 }
 
 }  // extern "C"
