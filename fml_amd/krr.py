@@ -68,19 +68,32 @@ class FitReport:
     iterations: int = 0
     shift: float = 0.0
     backward_error: float = float("nan")
+    residual: float = float("nan")
     history: list = field(default_factory=list)
+
+
+class ConvergenceError(RuntimeError):
+    """The mixed-precision fit stopped short of its tolerance; .alpha holds
+    the last iterate and .report the fit's figures."""
+
+    def __init__(self, msg, alpha=None, report=None):
+        super().__init__(msg)
+        self.alpha = alpha
+        self.report = report
 
 
 class KRR:
     """Device-resident KRR model.  X rows are samples (n, d)."""
 
-    def __init__(self, sigma, lam, kernel="gaussian", precision="fp64", tol=1e-14, max_iter=200):
+    def __init__(self, sigma, lam, kernel="gaussian", precision="fp64", tol=1e-12, max_iter=400,
+                 check_every=10):
         self.sigma = float(sigma)
         self.lam = float(lam)
         self.kernel = kernel
         self.precision = precision
         self.tol = tol
         self.max_iter = max_iter
+        self.check_every = max(1, int(check_every))
         self.alpha = None
         self.X = None
         self.report = None
@@ -121,6 +134,14 @@ class KRR:
         return self
 
     def _fit_mixed(self, K, yd, rep):
+        """FP32 factor of K + (lambda + mu) I (mu the smallest shift, from 0
+        up, with which it factors) as the preconditioner of conjugate
+        gradients on the FP64 system (K + lambda I) alpha = y, every residual
+        and update in FP64.  The iteration stays on the device: step lengths
+        are device scalars, and the host reads the residual norm once every
+        `check_every` iterations.  Raises ConvergenceError when the
+        residual has not reached tol * |y| after max_iter iterations (or
+        the preconditioned operator stops being positive definite)."""
         t = torch()
         n = int(K.shape[0])
         t1 = time.perf_counter()
@@ -142,49 +163,75 @@ class KRR:
         rep.factor_s = time.perf_counter() - t1
         rep.shift = shift
         t2 = time.perf_counter()
+        z32 = t.empty(n, dtype=t.float32, device="cuda")
 
-        def precond(r):
-            z = r.to(t.float32)
-            potrs_inplace(F, z)
-            return z.to(t.float64)
+        def precond(r, out):
+            z32.copy_(r)
+            potrs_inplace(F, z32)
+            out.copy_(z32)
+            return out
+
+        # the products with K + lam I read the full (symmetric) FP64 kernel row by
+        # row, coalesced: K x + lam x
+        def apply(v, out):
+            predict_from_kernel(K, v, out=out)
+            out.add_(v, alpha=self.lam)
+            return out
+
+        def residual(v, out):  # y - (K + lam I) v
+            apply(v, out)
+            out.neg_().add_(yd)
+            return out
 
         # preconditioned CG on (K + lam I) x = y, FP64 residuals
-        x = precond(yd)
+        x = empty((n,), np.float64)
+        precond(yd, x)
         r = empty((n,), np.float64)
-        sym_residual(K, self.lam, x, yd, r)
-        ynorm = float(t.linalg.norm(yd).item())
-        z = precond(r)
+        residual(x, r)
+        ynorm = t.linalg.norm(yd)
+        z = empty((n,), np.float64)
+        precond(r, z)
         p = z.clone()
-        rz = float(t.dot(r, z).item())
+        rz = t.dot(r, z)
         Ap = empty((n,), np.float64)
-        zero = t.zeros(n, dtype=t.float64, device="cuda")
+        tol = float(self.tol)
         it = 0
+        converged = False
+        pap_ok = t.ones((), dtype=t.bool, device="cuda")
         for it in range(1, self.max_iter + 1):
-            # Ap = (K + lam I) p  ==  -(0 - (K + lam I) p)
-            sym_residual(K, self.lam, p, zero, Ap)
-            Ap.neg_()
-            pAp = float(t.dot(p, Ap).item())
-            if pAp <= 0:
-                break
+            apply(p, Ap)
+            pAp = t.dot(p, Ap)
+            pap_ok &= pAp > 0
             a = rz / pAp
-            x.add_(p, alpha=a)
-            r.add_(Ap, alpha=-a)
-            rn = float(t.linalg.norm(r).item())
-            rep.history.append(rn / ynorm)
-            if rn <= self.tol * ynorm:
-                break
-            if it % 10 == 0:  # re-anchor the recurrence on the true FP64 residual
-                sym_residual(K, self.lam, x, yd, r)
-            z = precond(r)
-            rz_new = float(t.dot(r, z).item())
+            x.add_(p * a)
+            r.sub_(Ap * a)
+            if it % self.check_every == 0 or it == self.max_iter:
+                residual(x, r)  # re-anchor on the true FP64 residual
+                rel = float((t.linalg.norm(r) / ynorm).item())  # one host read per check
+                rep.history.append(rel)
+                if not bool(pap_ok.item()):
+                    break
+                if rel <= tol:
+                    converged = True
+                    break
+            precond(r, z)
+            rz_new = t.dot(r, z)
             p.mul_(rz_new / rz).add_(z)
             rz = rz_new
         rep.iterations = it
         t.cuda.synchronize()
         rep.solve_s = time.perf_counter() - t2
-        sym_residual(K, self.lam, x, yd, r)
+        residual(x, r)
+        rep.residual = float((t.linalg.norm(r) / ynorm).item())
         knorm = float(t.linalg.matrix_norm(K, ord="fro").item()) if n <= 20000 else float("nan")
         rep.backward_error = float(t.linalg.norm(r).item()) / (knorm * float(t.linalg.norm(x).item()))
+        if not converged:
+            why = ("the preconditioned operator was not positive definite" if not bool(pap_ok.item())
+                   else f"{it} iterations")
+            raise ConvergenceError(
+                f"mixed-precision fit did not reach |y - (K + lambda I) alpha| <= {tol:g} |y| "
+                f"({why}; relative residual {rep.residual:.3g}, FP32 shift {shift:g}); "
+                "raise max_iter or use precision='fp64'", alpha=x, report=rep)
         return x
 
     # -- predict -----------------------------------------------------------

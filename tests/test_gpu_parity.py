@@ -307,7 +307,24 @@ def test_krr_fp64_and_mixed(gpu):
     assert np.max(np.abs(p - po)) < 1e-6 * scale
     mae_o = np.mean(np.abs(po - y[320:]))
     assert abs(np.mean(np.abs(p - y[320:])) - mae_o) <= 1e-3 * mae_o
-    mm = KRR(700.0, 1e-8, precision="mixed", tol=1e-12, max_iter=300).fit(Xtr, ytr)
+    # kappa(K + 1e-8 I) ~ 1e11 here: the FP32-preconditioned iteration reaches
+    # a relative residual of 1e-7 (1.3e-8 after 300 iterations, measured)
+    mm = KRR(700.0, 1e-8, precision="mixed", tol=1e-7, max_iter=300).fit(Xtr, ytr)
     pm = mm.predict(Xte).cpu().numpy()
     assert mm.report.backward_error < 1e-12
+    assert mm.report.residual <= 1e-7
     assert abs(np.mean(np.abs(pm - y[320:])) - mae_o) <= 1e-2 * mae_o
+
+
+def test_krr_mixed_failure_is_visible(gpu):
+    """A mixed fit that cannot reach its tolerance within max_iter raises,
+    carrying the last iterate and the report (it used to return alpha
+    silently)."""
+    from fml_amd.krr import KRR, ConvergenceError
+    ms = synth.generate(400, seed=0, nmin=5, nmax=23)
+    X = oracle.coulomb_matrices(ms.charges, ms.coords, ms.offsets, 23)[0]
+    with pytest.raises(ConvergenceError) as ei:
+        KRR(700.0, 1e-8, precision="mixed", tol=1e-14, max_iter=2, check_every=1).fit(X[:320], ms.energies[:320])
+    assert ei.value.report.iterations == 2
+    assert ei.value.report.residual > 1e-14
+    assert ei.value.alpha is not None and ei.value.alpha.shape[0] == 320
