@@ -39,8 +39,10 @@ SIGNATURES = {
                                     c_void_p, c_void_p, c_void_p]),
     "fmlh_gaussian_kernel": (c_int, [c_void_p, c_int64, c_int, c_void_p, c_int64, c_int, c_int, c_double,
                                      c_void_p, c_int64, c_void_p]),
+    "fmlh_gaussian_kernel_f32_ws_bytes": (c_size_t, [c_int, c_int, c_int, c_int, c_int]),
     "fmlh_gaussian_kernel_f32": (c_int, [c_void_p, c_int64, c_int, c_void_p, c_int64, c_int, c_int,
-                                         c_double, c_int, c_void_p, c_void_p, c_int64, c_void_p]),
+                                         c_double, c_int, c_void_p, c_void_p, c_int64, c_void_p,
+                                         c_size_t, c_void_p, c_void_p]),
     "fmlh_laplacian_kernel": (c_int, [c_void_p, c_int64, c_int, c_void_p, c_int64, c_int, c_int, c_double,
                                       c_void_p, c_int64, c_void_p]),
     "fmlh_laplacian_kernel_f32": (c_int, [c_void_p, c_int64, c_int, c_void_p, c_int64, c_int, c_int,

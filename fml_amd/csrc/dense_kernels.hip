@@ -294,19 +294,39 @@ int fmlh_gaussian_kernel(const double *a, int64_t lda, int na, const double *b, 
   return 0;
 }
 
-int fmlh_gaussian_kernel_bf16x3(const float *a, int64_t lda, int na, const float *b, int64_t ldb,
-                                int nb, int d, double sigma, float *k32, double *k64, int64_t ldk,
-                                void *stream);
+}  // extern "C"
+
+namespace fmlh {  // precision.hip
+size_t gaussian_bf16x3_ws_bytes(const float *a, int64_t lda, int na, const float *b, int64_t ldb,
+                                int nb, int d);
+int gaussian_bf16x3(const float *a, int64_t lda, int na, const float *b, int64_t ldb, int nb,
+                    int d, double sigma, float *k32, double *k64, int64_t ldk, void *ws,
+                    size_t ws_bytes, unsigned long long *recomputed, hipStream_t st);
+}  // namespace fmlh
+
+extern "C" {
+
+size_t fmlh_gaussian_kernel_f32_ws_bytes(int na, int nb, int d, int mode, int same) {
+  if (na < 0 || nb < 0 || d < 0 || mode != 1) return 0;
+  const float *a = reinterpret_cast<const float *>(16);
+  const float *b = same ? a : reinterpret_cast<const float *>(32);
+  return gaussian_bf16x3_ws_bytes(a, d, na, b, d, same ? na : nb, d);
+}
 
 int fmlh_gaussian_kernel_f32(const float *a, int64_t lda, int na, const float *b, int64_t ldb,
                              int nb, int d, double sigma, int mode, float *k32, double *k64,
-                             int64_t ldk, void *stream) {
+                             int64_t ldk, void *ws, size_t ws_bytes, unsigned long long *recomputed,
+                             void *stream) {
   int rc = check_dense_args(a, lda, na, b, ldb, nb, d, sigma, k32 ? (void *)k32 : (void *)k64, ldk);
   if (rc) return rc;
   FMLH_ARG_CHECK(mode == 0 || mode == 1, 9);
   FMLH_ARG_CHECK((k32 == nullptr) != (k64 == nullptr), 10);
   if (na == 0 || nb == 0) return 0;
-  if (mode == 1) return fmlh_gaussian_kernel_bf16x3(a, lda, na, b, ldb, nb, d, sigma, k32, k64, ldk, stream);
+  if (mode == 1) {
+    FMLH_ARG_CHECK(ws != nullptr && ws_bytes >= gaussian_bf16x3_ws_bytes(a, lda, na, b, ldb, nb, d), 13);
+    return gaussian_bf16x3(a, lda, na, b, ldb, nb, d, sigma, k32, k64, ldk, ws, ws_bytes, recomputed,
+                           as_stream(stream));
+  }
   const double inv = -0.5 / (sigma * sigma);
   timing_begin(as_stream(stream));
   if (k32)

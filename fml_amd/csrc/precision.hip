@@ -1,19 +1,26 @@
 // Split-BF16x3 Gaussian kernel (precision form of fgaussian_kernel,
 // fml/kernels/fkernels.f90:317-349).
 //
-// Each float operand x is split into hi = bf16(x) and lo = bf16(x - hi)
-// (round to nearest even) while it is staged into LDS; the dot products
-// a.b ~ hi_a.hi_b + hi_a.lo_b + lo_a.hi_b are three v_mfma_f32_16x16x32_bf16
-// per 16x16 block and 32-wide k step, accumulated in FP32.  The dropped
-// lo.lo term and the bf16 rounding of lo leave ~2^-16 relative error per
-// product, so |a|^2 + |b|^2 - 2 a.b loses accuracy when d2 is small against
-// the norms: entries with d2 <= tau (|a|^2 + |b|^2) are recomputed from direct
-// FP32 differences by their wavefront (near_pairs.h).  Norms are FP32 sums of the original floats.
+// Each float operand row is split once, by k_split_bf16, into two bf16
+// planes in the workspace: hi = bf16(x), lo = bf16(x - hi) (round to nearest
+// even), columns zero-padded to a multiple of 32, with the row's FP32
+// squared norm (sequential fused sum of the original floats, so a row's norm
+// does not depend on the tile that uses it).  The dot products
+//   a.b ~ hi_a.hi_b + hi_a.lo_b + lo_a.hi_b
+// are three v_mfma_f32_16x16x32_bf16 per 16x16 block and 32-wide k step,
+// accumulated in FP32.  The dropped lo.lo term and the bf16 rounding of lo
+// leave ~2^-16 relative error per product, so |a|^2 + |b|^2 - 2 a.b loses
+// accuracy when d2 is small against the norms: entries with d2 <= tau
+// (|a|^2 + |b|^2) are recomputed from direct FP32 differences by their
+// wavefront (near_pairs.h), and counted when the caller asks.
 //
-// Tile: 64 x 64 outputs per 256 threads, waves 2 x 2, each 32 x 32 from
-// 2 x 2 16x16 blocks; 32-wide k stages double buffered in LDS (hi and lo
-// planes of both operands, rows padded to 40 bf16 = 80 bytes so the
-// 16-byte reads of 8 consecutive rows hit distinct banks).
+// Tile: 128 x 128 outputs per 256 threads, waves 2 x 2, each wave 64 x 64
+// from 4 x 4 16x16 blocks (48 MFMAs per k step).  A k step stages 128 rows x
+// 32 columns of the four planes (32 KB) in LDS, rows padded to 40 bf16 = 80
+// bytes so the 16-byte fragment reads of 16 consecutive rows fall in
+// distinct banks; the next step's planes are loaded into registers while the
+// current one is multiplied.  The grid is 1-D in XCD-grouped tile order
+// (gemm_nt.h), so the tiles one XCD runs share operand rows in its L2.
 //
 // bf16 16x16x32 MFMA lane layout on gfx950 (checked with an in-bounds probe):
 //   A lane l -> A[l%16][8*(l/16) + e], e = 0..7
@@ -22,7 +29,10 @@
 #include <math.h>
 
 #include "common.h"
+#include "gemm_nt.h"
 #include "near_pairs.h"
+
+#pragma clang fp contract(off)
 
 namespace {
 
@@ -30,18 +40,12 @@ using namespace fmlh;
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f4v __attribute__((ext_vector_type(4)));
+typedef unsigned int u4v __attribute__((ext_vector_type(4)));
 
-constexpr int kBK = 32;
-constexpr int kRow = 40;  // bf16 per LDS row
-
-struct B3Smem {
-  uint16_t ph[2][64 * kRow];
-  uint16_t pl[2][64 * kRow];
-  uint16_t qh[2][64 * kRow];
-  uint16_t ql[2][64 * kRow];
-  float pn[64];
-  float qn[64];
-};
+constexpr int kBK = 32;     // k step
+constexpr int kRow = 40;    // bf16 per LDS row (32 + 8 padding)
+constexpr int kMT = 128;    // tile edge
+constexpr float kTau = 2e-2f;
 
 __device__ __forceinline__ uint16_t bf16_rne(float x) {
   const uint32_t u = __float_as_uint(x);
@@ -49,113 +53,149 @@ __device__ __forceinline__ uint16_t bf16_rne(float x) {
 }
 __device__ __forceinline__ float bf16_to_f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
-template <typename OutT>
-__global__ __launch_bounds__(256) void k_gaussian_bf16x3(const float *__restrict__ A, int64_t lda,
-                                                         int na, const float *__restrict__ B,
-                                                         int64_t ldb, int nb, int d, double inv,
-                                                         OutT *__restrict__ K, int64_t ldk,
-                                                         float tau) {
-  __shared__ __attribute__((aligned(16))) B3Smem s;
-  const int tid = threadIdx.x;
-  const int lane = tid & 63, wave = tid >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
-  const int n0 = blockIdx.x * 64;  // A rows (output columns)
-  const int m0 = blockIdx.y * 64;  // B rows (output rows)
-  const int lrow = tid >> 2;       // staged row
-  const int lk = (tid & 3) * 8;    // staged k offset
-  const bool prow_ok = m0 + lrow < nb, qrow_ok = n0 + lrow < na;
-  const float *prow = B + (int64_t)(m0 + lrow) * ldb;
-  const float *qrow = A + (int64_t)(n0 + lrow) * lda;
+__host__ __device__ inline int padded_cols(int d) { return (d + kBK - 1) / kBK * kBK; }
 
-  float rp[8], rq[8];
-  float pn_acc = 0.f, qn_acc = 0.f;
+// One thread per (row, 8-column chunk) for the planes; the chunk-0 thread of
+// each row also forms the row's squared norm.
+__global__ __launch_bounds__(256) void k_split_bf16(const float *__restrict__ X, int64_t ld, int n,
+                                                    int d, int dp, uint16_t *__restrict__ hi,
+                                                    uint16_t *__restrict__ lo,
+                                                    float *__restrict__ nrm) {
+  const int chunks = dp / 8;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)n * chunks) return;
+  const int row = (int)(t / chunks), c0 = (int)(t % chunks) * 8;
+  const float *x = X + (int64_t)row * ld;
+  uint16_t h[8], l[8];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const float v = c0 + e < d ? x[c0 + e] : 0.f;
+    h[e] = bf16_rne(v);
+    l[e] = bf16_rne(v - bf16_to_f(h[e]));
+  }
+  u4v hv, lv;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    hv[e] = (uint32_t)h[2 * e] | ((uint32_t)h[2 * e + 1] << 16);
+    lv[e] = (uint32_t)l[2 * e] | ((uint32_t)l[2 * e + 1] << 16);
+  }
+  *reinterpret_cast<u4v *>(hi + (int64_t)row * dp + c0) = hv;
+  *reinterpret_cast<u4v *>(lo + (int64_t)row * dp + c0) = lv;
+  if (c0 == 0) {
+    float s = 0.f;
+    for (int k = 0; k < d; ++k) s = fmaf(x[k], x[k], s);
+    nrm[row] = s;
+  }
+}
+
+struct Plane {
+  const uint16_t *hi, *lo;
+  const float *nrm;
+};
+
+struct B3Smem {
+  uint16_t ah[kMT * kRow];
+  uint16_t al[kMT * kRow];
+  uint16_t bh[kMT * kRow];
+  uint16_t bl[kMT * kRow];
+};
+
+// K[gm][gn] for B rows gm (output rows) and A rows gn (output columns).
+template <typename OutT>
+__global__ __launch_bounds__(256, 2) void k_gauss_b3(Plane pa, int na, Plane pb, int nb, int dp,
+                                                     const float *__restrict__ A, int64_t lda,
+                                                     const float *__restrict__ B, int64_t ldb,
+                                                     int d, double inv, OutT *__restrict__ K,
+                                                     int64_t ldk, unsigned long long *nrecomp,
+                                                     int gx, int gy) {
+  __shared__ __attribute__((aligned(16))) B3Smem s;
+  int tx, ty;
+  if (!xcd_tile(gx, gy, tx, ty)) return;
+  const int n0 = tx * kMT;  // A rows (output columns)
+  const int m0 = ty * kMT;  // B rows (output rows)
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  // staging: each thread moves 2 chunks of 8 bf16 per plane and k step
+  // (rows tid / 4 and 64 + tid / 4, columns 8 (tid % 4))
+  const int srow = tid >> 2, scol = (tid & 3) * 8;
+  u4v rah[2], ral[2], rbh[2], rbl[2];
   auto gload = [&](int k0) {
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      const int k = k0 + lk + e;
-      rp[e] = (prow_ok && k < d) ? prow[k] : 0.f;
-      rq[e] = (qrow_ok && k < d) ? qrow[k] : 0.f;
+    for (int h = 0; h < 2; ++h) {
+      const int r = srow + 64 * h;
+      const int ga = n0 + r, gb = m0 + r;
+      const int64_t oa = (int64_t)min(ga, na - 1) * dp + k0 + scol;
+      const int64_t ob = (int64_t)min(gb, nb - 1) * dp + k0 + scol;
+      rah[h] = *reinterpret_cast<const u4v *>(pa.hi + oa);
+      ral[h] = *reinterpret_cast<const u4v *>(pa.lo + oa);
+      rbh[h] = *reinterpret_cast<const u4v *>(pb.hi + ob);
+      rbl[h] = *reinterpret_cast<const u4v *>(pb.lo + ob);
     }
   };
-  auto lstore = [&](int buf) {
-    uint16_t hp[8], lp[8], hq[8], lq[8];
+  auto lstore = [&]() {
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      pn_acc = pn_acc + rp[e] * rp[e];
-      qn_acc = qn_acc + rq[e] * rq[e];
-      hp[e] = bf16_rne(rp[e]);
-      lp[e] = bf16_rne(rp[e] - bf16_to_f(hp[e]));
-      hq[e] = bf16_rne(rq[e]);
-      lq[e] = bf16_rne(rq[e] - bf16_to_f(hq[e]));
-    }
-    const int o = lrow * kRow + lk;
-#pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      s.ph[buf][o + e] = hp[e];
-      s.pl[buf][o + e] = lp[e];
-      s.qh[buf][o + e] = hq[e];
-      s.ql[buf][o + e] = lq[e];
+    for (int h = 0; h < 2; ++h) {
+      const int o = (srow + 64 * h) * kRow + scol;
+      *reinterpret_cast<u4v *>(&s.ah[o]) = rah[h];
+      *reinterpret_cast<u4v *>(&s.al[o]) = ral[h];
+      *reinterpret_cast<u4v *>(&s.bh[o]) = rbh[h];
+      *reinterpret_cast<u4v *>(&s.bl[o]) = rbl[h];
     }
   };
 
-  f4v acc[2][2];
+  f4v acc[4][4];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) acc[i][j] = f4v{0.f, 0.f, 0.f, 0.f};
+    for (int j = 0; j < 4; ++j) acc[i][j] = f4v{0.f, 0.f, 0.f, 0.f};
 
-  const int nk = (d + kBK - 1) / kBK;
+  const int nk = dp / kBK;
   gload(0);
-  lstore(0);
-  __syncthreads();
   for (int kt = 0; kt < nk; ++kt) {
-    const int buf = kt & 1;
+    __syncthreads();  // the previous step's readers are done
+    lstore();
+    __syncthreads();
     if (kt + 1 < nk) gload((kt + 1) * kBK);
     const int ko = 8 * (lane >> 4);
-    bf16x8 ah[2], al[2], bh[2], bl[2];
+    bf16x8 bh[4], bl[4];
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
-      const int pr = (wm * 32 + i * 16 + (lane & 15)) * kRow + ko;
-      const int qr = (wn * 32 + i * 16 + (lane & 15)) * kRow + ko;
-      ah[i] = *reinterpret_cast<const bf16x8 *>(&s.ph[buf][pr]);
-      al[i] = *reinterpret_cast<const bf16x8 *>(&s.pl[buf][pr]);
-      bh[i] = *reinterpret_cast<const bf16x8 *>(&s.qh[buf][qr]);
-      bl[i] = *reinterpret_cast<const bf16x8 *>(&s.ql[buf][qr]);
+    for (int j = 0; j < 4; ++j) {
+      const int r = (wm * 64 + j * 16 + (lane & 15)) * kRow + ko;
+      bh[j] = *reinterpret_cast<const bf16x8 *>(&s.bh[r]);
+      bl[j] = *reinterpret_cast<const bf16x8 *>(&s.bl[r]);
     }
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < 4; ++i) {
+      const int r = (wn * 64 + i * 16 + (lane & 15)) * kRow + ko;
+      const bf16x8 ah = *reinterpret_cast<const bf16x8 *>(&s.ah[r]);
+      const bf16x8 al = *reinterpret_cast<const bf16x8 *>(&s.al[r]);
 #pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+      for (int j = 0; j < 4; ++j) {
+        // block (j, i): rows from B (A operand of the MFMA), columns from A
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[j], ah, acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], al, acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], ah, acc[i][j], 0, 0, 0);
       }
-    if (kt + 1 < nk) lstore(buf ^ 1);
-    __syncthreads();
+    }
   }
-  // row norms: the four threads of a row sit in consecutive lanes
-  pn_acc += __shfl_xor(pn_acc, 1, 64);
-  pn_acc += __shfl_xor(pn_acc, 2, 64);
-  qn_acc += __shfl_xor(qn_acc, 1, 64);
-  qn_acc += __shfl_xor(qn_acc, 2, 64);
-  if ((tid & 3) == 0) {
-    s.pn[lrow] = pn_acc;
-    s.qn[lrow] = qn_acc;
-  }
-  __syncthreads();
+  // epilogue: d2 = |b|^2 + |a|^2 - 2 a.b, near pairs recomputed, exp
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < 4; ++i) {
+    const int gn = n0 + wn * 64 + i * 16 + (lane & 15);
+    const float qn = gn < na ? pa.nrm[gn] : 0.f;
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < 4; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int m = wm * 32 + i * 16 + 4 * (lane >> 4) + r;
-        const int n = wn * 32 + j * 16 + (lane & 15);
-        const int gm = m0 + m, gn = n0 + n;
+        const int gm = m0 + wm * 64 + j * 16 + 4 * (lane >> 4) + r;
         const bool valid = gm < nb && gn < na;
-        const float pn = s.pn[m], qn = s.qn[n];
-        float d2 = pn + qn - 2.f * acc[i][j][r];
-        const bool near = valid && d2 <= tau * (pn + qn);
+        const float pn = valid ? pb.nrm[gm] : 0.f;
+        float d2 = fmaf(-2.f, acc[i][j][r], pn + qn);
+        const bool near = valid && d2 <= kTau * (pn + qn);
+        if (nrecomp) {
+          const unsigned long long m = __ballot(near);
+          if (lane == 0 && m) atomicAdd(nrecomp, (unsigned long long)__popcll(m));
+        }
         d2 = recompute_near(near, d2, A, lda, gn, B, ldb, gm, d);
         if (!valid) continue;
         if (d2 < 0.f) d2 = 0.f;
@@ -166,24 +206,70 @@ __global__ __launch_bounds__(256) void k_gaussian_bf16x3(const float *__restrict
           v = (OutT)expf((float)inv * d2);
         __builtin_nontemporal_store(v, K + (int64_t)gn + (int64_t)gm * ldk);
       }
+  }
+}
+
+size_t plane_bytes(int n, int d) {
+  const size_t dp = (size_t)padded_cols(d);
+  return 2 * WsCarver::need(sizeof(uint16_t) * (size_t)n * dp) + WsCarver::need(sizeof(float) * (size_t)n);
+}
+
+bool same_operand(const float *a, int64_t lda, int na, const float *b, int64_t ldb, int nb) {
+  return a == b && lda == ldb && na == nb;
+}
+
+int split(WsCarver &w, const float *X, int64_t ld, int n, int d, Plane &p, hipStream_t st) {
+  const int dp = padded_cols(d);
+  uint16_t *hi = w.take<uint16_t>((size_t)n * dp);
+  uint16_t *lo = w.take<uint16_t>((size_t)n * dp);
+  float *nrm = w.take<float>((size_t)n);
+  if (!hi || !lo || !nrm) return -1;
+  const int64_t work = (int64_t)n * (dp / 8);
+  hipLaunchKernelGGL(k_split_bf16, dim3((unsigned)ceil_div64(work, 256)), dim3(256), 0, st, X, ld,
+                     n, d, dp, hi, lo, nrm);
+  FMLH_LAUNCH_CHECK("k_split_bf16");
+  p = Plane{hi, lo, nrm};
+  return 0;
 }
 
 }  // namespace
 
-extern "C" int fmlh_gaussian_kernel_bf16x3(const float *a, int64_t lda, int na, const float *b,
-                                           int64_t ldb, int nb, int d, double sigma, float *k32,
-                                           double *k64, int64_t ldk, void *stream) {
+namespace fmlh {
+
+size_t gaussian_bf16x3_ws_bytes(const float *a, int64_t lda, int na, const float *b, int64_t ldb,
+                                int nb, int d) {
+  size_t bytes = plane_bytes(na, d);
+  if (!same_operand(a, lda, na, b, ldb, nb)) bytes += plane_bytes(nb, d);
+  return bytes;
+}
+
+int gaussian_bf16x3(const float *a, int64_t lda, int na, const float *b, int64_t ldb, int nb,
+                    int d, double sigma, float *k32, double *k64, int64_t ldk, void *ws,
+                    size_t ws_bytes, unsigned long long *recomputed, hipStream_t st) {
   const double inv = -0.5 / (sigma * sigma);
-  dim3 grid(ceil_div(na, 64), ceil_div(nb, 64));
-  hipStream_t st = as_stream(stream);
+  WsCarver w(ws, ws_bytes);
+  Plane pa, pb;
+  int rc = split(w, a, lda, na, d, pa, st);
+  if (rc) return rc == -1 ? -12 : rc;
+  if (same_operand(a, lda, na, b, ldb, nb)) {
+    pb = pa;
+  } else {
+    rc = split(w, b, ldb, nb, d, pb, st);
+    if (rc) return rc == -1 ? -12 : rc;
+  }
+  const int gx = ceil_div(na, kMT), gy = ceil_div(nb, kMT);
+  const unsigned grid = (unsigned)xcd_grid(gx, gy);
+  const int dp = padded_cols(d);
   timing_begin(st);
   if (k32)
-    hipLaunchKernelGGL(k_gaussian_bf16x3<float>, grid, dim3(256), 0, st, a, lda, na, b, ldb, nb, d,
-                       inv, k32, ldk, 2e-2f);
+    hipLaunchKernelGGL(k_gauss_b3<float>, dim3(grid), dim3(256), 0, st, pa, na, pb, nb, dp, a, lda,
+                       b, ldb, d, inv, k32, ldk, recomputed, gx, gy);
   else
-    hipLaunchKernelGGL(k_gaussian_bf16x3<double>, grid, dim3(256), 0, st, a, lda, na, b, ldb, nb, d,
-                       inv, k64, ldk, 2e-2f);
+    hipLaunchKernelGGL(k_gauss_b3<double>, dim3(grid), dim3(256), 0, st, pa, na, pb, nb, dp, a, lda,
+                       b, ldb, d, inv, k64, ldk, recomputed, gx, gy);
   timing_end(st);
-  FMLH_LAUNCH_CHECK("k_gaussian_bf16x3");
+  FMLH_LAUNCH_CHECK("k_gauss_b3");
   return 0;
 }
+
+}  // namespace fmlh

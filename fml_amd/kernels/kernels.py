@@ -37,10 +37,12 @@ def _rows(x, dtype):
     return to_device(np.asarray(x).T, dtype)
 
 
-def gaussian_kernel_rows(X1, X2, sigma, precision="fp64", out=None, out_dtype=None):
+def gaussian_kernel_rows(X1, X2, sigma, precision="fp64", out=None, out_dtype=None, recomputed=None):
     """Device form: X1 (na, d), X2 (nb, d) contiguous CUDA tensors ->
     tensor (nb, na) whose [i, j] is K(j, i) (the memory of the Fortran
-    (na, nb) kernel).  precision: fp64 | fp32 (FP32 MFMA) | bf16x3."""
+    (na, nb) kernel).  precision: fp64 | fp32 (FP32 MFMA) | bf16x3.
+    recomputed: optional device int64 tensor of one element, incremented by
+    the number of near pairs the bf16x3 form recomputed."""
     na, d = X1.shape
     nb = X2.shape[0]
     mode = _PRECISION[precision]
@@ -53,9 +55,13 @@ def gaussian_kernel_rows(X1, X2, sigma, precision="fp64", out=None, out_dtype=No
     od = out_dtype or np.float32
     k = out if out is not None else empty((nb, na), od)
     is64 = k.dtype == torch().float64
+    same = int(X1.data_ptr() == X2.data_ptr() and X1.stride(0) == X2.stride(0) and na == nb)
+    wsb = _native.load().fmlh_gaussian_kernel_f32_ws_bytes(na, nb, d, mode, same)
+    ws = workspace(wsb) if wsb else None
     _native.call("fmlh_gaussian_kernel_f32", _native.ptr(X1), X1.stride(0), na, _native.ptr(X2),
                  X2.stride(0), nb, d, float(sigma), mode, None if is64 else _native.ptr(k),
-                 _native.ptr(k) if is64 else None, k.stride(0), _native.stream_ptr())
+                 _native.ptr(k) if is64 else None, k.stride(0), _native.ptr(ws), wsb,
+                 _native.ptr(recomputed), _native.stream_ptr())
     return k
 
 

@@ -25,11 +25,13 @@ from .kernels.kernels import gaussian_kernel_rows, laplacian_kernel_rows
 from .math.cho_solve import potrf_inplace, potrs_inplace
 
 
-def kernel_matrix(X1, X2, sigma, kernel="gaussian", precision="fp64", out=None, out_dtype=None):
+def kernel_matrix(X1, X2, sigma, kernel="gaussian", precision="fp64", out=None, out_dtype=None,
+                  recomputed=None):
     """Device tensor (n2, n1) whose [i, j] is K(x1_j, x2_i) (the memory of
     the reference's Fortran (n1, n2) kernel)."""
     if kernel == "gaussian":
-        return gaussian_kernel_rows(X1, X2, sigma, precision=precision, out=out, out_dtype=out_dtype)
+        return gaussian_kernel_rows(X1, X2, sigma, precision=precision, out=out, out_dtype=out_dtype,
+                                    recomputed=recomputed)
     if kernel == "laplacian":
         return laplacian_kernel_rows(X1, X2, sigma, out=out, out_dtype=out_dtype)
     raise ValueError(kernel)

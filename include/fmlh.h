@@ -132,12 +132,19 @@ int fmlh_gaussian_kernel(const double *a, int64_t lda, int na,
 
 /* Precision forms on float descriptors.  mode: 0 = FP32 MFMA products with
  * FP32 accumulation; 1 = split-BF16x3 MFMA (hi*hi + hi*lo + lo*hi) with FP32
- * accumulation.  Both recompute near pairs by direct FP32 differences.
- * Output is float (k32) or double (k64); exactly one must be non-NULL. */
+ * accumulation, on hi/lo bf16 planes split once per operand into ws (no
+ * reference counterpart; fmlh_gaussian_kernel_f32_ws_bytes(na, nb, d, mode,
+ * same) bytes, same = 1 when a and b are the same rows; 0 bytes for mode 0).
+ * Both recompute near pairs (d2 <= tau (|a|^2 + |b|^2)) by direct FP32
+ * differences; for mode 1, recomputed (device counter, may be NULL) is
+ * incremented by their number.  Output is float (k32) or double (k64);
+ * exactly one must be non-NULL. */
+size_t fmlh_gaussian_kernel_f32_ws_bytes(int na, int nb, int d, int mode, int same);
 int fmlh_gaussian_kernel_f32(const float *a, int64_t lda, int na,
                              const float *b, int64_t ldb, int nb, int d,
                              double sigma, int mode, float *k32, double *k64,
-                             int64_t ldk, void *stream);
+                             int64_t ldk, void *ws, size_t ws_bytes,
+                             unsigned long long *recomputed, void *stream);
 
 /* ------------------------------------------------------------------------
  * (c) Laplacian kernel K(j,i) = exp(-1/sigma * sum_t |a_jt - b_it|)
