@@ -57,7 +57,9 @@ _ref = {}
 def lib():
     global _lib
     if _lib is None:
-        p = os.path.join(_HERE, "liboracle_fml.so")
+        # FML_ORACLE_LIB: an alternative build of the same source (the
+        # sanitizer build of tests/test_sanitize.py)
+        p = os.environ.get("FML_ORACLE_LIB") or os.path.join(_HERE, "liboracle_fml.so")
         if not os.path.exists(p):
             build()
         _lib = _load(p)
