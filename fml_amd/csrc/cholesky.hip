@@ -31,8 +31,6 @@
 #include <math.h>
 #include <stdio.h>
 
-#include <map>
-#include <memory>
 #include <mutex>
 
 #include "common.h"
@@ -251,11 +249,13 @@ int trailing_update(T *a, int64_t lda, int n, int r0, int kb, int ke, int32_t *i
   return 0;
 }
 
-// Auxiliary streams (non-blocking, default priority) with two events each,
-// created on first use, for the factorisation's look-ahead: one per device,
-// and one per device and CU mask for callers on a CU-masked stream made by
-// fmlh_stream_create, so the look-ahead's panel work stays on the CUs the
-// caller's stream may use.
+// One auxiliary stream (non-blocking, default priority) and two events per
+// device, created on first use, for the factorisation's look-ahead.  A
+// caller on a CU-masked stream made by fmlh_stream_create (the serving
+// loop's fit stream) gets no look-ahead: a stream of the library's own would
+// run the panel work outside the caller's CUs, and every further stream
+// takes one of the process's few hardware queues, which the serving loop's
+// prediction stream must not share.
 struct Aux {
   hipStream_t s = nullptr;
   hipEvent_t fork = nullptr, join = nullptr;
@@ -264,34 +264,29 @@ struct Aux {
 
 bool aux_for_stream(hipStream_t caller, Aux &out) {
   static std::mutex mu;
-  static std::map<std::pair<int, std::vector<uint32_t>>, Aux> table;
-  static std::map<std::pair<int, std::vector<uint32_t>>, std::unique_ptr<std::mutex>> locks;
+  static std::mutex enqueue_mu[64];
+  static Aux table[64];
+  std::vector<uint32_t> mask;
+  if (stream_cu_mask(caller, mask)) return false;
   int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess) {
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) {
     (void)hipGetLastError();
     return false;
   }
-  std::vector<uint32_t> mask;  // empty: every CU
-  const bool masked = stream_cu_mask(caller, mask);
-  const auto key = std::make_pair(dev, mask);
   std::lock_guard<std::mutex> lk(mu);
-  auto it = table.find(key);
-  if (it == table.end()) {
+  Aux &x = table[dev];
+  if (x.s == nullptr) {
     Aux y;
-    hipError_t e = masked ? hipExtStreamCreateWithCUMask(&y.s, (uint32_t)mask.size(), mask.data())
-                          : hipStreamCreateWithFlags(&y.s, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&y.fork, hipEventDisableTiming);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&y.join, hipEventDisableTiming);
-    if (e != hipSuccess) {
+    if (hipStreamCreateWithFlags(&y.s, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&y.fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&y.join, hipEventDisableTiming) != hipSuccess) {
       (void)hipGetLastError();
       return false;
     }
-    auto &m = locks[key];
-    m.reset(new std::mutex);
-    y.enqueue = m.get();
-    it = table.emplace(key, y).first;
+    y.enqueue = &enqueue_mu[dev];
+    x = y;
   }
-  out = it->second;
+  out = x;
   return true;
 }
 

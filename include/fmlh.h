@@ -59,9 +59,9 @@ int fmlh_init(void);
  * a stream handed to torch (ExternalStream) is recorded on by torch's
  * pinned-memory allocator whenever a pinned block used on it is freed, so
  * the serving loop keeps its streams for the life of the process.
- * Work an entry point forks onto a stream of its own (fmlh_potrf's panel
- * look-ahead) runs on a stream with the same CU mask when the caller's
- * stream was made here. */
+ * fmlh_potrf called on such a stream runs without its panel look-ahead
+ * (which would fork work onto a stream of the library's own, outside the
+ * mask and on a further hardware queue). */
 int fmlh_stream_create(int cu_count, void **stream);
 int fmlh_stream_destroy(void *stream);
 
