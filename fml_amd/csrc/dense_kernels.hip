@@ -6,7 +6,8 @@
 //   products on MFMA (128x128 or 64x64 tiles, see gemm_nt.h) and exp fused in the
 //   epilogue; an entry whose GEMM-form distance is within the cancellation
 //   band (d2 <= tau*(||a||^2+||b||^2)) is recomputed from direct differences
-//   by its wavefront (near_pairs.h).
+//   after the tile's other entries are stored (recompute_near_deferred,
+//   near_pairs.h).
 // Laplacian K(j,i) = exp(-1/sigma * sum_t |a_jt - b_it|)
 //   fml/kernels/fkernels.f90:351-377.  64x64 output tile per 256 threads,
 //   4x4 entries per thread, operands staged k-major in LDS; every entry's
@@ -32,6 +33,13 @@ __device__ __forceinline__ void kstore(O *p, double v) {
   __builtin_nontemporal_store((O)v, p);
 }
 
+// exp(inv d2), formed in double for double output
+template <typename T, typename OutT>
+__device__ __forceinline__ double gauss_value(T d2, double inv) {
+  if (sizeof(OutT) == 8) return exp(inv * (double)d2);
+  return (double)kexp((T)((T)inv * d2));
+}
+
 // Gaussian on MFMA, MT x MT output tiles (MT = 128 for large kernels, 64
 // for small ones).  A 1-D grid in XCD-contiguous tile order (gemm_nt.h):
 // gx tiles over A (na), gy over B (nb).  OutT is float or double; the
@@ -52,10 +60,17 @@ __global__ __launch_bounds__(MT == 128 ? 512 : 256, (MT == 128 ? 2 : 1)) void k_
   TileAcc<T, MT, WR> acc;
   tile_nt<T, true, MT, WR>(B + (int64_t)m0 * ldb, ldb, nb - m0, A + (int64_t)n0 * lda, lda, na - n0,
                            d, s, acc);
+  constexpr int NBM = TileAcc<T, MT, WR>::NBM, NBN = TileAcc<T, MT, WR>::NBN;
+  static_assert(NBM * NBN * 4 <= 64, "one ballot bit per epilogue step");
+  // entries in the cancellation band are stored after the tile's others, from
+  // direct differences: step b = 4 (NBN im + in) + r records its ballot
+  __shared__ unsigned long long s_near[2 * WR][NBM * NBN * 4];
+  unsigned long long *masks = s_near[threadIdx.x >> 6];
+  unsigned long long steps = 0;
 #pragma unroll
-  for (int im = 0; im < TileAcc<T, MT, WR>::NBM; ++im)
+  for (int im = 0; im < NBM; ++im)
 #pragma unroll
-    for (int in = 0; in < TileAcc<T, MT, WR>::NBN; ++in)
+    for (int in = 0; in < NBN; ++in)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int m = tile_row<T, MT, WR>(im, r);
@@ -65,17 +80,27 @@ __global__ __launch_bounds__(MT == 128 ? 512 : 256, (MT == 128 ? 2 : 1)) void k_
         const T pn = s.pn[m], qn = s.qn[n];
         T d2 = fma(T(-2), acc.v[im][in][r], pn + qn);
         const bool near = valid && d2 <= tau * (pn + qn);
-        d2 = recompute_near(near, d2, A, lda, gn, B, ldb, gm, d);
-        if (valid) {
+        const unsigned long long mk = __ballot(near);
+        if (mk) {
+          masks[4 * (NBN * im + in) + r] = mk;
+          steps |= 1ull << (4 * (NBN * im + in) + r);
+        }
+        if (valid && !near) {
           if (d2 < T(0)) d2 = T(0);
-          double v;
-          if (sizeof(OutT) == 8)
-            v = exp(inv * (double)d2);
-          else
-            v = (double)kexp((T)((T)inv * d2));
-          kstore(K + (int64_t)gn + (int64_t)gm * ldk, v);
+          kstore(K + (int64_t)gn + (int64_t)gm * ldk, gauss_value<T, OutT>(d2, inv));
         }
       }
+  if (steps) {
+    auto idx = [&](int b, int L, int &gn, int &gm) {
+      const int im = b / (4 * NBN), in = (b >> 2) % NBN, r = b & 3, q = L >> 4;
+      gm = m0 + tile_wm() * (MT / WR) + im * 16 + (sizeof(T) == 8 ? q + 4 * r : 4 * q + r);
+      gn = n0 + tile_wn() * (MT / 2) + in * 16 + (L & 15);
+    };
+    auto put = [&](int gn, int gm, T d2) {
+      kstore(K + (int64_t)gn + (int64_t)gm * ldk, gauss_value<T, OutT>(d2, inv));
+    };
+    recompute_near_deferred<T>(steps, masks, idx, A, lda, B, ldb, d, put);
+  }
 }
 
 // Tile size for an na x nb kernel: the 128 tile once both sides span a few

@@ -11,8 +11,10 @@
 // accumulated in FP32.  The dropped lo.lo term and the bf16 rounding of lo
 // leave ~2^-16 relative error per product, so |a|^2 + |b|^2 - 2 a.b loses
 // accuracy when d2 is small against the norms: entries with d2 <= tau
-// (|a|^2 + |b|^2) are recomputed from direct FP32 differences by their
-// wavefront (near_pairs.h), and counted when the caller asks.
+// (|a|^2 + |b|^2) are recomputed from direct FP32 differences after the
+// tile's other entries are stored, four at a time per wavefront
+// (recompute_near_deferred, near_pairs.h), and counted when the caller asks.
+// FP32 outputs are exp2(d2 * (log2(e) inv)) with the scale rounded once.
 //
 // Tile: 128 x 128 outputs per 256 threads, waves 2 x 2, each wave 64 x 64
 // from 4 x 4 16x16 blocks (48 MFMAs per k step).  A k step stages 128 rows x
@@ -93,6 +95,14 @@ struct Plane {
   const float *nrm;
 };
 
+// Tile row held by LDS row s of the A operand.  LDS row blk*64 + 16 i + c
+// (the column c of MFMA block i) holds tile row blk*64 + 4 c + i, so the lane
+// that owns output column c of the four blocks i = 0..3 owns four
+// consecutive columns of K and stores them with one vector store.
+__device__ __forceinline__ int a_row(int s) {
+  return (s & ~63) | ((s & 15) << 2) | ((s >> 4) & 3);
+}
+
 struct B3Smem {
   uint16_t ah[kMT * kRow];
   uint16_t al[kMT * kRow];
@@ -100,15 +110,23 @@ struct B3Smem {
   uint16_t bl[kMT * kRow];
 };
 
+template <typename OutT>
+__device__ __forceinline__ OutT gauss_value(float d2, double inv, float inv_log2) {
+  if (sizeof(OutT) == 8) return (OutT)exp(inv * (double)d2);
+  return (OutT)exp2f(inv_log2 * d2);
+}
+
 // K[gm][gn] for B rows gm (output rows) and A rows gn (output columns).
 template <typename OutT>
 __global__ __launch_bounds__(256, 2) void k_gauss_b3(Plane pa, int na, Plane pb, int nb, int dp,
                                                      const float *__restrict__ A, int64_t lda,
                                                      const float *__restrict__ B, int64_t ldb,
-                                                     int d, double inv, OutT *__restrict__ K,
+                                                     int d, double inv, float inv_log2,
+                                                     OutT *__restrict__ K,
                                                      int64_t ldk, unsigned long long *nrecomp,
                                                      int gx, int gy) {
   __shared__ __attribute__((aligned(16))) B3Smem s;
+  __shared__ unsigned long long s_near[4][64];
   int tx, ty;
   if (!xcd_tile(gx, gy, tx, ty)) return;
   const int n0 = tx * kMT;  // A rows (output columns)
@@ -123,7 +141,7 @@ __global__ __launch_bounds__(256, 2) void k_gauss_b3(Plane pa, int na, Plane pb,
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       const int r = srow + 64 * h;
-      const int ga = n0 + r, gb = m0 + r;
+      const int ga = n0 + a_row(r), gb = m0 + r;
       const int64_t oa = (int64_t)min(ga, na - 1) * dp + k0 + scol;
       const int64_t ob = (int64_t)min(gb, nb - 1) * dp + k0 + scol;
       rah[h] = *reinterpret_cast<const u4v *>(pa.hi + oa);
@@ -178,35 +196,72 @@ __global__ __launch_bounds__(256, 2) void k_gauss_b3(Plane pa, int na, Plane pb,
       }
     }
   }
-  // epilogue: d2 = |b|^2 + |a|^2 - 2 a.b, near pairs recomputed, exp
+  // epilogue: d2 = |b|^2 + |a|^2 - 2 a.b, near pairs recomputed, exp.
+  // This lane's output columns are gn0 + i (i = 0..3, from block i) and its
+  // rows m0 + wm*64 + 16 j + 4 (lane / 16) + r.
+  const int gn0 = n0 + wn * 64 + 4 * (lane & 15);
+  float qn[4], pn[4][4];
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int gn = n0 + wn * 64 + i * 16 + (lane & 15);
-    const float qn = gn < na ? pa.nrm[gn] : 0.f;
+  for (int i = 0; i < 4; ++i) qn[i] = gn0 + i < na ? pa.nrm[gn0 + i] : 0.f;
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+  for (int j = 0; j < 4; ++j)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int gm = m0 + wm * 64 + j * 16 + 4 * (lane >> 4) + r;
+    for (int r = 0; r < 4; ++r) {
+      const int gm = m0 + wm * 64 + j * 16 + 4 * (lane >> 4) + r;
+      pn[j][r] = gm < nb ? pb.nrm[gm] : 0.f;
+    }
+  // whole tile inside K and 4-element stores aligned: one vector store per row
+  const bool vec = n0 + kMT <= na && m0 + kMT <= nb && (ldk & 3) == 0 &&
+                   (reinterpret_cast<uintptr_t>(K) & (4 * sizeof(OutT) - 1)) == 0;
+  // entries in the cancellation band are stored after the tile, from direct
+  // differences: step b = 16 j + 4 r + i records its ballot in masks[b]
+  unsigned long long *masks = s_near[wave];
+  unsigned long long steps = 0;
+  unsigned int near_count = 0;  // this wave's recomputed entries
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int gm = m0 + wm * 64 + j * 16 + 4 * (lane >> 4) + r;
+      OutT v[4];
+      bool nr[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int gn = gn0 + i;
         const bool valid = gm < nb && gn < na;
-        const float pn = valid ? pb.nrm[gm] : 0.f;
-        float d2 = fmaf(-2.f, acc[i][j][r], pn + qn);
-        const bool near = valid && d2 <= kTau * (pn + qn);
-        if (nrecomp) {
-          const unsigned long long m = __ballot(near);
-          if (lane == 0 && m) atomicAdd(nrecomp, (unsigned long long)__popcll(m));
+        const float nsum = pn[j][r] + qn[i];
+        float d2 = fmaf(-2.f, acc[i][j][r], nsum);
+        nr[i] = valid && d2 <= kTau * nsum;
+        const unsigned long long mk = __ballot(nr[i]);
+        if (mk) {
+          masks[16 * j + 4 * r + i] = mk;
+          steps |= 1ull << (16 * j + 4 * r + i);
+          near_count += (unsigned int)__popcll(mk);
         }
-        d2 = recompute_near(near, d2, A, lda, gn, B, ldb, gm, d);
-        if (!valid) continue;
         if (d2 < 0.f) d2 = 0.f;
-        OutT v;
-        if (sizeof(OutT) == 8)
-          v = (OutT)exp(inv * (double)d2);
-        else
-          v = (OutT)expf((float)inv * d2);
-        __builtin_nontemporal_store(v, K + (int64_t)gn + (int64_t)gm * ldk);
+        v[i] = gauss_value<OutT>(d2, inv, inv_log2);
       }
+      OutT *row = K + (int64_t)gm * ldk + gn0;
+      if (vec && !(nr[0] || nr[1] || nr[2] || nr[3])) {
+        typedef OutT o4v __attribute__((ext_vector_type(4)));
+        __builtin_nontemporal_store(o4v{v[0], v[1], v[2], v[3]}, reinterpret_cast<o4v *>(row));
+      } else if (gm < nb) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+          if (gn0 + i < na && !nr[i]) __builtin_nontemporal_store(v[i], row + i);
+      }
+    }
+  if (steps) {
+    auto idx = [&](int b, int L, int &gn, int &gm) {
+      gn = n0 + wn * 64 + 4 * (L & 15) + (b & 3);
+      gm = m0 + wm * 64 + 16 * (b >> 4) + 4 * (L >> 4) + ((b >> 2) & 3);
+    };
+    auto put = [&](int gn, int gm, float d2) {
+      __builtin_nontemporal_store(gauss_value<OutT>(d2, inv, inv_log2), K + (int64_t)gm * ldk + gn);
+    };
+    recompute_near_deferred<float>(steps, masks, idx, A, lda, B, ldb, d, put);
   }
+  if (nrecomp && lane == 0 && near_count) atomicAdd(nrecomp, (unsigned long long)near_count);
 }
 
 size_t plane_bytes(int n, int d) {
@@ -247,6 +302,8 @@ int gaussian_bf16x3(const float *a, int64_t lda, int na, const float *b, int64_t
                     int d, double sigma, float *k32, double *k64, int64_t ldk, void *ws,
                     size_t ws_bytes, unsigned long long *recomputed, hipStream_t st) {
   const double inv = -0.5 / (sigma * sigma);
+  // exp(inv d2) = exp2(inv log2(e) d2) for the FP32 outputs
+  const float inv_log2 = (float)(inv * 1.4426950408889634);
   WsCarver w(ws, ws_bytes);
   Plane pa, pb;
   int rc = split(w, a, lda, na, d, pa, st);
@@ -263,10 +320,10 @@ int gaussian_bf16x3(const float *a, int64_t lda, int na, const float *b, int64_t
   timing_begin(st);
   if (k32)
     hipLaunchKernelGGL(k_gauss_b3<float>, dim3(grid), dim3(256), 0, st, pa, na, pb, nb, dp, a, lda,
-                       b, ldb, d, inv, k32, ldk, recomputed, gx, gy);
+                       b, ldb, d, inv, inv_log2, k32, ldk, recomputed, gx, gy);
   else
     hipLaunchKernelGGL(k_gauss_b3<double>, dim3(grid), dim3(256), 0, st, pa, na, pb, nb, dp, a, lda,
-                       b, ldb, d, inv, k64, ldk, recomputed, gx, gy);
+                       b, ldb, d, inv, inv_log2, k64, ldk, recomputed, gx, gy);
   timing_end(st);
   FMLH_LAUNCH_CHECK("k_gauss_b3");
   return 0;
