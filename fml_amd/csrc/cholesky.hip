@@ -808,13 +808,24 @@ __global__ __launch_bounds__(256) void k_trsv_chain(const T *__restrict__ a, int
       step(j + 1, cb);
     }
     if (j < b) step(j, ca);
+    // the 16 row sums over the wave's lanes by a transposing butterfly: at
+    // exchange distance 32, 16, 8, 4 a lane keeps half of its partial rows
+    // and adds its partner's copy of them (8 + 4 + 2 + 1 exchanges), then
+    // two plain exchanges; lane l ends with row (l / 4) % 16
+    const bool up32 = lane & 32, up16 = lane & 16, up8 = lane & 8, up4 = lane & 4;
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
-      T v = acc[i];
+    for (int k = 0; k < 8; ++k)
+      acc[k] = (up32 ? acc[k + 8] : acc[k]) + __shfl_xor(up32 ? acc[k] : acc[k + 8], 32, 64);
 #pragma unroll
-      for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-      if (lane == 0) part[0][wave * 16 + i] = v;
-    }
+    for (int k = 0; k < 4; ++k)
+      acc[k] = (up16 ? acc[k + 4] : acc[k]) + __shfl_xor(up16 ? acc[k] : acc[k + 4], 16, 64);
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+      acc[k] = (up8 ? acc[k + 2] : acc[k]) + __shfl_xor(up8 ? acc[k] : acc[k + 2], 8, 64);
+    T rsum = (up4 ? acc[1] : acc[0]) + __shfl_xor(up4 ? acc[0] : acc[1], 4, 64);
+    rsum += __shfl_xor(rsum, 2, 64);
+    rsum += __shfl_xor(rsum, 1, 64);
+    if ((lane & 3) == 0) part[0][wave * 16 + ((lane >> 2) & 15)] = rsum;
     __syncthreads();
     if (tid < bs) rhs[tid] = xown - part[0][tid];
   } else {
@@ -857,7 +868,37 @@ __global__ __launch_bounds__(256) void k_trsv_chain(const T *__restrict__ a, int
     // the block's substitution in one wave: lane r holds row r; each pivot
     // value is broadcast with readlane (a register read, no LDS round trip)
     T v = lane < bs ? rhs[lane] : T(0);
-    if (kForward) {
+    if (bs == kNB) {
+      // full block: fully unrolled, the factor entries this lane needs
+      // (forward L[lane][c], backward L[c][lane]) read from LDS 16 steps
+      // ahead, so a step's chain is readlane, multiply, fma; lane c scales
+      // its own value by its own reciprocal pivot.  Entries outside the
+      // triangle are zero in L, so a lane the step does not update keeps
+      // its value; the select keeps it exact for any x_c.
+      const T myr = rinv[lane];
+      T cur[16], nxt[16];
+      auto fetch = [&](int q, T *dst) {
+#pragma unroll
+        for (int k = 0; k < 16; ++k)
+          dst[k] = kForward ? L[lane][16 * q + k] : L[kNB - 1 - 16 * q - k][lane];
+      };
+      fetch(0, cur);
+#pragma unroll
+      for (int q = 0; q < kNB / 16; ++q) {
+        if (q + 1 < kNB / 16) fetch(q + 1, nxt);
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+          const int c = kForward ? 16 * q + k : kNB - 1 - 16 * q - k;
+          const T xc = lane_value(v * myr, c);
+          const T upd = fma(-cur[k], xc, v);
+          v = lane == c ? xc : ((kForward ? lane > c : lane < c) ? upd : v);
+        }
+        if (q + 1 < kNB / 16) {
+#pragma unroll
+          for (int k = 0; k < 16; ++k) cur[k] = nxt[k];
+        }
+      }
+    } else if (kForward) {
       T lnext = L[lane][0];
       for (int c = 0; c < bs; ++c) {
         const T lc = lnext;
