@@ -93,9 +93,51 @@ struct Rec {  // one neighbour: distance, sine term, two angular channels, compa
   int32_t ty, pad;
 };
 
+// Records in memory (workspace and LDS stages) as two arrays: the four
+// doubles {d, s, a, b} (32 bytes, read as two 16-byte loads) and the type id,
+// 36 bytes a record, so that the stage of a tile with one 29-atom set-2
+// molecule (841 records) leaves room for four workgroups in a CU's LDS.
+struct alignas(16) Rec4 {
+  double d, s, a, b;
+};
+struct RecArr {
+  Rec4 *r;
+  int32_t *ty;
+  __device__ __forceinline__ Rec at(int64_t m) const {
+    const Rec4 v = r[m];
+    return Rec{v.d, v.s, v.a, v.b, ty[m], 0};
+  }
+  __device__ __forceinline__ double d(int64_t m) const { return r[m].d; }
+  __device__ __forceinline__ void put(int64_t m, const Rec &x) const {
+    r[m] = Rec4{x.d, x.s, x.a, x.b};
+    ty[m] = x.ty;
+  }
+  __device__ __forceinline__ void copy(int64_t m, const RecArr &src, int64_t o) const {
+    r[m] = src.r[o];
+    ty[m] = src.ty[o];
+  }
+  __host__ __device__ __forceinline__ RecArr offset(int64_t o) const {
+    return RecArr{r + o, ty + o};
+  }
+};
+
+// bytes of n records as carved from LDS (each array 16-byte aligned)
+__host__ __device__ constexpr size_t rec_bytes(size_t n) {
+  return 32 * n + ((4 * n + 15) & ~size_t(15));
+}
+
+// carves n records from an LDS pointer
+__device__ __forceinline__ RecArr carve_recs(char *&p, size_t n) {
+  RecArr a;
+  a.r = (Rec4 *)p;
+  a.ty = (int32_t *)(p + 32 * n);
+  p += rec_bytes(n);
+  return a;
+}
+
 // Per-set prepared neighbour data (workspace).
 struct NbSet {
-  Rec *rec;         // [slot*amax + m]
+  RecArr rec;       // [slot*amax + m]
   double2 *rc;      // [slot*amax + m] raw (row, col) type channels
   int32_t *cnt;     // [slot]
   int32_t *asc;     // [slot] 1 when d[0..cnt) is ascending
@@ -160,7 +202,7 @@ __global__ __launch_bounds__(256) void k_arad_prep(const double *__restrict__ q,
       // global kernel: sine only where d < cut (farad_kernels.f90:526,539)
       const double sv = (!sin_guard || d < p.cut) ? 1.0 - sin_libm(d * p.inv_cut, c_sincos) : 0.0;
       const int64_t o = (int64_t)slot * amax + m;
-      set.rec[o] = Rec{d, sv, x[3 * amax + m], x[4 * amax + m], kTypeNone, 0};
+      set.rec.put(o, Rec{d, sv, x[3 * amax + m], x[4 * amax + m], kTypeNone, 0});
       set.rc[o] = make_double2(rowv, colv);
       if (small_int(rowv, kTypeRows) && small_int(colv, kTypeCols)) {
         // a handful of types cover every neighbour: set each bit once
@@ -242,14 +284,22 @@ __global__ __launch_bounds__(256) void k_arad_type_ids(NbSet set, int nslots, in
       if (c >= 0) id = c;
     }
   }
-  if (m < set.cnt[slot]) set.rec[o].ty = id;
+  if (m < set.cnt[slot]) set.rec.ty[o] = id;
 }
 
 // ---- the overlap ------------------------------------------------------
 
+// One atom's neighbour list: the arrays' bases (the same for every lane of
+// a launch or workgroup, so they stay in scalar registers) and the list's
+// first index.
 struct NbView {
-  const Rec *rec;
+  RecArr rec;
   const double2 *rc;  // raw channels, generic path only
+  int off;            // first record
+  int roff;           // first raw-channel pair
+  __device__ __forceinline__ Rec at(int m) const { return rec.at(off + m); }
+  __device__ __forceinline__ double d(int m) const { return rec.d(off + m); }
+  __device__ __forceinline__ double2 raw(int m) const { return rc[roff + m]; }
 };
 
 template <typename TC>
@@ -289,7 +339,7 @@ __device__ TC arad_distl2(const NbView &x, int cnt1, bool asc1, const NbView &y,
     v = v * (TC)yr.s;
     TC t;
     if (kGeneric) {
-      const double2 rc2 = y.rc[m2];
+      const double2 rc2 = y.raw(m2);
       t = (TC)stoch(rc1.x - rc2.x, rc1.y - rc2.y, rw2, cw2);
     } else {
       t = ttab[trow + yr.ty];
@@ -300,53 +350,52 @@ __device__ TC arad_distl2(const NbView &x, int cnt1, bool asc1, const NbView &y,
     return v * ang;
   };
   auto term = [&](int m2, TC d1, TC s1, TC a1, TC b1, int trow, double2 rc1) {
-    acc = acc + rec_value(y.rec[m2], m2, d1, s1, a1, b1, trow, rc1);
+    acc = acc + rec_value(y.at(m2), m2, d1, s1, a1, b1, trow, rc1);
   };
   if (asc1 && asc2) {
     int lo = 0, hi = 0;
     for (int m1 = 0; m1 < cnt1; ++m1) {
-      const Rec xr = x.rec[m1];
+      const Rec xr = x.at(m1);
       const TC d1 = (TC)xr.d;
       while (lo < cnt2) {
-        const TC d2 = (TC)y.rec[lo].d;
+        const TC d2 = (TC)y.d(lo);
         const TC dr = d2 - d1;
         if (dr * dr < maxg || d2 > d1) break;
         ++lo;
       }
       if (hi < lo) hi = lo;
       while (hi < cnt2) {
-        const TC dr = (TC)y.rec[hi].d - d1;
+        const TC dr = (TC)y.d(hi) - d1;
         if (!(dr * dr < maxg)) break;
         ++hi;
       }
       if (lo < hi) {
         const int trow = kGeneric ? 0 : xr.ty * kMaxTypes;
-        const double2 rc1 = kGeneric ? x.rc[m1] : make_double2(0.0, 0.0);
+        const double2 rc1 = kGeneric ? x.raw(m1) : make_double2(0.0, 0.0);
         const TC s1 = (TC)xr.s, a1 = (TC)xr.a, b1 = (TC)xr.b;
         // two terms per trip, added to acc one after the other in m2 order;
         // their records are read at the top of the trip (the other waves of
         // the SIMD cover the LDS latency)
-        const Rec *yp = y.rec + lo;
         int m2 = lo;
-        for (int k = (hi - lo) >> 1; k > 0; --k, m2 += 2, yp += 2) {
-          const Rec r0 = yp[0], r1 = yp[1];
+        for (int k = (hi - lo) >> 1; k > 0; --k, m2 += 2) {
+          const Rec r0 = y.at(m2), r1 = y.at(m2 + 1);
           const TC v0 = rec_value(r0, m2, d1, s1, a1, b1, trow, rc1);
           const TC v1 = rec_value(r1, m2 + 1, d1, s1, a1, b1, trow, rc1);
           acc = acc + v0;
           acc = acc + v1;
         }
-        if ((hi - lo) & 1) acc = acc + rec_value(yp[0], m2, d1, s1, a1, b1, trow, rc1);
+        if ((hi - lo) & 1) acc = acc + rec_value(y.at(m2), m2, d1, s1, a1, b1, trow, rc1);
       }
     }
   } else {
     for (int m1 = 0; m1 < cnt1; ++m1) {
-      const Rec xr = x.rec[m1];
+      const Rec xr = x.at(m1);
       const TC d1 = (TC)xr.d;
       const int trow = kGeneric ? 0 : xr.ty * kMaxTypes;
-      const double2 rc1 = kGeneric ? x.rc[m1] : make_double2(0.0, 0.0);
+      const double2 rc1 = kGeneric ? x.raw(m1) : make_double2(0.0, 0.0);
       const TC s1 = (TC)xr.s, a1 = (TC)xr.a, b1 = (TC)xr.b;
       for (int m2 = 0; m2 < cnt2; ++m2) {
-        const TC dr = (TC)y.rec[m2].d - d1;
+        const TC dr = (TC)y.d(m2) - d1;
         if (dr * dr < maxg) term(m2, d1, s1, a1, b1, trow, rc1);
       }
     }
@@ -355,8 +404,7 @@ __device__ TC arad_distl2(const NbView &x, int cnt1, bool asc1, const NbView &y,
 }
 
 __device__ __forceinline__ NbView global_view(const NbSet &s, int64_t slot, int amax) {
-  const int64_t o = slot * amax;
-  return NbView{s.rec + o, s.rc + o};
+  return NbView{s.rec, s.rc, (int)(slot * amax), (int)(slot * amax)};
 }
 
 // LDS copies of the type-pair table and of exp_neg's table.
@@ -400,7 +448,7 @@ __global__ __launch_bounds__(256) void k_arad_self(NbSet set, int nslots, int am
 
 // LDS image of one molecule.
 struct MolStage {
-  Rec *rec;
+  RecArr rec;
   int *cnt, *asc;
   double *self, *zr, *zc;
 };
@@ -408,11 +456,8 @@ struct MolStage {
 __device__ __forceinline__ MolStage carve_stage(char *&p, int amax, bool records) {
   MolStage m;
   const int nn = amax * amax;
-  m.rec = nullptr;
-  if (records) {
-    m.rec = (Rec *)p;
-    p += sizeof(Rec) * nn;
-  }
+  m.rec = RecArr{nullptr, nullptr};
+  if (records) m.rec = carve_recs(p, nn);
   m.self = (double *)p;
   p += sizeof(double) * amax;
   m.zr = (double *)p;
@@ -429,7 +474,7 @@ __device__ __forceinline__ MolStage carve_stage(char *&p, int amax, bool records
 size_t stage_bytes(int amax, bool records) {
   const size_t nn = (size_t)amax * amax;
   size_t b = (sizeof(double) * 3 + sizeof(int) * 2) * (size_t)amax;
-  if (records) b += sizeof(Rec) * nn;
+  if (records) b += rec_bytes(nn);
   return b;
 }
 
@@ -448,7 +493,7 @@ __device__ void stage_molecule(MolStage &m, const NbSet &set, int mol, int natom
   const int total = natoms * amax;
   for (int t = threadIdx.x; t < total; t += blockDim.x) {
     const int64_t o = slot0 * amax + t;
-    m.rec[t] = set.rec[o];
+    m.rec.copy(t, set.rec, o);
   }
 }
 
@@ -535,11 +580,11 @@ __global__ __launch_bounds__(kPairThreads) void k_arad_pairs(NbSet set1, NbSet s
           const int64_t gx = ((int64_t)i * amax + a) * amax, gy = ((int64_t)j * amax + b) * amax;
           NbView x, y;
           if (kStage) {
-            x = NbView{mi.rec + a * amax, set1.rc + gx};
-            y = NbView{mj.rec + b * amax, set2.rc + gy};
+            x = NbView{mi.rec, set1.rc, a * amax, (int)gx};
+            y = NbView{mj.rec, set2.rc, b * amax, (int)gy};
           } else {
-            x = NbView{set1.rec + gx, set1.rc + gx};
-            y = NbView{set2.rec + gy, set2.rc + gy};
+            x = NbView{set1.rec, set1.rc, (int)gx, (int)gx};
+            y = NbView{set2.rec, set2.rc, (int)gy, (int)gy};
           }
           const TC S = generic ? arad_distl2<TC, true>(x, mi.cnt[a], mi.asc[a] != 0, y, mj.cnt[b],
                                                        mj.asc[b] != 0, inv_w, maxg, ttab, etab,
@@ -790,7 +835,7 @@ __global__ __launch_bounds__(64) void k_plan_emit(const int32_t *__restrict__ n1
 }
 
 template <bool kF32Self, bool kRecs>
-__device__ void stage_compact(Rec *rec, int *cnt, int *asc, double *self,
+__device__ void stage_compact(const RecArr &rec, int *cnt, int *asc, double *self,
                               double *zr, double *zc, const NbSet &set, int mol, int natoms,
                               int amax, int atom0) {
   const int64_t slot0 = (int64_t)mol * amax;
@@ -807,7 +852,7 @@ __device__ void stage_compact(Rec *rec, int *cnt, int *asc, double *self,
   for (int t = threadIdx.x; t < total; t += blockDim.x) {
     const int a = t / natoms, m = t % natoms;
     const int64_t o = (slot0 + a) * amax + m;
-    rec[(atom0 + a) * natoms + m] = set.rec[o];
+    rec.copy((atom0 + a) * natoms + m, set.rec, o);
   }
 }
 
@@ -841,8 +886,7 @@ __global__ __launch_bounds__(kPairThreads, kWaves) void k_arad_tiles(NbSet set1,
   ptr += sizeof(int) * ta.amax;
   // J-molecule stage (reused as the reduction buffer after compute)
   char *jbase = ptr;
-  Rec *recj = (Rec *)ptr;
-  ptr += sizeof(Rec) * ta.stage_j_recs;
+  const RecArr recj = carve_recs(ptr, ta.stage_j_recs);
   double *selfj = (double *)ptr;
   ptr += sizeof(double) * ta.stage_j_atoms;
   double *zrj = (double *)ptr;
@@ -871,7 +915,7 @@ __global__ __launch_bounds__(kPairThreads, kWaves) void k_arad_tiles(NbSet set1,
     const int i = tl.i, J = tl.cnt, s = tl.s;
     const int ni = min(max(ta.n1[i], 0), amax);
     if (i != cur_i) {
-      stage_compact<sizeof(TC) == 4, false>(nullptr, cnti, asci, selfi, zri, zci, set1, i,
+      stage_compact<sizeof(TC) == 4, false>(RecArr{nullptr, nullptr}, cnti, asci, selfi, zri, zci, set1, i,
                                             ni, amax, 0);
       cur_i = i;
     }
@@ -910,8 +954,8 @@ __global__ __launch_bounds__(kPairThreads, kWaves) void k_arad_tiles(NbSet set1,
       const int bj = jj * s + b;  // staged atom index of b
       const int j = ta.members[tl.off + jj];
       const int64_t xs = ((int64_t)i * amax + a) * amax;
-      NbView x{set1.rec + xs, set1.rc + xs};
-      NbView y{recj + bj * s, set2.rc + ((int64_t)j * amax + b) * amax};
+      NbView x{set1.rec, set1.rc, (int)xs, (int)xs};
+      NbView y{recj, set2.rc, bj * s, (j * amax + b) * amax};
       const TC S = arad_distl2<TC, kGeneric>(x, cnti[a], asci[a] != 0, y, cntj[bj],
                                                    ascj[bj] != 0, inv_w, maxg, ttab, etab, p.rw2,
                                                    p.cw2);
@@ -1001,7 +1045,7 @@ __global__ __launch_bounds__(256) void k_arad_atomic(NbSet set1, NbSet set2, int
 size_t set_bytes(int64_t nslots, int amax) {
   const size_t nn = (size_t)nslots * amax;
   size_t b = 0;
-  b += WsCarver::need(sizeof(Rec) * nn);
+  b += WsCarver::need(sizeof(Rec4) * nn) + WsCarver::need(sizeof(int32_t) * nn);
   b += WsCarver::need(sizeof(double2) * nn);
   b += WsCarver::need(sizeof(int32_t) * nslots) * 2;
   b += WsCarver::need(sizeof(double) * nslots) * 3;
@@ -1011,7 +1055,8 @@ size_t set_bytes(int64_t nslots, int amax) {
 
 bool carve_set(WsCarver &w, int64_t nslots, int amax, NbSet &s) {
   const size_t nn = (size_t)nslots * amax;
-  s.rec = w.take<Rec>(nn);
+  s.rec.r = w.take<Rec4>(nn);
+  s.rec.ty = w.take<int32_t>(nn);
   s.rc = w.take<double2>(nn);
   s.cnt = w.take<int32_t>(nslots);
   s.asc = w.take<int32_t>(nslots);
@@ -1019,7 +1064,7 @@ bool carve_set(WsCarver &w, int64_t nslots, int amax, NbSet &s) {
   s.zr = w.take<double>(nslots);
   s.zc = w.take<double>(nslots);
   s.self32 = w.take<float>(nslots);
-  return s.rec && s.rc && s.cnt && s.asc && s.self && s.zr && s.zc && s.self32;
+  return s.rec.r && s.rec.ty && s.rc && s.cnt && s.asc && s.self && s.zr && s.zc && s.self32;
 }
 
 AradParams make_params(double width, double cut, double rw, double cw, bool eps_1e11) {
@@ -1088,7 +1133,7 @@ TileLds tile_lds(int amax, int nsig, int lanes) {
   const size_t red = sizeof(double) * (size_t)pairs * std::max(nsig, 1);
   int jr = j_recs;
   auto stage = [&](int r) {
-    return sizeof(Rec) * (size_t)r + 32 * (size_t)L.j_atoms;
+    return rec_bytes((size_t)r) + 32 * (size_t)L.j_atoms;
   };
   while (stage(jr) < red) jr += 64;  // grow the record area until the stage covers the buffer
   L.j_recs = jr;
