@@ -53,7 +53,7 @@ ARAD_OPS_PER_EVAL = 25
 # set-1 atom fastest), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of
 # this script, each scaled by the counter's reading for a device copy of known
 # size (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0): profiles/hbm_traffic_counters.md.
-ARAD_TRAFFIC_BYTES = {"arad500": 2.0 * 550153.2 * 1024 + 17096.3 * 1024}
+ARAD_TRAFFIC_BYTES = {"arad500": 2.0 * 457764.4 * 1024 + 18797.98 * 1024}
 
 
 def parse():
