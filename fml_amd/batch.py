@@ -17,6 +17,8 @@ values of the same shape are copied in with ``set_*`` between runs.
                     farad_kernels.f90:95-755) over size-sorted molecule-pair
                     tiles planned on the device; ``autotune()`` picks its
                     tile shape by timing candidates on the device.
+* ``KRRCall``       plain FP64 KRR fit + predict of a molecule batch
+                    (Coulomb matrices, kernels, cho_solve, K^T alpha).
 """
 from __future__ import annotations
 
@@ -186,3 +188,68 @@ class AradKernelCall:
         if self.mode == _native.ARAD_KERNEL:
             return self.out.t()
         return self.out.permute(2, 1, 0)
+
+
+class KRRCall:
+    """Plain FP64 kernel-ridge regression of a fixed molecule batch, fit and
+    predict, on resident buffers: the reference's chain (README.md:33-69)
+    sorted Coulomb matrices (frepresentations.f90:23-110) -> Gaussian kernel
+    of the training rows (fkernels.f90:317-349) -> K + lambda I ->
+    cho_solve (fcho_solve.f90:23-47) -> Gaussian kernel of the test rows
+    against the training rows -> K_test^T alpha.
+
+    The first ``ntrain`` molecules train, the rest are predicted.  Every
+    launch reads and writes buffers the object owns (``alpha``, ``pred``,
+    ``info`` are the results), so ``run()`` can be captured into a
+    ``fml_amd.graphs.Graph`` and replayed; ``set_coords`` / ``set_targets``
+    change the inputs between runs.  ``info`` holds the factorisation's
+    LAPACK info after a run (0: K + lambda I positive definite)."""
+
+    def __init__(self, charges, coords, offsets, ntrain, y, sigma, lam, nmax=23):
+        self.cm = CoulombCall(charges, coords, offsets, nmax=nmax)
+        nmol = self.cm.nmol
+        if not 0 < int(ntrain) <= nmol:
+            raise ValueError(f"ntrain must be in 1..{nmol}")
+        self.ntrain = n = int(ntrain)
+        self.ntest = nmol - n
+        self.sigma = float(sigma)
+        self.lam = float(lam)
+        t = _native.torch_mod()
+        self.y = to_device(np.asarray(y, dtype=np.float64)[:n], np.float64)
+        if int(self.y.shape[0]) != n:
+            raise ValueError("y needs one target per training molecule")
+        self.K = empty((n, n), np.float64)
+        self.Kt = empty((max(self.ntest, 1), n), np.float64)
+        self.alpha = empty((n,), np.float64)
+        self.pred = empty((max(self.ntest, 1),), np.float64)
+        self.info = t.zeros(1, dtype=t.int32, device="cuda")
+        self.ws_bytes = int(_native.load().fmlh_potrs_ws_bytes(n, 1))
+        self.ws = workspace(self.ws_bytes)
+
+    def set_coords(self, coords):
+        self.cm.set_coords(coords)
+
+    def set_targets(self, y):
+        self.y.copy_(to_device(np.asarray(y, dtype=np.float64)[:self.ntrain], np.float64))
+
+    def run(self):
+        from .krr import add_diagonal, predict_from_kernel
+        n, st = self.ntrain, _native.stream_ptr()
+        X = self.cm.cm
+        d = int(X.shape[1])
+        self.cm.run()
+        _native.call("fmlh_gaussian_kernel", _native.ptr(X), X.stride(0), n, _native.ptr(X),
+                     X.stride(0), n, d, self.sigma, _native.ptr(self.K), self.K.stride(0), st)
+        add_diagonal(self.K, self.lam)
+        _native.call("fmlh_potrf", _native.ptr(self.K), self.K.stride(0), n, _native.ptr(self.info),
+                     None, 0, st)
+        # alpha = y by an elementwise kernel (a graph then holds no memory-copy node)
+        _native.torch_mod().mul(self.y, 1.0, out=self.alpha)
+        _native.call("fmlh_potrs", _native.ptr(self.K), self.K.stride(0), n, _native.ptr(self.alpha),
+                     n, 1, _native.ptr(self.ws), self.ws_bytes, st)
+        if self.ntest:
+            Xt = X[n:]
+            _native.call("fmlh_gaussian_kernel", _native.ptr(X), X.stride(0), n, _native.ptr(Xt),
+                         Xt.stride(0), self.ntest, d, self.sigma, _native.ptr(self.Kt),
+                         self.Kt.stride(0), st)
+            predict_from_kernel(self.Kt[:self.ntest], self.alpha, out=self.pred)

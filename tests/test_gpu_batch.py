@@ -141,3 +141,59 @@ def test_graph_replays_coulomb_and_arad(gpu, uneven):
     ko = oracle.kernels_arad(X1b, X2, z1, z2, n1, n2, [1.0, 4.0], 0.2, 5.0)
     assert _rel(ac.result().cpu().numpy(), ko) < 1e-12
     g.destroy()
+
+
+def test_krr_call_graph_replays(gpu):
+    """The prepared FP64 KRR chain (Coulomb matrices -> kernel -> K + lambda I
+    -> cho_solve -> test kernel -> predictions): eager run bitwise equal to
+    the KRR class on the same molecules, predictions within 1e-6 of the
+    LAPACK solution's, and HIP-graph replays (with eager work between them,
+    outputs cleared before each) bitwise equal to the eager run, also after
+    new targets are copied in."""
+    torch = gpu
+    from fml_amd.batch import KRRCall
+    from fml_amd.graphs import Graph
+    from fml_amd.krr import KRR
+    from fml_amd.representations import generate_coulomb_matrices
+    ms = synth.generate(400, seed=35, nmin=5, nmax=23)
+    n = 320
+    y = ms.energies
+    call = KRRCall(ms.charges, ms.coords, ms.offsets, n, y[:n], 700.0, 1e-8, nmax=23)
+    call.run()
+    torch.cuda.synchronize()
+    assert int(call.info.item()) == 0
+    pred, alpha = call.pred.cpu().numpy(), call.alpha.cpu().numpy()
+    X = generate_coulomb_matrices(ms.charges, ms.coords, ms.offsets, nmax=23)
+    m = KRR(700.0, 1e-8).fit(X[:n], y[:n])
+    assert np.array_equal(m.alpha.cpu().numpy(), alpha)
+    assert np.array_equal(m.predict(X[n:]).cpu().numpy(), pred)
+    Xo = oracle.coulomb_matrices(ms.charges, ms.coords, ms.offsets, 23)[0]
+    K = oracle.gaussian_kernel(Xo[:n].T, Xo[:n].T, 700.0)
+    K[np.diag_indices_from(K)] += 1e-8
+    ao, info = oracle.cho_solve(K, y[:n])
+    assert info == 0
+    po = oracle.gaussian_kernel(Xo[:n].T, Xo[n:].T, 700.0).T @ ao
+    assert np.max(np.abs(pred - po)) <= 1e-6 * np.max(np.abs(po))
+
+    g = Graph.capture(call.run, keep=[call])
+    for rep in range(3):
+        call.alpha.fill_(-1.0)
+        call.pred.fill_(-1.0)
+        call.info.fill_(-1)
+        torch.cuda.synchronize()
+        g.launch()
+        torch.cuda.synchronize()
+        assert int(call.info.item()) == 0, rep
+        assert np.array_equal(call.alpha.cpu().numpy(), alpha), rep
+        assert np.array_equal(call.pred.cpu().numpy(), pred), rep
+        _ = (torch.ones(1000, device="cuda") * 2).sum().item()
+    y2 = y[:n] + np.random.default_rng(36).normal(0.0, 0.5, n)
+    call.set_targets(y2)
+    g.launch()
+    torch.cuda.synchronize()
+    pred_g = call.pred.cpu().numpy()
+    call.run()
+    torch.cuda.synchronize()
+    assert np.array_equal(pred_g, call.pred.cpu().numpy())
+    assert not np.array_equal(pred_g, pred)
+    g.destroy()
