@@ -39,6 +39,7 @@ sys.path.insert(0, ROOT)
 PEAK_HBM_GBS = 8000.0
 PEAK_FP64_TFLOPS = 78.6            # vector and matrix, FMA = 2 flop
 PEAK_FP32_TFLOPS = 157.3
+PEAK_INT8_POPS = 5.0               # dense INT8 matrix
 PEAK_FP64_VALU_TOPS = 39.3         # FP64 VALU lane-operations/s: the FP64 vector peak / 2 (FMA = 1 op)
 PEAK_FP32_VALU_TOPS = 78.6         # FP32 VALU lane-operations/s: the FP32 vector peak / 2
 
@@ -64,7 +65,7 @@ def parse():
     ap.add_argument("--workload", default="arad",
                     choices=["arad", "krr", "gaussian64", "gaussian32", "gaussianbf16x3",
                              "laplacian64", "laplacian32", "cholesky64", "cholesky32",
-                             "krr36k_fp64", "krr36k_mixed", "serving", "arad4096", "arad4096_fp32", "copy",
+                             "krr36k_fp64", "krr36k_mixed", "krr36k_ozaki", "serving", "arad4096", "arad4096_fp32", "copy",
                              "atomic", "atomic_lap", "atomic_mfma", "aras", "inverse64", "solve64", "solve32"])
     ap.add_argument("--duration", type=float, default=5.0, help="serving run length in seconds")
     ap.add_argument("--n", type=int, default=0, help="problem size override for the 36k workloads")
@@ -834,7 +835,8 @@ def _train_36k(n_total, n_train, precision="fp64", sigma=1000.0, lam=1e-6):
 def krr36k_workload(args, precision):
     """Precision form at scale: fit a 36,000-molecule Gaussian KRR (sigma
     1000, lambda 1e-6) and predict 4,000 held-out molecules, FP64 plain form
-    vs the mixed form (FP32 factor preconditioning FP64-residual CG)."""
+    vs the mixed form (FP32 factor preconditioning FP64-residual CG) or the
+    Ozaki form (INT8-core factor refined with FP64 residuals)."""
     import torch
     from fml_amd.krr import KRR
     n = args.n or 40000
@@ -859,7 +861,8 @@ def krr36k_workload(args, precision):
         state["m"] = m
         state["pred"] = m.predict(X[ntr:])
 
-    info = {"metric": "KRR fit+predict time", "unit": "ms", "dtype": "f64" if precision == "fp64" else "f32+f64",
+    info = {"metric": "KRR fit+predict time", "unit": "ms",
+            "dtype": {"fp64": "f64", "mixed": "f32+f64", "ozaki": "i8+f64"}[precision],
             "higher_is_better": False,
             "config": {"workload": f"Gaussian KRR fit {ntr} + predict {n - ntr} molecules (nmax 29, seed 0), "
                                    f"sigma 1000, lambda 1e-6, precision {precision}"}}
@@ -912,12 +915,19 @@ def krr36k_workload(args, precision):
             del K
         # the dominant kernel work: the Cholesky factorisation, n^3/3 flop
         fl = ntr ** 3 / 3.0
-        peak = PEAK_FP64_TFLOPS if precision == "fp64" else PEAK_FP32_TFLOPS
         ach = fl / rep.factor_s / 1e12
-        res["roofline"] = {"bound": "mfma", "achieved": round(ach, 2), "peak": peak, "unit": "TFLOP/s",
-                           "frac": round(ach / peak, 4), "traffic": None,
-                           "note": f"the {'FP64' if precision == 'fp64' else 'FP32'} factorisation of "
-                                   f"K + lambda I (n^3/3 flop), timed inside the fit"}
+        if precision == "ozaki":
+            # INT8 work: 21 slice products per FP64 multiply-add of the updates
+            res["roofline"] = {"bound": "mfma", "achieved": round(ach * 21 / 1e3, 3), "peak": PEAK_INT8_POPS,
+                               "unit": "POP/s", "frac": round(ach * 21 / 1e3 / PEAK_INT8_POPS, 4), "traffic": None,
+                               "note": "the factorisation of K + lambda I with its updates on INT8 cores: "
+                                       "n^3/3 FP64-equivalent flop x 21 slice products, timed inside the fit"}
+        else:
+            peak = PEAK_FP64_TFLOPS if precision == "fp64" else PEAK_FP32_TFLOPS
+            res["roofline"] = {"bound": "mfma", "achieved": round(ach, 2), "peak": peak, "unit": "TFLOP/s",
+                               "frac": round(ach / peak, 4), "traffic": None,
+                               "note": f"the {'FP64' if precision == 'fp64' else 'FP32'} factorisation of "
+                                       f"K + lambda I (n^3/3 flop), timed inside the fit"}
 
     def cpu_baseline(rows):
         # the reference on the host: Gaussian kernel and fcho_solve on a 12,000-molecule

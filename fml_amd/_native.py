@@ -83,6 +83,8 @@ SIGNATURES = {
                                  c_void_p, c_void_p, c_size_t, c_void_p]),
     "fmlh_potrf_ws_bytes": (c_size_t, [c_int]),
     "fmlh_potrf": (c_int, [c_void_p, c_int64, c_int, c_void_p, c_void_p, c_size_t, c_void_p]),
+    "fmlh_potrf_ozaki_ws_bytes": (c_size_t, [c_int]),
+    "fmlh_potrf_ozaki": (c_int, [c_void_p, c_int64, c_int, c_void_p, c_void_p, c_size_t, c_void_p]),
     "fmlh_potri_ws_bytes": (c_size_t, [c_int]),
     "fmlh_potri": (c_int, [c_void_p, c_int64, c_int, c_void_p, c_void_p, c_size_t, c_void_p]),
     "fmlh_potrs_ws_bytes": (c_size_t, [c_int, c_int]),

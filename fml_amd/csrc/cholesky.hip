@@ -33,8 +33,11 @@
 
 #include <mutex>
 
+#include <type_traits>
+
 #include "common.h"
 #include "gemm_nt.h"
+#include "ozaki.h"
 
 namespace {
 
@@ -297,8 +300,13 @@ bool aux_for_stream(hipStream_t caller, Aux &out) {
 // (disjoint columns), and the next update waits for that factor.  Inside a
 // graph capture, or without the auxiliary stream, everything runs on the
 // caller's stream in the same order.
+//
+// oz_ws (FP64 only, fmlh_potrf_ozaki): the next panel's columns and the
+// trailing matrix are updated on INT8 matrix cores from the Ozaki split of
+// the finished panel (ozaki.h) instead of FP64 MFMA tiles; the panels
+// themselves are factored in FP64 as always.
 template <typename T>
-int potrf_impl(T *a, int64_t lda, int n, int32_t *info, hipStream_t st) {
+int potrf_impl(T *a, int64_t lda, int n, int32_t *info, hipStream_t st, void *oz_ws = nullptr) {
   hipLaunchKernelGGL(k_set_int_impl, dim3(1), dim3(1), 0, st, info, 0);
   FMLH_LAUNCH_CHECK("k_set_int");
   Aux aux;
@@ -324,7 +332,14 @@ int potrf_impl(T *a, int64_t lda, int n, int32_t *info, hipStream_t st) {
     if (kend >= n) break;
     const int nend = min(kend + kNBO, n);  // the next panel is [kend, nend)
     // the next panel's columns, all rows below kend, with this panel
-    if (n - kend >= 4 * 128) {
+    const bool oz = std::is_same<T, double>::value && oz_ws != nullptr;
+    if (oz) {
+      double *ad = reinterpret_cast<double *>(a);
+      rc = oz_split(ad, lda, kend, n - kend, K0, kend - K0, kOzSlices, oz_ws, info, st);
+      if (rc) return rc;
+      rc = oz_update(ad, lda, kend, n, kend, nend, kend, n - kend, kOzSlices, oz_ws, false, info, st);
+      if (rc) return rc;
+    } else if (n - kend >= 4 * 128) {
       const int gx = ceil_div(nend - kend, 128), gy = ceil_div(n - kend, 128);
       hipLaunchKernelGGL((k_update<T, 128, false>), dim3((unsigned)xcd_grid(gx, gy)), dim3(512), 0,
                          st, a, lda, kend, n, kend, nend, K0, kend, gx, gy, info);
@@ -342,14 +357,18 @@ int potrf_impl(T *a, int64_t lda, int n, int32_t *info, hipStream_t st) {
       if (rc) return rc;
       e = hipEventRecord(aux.join, aux.s);
       if (e != hipSuccess) return record_error(e, "potrf join record");
-      rc = trailing_update(a, lda, n, nend, K0, kend, info, st);
+      rc = oz ? oz_update(reinterpret_cast<double *>(a), lda, nend, n, nend, n, kend, n - kend,
+                          kOzSlices, oz_ws, true, info, st)
+              : trailing_update(a, lda, n, nend, K0, kend, info, st);
       if (rc) return rc;
       e = hipStreamWaitEvent(st, aux.join, 0);
       if (e != hipSuccess) return record_error(e, "potrf join");
     } else {
       rc = panel_factor(a, lda, n, kend, nend, info, st);
       if (rc) return rc;
-      rc = trailing_update(a, lda, n, nend, K0, kend, info, st);
+      rc = oz ? oz_update(reinterpret_cast<double *>(a), lda, nend, n, nend, n, kend, n - kend,
+                          kOzSlices, oz_ws, true, info, st)
+              : trailing_update(a, lda, n, nend, K0, kend, info, st);
       if (rc) return rc;
     }
   }
@@ -1083,6 +1102,26 @@ int fmlh_potrf(double *a, int64_t lda, int n, int32_t *info, void *ws, size_t ws
   }
   timing_begin(as_stream(stream));
   int rc = potrf_impl<double>(a, lda, n, info, as_stream(stream));
+  timing_end(as_stream(stream));
+  return rc;
+}
+
+size_t fmlh_potrf_ozaki_ws_bytes(int n) { return oz_ws_bytes(n, kOzSlices); }
+
+int fmlh_potrf_ozaki(double *a, int64_t lda, int n, int32_t *info, void *ws, size_t ws_bytes,
+                     void *stream) {
+  FMLH_ARG_CHECK(n >= 0, 3);
+  FMLH_ARG_CHECK(lda >= (n > 1 ? n : 1), 2);
+  FMLH_ARG_CHECK(info != nullptr, 4);
+  if (n > 0) FMLH_ARG_CHECK(a != nullptr, 1);
+  FMLH_ARG_CHECK(ws != nullptr && ws_bytes >= oz_ws_bytes(n, kOzSlices), 5);
+  if (n == 0) {
+    hipLaunchKernelGGL(k_set_int_impl, dim3(1), dim3(1), 0, as_stream(stream), info, 0);
+    FMLH_LAUNCH_CHECK("k_set_int");
+    return 0;
+  }
+  timing_begin(as_stream(stream));
+  int rc = potrf_impl<double>(a, lda, n, info, as_stream(stream), ws);
   timing_end(as_stream(stream));
   return rc;
 }

@@ -189,6 +189,14 @@ __device__ __forceinline__ void tile_nt(const T *__restrict__ P, int64_t ldp, in
   }
 }
 
+// LDS swizzle for tiles staged as 64-byte rows read in 16-byte chunks: chunk
+// c of row r is stored at chunk c ^ lds_swizzle(r).  A row-wise staging
+// write covers whole rows (consecutive banks), and the fragment read of 16
+// consecutive rows at one logical chunk then touches 16 distinct 4-bank
+// groups for 64 banks (and each aligned 8 rows distinct groups for 32
+// banks), where an unswizzled or padded row stride repeats a bank group.
+__host__ __device__ inline int lds_swizzle(int r) { return ((r >> 1) ^ (r >> 2)) & 3; }
+
 // XCD-aware tile order.  Workgroups are dispatched round-robin over the
 // chip's 8 XCDs (workgroup L runs on XCD L % 8), each XCD with its own L2.
 // The grid is padded to a multiple of 8; workgroup L takes tile

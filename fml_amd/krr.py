@@ -10,7 +10,12 @@ This module is that chain kept on the GPU end to end:
   residual, an FP32 copy (optionally shifted until it factors) is factored
   on FP32 MFMA, and the FP64 system is solved by conjugate gradients
   preconditioned with that FP32 factor, every residual and update in FP64
-  (iterative refinement is the special case of one-step convergence).
+  (iterative refinement is the special case of one-step convergence);
+* Ozaki form (``precision="ozaki"``): a copy of K + lambda I is factored with
+  its large updates on INT8 matrix cores (fmlh_potrf_ozaki: an exact
+  six-slice split of each panel, the factor within ~1e-12 of the FP64 one),
+  then refined with FP64 residuals y - (K + lambda I) alpha until the
+  residual reaches tol or stops falling.
 """
 from __future__ import annotations
 
@@ -22,7 +27,7 @@ import numpy as np
 from . import _native
 from .device import empty, to_device, torch
 from .kernels.kernels import gaussian_kernel_rows, laplacian_kernel_rows
-from .math.cho_solve import potrf_inplace, potrs_inplace
+from .math.cho_solve import potrf_inplace, potrf_ozaki_inplace, potrs_inplace
 
 
 def kernel_matrix(X1, X2, sigma, kernel="gaussian", precision="fp64", out=None, out_dtype=None,
@@ -128,6 +133,8 @@ class KRR:
             self.factor = K if keep_kernel else None
         elif self.precision == "mixed":
             alpha = self._fit_mixed(K, yd, rep)
+        elif self.precision == "ozaki":
+            alpha = self._fit_ozaki(K, yd, rep)
         else:
             raise ValueError(self.precision)
         rep.seconds = time.perf_counter() - t0
@@ -234,6 +241,60 @@ class KRR:
                 f"mixed-precision fit did not reach |y - (K + lambda I) alpha| <= {tol:g} |y| "
                 f"({why}; relative residual {rep.residual:.3g}, FP32 shift {shift:g}); "
                 "raise max_iter or use precision='fp64'", alpha=x, report=rep)
+        return x
+
+    def _fit_ozaki(self, K, yd, rep):
+        """Factor of K + lambda I with INT8-core updates (an unfactored K is
+        kept for the residuals), then iterative refinement: alpha += F^-1 r,
+        r = y - (K + lambda I) alpha in FP64.  Stops when |r| <= tol |y|, or
+        when a step no longer halves |r| (the FP64 floor of the residual);
+        raises ConvergenceError if that floor is above 1e-8 |y| or max_iter
+        steps did not reach it."""
+        t = torch()
+        n = int(K.shape[0])
+        t1 = time.perf_counter()
+        F = K.clone()
+        add_diagonal(F, self.lam)
+        info = potrf_ozaki_inplace(F)
+        t.cuda.synchronize()
+        rep.factor_s = time.perf_counter() - t1
+        if info != 0:
+            raise np.linalg.LinAlgError(f"K + lambda I is not positive definite (leading minor {info})")
+        t2 = time.perf_counter()
+        r = empty((n,), np.float64)
+
+        def residual(v, out):  # y - (K + lam I) v
+            predict_from_kernel(K, v, out=out)
+            out.add_(v, alpha=self.lam)
+            out.neg_().add_(yd)
+            return out
+
+        x = yd.clone()
+        potrs_inplace(F, x)
+        ynorm = float(t.linalg.norm(yd).item())
+        tol = float(self.tol)
+        converged = False
+        prev = None
+        for _ in range(self.max_iter):
+            residual(x, r)
+            rel = float(t.linalg.norm(r).item()) / ynorm  # one host read per step
+            rep.history.append(rel)
+            if rel <= tol or (prev is not None and rel > 0.5 * prev):
+                converged = rel <= max(tol, 1e-8)
+                break
+            prev = rel
+            potrs_inplace(F, r)
+            x.add_(r)
+        rep.iterations = len(rep.history) - 1
+        rep.residual = rep.history[-1] if rep.history else float("nan")
+        t.cuda.synchronize()
+        rep.solve_s = time.perf_counter() - t2
+        del F
+        if not converged:
+            raise ConvergenceError(
+                f"Ozaki-factor refinement stopped at relative residual {rep.residual:.3g} after "
+                f"{rep.iterations} steps (the factor is too inexact for this system's conditioning); "
+                "use precision='fp64'", alpha=x, report=rep)
         return x
 
     # -- predict -----------------------------------------------------------

@@ -54,6 +54,24 @@ def potrf_inplace(m, sync: bool = True):
     return int(info.item())
 
 
+def potrf_ozaki_inplace(m, sync: bool = True):
+    """potrf_inplace with the next-panel and trailing updates on INT8 matrix
+    cores (fmlh_potrf_ozaki, FP64 data only): a factor for FP64 iterative
+    refinement, within about 2^-40 per update of the FP64 factor."""
+    t = torch()
+    if m.dtype != t.float64:
+        raise TypeError("the Ozaki-split factor takes float64 matrices")
+    n = int(m.shape[0])
+    info = t.zeros(1, dtype=t.int32, device="cuda")
+    wsb = _native.load().fmlh_potrf_ozaki_ws_bytes(n)
+    ws = workspace(wsb)
+    _native.call("fmlh_potrf_ozaki", _native.ptr(m), m.stride(0), n, _native.ptr(info), _native.ptr(ws),
+                 wsb, _native.stream_ptr())
+    if not sync:
+        return info
+    return int(info.item())
+
+
 def potrs_inplace(m, x):
     """Solve with the factor m for x (n,) or (nrhs, n) rows, in place."""
     t = torch()

@@ -413,6 +413,21 @@ int fmlh_arad_atomic(int mode, const double *q1, const double *zt1,
 size_t fmlh_potrf_ws_bytes(int n);
 int fmlh_potrf(double *a, int64_t lda, int n, int32_t *info, void *ws,
                size_t ws_bytes, void *stream);
+/* Precision form of fmlh_potrf (no reference counterpart): the same
+ * factorisation, the panels in FP64, with the next-panel and trailing
+ * updates computed on INT8 matrix cores from an exact split of each
+ * finished 1024-column panel: every row scaled by a power of two and peeled
+ * into six signed 7-bit slices (the error-free splitting of Ozaki, Ogita,
+ * Oishi and Rump, with integer slices as Mukunoki et al. use on tensor
+ * cores), the slice products with t + u <= 7 summed exactly in int32 by
+ * v_mfma_i32_16x16x64_i8 and combined in FP64.  An update's error is below
+ * about 2^-40 |row| |column| summed over the panel (dropped slice products),
+ * so the factor serves as the solver of FP64 iterative refinement
+ * (KRR(precision="ozaki")), not as a stand-in for the FP64 factor.
+ * ws: fmlh_potrf_ozaki_ws_bytes(n) bytes (slice planes and row exponents). */
+size_t fmlh_potrf_ozaki_ws_bytes(int n);
+int fmlh_potrf_ozaki(double *a, int64_t lda, int n, int32_t *info, void *ws,
+                     size_t ws_bytes, void *stream);
 /* Solve U^T U x = b in place for nrhs right-hand sides, x (n, nrhs) Fortran
  * order with leading dimension ldx. */
 size_t fmlh_potrs_ws_bytes(int n, int nrhs);

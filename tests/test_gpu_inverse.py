@@ -141,3 +141,54 @@ def test_potrf_lookahead_concurrent_threads(gpu):
     for k in range(2):
         assert int(infos[k].item()) == 0
         assert torch.equal(torch.tril(out[k]), torch.tril(ref[k]))
+
+
+def test_ozaki_factor_and_refinement(gpu):
+    """fmlh_potrf_ozaki: the next-panel and trailing updates from the exact
+    six-slice INT8 split (n = 3000: two panels of look-ahead updates).  The
+    factor is within 1e-10 of the FP64 factor and |L L^T - A| / |A| <= 1e-10
+    (the dropped slice products are below ~2^-40 of the row and column
+    scales per update); KRR(precision="ozaki") refines to a relative residual
+    no larger than the FP64 direct solve's, and its predictions agree with
+    the FP64 model's to 1e-9 relative."""
+    torch = gpu
+    from fml_amd import synth
+    from fml_amd.krr import KRR, kernel_matrix, predict_from_kernel
+    from fml_amd.math.cho_solve import potrf_inplace, potrf_ozaki_inplace
+    from fml_amd.representations import generate_coulomb_matrices
+    rng = np.random.default_rng(41)
+    n = 3000
+    B = rng.normal(size=(n, n))
+    A = B @ B.T / n + np.eye(n)
+    M1 = torch.from_numpy(A).cuda()
+    M2 = M1.clone()
+    assert potrf_inplace(M1) == 0 and potrf_ozaki_inplace(M2) == 0
+    L1, L2 = torch.tril(M1), torch.tril(M2)
+    Ad = torch.from_numpy(A).cuda()
+    diff = float((L1 - L2).abs().max() / L1.abs().max())
+    berr = float((L2 @ L2.T - Ad).abs().max() / Ad.abs().max())
+    print(f"\nOzaki factor n={n}: vs FP64 factor {diff:.3g}, |LL^T - A|/|A| {berr:.3g}")
+    assert 0 < diff <= 1e-10 and berr <= 1e-10
+    # a KRR system: 3000 molecules, 2600 train
+    ms = synth.generate(3000, seed=42, nmin=3, nmax=29)
+    X = generate_coulomb_matrices(ms.charges, ms.coords, ms.offsets, nmax=29)
+    y = ms.energies
+    nt = 2600
+    m64 = KRR(1000.0, 1e-6).fit(X[:nt], y[:nt])
+    moz = KRR(1000.0, 1e-6, precision="ozaki", tol=1e-12, max_iter=20).fit(X[:nt], y[:nt])
+    K = kernel_matrix(m64.X, m64.X, 1000.0)
+    yd = torch.from_numpy(y[:nt].copy()).cuda()
+    r = torch.empty_like(yd)
+
+    def rel_res(a):
+        predict_from_kernel(K, a, out=r)
+        r.add_(a, alpha=1e-6)
+        return float(torch.linalg.norm(yd - r) / torch.linalg.norm(yd))
+
+    r64, roz = rel_res(m64.alpha), rel_res(moz.alpha)
+    p64, poz = m64.predict(X[nt:]), moz.predict(X[nt:])
+    pdiff = float((p64 - poz).abs().max() / p64.abs().max())
+    print(f"KRR 2600: relative residual FP64 {r64:.3g}, Ozaki refined {roz:.3g} "
+          f"({moz.report.iterations} steps), prediction diff {pdiff:.3g}")
+    assert roz <= max(r64, 1e-12)
+    assert pdiff <= 1e-9
