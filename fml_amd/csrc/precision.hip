@@ -18,10 +18,11 @@
 //
 // Tile: 128 x 128 outputs per 256 threads, waves 2 x 2, each wave 64 x 64
 // from 4 x 4 16x16 blocks (48 MFMAs per k step).  A k step stages 128 rows x
-// 32 columns of the four planes (32 KB) in LDS, rows padded to 40 bf16 = 80
-// bytes so the 16-byte fragment reads of 16 consecutive rows fall in
-// distinct banks; the next step's planes are loaded into registers while the
-// current one is multiplied.  The grid is 1-D in XCD-grouped tile order
+// 32 columns of the four planes (32 KB) in LDS, 64 bytes a row with the
+// 16-byte chunks of row r permuted by lds_swizzle(r) (gemm_nt.h), so
+// neither the staging writes nor the 16-byte fragment reads of 16
+// consecutive rows collide in a bank; the next step's planes are loaded into
+// registers while the current one is multiplied.  The grid is 1-D in XCD-grouped tile order
 // (gemm_nt.h), so the tiles one XCD runs share operand rows in its L2.
 //
 // bf16 16x16x32 MFMA lane layout on gfx950 (checked with an in-bounds probe):
@@ -45,7 +46,7 @@ typedef float f4v __attribute__((ext_vector_type(4)));
 typedef unsigned int u4v __attribute__((ext_vector_type(4)));
 
 constexpr int kBK = 32;     // k step
-constexpr int kRow = 40;    // bf16 per LDS row (32 + 8 padding)
+constexpr int kRow = 32;    // bf16 per LDS row (64 bytes; 16-byte chunks swizzled)
 constexpr int kMT = 128;    // tile edge
 constexpr float kTau = 2e-2f;
 
@@ -153,7 +154,8 @@ __global__ __launch_bounds__(256, 2) void k_gauss_b3(Plane pa, int na, Plane pb,
   auto lstore = [&]() {
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
-      const int o = (srow + 64 * h) * kRow + scol;
+      const int row = srow + 64 * h;
+      const int o = row * kRow + 8 * ((scol >> 3) ^ lds_swizzle(row));
       *reinterpret_cast<u4v *>(&s.ah[o]) = rah[h];
       *reinterpret_cast<u4v *>(&s.al[o]) = ral[h];
       *reinterpret_cast<u4v *>(&s.bh[o]) = rbh[h];
@@ -178,13 +180,15 @@ __global__ __launch_bounds__(256, 2) void k_gauss_b3(Plane pa, int na, Plane pb,
     bf16x8 bh[4], bl[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      const int r = (wm * 64 + j * 16 + (lane & 15)) * kRow + ko;
+      const int row = wm * 64 + j * 16 + (lane & 15);
+      const int r = row * kRow + 8 * ((ko >> 3) ^ lds_swizzle(row));
       bh[j] = *reinterpret_cast<const bf16x8 *>(&s.bh[r]);
       bl[j] = *reinterpret_cast<const bf16x8 *>(&s.bl[r]);
     }
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      const int r = (wn * 64 + i * 16 + (lane & 15)) * kRow + ko;
+      const int row = wn * 64 + i * 16 + (lane & 15);
+      const int r = row * kRow + 8 * ((ko >> 3) ^ lds_swizzle(row));
       const bf16x8 ah = *reinterpret_cast<const bf16x8 *>(&s.ah[r]);
       const bf16x8 al = *reinterpret_cast<const bf16x8 *>(&s.al[r]);
 #pragma unroll
