@@ -294,10 +294,10 @@ bool aux_for_stream(hipStream_t caller, Aux &out) {
 }
 
 // Right-looking factorisation over 1024-column outer panels with one panel
-// of look-ahead: after panel p is factored, the columns of panel p + 1 are
-// updated with it first; panel p + 1 is then factored on the auxiliary
-// stream while the caller's stream updates the rest of the trailing matrix
-// (disjoint columns), and the next update waits for that factor.  Inside a
+// of look-ahead: after panel p is factored, the auxiliary stream updates the
+// columns of panel p + 1 with it and factors panel p + 1, while the caller's
+// stream updates the rest of the trailing matrix (disjoint columns); the
+// next update waits for that factor.  Inside a
 // graph capture, or without the auxiliary stream, everything runs on the
 // caller's stream in the same order.
 //
@@ -331,44 +331,58 @@ int potrf_impl(T *a, int64_t lda, int n, int32_t *info, hipStream_t st, void *oz
     const int kend = min(K0 + kNBO, n);
     if (kend >= n) break;
     const int nend = min(kend + kNBO, n);  // the next panel is [kend, nend)
-    // the next panel's columns, all rows below kend, with this panel
     const bool oz = std::is_same<T, double>::value && oz_ws != nullptr;
-    if (oz) {
-      double *ad = reinterpret_cast<double *>(a);
-      rc = oz_split(ad, lda, kend, n - kend, K0, kend - K0, kOzSlices, oz_ws, info, st);
+    if (oz) {  // the finished panel's rows below it, split once for both updates
+      rc = oz_split(reinterpret_cast<double *>(a), lda, kend, n - kend, K0, kend - K0, kOzSlices,
+                    oz_ws, info, st);
       if (rc) return rc;
-      rc = oz_update(ad, lda, kend, n, kend, nend, kend, n - kend, kOzSlices, oz_ws, false, info, st);
-      if (rc) return rc;
-    } else if (n - kend >= 4 * 128) {
-      const int gx = ceil_div(nend - kend, 128), gy = ceil_div(n - kend, 128);
-      hipLaunchKernelGGL((k_update<T, 128, false>), dim3((unsigned)xcd_grid(gx, gy)), dim3(512), 0,
-                         st, a, lda, kend, n, kend, nend, K0, kend, gx, gy, info);
-    } else {
-      const int gx = ceil_div(nend - kend, kTile), gy = ceil_div(n - kend, kTile);
-      hipLaunchKernelGGL((k_update<T, kTile, false>), dim3((unsigned)xcd_grid(gx, gy)), dim3(256), 0,
-                         st, a, lda, kend, n, kend, nend, K0, kend, gx, gy, info);
     }
-    FMLH_LAUNCH_CHECK("k_update(next panel)");
+    // the next panel's columns, all rows below kend, with this panel
+    auto next_update = [&](hipStream_t s) -> int {
+      if (oz)
+        return oz_update(reinterpret_cast<double *>(a), lda, kend, n, kend, nend, kend, n - kend,
+                         kOzSlices, oz_ws, false, info, s);
+      if (n - kend >= 4 * 128) {
+        const int gx = ceil_div(nend - kend, 128), gy = ceil_div(n - kend, 128);
+        hipLaunchKernelGGL((k_update<T, 128, false>), dim3((unsigned)xcd_grid(gx, gy)), dim3(512),
+                           0, s, a, lda, kend, n, kend, nend, K0, kend, gx, gy, info);
+      } else {
+        const int gx = ceil_div(nend - kend, kTile), gy = ceil_div(n - kend, kTile);
+        hipLaunchKernelGGL((k_update<T, kTile, false>), dim3((unsigned)xcd_grid(gx, gy)),
+                           dim3(256), 0, s, a, lda, kend, n, kend, nend, K0, kend, gx, gy, info);
+      }
+      FMLH_LAUNCH_CHECK("k_update(next panel)");
+      return 0;
+    };
+    // the rest of the trailing matrix (columns from nend), with this panel
+    auto rest_update = [&]() -> int {
+      return oz ? oz_update(reinterpret_cast<double *>(a), lda, nend, n, nend, n, kend, n - kend,
+                            kOzSlices, oz_ws, true, info, st)
+                : trailing_update(a, lda, n, nend, K0, kend, info, st);
+    };
     if (ahead) {
+      // the auxiliary stream updates the next panel's columns and factors
+      // them while the caller's stream updates the rest (disjoint columns,
+      // both reading only this panel)
       hipError_t e = hipEventRecord(aux.fork, st);
       if (e == hipSuccess) e = hipStreamWaitEvent(aux.s, aux.fork, 0);
       if (e != hipSuccess) return record_error(e, "potrf fork");
+      rc = next_update(aux.s);
+      if (rc) return rc;
       rc = panel_factor(a, lda, n, kend, nend, info, aux.s);
       if (rc) return rc;
       e = hipEventRecord(aux.join, aux.s);
       if (e != hipSuccess) return record_error(e, "potrf join record");
-      rc = oz ? oz_update(reinterpret_cast<double *>(a), lda, nend, n, nend, n, kend, n - kend,
-                          kOzSlices, oz_ws, true, info, st)
-              : trailing_update(a, lda, n, nend, K0, kend, info, st);
+      rc = rest_update();
       if (rc) return rc;
       e = hipStreamWaitEvent(st, aux.join, 0);
       if (e != hipSuccess) return record_error(e, "potrf join");
     } else {
+      rc = next_update(st);
+      if (rc) return rc;
       rc = panel_factor(a, lda, n, kend, nend, info, st);
       if (rc) return rc;
-      rc = oz ? oz_update(reinterpret_cast<double *>(a), lda, nend, n, nend, n, kend, n - kend,
-                          kOzSlices, oz_ws, true, info, st)
-              : trailing_update(a, lda, n, nend, K0, kend, info, st);
+      rc = rest_update();
       if (rc) return rc;
     }
   }
