@@ -786,10 +786,19 @@ __global__ void k_trsv_prep(int32_t *ticket, T *pub, int n) {
   if (i < n) reinterpret_cast<U *>(pub)[i] = PubBits<T>::kSentinel;
 }
 
+// FP32 (the mixed form's preconditioner solve): a full diagonal block is
+// applied as its inverse W = L_bb^-1, formed by wave 0 while the workgroup
+// waits for the blocks before it, so the block's own step on the chain is
+// a 64 x 64 product over all four waves instead of 64 dependent
+// substitution steps.  FP64 keeps the substitution (its rounding matches a
+// triangular solve's backward-error bound; the inverse's does not).
 template <typename T, bool kForward>
 __global__ __launch_bounds__(256) void k_trsv_chain(const T *__restrict__ a, int64_t lda, int n,
                                                     T *x, int32_t *ctl, T *pub) {
+  constexpr bool kInv = sizeof(T) == 4;
+  constexpr int kWRows = kInv ? kNB : 1;
   __shared__ T L[kNB][kNB + 1];
+  __shared__ T W[kWRows][kNB + 1];
   __shared__ T part[4][kNB];
   __shared__ T rhs[kNB];
   __shared__ T rinv[kNB];
@@ -811,6 +820,26 @@ __global__ __launch_bounds__(256) void k_trsv_chain(const T *__restrict__ a, int
   const T xown = tid < bs ? x[r0 + tid] : T(0);
   __syncthreads();
   if (tid < bs) rinv[tid] = T(1) / L[tid][tid];
+  const bool use_inv = kInv && bs == kNB;
+  if (use_inv) {
+    __syncthreads();  // rinv
+    if (wave == 0) {
+      // lane j: column j of W by right-looking substitution of e_j (the
+      // entries above the diagonal stay zero)
+      T w[kNB];
+#pragma unroll
+      for (int r = 0; r < kNB; ++r) w[r] = r == lane ? T(1) : T(0);
+#pragma unroll
+      for (int c = 0; c < kNB; ++c) {
+        const T wc = w[c] * rinv[c];
+        w[c] = wc;
+#pragma unroll
+        for (int r = c + 1; r < kNB; ++r) w[r] = fma(-L[r][c], wc, w[r]);
+      }
+#pragma unroll
+      for (int r = 0; r < kNB; ++r) W[kInv ? r : 0][lane] = w[r];
+    }
+  }
   // The products with the solved blocks: the tile of block j is loaded one
   // step ahead (it does not depend on x), so each step waits only for the
   // flag and the 64 values of x_j.
@@ -897,6 +926,24 @@ __global__ __launch_bounds__(256) void k_trsv_chain(const T *__restrict__ a, int
     if (tid < bs) rhs[tid] = xown - (((part[0][tid] + part[1][tid]) + part[2][tid]) + part[3][tid]);
   }
   __syncthreads();
+  if (use_inv) {
+    // x = W rhs (forward) or W^T rhs (backward): thread (row lane, quarter
+    // wave) sums 16 columns, the quarters are added in order
+    T sp = T(0);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const int c = 16 * wave + k;
+      sp = fma(kForward ? W[kInv ? lane : 0][c] : W[kInv ? c : 0][lane], rhs[c], sp);
+    }
+    part[wave][lane] = sp;
+    __syncthreads();
+    if (wave == 0) {
+      const T v = ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane];
+      x[r0 + lane] = v;
+      publish(pub, r0 + lane, v);
+    }
+    return;
+  }
   if (wave == 0) {
     // the block's substitution in one wave: lane r holds row r; each pivot
     // value is broadcast with readlane (a register read, no LDS round trip)

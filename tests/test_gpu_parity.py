@@ -249,6 +249,36 @@ def test_cho_solve_sizes_and_failure(gpu):
         cho_solve(np.ones((3, 4)), np.ones(3))
 
 
+def test_spotrs_against_fp64_solve_of_same_factor(gpu):
+    """FP32 solve (one right-hand side: the chained kernel, whose full 64-row
+    blocks multiply by the diagonal block's inverse and whose ragged last
+    block substitutes; three: the blocked solve) against an FP64 solve with
+    the same FP32 factor U (A = U^T U).  Bound: 2e-5 relative in the max
+    norm on matrices of condition number below 10; a wrong inverse entry or
+    a skipped block gives errors of order one."""
+    import scipy.linalg as sl
+    import torch
+    from fml_amd.math.cho_solve import potrf_inplace, potrs_inplace
+    rng = np.random.default_rng(7)
+    for n in (1, 63, 64, 65, 128, 200, 257, 1000, 4100):
+        B = rng.standard_normal((n, n))
+        A = (B @ B.T / n + 2.0 * np.eye(n)).astype(np.float32)
+        m = torch.from_numpy(np.ascontiguousarray(A.T)).cuda()
+        assert potrf_inplace(m) == 0
+        # Fortran memory: U(i, j), i <= j, sits at m[j, i]
+        U = np.triu(m.cpu().numpy().T).astype(np.float64)
+        for nrhs in (1, 3):
+            y = rng.standard_normal((nrhs, n)).astype(np.float32)
+            x = potrs_inplace(m, torch.from_numpy(y.copy()).cuda().squeeze(0) if nrhs == 1
+                              else torch.from_numpy(y.copy()).cuda())
+            x = x.cpu().numpy().reshape(nrhs, n).astype(np.float64)
+            for j in range(nrhs):
+                z = sl.solve_triangular(U, y[j].astype(np.float64), trans="T", lower=False)
+                xr = sl.solve_triangular(U, z, lower=False)
+                err = np.max(np.abs(x[j] - xr)) / np.max(np.abs(xr))
+                assert err < 2e-5, (n, nrhs, j, err)
+
+
 # ------------------------------------------------------------ precision forms
 
 def test_gaussian_precision_forms(gpu, mols):
