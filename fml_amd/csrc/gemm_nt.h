@@ -20,6 +20,8 @@
 //   D (f64)    lane l, reg r -> D[l/16 + 4*r][l%16]
 #pragma once
 
+#include <math.h>
+
 #include "common.h"
 
 namespace fmlh {
@@ -226,6 +228,39 @@ __device__ __forceinline__ bool xcd_tile(int gx, int gy, int &tx, int &ty) {
   ty = first + (int)(in % rows);
   tx = (int)(in / rows);
   return true;
+}
+
+// Tile (tx, ty), tx <= ty < g, of the lower triangle of a g x g tile grid
+// at position `local` of this order: groups of kGroup tile rows, each group
+// walked column by column, so a group's row tiles stay in L2 while its
+// column tiles stream past once (the row-major triangle would stream every
+// column tile once per row).  Returns false past the end.
+__host__ __device__ inline bool tri_tile(int g, int64_t local, int &tx, int &ty) {
+  if (local >= (int64_t)g * (g + 1) / 2) return false;
+  // row of `local` in row-major order: T(r) <= local < T(r + 1), T(r) = r(r+1)/2
+  int r = (int)((sqrt(8.0 * (double)local + 1.0) - 1.0) * 0.5);
+  while ((int64_t)(r + 1) * (r + 2) / 2 <= local) ++r;
+  while ((int64_t)r * (r + 1) / 2 > local) --r;
+  const int r0 = (r / kGroup) * kGroup;  // first row of the group (it starts at T(r0))
+  const int R = kGroup < g - r0 ? kGroup : g - r0;
+  int64_t o = local - (int64_t)r0 * (r0 + 1) / 2;
+  const int64_t full = (int64_t)(r0 + 1) * R;  // columns 0..r0 hold R tiles each
+  if (o < full) {
+    tx = (int)(o / R);
+    ty = r0 + (int)(o % R);
+    return true;
+  }
+  o -= full;
+  for (int c = r0 + 1; c < r0 + R; ++c) {  // column c holds rows c..r0+R-1
+    const int cnt = r0 + R - c;
+    if (o < cnt) {
+      tx = c;
+      ty = c + (int)o;
+      return true;
+    }
+    o -= cnt;
+  }
+  return false;
 }
 
 }  // namespace fmlh

@@ -13,7 +13,7 @@ This module is that chain kept on the GPU end to end:
   (iterative refinement is the special case of one-step convergence);
 * Ozaki form (``precision="ozaki"``): a copy of K + lambda I is factored with
   its large updates on INT8 matrix cores (fmlh_potrf_ozaki: an exact
-  six-slice split of each panel, the factor within ~1e-12 of the FP64 one),
+  five-digit split of each panel, the factor within ~1e-12 of the FP64 one),
   then refined with FP64 residuals y - (K + lambda I) alpha until the
   residual reaches tol or stops falling.
 """

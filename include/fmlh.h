@@ -416,12 +416,13 @@ int fmlh_potrf(double *a, int64_t lda, int n, int32_t *info, void *ws,
 /* Precision form of fmlh_potrf (no reference counterpart): the same
  * factorisation, the panels in FP64, with the next-panel and trailing
  * updates computed on INT8 matrix cores from an exact split of each
- * finished 1024-column panel: every row scaled by a power of two and peeled
- * into six signed 7-bit slices (the error-free splitting of Ozaki, Ogita,
- * Oishi and Rump, with integer slices as Mukunoki et al. use on tensor
- * cores), the slice products with t + u <= 7 summed exactly in int32 by
- * v_mfma_i32_16x16x64_i8 and combined in FP64.  An update's error is below
- * about 2^-40 |row| |column| summed over the panel (dropped slice products),
+ * finished 1024-column panel: every row scaled by a power of two, truncated
+ * to a 40-bit integer and written in five signed base-256 digits (the
+ * error-free splitting of Ozaki, Ogita, Oishi and Rump, with integer slices
+ * as Mukunoki et al. use on tensor cores), the 15 digit products with
+ * t + u <= 6 summed exactly in int32 by v_mfma_i32_32x32x32_i8 and combined
+ * in FP64.  An update's error is below about 2^-38 |row| |column| summed
+ * over the panel (dropped digit products and the truncation; zero-mean),
  * so the factor serves as the solver of FP64 iterative refinement
  * (KRR(precision="ozaki")), not as a stand-in for the FP64 factor.
  * ws: fmlh_potrf_ozaki_ws_bytes(n) bytes (slice planes and row exponents). */

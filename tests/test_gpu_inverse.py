@@ -145,9 +145,9 @@ def test_potrf_lookahead_concurrent_threads(gpu):
 
 def test_ozaki_factor_and_refinement(gpu):
     """fmlh_potrf_ozaki: the next-panel and trailing updates from the exact
-    six-slice INT8 split (n = 3000: two panels of look-ahead updates).  The
+    five-digit INT8 split (n = 3000: two panels of look-ahead updates).  The
     factor is within 1e-10 of the FP64 factor and |L L^T - A| / |A| <= 1e-10
-    (the dropped slice products are below ~2^-40 of the row and column
+    (the dropped digit products are below ~2^-38 of the row and column
     scales per update); KRR(precision="ozaki") refines to a relative residual
     no larger than the FP64 direct solve's, and its predictions agree with
     the FP64 model's to 1e-9 relative."""
