@@ -895,7 +895,7 @@ def krr36k_workload(args, precision):
             # the plain FP64 form on the same inputs: time, MAE and alpha
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            m64 = KRR(1000.0, 1e-6).fit(X[:ntr], y[:ntr])
+            m64 = KRR(1000.0, 1e-6).fit(X[:ntr], y[:ntr], keep_kernel=True)
             p64 = m64.predict(X[ntr:])
             torch.cuda.synchronize()
             t64 = time.perf_counter() - t0
@@ -909,8 +909,21 @@ def krr36k_workload(args, precision):
                 "alpha_rel_err": float((torch.linalg.norm(m.alpha - a64) / torch.linalg.norm(a64)).item()),
                 "prediction_max_rel_diff": float(((state["pred"] - p64).abs().max() / p64.abs().max()).item())}
             # lambda_max(K + lambda I) by power iteration (its lambda_min is >= lambda: K is PSD)
-            from fml_amd.krr import kernel_matrix, predict_from_kernel
+            from fml_amd.krr import kernel_matrix, predict_from_kernel, refine_dd
             K = kernel_matrix(X[:ntr], X[:ntr], 1000.0)
+            # the exact solution of the FP64 system: the FP64 factor's solution refined
+            # with double-length residuals (independent of the form being measured)
+            yd = torch.from_numpy(np.ascontiguousarray(y[:ntr])).cuda()
+            ax = a64.clone()
+            dx_ref, _ = refine_dd(K, 1e-6, m64.factor, yd, ax, tol=1e-14, max_iter=30)
+            m64.factor = None
+            axn = torch.linalg.norm(ax)
+            res["vs_exact"] = {
+                "reference": "the FP64 factor's solution refined with double-length residuals",
+                "reference_steps": len(dx_ref), "reference_last_correction_rel": dx_ref[-1],
+                "alpha_rel_err": float((torch.linalg.norm(m.alpha - ax) / axn).item()),
+                "fp64_plain_alpha_rel_err": float((torch.linalg.norm(a64 - ax) / axn).item())}
+            del ax
             v = torch.ones(ntr, dtype=torch.float64, device="cuda") / ntr ** 0.5
             w = torch.empty_like(v)
             lam_max = 0.0
@@ -1069,10 +1082,13 @@ def krr_forms_clause(args):
     """The metric's fit-time clause at the 36k configuration: Gaussian KRR
     fit of 36,000 molecules + prediction of 4,000 (nmax 29, sigma 1000,
     lambda 1e-6) in the plain FP64 form (timed after one untimed run) and
-    in the Ozaki and mixed forms, each against the FP64 model."""
+    in the Ozaki and mixed forms, each against the FP64 model and against
+    the exact solution of the FP64 system (the FP64 factor's solution
+    refined with double-length residuals: a reference independent of the
+    INT8 factor).  Residuals are double-length (exact to working accuracy)."""
     import torch
     from fml_amd.device import to_device
-    from fml_amd.krr import KRR, ConvergenceError, kernel_matrix, predict_from_kernel
+    from fml_amd.krr import KRR, ConvergenceError, kernel_matrix, refine_dd, residual_dd
     from fml_amd.representations import generate_coulomb_matrices
     n, ntr = 40000, 36000
     ms = _cm_set(n, 0, 3, 29, 29)
@@ -1080,12 +1096,13 @@ def krr_forms_clause(args):
                                   to_device(ms.offsets, np.int32), nmax=29, on_device=True)
     y = ms.energies
 
-    def run(prec):
+    def run(prec, keep=False):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         err = None
         try:
-            m = KRR(1000.0, 1e-6, precision=prec, tol=args.tol, max_iter=args.max_iter).fit(X[:ntr], y[:ntr])
+            m = KRR(1000.0, 1e-6, precision=prec, tol=args.tol, max_iter=args.max_iter).fit(
+                X[:ntr], y[:ntr], keep_kernel=keep)
         except ConvergenceError as e:
             m = KRR(1000.0, 1e-6, precision=prec)
             m.X, m.alpha, m.report = X[:ntr].contiguous(), e.alpha, e.report
@@ -1095,7 +1112,7 @@ def krr_forms_clause(args):
         return m, p, time.perf_counter() - t0, err
 
     run("fp64")  # first-call set-up outside the comparison
-    m64, p64, t64, _ = run("fp64")
+    m64, p64, t64, _ = run("fp64", keep=True)
     ph = p64.cpu().numpy()
     mae64 = float(np.mean(np.abs(ph - y[ntr:])))
     K = kernel_matrix(m64.X, m64.X, 1000.0)
@@ -1103,14 +1120,25 @@ def krr_forms_clause(args):
     r = torch.empty_like(yd)
 
     def rel_res(a):
-        predict_from_kernel(K, a, out=r)
-        r.add_(a, alpha=1e-6)
-        return float((torch.linalg.norm(yd - r) / torch.linalg.norm(yd)).item())
+        residual_dd(K, 1e-6, a, yd, r)
+        return float((torch.linalg.norm(r) / torch.linalg.norm(yd)).item())
+
+    # the exact solution of the FP64 system, from the FP64 factor
+    ax = m64.alpha.clone()
+    dx_ref, _ = refine_dd(K, 1e-6, m64.factor, yd, ax, tol=1e-14, max_iter=30)
+    m64.factor = None
+    axn = torch.linalg.norm(ax)
+
+    def err_exact(a):
+        return float((torch.linalg.norm(a - ax) / axn).item())
 
     out = {"workload": "Gaussian KRR fit 36000 + predict 4000 molecules (nmax 29, 40k seed 0), sigma 1000, "
                        "lambda 1e-6; times are fit + predict on HBM-resident features",
+           "exact_alpha": {"reference": "the FP64 factor's solution refined with double-length residuals "
+                                        "(fmlh_residual_dd), independent of the INT8 factor",
+                           "steps": len(dx_ref), "last_correction_rel": dx_ref[-1]},
            "fp64": {"fit_predict_ms": round(t64 * 1e3, 2), "mae": mae64,
-                    "relative_residual": rel_res(m64.alpha)}}
+                    "relative_residual": rel_res(m64.alpha), "alpha_rel_err_vs_exact": err_exact(m64.alpha)}}
     for prec in ("ozaki", "mixed"):
         m, p, t, err = run(prec)
         pe = p.cpu().numpy()
@@ -1119,10 +1147,11 @@ def krr_forms_clause(args):
                      "mae": mae, "mae_rel_diff": abs(mae - mae64) / mae64,
                      "alpha_rel_diff_vs_fp64": float((torch.linalg.norm(m.alpha - m64.alpha)
                                                       / torch.linalg.norm(m64.alpha)).item()),
+                     "alpha_rel_err_vs_exact": err_exact(m.alpha),
                      "relative_residual": rel_res(m.alpha), "iterations": m.report.iterations,
                      "converged": err is None}
         del m, p
-    del K, m64, X
+    del K, m64, X, ax
     torch.cuda.empty_cache()
     return out
 

@@ -284,6 +284,73 @@ __global__ __launch_bounds__(256) void k_gemv_t(const double *__restrict__ k, in
   if (threadIdx.x == 0) y[i] = part[0];
 }
 
+// r_i = y_i - (sum_j K(j,i) x_j + lambda x_i) for K (n, n) Fortran order
+// (column i contiguous), one workgroup per output, every product and sum
+// error-free: the product by TwoProduct (p = k x, e = fma(k, x, -p)) and
+// the running sum as an unevaluated pair (hi, lo) through TwoSum, the
+// accumulation of Ogita, Rump and Oishi ("Accurate sum and dot product",
+// SIAM J. Sci. Comput. 26 (2005), Dot2), so the sum is as accurate as if
+// computed in twice the working precision and then rounded once.  Partial
+// pairs are combined in a fixed tree order.  Contraction is off in all
+// three: fused into an FMA, hi + a * b would not be the rounded sum the
+// error terms assume.
+__device__ __forceinline__ void two_sum(double a, double b, double &s, double &e) {
+#pragma clang fp contract(off)
+  s = a + b;
+  const double z = s - a;
+  e = (a - (s - z)) + (b - z);
+}
+
+__device__ __forceinline__ void dd_add(double &hi, double &lo, double bh, double bl) {
+#pragma clang fp contract(off)
+  double s, e;
+  two_sum(hi, bh, s, e);
+  e += lo + bl;
+  hi = s + e;
+  lo = e - (hi - s);
+}
+
+__global__ __launch_bounds__(256) void k_residual_dd(const double *__restrict__ k, int64_t ldk,
+                                                     int n, double lambda,
+                                                     const double *__restrict__ x,
+                                                     const double *__restrict__ y,
+                                                     double *__restrict__ r) {
+#pragma clang fp contract(off)
+  __shared__ double ph[256], pl[256];
+  const int i = blockIdx.x;
+  const double *col = k + (int64_t)i * ldk;
+  double hi = 0.0, lo = 0.0;
+  for (int j = threadIdx.x; j < n; j += 256) {
+    const double a = col[j], b = x[j];
+    const double p = a * b;
+    const double e = fma(a, b, -p);
+    double s, t;
+    two_sum(hi, p, s, t);
+    hi = s;
+    lo += t + e;
+  }
+  ph[threadIdx.x] = hi;
+  pl[threadIdx.x] = lo;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      double h = ph[threadIdx.x], l = pl[threadIdx.x];
+      dd_add(h, l, ph[threadIdx.x + w], pl[threadIdx.x + w]);
+      ph[threadIdx.x] = h;
+      pl[threadIdx.x] = l;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double h = ph[0], l = pl[0];
+    const double p = lambda * x[i];
+    dd_add(h, l, p, fma(lambda, x[i], -p));  // + lambda x_i, exactly
+    double s, e;
+    two_sum(y[i], -h, s, e);  // y_i - (h + l)
+    r[i] = s + (e - l);
+  }
+}
+
 template <typename T>
 int check_dense_args(const T *a, int64_t lda, int na, const T *b, int64_t ldb, int nb, int d,
                      double sigma, const void *k, int64_t ldk) {
@@ -431,6 +498,21 @@ int fmlh_gemv_t(const double *k, int64_t ldk, int na, int nb, const double *alph
   FMLH_ARG_CHECK(k && alpha && y, 1);
   hipLaunchKernelGGL(k_gemv_t, dim3(nb), dim3(256), 0, as_stream(stream), k, ldk, na, alpha, y);
   FMLH_LAUNCH_CHECK("k_gemv_t");
+  return 0;
+}
+
+int fmlh_residual_dd(const double *k, int64_t ldk, int n, double lambda, const double *x,
+                     const double *y, double *r, void *stream) {
+  FMLH_ARG_CHECK(n >= 0, 3);
+  FMLH_ARG_CHECK(ldk >= n, 2);
+  if (n == 0) return 0;
+  FMLH_ARG_CHECK(k != nullptr, 1);
+  FMLH_ARG_CHECK(x != nullptr, 5);
+  FMLH_ARG_CHECK(y != nullptr, 6);
+  FMLH_ARG_CHECK(r != nullptr && r != x && r != y, 7);
+  hipLaunchKernelGGL(k_residual_dd, dim3(n), dim3(256), 0, as_stream(stream), k, ldk, n, lambda, x,
+                     y, r);
+  FMLH_LAUNCH_CHECK("k_residual_dd");
   return 0;
 }
 

@@ -57,7 +57,12 @@ typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kOzT = 128;     // tile edge
 constexpr int kOzKS = 32;     // k step: bytes of a slice row per MFMA
-constexpr int kOzStages = 3;  // LDS stages (loads two steps ahead)
+// LDS stages, loads kOzStages - 1 steps ahead (at 36k: 3 stages 169 ms per
+// factorisation, 4 stages 190 ms -- four leave no LDS for the exponents and
+// spill more of the epilogue)
+constexpr int kOzStages = 3;
+// the row and column exponents of a tile are staged in LDS when they fit
+constexpr bool kOzExpLds = kOzStages <= 3;
 constexpr int kOzSplitRows = 8;
 
 __host__ __device__ inline size_t oz_plane_bytes(int nrows) {
@@ -164,7 +169,7 @@ __global__ __launch_bounds__(512, 1) void k_oz_update(double *__restrict__ c, in
                                                       const int32_t *__restrict__ info) {
   if (*info != 0) return;
   __shared__ __attribute__((aligned(16))) OzStage<S> st[kOzStages];
-  __shared__ int ea[kOzT], eb[kOzT];
+  __shared__ int ea[kOzExpLds ? kOzT : 1], eb[kOzExpLds ? kOzT : 1];
   int tx, ty;
   if (tri) {
     const int64_t t = (int64_t)gy * (gy + 1) / 2;
@@ -180,10 +185,12 @@ __global__ __launch_bounds__(512, 1) void k_oz_update(double *__restrict__ c, in
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int wm = wave >> 1, wn = wave & 1;
-  if (tid < kOzT)
-    ea[tid] = expo[min(tr + tid, r1 - 1) - srow0];
-  else if (tid < 2 * kOzT)
-    eb[tid - kOzT] = expo[min(tc + tid - kOzT, c1 - 1) - srow0];
+  if (kOzExpLds) {
+    if (tid < kOzT)
+      ea[tid] = expo[min(tr + tid, r1 - 1) - srow0];
+    else if (tid < 2 * kOzT)
+      eb[tid - kOzT] = expo[min(tc + tid - kOzT, c1 - 1) - srow0];
+  }
 
   // this wave's S of the stage's 8 S one-KB blocks: block j = (operand,
   // plane, 32-row group); lane l fills LDS slot l (row 32 q + l / 2, half
@@ -223,17 +230,23 @@ __global__ __launch_bounds__(512, 1) void k_oz_update(double *__restrict__ c, in
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[l][i][r] = 0;
 
-  issue(0, 0);
-  if (nk > 1) issue(1, 1);
+  constexpr int ahead = kOzStages - 1;
+#pragma unroll
+  for (int k = 0; k < ahead; ++k)
+    if (k < nk) issue(k, k);
   for (int kt = 0; kt < nk; ++kt) {
-    // this wave's loads of step kt are done (the S of step kt + 1 may still
-    // be in flight); after the barrier every wave's are, and every wave has
-    // finished step kt - 1, whose stage the loads of step kt + 2 overwrite
-    if (kt + 1 < nk)
+    // this wave's loads of step kt are done (the S loads of each later step
+    // already issued may still be in flight); after the barrier every wave's
+    // are, and every wave has finished step kt - 1, whose stage the loads of
+    // step kt + ahead overwrite
+    const int later = min(ahead - 1, nk - 1 - kt);
+    if (later >= 2)
+      asm volatile("s_waitcnt vmcnt(%0) lgkmcnt(0)\n\ts_barrier" ::"n"(2 * S) : "memory");
+    else if (later == 1)
       asm volatile("s_waitcnt vmcnt(%0) lgkmcnt(0)\n\ts_barrier" ::"n"(S) : "memory");
     else
       asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)\n\ts_barrier" ::: "memory");
-    if (kt + 2 < nk) issue(kt + 2, (kt + 2) % kOzStages);
+    if (kt + ahead < nk) issue(kt + ahead, (kt + ahead) % kOzStages);
     const OzStage<S> &s = st[kt % kOzStages];
     i32x4 af[S];
 #pragma unroll
@@ -260,6 +273,7 @@ __global__ __launch_bounds__(512, 1) void k_oz_update(double *__restrict__ c, in
 #pragma unroll
   for (int i = 0; i < 2; ++i) {
     const int cl = wn * 64 + i * 32 + (lane & 31);
+    const int gc = tc + cl;
     double v[16], old[16];
     uint32_t off[16];
 #pragma unroll
@@ -268,7 +282,9 @@ __global__ __launch_bounds__(512, 1) void k_oz_update(double *__restrict__ c, in
       double x = 0.0;
 #pragma unroll
       for (int l = S - 1; l >= 0; --l) x += ldexp((double)acc[l][i][r], -8 * (l + 2));
-      v[r] = ldexp(x, ea[rl] + eb[cl]);
+      const int e2 = kOzExpLds ? ea[rl] + eb[cl]
+                               : expo[min(tr + rl, r1 - 1) - srow0] + expo[min(gc, c1 - 1) - srow0];
+      v[r] = ldexp(x, e2);
       off[r] = (uint32_t)(min(rl, rmax) * (int)lda + min(cl, cmax));
     }
 #pragma unroll

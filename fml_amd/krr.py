@@ -14,8 +14,10 @@ This module is that chain kept on the GPU end to end:
 * Ozaki form (``precision="ozaki"``): a copy of K + lambda I is factored with
   its large updates on INT8 matrix cores (fmlh_potrf_ozaki: an exact
   five-digit split of each panel, the factor within ~1e-12 of the FP64 one),
-  then refined with FP64 residuals y - (K + lambda I) alpha until the
-  residual reaches tol or stops falling.
+  then refined with residuals y - (K + lambda I) alpha computed in double
+  length (fmlh_residual_dd) until alpha's correction falls below tol: alpha
+  converges to the solution of the FP64 system to working accuracy, where a
+  plain FP64 solve is off by up to kappa times the unit roundoff.
 """
 from __future__ import annotations
 
@@ -53,6 +55,39 @@ def sym_residual(K, lam, x, y, r):
     _native.call("fmlh_sym_residual", _native.ptr(K), K.stride(0), n, float(lam), _native.ptr(x),
                  _native.ptr(y), _native.ptr(r), _native.stream_ptr())
     return r
+
+
+def residual_dd(K, lam, x, y, r):
+    """r = y - (K + lam I) x for the full symmetric device kernel K (n, n),
+    each entry accumulated in double length and rounded once."""
+    n = int(K.shape[0])
+    _native.call("fmlh_residual_dd", _native.ptr(K), K.stride(0), n, float(lam), _native.ptr(x),
+                 _native.ptr(y), _native.ptr(r), _native.stream_ptr())
+    return r
+
+
+def refine_dd(K, lam, F, y, x, tol=1e-12, max_iter=30):
+    """Iterative refinement of x in place for (K + lam I) x = y with the
+    factor F (fmlh_potrf layout) as the solver and double-length residuals:
+    x += F^-1 (y - (K + lam I) x) until the correction's norm is at most tol
+    times |x| or stops halving (the working-precision floor).  Returns
+    (history of |correction| / |x|, history of |residual| / |y|), one host
+    read per step."""
+    t = torch()
+    r = empty((int(K.shape[0]),), np.float64)
+    ynorm = t.linalg.norm(y)
+    dx, res = [], []
+    for _ in range(max_iter):
+        residual_dd(K, lam, x, y, r)
+        rn = t.linalg.norm(r) / ynorm
+        potrs_inplace(F, r)
+        x.add_(r)
+        v = t.stack([t.linalg.norm(r) / t.linalg.norm(x), rn]).cpu()  # one host read per step
+        dx.append(float(v[0]))
+        res.append(float(v[1]))
+        if dx[-1] <= tol or (len(dx) > 1 and dx[-1] > 0.5 * dx[-2]):
+            break
+    return dx, res
 
 
 def predict_from_kernel(Kt, alpha, out=None):
@@ -245,13 +280,12 @@ class KRR:
 
     def _fit_ozaki(self, K, yd, rep):
         """Factor of K + lambda I with INT8-core updates (an unfactored K is
-        kept for the residuals), then iterative refinement: alpha += F^-1 r,
-        r = y - (K + lambda I) alpha in FP64.  Stops when |r| <= tol |y|, or
-        when a step no longer halves |r| (the FP64 floor of the residual);
-        raises ConvergenceError if that floor is above 1e-8 |y| or max_iter
-        steps did not reach it."""
+        kept for the residuals), then iterative refinement with double-length
+        residuals (refine_dd) until alpha's correction is at most tol times
+        |alpha|.  Raises ConvergenceError if the refinement stalls (or runs
+        out of steps) with a correction above 1e-8 |alpha|: the factor is then
+        too inexact for the system's conditioning."""
         t = torch()
-        n = int(K.shape[0])
         t1 = time.perf_counter()
         F = K.clone()
         add_diagonal(F, self.lam)
@@ -261,40 +295,20 @@ class KRR:
         if info != 0:
             raise np.linalg.LinAlgError(f"K + lambda I is not positive definite (leading minor {info})")
         t2 = time.perf_counter()
-        r = empty((n,), np.float64)
-
-        def residual(v, out):  # y - (K + lam I) v
-            predict_from_kernel(K, v, out=out)
-            out.add_(v, alpha=self.lam)
-            out.neg_().add_(yd)
-            return out
-
         x = yd.clone()
         potrs_inplace(F, x)
-        ynorm = float(t.linalg.norm(yd).item())
-        tol = float(self.tol)
-        converged = False
-        prev = None
-        for _ in range(self.max_iter):
-            residual(x, r)
-            rel = float(t.linalg.norm(r).item()) / ynorm  # one host read per step
-            rep.history.append(rel)
-            if rel <= tol or (prev is not None and rel > 0.5 * prev):
-                converged = rel <= max(tol, 1e-8)
-                break
-            prev = rel
-            potrs_inplace(F, r)
-            x.add_(r)
-        rep.iterations = len(rep.history) - 1
-        rep.residual = rep.history[-1] if rep.history else float("nan")
+        dx, res = refine_dd(K, self.lam, F, yd, x, tol=float(self.tol), max_iter=self.max_iter)
+        rep.history = dx
+        rep.iterations = len(dx)
+        rep.residual = res[-1] if res else float("nan")
         t.cuda.synchronize()
         rep.solve_s = time.perf_counter() - t2
         del F
-        if not converged:
+        if not dx or dx[-1] > max(float(self.tol), 1e-8):
             raise ConvergenceError(
-                f"Ozaki-factor refinement stopped at relative residual {rep.residual:.3g} after "
-                f"{rep.iterations} steps (the factor is too inexact for this system's conditioning); "
-                "use precision='fp64'", alpha=x, report=rep)
+                f"Ozaki-factor refinement stopped with alpha's correction at {dx[-1] if dx else float('nan'):.3g} "
+                f"of |alpha| after {rep.iterations} steps (the factor is too inexact for this system's "
+                "conditioning); use precision='fp64'", alpha=x, report=rep)
         return x
 
     # -- predict -----------------------------------------------------------

@@ -484,6 +484,14 @@ int fmlh_add_diagonal(double *a, int64_t lda, int n, double lambda,
 int fmlh_sym_residual(const double *a, int64_t lda, int n, double lambda,
                       const double *x, const double *y, double *r,
                       void *stream);
+/* r = y - (K + lambda I) x for the full symmetric K (n, n) Fortran order,
+ * each r(i) from error-free products and a double-length running sum
+ * (Ogita-Rump-Oishi Dot2), rounded once: the residual of iterative
+ * refinement in extra precision, so the refined x converges to the solution
+ * of the FP64 system to working accuracy instead of to its condition
+ * number times the unit roundoff.  r must not alias x or y. */
+int fmlh_residual_dd(const double *k, int64_t ldk, int n, double lambda,
+                     const double *x, const double *y, double *r, void *stream);
 /* y(i) = sum_j K(j, i) * alpha(j) for K (na, nb) Fortran order: the
  * prediction K_test^T alpha. */
 int fmlh_gemv_t(const double *k, int64_t ldk, int na, int nb,

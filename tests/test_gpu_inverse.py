@@ -15,6 +15,7 @@ import numpy as np
 import pytest
 
 import oracle
+from exact_residual import exact_refined_solution, exact_residual
 from fml_amd import synth
 from fml_amd.arad import ARAD
 from fml_amd.kernels.kernels import arad_types
@@ -188,7 +189,54 @@ def test_ozaki_factor_and_refinement(gpu):
     r64, roz = rel_res(m64.alpha), rel_res(moz.alpha)
     p64, poz = m64.predict(X[nt:]), moz.predict(X[nt:])
     pdiff = float((p64 - poz).abs().max() / p64.abs().max())
+    # alpha against the solution of the same FP64 system, refined on the CPU
+    # with exactly rounded residuals
+    xs = exact_refined_solution(K.cpu().numpy(), 1e-6, y[:nt])
+    e_oz = float(np.linalg.norm(moz.alpha.cpu().numpy() - xs) / np.linalg.norm(xs))
+    e_64 = float(np.linalg.norm(m64.alpha.cpu().numpy() - xs) / np.linalg.norm(xs))
+    # the same refinement from the FP64 factor, on the device
+    from fml_amd.krr import refine_dd
+    F64 = K.clone()
+    F64.diagonal().add_(1e-6)
+    assert potrf_inplace(F64) == 0
+    a2 = m64.alpha.clone()
+    h2, _ = refine_dd(K, 1e-6, F64, yd, a2, tol=1e-14)
+    e_2 = float(np.linalg.norm(a2.cpu().numpy() - xs) / np.linalg.norm(xs))
     print(f"KRR 2600: relative residual FP64 {r64:.3g}, Ozaki refined {roz:.3g} "
-          f"({moz.report.iterations} steps), prediction diff {pdiff:.3g}")
+          f"({moz.report.iterations} steps: corrections {moz.report.history}), prediction diff "
+          f"{pdiff:.3g}; alpha vs the exact solution: Ozaki {e_oz:.3g}, plain FP64 {e_64:.3g}, "
+          f"FP64 factor refined on the device {e_2:.3g} (corrections {h2})")
     assert roz <= max(r64, 1e-12)
     assert pdiff <= 1e-9
+    assert e_oz <= 1e-8
+
+
+def test_residual_dd_against_exact(gpu):
+    """fmlh_residual_dd equals the exactly rounded residual to a few units
+    in the last place (Dot2 bound) on a KRR system whose FP64 residual is
+    dominated by cancellation, and differs from the plain FP64 residual."""
+    torch = gpu
+    from fml_amd import synth
+    from fml_amd.krr import kernel_matrix, residual_dd, sym_residual
+    from fml_amd.math.cho_solve import potrf_inplace, potrs_inplace
+    from fml_amd.representations import generate_coulomb_matrices
+    ms = synth.generate(1500, seed=7, nmin=3, nmax=29)
+    X = torch.from_numpy(generate_coulomb_matrices(ms.charges, ms.coords, ms.offsets, nmax=29)).cuda()
+    K = kernel_matrix(X, X, 1000.0)
+    n = int(K.shape[0])
+    y = torch.from_numpy(ms.energies.copy()).cuda()
+    F = K.clone()
+    F.diagonal().add_(1e-6)
+    assert potrf_inplace(F) == 0
+    x = y.clone()
+    potrs_inplace(F, x)
+    r = torch.empty_like(y)
+    residual_dd(K, 1e-6, x, y, r)
+    ex = exact_residual(K.cpu().numpy(), 1e-6, x.cpu().numpy(), ms.energies)
+    err = float(np.max(np.abs(r.cpu().numpy() - ex)) / np.max(np.abs(ex)))
+    r64 = torch.empty_like(y)
+    sym_residual(K, 1e-6, x, y, r64)
+    err64 = float(np.max(np.abs(r64.cpu().numpy() - ex)) / np.max(np.abs(ex)))
+    print(f"\nresidual n={n}: double-length vs exact {err:.3g}, plain FP64 vs exact {err64:.3g}")
+    assert err <= 1e-14
+    assert err64 > 10 * err

@@ -181,3 +181,35 @@ def test_fml_import_path():
     m = fml.Molecule()
     m.read_xyz(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "xyz", "01.xyz"))
     assert m.natoms > 0
+
+
+def test_exact_residual_helper():
+    """The exactly rounded residual used to check refined KRR solutions
+    equals y - (A + lam I) x evaluated in rational arithmetic, rounded once;
+    the refined solution it yields solves a nearly singular system to
+    working accuracy (its exact residual is below one unit in the last
+    place of the terms)."""
+    from fractions import Fraction
+    from exact_residual import exact_refined_solution, exact_residual, two_product
+    rng = np.random.default_rng(3)
+    a, b = rng.normal(size=300) * 1e4, rng.normal(size=300)
+    p, e = two_product(a, b)
+    assert all(Fraction(p[i]) + Fraction(e[i]) == Fraction(a[i]) * Fraction(b[i]) for i in range(300))
+    n = 60
+    B = rng.normal(size=(n, n))
+    A = B @ B.T / n
+    y = rng.normal(size=n)
+    x = np.linalg.solve(A + 1e-3 * np.eye(n), y)
+    r = exact_residual(A, 1e-3, x, y)
+    lam = Fraction(1e-3)
+    for i in range(n):
+        ex = Fraction(y[i]) - sum(Fraction(A[i, j]) * Fraction(x[j]) for j in range(n)) - lam * Fraction(x[i])
+        assert r[i] == float(ex)
+    # kappa ~ 1e12: plain LAPACK's solution is off by ~1e-5, the refined one is not
+    w = np.linspace(1.0, 1e-12, n)
+    Q, _ = np.linalg.qr(rng.normal(size=(n, n)))
+    A2 = (Q * w) @ Q.T
+    A2 = (A2 + A2.T) / 2
+    xs = exact_refined_solution(A2, 0.0, y)
+    rs = exact_residual(A2, 0.0, xs, y)
+    assert np.max(np.abs(rs)) <= 1e-15 * np.max(np.abs(A2) @ np.abs(xs))
