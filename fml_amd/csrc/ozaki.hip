@@ -32,9 +32,13 @@
 // source address each lane loads.  One barrier a step: a wave waits for its
 // own loads of the step (s_waitcnt vmcnt leaving the next step's S loads in
 // flight), and the barrier makes every wave's loads visible and frees the
-// stage the next loads overwrite.  The epilogue forms sum_l acc_l 2^-8(l+2)
-// in FP64 (each term exact, smallest level first) and subtracts it, scaled
-// by 2^(e_r + e_c), from the FP64 entry.
+// stage the next loads overwrite; a step's loads are issued one per group
+// of MFMAs, and each B fragment is read one group ahead of its MFMAs.  The
+// epilogue forms sum_l acc_l 2^-8(l+2) in FP64 (each term exact, smallest
+// level first) and subtracts it, scaled by 2^(e_r + e_c), from the FP64
+// entry.  Measured alone on a 32768 x 32768 lower triangle: 7.5 ms (2.2
+// POP/s of tile work); without the loads 5.5 ms, the MFMAs and barriers
+// alone 4.1 ms (4.0 POP/s) -- the INT8 MFMA pipe is not the limit.
 //
 // int8 32x32x32 MFMA lane layout on gfx950 (checked with an in-bounds
 // probe): A lane l -> row l % 32, k bytes 16 (l / 32) + e; B lane l ->
@@ -153,11 +157,17 @@ __device__ __forceinline__ int oz_half(int r, int h) { return h ^ (((r >> 2) ^ (
 // Issued from asm: the compiler, which cannot tell the three stages apart,
 // would otherwise wait for every DMA in flight before each LDS read; the
 // kernel waits for its own loads with s_waitcnt vmcnt instead.  The
-// instruction has no result registers (its data goes to LDS).
+// instruction has no result registers (its data goes to LDS).  M0 holds the
+// LDS address; it is declared clobbered, and no other code of this file
+// uses M0 (the compiler sets M0 only right before its own uses, of which
+// k_oz_update has none), so the reserved-register warning is silenced here.
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Winline-asm"
 __device__ __forceinline__ void oz_dma16(const void *src, uint32_t lds) {
   asm volatile("s_mov_b32 m0, %1\n\tglobal_load_lds_dwordx4 %0, off" ::"v"(src), "s"(lds)
                : "memory", "m0");
 }
+#pragma clang diagnostic pop
 
 template <int S>
 __global__ __launch_bounds__(512, 1) void k_oz_update(double *__restrict__ c, int64_t lda, int r0,
@@ -246,14 +256,25 @@ __global__ __launch_bounds__(512, 1) void k_oz_update(double *__restrict__ c, in
       asm volatile("s_waitcnt vmcnt(%0) lgkmcnt(0)\n\ts_barrier" ::"n"(S) : "memory");
     else
       asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)\n\ts_barrier" ::: "memory");
-    if (kt + ahead < nk) issue(kt + ahead, (kt + ahead) % kOzStages);
+    // this step's S loads of step kt + ahead are issued one per u step below,
+    // between the MFMA groups, rather than in a burst
+    const bool more = kt + ahead < nk;
+    const uint32_t nbase = lds0 + (uint32_t)(((kt + ahead) % kOzStages) * (int)sizeof(OzStage<S>));
+    const int64_t koff = (int64_t)(kt + ahead) * kstride;
     const OzStage<S> &s = st[kt % kOzStages];
     i32x4 af[S];
 #pragma unroll
     for (int t = 0; t < S; ++t) af[t] = frag(s.a[t], wm * 32);
+    // B fragments one u step ahead of their MFMAs
+    i32x4 bq0 = frag(s.b[0], wn * 64), bq1 = frag(s.b[0], wn * 64 + 32);
 #pragma unroll
     for (int u = 0; u < S; ++u) {
-      const i32x4 b0 = frag(s.b[u], wn * 64), b1 = frag(s.b[u], wn * 64 + 32);
+      const i32x4 b0 = bq0, b1 = bq1;
+      if (u + 1 < S) {
+        bq0 = frag(s.b[u + 1], wn * 64);
+        bq1 = frag(s.b[u + 1], wn * 64 + 32);
+      }
+      if (more) oz_dma16(src[u] + koff, nbase + (uint32_t)dst[u]);
       // levels t + u < S; t descending keeps two MFMAs on one accumulator
       // a whole u step apart
 #pragma unroll
@@ -263,19 +284,20 @@ __global__ __launch_bounds__(512, 1) void k_oz_update(double *__restrict__ c, in
       }
     }
   }
-  // epilogue: level l (0-based t + u) carries 2^-8(l + 2).  Per 32 x 32
-  // block, the lane's 16 values are formed first (freeing the block's
-  // accumulators), then its 16 old entries are loaded before the first use
-  // (32-bit element offsets from the tile's first entry, clamped into the
-  // region), then the entries inside the region are stored.
+  // epilogue: level l (0-based t + u) carries 2^-8(l + 2).  First every
+  // value of the lane is formed (freeing all accumulators), then per 32 x 32
+  // block its 16 old entries are loaded before the first use and the
+  // entries inside the region stored, non-temporal (C is streamed once per
+  // update).  Byte offsets from the tile's first entry are 32-bit (128 rows
+  // of lda) and clamped into the region.  The scheduling barrier keeps the
+  // loads from being hoisted among the conversions, where they would spill.
   double *tile = c + (int64_t)tr * lda + tc;
   const int rmax = r1 - 1 - tr, cmax = c1 - 1 - tc;
+  double v[2][16];
 #pragma unroll
   for (int i = 0; i < 2; ++i) {
     const int cl = wn * 64 + i * 32 + (lane & 31);
     const int gc = tc + cl;
-    double v[16], old[16];
-    uint32_t off[16];
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
       const int rl = wm * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
@@ -284,15 +306,28 @@ __global__ __launch_bounds__(512, 1) void k_oz_update(double *__restrict__ c, in
       for (int l = S - 1; l >= 0; --l) x += ldexp((double)acc[l][i][r], -8 * (l + 2));
       const int e2 = kOzExpLds ? ea[rl] + eb[cl]
                                : expo[min(tr + rl, r1 - 1) - srow0] + expo[min(gc, c1 - 1) - srow0];
-      v[r] = ldexp(x, e2);
-      off[r] = (uint32_t)(min(rl, rmax) * (int)lda + min(cl, cmax));
+      v[i][r] = ldexp(x, e2);
     }
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  const int rl0 = wm * 32 + 4 * (lane >> 5);
+  auto offs = [&](int i, int r) -> uint32_t {
+    const int rl = rl0 + 8 * (r >> 2) + (r & 3);
+    const int cl = wn * 64 + i * 32 + (lane & 31);
+    return (uint32_t)(min(rl, rmax) * (int)lda + min(cl, cmax)) * 8u;
+  };
+  auto at = [&](uint32_t o) { return reinterpret_cast<double *>(reinterpret_cast<char *>(tile) + o); };
 #pragma unroll
-    for (int r = 0; r < 16; ++r) old[r] = tile[off[r]];
+  for (int i = 0; i < 2; ++i) {
+    double old[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) old[r] = __builtin_nontemporal_load(at(offs(i, r)));
+    const int cl = wn * 64 + i * 32 + (lane & 31);
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
-      const int rl = wm * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-      if (rl <= rmax && cl <= cmax && tc + cl <= tr + rl) tile[off[r]] = old[r] - v[r];
+      const int rl = rl0 + 8 * (r >> 2) + (r & 3);
+      if (rl <= rmax && cl <= cmax && tc + cl <= tr + rl)
+        __builtin_nontemporal_store(old[r] - v[i][r], at(offs(i, r)));
     }
   }
 }
