@@ -18,8 +18,9 @@ host threads).
 
 Other workloads: arad4096 / arad4096_fp32 (4096 x 4096 uneven molecules),
 krr (1000-molecule FP64 KRR), gaussian64 / gaussian32 / gaussianbf16x3 /
-laplacian64 / laplacian32 (36k x 36k kernels), cholesky64 / cholesky32,
-krr36k_fp64 / krr36k_mixed, serving, copy (counter calibration).
+laplacian64 / laplacian32 (36k x 36k kernels), cholesky64 / cholesky32 /
+cholesky_oz, krr36k_fp64 / krr36k_mixed / krr36k_ozaki, serving, copy
+(counter calibration).
 """
 from __future__ import annotations
 
@@ -64,7 +65,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="arad",
                     choices=["arad", "krr", "gaussian64", "gaussian32", "gaussianbf16x3",
-                             "laplacian64", "laplacian32", "cholesky64", "cholesky32",
+                             "laplacian64", "laplacian32", "cholesky64", "cholesky32", "cholesky_oz",
                              "krr36k_fp64", "krr36k_mixed", "krr36k_ozaki", "serving", "arad4096", "arad4096_fp32", "copy",
                              "atomic", "atomic_lap", "atomic_mfma", "aras", "inverse64", "solve64", "solve32"])
     ap.add_argument("--duration", type=float, default=5.0, help="serving run length in seconds")
@@ -678,11 +679,14 @@ def copy_workload(args):
 
 
 def cholesky_workload(args, kind):
-    """Cholesky of K + lambda I for the 36k Gaussian kernel (lambda 1e-6)."""
+    """Cholesky of K + lambda I for the 36k Gaussian kernel (lambda 1e-6):
+    FP64 (cholesky64), FP32 of the shifted matrix (cholesky32) or the Ozaki
+    form with its updates on INT8 cores (cholesky_oz, the factor of the Ozaki
+    KRR fit)."""
     import torch
     from fml_amd.device import to_device
     from fml_amd.krr import add_diagonal, kernel_matrix
-    from fml_amd.math.cho_solve import potrf_inplace
+    from fml_amd.math.cho_solve import potrf_inplace, potrf_ozaki_inplace
     from fml_amd.representations import generate_coulomb_matrices
     n = args.n or 36000
     ms = _cm_set(n, 0, 3, 29, 29)
@@ -697,16 +701,24 @@ def cholesky_workload(args, kind):
     work = torch.empty_like(K)
     state = {}
 
+    factor = potrf_ozaki_inplace if kind == "cholesky_oz" else potrf_inplace
+
     def step():
         work.copy_(K)
-        state["info"] = potrf_inplace(work, sync=False)
+        state["info"] = factor(work, sync=False)
 
-    info = {"metric": "Cholesky TFLOP/s", "unit": "TFLOP/s", "dtype": "f64" if kind == "cholesky64" else "f32",
+    info = {"metric": "Cholesky TFLOP/s", "unit": "TFLOP/s",
+            "dtype": {"cholesky64": "f64", "cholesky32": "f32", "cholesky_oz": "i8+f64"}[kind],
             "config": {"workload": f"{kind}: potrf of the {n}x{n} Gaussian K + lambda I"}}
     flops = n ** 3 / 3.0
 
     def roofline(t):
         ach = flops / t / 1e12
+        if kind == "cholesky_oz":
+            # 15 INT8 digit products per FP64-equivalent multiply-add of the updates
+            return {"bound": "mfma", "achieved": round(ach * 15 / 1e3, 3), "peak": PEAK_INT8_POPS,
+                    "unit": "POP/s", "frac": round(ach * 15 / 1e3 / PEAK_INT8_POPS, 4), "traffic": None,
+                    "note": "n^3/3 FP64-equivalent flop x 15 digit products over the whole factorisation"}
         peak = PEAK_FP64_TFLOPS if kind == "cholesky64" else PEAK_FP32_TFLOPS
         return {"bound": "mfma", "achieved": round(ach, 2), "peak": peak, "unit": "TFLOP/s",
                 "frac": round(ach / peak, 4), "traffic": None}
