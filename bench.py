@@ -69,6 +69,7 @@ def parse():
                              "krr36k_fp64", "krr36k_mixed", "krr36k_ozaki", "serving", "arad4096", "arad4096_fp32", "copy",
                              "atomic", "atomic_lap", "atomic_mfma", "aras", "inverse64", "solve64", "solve32"])
     ap.add_argument("--duration", type=float, default=5.0, help="serving run length in seconds")
+    ap.add_argument("--rate", type=float, default=2000.0, help="serving: Poisson predict arrivals per second")
     ap.add_argument("--n", type=int, default=0, help="problem size override for the 36k workloads")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-clauses", action="store_true",
@@ -1004,7 +1005,7 @@ def serving_workload(args):
     pool_mols = [pool.molecule(i) for i in range(pool.nmol)]
     rng = np.random.default_rng(2)
     duration = args.duration
-    rate = 2000.0
+    rate = float(getattr(args, "rate", 2000.0))
     srv = KRRServer(X[:ntr], m.alpha, 1000.0, nmax=29, max_batch_molecules=16384).start()
     # fit requests: sizes drawn up front (their own stream of numbers) and the
     # float32 training rows prepared on the host before the timed loop
@@ -1085,7 +1086,7 @@ def serving_workload(args):
             "overlapping_fits": lat_fit, "outside_fits": lat_nofit,
             "roofline": roof, "cpu_baseline": cpu,
             "higher_is_better": False, "dtype": "f64", "data": "synthetic",
-            "config": {"workload": f"serving: {ntr}-molecule Gaussian KRR model resident, Poisson 2000 req/s, "
+            "config": {"workload": f"serving: {ntr}-molecule Gaussian KRR model resident, Poisson {rate:g} req/s, "
                                    "1-256 molecules per request, fit requests 1k-16k admitted by free HBM",
                        "duration_s": duration}}
 
@@ -1179,12 +1180,16 @@ def metric_clauses(args):
             if fn is not None:
                 out[name] = fn(args)
             else:
+                keys = ("p50_ms", "p99_ms", "mean_ms", "requests", "throughput_req_s", "mean_batch_requests",
+                        "fits", "overlapping_fits", "outside_fits", "cpu_baseline", "config")
                 sa = argparse.Namespace(**vars(args))
-                sa.duration, sa.n = args.clause_duration, 0
+                sa.duration, sa.n, sa.rate = args.clause_duration, 0, 2000.0
                 srv = serving_workload(sa)
-                out[name] = {k: srv[k] for k in ("p50_ms", "p99_ms", "mean_ms", "requests", "throughput_req_s",
-                                                 "mean_batch_requests", "fits", "overlapping_fits",
-                                                 "outside_fits", "cpu_baseline", "config") if k in srv}
+                out[name] = {k: srv[k] for k in keys if k in srv}
+                # the same loop at four times the arrival rate: continuous batching under load
+                sa.duration, sa.rate = 2.0, 8000.0
+                srv = serving_workload(sa)
+                out[name]["at_8000_req_s"] = {k: srv[k] for k in keys[:8] if k in srv}
         except Exception as e:  # a clause that fails is reported, the headline line still prints
             out[name] = {"error": f"{type(e).__name__}: {e}"}
         torch.cuda.empty_cache()
