@@ -46,14 +46,28 @@ __device__ __forceinline__ double gauss_value(T d2, double inv) {
 // exponent is formed in double for double output.  The distance of an
 // entry does not depend on MT: the dot product runs over k in the same
 // 4-wide MFMA steps and the row norms are sequential fused sums.
-template <typename T, typename OutT, int MT>
+// kSym (A and B the same rows): only the lower-triangle tiles run, in
+// tri_tile order, and an off-diagonal tile stores each value at (m, n) and
+// (n, m).  An entry is bitwise the same either way round -- the MFMA sums
+// the same products over k in the same order, the norms' sum commutes and
+// a near pair's sum of squared differences is symmetric -- so the kernel
+// equals the one the full grid writes, at half the dot products.
+template <typename T, typename OutT, int MT, bool kSym = false>
 __global__ __launch_bounds__(MT == 128 ? 512 : 256, (MT == 128 ? 2 : 1)) void k_gaussian(const T *__restrict__ A, int64_t lda, int na,
                                                   const T *__restrict__ B, int64_t ldb, int nb,
                                                   int d, double inv, OutT *__restrict__ K,
                                                   int64_t ldk, T tau, int gx, int gy) {
   __shared__ TileSmem<T, MT> s;
   int tx, ty;
-  if (!xcd_tile(gx, gy, tx, ty)) return;
+  if (kSym) {
+    const int64_t t = (int64_t)gy * (gy + 1) / 2;
+    const int64_t per = (t + kXcds - 1) / kXcds;
+    const int64_t L = blockIdx.x;
+    if (!tri_tile(gy, (L % kXcds) * per + L / kXcds, tx, ty)) return;
+  } else if (!xcd_tile(gx, gy, tx, ty)) {
+    return;
+  }
+  const bool mirror = kSym && tx != ty;
   const int n0 = tx * MT;
   const int m0 = ty * MT;
   constexpr int WR = MT == 128 ? 4 : 2;
@@ -87,7 +101,9 @@ __global__ __launch_bounds__(MT == 128 ? 512 : 256, (MT == 128 ? 2 : 1)) void k_
         }
         if (valid && !near) {
           if (d2 < T(0)) d2 = T(0);
-          kstore(K + (int64_t)gn + (int64_t)gm * ldk, gauss_value<T, OutT>(d2, inv));
+          const double kv = gauss_value<T, OutT>(d2, inv);
+          kstore(K + (int64_t)gn + (int64_t)gm * ldk, kv);
+          if (mirror) kstore(K + (int64_t)gm + (int64_t)gn * ldk, kv);
         }
       }
   if (steps) {
@@ -97,7 +113,9 @@ __global__ __launch_bounds__(MT == 128 ? 512 : 256, (MT == 128 ? 2 : 1)) void k_
       gn = n0 + tile_wn() * (MT / 2) + in * 16 + (L & 15);
     };
     auto put = [&](int gn, int gm, T d2) {
-      kstore(K + (int64_t)gn + (int64_t)gm * ldk, gauss_value<T, OutT>(d2, inv));
+      const double kv = gauss_value<T, OutT>(d2, inv);
+      kstore(K + (int64_t)gn + (int64_t)gm * ldk, kv);
+      if (mirror) kstore(K + (int64_t)gm + (int64_t)gn * ldk, kv);
     };
     recompute_near_deferred<T>(steps, masks, idx, A, lda, B, ldb, d, put);
   }
@@ -113,7 +131,13 @@ inline int gauss_tile(int na, int nb) {
 template <typename T, typename OutT>
 void launch_gaussian(const T *a, int64_t lda, int na, const T *b, int64_t ldb, int nb, int d,
                      double inv, OutT *k, int64_t ldk, T tau, hipStream_t st) {
-  if (gauss_tile(na, nb) == 128) {
+  if (a == b && na == nb && lda == ldb && gauss_tile(na, nb) == 128) {
+    // a kernel of a set with itself: lower-triangle tiles, mirrored stores
+    const int g = ceil_div(na, 128);
+    const int64_t t = (int64_t)g * (g + 1) / 2;
+    hipLaunchKernelGGL((k_gaussian<T, OutT, 128, true>), dim3((unsigned)xcd_grid((int)t, 1)),
+                       dim3(512), 0, st, a, lda, na, b, ldb, nb, d, inv, k, ldk, tau, g, g);
+  } else if (gauss_tile(na, nb) == 128) {
     const int gx = ceil_div(na, 128), gy = ceil_div(nb, 128);
     hipLaunchKernelGGL((k_gaussian<T, OutT, 128>), dim3((unsigned)xcd_grid(gx, gy)), dim3(512), 0,
                        st, a, lda, na, b, ldb, nb, d, inv, k, ldk, tau, gx, gy);
