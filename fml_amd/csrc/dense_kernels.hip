@@ -155,10 +155,13 @@ void launch_gaussian(const T *a, int64_t lda, int na, const T *b, int64_t ldb, i
 // tile).  Operands are staged k-major through two LDS buffers, kLapBK
 // columns at a time.  The launcher uses the 64 tile: the 128 tile (8 x 8
 // outputs per thread) needs 210 registers, runs two waves per SIMD and is
-// no faster.
+// no faster.  kSym (A and B the same rows): lower-triangle tiles only, each
+// off-diagonal value stored at (m, n) and (n, m) -- |a - b| and |b - a| are
+// the same number and the sum runs over k in the same order, so the kernel
+// is bitwise the one the full grid writes.
 constexpr int kLapBK = 16;
 
-template <typename T, typename OutT, int MT>
+template <typename T, typename OutT, int MT, bool kSym = false>
 __global__ __launch_bounds__(256, (MT == 128 ? 2 : 1)) void k_laplacian(
     const T *__restrict__ A, int64_t lda, int na, const T *__restrict__ B, int64_t ldb, int nb,
     int d, double inv, OutT *__restrict__ K, int64_t ldk, int gx, int gy) {
@@ -170,7 +173,15 @@ __global__ __launch_bounds__(256, (MT == 128 ? 2 : 1)) void k_laplacian(
   __shared__ T As[2][kLapBK * kStride];
   __shared__ T Bs[2][kLapBK * kStride];
   int bx, by;
-  if (!xcd_tile(gx, gy, bx, by)) return;
+  if (kSym) {
+    const int64_t t = (int64_t)gy * (gy + 1) / 2;
+    const int64_t per = (t + kXcds - 1) / kXcds;
+    const int64_t L = blockIdx.x;
+    if (!tri_tile(gy, (L % kXcds) * per + L / kXcds, bx, by)) return;
+  } else if (!xcd_tile(gx, gy, bx, by)) {
+    return;
+  }
+  const bool mirror = kSym && bx != by;
   const int tid = threadIdx.x;
   const int tx = tid & 15, ty = tid >> 4;
   const int n0 = bx * MT, m0 = by * MT;
@@ -236,6 +247,7 @@ __global__ __launch_bounds__(256, (MT == 128 ? 2 : 1)) void k_laplacian(
       else
         v = (double)kexp((T)((T)inv * acc[i][j]));
       kstore(K + (int64_t)gn + (int64_t)gm * ldk, v);
+      if (mirror) kstore(K + (int64_t)gm + (int64_t)gn * ldk, v);
     }
   }
 }
@@ -244,6 +256,12 @@ template <typename T, typename OutT>
 void launch_laplacian(const T *a, int64_t lda, int na, const T *b, int64_t ldb, int nb, int d,
                       double inv, OutT *k, int64_t ldk, hipStream_t st) {
   const int gx = ceil_div(na, 64), gy = ceil_div(nb, 64);
+  if (a == b && na == nb && lda == ldb) {  // a set with itself: mirrored lower triangle
+    const int64_t t = (int64_t)gy * (gy + 1) / 2;
+    hipLaunchKernelGGL((k_laplacian<T, OutT, 64, true>), dim3((unsigned)xcd_grid((int)t, 1)),
+                       dim3(256), 0, st, a, lda, na, b, ldb, nb, d, inv, k, ldk, gy, gy);
+    return;
+  }
   hipLaunchKernelGGL((k_laplacian<T, OutT, 64>), dim3((unsigned)xcd_grid(gx, gy)), dim3(256), 0, st,
                      a, lda, na, b, ldb, nb, d, inv, k, ldk, gx, gy);
 }

@@ -111,6 +111,30 @@ def test_gaussian_ragged_shapes(gpu):
         assert _rel(laplacian_kernel(A, B, 2.0), oracle.laplacian_kernel(A, B, 2.0)) < 4e-16
 
 
+def test_kernels_of_a_set_with_itself(gpu):
+    """A kernel of a device set with itself runs only the lower-triangle
+    tiles and mirrors their stores; it is bitwise the kernel the full grid
+    writes for the same rows held in a second buffer, for the FP64 and FP32
+    Gaussian and Laplacian, at ragged sizes (600 and 1300 rows: 128 and
+    64 tiles with partial edges; Coulomb matrices, so near pairs occur)."""
+    torch = gpu
+    from fml_amd.kernels.kernels import gaussian_kernel_rows, laplacian_kernel_rows
+    from fml_amd import synth
+    from fml_amd.representations import generate_coulomb_matrices
+    for nmol in (600, 1300):
+        ms = synth.generate(nmol, seed=nmol, nmin=3, nmax=29)
+        X = torch.from_numpy(generate_coulomb_matrices(ms.charges, ms.coords, ms.offsets, nmax=29)).cuda()
+        Xc = X.clone()
+        for dt in (torch.float64, torch.float32):
+            Xs, Xo = X.to(dt), Xc.to(dt)
+            prec = "fp64" if dt == torch.float64 else "fp32"
+            ks, kf = gaussian_kernel_rows(Xs, Xs, 1000.0, precision=prec), gaussian_kernel_rows(Xs, Xo, 1000.0, precision=prec)
+            assert torch.equal(ks, kf), (nmol, dt, "gaussian")
+            assert torch.equal(ks, ks.T)
+            ls, lf = laplacian_kernel_rows(Xs, Xs, 10000.0), laplacian_kernel_rows(Xs, Xo, 10000.0)
+            assert torch.equal(ls, lf), (nmol, dt, "laplacian")
+
+
 def test_laplacian_fp64(gpu, mols):
     from fml_amd.kernels import laplacian_kernel
     X = _cm(mols)
