@@ -904,6 +904,7 @@ def krr36k_workload(args, precision):
         res["fit_breakdown_s"] = {"kernel": round(rep.kernel_s, 4), "factor": round(rep.factor_s, 4),
                                   "solve": round(rep.solve_s, 4), "total": round(rep.seconds, 4)}
         res["iterations"] = rep.iterations
+        res["alpha_error_estimate"] = rep.alpha_error_estimate
         res["shift"] = rep.shift
         res["relative_residual"] = rep.residual
         res["converged"] = state["error"] is None
@@ -934,7 +935,7 @@ def krr36k_workload(args, precision):
             # with double-length residuals (independent of the form being measured)
             yd = torch.from_numpy(np.ascontiguousarray(y[:ntr])).cuda()
             ax = a64.clone()
-            dx_ref, _ = refine_dd(K, 1e-6, m64.factor, yd, ax, tol=1e-14, max_iter=30)
+            dx_ref, _, _ = refine_dd(K, 1e-6, m64.factor, yd, ax, tol=1e-14, max_iter=30)
             m64.factor = None
             axn = torch.linalg.norm(ax)
             res["vs_exact"] = {
@@ -1144,7 +1145,7 @@ def krr_forms_clause(args):
 
     # the exact solution of the FP64 system, from the FP64 factor
     ax = m64.alpha.clone()
-    dx_ref, _ = refine_dd(K, 1e-6, m64.factor, yd, ax, tol=1e-14, max_iter=30)
+    dx_ref, _, _ = refine_dd(K, 1e-6, m64.factor, yd, ax, tol=1e-14, max_iter=30)
     m64.factor = None
     axn = torch.linalg.norm(ax)
 
@@ -1167,6 +1168,7 @@ def krr_forms_clause(args):
                      "alpha_rel_diff_vs_fp64": float((torch.linalg.norm(m.alpha - m64.alpha)
                                                       / torch.linalg.norm(m64.alpha)).item()),
                      "alpha_rel_err_vs_exact": err_exact(m.alpha),
+                     "alpha_error_estimate": getattr(m.report, "alpha_error_estimate", None),
                      "relative_residual": rel_res(m.alpha), "iterations": m.report.iterations,
                      "converged": err is None}
         del m, p

@@ -69,14 +69,18 @@ def residual_dd(K, lam, x, y, r):
 def refine_dd(K, lam, F, y, x, tol=1e-12, max_iter=30):
     """Iterative refinement of x in place for (K + lam I) x = y with the
     factor F (fmlh_potrf layout) as the solver and double-length residuals:
-    x += F^-1 (y - (K + lam I) x) until the correction's norm is at most tol
-    times |x| or stops halving (the working-precision floor).  Returns
-    (history of |correction| / |x|, history of |residual| / |y|), one host
-    read per step."""
+    x += F^-1 (y - (K + lam I) x).  The corrections shrink geometrically, by
+    rho = |d_k| / |d_(k-1)| a step, so the error left after a step is about
+    |d_k| rho / (1 - rho); refinement stops when that estimate (or the
+    correction itself) is at most tol times |x|, or when a correction no
+    longer halves (the working-precision floor).  Returns (history of
+    |correction| / |x|, history of |residual| / |y|, the final error
+    estimate), one host read per step."""
     t = torch()
     r = empty((int(K.shape[0]),), np.float64)
     ynorm = t.linalg.norm(y)
     dx, res = [], []
+    est = float("inf")
     for _ in range(max_iter):
         residual_dd(K, lam, x, y, r)
         rn = t.linalg.norm(r) / ynorm
@@ -85,9 +89,16 @@ def refine_dd(K, lam, F, y, x, tol=1e-12, max_iter=30):
         v = t.stack([t.linalg.norm(r) / t.linalg.norm(x), rn]).cpu()  # one host read per step
         dx.append(float(v[0]))
         res.append(float(v[1]))
-        if dx[-1] <= tol or (len(dx) > 1 and dx[-1] > 0.5 * dx[-2]):
+        if len(dx) > 1:
+            rho = dx[-1] / dx[-2] if dx[-2] > 0 else 0.0
+            if rho > 0.5:  # no longer converging: the floor
+                est = dx[-1]
+                break
+            est = dx[-1] * rho / (1.0 - rho)
+        if dx[-1] <= tol or est <= tol:
+            est = min(est, dx[-1])
             break
-    return dx, res
+    return dx, res, est
 
 
 def predict_from_kernel(Kt, alpha, out=None):
@@ -111,6 +122,7 @@ class FitReport:
     shift: float = 0.0
     backward_error: float = float("nan")
     residual: float = float("nan")
+    alpha_error_estimate: float = float("nan")
     history: list = field(default_factory=list)
 
 
@@ -297,17 +309,18 @@ class KRR:
         t2 = time.perf_counter()
         x = yd.clone()
         potrs_inplace(F, x)
-        dx, res = refine_dd(K, self.lam, F, yd, x, tol=float(self.tol), max_iter=self.max_iter)
+        dx, res, est = refine_dd(K, self.lam, F, yd, x, tol=float(self.tol), max_iter=self.max_iter)
         rep.history = dx
         rep.iterations = len(dx)
         rep.residual = res[-1] if res else float("nan")
+        rep.alpha_error_estimate = est
         t.cuda.synchronize()
         rep.solve_s = time.perf_counter() - t2
         del F
-        if not dx or dx[-1] > max(float(self.tol), 1e-8):
+        if not dx or est > max(float(self.tol), 1e-8):
             raise ConvergenceError(
-                f"Ozaki-factor refinement stopped with alpha's correction at {dx[-1] if dx else float('nan'):.3g} "
-                f"of |alpha| after {rep.iterations} steps (the factor is too inexact for this system's "
+                f"Ozaki-factor refinement stopped with alpha's estimated error at {est:.3g} of |alpha| "
+                f"after {rep.iterations} steps (the factor is too inexact for this system's "
                 "conditioning); use precision='fp64'", alpha=x, report=rep)
         return x
 

@@ -200,7 +200,7 @@ def test_ozaki_factor_and_refinement(gpu):
     F64.diagonal().add_(1e-6)
     assert potrf_inplace(F64) == 0
     a2 = m64.alpha.clone()
-    h2, _ = refine_dd(K, 1e-6, F64, yd, a2, tol=1e-14)
+    h2, _, _ = refine_dd(K, 1e-6, F64, yd, a2, tol=1e-14)
     e_2 = float(np.linalg.norm(a2.cpu().numpy() - xs) / np.linalg.norm(xs))
     print(f"KRR 2600: relative residual FP64 {r64:.3g}, Ozaki refined {roz:.3g} "
           f"({moz.report.iterations} steps: corrections {moz.report.history}), prediction diff "
