@@ -59,14 +59,22 @@ __device__ __forceinline__ float lane_value(float v, int l) {
 
 // One wave factors the diagonal block: lane r holds row r in registers.
 // Step j takes the square root of the pivot (lane j's diagonal, read by
-// every lane), scales column j below it, then every lane r updates its row
-// M[r][c] -= L[r][j] L[c][j] for c > j, L[c][j] read from an LDS copy of
-// column j (entries above the diagonal are updated too and never stored).  Fully unrolled,
-// so the row stays in registers.
+// every lane) and scales column j below it; every lane r then updates its
+// row, M[r][c] -= L[r][j] L[c][j] for c > j (entries above the diagonal are
+// updated too and never stored).  The steps go kPB columns at a time: inside
+// a block, L[c][j] of the block's own later columns comes from lane c by
+// readlane; the block's kPB columns then go through LDS once, and the rows'
+// columns right of the block subtract their kPB products in step order.
+// Every entry sees the same operations in the same order as one step at a
+// time, with a quarter of the LDS round trips.  Fully unrolled, so the row
+// stays in registers.
+constexpr int kPB = 4;
+constexpr int kPC = 8;
+
 template <typename T>
-__global__ __launch_bounds__(64) void k_potf2(T *__restrict__ a, int64_t lda, int k0, int b,
-                                              int32_t *__restrict__ info) {
-  __shared__ T col[kNB];
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_potf2(
+    T *__restrict__ a, int64_t lda, int k0, int b, int32_t *__restrict__ info) {
+  __shared__ T colb[kNB][kPB];  // colb[c][jj] = L[c][j0 + jj]
   if (*info != 0) return;
   const int r = threadIdx.x;
   T *grow = a + (int64_t)(k0 + r) * lda + k0;
@@ -74,22 +82,47 @@ __global__ __launch_bounds__(64) void k_potf2(T *__restrict__ a, int64_t lda, in
 #pragma unroll
   for (int c = 0; c < kNB; ++c) row[c] = (r < b && c <= r) ? grow[c] : T(0);
 #pragma unroll
-  for (int j = 0; j < kNB; ++j) {
-    if (j < b) {
-      const T d = lane_value(row[j], j);
-      if (!(d > T(0))) {  // uniform: every lane read the same pivot
-        if (r == 0) *info = k0 + j + 1;
-        return;
-      }
-      const T piv = sqrt(d);
-      const T lrj = (r > j && r < b) ? row[j] / piv : T(0);
-      row[j] = (r == j) ? piv : (r > j ? lrj : row[j]);
-      // column j of L through LDS: every lane reads L[c][j] at the same
-      // address (a broadcast) instead of one readlane pair per element
-      col[r] = lrj;
-      __syncthreads();
+  for (int j0 = 0; j0 < kNB; j0 += kPB) {
+    if (j0 < b) {
+      T lr[kPB];
 #pragma unroll
-      for (int c = j + 1; c < kNB; ++c) row[c] -= lrj * col[c];
+      for (int jj = 0; jj < kPB; ++jj) {
+        const int j = j0 + jj;
+        lr[jj] = T(0);
+        if (j < b) {
+          const T d = lane_value(row[j], j);
+          if (!(d > T(0))) {  // uniform: every lane read the same pivot
+            if (r == 0) *info = k0 + j + 1;
+            return;
+          }
+          const T piv = sqrt(d);
+          const T lrj = (r > j && r < b) ? row[j] / piv : T(0);
+          row[j] = (r == j) ? piv : (r > j ? lrj : row[j]);
+          lr[jj] = lrj;
+#pragma unroll
+          for (int cc = jj + 1; cc < kPB; ++cc) row[j0 + cc] -= lrj * lane_value(lrj, j0 + cc);
+        }
+      }
+#pragma unroll
+      for (int jj = 0; jj < kPB; ++jj) colb[r][jj] = lr[jj];
+      __syncthreads();
+      // kPC columns' values are read before their products (the scheduling
+      // barrier keeps the reads together), so one LDS wait covers kPC reads
+#pragma unroll
+      for (int c0 = j0 + kPB; c0 < kNB; c0 += kPC) {
+        T v[kPC][kPB];
+#pragma unroll
+        for (int cc = 0; cc < kPC; ++cc)
+#pragma unroll
+          for (int jj = 0; jj < kPB; ++jj) v[cc][jj] = c0 + cc < kNB ? colb[c0 + cc][jj] : T(0);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int cc = 0; cc < kPC; ++cc)
+          if (c0 + cc < kNB) {
+#pragma unroll
+            for (int jj = 0; jj < kPB; ++jj) row[c0 + cc] -= lr[jj] * v[cc][jj];
+          }
+      }
       __syncthreads();
     }
   }
@@ -101,35 +134,82 @@ __global__ __launch_bounds__(64) void k_potf2(T *__restrict__ a, int64_t lda, in
 }
 
 // Rows r in [r0, n): x = M[r][k0:k0+b] solves x L^T = m_r (forward
-// substitution against the factored diagonal block, held in LDS).
-template <typename T>
+// substitution against the factored diagonal block, held in LDS).  A full
+// block (kFull) stages the factor with shifts instead of divisions by b and
+// moves each row in 16-byte pieces.
+template <typename T, bool kFull>
 __global__ __launch_bounds__(256) void k_trsm(T *__restrict__ a, int64_t lda, int n, int k0, int b,
                                               int r0, const int32_t *__restrict__ info) {
   __shared__ T L[kNB][kNB + 1];
   if (*info != 0) return;
-  for (int idx = threadIdx.x; idx < b * b; idx += 256) {
-    const int r = idx / b, c = idx % b;
-    L[r][c] = (c <= r) ? a[(int64_t)(k0 + r) * lda + k0 + c] : T(0);
+  if (kFull) {
+#pragma unroll
+    for (int idx = threadIdx.x; idx < kNB * kNB; idx += 256) {
+      const int r = idx >> 6, c = idx & 63;
+      L[r][c] = (c <= r) ? a[(int64_t)(k0 + r) * lda + k0 + c] : T(0);
+    }
+  } else {
+    for (int idx = threadIdx.x; idx < b * b; idx += 256) {
+      const int r = idx / b, c = idx % b;
+      L[r][c] = (c <= r) ? a[(int64_t)(k0 + r) * lda + k0 + c] : T(0);
+    }
   }
   __syncthreads();
   const int r = r0 + blockIdx.x * 256 + threadIdx.x;
   if (r >= n) return;
   T *row = a + (int64_t)r * lda + k0;
   T x[kNB];
+  // 16-byte pieces when the row's start is 16-byte aligned (lda even, k0 a
+  // multiple of 64)
+  const bool vec = kFull && sizeof(T) == 8 && ((reinterpret_cast<uintptr_t>(row) & 15) == 0);
+  if (vec) {
+    typedef T T2 __attribute__((ext_vector_type(2)));
 #pragma unroll
-  for (int c = 0; c < kNB; ++c) {
-    if (c < b) {
-      T v = row[c];
+    for (int c = 0; c < kNB; c += 2) {
+      const T2 p = *reinterpret_cast<const T2 *>(row + c);
+      x[c] = p.x;
+      x[c + 1] = p.y;
+    }
+  } else {
 #pragma unroll
-      for (int t = 0; t < c; ++t) v -= x[t] * L[c][t];
-      x[c] = v / L[c][c];
-    } else {
-      x[c] = T(0);
+    for (int c = 0; c < kNB; ++c) x[c] = (kFull || c < b) ? row[c] : T(0);
+  }
+  // entries c0 .. c0 + 3 together: each is x[c] = (m[c] - sum_{t < c}
+  // x[t] L[c][t]) / L[c][c] with the products subtracted in t order, as
+  // one entry at a time; their chains run interleaved up to t = c0, then
+  // the block's own triangle in order
+#pragma unroll
+  for (int c0 = 0; c0 < kNB; c0 += 4) {
+    if (c0 < b) {
+      T v[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) v[i] = x[c0 + i];
+#pragma unroll
+      for (int t = 0; t < c0; ++t)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) v[i] -= x[t] * L[c0 + i][t];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int c = c0 + i;
+        if (c < b) {
+#pragma unroll
+          for (int t = c0; t < c; ++t) v[i] -= x[t] * L[c][t];
+          x[c] = v[i] / L[c][c];
+        } else {
+          x[c] = T(0);
+        }
+      }
     }
   }
+  if (vec) {
+    typedef T T2 __attribute__((ext_vector_type(2)));
 #pragma unroll
-  for (int c = 0; c < kNB; ++c)
-    if (c < b) row[c] = x[c];
+    for (int c = 0; c < kNB; c += 2) *reinterpret_cast<T2 *>(row + c) = T2{x[c], x[c + 1]};
+  } else {
+#pragma unroll
+    for (int c = 0; c < kNB; ++c)
+      if (kFull || c < b) row[c] = x[c];
+  }
 }
 
 // C[r][c] -= sum_{t in [kb, ke)} M[r][t] M[c][t] for r in [r0, r1),
@@ -184,8 +264,12 @@ int panel_factor(T *a, int64_t lda, int n, int K0, int kend, int32_t *info, hipS
     FMLH_LAUNCH_CHECK("k_potf2");
     const int r0 = k0 + b;
     if (r0 < n) {
-      hipLaunchKernelGGL(k_trsm<T>, dim3(ceil_div(n - r0, 256)), dim3(256), 0, st, a, lda, n, k0, b,
-                         r0, info);
+      if (b == kNB)
+        hipLaunchKernelGGL((k_trsm<T, true>), dim3(ceil_div(n - r0, 256)), dim3(256), 0, st, a, lda, n,
+                           k0, b, r0, info);
+      else
+        hipLaunchKernelGGL((k_trsm<T, false>), dim3(ceil_div(n - r0, 256)), dim3(256), 0, st, a, lda,
+                           n, k0, b, r0, info);
       FMLH_LAUNCH_CHECK("k_trsm");
       if (r0 < kend) {  // rest of the outer panel
         const int gx = ceil_div(kend - r0, kTile), gy = ceil_div(n - r0, kTile);
