@@ -326,6 +326,50 @@ __global__ __launch_bounds__(256) void k_gemv_t(const double *__restrict__ k, in
   if (threadIdx.x == 0) y[i] = part[0];
 }
 
+// Predictions y_i = sum_j K(j,i) alpha_j in two passes, so that a batch of
+// a few molecules still spreads over the chip: k_gemv_part sums a fixed
+// chunk of kGemvChunk training rows of one column (eight loads in flight
+// per thread), k_gemv_sum adds a column's chunk sums in chunk order.  The
+// chunking depends on the training-set size only, so every prediction is
+// the same sum in the same order whatever batch its molecule came in.
+constexpr int kGemvChunk = 2048;
+
+__global__ __launch_bounds__(256) void k_gemv_part(const double *__restrict__ k, int64_t ldk,
+                                                   int na, int nchunk,
+                                                   const double *__restrict__ alpha,
+                                                   double *__restrict__ part) {
+  __shared__ double wsum[4];
+  const int chunk = blockIdx.x % nchunk;
+  const int64_t col = blockIdx.x / nchunk;
+  const double *c = k + col * ldk;
+  const int j0 = chunk * kGemvChunk + (int)threadIdx.x;
+  double kv[8], av[8];
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    const int j = j0 + 256 * u;
+    kv[u] = j < na ? c[j] : 0.0;
+    av[u] = j < na ? alpha[j] : 0.0;
+  }
+  double s = 0.0;
+#pragma unroll
+  for (int u = 0; u < 8; ++u) s += kv[u] * av[u];
+#pragma unroll
+  for (int w = 32; w > 0; w >>= 1) s += __shfl_xor(s, w, 64);
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) part[col * nchunk + chunk] = (wsum[0] + wsum[1]) + (wsum[2] + wsum[3]);
+}
+
+__global__ __launch_bounds__(256) void k_gemv_sum(const double *__restrict__ part, int nb,
+                                                  int nchunk, double *__restrict__ y) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= nb) return;
+  const double *p = part + (int64_t)i * nchunk;
+  double s = 0.0;
+  for (int c = 0; c < nchunk; ++c) s += p[c];
+  y[i] = s;
+}
+
 // r_i = y_i - (sum_j K(j,i) x_j + lambda x_i) for K (n, n) Fortran order
 // (column i contiguous), one workgroup per output, every product and sum
 // error-free: the product by TwoProduct (p = k x, e = fma(k, x, -p)) and
@@ -563,8 +607,10 @@ size_t fmlh_krr_predict_ws_bytes(int nmol, int ntrain, int nmax) {
   const size_t packed = (size_t)nmax * (nmax + 1) / 2;
   // the kernel block is taken with at least one column, also for an empty
   // training set (whose predictions are zeros)
+  const size_t nchunk = (size_t)ceil_div(ntrain > 0 ? ntrain : 1, kGemvChunk);
   return WsCarver::need(sizeof(double) * (size_t)nmol * packed) +
-         WsCarver::need(sizeof(double) * (size_t)nmol * (size_t)(ntrain > 0 ? ntrain : 1));
+         WsCarver::need(sizeof(double) * (size_t)nmol * (size_t)(ntrain > 0 ? ntrain : 1)) +
+         WsCarver::need(sizeof(double) * (size_t)nmol * nchunk);
 }
 
 int fmlh_coulomb_sorted(const double *z, const double *r, const int32_t *offsets, int nmol,
@@ -589,7 +635,10 @@ int fmlh_krr_predict(const double *z, const double *r, const int32_t *offsets, i
   WsCarver w(ws, ws_bytes);
   double *cm = w.take<double>((size_t)nmol * packed);
   double *k = w.take<double>((size_t)nmol * (size_t)(ntrain > 0 ? ntrain : 1));
-  if (!cm || !k) return -13;
+  const int nchunk = ceil_div(ntrain > 0 ? ntrain : 1, kGemvChunk);
+  double *part = w.take<double>((size_t)nmol * nchunk);
+  if (!cm || !k || !part) return -13;
+  FMLH_ARG_CHECK((int64_t)nmol * nchunk <= INT32_MAX, 4);
   int rc = fmlh_coulomb_sorted(z, r, offsets, nmol, nmax, cm, nullptr, nullptr, status, stream);
   if (rc) return rc;
   hipStream_t st = as_stream(stream);
@@ -602,8 +651,11 @@ int fmlh_krr_predict(const double *z, const double *r, const int32_t *offsets, i
   launch_gaussian<double, double>(xtrain, ldx, ntrain, cm, packed, nmol, packed, inv, k, ntrain,
                                   1e-3, st);
   FMLH_LAUNCH_CHECK("k_gaussian<f64>(predict)");
-  hipLaunchKernelGGL(k_gemv_t, dim3(nmol), dim3(256), 0, st, k, (int64_t)ntrain, ntrain, alpha, y);
-  FMLH_LAUNCH_CHECK("k_gemv_t(predict)");
+  hipLaunchKernelGGL(k_gemv_part, dim3(nmol * nchunk), dim3(256), 0, st, k, (int64_t)ntrain, ntrain,
+                     nchunk, alpha, part);
+  FMLH_LAUNCH_CHECK("k_gemv_part(predict)");
+  hipLaunchKernelGGL(k_gemv_sum, dim3(ceil_div(nmol, 256)), dim3(256), 0, st, part, nmol, nchunk, y);
+  FMLH_LAUNCH_CHECK("k_gemv_sum(predict)");
   return 0;
 }
 

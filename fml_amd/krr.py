@@ -159,23 +159,23 @@ class KRR:
         yd = to_device(y, np.float64)
         n = int(self.X.shape[0])
         rep = FitReport(self.precision, n)
-        t.cuda.synchronize()
+        t.cuda.current_stream().synchronize()
         t0 = time.perf_counter()
         K = kernel_matrix(self.X, self.X, self.sigma, self.kernel)
-        t.cuda.synchronize()
+        t.cuda.current_stream().synchronize()
         rep.kernel_s = time.perf_counter() - t0
         if self.precision == "fp64":
             t1 = time.perf_counter()
             add_diagonal(K, self.lam)
             info = potrf_inplace(K)
-            t.cuda.synchronize()
+            t.cuda.current_stream().synchronize()
             rep.factor_s = time.perf_counter() - t1
             if info != 0:
                 raise np.linalg.LinAlgError(f"K + lambda I is not positive definite (leading minor {info})")
             t2 = time.perf_counter()
             alpha = yd.clone()
             potrs_inplace(K, alpha)
-            t.cuda.synchronize()
+            t.cuda.current_stream().synchronize()
             rep.solve_s = time.perf_counter() - t2
             self.factor = K if keep_kernel else None
         elif self.precision == "mixed":
@@ -215,7 +215,7 @@ class KRR:
         else:
             raise np.linalg.LinAlgError("FP32 factor failed even with a shift")
         del K32
-        t.cuda.synchronize()
+        t.cuda.current_stream().synchronize()
         rep.factor_s = time.perf_counter() - t1
         rep.shift = shift
         t2 = time.perf_counter()
@@ -275,7 +275,7 @@ class KRR:
             p.mul_(rz_new / rz).add_(z)
             rz = rz_new
         rep.iterations = it
-        t.cuda.synchronize()
+        t.cuda.current_stream().synchronize()
         rep.solve_s = time.perf_counter() - t2
         residual(x, r)
         rep.residual = float((t.linalg.norm(r) / ynorm).item())
@@ -302,7 +302,7 @@ class KRR:
         F = K.clone()
         add_diagonal(F, self.lam)
         info = potrf_ozaki_inplace(F)
-        t.cuda.synchronize()
+        t.cuda.current_stream().synchronize()
         rep.factor_s = time.perf_counter() - t1
         if info != 0:
             raise np.linalg.LinAlgError(f"K + lambda I is not positive definite (leading minor {info})")
@@ -314,7 +314,7 @@ class KRR:
         rep.iterations = len(dx)
         rep.residual = res[-1] if res else float("nan")
         rep.alpha_error_estimate = est
-        t.cuda.synchronize()
+        t.cuda.current_stream().synchronize()
         rep.solve_s = time.perf_counter() - t2
         del F
         if not dx or est > max(float(self.tol), 1e-8):

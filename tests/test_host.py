@@ -213,3 +213,25 @@ def test_exact_residual_helper():
     xs = exact_refined_solution(A2, 0.0, y)
     rs = exact_residual(A2, 0.0, xs, y)
     assert np.max(np.abs(rs)) <= 1e-15 * np.max(np.abs(A2) @ np.abs(xs))
+
+
+def test_predict_request_packing():
+    """A predict request packs its molecules into contiguous charges,
+    coordinates and atom counts on construction, and keeps the error of a
+    request that cannot be served instead of raising."""
+    from fml_amd.serving import PredictRequest
+    ms = synth.generate(12, seed=21, nmin=3, nmax=29)
+    ch = [ms.molecule(i)[0] for i in range(ms.nmol)]
+    co = [ms.molecule(i)[1] for i in range(ms.nmol)]
+    r = PredictRequest(ch, co, 29)
+    assert r.error is None and r.nmol == 12
+    assert r.z.dtype == np.float64 and np.array_equal(r.z, ms.charges)
+    assert r.r.shape == (len(ms.charges), 3) and np.array_equal(r.r, ms.coords)
+    assert np.array_equal(np.concatenate([[0], np.cumsum(r.counts)]), ms.offsets)
+    # flat coordinate lists are read as rows of three
+    assert np.array_equal(PredictRequest([[1, 6]], [[0, 0, 0, 1, 1, 1]]).r, [[0, 0, 0], [1, 1, 1]])
+    assert PredictRequest([], []).nmol == 0
+    big = int(r.counts.max())
+    assert "nmax" in str(PredictRequest(ch, co, big - 1).error)
+    assert "match" in str(PredictRequest(ch, co[:-1]).error)
+    assert "match" in str(PredictRequest([[1, 6]], [[0, 0, 0]]).error)
