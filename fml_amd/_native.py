@@ -90,6 +90,7 @@ SIGNATURES = {
     "fmlh_potrs_ws_bytes": (c_size_t, [c_int, c_int]),
     "fmlh_potrs": (c_int, [c_void_p, c_int64, c_int, c_void_p, c_int64, c_int, c_void_p, c_size_t,
                            c_void_p]),
+    "fmlh_potrs_correction": (c_int, [c_void_p, c_int64, c_int, c_void_p, c_void_p, c_size_t, c_void_p]),
     "fmlh_potrs_noalloc_max_bytes": (c_size_t, []),
     "fmlh_potrs_noalloc": (c_int, [c_void_p, c_int64, c_int, c_void_p, c_int64, c_int, c_void_p]),
     "fmlh_spotrf_ws_bytes": (c_size_t, [c_int]),

@@ -837,16 +837,21 @@ __global__ void k_trsv_prep(int32_t *ticket, T *pub, int n) {
   if (i < n) reinterpret_cast<U *>(pub)[i] = PubBits<T>::kSentinel;
 }
 
-// FP32 (the mixed form's preconditioner solve): a full diagonal block is
-// applied as its inverse W = L_bb^-1, formed by wave 0 while the workgroup
-// waits for the blocks before it, so the block's own step on the chain is
-// a 64 x 64 product over all four waves instead of 64 dependent
-// substitution steps.  FP64 keeps the substitution (its rounding matches a
-// triangular solve's backward-error bound; the inverse's does not).
-template <typename T, bool kForward>
-__global__ __launch_bounds__(256) void k_trsv_chain(const T *__restrict__ a, int64_t lda, int n,
+// kInv: a full diagonal block is applied as its inverse W = L_bb^-1,
+// formed by wave 0 while the workgroup waits for the blocks before it, so
+// the block's own step on the chain is a 64 x 64 product over all four
+// waves instead of 64 dependent substitution steps.  Used by FP32 (the
+// mixed form's preconditioner solve) and by the FP64 correction solve of
+// iterative refinement.  The plain FP64 solve keeps the substitution: its
+// rounding meets a triangular solve's backward-error bound, the inverse's
+// only a bound that grows with the diagonal block's condition number.
+// Two workgroups per CU (at most 256 registers a lane, LDS for two): with
+// one, only 256 of the 563 blocks of a 36,000-row solve are resident and
+// the later ones start too late to clear their backlog of products before
+// the chain reaches them.
+template <typename T, bool kForward, bool kInv>
+__global__ __launch_bounds__(256, 2) void k_trsv_chain(const T *__restrict__ a, int64_t lda, int n,
                                                     T *x, int32_t *ctl, T *pub) {
-  constexpr bool kInv = sizeof(T) == 4;
   constexpr int kWRows = kInv ? kNB : 1;
   __shared__ T L[kNB][kNB + 1];
   __shared__ T W[kWRows][kNB + 1];
@@ -1064,9 +1069,11 @@ size_t potrs_ws(int n, int nrhs) {
   return blocked > chain ? blocked : chain;
 }
 
+// inv_diag: the one-launch solve applies full diagonal blocks as their
+// inverses (always for FP32; for FP64 only the correction solve).
 template <typename T>
 int potrs_impl(const T *a, int64_t lda, int n, T *x, int64_t ldx, int nrhs, void *ws,
-               hipStream_t st) {
+               hipStream_t st, bool inv_diag = sizeof(T) == 4) {
   const int nblk = ceil_div(n, kNB);
   if (nrhs == 1) {  // one launch per direction
     WsCarver w(ws, potrs_ws<T>(n, nrhs));
@@ -1076,12 +1083,21 @@ int potrs_impl(const T *a, int64_t lda, int n, T *x, int64_t ldx, int nrhs, void
     const unsigned g = (unsigned)ceil_div(n, 256);
     hipLaunchKernelGGL(k_trsv_prep<T>, dim3(g), dim3(256), 0, st, ctl, pub, n);
     FMLH_LAUNCH_CHECK("k_trsv_prep");
-    hipLaunchKernelGGL((k_trsv_chain<T, true>), dim3(nblk), dim3(256), 0, st, a, lda, n, x, ctl, pub);
+    if (inv_diag)
+      hipLaunchKernelGGL((k_trsv_chain<T, true, true>), dim3(nblk), dim3(256), 0, st, a, lda, n, x,
+                         ctl, pub);
+    else
+      hipLaunchKernelGGL((k_trsv_chain<T, true, false>), dim3(nblk), dim3(256), 0, st, a, lda, n,
+                         x, ctl, pub);
     FMLH_LAUNCH_CHECK("k_trsv_chain<fwd>");
     hipLaunchKernelGGL(k_trsv_prep<T>, dim3(g), dim3(256), 0, st, ctl + 1, pub, n);
     FMLH_LAUNCH_CHECK("k_trsv_prep");
-    hipLaunchKernelGGL((k_trsv_chain<T, false>), dim3(nblk), dim3(256), 0, st, a, lda, n, x,
-                       ctl + 1, pub);
+    if (inv_diag)
+      hipLaunchKernelGGL((k_trsv_chain<T, false, true>), dim3(nblk), dim3(256), 0, st, a, lda, n,
+                         x, ctl + 1, pub);
+    else
+      hipLaunchKernelGGL((k_trsv_chain<T, false, false>), dim3(nblk), dim3(256), 0, st, a, lda, n,
+                         x, ctl + 1, pub);
     FMLH_LAUNCH_CHECK("k_trsv_chain<bwd>");
     return 0;
   }
@@ -1300,6 +1316,20 @@ int fmlh_potrs(const double *a, int64_t lda, int n, double *x, int64_t ldx, int 
   FMLH_ARG_CHECK(ws != nullptr && ws_bytes >= potrs_ws<double>(n, nrhs), 7);
   timing_begin(as_stream(stream));
   const int rc = potrs_impl<double>(a, lda, n, x, ldx, nrhs, ws, as_stream(stream));
+  timing_end(as_stream(stream));
+  return rc;
+}
+
+int fmlh_potrs_correction(const double *a, int64_t lda, int n, double *x, void *ws,
+                          size_t ws_bytes, void *stream) {
+  FMLH_ARG_CHECK(n >= 0, 3);
+  FMLH_ARG_CHECK(lda >= (n > 1 ? n : 1), 2);
+  if (n == 0) return 0;
+  FMLH_ARG_CHECK(a != nullptr, 1);
+  FMLH_ARG_CHECK(x != nullptr, 4);
+  FMLH_ARG_CHECK(ws != nullptr && ws_bytes >= potrs_ws<double>(n, 1), 5);
+  timing_begin(as_stream(stream));
+  const int rc = potrs_impl<double>(a, lda, n, x, n, 1, ws, as_stream(stream), true);
   timing_end(as_stream(stream));
   return rc;
 }

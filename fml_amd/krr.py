@@ -29,7 +29,7 @@ import numpy as np
 from . import _native
 from .device import empty, to_device, torch
 from .kernels.kernels import gaussian_kernel_rows, laplacian_kernel_rows
-from .math.cho_solve import potrf_inplace, potrf_ozaki_inplace, potrs_inplace
+from .math.cho_solve import potrf_inplace, potrf_ozaki_inplace, potrs_correction_inplace, potrs_inplace
 
 
 def kernel_matrix(X1, X2, sigma, kernel="gaussian", precision="fp64", out=None, out_dtype=None,
@@ -68,7 +68,8 @@ def residual_dd(K, lam, x, y, r):
 
 def refine_dd(K, lam, F, y, x, tol=1e-12, max_iter=30):
     """Iterative refinement of x in place for (K + lam I) x = y with the
-    factor F (fmlh_potrf layout) as the solver and double-length residuals:
+    factor F (fmlh_potrf layout) as the solver (its correction solve, the
+    diagonal blocks applied as inverses) and double-length residuals:
     x += F^-1 (y - (K + lam I) x).  The corrections shrink geometrically, by
     rho = |d_k| / |d_(k-1)| a step, so the error left after a step is about
     |d_k| rho / (1 - rho); refinement stops when that estimate (or the
@@ -84,7 +85,7 @@ def refine_dd(K, lam, F, y, x, tol=1e-12, max_iter=30):
     for _ in range(max_iter):
         residual_dd(K, lam, x, y, r)
         rn = t.linalg.norm(r) / ynorm
-        potrs_inplace(F, r)
+        potrs_correction_inplace(F, r)
         x.add_(r)
         v = t.stack([t.linalg.norm(r) / t.linalg.norm(x), rn]).cpu()  # one host read per step
         dx.append(float(v[0]))
@@ -308,7 +309,7 @@ class KRR:
             raise np.linalg.LinAlgError(f"K + lambda I is not positive definite (leading minor {info})")
         t2 = time.perf_counter()
         x = yd.clone()
-        potrs_inplace(F, x)
+        potrs_correction_inplace(F, x)
         dx, res, est = refine_dd(K, self.lam, F, yd, x, tol=float(self.tol), max_iter=self.max_iter)
         rep.history = dx
         rep.iterations = len(dx)

@@ -90,6 +90,20 @@ def potrs_inplace(m, x):
     return x
 
 
+def potrs_correction_inplace(m, x):
+    """The correction solve of iterative refinement with the FP64 factor m
+    for one right-hand side x (n,), in place: full diagonal blocks applied as
+    their inverses (fmlh_potrs_correction)."""
+    n = int(m.shape[0])
+    if m.dtype != torch().float64 or x.dim() != 1:
+        raise ValueError("potrs_correction_inplace takes an FP64 factor and one right-hand side")
+    wsb = _native.load().fmlh_potrs_ws_bytes(n, 1)
+    ws = workspace(wsb)
+    _native.call("fmlh_potrs_correction", _native.ptr(m), m.stride(0), n, _native.ptr(x), _native.ptr(ws),
+                 wsb, _native.stream_ptr())
+    return x
+
+
 def cho_factor(A) -> CholeskyFactor:
     m = _fortran_memory(A)
     info = potrf_inplace(m)

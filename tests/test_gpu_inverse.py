@@ -240,3 +240,43 @@ def test_residual_dd_against_exact(gpu):
     print(f"\nresidual n={n}: double-length vs exact {err:.3g}, plain FP64 vs exact {err64:.3g}")
     assert err <= 1e-14
     assert err64 > 10 * err
+
+
+def test_potrs_correction_against_triangular_solves(gpu):
+    """fmlh_potrs_correction (full diagonal blocks applied as inverses, the
+    refinement's correction solve) against two LAPACK triangular solves
+    (scipy solve_triangular) with the same factor: to working accuracy on
+    well-conditioned matrices, including a ragged last block, and close to
+    the substitution solve's own error on an ill-conditioned KRR factor."""
+    torch = gpu
+    from scipy.linalg import solve_triangular
+    from fml_amd import synth
+    from fml_amd.krr import kernel_matrix
+    from fml_amd.math.cho_solve import potrf_inplace, potrs_correction_inplace, potrs_inplace
+    from fml_amd.representations import generate_coulomb_matrices
+    rng = np.random.default_rng(43)
+    cases = []
+    for n in (3000, 1037, 64, 50):
+        B = rng.normal(size=(n, n))
+        cases.append((f"wishart {n}", torch.from_numpy(B @ B.T / n + np.eye(n)).cuda(), 1e-13))
+    ms = synth.generate(1500, seed=44, nmin=3, nmax=29)
+    X = torch.from_numpy(generate_coulomb_matrices(ms.charges, ms.coords, ms.offsets, nmax=29)).cuda()
+    K = kernel_matrix(X, X, 1000.0)
+    K.diagonal().add_(1e-6)
+    cases.append(("KRR 1500, lambda 1e-6", K, None))
+    for name, A, tol in cases:
+        n = int(A.shape[0])
+        F = A.clone()
+        assert potrf_inplace(F) == 0
+        b = rng.normal(size=n)
+        xs = potrs_inplace(F, torch.from_numpy(b.copy()).cuda()).cpu().numpy()
+        xc = potrs_correction_inplace(F, torch.from_numpy(b.copy()).cuda()).cpu().numpy()
+        L = torch.tril(F).cpu().numpy()
+        xr = solve_triangular(L.T, solve_triangular(L, b, lower=True), lower=False)
+        e_s = float(np.linalg.norm(xs - xr) / np.linalg.norm(xr))
+        e_c = float(np.linalg.norm(xc - xr) / np.linalg.norm(xr))
+        print(f"\n{name}: substitution vs LAPACK {e_s:.3g}, inverse diagonal blocks vs LAPACK {e_c:.3g}")
+        if tol is not None:
+            assert e_s <= tol and e_c <= tol
+        else:
+            assert e_c <= max(100 * e_s, 1e-12)
