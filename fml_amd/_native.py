@@ -104,6 +104,9 @@ SIGNATURES = {
     "fmlh_gemv_t": (c_int, [c_void_p, c_int64, c_int, c_int, c_void_p, c_void_p, c_void_p]),
     "fmlh_residual_dd": (c_int, [c_void_p, c_int64, c_int, c_double, c_void_p, c_void_p, c_void_p,
                                  c_void_p]),
+    "fmlh_residual_dd_lower_ws_bytes": (c_size_t, [c_int]),
+    "fmlh_residual_dd_lower": (c_int, [c_void_p, c_int64, c_int, c_void_p, c_double, c_void_p, c_void_p,
+                                       c_void_p, c_void_p, c_size_t, c_void_p]),
     "fmlh_krr_predict_ws_bytes": (c_size_t, [c_int, c_int, c_int]),
     "fmlh_krr_predict": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p, c_int64, c_int,
                                  c_void_p, c_double, c_void_p, c_void_p, c_void_p, c_size_t, c_void_p]),

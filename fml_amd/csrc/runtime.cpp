@@ -27,7 +27,7 @@ int record_message(int rc, const char *text) {
 
 extern "C" {
 
-int fmlh_abi_version(void) { return 6; }
+int fmlh_abi_version(void) { return 7; }
 
 const char *fmlh_last_error(void) { return g_last_error.c_str(); }
 

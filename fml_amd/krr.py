@@ -27,7 +27,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _native
-from .device import empty, to_device, torch
+from .device import empty, to_device, torch, workspace
 from .kernels.kernels import gaussian_kernel_rows, laplacian_kernel_rows
 from .math.cho_solve import potrf_inplace, potrf_ozaki_inplace, potrs_correction_inplace, potrs_inplace
 
@@ -66,7 +66,20 @@ def residual_dd(K, lam, x, y, r):
     return r
 
 
-def refine_dd(K, lam, F, y, x, tol=1e-12, max_iter=30):
+def residual_dd_lower(A, diag, lam, x, y, r):
+    """r = y - (K + lam I) x for K held as A's strictly upper triangle (the
+    row-major view of the Fortran strictly lower one) plus its diagonal
+    `diag`; A's lower triangle (where a factor of K + lam I may sit) is not
+    read.  Double length as residual_dd."""
+    n = int(A.shape[0])
+    wsb = _native.load().fmlh_residual_dd_lower_ws_bytes(n)
+    ws = workspace(wsb)
+    _native.call("fmlh_residual_dd_lower", _native.ptr(A), A.stride(0), n, _native.ptr(diag), float(lam),
+                 _native.ptr(x), _native.ptr(y), _native.ptr(r), _native.ptr(ws), wsb, _native.stream_ptr())
+    return r
+
+
+def refine_dd(K, lam, F, y, x, tol=1e-12, max_iter=30, diag=None):
     """Iterative refinement of x in place for (K + lam I) x = y with the
     factor F (fmlh_potrf layout) as the solver (its correction solve, the
     diagonal blocks applied as inverses) and double-length residuals:
@@ -76,14 +89,19 @@ def refine_dd(K, lam, F, y, x, tol=1e-12, max_iter=30):
     correction itself) is at most tol times |x|, or when a correction no
     longer halves (the working-precision floor).  Returns (history of
     |correction| / |x|, history of |residual| / |y|, the final error
-    estimate), one host read per step."""
+    estimate), one host read per step.  diag: K's diagonal when K holds the
+    matrix in its strictly upper triangle only (F may then be K itself, its
+    lower triangle the factor): residuals by residual_dd_lower."""
     t = torch()
     r = empty((int(K.shape[0]),), np.float64)
     ynorm = t.linalg.norm(y)
     dx, res = [], []
     est = float("inf")
     for _ in range(max_iter):
-        residual_dd(K, lam, x, y, r)
+        if diag is None:
+            residual_dd(K, lam, x, y, r)
+        else:
+            residual_dd_lower(K, diag, lam, x, y, r)
         rn = t.linalg.norm(r) / ynorm
         potrs_correction_inplace(F, r)
         x.add_(r)
@@ -292,32 +310,35 @@ class KRR:
         return x
 
     def _fit_ozaki(self, K, yd, rep):
-        """Factor of K + lambda I with INT8-core updates (an unfactored K is
-        kept for the residuals), then iterative refinement with double-length
+        """Factor of K + lambda I with INT8-core updates, in place in K's lower
+        triangle (K itself stays in the strictly upper one and its diagonal
+        aside, for the residuals), then iterative refinement with double-length
         residuals (refine_dd) until alpha's correction is at most tol times
         |alpha|.  Raises ConvergenceError if the refinement stalls (or runs
         out of steps) with a correction above 1e-8 |alpha|: the factor is then
         too inexact for the system's conditioning."""
         t = torch()
         t1 = time.perf_counter()
-        F = K.clone()
-        add_diagonal(F, self.lam)
-        info = potrf_ozaki_inplace(F)
+        # one matrix for both: the factor of K + lambda I overwrites the lower
+        # triangle (row-major view) in place, K stays in the strictly upper
+        # one, and its diagonal is kept aside for the residuals
+        d = K.diagonal().clone()
+        add_diagonal(K, self.lam)
+        info = potrf_ozaki_inplace(K)
         t.cuda.current_stream().synchronize()
         rep.factor_s = time.perf_counter() - t1
         if info != 0:
             raise np.linalg.LinAlgError(f"K + lambda I is not positive definite (leading minor {info})")
         t2 = time.perf_counter()
         x = yd.clone()
-        potrs_correction_inplace(F, x)
-        dx, res, est = refine_dd(K, self.lam, F, yd, x, tol=float(self.tol), max_iter=self.max_iter)
+        potrs_correction_inplace(K, x)
+        dx, res, est = refine_dd(K, self.lam, K, yd, x, tol=float(self.tol), max_iter=self.max_iter, diag=d)
         rep.history = dx
         rep.iterations = len(dx)
         rep.residual = res[-1] if res else float("nan")
         rep.alpha_error_estimate = est
         t.cuda.current_stream().synchronize()
         rep.solve_s = time.perf_counter() - t2
-        del F
         if not dx or est > max(float(self.tol), 1e-8):
             raise ConvergenceError(
                 f"Ozaki-factor refinement stopped with alpha's estimated error at {est:.3g} of |alpha| "

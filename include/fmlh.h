@@ -29,7 +29,7 @@ extern "C" {
 
 #define FMLH_EHIP (-1000)
 
-/* ABI version (bumped on any signature change). */
+/* ABI version (bumped on any signature change or added entry point). */
 int fmlh_abi_version(void);
 /* Text of the last HIP error seen by this thread ("" if none). */
 const char *fmlh_last_error(void);
@@ -505,6 +505,22 @@ int fmlh_sym_residual(const double *a, int64_t lda, int n, double lambda,
  * number times the unit roundoff.  r must not alias x or y. */
 int fmlh_residual_dd(const double *k, int64_t ldk, int n, double lambda,
                      const double *x, const double *y, double *r, void *stream);
+/* fmlh_residual_dd for a matrix that shares its storage with its own
+ * Cholesky factor: K's strictly lower triangle in a (n, n) Fortran order
+ * (K(i,j), i > j, at a[i + j lda]) and its diagonal in diag (n); a's upper
+ * triangle, where fmlh_potrf / fmlh_potrf_ozaki leave the factor of
+ * K + lambda I, is not read.  Each tile of the triangle is read once and
+ * serves both its rows' and its columns' sums, so the residual reads half
+ * the bytes of the full-matrix form, and the caller keeps one n x n matrix
+ * instead of K and a factored copy.  Same accuracy as fmlh_residual_dd
+ * (error-free products, double-length sums, one rounding; the order of
+ * the sums differs).  r must not alias x, y or diag.
+ * ws: fmlh_residual_dd_lower_ws_bytes(n) bytes (double-length partial
+ * sums, 16 ceil(n/64) n bytes). */
+size_t fmlh_residual_dd_lower_ws_bytes(int n);
+int fmlh_residual_dd_lower(const double *a, int64_t lda, int n, const double *diag,
+                           double lambda, const double *x, const double *y, double *r,
+                           void *ws, size_t ws_bytes, void *stream);
 /* y(i) = sum_j K(j, i) * alpha(j) for K (na, nb) Fortran order: the
  * prediction K_test^T alpha. */
 int fmlh_gemv_t(const double *k, int64_t ldk, int na, int nb,

@@ -42,7 +42,7 @@ def test_binding_covers_header():
 
 def test_version_and_errors_without_gpu():
     lib = nat.load()
-    assert lib.fmlh_abi_version() == 6
+    assert lib.fmlh_abi_version() == 7
     # argument validation happens before any device call
     assert lib.fmlh_potrf(None, 1, -1, None, None, 0, None) == -3
     assert lib.fmlh_gaussian_kernel(None, 0, 5, None, 0, 5, 3, 1.0, None, 5, None) == -2

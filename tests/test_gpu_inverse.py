@@ -280,3 +280,66 @@ def test_potrs_correction_against_triangular_solves(gpu):
             assert e_s <= tol and e_c <= tol
         else:
             assert e_c <= max(100 * e_s, 1e-12)
+
+
+def test_factor_leaves_the_other_triangle(gpu):
+    """fmlh_potrf and fmlh_potrf_ozaki write only the factor's triangle
+    (row-major lower): the strictly upper part is bitwise the caller's, on
+    sizes with several 1024-column panels and ragged edges, so K can share
+    its storage with its factor."""
+    torch = gpu
+    from fml_amd.math.cho_solve import potrf_inplace, potrf_ozaki_inplace
+    rng = np.random.default_rng(45)
+    for n in (2100, 3000):
+        B = rng.normal(size=(n, n))
+        A = torch.from_numpy(B @ B.T / n + np.eye(n)).cuda()
+        for fn in (potrf_inplace, potrf_ozaki_inplace):
+            F = A.clone()
+            assert fn(F) == 0
+            assert torch.equal(torch.triu(F, 1), torch.triu(A, 1)), (fn.__name__, n)
+
+
+def test_residual_dd_lower_matches_full(gpu):
+    """fmlh_residual_dd_lower (K's strictly lower triangle in Fortran order
+    plus its diagonal, the other triangle holding the factor) against the
+    exactly rounded residual, on a KRR system whose plain FP64 residual is
+    dominated by cancellation, and on ragged sizes; then an Ozaki fit on the
+    shared storage reaches the exact solution."""
+    torch = gpu
+    from fml_amd import synth
+    from fml_amd.krr import KRR, kernel_matrix, residual_dd, residual_dd_lower
+    from fml_amd.math.cho_solve import potrf_inplace, potrs_inplace
+    from fml_amd.representations import generate_coulomb_matrices
+    for nmol, seed in ((1500, 7), (1000, 8), (63, 9), (1, 10)):
+        ms = synth.generate(nmol, seed=seed, nmin=3, nmax=29)
+        X = torch.from_numpy(generate_coulomb_matrices(ms.charges, ms.coords, ms.offsets, nmax=29)).cuda()
+        K = kernel_matrix(X, X, 1000.0)
+        n = int(K.shape[0])
+        y = torch.from_numpy(ms.energies.copy()).cuda()
+        F = K.clone()
+        F.diagonal().add_(1e-6)
+        assert potrf_inplace(F) == 0
+        x = y.clone()
+        potrs_inplace(F, x)
+        ex = exact_residual(K.cpu().numpy(), 1e-6, x.cpu().numpy(), ms.energies)
+        # the shared matrix: factor in the lower triangle, K above it
+        S = torch.tril(F) + torch.triu(K, 1)
+        d = K.diagonal().clone()
+        r = torch.empty_like(y)
+        residual_dd_lower(S, d, 1e-6, x, y, r)
+        r_full = torch.empty_like(y)
+        residual_dd(K, 1e-6, x, y, r_full)
+        scale = max(np.max(np.abs(ex)), 1e-16 * np.max(np.abs(ms.energies)))  # n = 1 may cancel exactly
+        err = float(np.max(np.abs(r.cpu().numpy() - ex)) / scale)
+        err_full = float(np.max(np.abs(r_full.cpu().numpy() - ex)) / scale)
+        print(f"\nresidual n={n}: triangle form vs exact {err:.3g}, full form {err_full:.3g}")
+        assert err <= 1e-14
+    # the Ozaki fit (shared storage) against the exact solution
+    ms = synth.generate(2000, seed=11, nmin=3, nmax=29)
+    X = generate_coulomb_matrices(ms.charges, ms.coords, ms.offsets, nmax=29)
+    m = KRR(1000.0, 1e-6, precision="ozaki", tol=1e-12, max_iter=20).fit(X, ms.energies)
+    K = kernel_matrix(m.X, m.X, 1000.0)
+    xs = exact_refined_solution(K.cpu().numpy(), 1e-6, ms.energies)
+    e = float(np.linalg.norm(m.alpha.cpu().numpy() - xs) / np.linalg.norm(xs))
+    print(f"Ozaki fit on the shared matrix, n=2000: alpha vs exact {e:.3g} ({m.report.iterations} steps)")
+    assert e <= 1e-8
