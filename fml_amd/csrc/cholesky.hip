@@ -134,24 +134,38 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void
 }
 
 // Rows r in [r0, n): x = M[r][k0:k0+b] solves x L^T = m_r (forward
-// substitution against the factored diagonal block, held in LDS).  A full
-// block (kFull) stages the factor with shifts instead of divisions by b and
-// moves each row in 16-byte pieces.
+// substitution against the factored diagonal block, held in LDS).  Each
+// entry is x[c] = (m[c] - sum_{t < c} x[t] L[c][t]) / L[c][c] with the
+// products subtracted in t order; four entries' chains run interleaved up
+// to the first of them, then the four's own triangle in order.  A full
+// block (kFull) stages the factor transposed (LT[t][c] = L[c][t], so the
+// four entries' factor values at one t are contiguous) and reads them for
+// eight t's at a time before their products: the substitution is bound by
+// LDS round trips, and reading one value per product took 31 of the kernel's
+// 35 us (20 us this way, the same operations in the same order).  Rows move
+// in 16-byte pieces when aligned.
+template <typename T>
+struct TrsmLds {
+  // row strides keeping every row 16-byte aligned
+  static constexpr int kStride = sizeof(T) == 8 ? kNB + 2 : kNB + 4;
+};
+
 template <typename T, bool kFull>
 __global__ __launch_bounds__(256) void k_trsm(T *__restrict__ a, int64_t lda, int n, int k0, int b,
                                               int r0, const int32_t *__restrict__ info) {
-  __shared__ T L[kNB][kNB + 1];
+  constexpr int kS = TrsmLds<T>::kStride;
+  __shared__ __attribute__((aligned(16))) T L[kNB * kS];  // full: LT[t][c]; ragged: L[r][c]
   if (*info != 0) return;
   if (kFull) {
 #pragma unroll
     for (int idx = threadIdx.x; idx < kNB * kNB; idx += 256) {
       const int r = idx >> 6, c = idx & 63;
-      L[r][c] = (c <= r) ? a[(int64_t)(k0 + r) * lda + k0 + c] : T(0);
+      L[c * kS + r] = (c <= r) ? a[(int64_t)(k0 + r) * lda + k0 + c] : T(0);
     }
   } else {
     for (int idx = threadIdx.x; idx < b * b; idx += 256) {
       const int r = idx / b, c = idx % b;
-      L[r][c] = (c <= r) ? a[(int64_t)(k0 + r) * lda + k0 + c] : T(0);
+      L[r * kS + c] = (c <= r) ? a[(int64_t)(k0 + r) * lda + k0 + c] : T(0);
     }
   }
   __syncthreads();
@@ -159,11 +173,11 @@ __global__ __launch_bounds__(256) void k_trsm(T *__restrict__ a, int64_t lda, in
   if (r >= n) return;
   T *row = a + (int64_t)r * lda + k0;
   T x[kNB];
+  typedef T T2 __attribute__((ext_vector_type(2)));
   // 16-byte pieces when the row's start is 16-byte aligned (lda even, k0 a
   // multiple of 64)
   const bool vec = kFull && sizeof(T) == 8 && ((reinterpret_cast<uintptr_t>(row) & 15) == 0);
   if (vec) {
-    typedef T T2 __attribute__((ext_vector_type(2)));
 #pragma unroll
     for (int c = 0; c < kNB; c += 2) {
       const T2 p = *reinterpret_cast<const T2 *>(row + c);
@@ -174,35 +188,79 @@ __global__ __launch_bounds__(256) void k_trsm(T *__restrict__ a, int64_t lda, in
 #pragma unroll
     for (int c = 0; c < kNB; ++c) x[c] = (kFull || c < b) ? row[c] : T(0);
   }
-  // entries c0 .. c0 + 3 together: each is x[c] = (m[c] - sum_{t < c}
-  // x[t] L[c][t]) / L[c][c] with the products subtracted in t order, as
-  // one entry at a time; their chains run interleaved up to t = c0, then
-  // the block's own triangle in order
+  if (kFull) {
+    constexpr int kCh = 8;
 #pragma unroll
-  for (int c0 = 0; c0 < kNB; c0 += 4) {
-    if (c0 < b) {
+    for (int c0 = 0; c0 < kNB; c0 += 4) {
       T v[4];
 #pragma unroll
       for (int i = 0; i < 4; ++i) v[i] = x[c0 + i];
 #pragma unroll
-      for (int t = 0; t < c0; ++t)
+      for (int t0 = 0; t0 < c0; t0 += kCh) {
+        T l[kCh][4];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) v[i] -= x[t] * L[c0 + i][t];
+        for (int tt = 0; tt < kCh; ++tt)
+          if (t0 + tt < c0) {
+            const T *src = L + (t0 + tt) * kS + c0;
+            if (sizeof(T) == 8) {
+              const T2 p0 = *reinterpret_cast<const T2 *>(src);
+              const T2 p1 = *reinterpret_cast<const T2 *>(src + 2);
+              l[tt][0] = p0.x;
+              l[tt][1] = p0.y;
+              l[tt][2] = p1.x;
+              l[tt][3] = p1.y;
+            } else {
+              typedef T T4 __attribute__((ext_vector_type(4)));
+              const T4 p = *reinterpret_cast<const T4 *>(src);
+              l[tt][0] = p.x;
+              l[tt][1] = p.y;
+              l[tt][2] = p.z;
+              l[tt][3] = p.w;
+            }
+          }
+        // the chunk's reads issue together, one wait before its products
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int tt = 0; tt < kCh; ++tt)
+          if (t0 + tt < c0) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) v[i] -= x[t0 + tt] * l[tt][i];
+          }
+      }
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int c = c0 + i;
-        if (c < b) {
 #pragma unroll
-          for (int t = c0; t < c; ++t) v[i] -= x[t] * L[c][t];
-          x[c] = v[i] / L[c][c];
-        } else {
-          x[c] = T(0);
+        for (int t = c0; t < c; ++t) v[i] -= x[t] * L[t * kS + c];
+        x[c] = v[i] / L[c * kS + c];
+      }
+    }
+  } else {
+#pragma unroll
+    for (int c0 = 0; c0 < kNB; c0 += 4) {
+      if (c0 < b) {
+        T v[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) v[i] = x[c0 + i];
+#pragma unroll
+        for (int t = 0; t < c0; ++t)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) v[i] -= x[t] * L[(c0 + i) * kS + t];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int c = c0 + i;
+          if (c < b) {
+#pragma unroll
+            for (int t = c0; t < c; ++t) v[i] -= x[t] * L[c * kS + t];
+            x[c] = v[i] / L[c * kS + c];
+          } else {
+            x[c] = T(0);
+          }
         }
       }
     }
   }
   if (vec) {
-    typedef T T2 __attribute__((ext_vector_type(2)));
 #pragma unroll
     for (int c = 0; c < kNB; c += 2) *reinterpret_cast<T2 *>(row + c) = T2{x[c], x[c + 1]};
   } else {
