@@ -434,13 +434,14 @@ int potrf_impl(T *a, int64_t lda, int n, int32_t *info, hipStream_t st, void *oz
   // paired; the device work of the calls still overlaps on their streams).
   std::unique_lock<std::mutex> lk;
   if (ahead) lk = std::unique_lock<std::mutex>(*aux.enqueue);
-  int rc = panel_factor(a, lda, n, 0, min(kNBO, n), info, st);
+  const bool oz = std::is_same<T, double>::value && oz_ws != nullptr;
+  const int nbo = oz ? kOzPanel : kNBO;  // outer panel width
+  int rc = panel_factor(a, lda, n, 0, min(nbo, n), info, st);
   if (rc) return rc;
-  for (int K0 = 0; K0 < n; K0 += kNBO) {
-    const int kend = min(K0 + kNBO, n);
+  for (int K0 = 0; K0 < n; K0 += nbo) {
+    const int kend = min(K0 + nbo, n);
     if (kend >= n) break;
-    const int nend = min(kend + kNBO, n);  // the next panel is [kend, nend)
-    const bool oz = std::is_same<T, double>::value && oz_ws != nullptr;
+    const int nend = min(kend + nbo, n);  // the next panel is [kend, nend)
     if (oz) {  // the finished panel's rows below it, split once for both updates
       rc = oz_split(reinterpret_cast<double *>(a), lda, kend, n - kend, K0, kend - K0, kOzSlices,
                     oz_ws, info, st);

@@ -3,8 +3,9 @@
 //
 // k_oz_split: a workgroup of 256 threads takes 8 rows.  Each wave finds the
 // largest |entry| of two rows (a butterfly over 16 entries a lane) and
-// picks the row's exponent e_r; then each thread takes one 32-entry chunk
-// of one row: X = trunc(M 2^(8S - e_r)) is exact in int64 (|X| < 2^39 for
+// picks the row's exponent e_r; then each thread takes the 32-entry chunks
+// c, c + 32, ... of one row (one chunk at the 1024-column panel width): X =
+// trunc(M 2^(8S - e_r)) is exact in int64 (|X| < 2^39 for
 // S = 5), and its signed base-256 digits come from the low byte, b - 256
 // when b >= 128, and an exact shift of X - b.  The chunk's digits of plane t
 // are one 32-byte store, and the 8 rows of a chunk are 256 contiguous bytes.
@@ -111,36 +112,39 @@ __global__ __launch_bounds__(256) void k_oz_split(const double *__restrict__ a, 
     }
   }
   __syncthreads();
-  const int rl = tid & (kOzSplitRows - 1), chunk = tid >> 3;  // 32 chunks of 32 entries
+  const int rl = tid & (kOzSplitRows - 1);
   const int r = rb + rl;
   if (r >= nrows) return;
   const int e = sh_e[rl];
-  const double *src = a + (int64_t)(row0 + r) * lda + kb + chunk * kOzKS;
-  uint32_t packed[S][kOzKS / 4];
+  // 32 threads a row, each taking every 32nd chunk of 32 entries
+  for (int chunk = tid >> 3; chunk < kOzPanel / kOzKS; chunk += 32) {
+    const double *src = a + (int64_t)(row0 + r) * lda + kb + chunk * kOzKS;
+    uint32_t packed[S][kOzKS / 4];
 #pragma unroll
-  for (int t = 0; t < S; ++t)
+    for (int t = 0; t < S; ++t)
 #pragma unroll
-    for (int w = 0; w < kOzKS / 4; ++w) packed[t][w] = 0;
+      for (int w = 0; w < kOzKS / 4; ++w) packed[t][w] = 0;
 #pragma unroll
-  for (int i = 0; i < kOzKS; ++i) {
-    const int k = chunk * kOzKS + i;
-    const double x = src[min(k, kw - 1) - chunk * kOzKS];  // a load on every path
-    const double v = k < kw ? x : 0.0;
-    long long X = (long long)ldexp(v, 8 * S - e);  // exact scaling, truncation toward zero
+    for (int i = 0; i < kOzKS; ++i) {
+      const int k = chunk * kOzKS + i;
+      const double x = src[min(k, kw - 1) - chunk * kOzKS];  // a load on every path
+      const double v = k < kw ? x : 0.0;
+      long long X = (long long)ldexp(v, 8 * S - e);  // exact scaling, truncation toward zero
 #pragma unroll
-    for (int t = S - 1; t >= 0; --t) {
-      int b = (int)(X & 255);
-      b = b >= 128 ? b - 256 : b;
-      X = (X - b) >> 8;  // exact: X - b is a multiple of 256
-      packed[t][i >> 2] |= (uint32_t)(uint8_t)(int8_t)b << (8 * (i & 3));
+      for (int t = S - 1; t >= 0; --t) {
+        int b = (int)(X & 255);
+        b = b >= 128 ? b - 256 : b;
+        X = (X - b) >> 8;  // exact: X - b is a multiple of 256
+        packed[t][i >> 2] |= (uint32_t)(uint8_t)(int8_t)b << (8 * (i & 3));
+      }
     }
-  }
-  const int64_t off = (int64_t)chunk * ((int64_t)nrows * kOzKS) + (int64_t)r * kOzKS;
+    const int64_t off = (int64_t)chunk * ((int64_t)nrows * kOzKS) + (int64_t)r * kOzKS;
 #pragma unroll
-  for (int t = 0; t < S; ++t) {
-    u32x4 *dst = reinterpret_cast<u32x4 *>(planes + t * plane + off);
-    dst[0] = u32x4{packed[t][0], packed[t][1], packed[t][2], packed[t][3]};
-    dst[1] = u32x4{packed[t][4], packed[t][5], packed[t][6], packed[t][7]};
+    for (int t = 0; t < S; ++t) {
+      u32x4 *dst = reinterpret_cast<u32x4 *>(planes + t * plane + off);
+      dst[0] = u32x4{packed[t][0], packed[t][1], packed[t][2], packed[t][3]};
+      dst[1] = u32x4{packed[t][4], packed[t][5], packed[t][6], packed[t][7]};
+    }
   }
 }
 
