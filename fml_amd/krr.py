@@ -11,13 +11,16 @@ This module is that chain kept on the GPU end to end:
   on FP32 MFMA, and the FP64 system is solved by conjugate gradients
   preconditioned with that FP32 factor, every residual and update in FP64
   (iterative refinement is the special case of one-step convergence);
-* Ozaki form (``precision="ozaki"``): a copy of K + lambda I is factored with
-  its large updates on INT8 matrix cores (fmlh_potrf_ozaki: an exact
-  five-digit split of each panel, the factor within ~1e-12 of the FP64 one),
-  then refined with residuals y - (K + lambda I) alpha computed in double
-  length (fmlh_residual_dd) until alpha's correction falls below tol: alpha
-  converges to the solution of the FP64 system to working accuracy, where a
-  plain FP64 solve is off by up to kappa times the unit roundoff.
+* Ozaki form (``precision="ozaki"``): K + lambda I is factored in place in
+  K's lower triangle with its large updates on INT8 matrix cores
+  (fmlh_potrf_ozaki: an exact five-digit split of each panel, the factor
+  within ~1e-12 of the FP64 one), K itself staying in the other triangle
+  and its diagonal aside; alpha is then refined with residuals
+  y - (K + lambda I) alpha computed in double length from that triangle
+  (fmlh_residual_dd_lower) and corrections from the factor
+  (fmlh_potrs_correction) until alpha's estimated error falls below tol:
+  alpha converges to the solution of the FP64 system to working accuracy,
+  where a plain FP64 solve is off by up to kappa times the unit roundoff.
 """
 from __future__ import annotations
 
