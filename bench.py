@@ -425,7 +425,7 @@ def dense36k_workload(args, kind):
 
     def roofline(t):
         fl = 2.0 * n * n * d
-        if kind in ("gaussian64", "gaussian32"):
+        if kind in ("gaussian64", "gaussian32", "gaussianbf16x3"):
             # a set against itself runs only the lower-triangle 128-tiles (the
             # diagonal tiles whole) and mirrors their stores
             g = -(-n // 128)

@@ -81,10 +81,18 @@ __global__ __launch_bounds__(MT == 128 ? 512 : 256, (MT == 128 ? 2 : 1)) void k_
   __shared__ unsigned long long s_near[2 * WR][NBM * NBN * 4];
   unsigned long long *masks = s_near[threadIdx.x >> 6];
   unsigned long long steps = 0;
+  // FP32 tiles: a lane's four r are four consecutive rows gm, so the mirror
+  // (row gn, columns gm .. gm + 3) is one vector store when all four are
+  // plain entries of a whole tile (scattered 4-byte stores cost more than
+  // the FP32 tile's arithmetic)
+  const bool mvec = mirror && sizeof(T) == 4 && n0 + MT <= na && m0 + MT <= nb && (ldk & 3) == 0 &&
+                    (reinterpret_cast<uintptr_t>(K) & (4 * sizeof(OutT) - 1)) == 0;
 #pragma unroll
   for (int im = 0; im < NBM; ++im)
 #pragma unroll
-    for (int in = 0; in < NBN; ++in)
+    for (int in = 0; in < NBN; ++in) {
+      OutT kvs[4];
+      bool plain4 = true;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int m = tile_row<T, MT, WR>(im, r);
@@ -99,13 +107,32 @@ __global__ __launch_bounds__(MT == 128 ? 512 : 256, (MT == 128 ? 2 : 1)) void k_
           masks[4 * (NBN * im + in) + r] = mk;
           steps |= 1ull << (4 * (NBN * im + in) + r);
         }
+        plain4 = plain4 && valid && !near;
+        if (d2 < T(0)) d2 = T(0);
+        const double kv = gauss_value<T, OutT>(d2, inv);
+        kvs[r] = (OutT)kv;
         if (valid && !near) {
-          if (d2 < T(0)) d2 = T(0);
-          const double kv = gauss_value<T, OutT>(d2, inv);
           kstore(K + (int64_t)gn + (int64_t)gm * ldk, kv);
-          if (mirror) kstore(K + (int64_t)gm + (int64_t)gn * ldk, kv);
+          if (mirror && !mvec) kstore(K + (int64_t)gm + (int64_t)gn * ldk, kv);
         }
       }
+      if (mvec) {
+        const int gm0 = m0 + tile_row<T, MT, WR>(im, 0), gn = n0 + tile_col<MT>(in);
+        OutT *dst = K + (int64_t)gm0 + (int64_t)gn * ldk;
+        if (plain4) {
+          typedef OutT o4v __attribute__((ext_vector_type(4)));
+          __builtin_nontemporal_store(o4v{kvs[0], kvs[1], kvs[2], kvs[3]}, reinterpret_cast<o4v *>(dst));
+        } else {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int gm = m0 + tile_row<T, MT, WR>(im, r);
+            const T d2r = fma(T(-2), acc.v[im][in][r], s.pn[gm - m0] + s.qn[gn - n0]);
+            if (!(d2r <= tau * (s.pn[gm - m0] + s.qn[gn - n0])))
+              __builtin_nontemporal_store(kvs[r], dst + r);
+          }
+        }
+      }
+    }
   if (steps) {
     auto idx = [&](int b, int L, int &gn, int &gm) {
       const int im = b / (4 * NBN), in = (b >> 2) % NBN, r = b & 3, q = L >> 4;
@@ -170,8 +197,14 @@ __global__ __launch_bounds__(256, (MT == 128 ? 2 : 1)) void k_laplacian(
   constexpr int kStride = MT + 2;
   constexpr int EPT = MT * kLapBK / 256;  // staged elements per thread and operand
   constexpr int TPR = kLapBK / EPT;       // threads per staged row
-  __shared__ T As[2][kLapBK * kStride];
-  __shared__ T Bs[2][kLapBK * kStride];
+  // the staging buffers, then (mirrored tiles) the tile's values for the
+  // transposed store
+  constexpr int kStage = 2 * kLapBK * kStride;
+  constexpr size_t kStageBytes = 2 * kStage * sizeof(T);
+  constexpr size_t kMirrorBytes = kSym ? (size_t)MT * (MT + 1) * sizeof(OutT) : 0;
+  __shared__ __attribute__((aligned(16))) unsigned char smem[kStageBytes > kMirrorBytes ? kStageBytes : kMirrorBytes];
+  T (*As)[kLapBK * kStride] = reinterpret_cast<T (*)[kLapBK * kStride]>(smem);
+  T (*Bs)[kLapBK * kStride] = reinterpret_cast<T (*)[kLapBK * kStride]>(smem + kStage * sizeof(T));
   int bx, by;
   if (kSym) {
     const int64_t t = (int64_t)gy * (gy + 1) / 2;
@@ -233,21 +266,38 @@ __global__ __launch_bounds__(256, (MT == 128 ? 2 : 1)) void k_laplacian(
     if (kt + 1 < nk) lstore(buf ^ 1);
     __syncthreads();
   }
+  // (the loop's last barrier: every read of the staging buffers is done)
+  OutT *Cs = reinterpret_cast<OutT *>(smem);  // Cs[n][m], row stride MT + 1
 #pragma unroll
   for (int i = 0; i < R; ++i) {
     const int gm = m0 + ty + 16 * i;
-    if (gm >= nb) continue;
 #pragma unroll
     for (int j = 0; j < R; ++j) {
       const int gn = n0 + tx + 16 * j;
-      if (gn >= na) continue;
       double v;
       if (sizeof(OutT) == 8)
         v = exp(inv * (double)acc[i][j]);
       else
         v = (double)kexp((T)((T)inv * acc[i][j]));
-      kstore(K + (int64_t)gn + (int64_t)gm * ldk, v);
-      if (mirror) kstore(K + (int64_t)gm + (int64_t)gn * ldk, v);
+      if (gm < nb && gn < na) kstore(K + (int64_t)gn + (int64_t)gm * ldk, v);
+      if (mirror) Cs[(tx + 16 * j) * (MT + 1) + ty + 16 * i] = (OutT)v;
+    }
+  }
+  if (mirror) {
+    // the mirror, row n0 + c over columns m0 .. m0 + MT, from the tile
+    // transposed through LDS (single-value stores down the columns scatter)
+    __syncthreads();
+    // a wave writes whole rows, its lanes over 64 consecutive columns
+    const int lane = tid & 63, wave = tid >> 6;
+#pragma unroll
+    for (int cc = 0; cc < MT / 4; ++cc) {
+      const int c = wave * (MT / 4) + cc, gn = n0 + c;
+#pragma unroll
+      for (int h = 0; h < MT / 64; ++h) {
+        const int r = 64 * h + lane;
+        if (gn < na && m0 + r < nb)
+          __builtin_nontemporal_store(Cs[c * (MT + 1) + r], K + (int64_t)gn * ldk + m0 + r);
+      }
     }
   }
 }

@@ -118,7 +118,12 @@ __device__ __forceinline__ OutT gauss_value(float d2, double inv, float inv_log2
 }
 
 // K[gm][gn] for B rows gm (output rows) and A rows gn (output columns).
-template <typename OutT>
+// kSym (A and B the same rows): only the lower-triangle tiles run (tri_tile
+// order); each entry with gn <= gm is stored at (gm, gn) and, when gn < gm,
+// at (gn, gm) too, so the kernel is exactly symmetric.  (A full-grid entry
+// above the diagonal may differ from its mirror in the last FP32 bits: its
+// tile adds the two cross products in the other order.)
+template <typename OutT, bool kSym = false>
 __global__ __launch_bounds__(256, 2) void k_gauss_b3(Plane pa, int na, Plane pb, int nb, int dp,
                                                      const float *__restrict__ A, int64_t lda,
                                                      const float *__restrict__ B, int64_t ldb,
@@ -129,7 +134,16 @@ __global__ __launch_bounds__(256, 2) void k_gauss_b3(Plane pa, int na, Plane pb,
   __shared__ __attribute__((aligned(16))) B3Smem s;
   __shared__ unsigned long long s_near[4][64];
   int tx, ty;
-  if (!xcd_tile(gx, gy, tx, ty)) return;
+  if (kSym) {
+    const int64_t t = (int64_t)gy * (gy + 1) / 2;
+    const int64_t per = (t + kXcds - 1) / kXcds;
+    const int64_t L = blockIdx.x;
+    if (!tri_tile(gy, (L % kXcds) * per + L / kXcds, tx, ty)) return;
+  } else if (!xcd_tile(gx, gy, tx, ty)) {
+    return;
+  }
+  const bool diag = kSym && tx == ty;  // stores only gn <= gm
+  const bool mirror = kSym;            // entries with gn < gm also at (gn, gm)
   const int n0 = tx * kMT;  // A rows (output columns)
   const int m0 = ty * kMT;  // B rows (output rows)
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -215,53 +229,92 @@ __global__ __launch_bounds__(256, 2) void k_gauss_b3(Plane pa, int na, Plane pb,
       pn[j][r] = gm < nb ? pb.nrm[gm] : 0.f;
     }
   // whole tile inside K and 4-element stores aligned: one vector store per row
-  const bool vec = n0 + kMT <= na && m0 + kMT <= nb && (ldk & 3) == 0 &&
+  const bool vec = !diag && n0 + kMT <= na && m0 + kMT <= nb && (ldk & 3) == 0 &&
                    (reinterpret_cast<uintptr_t>(K) & (4 * sizeof(OutT) - 1)) == 0;
   // entries in the cancellation band are stored after the tile, from direct
   // differences: step b = 16 j + 4 r + i records its ballot in masks[b]
   unsigned long long *masks = s_near[wave];
   unsigned long long steps = 0;
-  unsigned int near_count = 0;  // this wave's recomputed entries
+  unsigned long long mynear = 0;  // this lane's band entries, bit 16 j + 4 r + i
+  unsigned int near_count = 0;    // this wave's entries of K written from direct differences
+  OutT v[4][4][4];                // [j][r][i]
 #pragma unroll
   for (int j = 0; j < 4; ++j)
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int gm = m0 + wm * 64 + j * 16 + 4 * (lane >> 4) + r;
-      OutT v[4];
-      bool nr[4];
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int gn = gn0 + i;
-        const bool valid = gm < nb && gn < na;
+        const bool valid = gm < nb && gn < na && !(diag && gn > gm);
         const float nsum = pn[j][r] + qn[i];
         float d2 = fmaf(-2.f, acc[i][j][r], nsum);
-        nr[i] = valid && d2 <= kTau * nsum;
-        const unsigned long long mk = __ballot(nr[i]);
+        const bool nr = valid && d2 <= kTau * nsum;
+        const unsigned long long mk = __ballot(nr);
         if (mk) {
           masks[16 * j + 4 * r + i] = mk;
           steps |= 1ull << (16 * j + 4 * r + i);
-          near_count += (unsigned int)__popcll(mk);
+          // a mirrored entry is written twice
+          near_count += (unsigned int)__popcll(mk) + (unsigned int)__popcll(__ballot(mirror && nr && gn < gm));
         }
+        if (nr) mynear |= 1ull << (16 * j + 4 * r + i);
         if (d2 < 0.f) d2 = 0.f;
-        v[i] = gauss_value<OutT>(d2, inv, inv_log2);
+        v[j][r][i] = gauss_value<OutT>(d2, inv, inv_log2);
       }
+    }
+  typedef OutT o4v __attribute__((ext_vector_type(4)));
+  // row gm, columns gn0 .. gn0 + 3: one vector store when whole
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int gm = m0 + wm * 64 + j * 16 + 4 * (lane >> 4) + r;
+      const unsigned long long nb4 = (mynear >> (16 * j + 4 * r)) & 15ull;
       OutT *row = K + (int64_t)gm * ldk + gn0;
-      if (vec && !(nr[0] || nr[1] || nr[2] || nr[3])) {
-        typedef OutT o4v __attribute__((ext_vector_type(4)));
-        __builtin_nontemporal_store(o4v{v[0], v[1], v[2], v[3]}, reinterpret_cast<o4v *>(row));
+      if (vec && nb4 == 0) {
+        __builtin_nontemporal_store(o4v{v[j][r][0], v[j][r][1], v[j][r][2], v[j][r][3]},
+                                    reinterpret_cast<o4v *>(row));
       } else if (gm < nb) {
 #pragma unroll
         for (int i = 0; i < 4; ++i)
-          if (gn0 + i < na && !nr[i]) __builtin_nontemporal_store(v[i], row + i);
+          if (gn0 + i < na && !((nb4 >> i) & 1ull) && !(diag && gn0 + i > gm))
+            __builtin_nontemporal_store(v[j][r][i], row + i);
       }
     }
+  // the mirror: row gn0 + i, columns gm .. gm + 3 (this lane's four r), one
+  // vector store when whole (every entry of an off-diagonal tile has gn < gm)
+  if (mirror) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int gmb = m0 + wm * 64 + j * 16 + 4 * (lane >> 4);
+        const int gn = gn0 + i;
+        bool band = false;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) band = band || ((mynear >> (16 * j + 4 * r + i)) & 1ull);
+        OutT *col = K + (int64_t)gn * ldk + gmb;
+        if (vec && !band) {
+          __builtin_nontemporal_store(o4v{v[j][0][i], v[j][1][i], v[j][2][i], v[j][3][i]},
+                                      reinterpret_cast<o4v *>(col));
+        } else if (gn < na) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            if (gmb + r < nb && gn < gmb + r && !((mynear >> (16 * j + 4 * r + i)) & 1ull))
+              __builtin_nontemporal_store(v[j][r][i], col + r);
+        }
+      }
+  }
   if (steps) {
     auto idx = [&](int b, int L, int &gn, int &gm) {
       gn = n0 + wn * 64 + 4 * (L & 15) + (b & 3);
       gm = m0 + wm * 64 + 16 * (b >> 4) + 4 * (L >> 4) + ((b >> 2) & 3);
     };
     auto put = [&](int gn, int gm, float d2) {
-      __builtin_nontemporal_store(gauss_value<OutT>(d2, inv, inv_log2), K + (int64_t)gm * ldk + gn);
+      if (diag && gn > gm) return;
+      const OutT kv = gauss_value<OutT>(d2, inv, inv_log2);
+      __builtin_nontemporal_store(kv, K + (int64_t)gm * ldk + gn);
+      if (mirror && gn < gm) __builtin_nontemporal_store(kv, K + (int64_t)gn * ldk + gm);
     };
     recompute_near_deferred<float>(steps, masks, idx, A, lda, B, ldb, d, put);
   }
@@ -319,15 +372,26 @@ int gaussian_bf16x3(const float *a, int64_t lda, int na, const float *b, int64_t
     if (rc) return rc == -1 ? -12 : rc;
   }
   const int gx = ceil_div(na, kMT), gy = ceil_div(nb, kMT);
-  const unsigned grid = (unsigned)xcd_grid(gx, gy);
   const int dp = padded_cols(d);
   timing_begin(st);
-  if (k32)
-    hipLaunchKernelGGL(k_gauss_b3<float>, dim3(grid), dim3(256), 0, st, pa, na, pb, nb, dp, a, lda,
-                       b, ldb, d, inv, inv_log2, k32, ldk, recomputed, gx, gy);
-  else
-    hipLaunchKernelGGL(k_gauss_b3<double>, dim3(grid), dim3(256), 0, st, pa, na, pb, nb, dp, a, lda,
-                       b, ldb, d, inv, inv_log2, k64, ldk, recomputed, gx, gy);
+  if (same_operand(a, lda, na, b, ldb, nb)) {
+    // a kernel of a set with itself: lower-triangle tiles, mirrored stores
+    const unsigned grid = (unsigned)xcd_grid((int)((int64_t)gy * (gy + 1) / 2), 1);
+    if (k32)
+      hipLaunchKernelGGL((k_gauss_b3<float, true>), dim3(grid), dim3(256), 0, st, pa, na, pb, nb, dp,
+                         a, lda, b, ldb, d, inv, inv_log2, k32, ldk, recomputed, gx, gy);
+    else
+      hipLaunchKernelGGL((k_gauss_b3<double, true>), dim3(grid), dim3(256), 0, st, pa, na, pb, nb, dp,
+                         a, lda, b, ldb, d, inv, inv_log2, k64, ldk, recomputed, gx, gy);
+  } else {
+    const unsigned grid = (unsigned)xcd_grid(gx, gy);
+    if (k32)
+      hipLaunchKernelGGL(k_gauss_b3<float>, dim3(grid), dim3(256), 0, st, pa, na, pb, nb, dp, a, lda,
+                         b, ldb, d, inv, inv_log2, k32, ldk, recomputed, gx, gy);
+    else
+      hipLaunchKernelGGL(k_gauss_b3<double>, dim3(grid), dim3(256), 0, st, pa, na, pb, nb, dp, a, lda,
+                         b, ldb, d, inv, inv_log2, k64, ldk, recomputed, gx, gy);
+  }
   timing_end(st);
   FMLH_LAUNCH_CHECK("k_gauss_b3");
   return 0;

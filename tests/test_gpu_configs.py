@@ -169,6 +169,8 @@ def test_kernels_36k(gpu, set40k):
         kernel_matrix(X32, X32, 1000.0, "gaussian", precision=prec, out=out32)
         figures["gaussian_" + prec] = _rel(out32[:rows].double().cpu().numpy(), go.T)
         assert torch.all(torch.diagonal(out32) == 1.0)
+        sub = out32[:640, :640]
+        assert torch.equal(sub, sub.T), prec  # mirrored lower-triangle tiles
     kernel_matrix(X32, X32, 1e4, "laplacian", out=out32)
     figures["laplacian32"] = _rel(out32[:rows].double().cpu().numpy(), lo.T)
     del out32

@@ -321,6 +321,9 @@ def test_gaussian_precision_forms(gpu, mols):
             assert _rel(kh, ko) < tol, (prec, od)
         kk = gaussian_kernel_rows(X32, X32, 1000.0, precision=prec).cpu().numpy()
         assert np.all(np.diag(kk) == 1.0)
+        # a set with itself: lower-triangle tiles, mirrored -- exactly symmetric
+        assert np.array_equal(kk, kk.T), prec
+        assert _rel(kk[:100, 100:].astype(np.float64), ko) < tol, prec
 
 
 def test_laplacian_fp32(gpu, mols):
