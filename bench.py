@@ -1171,6 +1171,7 @@ def krr_forms_clause(args):
                            "steps": len(dx_ref), "last_correction_rel": dx_ref[-1]},
            "fp64": {"fit_predict_ms": round(t64 * 1e3, 2), "mae": mae64,
                     "relative_residual": rel_res(m64.alpha), "alpha_rel_err_vs_exact": err_exact(m64.alpha)}}
+    run("ozaki")  # its first call's set-up (kernel loading, the digit planes' allocation) likewise
     for prec in ("ozaki", "mixed"):
         m, p, t, err = run(prec)
         pe = p.cpu().numpy()

@@ -311,17 +311,28 @@ __global__ __launch_bounds__(MT == 128 ? 512 : 256, (MT == 128 ? 2 : 1)) void k_
 
 __global__ void k_set_int_impl(int32_t *p, int v) { *p = v; }
 
-// The inner blocks of outer panel [K0, kend): each 64-column block is
-// factored (k_potf2), the rows below it solved (k_trsm) and the rest of the
-// panel's columns updated with it (k_update, 64 tiles).
+// The inner blocks of outer panel [K0, kend), in middle blocks of kMid
+// columns: each 64-column block is factored (k_potf2), the rows below it
+// solved (k_trsm) and the rest of its middle block's columns updated with it
+// (k_update, 64 tiles); then the rest of the outer panel's columns are
+// updated with the whole middle block at once, so that they are read and
+// written once per kMid columns of k instead of once per 64.
+#ifndef FMLH_PANEL_MID
+#define FMLH_PANEL_MID 256
+#endif
+constexpr int kMid = FMLH_PANEL_MID;
+static_assert(kMid % kNB == 0, "middle blocks hold whole inner blocks");
+
 template <typename T>
 int panel_factor(T *a, int64_t lda, int n, int K0, int kend, int32_t *info, hipStream_t st) {
-  for (int k0 = K0; k0 < kend; k0 += kNB) {
-    const int b = min(kNB, kend - k0);
-    hipLaunchKernelGGL(k_potf2<T>, dim3(1), dim3(64), 0, st, a, lda, k0, b, info);
-    FMLH_LAUNCH_CHECK("k_potf2");
-    const int r0 = k0 + b;
-    if (r0 < n) {
+  for (int m0 = K0; m0 < kend; m0 += kMid) {
+    const int mend = min(m0 + kMid, kend);
+    for (int k0 = m0; k0 < mend; k0 += kNB) {
+      const int b = min(kNB, mend - k0);
+      hipLaunchKernelGGL(k_potf2<T>, dim3(1), dim3(64), 0, st, a, lda, k0, b, info);
+      FMLH_LAUNCH_CHECK("k_potf2");
+      const int r0 = k0 + b;
+      if (r0 >= n) continue;
       if (b == kNB)
         hipLaunchKernelGGL((k_trsm<T, true>), dim3(ceil_div(n - r0, 256)), dim3(256), 0, st, a, lda, n,
                            k0, b, r0, info);
@@ -329,12 +340,24 @@ int panel_factor(T *a, int64_t lda, int n, int K0, int kend, int32_t *info, hipS
         hipLaunchKernelGGL((k_trsm<T, false>), dim3(ceil_div(n - r0, 256)), dim3(256), 0, st, a, lda,
                            n, k0, b, r0, info);
       FMLH_LAUNCH_CHECK("k_trsm");
-      if (r0 < kend) {  // rest of the outer panel
-        const int gx = ceil_div(kend - r0, kTile), gy = ceil_div(n - r0, kTile);
+      if (r0 < mend) {  // rest of the middle block
+        const int gx = ceil_div(mend - r0, kTile), gy = ceil_div(n - r0, kTile);
         hipLaunchKernelGGL((k_update<T, kTile, false>), dim3((unsigned)xcd_grid(gx, gy)), dim3(256),
-                           0, st, a, lda, r0, n, r0, kend, k0, k0 + b, gx, gy, info);
+                           0, st, a, lda, r0, n, r0, mend, k0, k0 + b, gx, gy, info);
         FMLH_LAUNCH_CHECK("k_update(panel)");
       }
+    }
+    if (mend < kend) {  // rest of the outer panel, all rows below mend, with this middle block
+      if (n - mend >= 4 * 128) {
+        const int gx = ceil_div(kend - mend, 128), gy = ceil_div(n - mend, 128);
+        hipLaunchKernelGGL((k_update<T, 128, false>), dim3((unsigned)xcd_grid(gx, gy)), dim3(512),
+                           0, st, a, lda, mend, n, mend, kend, m0, mend, gx, gy, info);
+      } else {
+        const int gx = ceil_div(kend - mend, kTile), gy = ceil_div(n - mend, kTile);
+        hipLaunchKernelGGL((k_update<T, kTile, false>), dim3((unsigned)xcd_grid(gx, gy)), dim3(256),
+                           0, st, a, lda, mend, n, mend, kend, m0, mend, gx, gy, info);
+      }
+      FMLH_LAUNCH_CHECK("k_update(panel rest)");
     }
   }
   return 0;
