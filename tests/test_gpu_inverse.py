@@ -299,6 +299,28 @@ def test_factor_leaves_the_other_triangle(gpu):
             assert torch.equal(torch.triu(F, 1), torch.triu(A, 1)), (fn.__name__, n)
 
 
+def test_potrf_ragged_middle_blocks(gpu):
+    """Outer panels factor their 64-column inner blocks in 256-column middle
+    blocks, the rest of the panel updated once per middle block.  Sizes whose
+    last panel ends in a short middle block after full ones (1324: 256 + 44;
+    2900: 3 x 256 + 84) against LAPACK's factor, FP64 and the Ozaki form."""
+    torch = gpu
+    from fml_amd.math.cho_solve import potrf_inplace, potrf_ozaki_inplace
+    rng = np.random.default_rng(46)
+    for n in (1324, 2900):
+        B = rng.normal(size=(n, n))
+        A = B @ B.T / n + np.eye(n)
+        ref = np.linalg.cholesky(A)
+        Ad = torch.from_numpy(A).cuda()
+        for fn, tol in ((potrf_inplace, 1e-12), (potrf_ozaki_inplace, 1e-10)):
+            F = Ad.clone()
+            assert fn(F) == 0
+            L = torch.tril(F)
+            err = np.max(np.abs(L.cpu().numpy() - ref)) / np.max(np.abs(ref))
+            berr = float((L @ L.T - Ad).abs().max() / Ad.abs().max())
+            assert err <= tol and berr <= tol, (fn.__name__, n, err, berr)
+
+
 def test_residual_dd_lower_matches_full(gpu):
     """fmlh_residual_dd_lower (K's strictly lower triangle in Fortran order
     plus its diagonal, the other triangle holding the factor) against the
